@@ -1,0 +1,212 @@
+"""ORACLE — test infrastructure only (tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg).  Python bindings for
+
+  * oracle/liboracle.so   — the C++ restatement of the reference's hybrid CPU
+                            tiler plus the FP64 coverage / composite model;
+  * oracle/_ref/libpfref.so — the reference's own Tiler compiled from its
+                            unchanged sources (present only where it was built).
+
+Both return the same canonical form (see ``canonical`` below).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libpfref.so")
+
+_P = ctypes.c_void_p
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+_oracle = None
+_ref = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True, capture_output=True)
+
+
+def load_oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.pfo_run.restype = _P
+        lib.pfo_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int]
+        lib.pfo_counts.argtypes = [_P, _P]
+        lib.pfo_fetch.argtypes = [_P] * 17
+        lib.pfo_free.argtypes = [_P]
+        lib.pfo_tiling_seconds.restype = ctypes.c_double
+        lib.pfo_tiling_seconds.argtypes = [_P]
+        lib.pfo_fill_area.restype = ctypes.c_double
+        lib.pfo_fill_area.argtypes = [_P, ctypes.c_int, ctypes.c_int]
+        lib.pfo_coverage.argtypes = [ctypes.c_int64, _P, _P, _P, _P, _P]
+        lib.pfo_coverage_raw.argtypes = [ctypes.c_int64, _P, _P, _P, _P]
+        lib.pfo_apply_rule.restype = ctypes.c_double
+        lib.pfo_apply_rule.argtypes = [ctypes.c_double, ctypes.c_int]
+        lib.pfo_composite.argtypes = [ctypes.c_int64] + [_P] * 7 + [ctypes.c_int] * 4 + [_P]
+        _oracle = lib
+    return _oracle
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def load_ref():
+    global _ref
+    if _ref is None:
+        lib = ctypes.CDLL(REF_SO)
+        lib.pfref_run.restype = _P
+        lib.pfref_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, _P]
+        lib.pfref_counts.argtypes = [_P, _P]
+        lib.pfref_fetch.argtypes = [_P] * 14
+        lib.pfref_free.argtypes = [_P]
+        _ref = lib
+    return _ref
+
+
+def _scene_args(scene):
+    return [ctypes.c_int(scene.n_paths), _ptr(scene.path_contour_start), _ptr(scene.path_fill_rule),
+            _ptr(scene.path_opaque), _ptr(scene.contour_start), _ptr(scene.contour_closed),
+            _ptr(scene.points), _ptr(scene.point_flags), _ptr(scene.view_box)]
+
+
+def _alloc(counts):
+    T, W, F, E, NFB, P = [int(x) for x in counts[:6]]
+    return dict(
+        path_rect=np.zeros((P, 4), np.int32), tile_off=np.zeros(P + 1, np.int64),
+        col_off=np.zeros(P + 1, np.int64), fill_off=np.zeros(P + 1, np.int64),
+        tile_backdrop=np.zeros(T, np.int8), tile_alpha=np.zeros(T, np.uint8), col_sum=np.zeros(W, np.int32),
+        fill_seg=np.zeros((F, 4), np.uint16), fill_tile=np.zeros(F, np.uint32),
+        entry_fbtile=np.zeros(E, np.uint32), entry_path=np.zeros(E, np.uint32), zbuf=np.zeros(NFB, np.uint32),
+        fb_rect=np.zeros(4, np.int32))
+
+
+CANONICAL_KEYS = ("path_rect", "tile_off", "col_off", "fill_off", "tile_backdrop", "tile_alpha", "col_sum",
+                  "fill_seg", "fill_tile", "entry_fbtile", "entry_path", "zbuf", "fb_rect")
+
+
+def run_oracle(scene, zmode: int = 1) -> dict:
+    """Canonical tiling of ``scene`` by the oracle restatement.
+
+    zmode 0: the reference's z-buffer rule; 1: even-odd tiles with an even
+    backdrop never occlude (the rule the HIP path implements)."""
+    lib = load_oracle()
+    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(zmode))
+    counts = np.zeros(8, np.int64)
+    lib.pfo_counts(h, _ptr(counts))
+    out = _alloc(counts)
+    T, W, E = int(counts[0]), int(counts[1]), int(counts[3])
+    out["tile_delta"] = np.zeros(T, np.int8)
+    out["col_above"] = np.zeros(W, np.int32)
+    out["entry_tile"] = np.zeros(E, np.int64)
+    lib.pfo_fetch(h, *[_ptr(out[k]) for k in CANONICAL_KEYS], _ptr(out["tile_delta"]), _ptr(out["col_above"]),
+                  _ptr(out["entry_tile"]))
+    out["n_alpha"] = int(counts[6])
+    out["guard_hits"] = int(counts[7])
+    out["tiling_seconds"] = lib.pfo_tiling_seconds(h)
+    lib.pfo_free(h)
+    return out
+
+
+def run_ref(scene, threads: int = 1) -> dict:
+    """Canonical tiling of ``scene`` by the reference's own compiled Tiler."""
+    lib = load_ref()
+    el = np.zeros(1, np.float64)
+    h = lib.pfref_run(*_scene_args(scene), ctypes.c_int(threads), _ptr(el))
+    counts = np.zeros(6, np.int64)
+    lib.pfref_counts(h, _ptr(counts))
+    out = _alloc(counts)
+    lib.pfref_fetch(h, *[_ptr(out[k]) for k in CANONICAL_KEYS])
+    lib.pfref_free(h)
+    out["tiling_seconds"] = float(el[0])
+    return out
+
+
+def alpha_tiles(canon: dict, scene) -> dict:
+    """Alpha tiles in canonical order (path, row, column) with their fills grouped,
+    their backdrop and fill rule."""
+    T = canon["tile_alpha"].shape[0]
+    tile_path = np.repeat(np.arange(scene.n_paths), np.diff(canon["tile_off"]))
+    alpha_tiles = np.nonzero(canon["tile_alpha"])[0]
+    alpha_index = np.full(T, -1, np.int64)
+    alpha_index[alpha_tiles] = np.arange(alpha_tiles.shape[0])
+    fill_path = np.repeat(np.arange(scene.n_paths), np.diff(canon["fill_off"]))
+    fill_global_tile = canon["tile_off"][fill_path] + canon["fill_tile"].astype(np.int64)
+    fa = alpha_index[fill_global_tile]
+    order = np.argsort(fa, kind="stable")
+    counts = np.bincount(fa, minlength=alpha_tiles.shape[0]) if fa.size else np.zeros(alpha_tiles.shape[0], np.int64)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return dict(tile=alpha_tiles, alpha_index=alpha_index, fill_off=off,
+                fill_seg=np.ascontiguousarray(canon["fill_seg"][order]),
+                backdrop=np.ascontiguousarray(canon["tile_backdrop"][alpha_tiles]),
+                even_odd=np.ascontiguousarray(scene.path_fill_rule[tile_path[alpha_tiles]]),
+                path=tile_path[alpha_tiles])
+
+
+def coverage_fp64(at: dict, raw: bool = False) -> np.ndarray:
+    lib = load_oracle()
+    A = at["tile"].shape[0]
+    out = np.zeros((A, 16, 16), np.float64)
+    if raw:
+        lib.pfo_coverage_raw(ctypes.c_int64(A), _ptr(at["fill_off"]), _ptr(at["fill_seg"]), _ptr(at["backdrop"]),
+                             _ptr(out))
+    else:
+        lib.pfo_coverage(ctypes.c_int64(A), _ptr(at["fill_off"]), _ptr(at["fill_seg"]), _ptr(at["backdrop"]),
+                         _ptr(at["even_odd"]), _ptr(out))
+    return out
+
+
+def composite_fp64(canon: dict, at: dict, coverage: np.ndarray, scene) -> np.ndarray:
+    """FP64 premultiplied RGBA [H, W, 4] of the scene from the canonical entries."""
+    lib = load_oracle()
+    ent_tile = canon["entry_tile"]
+    ent_alpha = np.ascontiguousarray(at["alpha_index"][ent_tile])
+    ent_bd = np.ascontiguousarray(canon["tile_backdrop"][ent_tile])
+    fb = canon["fb_rect"]
+    fbw, fbh = int(fb[2] - fb[0]), int(fb[3] - fb[1])
+    out = np.zeros((scene.height, scene.width, 4), np.float64)
+    cov = np.ascontiguousarray(coverage, np.float64)
+    lib.pfo_composite(ctypes.c_int64(ent_tile.shape[0]), _ptr(canon["entry_fbtile"]), _ptr(canon["entry_path"]),
+                      _ptr(ent_alpha), _ptr(ent_bd), _ptr(scene.path_fill_rule), _ptr(scene.path_rgba), _ptr(cov),
+                      fbw, fbh, scene.width, scene.height, _ptr(out))
+    return out
+
+
+def full_model(scene, zmode: int = 1) -> dict:
+    """Canonical tiling, FP64 coverage and FP64 composite of a scene."""
+    canon = run_oracle(scene, zmode)
+    at = alpha_tiles(canon, scene)
+    cov = coverage_fp64(at)
+    img = composite_fp64(canon, at, cov, scene)
+    return dict(canon=canon, alpha=at, coverage=cov, image=img)
+
+
+def path_coverage_image(scene, model: dict, path: int) -> np.ndarray:
+    """FP64 coverage (after the fill rule) of one draw path over the canvas [H, W]."""
+    canon, at, cov = model["canon"], model["alpha"], model["coverage"]
+    img = np.zeros((scene.height + 16, scene.width + 16), np.float64)
+    l, t, r, b = [int(v) for v in canon["path_rect"][path]]
+    w = max(r - l, 0)
+    t0, t1 = int(canon["tile_off"][path]), int(canon["tile_off"][path + 1])
+    for i in range(t1 - t0):
+        tx, ty = l + i % w, t + i // w
+        x, y = tx * 16, ty * 16
+        a = at["alpha_index"][t0 + i]
+        if a >= 0:
+            img[y:y + 16, x:x + 16] = cov[a]
+        else:
+            img[y:y + 16, x:x + 16] = load_oracle().pfo_apply_rule(float(canon["tile_backdrop"][t0 + i]),
+                                                                  int(scene.path_fill_rule[path]))
+    return img[:scene.height, :scene.width]

@@ -1,0 +1,226 @@
+// Test infrastructure only: drives the reference's own hybrid-mode CPU tiler
+// (core/d3d9/tiler.cpp, object_builder.cpp, scene_builder.cpp), compiled from
+// its unchanged sources by oracle/build_ref.sh, so that the oracle
+// restatement can be checked against it.  Nothing in the product links this.
+//
+// Exports the same canonical form as oracle/hybrid_oracle.cpp (pfo_*):
+//   per path: tile rect, per-tile (backdrop, alpha flag), per-column final
+//   backdrop sum, fills in emission order with their tile-local index;
+//   per framebuffer tile: draw entries in composite order after the z-cull.
+#include <cstdint>
+#include <cstring>
+#include <chrono>
+#include <thread>
+#include <vector>
+
+#include "core/d3d9/scene_builder.h"
+#include "core/d3d9/tiler.h"
+#include "core/data/built_path.h"
+#include "core/scene.h"
+
+namespace Pathfinder {
+// Defined with external linkage in core/d3d9/scene_builder.cpp:18 (not in a header).
+std::vector<DrawTileBatchD3D9> build_tile_batches_for_draw_path_display_item(const Scene &scene,
+                                                                             const std::vector<BuiltDrawPath> &built_paths,
+                                                                             Range draw_path_range);
+} // namespace Pathfinder
+
+using namespace Pathfinder;
+
+namespace {
+
+struct HarnessBuilder : public SceneBuilderD3D9 {
+    void attach(Scene *s) {
+        scene = s;
+    }
+};
+
+struct RefResult {
+    std::vector<int32_t> path_rect;
+    std::vector<int64_t> tile_off, col_off, fill_off;
+    std::vector<int8_t> tile_backdrop;
+    std::vector<uint8_t> tile_alpha;
+    std::vector<int32_t> col_sum;
+    std::vector<uint16_t> fill_seg;
+    std::vector<uint32_t> fill_tile;
+    std::vector<uint32_t> entry_fbtile, entry_path;
+    std::vector<uint32_t> zbuf;
+    int32_t fb_rect[4] = {0, 0, 0, 0};
+};
+
+Outline make_outline(const float *pts, const uint8_t *pflags, const int32_t *contour_start, const uint8_t *closed,
+                     int c0, int c1) {
+    Outline outline;
+    for (int c = c0; c < c1; c++) {
+        Contour contour;
+        for (int i = contour_start[c]; i < contour_start[c + 1]; i++) {
+            PointFlag flag = pflags[i] == 0 ? PointFlag::ON_CURVE_POINT
+                                            : (pflags[i] == 1 ? PointFlag::CONTROL_POINT_0 : PointFlag::CONTROL_POINT_1);
+            contour.push_point(Vec2F(pts[2 * i], pts[2 * i + 1]), flag, true);
+        }
+        contour.closed = closed[c] != 0;
+        outline.push_contour(contour);
+    }
+    return outline;
+}
+
+} // namespace
+
+extern "C" {
+
+// Tiles every draw path of one scene with the reference Tiler.
+// threads > 1 reproduces SceneBuilderD3D9::build_paths_on_cpu's interleaved
+// split (core/d3d9/scene_builder.cpp:170-194); the canonical output does not
+// depend on it.  Returns an opaque handle, or nullptr.
+void *pfref_run(int n_paths, const int32_t *path_contour_start, const uint8_t *path_fill_rule,
+                const uint8_t *path_opaque, const int32_t *contour_start, const uint8_t *contour_closed,
+                const float *points, const uint8_t *point_flags, const float *view_box, int threads,
+                double *elapsed_s) {
+    auto *res = new RefResult();
+    Scene scene(0, RectF(view_box[0], view_box[1], view_box[2], view_box[3]));
+    HarnessBuilder builder;
+    builder.attach(&scene);
+    for (auto &n : builder.next_alpha_tile_indices) n = 0;
+    RectF vb = scene.get_view_box();
+
+    std::vector<Outline> outlines(n_paths);
+    for (int p = 0; p < n_paths; p++) {
+        outlines[p] = make_outline(points, point_flags, contour_start, contour_closed, path_contour_start[p],
+                                   path_contour_start[p + 1]);
+    }
+
+    std::vector<BuiltDrawPath> built(n_paths);
+    std::vector<std::vector<Fill>> fills(n_paths);
+    auto work = [&](int begin, int stride) {
+        for (int p = begin; p < n_paths; p += stride) {
+            TilingPathInfo info{};
+            info.type = TilingPathInfo::Type::Draw;
+            info.info.paint_id = 0;
+            info.info.blend_mode = BlendMode::SrcOver;
+            info.info.fill_rule = path_fill_rule[p] ? FillRule::EvenOdd : FillRule::Winding;
+            Tiler tiler(builder, (uint32_t)p, outlines[p], info.info.fill_rule, vb, nullptr, {}, info);
+            tiler.generate_tiles();
+            built[p].path = tiler.object_builder.built_path;
+            built[p].occludes = path_opaque[p] != 0;
+            built[p].blend_mode = BlendMode::SrcOver;
+            built[p].mask_fill_rule = info.info.fill_rule;
+            fills[p] = std::move(tiler.object_builder.fills);
+        }
+    };
+    auto t0 = std::chrono::steady_clock::now();
+    if (threads <= 1) {
+        work(0, 1);
+    } else {
+        std::vector<std::thread> pool;
+        for (int i = 0; i < threads; i++) pool.emplace_back(work, i, threads);
+        for (auto &t : pool) t.join();
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (elapsed_s) *elapsed_s = std::chrono::duration<double>(t1 - t0).count();
+
+    int64_t tsum = 0, csum = 0, fsum = 0;
+    res->tile_off.push_back(0);
+    res->col_off.push_back(0);
+    res->fill_off.push_back(0);
+    for (int p = 0; p < n_paths; p++) {
+        const BuiltPath &bp = built[p].path;
+        const RectI &r = bp.data.tiles.rect;
+        res->path_rect.insert(res->path_rect.end(), {r.left, r.top, r.right, r.bottom});
+        const auto &tiles = bp.data.tiles.data;
+        std::vector<int64_t> alpha_to_local;
+        for (size_t i = 0; i < tiles.size(); i++) {
+            res->tile_backdrop.push_back(tiles[i].backdrop);
+            res->tile_alpha.push_back(tiles[i].alpha_tile_id.is_valid() ? 1 : 0);
+        }
+        for (auto v : bp.data.backdrops) res->col_sum.push_back(v);
+        for (const Fill &f : fills[p]) {
+            res->fill_seg.insert(res->fill_seg.end(), {f.line_segment.from_x, f.line_segment.from_y,
+                                                       f.line_segment.to_x, f.line_segment.to_y});
+            uint32_t local = 0xffffffffu;
+            for (size_t i = 0; i < tiles.size(); i++) {
+                if (tiles[i].alpha_tile_id.value == f.link) {
+                    local = (uint32_t)i;
+                    break;
+                }
+            }
+            res->fill_tile.push_back(local);
+        }
+        tsum += (int64_t)tiles.size();
+        csum += (int64_t)bp.data.backdrops.size();
+        fsum += (int64_t)fills[p].size();
+        res->tile_off.push_back(tsum);
+        res->col_off.push_back(csum);
+        res->fill_off.push_back(fsum);
+    }
+
+    // Draw order and z-buffer through the reference's own batch builder.
+    if (n_paths > 0) {
+        auto batches = build_tile_batches_for_draw_path_display_item(scene, built, Range(0, (uint64_t)n_paths));
+        if (!batches.empty()) {
+            const auto &batch = batches[0];
+            const RectI &zr = batch.z_buffer_data.rect;
+            res->fb_rect[0] = zr.left;
+            res->fb_rect[1] = zr.top;
+            res->fb_rect[2] = zr.right;
+            res->fb_rect[3] = zr.bottom;
+            res->zbuf.assign(batch.z_buffer_data.data.begin(), batch.z_buffer_data.data.end());
+            int w = zr.width();
+            int64_t nfb = (int64_t)zr.width() * zr.height();
+            // Stable grouping by framebuffer tile keeps batch (draw) order, which is
+            // the order the reference rasterizes and blends the tile instances in.
+            std::vector<std::vector<uint32_t>> lists(nfb);
+            for (const auto &t : batch.tiles) {
+                int64_t fb = (int64_t)(t.tile_y - zr.top) * w + (t.tile_x - zr.left);
+                if ((int32_t)t.path_id < (int32_t)batch.z_buffer_data.data[fb]) continue; // tile.vert:114
+                lists[fb].push_back(t.path_id);
+            }
+            for (int64_t fb = 0; fb < nfb; fb++) {
+                for (auto pid : lists[fb]) {
+                    res->entry_fbtile.push_back((uint32_t)fb);
+                    res->entry_path.push_back(pid);
+                }
+            }
+        }
+    }
+    return res;
+}
+
+// counts[0..5] = tiles, columns, fills, entries, fb tiles, paths
+void pfref_counts(void *h, int64_t *counts) {
+    auto *r = (RefResult *)h;
+    counts[0] = (int64_t)r->tile_backdrop.size();
+    counts[1] = (int64_t)r->col_sum.size();
+    counts[2] = (int64_t)r->fill_tile.size();
+    counts[3] = (int64_t)r->entry_path.size();
+    counts[4] = (int64_t)r->zbuf.size();
+    counts[5] = (int64_t)r->path_rect.size() / 4;
+}
+
+void pfref_fetch(void *h, int32_t *path_rect, int64_t *tile_off, int64_t *col_off, int64_t *fill_off,
+                 int8_t *tile_backdrop, uint8_t *tile_alpha, int32_t *col_sum, uint16_t *fill_seg,
+                 uint32_t *fill_tile, uint32_t *entry_fbtile, uint32_t *entry_path, uint32_t *zbuf,
+                 int32_t *fb_rect) {
+    auto *r = (RefResult *)h;
+    auto cp = [](void *dst, const void *src, size_t bytes) {
+        if (bytes) memcpy(dst, src, bytes);
+    };
+    cp(path_rect, r->path_rect.data(), r->path_rect.size() * 4);
+    cp(tile_off, r->tile_off.data(), r->tile_off.size() * 8);
+    cp(col_off, r->col_off.data(), r->col_off.size() * 8);
+    cp(fill_off, r->fill_off.data(), r->fill_off.size() * 8);
+    cp(tile_backdrop, r->tile_backdrop.data(), r->tile_backdrop.size());
+    cp(tile_alpha, r->tile_alpha.data(), r->tile_alpha.size());
+    cp(col_sum, r->col_sum.data(), r->col_sum.size() * 4);
+    cp(fill_seg, r->fill_seg.data(), r->fill_seg.size() * 2);
+    cp(fill_tile, r->fill_tile.data(), r->fill_tile.size() * 4);
+    cp(entry_fbtile, r->entry_fbtile.data(), r->entry_fbtile.size() * 4);
+    cp(entry_path, r->entry_path.data(), r->entry_path.size() * 4);
+    cp(zbuf, r->zbuf.data(), r->zbuf.size() * 4);
+    cp(fb_rect, r->fb_rect, 16);
+}
+
+void pfref_free(void *h) {
+    delete (RefResult *)h;
+}
+
+} // extern "C"

@@ -1,0 +1,241 @@
+"""Seeded synthetic scene generator for the rasterization workloads.
+
+A scene is held as flat outline arrays, the post-``Path2d`` form the tiler
+consumes (core/data/path.h:12-57 ``Outline``/``Contour``; core/data/data.h:150-160
+``PointFlag``):
+
+    points           float32 [N, 2]
+    point_flags      uint8   [N]     0 on-curve, 1 first control, 2 second control
+    contour_start    int32   [C + 1] point range of each contour
+    contour_closed   uint8   [C]
+    path_contour_start int32 [P + 1] contour range of each draw path
+    path_fill_rule   uint8   [P]     0 nonzero (FillRule::Winding), 1 even-odd
+    path_rgba        uint32  [P]     solid paint, r | g << 8 | b << 16 | a << 24
+    view_box         float32 [4]     left, top, right, bottom
+
+Workloads (the scene families the benchmark and parity tests use):
+  * ``cubic_scene``: 512x512, 64 closed paths of 8 cubic Béziers with all
+    control points uniform in the canvas, nonzero rule, opaque colours.
+  * ``blob_scene``: W x H canvas, closed star-shaped paths of 4..64 segments
+    (line/quad/cubic uniform), centres uniform, radii log-uniform, alpha
+    U(0.2, 1), fill rule chosen by a coin.  With 4096x4096 and 10 000 paths
+    this is the 4096^2 single-scene workload.
+  * ``batch_scenes``: many blob scenes with side log-uniform in [256, 4096]
+    and path count log-uniform in [10, 20 000].
+  * ``star_scene``: star polygons {n/m} with winding up to m.
+
+Points are not deduplicated the way ``Path2d::line_to`` does
+(core/path2d.cpp:21-29); the arrays are outlines, so every consumer (the HIP
+path, the oracle and the reference harness) tiles exactly the same geometry.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+BASE_SEED = 0x9E3779B97F4A7C15
+
+
+@dataclasses.dataclass
+class Scene:
+    width: int
+    height: int
+    points: np.ndarray
+    point_flags: np.ndarray
+    contour_start: np.ndarray
+    contour_closed: np.ndarray
+    path_contour_start: np.ndarray
+    path_fill_rule: np.ndarray
+    path_rgba: np.ndarray
+    view_box: np.ndarray = None
+
+    def __post_init__(self):
+        if self.view_box is None:
+            self.view_box = np.array([0, 0, self.width, self.height], np.float32)
+        self.points = np.ascontiguousarray(self.points, np.float32).reshape(-1, 2)
+        self.point_flags = np.ascontiguousarray(self.point_flags, np.uint8)
+        self.contour_start = np.ascontiguousarray(self.contour_start, np.int32)
+        self.contour_closed = np.ascontiguousarray(self.contour_closed, np.uint8)
+        self.path_contour_start = np.ascontiguousarray(self.path_contour_start, np.int32)
+        self.path_fill_rule = np.ascontiguousarray(self.path_fill_rule, np.uint8)
+        self.path_rgba = np.ascontiguousarray(self.path_rgba, np.uint32)
+        self.view_box = np.ascontiguousarray(self.view_box, np.float32)
+
+    @property
+    def n_paths(self) -> int:
+        return int(self.path_contour_start.shape[0] - 1)
+
+    @property
+    def path_opaque(self) -> np.ndarray:
+        # ColorU::is_opaque (common/color.cpp:42-44) for a solid paint.
+        return ((self.path_rgba >> 24) == 255).astype(np.uint8)
+
+    def subset(self, paths) -> "Scene":
+        """The scene restricted to the given draw paths (in the given order)."""
+        paths = np.asarray(paths, np.int64)
+        pts, flg, cst, ccl, pcs = [], [], [0], [], [0]
+        for p in paths:
+            for c in range(self.path_contour_start[p], self.path_contour_start[p + 1]):
+                a, b = self.contour_start[c], self.contour_start[c + 1]
+                pts.append(self.points[a:b])
+                flg.append(self.point_flags[a:b])
+                cst.append(cst[-1] + (b - a))
+                ccl.append(self.contour_closed[c])
+            pcs.append(len(ccl))
+        return Scene(self.width, self.height,
+                     np.concatenate(pts) if pts else np.zeros((0, 2), np.float32),
+                     np.concatenate(flg) if flg else np.zeros(0, np.uint8),
+                     np.array(cst), np.array(ccl, np.uint8), np.array(pcs),
+                     self.path_fill_rule[paths], self.path_rgba[paths], self.view_box.copy())
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed & ((1 << 64) - 1)))
+
+
+def _pack_rgba(r, g, b, a) -> np.ndarray:
+    return (r.astype(np.uint32) | (g.astype(np.uint32) << 8) | (b.astype(np.uint32) << 16)
+            | (a.astype(np.uint32) << 24))
+
+
+def cubic_scene(seed: int = BASE_SEED ^ 0, size: int = 512, n_paths: int = 64, n_cubics: int = 8) -> Scene:
+    """move_to + n_cubics x cubic_to with every point uniform in the canvas, then close_path."""
+    rng = _rng(seed)
+    npp = 1 + 3 * n_cubics
+    pts = rng.uniform(0.0, size, size=(n_paths, npp, 2)).astype(np.float32)
+    flags = np.zeros(npp, np.uint8)
+    flags[1::3] = 1
+    flags[2::3] = 2
+    rgb = rng.integers(0, 256, size=(n_paths, 3))
+    rgba = _pack_rgba(rgb[:, 0], rgb[:, 1], rgb[:, 2], np.full(n_paths, 255))
+    return Scene(size, size, pts.reshape(-1, 2), np.tile(flags, n_paths),
+                 np.arange(n_paths + 1, dtype=np.int32) * npp, np.ones(n_paths, np.uint8),
+                 np.arange(n_paths + 1, dtype=np.int32), np.zeros(n_paths, np.uint8), rgba)
+
+
+def blob_scene(seed: int, width: int = 4096, height: int = 4096, n_paths: int = 10000,
+               r_min: float = 4.0, r_max: float = 512.0, seg_min: int = 4, seg_max: int = 64,
+               alpha_min: float = 0.2) -> Scene:
+    """Closed star-shaped blobs: n vertices at sorted random angles, radius r*U(0.6,1);
+    each edge a line, quadratic or cubic with control points at radius r*U(0.5,1.2)
+    between the edge's end angles."""
+    rng = _rng(seed)
+    P = int(n_paths)
+    cx = rng.uniform(0, width, P)
+    cy = rng.uniform(0, height, P)
+    r = np.exp(rng.uniform(math.log(r_min), math.log(max(r_max, r_min)), P))
+    n = rng.integers(seg_min, seg_max + 1, P)
+    alpha = np.rint(255.0 * rng.uniform(alpha_min, 1.0, P)).astype(np.int64)
+    rgb = rng.integers(0, 256, size=(P, 3))
+    rule = rng.integers(0, 2, P).astype(np.uint8)
+
+    V = int(n.sum())
+    pid = np.repeat(np.arange(P), n)
+    vstart = np.concatenate([[0], np.cumsum(n)[:-1]])
+    ang = rng.uniform(0, 2 * math.pi, V)
+    order = np.lexsort((ang, pid))
+    ang = ang[order]
+    vrad = r[pid] * rng.uniform(0.6, 1.0, V)
+    vx = cx[pid] + vrad * np.cos(ang)
+    vy = cy[pid] + vrad * np.sin(ang)
+    kind = rng.integers(0, 3, V)  # 0 line, 1 quad, 2 cubic: edge from vertex i to i+1
+    local = np.arange(V) - vstart[pid]
+    nxt = np.where(local + 1 == n[pid], vstart[pid], np.arange(V) + 1)
+    ang_next = np.where(local + 1 == n[pid], ang[nxt] + 2 * math.pi, ang[nxt])
+    f = np.sort(rng.uniform(0, 1, (V, 2)), axis=1)
+    crad = r[pid][:, None] * rng.uniform(0.5, 1.2, (V, 2))
+    ca = ang[:, None] + f * (ang_next - ang)[:, None]
+    cpx = cx[pid][:, None] + crad * np.cos(ca)
+    cpy = cy[pid][:, None] + crad * np.sin(ca)
+    quad_single = ang + 0.5 * (ang_next - ang)
+    qx = cx[pid] + crad[:, 0] * np.cos(quad_single)
+    qy = cy[pid] + crad[:, 0] * np.sin(quad_single)
+
+    is_last = local + 1 == n[pid]
+    # The final edge back to vertex 0: a line is left to the implicit close
+    # (contour.cpp:155-170); a curve is written out and ends on a copy of vertex 0,
+    # so the closing line is skipped by the approx_eq test.
+    last_curve = is_last & (kind != 0)
+    npts_v = 1 + kind + last_curve.astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(npts_v)])
+    N = int(off[-1])
+    X = np.empty(N)
+    Y = np.empty(N)
+    F = np.zeros(N, np.uint8)
+    o = off[:-1]
+    X[o] = vx
+    Y[o] = vy
+    q = kind == 1
+    X[o[q] + 1] = qx[q]
+    Y[o[q] + 1] = qy[q]
+    F[o[q] + 1] = 1
+    c = kind == 2
+    X[o[c] + 1] = cpx[c, 0]
+    Y[o[c] + 1] = cpy[c, 0]
+    F[o[c] + 1] = 1
+    X[o[c] + 2] = cpx[c, 1]
+    Y[o[c] + 2] = cpy[c, 1]
+    F[o[c] + 2] = 2
+    lc = last_curve
+    endi = o[lc] + 1 + kind[lc]
+    X[endi] = vx[vstart[pid[lc]]]
+    Y[endi] = vy[vstart[pid[lc]]]
+    pts = np.stack([X, Y], axis=1).astype(np.float32)
+    # Path point ranges: vertices of path p are contiguous.
+    path_pt_start = off[np.concatenate([vstart, [V]])]
+    rgba = _pack_rgba(rgb[:, 0], rgb[:, 1], rgb[:, 2], alpha)
+    return Scene(width, height, pts, F, path_pt_start.astype(np.int32), np.ones(P, np.uint8),
+                 np.arange(P + 1, dtype=np.int32), rule, rgba)
+
+
+def single_scene_4k(seed: int = BASE_SEED ^ 1, n_paths: int = 10000, size: int = 4096) -> Scene:
+    """The 4096x4096, 10 000-path blob workload."""
+    return blob_scene(seed, size, size, n_paths)
+
+
+def batch_scenes(n_scenes: int = 256, seed: int = BASE_SEED ^ 3, side_min: int = 256, side_max: int = 4096,
+                 paths_min: int = 10, paths_max: int = 20000):
+    """Scenes with side log-uniform in [side_min, side_max] and path count log-uniform in
+    [paths_min, paths_max]; each scene k is blob_scene(seed + k + 1) with radii
+    log-uniform in [4, min(512, side/2)]."""
+    rng = _rng(seed)
+    sides = np.rint(np.exp(rng.uniform(math.log(side_min), math.log(side_max), n_scenes))).astype(int)
+    counts = np.rint(np.exp(rng.uniform(math.log(paths_min), math.log(paths_max), n_scenes))).astype(int)
+    return [blob_scene(seed + k + 1, int(sides[k]), int(sides[k]), int(counts[k]),
+                       r_max=min(512.0, sides[k] / 2.0)) for k in range(n_scenes)]
+
+
+def star_scene(seed: int = BASE_SEED ^ 2, size: int = 4096, n_stars: int = 2000, n_min: int = 100,
+               n_max: int = 2000, m_max: int = 50, r_min: float = 16.0, r_max: float = 512.0) -> Scene:
+    """Star polygons {n/m} with gcd(n, m) = 1: centre winding +-m (sign by orientation)."""
+    rng = _rng(seed)
+    pts, flags, cst, rules, rgba = [], [], [0], [], []
+    for _ in range(n_stars):
+        while True:
+            n = int(rng.integers(n_min, n_max + 1))
+            m = int(rng.integers(1, m_max + 1))
+            if math.gcd(n, m) == 1 and 2 * m < n:
+                break
+        cx, cy = rng.uniform(0, size, 2)
+        r = math.exp(rng.uniform(math.log(r_min), math.log(r_max)))
+        orient = 1 if rng.integers(0, 2) else -1
+        k = np.arange(n)
+        a = orient * 2 * math.pi * ((k * m) % n) / n
+        p = np.stack([cx + r * np.cos(a), cy + r * np.sin(a)], 1)
+        pts.append(p)
+        flags.append(np.zeros(n, np.uint8))
+        cst.append(cst[-1] + n)
+        rules.append(int(rng.integers(0, 2)))
+        c = rng.integers(0, 256, 3)
+        rgba.append(int(c[0]) | int(c[1]) << 8 | int(c[2]) << 16 | int(round(255 * rng.uniform(0.2, 1.0))) << 24)
+    P = n_stars
+    return Scene(size, size, np.concatenate(pts).astype(np.float32), np.concatenate(flags), np.array(cst),
+                 np.ones(P, np.uint8), np.arange(P + 1), np.array(rules, np.uint8), np.array(rgba, np.uint32))
+
+
+def stats(scene: Scene) -> dict:
+    f = scene.point_flags
+    return {"paths": scene.n_paths, "points": int(f.shape[0]), "contours": int(scene.contour_closed.shape[0]),
+            "canvas": [scene.width, scene.height]}

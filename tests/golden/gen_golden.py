@@ -1,0 +1,140 @@
+"""Writes tests/golden/hybrid_fixtures.npz: scenes and the canonical tiling the
+reference's own compiled hybrid-mode Tiler (oracle/_ref/libpfref.so, built by
+oracle/build_ref.sh from the unchanged sources) produces for them.
+
+Run in the build container, where the reference tree is present:
+    python tests/golden/gen_golden.py
+The fixtures are data (inputs and expected outputs); the reference itself never
+travels.  tests/test_oracle_golden.py checks the oracle restatement against
+them bit-for-bit.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "pathfinder-cpp_amd"))
+
+import oracle  # noqa: E402
+import scenegen  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hybrid_fixtures.npz")
+
+
+def make_scene(w, h, paths):
+    """paths: list of (contours, fill_rule, rgba); a contour is (points, flags, closed)."""
+    pts, flg, cst, ccl, pcs, rules, rgba = [], [], [0], [], [0], [], []
+    for contours, rule, color in paths:
+        for p, f, closed in contours:
+            p = np.asarray(p, np.float32).reshape(-1, 2)
+            f = np.zeros(len(p), np.uint8) if f is None else np.asarray(f, np.uint8)
+            pts.append(p)
+            flg.append(f)
+            cst.append(cst[-1] + len(p))
+            ccl.append(1 if closed else 0)
+        pcs.append(len(ccl))
+        rules.append(rule)
+        rgba.append(color)
+    return scenegen.Scene(w, h, np.concatenate(pts) if pts else np.zeros((0, 2), np.float32),
+                          np.concatenate(flg) if flg else np.zeros(0, np.uint8), np.array(cst),
+                          np.array(ccl, np.uint8), np.array(pcs), np.array(rules, np.uint8),
+                          np.array(rgba, np.uint32))
+
+
+def rect(x0, y0, x1, y1, cw=True):
+    p = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
+    return (p if cw else p[::-1], None, True)
+
+
+def star(cx, cy, r, n, m):
+    k = np.arange(n)
+    a = 2 * math.pi * ((k * m) % n) / n
+    return (np.stack([cx + r * np.cos(a), cy + r * np.sin(a)], 1), None, True)
+
+
+OPAQ = 0xFF3366CC
+HALF = 0x80CC8833
+
+
+def edge_scenes():
+    s = {}
+    s["rect_in_tile"] = make_scene(64, 64, [([rect(2, 2, 10, 10)], 0, OPAQ)])
+    s["rect_on_tile_edges"] = make_scene(64, 64, [([rect(16, 16, 48, 48)], 0, OPAQ)])
+    s["rect_half_pixels"] = make_scene(64, 64, [([rect(3.5, 4.5, 40.5, 20.5)], 1, HALF)])
+    s["rect_ccw"] = make_scene(64, 64, [([rect(5, 5, 59, 37, cw=False)], 0, HALF)])
+    s["diag_corners"] = make_scene(96, 96, [([([(0, 0), (64, 64), (0, 64)], None, True)], 0, OPAQ)])
+    s["diag_corners_rev"] = make_scene(96, 96, [([([(64, 0), (0, 64), (64, 64)], None, True)], 0, OPAQ)])
+    s["vertical_tile_boundary"] = make_scene(96, 96, [([([(32, 5), (32, 80), (70, 80), (70, 5)], None, True)], 0, OPAQ)])
+    s["partly_outside"] = make_scene(64, 64, [([([(-20, -30), (90, 10), (50, 100), (-10, 40)], None, True)], 0, HALF)])
+    s["above_view_box"] = make_scene(64, 64, [([([(10, -50), (50, -40), (40, 30), (20, 30)], None, True)], 1, HALF)])
+    s["fully_outside"] = make_scene(64, 64, [([rect(100, 100, 200, 200)], 0, OPAQ), ([rect(-80, 10, -20, 30)], 0, OPAQ)])
+    s["far_outside_coords"] = make_scene(64, 64, [([([(-1e5, 20), (1e5, 30), (1e4, 1e5)], None, True)], 0, HALF)])
+    s["quad_blob"] = make_scene(80, 80, [([([(10, 40), (40, -5), (70, 40), (40, 85), (10, 40)],
+                                           [0, 1, 0, 1, 0], True)], 0, OPAQ)])
+    s["open_contour"] = make_scene(64, 64, [([([(5, 5), (60, 12), (30, 55)], None, False)], 0, HALF)])
+    s["explicit_close"] = make_scene(64, 64, [([([(5, 5), (60, 12), (30, 55), (5, 5)], None, True)], 0, HALF)])
+    s["near_close"] = make_scene(64, 64, [([([(5, 5), (60, 12), (30, 55), (5.00005, 5)], None, True)], 0, HALF)])
+    s["leading_control_point"] = make_scene(64, 64, [([([(5, 5), (60, 12), (30, 55)], [1, 0, 0], True)], 0, HALF),
+                                                     ([rect(8, 8, 20, 20)], 0, OPAQ)])
+    s["holes_nonzero"] = make_scene(96, 96, [([rect(4, 4, 92, 92), rect(30, 30, 60, 60)], 0, HALF)])
+    s["holes_evenodd"] = make_scene(96, 96, [([rect(4, 4, 92, 92), rect(30, 30, 60, 60)], 1, HALF)])
+    s["star_7_3_nonzero"] = make_scene(128, 128, [([star(64, 64, 60, 7, 3)], 0, HALF)])
+    s["star_7_3_evenodd"] = make_scene(128, 128, [([star(64, 64, 60, 7, 3)], 1, HALF)])
+    s["zcull_opaque"] = make_scene(96, 96, [([rect(0, 0, 96, 96)], 0, HALF), ([rect(8, 8, 88, 88)], 0, OPAQ),
+                                            ([rect(20, 20, 50, 50)], 0, HALF)])
+    # Even-odd path covering solid tiles twice: backdrop 2 -> covers nothing, but the
+    # reference still writes z for it (core/d3d9/scene_builder.cpp:65-78).
+    s["zcull_evenodd_even"] = make_scene(96, 96, [([rect(0, 0, 96, 96)], 0, HALF),
+                                                  ([rect(4, 4, 92, 92), rect(2, 2, 94, 94)], 1, OPAQ)])
+    s["subpixel_features"] = make_scene(32, 32, [([([(3.1, 3.1), (3.6, 3.2), (3.3, 3.8)], None, True)], 0, OPAQ),
+                                                 ([rect(10.2, 10.2, 10.7, 20.9)], 1, HALF)])
+    s["zero_area_back_forth"] = make_scene(64, 64, [([([(5, 5), (50, 40), (5, 5.5)], None, True)], 0, HALF)])
+    s["odd_canvas_100x70"] = make_scene(100, 70, [([([(-5, 3), (105, 20), (60, 75)], None, True)], 0, HALF)])
+    s["collinear_cubic"] = make_scene(64, 64, [([([(2, 2), (20, 20), (40, 40), (60, 60), (2, 60)],
+                                                  [0, 1, 2, 0, 0], True)], 0, OPAQ)])
+    # x*256 lands exactly on .5: round-half-even (common/f32x4.h:64-66).
+    s["half_even_rounding"] = make_scene(64, 64, [([([(1 / 512, 3 / 512), (40 + 5 / 512, 7 / 512),
+                                                     (33 + 9 / 512, 30 + 11 / 512)], None, True)], 0, HALF)])
+    s["tangent_cubic"] = make_scene(128, 128, [([([(10, 100), (10, 0), (118, 0), (118, 100)], [0, 1, 2, 0], True)],
+                                                1, HALF)])
+    return s
+
+
+def random_scenes():
+    s = {}
+    for k in range(8):
+        s[f"blob_small_{k}"] = scenegen.blob_scene(1000 + k, 256, 192 + 16 * k, 12 + 5 * k, r_max=96.0)
+    s["stars_small"] = scenegen.star_scene(seed=77, size=512, n_stars=6, n_min=20, n_max=120, m_max=9, r_max=200.0)
+    s["cubic_512_64x8"] = scenegen.cubic_scene()
+    return s
+
+
+SCENE_KEYS = ("points", "point_flags", "contour_start", "contour_closed", "path_contour_start", "path_fill_rule",
+              "path_rgba", "view_box")
+
+
+def main():
+    assert oracle.ref_available(), "build oracle/_ref first: oracle/build_ref.sh"
+    scenes = {**edge_scenes(), **random_scenes()}
+    data = {}
+    names = []
+    for name, sc in scenes.items():
+        ref = oracle.run_ref(sc, threads=1)
+        names.append(name)
+        data[f"{name}/size"] = np.array([sc.width, sc.height], np.int32)
+        for k in SCENE_KEYS:
+            data[f"{name}/scene/{k}"] = getattr(sc, k)
+        for k in oracle.CANONICAL_KEYS:
+            data[f"{name}/ref/{k}"] = ref[k]
+        print(f"{name:28s} paths {sc.n_paths:4d} tiles {ref['tile_alpha'].size:7d} fills {ref['fill_tile'].size:7d}"
+              f" entries {ref['entry_path'].size:6d}")
+    data["names"] = np.array(names)
+    np.savez_compressed(OUT, **data)
+    print("wrote", OUT, os.path.getsize(OUT), "bytes")
+
+
+if __name__ == "__main__":
+    main()
