@@ -1,0 +1,138 @@
+/*
+ * pfh.h — C ABI of the MI355X (gfx950) HIP rasterizer that replaces the
+ * reference's GPU-driven rendering path for a scene:
+ *
+ *   Canvas::draw (core/canvas.cpp:557-567)
+ *     -> SceneBuilder::build (core/scene_builder.h:16-32)
+ *     -> Renderer::draw      (core/renderer.h:83)
+ *
+ * i.e. SceneBuilderD3D11::build (core/d3d11/scene_builder.cpp:209-223) plus
+ * RendererD3D11::draw (core/d3d11/renderer.cpp:302-336) and its seven compute
+ * stages (shaders/d3d11/{bound,dice,bin,propagate,sort,fill,tile}.comp), with
+ * the hybrid CPU tiler's arithmetic (core/d3d9/tiler.cpp) so that tiles, fills,
+ * backdrops and draw order are bit-identical to it.
+ *
+ * Conventions: plain pointers and sizes only; host arrays are borrowed for the
+ * duration of the call; the return value is PFH_OK (0) or a negative PFH_E*
+ * code, and pfh_last_error() describes the failure.  A context is externally
+ * synchronised (one host thread at a time).  Nothing throws across the ABI.
+ */
+#ifndef PFH_H
+#define PFH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFH_OK 0
+#define PFH_E_ARG (-1)       /* invalid argument */
+#define PFH_E_HIP (-2)       /* HIP runtime error */
+#define PFH_E_NOMEM (-3)     /* device allocation failed */
+#define PFH_E_STATE (-4)     /* call out of order (e.g. dump before rasterize) */
+#define PFH_E_LIMIT (-5)     /* a size exceeds a documented limit */
+#define PFH_E_OVERFLOW (-6)  /* capacity exceeded and retries exhausted */
+
+/* pfh_rasterize flags */
+#define PFH_PLAIN 0x0u      /* per scene, stage by stage, exact sizes read back (RendererD3D11::draw structure) */
+#define PFH_BATCHED 0x1u    /* all scenes in one pass, no host round trips; replayed from a captured HIP graph */
+#define PFH_PREDICT 0x2u    /* size fill/alpha/entry buffers from the learned predictor (PFH_BATCHED) */
+#define PFH_DUMP 0x4u       /* keep intermediates for pfh_dump (canonical fills in emission order, masks) */
+#define PFH_LOAD_DST 0x8u   /* blend over the current target contents instead of clearing (clear_dst = false) */
+#define PFH_NO_GRAPH 0x10u  /* PFH_BATCHED without graph capture (eager launches) */
+
+/* Fill rules (core/data/data.h:166-172 FillRule). */
+#define PFH_FILL_NONZERO 0
+#define PFH_FILL_EVEN_ODD 1
+
+typedef struct pfh_ctx pfh_ctx;
+
+/*
+ * A scene as outlines: the data of Scene::draw_paths (core/scene.h:57-61) after
+ * Path2d/Canvas have applied transforms, strokes and dashes (core/canvas.cpp:180-301).
+ * Point flags follow PointFlag (core/data/data.h:150-160): 0 on-curve,
+ * 1 first control point (quadratic or cubic), 2 second control point (cubic).
+ * Colours are solid paints, r | g << 8 | b << 16 | a << 24 (ColorU).
+ */
+typedef struct pfh_scene_desc {
+    int32_t width, height;              /* canvas size in pixels */
+    float view_box[4];                  /* left, top, right, bottom (Scene::set_view_box, core/scene.cpp:170-179) */
+    int32_t n_paths, n_contours, n_points;
+    const float *points;                /* [2 * n_points] */
+    const uint8_t *point_flags;         /* [n_points] */
+    const int32_t *contour_start;       /* [n_contours + 1] */
+    const uint8_t *contour_closed;      /* [n_contours] */
+    const int32_t *path_contour_start;  /* [n_paths + 1] */
+    const uint8_t *path_fill_rule;      /* [n_paths] PFH_FILL_* */
+    const uint32_t *path_rgba;          /* [n_paths] */
+} pfh_scene_desc;
+
+typedef struct pfh_stats {
+    int64_t paths, segments, tiles, columns, fills, alpha_tiles, entries, fb_tiles;
+    int64_t retries;          /* capacity overflows that forced a re-run */
+    int64_t guard_hits;       /* loop guards tripped (inputs on which the reference would not terminate) */
+    double ms_total;          /* device time of the last pfh_rasterize (HIP events) */
+    double ms_stage[8];       /* plain mode: count, scan, emit, propagate, entries, sort, fill, composite */
+    int64_t capacity_fills, capacity_alpha, capacity_entries;
+} pfh_stats;
+
+/* Context on HIP device `device`.  Replaces Renderer's Device/Queue pair (core/renderer.cpp:13-37). */
+int pfh_create(int device, pfh_ctx **out);
+void pfh_destroy(pfh_ctx *ctx);
+const char *pfh_last_error(const pfh_ctx *ctx);
+
+/*
+ * Packs a scene (SegmentsIter order, core/data/contour.cpp:110-173; path tile rects,
+ * core/data/built_path.cpp:8-34) and makes it resident on the device.  Replaces
+ * SceneBuilderD3D11::build's CPU packing (core/d3d11/gpu_data.cpp:79-115,
+ * core/d3d11/scene_builder.cpp:26-58).  *scene_id receives a handle.
+ */
+int pfh_scene_upload(pfh_ctx *ctx, const pfh_scene_desc *desc, int32_t *scene_id);
+int pfh_scene_release(pfh_ctx *ctx, int32_t scene_id);
+
+/* Per-path features of an uploaded scene for the sizing predictor: [n_paths * 12] floats. */
+int pfh_scene_features(pfh_ctx *ctx, int32_t scene_id, float *out);
+
+/*
+ * Rasterizes n scenes into their targets (RendererD3D11::draw, core/d3d11/renderer.cpp:302-336).
+ * Without a target set by pfh_scene_set_target the context owns an RGBA8 image per scene.
+ */
+int pfh_rasterize(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t flags);
+
+/* Points a scene's output at caller-owned device memory (Canvas::set_dst_texture, core/canvas.cpp:538-540). */
+int pfh_scene_set_target(pfh_ctx *ctx, int32_t scene_id, void *device_rgba8, int64_t pitch_bytes);
+/* Device address and pitch of a scene's RGBA8 output (premultiplied). */
+int pfh_scene_target(pfh_ctx *ctx, int32_t scene_id, void **device_rgba8, int64_t *pitch_bytes);
+/* Copies a scene's RGBA8 output to host memory (CommandEncoder::read_texture, gpu/command_encoder.h:228). */
+int pfh_readback_rgba8(pfh_ctx *ctx, int32_t scene_id, void *dst, int64_t pitch_bytes);
+
+/*
+ * Canonical intermediates of the last PFH_DUMP rasterize for one scene, in the
+ * oracle's layout.  counts: tiles, columns, fills, entries, fb tiles, paths, alpha tiles, guard hits.
+ */
+int pfh_dump_counts(pfh_ctx *ctx, int32_t scene_id, int64_t *counts);
+int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_off, int64_t *col_off,
+             int64_t *fill_off, int8_t *tile_backdrop, uint8_t *tile_alpha, int32_t *col_sum, uint16_t *fill_seg,
+             uint32_t *fill_tile, uint32_t *entry_fbtile, uint32_t *entry_path, uint32_t *zbuf, int32_t *fb_rect,
+             uint8_t *masks /* alpha tiles in (path, row, column) order, 256 bytes each */);
+
+int pfh_stats_get(pfh_ctx *ctx, pfh_stats *out);
+
+/* Sizing predictor weights (12 -> 64 -> 64 -> 2 MLP, FP32); see pfh_mlp_* below. */
+int pfh_set_predictor(pfh_ctx *ctx, const float *weights, int64_t n_floats, float safety);
+
+/* Device kernels of the sizing MLP, exposed for training and gradient checks (device pointers). */
+int pfh_mlp_forward(pfh_ctx *ctx, const float *d_params, const float *d_x, int64_t n, float *d_hidden,
+                    float *d_out);
+int pfh_mlp_backward(pfh_ctx *ctx, const float *d_params, const float *d_x, int64_t n, const float *d_hidden,
+                     const float *d_out, const float *d_target, float *d_grad, float *d_loss);
+int pfh_mlp_sgd(pfh_ctx *ctx, float *d_params, float *d_m, float *d_v, const float *d_grad, int64_t n_params,
+                float lr, int32_t step);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFH_H */

@@ -322,7 +322,9 @@ struct Builder {
             return;
         }
         bool flat = s.kind == K_QUAD ? flat_quad(s, tol) : flat_cubic(s, tol);
-        if (flat || depth >= 40) {
+        // Depth guard: a finite curve inside |coordinates| < 2^48 is flat long before
+        // depth 24 (the flatness measure shrinks 16x per halving).
+        if (flat || depth >= 24) {
             if (!flat) pt->guard_hits++;
             process_line(s.p0, s.p1);
             return;
@@ -402,7 +404,15 @@ static void tile_path(const Scene &sc, int p, PathTiling &pt) {
         segs.clear();
         contour_segments((const V2 *)(sc.points + 2 * c0), sc.point_flags + c0, c1 - c0, sc.contour_closed[c] != 0,
                          segs);
-        for (const Seg &sg : segs) bld.process_segment(sg, 0);
+        for (const Seg &sg : segs) {
+            // Any non-finite point makes every flattened piece non-finite, and those are
+            // dropped by process_line_segment (tiler.cpp:133-136).
+            bool finite = std::isfinite(sg.p0.x) && std::isfinite(sg.p0.y) && std::isfinite(sg.p1.x) &&
+                          std::isfinite(sg.p1.y);
+            if (sg.kind != K_LINE) finite = finite && std::isfinite(sg.c0.x) && std::isfinite(sg.c0.y);
+            if (sg.kind == K_CUBIC) finite = finite && std::isfinite(sg.c1.x) && std::isfinite(sg.c1.y);
+            if (finite) bld.process_segment(sg, 0);
+        }
     }
     bld.prepare_tiles();
 }

@@ -1,0 +1,82 @@
+// Internal layouts shared by the HIP kernels (pfh_kernels.hip) and the host
+// runtime (pfh_runtime.hip).  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pfh {
+
+constexpr int TILE = 16;                 // TILE_WIDTH / TILE_HEIGHT, core/data/data.h:15-16
+constexpr int MAX_FLATTEN_DEPTH = 24;    // recursion cap of process_segment (guard; see flatten)
+constexpr int FIX_SHIFT = 20;            // fixed-point coverage accumulation: 2^-20 px^2 units
+constexpr uint32_t NO_ALPHA = 0xffffffffu;
+
+enum SegKind : uint32_t { SEG_LINE = 1, SEG_QUAD = 2, SEG_CUBIC = 3 };
+
+// Counters written by the device pipeline (one block per batch).
+enum Counter {
+    C_FILLS = 0,       // total fills (count pass)
+    C_ALPHA = 1,       // total alpha tiles
+    C_ENTRIES = 2,     // total draw entries after the z-cull
+    C_OVERFLOW = 3,    // bit 0 fills, bit 1 alpha tiles, bit 2 entries exceeded capacity
+    C_GUARD = 4,       // loop guards tripped
+    C_COUNT = 8
+};
+
+// One scene of a batch as the kernels see it.  Scene-local arrays are resident
+// from upload; *_base fields place the scene inside the batch's work arrays.
+struct SceneDev {
+    const float2 *pts;            // packed points
+    const uint2 *segs;            // {first point, SegKind}
+    const uint32_t *seg_path;     // local path of each segment
+    const int4 *path_rect;        // tile rect (l, t, r, b) per path
+    const uint32_t *path_tile_off;  // [n_paths + 1] local tile offsets
+    const uint32_t *path_col_off;   // [n_paths + 1] local column offsets
+    const uint32_t *col_path;     // local column -> local path
+    const uint32_t *path_rgba;    // r | g << 8 | b << 16 | a << 24
+    const uint8_t *path_rule;     // 0 nonzero, 1 even-odd
+    float vb_l, vb_t, vb_r, vb_b; // view box (right edge rounded up to 16)
+    int32_t fb_x0, fb_y0, fb_w, fb_h;
+    int32_t width, height;
+    uint32_t n_paths, n_segs, n_tiles, n_cols, n_fb;
+    uint32_t seg_base, path_base, tile_base, col_base, fb_base;
+    uint8_t *out;                 // RGBA8 target
+    int64_t out_pitch;
+    uint32_t load_dst;            // blend over existing target contents
+    uint32_t pad;
+};
+
+// Batch-wide work arrays (device pointers).  Sizes: S segments, T tiles,
+// W columns, FB framebuffer tiles, F/A/E capacities for fills, alpha tiles, entries.
+struct BatchDev {
+    const SceneDev *scenes;
+    int32_t n_scenes;
+    const uint32_t *seg_bases;    // [n_scenes + 1]
+    const uint32_t *tile_bases;
+    const uint32_t *col_bases;
+    const uint32_t *fb_bases;
+    uint32_t n_segs, n_tiles, n_cols, n_fb;
+    uint32_t cap_fills, cap_alpha, cap_entries;
+    int32_t *tile_delta;          // [T] backdrop delta (int8 arithmetic, stored widened)
+    uint32_t *tile_cnt;           // [T] fills per tile, then emit cursor
+    uint32_t *tile_fill_off;      // [T + 1] exclusive scan of fill counts
+    uint32_t *tile_alpha_idx;     // [T + 1] exclusive scan of alpha flags
+    int8_t *tile_bd;              // [T] propagated backdrop
+    uint32_t *tile_entry;         // [T + 1] entry flag, then exclusive scan
+    int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
+    uint32_t *seg_cnt;            // [S + 1] fills per segment, scanned (dump order)
+    uint2 *fills;                 // [F] grouped by tile: (x0 | y0 << 16, x1 | y1 << 16)
+    uint4 *dump_fills;            // [F] emission order: (x0|y0<<16, x1|y1<<16, global tile, 0)
+    uint4 *alpha_rec;             // [A] (fill begin, fill end, backdrop | rule << 8, tile)
+    uint8_t *masks;               // [A * 256]
+    uint32_t *zbuf;               // [FB]
+    uint32_t *fb_cnt;             // [FB + 1] entries per framebuffer tile, then offsets
+    uint32_t *keys, *vals;        // [E] sort keys / values
+    uint32_t *keys_alt, *vals_alt;
+    uint4 *entry_rec;             // [E] (alpha index | NO_ALPHA, rgba (alpha 0 if covers nothing), local path, fb)
+    uint32_t *counters;           // [C_COUNT]
+    int32_t dump;
+};
+
+}  // namespace pfh

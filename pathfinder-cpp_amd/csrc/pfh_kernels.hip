@@ -1,0 +1,861 @@
+// gfx950 kernels of the rasterization pipeline.
+//
+// Stage map (reference GLSL compute stage -> kernel here):
+//   dice.comp + bin.comp        -> k_tile_segments<COUNT/EMIT>: the hybrid tiler's
+//                                  flatten (core/d3d9/tiler.cpp:284-315) and tile walk
+//                                  (tiler.cpp:131-279) per segment, exact float order
+//   bound.comp                  -> tile arrays cleared by k_fill_u32 (pure write stream)
+//   propagate.comp              -> k_propagate (prepare_tiles, tiler.cpp:369-439) + z-buffer
+//   sort.comp                   -> k_entries_* + stable LSD radix sort by framebuffer tile
+//   fill.comp / fill.frag       -> k_fill_masks: analytic box-filtered area, fixed-point
+//   tile.comp / tile.frag       -> k_composite: premultiplied src-over in fp32, one RGBA8 store
+//
+// This translation unit is built with -ffp-contract=off: the flatten/bin
+// arithmetic must round exactly like the reference's SSE code (no FMA).
+#include "pfh_internal.h"
+
+namespace pfh {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int find_scene(const uint32_t *bases, int n, uint32_t i) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (bases[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t value, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = value;
+}
+
+// _mm_max_ps(a, b) / _mm_min_ps(a, b): the second operand wins on NaN or equality.
+__device__ __forceinline__ float sse_max(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float sse_min(float a, float b) { return a < b ? a : b; }
+
+struct Seg4 {
+    float2 p0, c0, c1, p1;
+};
+
+__device__ __forceinline__ float2 f2add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 f2sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 f2muls(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 lerp2(float2 a, float2 b, float t) { return f2add(a, f2muls(f2sub(b, a), t)); }
+
+// core/data/segment.cpp:12-20 is_flat_cubic
+__device__ __forceinline__ bool flat_cubic(const Seg4 &s) {
+    float u0 = 3.0f * s.c0.x - s.p0.x - s.p0.x - s.p1.x;
+    float u1 = 3.0f * s.c0.y - s.p0.y - s.p0.y - s.p1.y;
+    float u2 = 3.0f * s.c1.x - s.p1.x - s.p1.x - s.p0.x;
+    float u3 = 3.0f * s.c1.y - s.p1.y - s.p1.y - s.p0.y;
+    u0 = u0 * u0;
+    u1 = u1 * u1;
+    u2 = u2 * u2;
+    u3 = u3 * u3;
+    return sse_max(u0, u2) + sse_max(u1, u3) <= 1.0f;  // FLATTENING_TOLERANCE, tiler.cpp:15
+}
+
+// core/data/segment.cpp:22-36 is_flat_quadratic
+__device__ __forceinline__ bool flat_quad(const Seg4 &s) {
+    float2 mid = f2muls(f2add(s.p0, s.p1), 0.5f);
+    float2 dev = f2sub(s.c0, mid);
+    return dev.x * dev.x + dev.y * dev.y <= 1.0f * 0.25f;
+}
+
+// ---------------------------------------------------------------------------
+// Flatten + tile walk per segment (hybrid semantics)
+// ---------------------------------------------------------------------------
+
+template <bool EMIT>
+struct Walker {
+    // scene / path state
+    float vb_l, vb_r, vb_b;
+    int32_t tl, tt, tr, tb, tw, th;
+    uint32_t tbase, cbase;
+    const BatchDev *b;
+    uint32_t count;          // fills emitted by this segment so far
+    uint32_t dump_base;      // first emission slot of this segment (EMIT + dump)
+    uint32_t guard;
+
+    // core/d3d9/object_builder.cpp:19-64 add_fill
+    __device__ __forceinline__ void add_fill(float2 from, float2 to, int tx, int ty) {
+        if (!(tl <= tx && tx <= tr - 1 && tt <= ty && ty <= tb - 1)) return;
+        float ox = (float)tx * 16.0f, oy = (float)ty * 16.0f;
+        float v0 = (from.x - ox) * 256.0f, v1 = (from.y - oy) * 256.0f;
+        float v2 = (to.x - ox) * 256.0f, v3 = (to.y - oy) * 256.0f;
+        uint32_t q0 = (uint32_t)rintf(sse_min(sse_max(v0, 0.0f), 4095.0f));
+        uint32_t q1 = (uint32_t)rintf(sse_min(sse_max(v1, 0.0f), 4095.0f));
+        uint32_t q2 = (uint32_t)rintf(sse_min(sse_max(v2, 0.0f), 4095.0f));
+        uint32_t q3 = (uint32_t)rintf(sse_min(sse_max(v3, 0.0f), 4095.0f));
+        if (q0 == q2) return;
+        uint32_t t = tbase + (uint32_t)((ty - tt) * tw + (tx - tl));
+        if (!EMIT) {
+            atomicAdd(&b->tile_cnt[t], 1u);
+        } else {
+            uint32_t slot = atomicSub(&b->tile_cnt[t], 1u) - 1u;
+            uint32_t pos = b->tile_fill_off[t] + slot;
+            uint2 packed = make_uint2(q0 | (q1 << 16), q2 | (q3 << 16));
+            if (pos < b->cap_fills) b->fills[pos] = packed;
+            if (b->dump) {
+                uint32_t d = dump_base + count;
+                if (d < b->cap_fills) b->dump_fills[d] = make_uint4(packed.x, packed.y, t, 0u);
+            }
+        }
+        count++;
+    }
+
+    // core/d3d9/object_builder.cpp:96-114 adjust_alpha_tile_backdrop
+    __device__ __forceinline__ void adjust(int tx, int ty, int delta) {
+        if (EMIT) return;  // backdrops are accumulated once, in the count pass
+        int ox = tx - tl, oy = ty - tt;
+        if (ox < 0 || ox >= tw || oy >= th) return;
+        if (oy < 0) {
+            atomicAdd(&b->col_bd[cbase + ox], delta);
+            return;
+        }
+        atomicAdd(&b->tile_delta[tbase + (uint32_t)(oy * tw + ox)], delta);
+    }
+
+    __device__ __forceinline__ static uint32_t outcode(float2 p, float l, float t, float r, float bb) {
+        uint32_t f = 0;
+        if (p.x < l) f |= 1;
+        else if (p.x > r) f |= 2;
+        if (p.y < t) f |= 4;
+        else if (p.y > bb) f |= 8;
+        return f;
+    }
+
+    // common/math/basic.h:45-53 lerp with t clamped to [0, 1]
+    __device__ __forceinline__ static float lerpc(float a, float bv, float t) {
+        t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+        return a + (bv - a) * t;
+    }
+
+    // core/d3d9/tiler.cpp:65-125 Cohen–Sutherland against the view box with top = -inf
+    __device__ bool clip(float2 &from, float2 &to) {
+        const float l = vb_l, t = -INFINITY, r = vb_r, bb = vb_b;
+        uint32_t of = outcode(from, l, t, r, bb), ot = outcode(to, l, t, r, bb);
+        for (int iter = 0; iter < 64; iter++) {
+            if (of == 0 && ot == 0) return true;
+            if ((of & ot) != 0) return false;
+            bool clip_from = of > ot;
+            float2 p = clip_from ? from : to;
+            uint32_t oc = clip_from ? of : ot;
+            if (oc & 1) p = make_float2(l, lerpc(from.y, to.y, (l - from.x) / (to.x - from.x)));
+            else if (oc & 2) p = make_float2(r, lerpc(from.y, to.y, (r - from.x) / (to.x - from.x)));
+            else if (oc & 4) p = make_float2(lerpc(from.x, to.x, (t - from.y) / (to.y - from.y)), t);
+            else if (oc & 8) p = make_float2(lerpc(from.x, to.x, (bb - from.y) / (to.y - from.y)), bb);
+            if (clip_from) {
+                from = p;
+                of = outcode(p, l, t, r, bb);
+            } else {
+                to = p;
+                ot = outcode(p, l, t, r, bb);
+            }
+        }
+        guard++;
+        return false;
+    }
+
+    // core/d3d9/tiler.cpp:131-279 process_line_segment
+    __device__ void line(float2 from, float2 to) {
+        if (!isfinite(from.x) || !isfinite(from.y) || !isfinite(to.x) || !isfinite(to.y)) return;
+        if (!clip(from, to)) return;
+        const float inv = 1.0f / 16.0f;
+        int fx = (int)floorf(from.x * inv), fy = (int)floorf(from.y * inv);
+        int ex = (int)floorf(to.x * inv), ey = (int)floorf(to.y * inv);
+        float2 vec = f2sub(to, from);
+        int stx = vec.x < 0 ? -1 : 1, sty = vec.y < 0 ? -1 : 1;
+        float crx = ((float)fx + (vec.x >= 0 ? 1.0f : 0.0f)) * 16.0f;
+        float cry = ((float)fy + (vec.y >= 0 ? 1.0f : 0.0f)) * 16.0f;
+        float tmx = (crx - from.x) / vec.x, tmy = (cry - from.y) / vec.y;
+        float tdx = fabsf(16.0f / vec.x), tdy = fabsf(16.0f / vec.y);
+        float2 cur = from;
+        int cx = fx, cy = fy;
+        int last = 0;  // 0 none, 1 X, 2 Y
+        int64_t lim = llabs((int64_t)ex - fx) + llabs((int64_t)ey - fy) + 2;
+        for (int64_t it = 0;; it++) {
+            if (it > lim) {
+                guard++;
+                break;
+            }
+            int next;
+            if (tmx < tmy) next = 1;
+            else if (tmx > tmy) next = 2;
+            else next = stx > 0 ? 1 : 2;
+            float nt = next == 1 ? tmx : tmy;
+            nt = (1.0f < nt) ? 1.0f : nt;  // std::min(next_t, 1.0f)
+            if (cx == ex && cy == ey) next = 0;
+            float2 npos = f2add(from, f2muls(vec, nt));  // LineSegmentF::sample
+            add_fill(cur, npos, cx, cy);
+            float2 corner = make_float2((float)cx * 16.0f, (float)cy * 16.0f);
+            if (sty < 0 && next == 2) add_fill(npos, corner, cx, cy);
+            else if (sty > 0 && last == 2) add_fill(corner, cur, cx, cy);
+            if (stx < 0 && last == 1) adjust(cx, cy, 1);
+            else if (stx > 0 && next == 1) adjust(cx, cy, -1);
+            if (next == 1) {
+                tmx += tdx;
+                cx += stx;
+            } else if (next == 2) {
+                tmy += tdy;
+                cy += sty;
+            } else {
+                break;
+            }
+            cur = npos;
+            last = next;
+        }
+    }
+
+    // core/d3d9/tiler.cpp:284-315 process_segment, depth first, first half first;
+    // core/data/segment.cpp:42-126 split_cubic / split_quadratic at t = 0.5.
+    __device__ void segment(const Seg4 &root, uint32_t kind) {
+        if (kind == SEG_LINE) {
+            line(root.p0, root.p1);
+            return;
+        }
+        Seg4 stack[MAX_FLATTEN_DEPTH + 2];
+        int depth[MAX_FLATTEN_DEPTH + 2];
+        int sp = 0;
+        stack[0] = root;
+        depth[0] = 0;
+        sp = 1;
+        const bool quad = kind == SEG_QUAD;
+        while (sp > 0) {
+            sp--;
+            Seg4 s = stack[sp];
+            int d = depth[sp];
+            bool flat = quad ? flat_quad(s) : flat_cubic(s);
+            if (flat || d >= MAX_FLATTEN_DEPTH) {
+                if (!flat) guard++;
+                line(s.p0, s.p1);
+                continue;
+            }
+            Seg4 a, bseg;
+            const float t = 0.5f;
+            if (quad) {
+                float2 qa = lerp2(s.p0, s.c0, t), qb = lerp2(s.c0, s.p1, t), qc = lerp2(qa, qb, t);
+                a = Seg4{s.p0, qa, qa, qc};
+                bseg = Seg4{qc, qb, qb, s.p1};
+            } else {
+                float2 p01 = lerp2(s.p0, s.c0, t), p12 = lerp2(s.c0, s.c1, t), p23 = lerp2(s.c1, s.p1, t);
+                float2 p012 = lerp2(p01, p12, t), p123 = lerp2(p12, p23, t);
+                float2 p0123 = lerp2(p012, p123, t);
+                a = Seg4{s.p0, p01, p012, p0123};
+                bseg = Seg4{p0123, p123, p23, s.p1};
+            }
+            stack[sp] = bseg;
+            depth[sp] = d + 1;
+            stack[sp + 1] = a;
+            depth[sp + 1] = d + 1;
+            sp += 2;
+        }
+    }
+};
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_tile_segments(BatchDev b) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.n_segs) return;
+    int k = find_scene(b.seg_bases, b.n_scenes, g);
+    const SceneDev &sc = b.scenes[k];
+    uint32_t s = g - sc.seg_base;
+    uint2 sg = sc.segs[s];
+    uint32_t path = sc.seg_path[s];
+    int4 rect = sc.path_rect[path];
+    Walker<EMIT> w;
+    w.vb_l = sc.vb_l;
+    w.vb_r = sc.vb_r;
+    w.vb_b = sc.vb_b;
+    w.tl = rect.x;
+    w.tt = rect.y;
+    w.tr = rect.z;
+    w.tb = rect.w;
+    w.tw = rect.z - rect.x;
+    w.th = rect.w - rect.y;
+    w.tbase = sc.tile_base + sc.path_tile_off[path];
+    w.cbase = sc.col_base + sc.path_col_off[path];
+    w.b = &b;
+    w.count = 0;
+    w.guard = 0;
+    w.dump_base = EMIT && b.dump ? b.seg_cnt[g] : 0u;
+    // With no columns nothing can land (add_fill culls, adjust_alpha_tile_backdrop ignores
+    // x outside [0, width)); with columns but no rows, above-rect backdrops still accumulate.
+    if (w.tw <= 0) {
+        if (!EMIT) b.seg_cnt[g] = 0;
+        return;
+    }
+    const float2 *P = sc.pts + sg.x;
+    Seg4 root;
+    root.p0 = P[0];
+    if (sg.y == SEG_LINE) {
+        root.p1 = P[1];
+    } else if (sg.y == SEG_QUAD) {
+        root.c0 = P[1];
+        root.c1 = P[1];
+        root.p1 = P[2];
+    } else {
+        root.c0 = P[1];
+        root.c1 = P[2];
+        root.p1 = P[3];
+    }
+    // A non-finite point makes every flattened piece non-finite, and process_line_segment
+    // drops those (tiler.cpp:133-136): skip the segment outright.
+    bool finite = isfinite(root.p0.x) && isfinite(root.p0.y) && isfinite(root.p1.x) && isfinite(root.p1.y);
+    if (sg.y != SEG_LINE) finite = finite && isfinite(root.c0.x) && isfinite(root.c0.y);
+    if (sg.y == SEG_CUBIC) finite = finite && isfinite(root.c1.x) && isfinite(root.c1.y);
+    if (finite) w.segment(root, sg.y);
+    if (!EMIT) {
+        b.seg_cnt[g] = w.count;
+        if (w.guard) atomicAdd(&b.counters[C_GUARD], w.guard);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan (reduce / scan block sums / apply), 4096 items per block
+// ---------------------------------------------------------------------------
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+struct U2 {
+    uint32_t x, y;
+    __device__ __forceinline__ U2 operator+(const U2 &o) const { return U2{x + o.x, y + o.y}; }
+};
+
+__device__ __forceinline__ uint32_t shfl_up_val(uint32_t v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ U2 shfl_up_val(U2 v, int d) { return U2{__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64)}; }
+
+template <typename T>
+__device__ __forceinline__ T zero_of() { return T{}; }
+
+// Exclusive scan of one value per thread across a 256-thread block; returns the block total.
+template <typename T>
+__device__ T block_exclusive_scan(T v, T &total) {
+    __shared__ T wave_tot[SCAN_THREADS / 64];
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = shfl_up_val(inc, d);
+        if (lane >= d) inc = inc + o;
+    }
+    if (lane == 63) wave_tot[wave] = inc;
+    __syncthreads();
+    T base = zero_of<T>();
+    T all = zero_of<T>();
+#pragma unroll
+    for (int w = 0; w < SCAN_THREADS / 64; w++) {
+        if (w < wave) base = base + wave_tot[w];
+        all = all + wave_tot[w];
+    }
+    __syncthreads();
+    total = all;
+    T excl_in_wave = shfl_up_val(inc, 1);
+    if (lane == 0) excl_in_wave = zero_of<T>();
+    return base + excl_in_wave;
+}
+
+template <typename T, typename Load>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Load ld, uint32_t n, T *block_sums) {
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    T s = zero_of<T>();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t j = base + i;
+        if (j < n) s = s + ld((uint32_t)j);
+    }
+    T total;
+    block_exclusive_scan<T>(s, total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of nb block sums in place; block_sums[nb] = grand total.
+template <typename T>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(T *block_sums, uint32_t nb) {
+    T carry = zero_of<T>();
+    for (uint32_t start = 0; start < nb; start += SCAN_THREADS) {
+        uint32_t i = start + threadIdx.x;
+        T v = i < nb ? block_sums[i] : zero_of<T>();
+        T total;
+        T ex = block_exclusive_scan<T>(v, total);
+        if (i < nb) block_sums[i] = carry + ex;
+        carry = carry + total;
+    }
+    if (threadIdx.x == 0) block_sums[nb] = carry;
+}
+
+template <typename T, typename Load, typename Store>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Load ld, Store st, uint32_t n, const T *block_sums,
+                                                             uint32_t nb) {
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    T vals[SCAN_ITEMS];
+    T s = zero_of<T>();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t j = base + i;
+        vals[i] = j < n ? ld((uint32_t)j) : zero_of<T>();
+        s = s + vals[i];
+    }
+    T total;
+    T run = block_exclusive_scan<T>(s, total) + block_sums[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t j = base + i;
+        if (j < n) st((uint32_t)j, run);
+        run = run + vals[i];
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) st(n, block_sums[nb]);
+}
+
+// Scan functors --------------------------------------------------------------
+
+struct LoadU32 {
+    const uint32_t *p;
+    __device__ uint32_t operator()(uint32_t i) const { return p[i]; }
+};
+struct StoreU32 {
+    uint32_t *p;
+    __device__ void operator()(uint32_t i, uint32_t v) const { p[i] = v; }
+};
+// Tiles: (fill count, has fills) -> fill offsets, alpha indices; totals -> counters.
+struct LoadTileCnt {
+    const uint32_t *cnt;
+    __device__ U2 operator()(uint32_t i) const {
+        uint32_t c = cnt[i];
+        return U2{c, c > 0 ? 1u : 0u};
+    }
+};
+struct StoreTileOff {
+    uint32_t *fill_off, *alpha_idx, *counters;
+    uint32_t n;
+    __device__ void operator()(uint32_t i, U2 v) const {
+        fill_off[i] = v.x;
+        alpha_idx[i] = v.y;
+        if (i == n) {
+            counters[C_FILLS] = v.x;
+            counters[C_ALPHA] = v.y;
+        }
+    }
+};
+struct StoreEntryOff {
+    uint32_t *off, *counters;
+    uint32_t n;
+    __device__ void operator()(uint32_t i, uint32_t v) const {
+        off[i] = v;
+        if (i == n) counters[C_ENTRIES] = v;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Backdrop propagation down each path column (tiler.cpp:369-439) + z-buffer
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b.n_cols) return;
+    int k = find_scene(b.col_bases, b.n_scenes, c);
+    const SceneDev &sc = b.scenes[k];
+    uint32_t cl = c - sc.col_base;
+    uint32_t p = sc.col_path[cl];
+    int4 r = sc.path_rect[p];
+    int w = r.z - r.x, h = r.w - r.y;
+    int x = (int)(cl - sc.path_col_off[p]);
+    uint32_t tbase = sc.tile_base + sc.path_tile_off[p];
+    uint32_t rgba = sc.path_rgba[p];
+    uint32_t rule = sc.path_rule[p];
+    bool opaque = (rgba >> 24) == 255u;
+    int32_t running = b.col_bd[c];
+    for (int row = 0; row < h; row++) {
+        uint32_t t = tbase + (uint32_t)(row * w + x);
+        int8_t bd = (int8_t)running;
+        running += (int8_t)b.tile_delta[t];
+        b.tile_bd[t] = bd;
+        uint32_t f0 = b.tile_fill_off[t], f1 = b.tile_fill_off[t + 1];
+        if (f1 > f0) {
+            uint32_t a = b.tile_alpha_idx[t];
+            if (a < b.cap_alpha) b.alpha_rec[a] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd | (rule << 8), t);
+        } else if (opaque && bd != 0 && !(rule && (bd & 1) == 0)) {
+            int ty = r.y + row, tx = r.x + x;
+            uint32_t fb = sc.fb_base + (uint32_t)((ty - sc.fb_y0) * sc.fb_w + (tx - sc.fb_x0));
+            atomicMax(&b.zbuf[fb], p);
+        }
+    }
+    b.col_bd[c] = running;
+}
+
+// Draw entries: tiles with a mask or a nonzero backdrop that survive the z-cull
+// (core/d3d9/scene_builder.cpp:57-79, shaders/d3d9/tile.vert:103-118).
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_entries(BatchDev b) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b.n_cols) return;
+    int k = find_scene(b.col_bases, b.n_scenes, c);
+    const SceneDev &sc = b.scenes[k];
+    uint32_t cl = c - sc.col_base;
+    uint32_t p = sc.col_path[cl];
+    int4 r = sc.path_rect[p];
+    int w = r.z - r.x, h = r.w - r.y;
+    int x = (int)(cl - sc.path_col_off[p]);
+    uint32_t tbase = sc.tile_base + sc.path_tile_off[p];
+    uint32_t rgba = sc.path_rgba[p];
+    uint32_t rule = sc.path_rule[p];
+    for (int row = 0; row < h; row++) {
+        uint32_t t = tbase + (uint32_t)(row * w + x);
+        int ty = r.y + row, tx = r.x + x;
+        uint32_t fbl = (uint32_t)((ty - sc.fb_y0) * sc.fb_w + (tx - sc.fb_x0));
+        uint32_t fb = sc.fb_base + fbl;
+        if (!WRITE) {
+            bool alpha = b.tile_fill_off[t + 1] > b.tile_fill_off[t];
+            int8_t bd = b.tile_bd[t];
+            bool keep = (alpha || bd != 0) && (int32_t)p >= (int32_t)b.zbuf[fb];
+            b.tile_entry[t] = keep ? 1u : 0u;
+        } else {
+            uint32_t e = b.tile_entry[t];
+            if (b.tile_entry[t + 1] == e) continue;
+            if (e >= b.cap_entries) continue;
+            bool alpha = b.tile_fill_off[t + 1] > b.tile_fill_off[t];
+            uint32_t ai = alpha ? b.tile_alpha_idx[t] : NO_ALPHA;
+            uint32_t eff = rgba;
+            if (!alpha) {
+                int8_t bd = b.tile_bd[t];
+                if (rule && (bd & 1) == 0) eff &= 0x00ffffffu;  // even-odd, even winding: covers nothing
+            }
+            b.keys[e] = fb;
+            b.vals[e] = e;
+            b.entry_rec[e] = make_uint4(ai, eff, p, fbl);
+            atomicAdd(&b.fb_cnt[fb], 1u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (key, value) u32 pairs, 8 bits per pass
+// ---------------------------------------------------------------------------
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t *keys, const uint32_t *n_ptr, uint32_t cap,
+                                                        int shift, uint32_t *hist, uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t n = min(*n_ptr, cap);
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int i = 0; i < RS_ITEMS; i++) {
+        uint64_t j = base + (uint64_t)i * RS_THREADS + threadIdx.x;
+        if (j < n) atomicAdd(&h[(keys[j] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *keys, const uint32_t *vals,
+                                                           uint32_t *keys_out, uint32_t *vals_out,
+                                                           const uint32_t *n_ptr, uint32_t cap, int shift,
+                                                           const uint32_t *offs, uint32_t nblocks) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t n = min(*n_ptr, cap);
+    run[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < RS_ITEMS; r++) {
+        for (int q = threadIdx.x; q < (RS_THREADS / 64) * 256; q += RS_THREADS) (&wcnt[0][0])[q] = 0;
+        __syncthreads();
+        uint64_t j = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+        bool valid = j < n;
+        uint32_t key = valid ? keys[j] : 0u;
+        uint32_t val = valid ? vals[j] : 0u;
+        uint32_t d = (key >> shift) & 255u;
+        // lanes holding the same digit (multisplit peer mask)
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            uint64_t m = __ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? m : ~m;
+        }
+        uint32_t rank = __popcll(peers & lt_mask);
+        uint32_t cnt = __popcll(peers);
+        if (valid && rank == cnt - 1) wcnt[wave][d] = cnt;
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int w = 0; w < wave; w++) pos += wcnt[w][d];
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < RS_THREADS / 64; w++) add += wcnt[w][threadIdx.x];
+        run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coverage masks: one wave per alpha tile, 4 pixels (one row) per lane
+// ---------------------------------------------------------------------------
+
+// Signed area of a fill over pixel (px, py): sign(from.x - to.x) * ∫ clamp(py + 1 - y(x), 0, 1) dx
+// over [px, px + 1] ∩ [xmin, xmax] — the box-filtered quantity the reference's area LUT
+// tabulates (shaders/d3d9/fill.frag:25-45, shaders/d3d11/fill.comp:51-71), evaluated analytically.
+__device__ __forceinline__ float fill_area(float xl, float yl, float xr, float slope, float sign, float px,
+                                           float py1) {
+    float lo = fmaxf(px, xl), hi = fminf(px + 1.0f, xr);
+    if (!(hi > lo)) return 0.0f;
+    float ua = py1 - (yl + (lo - xl) * slope);
+    float ub = py1 - (yl + (hi - xl) * slope);
+    float a = fminf(ua, ub), bb = fmaxf(ua, ub);
+    float avg;
+    float du = bb - a;
+    if (du <= 0.0f) {
+        avg = fminf(fmaxf(a, 0.0f), 1.0f);
+    } else {
+        float full = bb > 1.0f ? bb - fmaxf(a, 1.0f) : 0.0f;
+        float l2 = fmaxf(a, 0.0f), h2 = fminf(bb, 1.0f);
+        float mid = h2 > l2 ? (h2 - l2) * (h2 + l2) * 0.5f : 0.0f;
+        avg = (full + mid) / du;
+    }
+    return sign * (hi - lo) * avg;
+}
+
+__device__ __forceinline__ float apply_rule(float c, uint32_t even_odd) {
+    if (!even_odd) return fminf(fabsf(c), 1.0f);
+    float m = c - 2.0f * floorf(c * 0.5f);  // GLSL mod(c, 2)
+    return fminf(fmaxf(1.0f - fabsf(1.0f - m), 0.0f), 1.0f);
+}
+
+__global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
+    const int lane = threadIdx.x & 63;
+    uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint32_t na = min(b.counters[C_ALPHA], b.cap_alpha);
+    if (a >= na) return;
+    uint4 rec = b.alpha_rec[a];
+    const float py1 = (float)(lane >> 2) + 1.0f;
+    const float px0 = (float)((lane & 3) * 4);
+    int32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+    const float scale = (float)(1 << FIX_SHIFT);
+    uint32_t f1 = min(rec.y, b.cap_fills);
+    for (uint32_t f = rec.x; f < f1; f++) {
+        uint2 fl = b.fills[f];
+        float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), y0 = (float)(fl.x >> 16) * (1.0f / 256.0f);
+        float x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f), y1 = (float)(fl.y >> 16) * (1.0f / 256.0f);
+        float sign = x0 > x1 ? 1.0f : -1.0f;
+        float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
+        float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
+        float slope = (yr - yl) / (xr - xl);
+        acc0 += (int32_t)rintf(fill_area(xl, yl, xr, slope, sign, px0 + 0.0f, py1) * scale);
+        acc1 += (int32_t)rintf(fill_area(xl, yl, xr, slope, sign, px0 + 1.0f, py1) * scale);
+        acc2 += (int32_t)rintf(fill_area(xl, yl, xr, slope, sign, px0 + 2.0f, py1) * scale);
+        acc3 += (int32_t)rintf(fill_area(xl, yl, xr, slope, sign, px0 + 3.0f, py1) * scale);
+    }
+    const float bd = (float)(int8_t)(rec.z & 0xffu);
+    const uint32_t rule = (rec.z >> 8) & 1u;
+    const float inv = 1.0f / scale;
+    uint32_t m0 = (uint32_t)rintf(apply_rule(bd + (float)acc0 * inv, rule) * 255.0f);
+    uint32_t m1 = (uint32_t)rintf(apply_rule(bd + (float)acc1 * inv, rule) * 255.0f);
+    uint32_t m2 = (uint32_t)rintf(apply_rule(bd + (float)acc2 * inv, rule) * 255.0f);
+    uint32_t m3 = (uint32_t)rintf(apply_rule(bd + (float)acc3 * inv, rule) * 255.0f);
+    reinterpret_cast<uint32_t *>(b.masks + (uint64_t)a * 256)[lane] = m0 | (m1 << 8) | (m2 << 16) | (m3 << 24);
+}
+
+// ---------------------------------------------------------------------------
+// Composite: one wave per framebuffer tile, 4 pixels (one row) per lane
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *sorted_vals) {
+    const int lane = threadIdx.x & 63;
+    uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= b.n_fb) return;
+    int k = find_scene(b.fb_bases, b.n_scenes, f);
+    const SceneDev &sc = b.scenes[k];
+    uint32_t fl = f - sc.fb_base;
+    int tx = sc.fb_x0 + (int)(fl % (uint32_t)sc.fb_w), ty = sc.fb_y0 + (int)(fl / (uint32_t)sc.fb_w);
+    const int row = lane >> 2, col = (lane & 3) * 4;
+    const int y = ty * TILE + row, x0 = tx * TILE + col;
+    float r[4], g[4], bl[4], al[4];
+    if (sc.load_dst && y < sc.height) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uint32_t px = 0;
+            if (x0 + j < sc.width) px = *reinterpret_cast<const uint32_t *>(sc.out + (int64_t)y * sc.out_pitch + (x0 + j) * 4);
+            r[j] = (float)(px & 255u) * (1.0f / 255.0f);
+            g[j] = (float)((px >> 8) & 255u) * (1.0f / 255.0f);
+            bl[j] = (float)((px >> 16) & 255u) * (1.0f / 255.0f);
+            al[j] = (float)(px >> 24) * (1.0f / 255.0f);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) r[j] = g[j] = bl[j] = al[j] = 0.0f;
+    }
+    uint32_t e0 = b.fb_cnt[f], e1 = b.fb_cnt[f + 1];
+    uint32_t ecap = b.cap_entries;
+    if (e1 > ecap) e1 = ecap;
+    for (uint32_t base = e0; base < e1; base += 64) {
+        uint32_t i = base + lane;
+        uint32_t my_alpha = NO_ALPHA, my_rgba = 0;
+        if (i < e1) {
+            uint4 rec = b.entry_rec[sorted_vals[i]];
+            my_alpha = rec.x;
+            my_rgba = rec.y;
+        }
+        uint32_t cnt = min(64u, e1 - base);
+        for (uint32_t j = 0; j < cnt; j++) {
+            uint32_t ai = __builtin_amdgcn_readlane(my_alpha, j);
+            if (ai != NO_ALPHA && ai >= b.cap_alpha) continue;  // capacity overflow: the run is retried
+            uint32_t c = __builtin_amdgcn_readlane(my_rgba, j);
+            float ca = (float)(c >> 24) * (1.0f / 255.0f);
+            if (ca == 0.0f) continue;
+            float cr = (float)(c & 255u) * (1.0f / 255.0f);
+            float cg = (float)((c >> 8) & 255u) * (1.0f / 255.0f);
+            float cb = (float)((c >> 16) & 255u) * (1.0f / 255.0f);
+            if (ai == NO_ALPHA) {
+                float k1 = 1.0f - ca;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    r[q] = cr * ca + r[q] * k1;
+                    g[q] = cg * ca + g[q] * k1;
+                    bl[q] = cb * ca + bl[q] * k1;
+                    al[q] = ca + al[q] * k1;
+                }
+            } else {
+                uint32_t m = reinterpret_cast<const uint32_t *>(b.masks + (uint64_t)ai * 256)[lane];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    float cov = (float)((m >> (8 * q)) & 255u) * (1.0f / 255.0f);
+                    float sa = ca * cov;
+                    float k1 = 1.0f - sa;
+                    r[q] = cr * sa + r[q] * k1;
+                    g[q] = cg * sa + g[q] * k1;
+                    bl[q] = cb * sa + bl[q] * k1;
+                    al[q] = sa + al[q] * k1;
+                }
+            }
+        }
+    }
+    if (y >= sc.height) return;
+    uint8_t *orow = sc.out + (int64_t)y * sc.out_pitch;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        int x = x0 + q;
+        if (x >= sc.width) break;
+        uint32_t pr = (uint32_t)rintf(fminf(fmaxf(r[q], 0.0f), 1.0f) * 255.0f);
+        uint32_t pg = (uint32_t)rintf(fminf(fmaxf(g[q], 0.0f), 1.0f) * 255.0f);
+        uint32_t pb = (uint32_t)rintf(fminf(fmaxf(bl[q], 0.0f), 1.0f) * 255.0f);
+        uint32_t pa = (uint32_t)rintf(fminf(fmaxf(al[q], 0.0f), 1.0f) * 255.0f);
+        reinterpret_cast<uint32_t *>(orow)[x] = pr | (pg << 8) | (pb << 16) | (pa << 24);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers (host side)
+// ---------------------------------------------------------------------------
+
+static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint32_t nb = (uint32_t)std::min<uint64_t>(blocks_for(n, 256), 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3(nb), dim3(256), 0, st, p, value, n);
+    return hipGetLastError();
+}
+
+size_t scan_temp_elems(uint64_t n) { return blocks_for(n, SCAN_TILE) + 1; }
+
+template <typename T, typename Load, typename Store>
+static hipError_t scan_impl(Load ld, Store st, uint32_t n, T *tmp, hipStream_t s) {
+    uint32_t nb = blocks_for(n, SCAN_TILE);
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL((k_scan_reduce<T, Load>), dim3(nb), dim3(SCAN_THREADS), 0, s, ld, n, tmp);
+    hipLaunchKernelGGL((k_scan_blocks<T>), dim3(1), dim3(SCAN_THREADS), 0, s, tmp, nb);
+    hipLaunchKernelGGL((k_scan_apply<T, Load, Store>), dim3(nb), dim3(SCAN_THREADS), 0, s, ld, st, n, tmp, nb);
+    return hipGetLastError();
+}
+
+// Exclusive scan of in[0..n) into out[0..n] (out[n] = total).  in may equal out.
+hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp, hipStream_t s) {
+    return scan_impl<uint32_t>(LoadU32{in}, StoreU32{out}, n, tmp, s);
+}
+
+hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, uint32_t *tmp, hipStream_t s) {
+    return scan_impl<uint32_t>(LoadU32{flags_inout}, StoreEntryOff{flags_inout, counters, n}, n, tmp, s);
+}
+
+hipError_t scan_tiles(const BatchDev &b, void *tmp, hipStream_t s) {
+    return scan_impl<U2>(LoadTileCnt{b.tile_cnt}, StoreTileOff{b.tile_fill_off, b.tile_alpha_idx, b.counters, b.n_tiles},
+                         b.n_tiles, reinterpret_cast<U2 *>(tmp), s);
+}
+
+hipError_t launch_tile_segments(const BatchDev &b, bool emit, hipStream_t s) {
+    if (b.n_segs == 0) return hipSuccess;
+    if (emit) hipLaunchKernelGGL(k_tile_segments<true>, dim3(blocks_for(b.n_segs, 256)), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL(k_tile_segments<false>, dim3(blocks_for(b.n_segs, 256)), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_propagate(const BatchDev &b, hipStream_t s) {
+    if (b.n_cols == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_propagate, dim3(blocks_for(b.n_cols, 256)), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s) {
+    if (b.n_cols == 0) return hipSuccess;
+    if (write) hipLaunchKernelGGL(k_entries<true>, dim3(blocks_for(b.n_cols, 256)), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL(k_entries<false>, dim3(blocks_for(b.n_cols, 256)), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+size_t radix_temp_elems(uint64_t cap) { return 256ull * blocks_for(cap, RS_TILE) + 1 + scan_temp_elems(256ull * blocks_for(cap, RS_TILE)); }
+
+// Sorts (keys, vals)[0 .. *n_ptr) by the low `bits` bits of the key; returns the buffer
+// pair holding the result via *result_alt (false: keys/vals, true: keys_alt/vals_alt).
+hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, const uint32_t *n_ptr,
+                      uint32_t cap, int bits, uint32_t *tmp, bool *result_alt, hipStream_t s) {
+    uint32_t nb = blocks_for(cap, RS_TILE);
+    if (nb == 0) nb = 1;
+    uint32_t nh = 256 * nb;
+    uint32_t *hist = tmp;
+    uint32_t *scan_tmp = tmp + nh + 1;
+    bool alt = false;
+    for (int shift = 0; shift < bits; shift += 8) {
+        const uint32_t *ki = alt ? keys_alt : keys;
+        const uint32_t *vi = alt ? vals_alt : vals;
+        uint32_t *ko = alt ? keys : keys_alt;
+        uint32_t *vo = alt ? vals : vals_alt;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, ki, n_ptr, cap, shift, hist, nb);
+        hipError_t e = scan_u32(hist, hist, nh, scan_tmp, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n_ptr, cap, shift, hist, nb);
+        alt = !alt;
+    }
+    *result_alt = alt;
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, hipStream_t s) {
+    if (grid_alpha == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_masks, dim3(blocks_for(grid_alpha, 4)), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, hipStream_t s) {
+    if (b.n_fb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_composite, dim3(blocks_for(b.n_fb, 4)), dim3(256), 0, s, b, sorted_vals);
+    return hipGetLastError();
+}
+
+}  // namespace pfh

@@ -1,0 +1,220 @@
+"""ctypes binding of the C ABI in include/pfh.h (libpfh.so, built in-tree).
+
+The product path is the HIP library; this module only marshals numpy arrays
+across the ABI.  It fails loudly when libpfh.so is missing: there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpfh.so")
+
+PFH_PLAIN = 0x0
+PFH_BATCHED = 0x1
+PFH_PREDICT = 0x2
+PFH_DUMP = 0x4
+PFH_LOAD_DST = 0x8
+PFH_NO_GRAPH = 0x10
+
+N_FEATURES = 12
+MLP_PARAMS = 5122
+PREDICTOR_FLOATS = 24 + MLP_PARAMS
+
+_P = ctypes.c_void_p
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("view_box", ctypes.c_float * 4),
+                ("n_paths", ctypes.c_int32), ("n_contours", ctypes.c_int32), ("n_points", ctypes.c_int32),
+                ("points", _P), ("point_flags", _P), ("contour_start", _P), ("contour_closed", _P),
+                ("path_contour_start", _P), ("path_fill_rule", _P), ("path_rgba", _P)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("paths", ctypes.c_int64), ("segments", ctypes.c_int64), ("tiles", ctypes.c_int64),
+                ("columns", ctypes.c_int64), ("fills", ctypes.c_int64), ("alpha_tiles", ctypes.c_int64),
+                ("entries", ctypes.c_int64), ("fb_tiles", ctypes.c_int64), ("retries", ctypes.c_int64),
+                ("guard_hits", ctypes.c_int64), ("ms_total", ctypes.c_double), ("ms_stage", ctypes.c_double * 8),
+                ("capacity_fills", ctypes.c_int64), ("capacity_alpha", ctypes.c_int64),
+                ("capacity_entries", ctypes.c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "ms_stage"}
+        d["ms_stage"] = list(self.ms_stage)
+        return d
+
+
+EXPORTS = ("pfh_create", "pfh_destroy", "pfh_last_error", "pfh_scene_upload", "pfh_scene_release",
+           "pfh_scene_features", "pfh_rasterize", "pfh_scene_set_target", "pfh_scene_target", "pfh_readback_rgba8",
+           "pfh_dump_counts", "pfh_dump", "pfh_stats_get", "pfh_set_predictor", "pfh_mlp_forward",
+           "pfh_mlp_backward", "pfh_mlp_sgd")
+
+_lib = None
+
+
+class PfhError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PfhError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    i32, i64, f32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_float
+    sig = {
+        "pfh_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+        "pfh_destroy": (None, [_P]),
+        "pfh_last_error": (ctypes.c_char_p, [_P]),
+        "pfh_scene_upload": (ctypes.c_int, [_P, ctypes.POINTER(SceneDesc), ctypes.POINTER(i32)]),
+        "pfh_scene_release": (ctypes.c_int, [_P, i32]),
+        "pfh_scene_features": (ctypes.c_int, [_P, i32, _P]),
+        "pfh_rasterize": (ctypes.c_int, [_P, _P, i32, ctypes.c_uint32]),
+        "pfh_scene_set_target": (ctypes.c_int, [_P, i32, _P, i64]),
+        "pfh_scene_target": (ctypes.c_int, [_P, i32, ctypes.POINTER(_P), ctypes.POINTER(i64)]),
+        "pfh_readback_rgba8": (ctypes.c_int, [_P, i32, _P, i64]),
+        "pfh_dump_counts": (ctypes.c_int, [_P, i32, _P]),
+        "pfh_dump": (ctypes.c_int, [_P, i32] + [_P] * 14),
+        "pfh_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+        "pfh_set_predictor": (ctypes.c_int, [_P, _P, i64, f32]),
+        "pfh_mlp_forward": (ctypes.c_int, [_P, _P, _P, i64, _P, _P]),
+        "pfh_mlp_backward": (ctypes.c_int, [_P, _P, _P, i64, _P, _P, _P, _P, _P]),
+        "pfh_mlp_sgd": (ctypes.c_int, [_P, _P, _P, _P, _P, i64, f32, i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One HIP context of the rasterizer (pfh_create)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = _P()
+        rc = self.lib.pfh_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise PfhError(f"pfh_create({device}) failed with {rc}")
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pfh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.pfh_last_error(self.h)
+            raise PfhError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def upload(self, scene) -> int:
+        d = SceneDesc()
+        d.width, d.height = int(scene.width), int(scene.height)
+        for i in range(4):
+            d.view_box[i] = float(scene.view_box[i])
+        d.n_paths = scene.n_paths
+        d.n_contours = int(scene.contour_closed.shape[0])
+        d.n_points = int(scene.point_flags.shape[0])
+        d.points = _ptr(scene.points)
+        d.point_flags = _ptr(scene.point_flags)
+        d.contour_start = _ptr(scene.contour_start)
+        d.contour_closed = _ptr(scene.contour_closed)
+        d.path_contour_start = _ptr(scene.path_contour_start)
+        d.path_fill_rule = _ptr(scene.path_fill_rule)
+        d.path_rgba = _ptr(scene.path_rgba)
+        sid = ctypes.c_int32()
+        self._check(self.lib.pfh_scene_upload(self.h, ctypes.byref(d), ctypes.byref(sid)), "pfh_scene_upload")
+        self._keep[sid.value] = (int(scene.width), int(scene.height), scene.n_paths)
+        return sid.value
+
+    def release(self, sid: int):
+        self._check(self.lib.pfh_scene_release(self.h, sid), "pfh_scene_release")
+        self._keep.pop(sid, None)
+
+    def rasterize(self, ids, flags: int = PFH_PLAIN):
+        arr = np.ascontiguousarray(np.atleast_1d(np.asarray(ids, np.int32)))
+        self._check(self.lib.pfh_rasterize(self.h, _ptr(arr), int(arr.shape[0]), flags), "pfh_rasterize")
+
+    def readback(self, sid: int) -> np.ndarray:
+        w, h, _ = self._keep[sid]
+        out = np.zeros((h, w, 4), np.uint8)
+        self._check(self.lib.pfh_readback_rgba8(self.h, sid, _ptr(out), w * 4), "pfh_readback_rgba8")
+        return out
+
+    def target(self, sid: int):
+        p, pitch = _P(), ctypes.c_int64()
+        self._check(self.lib.pfh_scene_target(self.h, sid, ctypes.byref(p), ctypes.byref(pitch)), "pfh_scene_target")
+        return p.value, pitch.value
+
+    def set_target(self, sid: int, device_ptr, pitch: int):
+        self._check(self.lib.pfh_scene_set_target(self.h, sid, _P(device_ptr), pitch), "pfh_scene_set_target")
+
+    def features(self, sid: int) -> np.ndarray:
+        n = self._keep[sid][2]
+        out = np.zeros((n, N_FEATURES), np.float32)
+        self._check(self.lib.pfh_scene_features(self.h, sid, _ptr(out)), "pfh_scene_features")
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self.lib.pfh_stats_get(self.h, ctypes.byref(s)), "pfh_stats_get")
+        return s.as_dict()
+
+    def dump(self, sid: int) -> dict:
+        """Canonical intermediates (the oracle's layout) of the last PFH_DUMP rasterize."""
+        counts = np.zeros(8, np.int64)
+        self._check(self.lib.pfh_dump_counts(self.h, sid, _ptr(counts)), "pfh_dump_counts")
+        T, W, F, E, NFB, P, A = [int(x) for x in counts[:7]]
+        out = dict(
+            path_rect=np.zeros((P, 4), np.int32), tile_off=np.zeros(P + 1, np.int64),
+            col_off=np.zeros(P + 1, np.int64), fill_off=np.zeros(P + 1, np.int64),
+            tile_backdrop=np.zeros(T, np.int8), tile_alpha=np.zeros(T, np.uint8), col_sum=np.zeros(W, np.int32),
+            fill_seg=np.zeros((F, 4), np.uint16), fill_tile=np.zeros(F, np.uint32),
+            entry_fbtile=np.zeros(E, np.uint32), entry_path=np.zeros(E, np.uint32), zbuf=np.zeros(NFB, np.uint32),
+            fb_rect=np.zeros(4, np.int32), masks=np.zeros((A, 16, 16), np.uint8))
+        keys = ("path_rect", "tile_off", "col_off", "fill_off", "tile_backdrop", "tile_alpha", "col_sum", "fill_seg",
+                "fill_tile", "entry_fbtile", "entry_path", "zbuf", "fb_rect", "masks")
+        self._check(self.lib.pfh_dump(self.h, sid, *[_ptr(out[k]) for k in keys]), "pfh_dump")
+        out["guard_hits"] = int(counts[7])
+        return out
+
+    def set_predictor(self, weights, safety: float = 1.25):
+        if weights is None:
+            self._check(self.lib.pfh_set_predictor(self.h, None, 0, safety), "pfh_set_predictor")
+            return
+        w = np.ascontiguousarray(weights, np.float32)
+        self._check(self.lib.pfh_set_predictor(self.h, _ptr(w), int(w.shape[0]), safety), "pfh_set_predictor")
+
+    # MLP kernels on device pointers (torch tensors or raw addresses)
+    def mlp_forward(self, params, x, n, hidden, out):
+        self._check(self.lib.pfh_mlp_forward(self.h, _P(params), _P(x), int(n), _P(hidden), _P(out)),
+                    "pfh_mlp_forward")
+
+    def mlp_backward(self, params, x, n, hidden, out, target, grad, loss):
+        self._check(self.lib.pfh_mlp_backward(self.h, _P(params), _P(x), int(n), _P(hidden), _P(out), _P(target),
+                                              _P(grad), _P(loss)), "pfh_mlp_backward")
+
+    def mlp_adam(self, params, m, v, grad, n_params, lr, step):
+        self._check(self.lib.pfh_mlp_sgd(self.h, _P(params), _P(m), _P(v), _P(grad), int(n_params), float(lr),
+                                         int(step)), "pfh_mlp_sgd")

@@ -1,0 +1,120 @@
+"""HIP pipeline vs the oracle, through the C ABI.
+
+Bit-exact: tile rects, per-tile backdrops and alpha flags, column sums, fills
+(tile-local 8.8 segments) in emission order per path, z-buffer and per
+framebuffer-tile draw order.  Tolerances (stated): coverage masks within 1/255
+of the FP64 model (u8 quantisation 0.5/255 + fp32 arithmetic); RGBA8 within 1 LSB
+of round(FP64 composite * 255).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import pfh
+import scenegen
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+COV_TOL = 1.0 / 255 + 1e-6
+RGBA_TOL = 1
+
+GOLDEN = load_golden()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pfh.Context(0)
+    yield c
+    c.close()
+
+
+def check_scene(ctx, scene, flags=pfh.PFH_PLAIN, name=""):
+    print(f"[parity] {name} paths={scene.n_paths} canvas={scene.width}x{scene.height} flags={flags}", flush=True)
+    sid = ctx.upload(scene)
+    ctx.rasterize([sid], flags | pfh.PFH_DUMP)
+    got = ctx.dump(sid)
+    img = ctx.readback(sid)
+    model = oracle.full_model(scene, zmode=1)
+    canon = model["canon"]
+    assert got["guard_hits"] == canon["guard_hits"] == 0
+    for k in oracle.CANONICAL_KEYS:
+        np.testing.assert_array_equal(got[k], canon[k], err_msg=f"{name}: {k}")
+    if got["masks"].size:
+        err = np.abs(got["masks"].astype(np.float64) / 255.0 - model["coverage"]).max()
+        assert err <= COV_TOL, f"{name}: coverage error {err * 255:.3f}/255"
+    want = np.rint(np.clip(model["image"], 0.0, 1.0) * 255.0).astype(np.int64)
+    px = np.abs(img.astype(np.int64) - want).max() if img.size else 0
+    assert px <= RGBA_TOL, f"{name}: rgba8 error {px}"
+    ctx.release(sid)
+    return got, img
+
+
+@pytest.mark.parametrize("name", list(GOLDEN))
+def test_golden_scene_parity(ctx, name):
+    scene, _ = GOLDEN[name]
+    check_scene(ctx, scene, name=name)
+
+
+@pytest.mark.parametrize("seed,size,paths", [(11, 1024, 300), (12, 2048, 1500), (13, 4096, 2500)])
+def test_blob_scene_parity(ctx, seed, size, paths):
+    check_scene(ctx, scenegen.blob_scene(seed, size, size, paths, r_max=min(512.0, size / 2)),
+                name=f"blob{size}")
+
+
+def test_star_scene_parity(ctx):
+    check_scene(ctx, scenegen.star_scene(seed=5, size=1024, n_stars=40, n_max=600, m_max=30, r_max=400.0),
+                name="stars")
+
+
+def test_empty_and_degenerate_scenes(ctx):
+    empty = scenegen.Scene(64, 48, np.zeros((0, 2), np.float32), np.zeros(0, np.uint8), np.array([0]),
+                           np.zeros(0, np.uint8), np.array([0]), np.zeros(0, np.uint8), np.zeros(0, np.uint32))
+    sid = ctx.upload(empty)
+    ctx.rasterize([sid])
+    assert (ctx.readback(sid) == 0).all()
+    ctx.release(sid)
+    # a path with an empty contour and one fully outside the view box
+    sc = GOLDEN["fully_outside"][0]
+    check_scene(ctx, sc, name="fully_outside")
+
+
+def test_batched_equals_plain(ctx):
+    scenes = scenegen.batch_scenes(n_scenes=6, seed=123, side_max=1024, paths_max=800)
+    ids = [ctx.upload(s) for s in scenes]
+    plain = []
+    for sid in ids:
+        ctx.rasterize([sid], pfh.PFH_PLAIN | pfh.PFH_DUMP)
+        plain.append((ctx.readback(sid), ctx.dump(sid)))
+    for flags in (pfh.PFH_BATCHED | pfh.PFH_NO_GRAPH, pfh.PFH_BATCHED, pfh.PFH_BATCHED):
+        ctx.rasterize(ids, flags | pfh.PFH_DUMP)
+        for sid, (img, d) in zip(ids, plain):
+            np.testing.assert_array_equal(ctx.readback(sid), img)
+            got = ctx.dump(sid)
+            for k in oracle.CANONICAL_KEYS + ("masks",):
+                np.testing.assert_array_equal(got[k], d[k], err_msg=k)
+    for sid in ids:
+        ctx.release(sid)
+
+
+def test_undersized_prediction_retries_without_wrong_pixels(ctx):
+    scenes = scenegen.batch_scenes(n_scenes=3, seed=321, side_max=512, paths_max=300)
+    ids = [ctx.upload(s) for s in scenes]
+    ref = []
+    for sid in ids:
+        ctx.rasterize([sid], pfh.PFH_PLAIN)
+        ref.append(ctx.readback(sid))
+    for sid in ids:
+        ctx.release(sid)
+    # A fresh context (no buffers grown yet) and a predictor whose outputs are ~0:
+    # every fill / alpha-tile capacity is too small, so the batch must be retried.
+    fresh = pfh.Context(0)
+    ids = [fresh.upload(s) for s in scenes]
+    w = np.zeros(pfh.PREDICTOR_FLOATS, np.float32)
+    w[12:24] = 1.0
+    fresh.set_predictor(w, safety=1.0)
+    fresh.rasterize(ids, pfh.PFH_BATCHED | pfh.PFH_PREDICT)
+    assert fresh.stats()["retries"] > 0
+    for sid, img in zip(ids, ref):
+        np.testing.assert_array_equal(fresh.readback(sid), img)
+    fresh.close()
