@@ -76,6 +76,7 @@ typedef struct pfh_stats {
     double ms_total;          /* device time of the last pfh_rasterize (HIP events) */
     double ms_stage[8];       /* plain mode: count, scan, emit, propagate, entries, sort, fill, composite */
     int64_t capacity_fills, capacity_alpha, capacity_entries;
+    int64_t alpha_entries;    /* draw entries that sample a coverage mask */
 } pfh_stats;
 
 /* Context on HIP device `device`.  Replaces Renderer's Device/Queue pair (core/renderer.cpp:13-37). */

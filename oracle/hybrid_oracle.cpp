@@ -322,9 +322,9 @@ struct Builder {
             return;
         }
         bool flat = s.kind == K_QUAD ? flat_quad(s, tol) : flat_cubic(s, tol);
-        // Depth guard: a finite curve inside |coordinates| < 2^48 is flat long before
-        // depth 24 (the flatness measure shrinks 16x per halving).
-        if (flat || depth >= 24) {
+        // Depth guard: a finite curve with coordinates below ~2^30 px is flat long before
+        // depth 16 (the flatness measure shrinks 16x per halving).
+        if (flat || depth >= 16) {
             if (!flat) pt->guard_hits++;
             process_line(s.p0, s.p1);
             return;

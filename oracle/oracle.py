@@ -72,6 +72,8 @@ def load_ref():
         lib.pfref_counts.argtypes = [_P, _P]
         lib.pfref_fetch.argtypes = [_P] * 14
         lib.pfref_free.argtypes = [_P]
+        lib.pfref_time.restype = ctypes.c_double
+        lib.pfref_time.argtypes = [ctypes.c_int] + [_P] * 7 + [ctypes.c_int, ctypes.c_int]
         _ref = lib
     return _ref
 
@@ -132,6 +134,15 @@ def run_ref(scene, threads: int = 1) -> dict:
     lib.pfref_free(h)
     out["tiling_seconds"] = float(el[0])
     return out
+
+
+def time_ref(scene, threads: int = 4, repeats: int = 1) -> float:
+    """Wall seconds for the reference Tiler to tile every path of ``scene`` ``repeats`` times."""
+    lib = load_ref()
+    return lib.pfref_time(ctypes.c_int(scene.n_paths), _ptr(scene.path_contour_start), _ptr(scene.path_fill_rule),
+                          _ptr(scene.contour_start), _ptr(scene.contour_closed), _ptr(scene.points),
+                          _ptr(scene.point_flags), _ptr(scene.view_box), ctypes.c_int(threads),
+                          ctypes.c_int(repeats))
 
 
 def alpha_tiles(canon: dict, scene) -> dict:

@@ -185,6 +185,43 @@ void *pfref_run(int n_paths, const int32_t *path_contour_start, const uint8_t *p
     return res;
 }
 
+// Wall time of tiling every draw path `repeats` times with the reference Tiler
+// (no canonical output): the CPU baseline.
+double pfref_time(int n_paths, const int32_t *path_contour_start, const uint8_t *path_fill_rule,
+                  const int32_t *contour_start, const uint8_t *contour_closed, const float *points,
+                  const uint8_t *point_flags, const float *view_box, int threads, int repeats) {
+    Scene scene(0, RectF(view_box[0], view_box[1], view_box[2], view_box[3]));
+    HarnessBuilder builder;
+    builder.attach(&scene);
+    RectF vb = scene.get_view_box();
+    std::vector<Outline> outlines(n_paths);
+    for (int p = 0; p < n_paths; p++)
+        outlines[p] = make_outline(points, point_flags, contour_start, contour_closed, path_contour_start[p],
+                                   path_contour_start[p + 1]);
+    std::vector<BuiltPath> built(n_paths);
+    std::vector<size_t> nfills(n_paths);
+    auto work = [&](int begin, int stride) {
+        for (int p = begin; p < n_paths; p += stride) {
+            TilingPathInfo info{};
+            info.type = TilingPathInfo::Type::Draw;
+            info.info.blend_mode = BlendMode::SrcOver;
+            info.info.fill_rule = path_fill_rule[p] ? FillRule::EvenOdd : FillRule::Winding;
+            Tiler tiler(builder, (uint32_t)p, outlines[p], info.info.fill_rule, vb, nullptr, {}, info);
+            tiler.generate_tiles();
+            built[p] = tiler.object_builder.built_path;
+            nfills[p] = tiler.object_builder.fills.size();
+        }
+    };
+    auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < repeats; r++) {
+        for (auto &n : builder.next_alpha_tile_indices) n = 0;
+        std::vector<std::thread> pool;
+        for (int i = 0; i < threads; i++) pool.emplace_back(work, i, threads);
+        for (auto &t : pool) t.join();
+    }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // counts[0..5] = tiles, columns, fills, entries, fb tiles, paths
 void pfref_counts(void *h, int64_t *counts) {
     auto *r = (RefResult *)h;

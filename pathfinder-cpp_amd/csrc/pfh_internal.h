@@ -8,7 +8,10 @@
 namespace pfh {
 
 constexpr int TILE = 16;                 // TILE_WIDTH / TILE_HEIGHT, core/data/data.h:15-16
-constexpr int MAX_FLATTEN_DEPTH = 24;    // recursion cap of process_segment (guard; see flatten)
+// Recursion cap of process_segment: the flatness measure shrinks 16x per halving, so a
+// curve with coordinates below ~2^30 px is flat long before this depth; deeper curves
+// are emitted as chords and counted in C_GUARD (the oracle applies the same cap).
+constexpr int MAX_FLATTEN_DEPTH = 16;
 constexpr int FIX_SHIFT = 20;            // fixed-point coverage accumulation: 2^-20 px^2 units
 constexpr uint32_t NO_ALPHA = 0xffffffffu;
 
@@ -21,6 +24,7 @@ enum Counter {
     C_ENTRIES = 2,     // total draw entries after the z-cull
     C_OVERFLOW = 3,    // bit 0 fills, bit 1 alpha tiles, bit 2 entries exceeded capacity
     C_GUARD = 4,       // loop guards tripped
+    C_ALPHA_ENTRIES = 5,  // draw entries that sample a mask
     C_COUNT = 8
 };
 

@@ -99,14 +99,9 @@ struct Walker {
         if (!EMIT) {
             atomicAdd(&b->tile_cnt[t], 1u);
         } else {
-            uint32_t slot = atomicSub(&b->tile_cnt[t], 1u) - 1u;
-            uint32_t pos = b->tile_fill_off[t] + slot;
-            uint2 packed = make_uint2(q0 | (q1 << 16), q2 | (q3 << 16));
-            if (pos < b->cap_fills) b->fills[pos] = packed;
-            if (b->dump) {
-                uint32_t d = dump_base + count;
-                if (d < b->cap_fills) b->dump_fills[d] = make_uint4(packed.x, packed.y, t, 0u);
-            }
+            // emission order; k_group_fills moves each fill into its tile's range
+            uint32_t d = dump_base + count;
+            if (d < b->cap_fills) b->dump_fills[d] = make_uint4(q0 | (q1 << 16), q2 | (q3 << 16), t, 0u);
         }
         count++;
     }
@@ -216,30 +211,37 @@ struct Walker {
 
     // core/d3d9/tiler.cpp:284-315 process_segment, depth first, first half first;
     // core/data/segment.cpp:42-126 split_cubic / split_quadratic at t = 0.5.
+    // The explicit stack holds pending right halves without their start point: a
+    // pending half starts exactly where the previously flattened piece ended (splits
+    // keep their endpoints bit-for-bit), so only (c0, c1, p1, depth) are stored.
     __device__ void segment(const Seg4 &root, uint32_t kind) {
         if (kind == SEG_LINE) {
             line(root.p0, root.p1);
             return;
         }
-        Seg4 stack[MAX_FLATTEN_DEPTH + 2];
-        int depth[MAX_FLATTEN_DEPTH + 2];
+        float stk[MAX_FLATTEN_DEPTH + 1][6];
+        int sdepth[MAX_FLATTEN_DEPTH + 1];
         int sp = 0;
-        stack[0] = root;
-        depth[0] = 0;
-        sp = 1;
         const bool quad = kind == SEG_QUAD;
-        while (sp > 0) {
-            sp--;
-            Seg4 s = stack[sp];
-            int d = depth[sp];
+        Seg4 s = root;
+        int d = 0;
+        while (true) {
             bool flat = quad ? flat_quad(s) : flat_cubic(s);
             if (flat || d >= MAX_FLATTEN_DEPTH) {
                 if (!flat) guard++;
                 line(s.p0, s.p1);
+                if (sp == 0) break;
+                sp--;
+                const float2 start = s.p1;
+                s.p0 = start;
+                s.c0 = make_float2(stk[sp][0], stk[sp][1]);
+                s.c1 = make_float2(stk[sp][2], stk[sp][3]);
+                s.p1 = make_float2(stk[sp][4], stk[sp][5]);
+                d = sdepth[sp];
                 continue;
             }
-            Seg4 a, bseg;
             const float t = 0.5f;
+            Seg4 a, bseg;
             if (quad) {
                 float2 qa = lerp2(s.p0, s.c0, t), qb = lerp2(s.c0, s.p1, t), qc = lerp2(qa, qb, t);
                 a = Seg4{s.p0, qa, qa, qc};
@@ -251,11 +253,16 @@ struct Walker {
                 a = Seg4{s.p0, p01, p012, p0123};
                 bseg = Seg4{p0123, p123, p23, s.p1};
             }
-            stack[sp] = bseg;
-            depth[sp] = d + 1;
-            stack[sp + 1] = a;
-            depth[sp + 1] = d + 1;
-            sp += 2;
+            stk[sp][0] = bseg.c0.x;
+            stk[sp][1] = bseg.c0.y;
+            stk[sp][2] = bseg.c1.x;
+            stk[sp][3] = bseg.c1.y;
+            stk[sp][4] = bseg.p1.x;
+            stk[sp][5] = bseg.p1.y;
+            sdepth[sp] = d + 1;
+            sp++;
+            s = a;
+            d = d + 1;
         }
     }
 };
@@ -285,7 +292,7 @@ __global__ __launch_bounds__(256) void k_tile_segments(BatchDev b) {
     w.b = &b;
     w.count = 0;
     w.guard = 0;
-    w.dump_base = EMIT && b.dump ? b.seg_cnt[g] : 0u;
+    w.dump_base = EMIT ? b.seg_cnt[g] : 0u;
     // With no columns nothing can land (add_fill culls, adjust_alpha_tile_backdrop ignores
     // x outside [0, width)); with columns but no rows, above-rect backdrops still accumulate.
     if (w.tw <= 0) {
@@ -315,6 +322,18 @@ __global__ __launch_bounds__(256) void k_tile_segments(BatchDev b) {
     if (!EMIT) {
         b.seg_cnt[g] = w.count;
         if (w.guard) atomicAdd(&b.counters[C_GUARD], w.guard);
+    }
+}
+
+// Fills in emission order -> grouped by tile (tile_fill_off ranges).  Within a tile
+// the order is arbitrary: coverage is accumulated in fixed point.
+__global__ __launch_bounds__(256) void k_group_fills(BatchDev b) {
+    uint32_t n = min(b.counters[C_FILLS], b.cap_fills);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint4 e = b.dump_fills[i];
+        uint32_t slot = atomicSub(&b.tile_cnt[e.z], 1u) - 1u;
+        uint32_t pos = b.tile_fill_off[e.z] + slot;
+        if (pos < b.cap_fills) b.fills[pos] = make_uint2(e.x, e.y);
     }
 }
 
@@ -497,43 +516,50 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_entries(BatchDev b) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= b.n_cols) return;
-    int k = find_scene(b.col_bases, b.n_scenes, c);
-    const SceneDev &sc = b.scenes[k];
-    uint32_t cl = c - sc.col_base;
-    uint32_t p = sc.col_path[cl];
-    int4 r = sc.path_rect[p];
-    int w = r.z - r.x, h = r.w - r.y;
-    int x = (int)(cl - sc.path_col_off[p]);
-    uint32_t tbase = sc.tile_base + sc.path_tile_off[p];
-    uint32_t rgba = sc.path_rgba[p];
-    uint32_t rule = sc.path_rule[p];
-    for (int row = 0; row < h; row++) {
-        uint32_t t = tbase + (uint32_t)(row * w + x);
-        int ty = r.y + row, tx = r.x + x;
-        uint32_t fbl = (uint32_t)((ty - sc.fb_y0) * sc.fb_w + (tx - sc.fb_x0));
-        uint32_t fb = sc.fb_base + fbl;
-        if (!WRITE) {
-            bool alpha = b.tile_fill_off[t + 1] > b.tile_fill_off[t];
-            int8_t bd = b.tile_bd[t];
-            bool keep = (alpha || bd != 0) && (int32_t)p >= (int32_t)b.zbuf[fb];
-            b.tile_entry[t] = keep ? 1u : 0u;
-        } else {
-            uint32_t e = b.tile_entry[t];
-            if (b.tile_entry[t + 1] == e) continue;
-            if (e >= b.cap_entries) continue;
-            bool alpha = b.tile_fill_off[t + 1] > b.tile_fill_off[t];
-            uint32_t ai = alpha ? b.tile_alpha_idx[t] : NO_ALPHA;
-            uint32_t eff = rgba;
-            if (!alpha) {
+    uint32_t n_alpha_entries = 0;
+    if (c < b.n_cols) {
+        int k = find_scene(b.col_bases, b.n_scenes, c);
+        const SceneDev &sc = b.scenes[k];
+        uint32_t cl = c - sc.col_base;
+        uint32_t p = sc.col_path[cl];
+        int4 r = sc.path_rect[p];
+        int w = r.z - r.x, h = r.w - r.y;
+        int x = (int)(cl - sc.path_col_off[p]);
+        uint32_t tbase = sc.tile_base + sc.path_tile_off[p];
+        uint32_t rgba = sc.path_rgba[p];
+        uint32_t rule = sc.path_rule[p];
+        for (int row = 0; row < h; row++) {
+            uint32_t t = tbase + (uint32_t)(row * w + x);
+            int ty = r.y + row, tx = r.x + x;
+            uint32_t fbl = (uint32_t)((ty - sc.fb_y0) * sc.fb_w + (tx - sc.fb_x0));
+            uint32_t fb = sc.fb_base + fbl;
+            if (!WRITE) {
+                bool alpha = b.tile_fill_off[t + 1] > b.tile_fill_off[t];
                 int8_t bd = b.tile_bd[t];
-                if (rule && (bd & 1) == 0) eff &= 0x00ffffffu;  // even-odd, even winding: covers nothing
+                bool keep = (alpha || bd != 0) && (int32_t)p >= (int32_t)b.zbuf[fb];
+                b.tile_entry[t] = keep ? 1u : 0u;
+            } else {
+                uint32_t e = b.tile_entry[t];
+                if (b.tile_entry[t + 1] == e) continue;
+                if (e >= b.cap_entries) continue;
+                bool alpha = b.tile_fill_off[t + 1] > b.tile_fill_off[t];
+                uint32_t ai = alpha ? b.tile_alpha_idx[t] : NO_ALPHA;
+                uint32_t eff = rgba;
+                if (!alpha) {
+                    int8_t bd = b.tile_bd[t];
+                    if (rule && (bd & 1) == 0) eff &= 0x00ffffffu;  // even-odd, even winding: covers nothing
+                }
+                b.keys[e] = fb;
+                b.vals[e] = e;
+                b.entry_rec[e] = make_uint4(ai, eff, p, fbl);
+                atomicAdd(&b.fb_cnt[fb], 1u);
+                n_alpha_entries += alpha ? 1u : 0u;
             }
-            b.keys[e] = fb;
-            b.vals[e] = e;
-            b.entry_rec[e] = make_uint4(ai, eff, p, fbl);
-            atomicAdd(&b.fb_cnt[fb], 1u);
         }
+    }
+    if (WRITE) {
+        for (int d = 32; d > 0; d >>= 1) n_alpha_entries += __shfl_xor(n_alpha_entries, d, 64);
+        if ((threadIdx.x & 63) == 0 && n_alpha_entries) atomicAdd(&b.counters[C_ALPHA_ENTRIES], n_alpha_entries);
     }
 }
 
@@ -606,161 +632,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *keys,
 }
 
 // ---------------------------------------------------------------------------
-// Coverage masks: one wave per alpha tile, 4 pixels (one row) per lane
-// ---------------------------------------------------------------------------
-
-// Signed area of a fill over pixel (px, py): sign(from.x - to.x) * ∫ clamp(py + 1 - y(x), 0, 1) dx
-// over [px, px + 1] ∩ [xmin, xmax] — the box-filtered quantity the reference's area LUT
-// tabulates (shaders/d3d9/fill.frag:25-45, shaders/d3d11/fill.comp:51-71), evaluated analytically.
-__device__ __forceinline__ float fill_area(float xl, float yl, float xr, float slope, float sign, float px,
-                                           float py1) {
-    float lo = fmaxf(px, xl), hi = fminf(px + 1.0f, xr);
-    if (!(hi > lo)) return 0.0f;
-    float ua = py1 - (yl + (lo - xl) * slope);
-    float ub = py1 - (yl + (hi - xl) * slope);
-    float a = fminf(ua, ub), bb = fmaxf(ua, ub);
-    float avg;
-    float du = bb - a;
-    if (du <= 0.0f) {
-        avg = fminf(fmaxf(a, 0.0f), 1.0f);
-    } else {
-        float full = bb > 1.0f ? bb - fmaxf(a, 1.0f) : 0.0f;
-        float l2 = fmaxf(a, 0.0f), h2 = fminf(bb, 1.0f);
-        float mid = h2 > l2 ? (h2 - l2) * (h2 + l2) * 0.5f : 0.0f;
-        avg = (full + mid) / du;
-    }
-    return sign * (hi - lo) * avg;
-}
-
-__device__ __forceinline__ float apply_rule(float c, uint32_t even_odd) {
-    if (!even_odd) return fminf(fabsf(c), 1.0f);
-    float m = c - 2.0f * floorf(c * 0.5f);  // GLSL mod(c, 2)
-    return fminf(fmaxf(1.0f - fabsf(1.0f - m), 0.0f), 1.0f);
-}
-
-__global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
-    const int lane = threadIdx.x & 63;
-    uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6);
-    uint32_t na = min(b.counters[C_ALPHA], b.cap_alpha);
-    if (a >= na) return;
-    uint4 rec = b.alpha_rec[a];
-    const float py1 = (float)(lane >> 2) + 1.0f;
-    const float px0 = (float)((lane & 3) * 4);
-    int32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
-    const float scale = (float)(1 << FIX_SHIFT);
-    uint32_t f1 = min(rec.y, b.cap_fills);
-    for (uint32_t f = rec.x; f < f1; f++) {
-        uint2 fl = b.fills[f];
-        float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), y0 = (float)(fl.x >> 16) * (1.0f / 256.0f);
-        float x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f), y1 = (float)(fl.y >> 16) * (1.0f / 256.0f);
-        float sign = x0 > x1 ? 1.0f : -1.0f;
-        float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
-        float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
-        float slope = (yr - yl) / (xr - xl);
-        acc0 += (int32_t)rintf(fill_area(xl, yl, xr, slope, sign, px0 + 0.0f, py1) * scale);
-        acc1 += (int32_t)rintf(fill_area(xl, yl, xr, slope, sign, px0 + 1.0f, py1) * scale);
-        acc2 += (int32_t)rintf(fill_area(xl, yl, xr, slope, sign, px0 + 2.0f, py1) * scale);
-        acc3 += (int32_t)rintf(fill_area(xl, yl, xr, slope, sign, px0 + 3.0f, py1) * scale);
-    }
-    const float bd = (float)(int8_t)(rec.z & 0xffu);
-    const uint32_t rule = (rec.z >> 8) & 1u;
-    const float inv = 1.0f / scale;
-    uint32_t m0 = (uint32_t)rintf(apply_rule(bd + (float)acc0 * inv, rule) * 255.0f);
-    uint32_t m1 = (uint32_t)rintf(apply_rule(bd + (float)acc1 * inv, rule) * 255.0f);
-    uint32_t m2 = (uint32_t)rintf(apply_rule(bd + (float)acc2 * inv, rule) * 255.0f);
-    uint32_t m3 = (uint32_t)rintf(apply_rule(bd + (float)acc3 * inv, rule) * 255.0f);
-    reinterpret_cast<uint32_t *>(b.masks + (uint64_t)a * 256)[lane] = m0 | (m1 << 8) | (m2 << 16) | (m3 << 24);
-}
-
-// ---------------------------------------------------------------------------
-// Composite: one wave per framebuffer tile, 4 pixels (one row) per lane
-// ---------------------------------------------------------------------------
-
-__global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *sorted_vals) {
-    const int lane = threadIdx.x & 63;
-    uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (f >= b.n_fb) return;
-    int k = find_scene(b.fb_bases, b.n_scenes, f);
-    const SceneDev &sc = b.scenes[k];
-    uint32_t fl = f - sc.fb_base;
-    int tx = sc.fb_x0 + (int)(fl % (uint32_t)sc.fb_w), ty = sc.fb_y0 + (int)(fl / (uint32_t)sc.fb_w);
-    const int row = lane >> 2, col = (lane & 3) * 4;
-    const int y = ty * TILE + row, x0 = tx * TILE + col;
-    float r[4], g[4], bl[4], al[4];
-    if (sc.load_dst && y < sc.height) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            uint32_t px = 0;
-            if (x0 + j < sc.width) px = *reinterpret_cast<const uint32_t *>(sc.out + (int64_t)y * sc.out_pitch + (x0 + j) * 4);
-            r[j] = (float)(px & 255u) * (1.0f / 255.0f);
-            g[j] = (float)((px >> 8) & 255u) * (1.0f / 255.0f);
-            bl[j] = (float)((px >> 16) & 255u) * (1.0f / 255.0f);
-            al[j] = (float)(px >> 24) * (1.0f / 255.0f);
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < 4; j++) r[j] = g[j] = bl[j] = al[j] = 0.0f;
-    }
-    uint32_t e0 = b.fb_cnt[f], e1 = b.fb_cnt[f + 1];
-    uint32_t ecap = b.cap_entries;
-    if (e1 > ecap) e1 = ecap;
-    for (uint32_t base = e0; base < e1; base += 64) {
-        uint32_t i = base + lane;
-        uint32_t my_alpha = NO_ALPHA, my_rgba = 0;
-        if (i < e1) {
-            uint4 rec = b.entry_rec[sorted_vals[i]];
-            my_alpha = rec.x;
-            my_rgba = rec.y;
-        }
-        uint32_t cnt = min(64u, e1 - base);
-        for (uint32_t j = 0; j < cnt; j++) {
-            uint32_t ai = __builtin_amdgcn_readlane(my_alpha, j);
-            if (ai != NO_ALPHA && ai >= b.cap_alpha) continue;  // capacity overflow: the run is retried
-            uint32_t c = __builtin_amdgcn_readlane(my_rgba, j);
-            float ca = (float)(c >> 24) * (1.0f / 255.0f);
-            if (ca == 0.0f) continue;
-            float cr = (float)(c & 255u) * (1.0f / 255.0f);
-            float cg = (float)((c >> 8) & 255u) * (1.0f / 255.0f);
-            float cb = (float)((c >> 16) & 255u) * (1.0f / 255.0f);
-            if (ai == NO_ALPHA) {
-                float k1 = 1.0f - ca;
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    r[q] = cr * ca + r[q] * k1;
-                    g[q] = cg * ca + g[q] * k1;
-                    bl[q] = cb * ca + bl[q] * k1;
-                    al[q] = ca + al[q] * k1;
-                }
-            } else {
-                uint32_t m = reinterpret_cast<const uint32_t *>(b.masks + (uint64_t)ai * 256)[lane];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    float cov = (float)((m >> (8 * q)) & 255u) * (1.0f / 255.0f);
-                    float sa = ca * cov;
-                    float k1 = 1.0f - sa;
-                    r[q] = cr * sa + r[q] * k1;
-                    g[q] = cg * sa + g[q] * k1;
-                    bl[q] = cb * sa + bl[q] * k1;
-                    al[q] = sa + al[q] * k1;
-                }
-            }
-        }
-    }
-    if (y >= sc.height) return;
-    uint8_t *orow = sc.out + (int64_t)y * sc.out_pitch;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        int x = x0 + q;
-        if (x >= sc.width) break;
-        uint32_t pr = (uint32_t)rintf(fminf(fmaxf(r[q], 0.0f), 1.0f) * 255.0f);
-        uint32_t pg = (uint32_t)rintf(fminf(fmaxf(g[q], 0.0f), 1.0f) * 255.0f);
-        uint32_t pb = (uint32_t)rintf(fminf(fmaxf(bl[q], 0.0f), 1.0f) * 255.0f);
-        uint32_t pa = (uint32_t)rintf(fminf(fmaxf(al[q], 0.0f), 1.0f) * 255.0f);
-        reinterpret_cast<uint32_t *>(orow)[x] = pr | (pg << 8) | (pb << 16) | (pa << 24);
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Launch helpers (host side)
 // ---------------------------------------------------------------------------
 
@@ -806,6 +677,13 @@ hipError_t launch_tile_segments(const BatchDev &b, bool emit, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_group_fills(const BatchDev &b, uint32_t grid_fills, hipStream_t s) {
+    if (grid_fills == 0) return hipSuccess;
+    uint32_t nb = std::min<uint32_t>(blocks_for(grid_fills, 256), 16384);
+    hipLaunchKernelGGL(k_group_fills, dim3(nb), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_propagate(const BatchDev &b, hipStream_t s) {
     if (b.n_cols == 0) return hipSuccess;
     hipLaunchKernelGGL(k_propagate, dim3(blocks_for(b.n_cols, 256)), dim3(256), 0, s, b);
@@ -843,18 +721,6 @@ hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32
         alt = !alt;
     }
     *result_alt = alt;
-    return hipGetLastError();
-}
-
-hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, hipStream_t s) {
-    if (grid_alpha == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_masks, dim3(blocks_for(grid_alpha, 4)), dim3(256), 0, s, b);
-    return hipGetLastError();
-}
-
-hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, hipStream_t s) {
-    if (b.n_fb == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_composite, dim3(blocks_for(b.n_fb, 4)), dim3(256), 0, s, b, sorted_vals);
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, u
 hipError_t scan_tiles(const BatchDev &b, void *tmp, hipStream_t s);
 hipError_t launch_tile_segments(const BatchDev &b, bool emit, hipStream_t s);
 hipError_t launch_propagate(const BatchDev &b, hipStream_t s);
+hipError_t launch_group_fills(const BatchDev &b, uint32_t grid_fills, hipStream_t s);
 hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s);
 size_t radix_temp_elems(uint64_t cap);
 hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, const uint32_t *n_ptr,
@@ -451,7 +452,7 @@ static void ensure_sizes(pfh_ctx *ctx, const Plan &pl, bool dump) {
     uint64_t scan_n = std::max<uint64_t>({pl.T + 1, pl.S + 1, pl.FB + 1, 1});
     ctx->scan_tmp.ensure(16 * scan_temp_elems(scan_n) + 64);
     ctx->fills.ensure(8ull * pl.cap_f);
-    if (dump) ctx->dump_fills.ensure(16ull * pl.cap_f);
+    ctx->dump_fills.ensure(16ull * pl.cap_f);
     ctx->alpha_rec.ensure(16ull * pl.cap_a);
     ctx->masks.ensure(256ull * pl.cap_a);
     ctx->keys.ensure(4ull * pl.cap_e);
@@ -573,7 +574,7 @@ static BatchDev batch_dev(pfh_ctx *ctx, const Plan &pl, bool dump) {
     b.col_bd = ctx->col_bd.as<int32_t>();
     b.seg_cnt = ctx->seg_cnt.as<uint32_t>();
     b.fills = ctx->fills.as<uint2>();
-    b.dump_fills = dump ? ctx->dump_fills.as<uint4>() : nullptr;
+    b.dump_fills = ctx->dump_fills.as<uint4>();
     b.alpha_rec = ctx->alpha_rec.as<uint4>();
     b.masks = ctx->masks.as<uint8_t>();
     b.zbuf = ctx->zbuf.as<uint32_t>();
@@ -605,8 +606,9 @@ static void stage_count(pfh_ctx *ctx, const BatchDev &b) {
 }
 
 static void stage_emit(pfh_ctx *ctx, const BatchDev &b) {
-    if (b.dump) HIP_TRY(scan_u32(b.seg_cnt, b.seg_cnt, b.n_segs, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
+    HIP_TRY(scan_u32(b.seg_cnt, b.seg_cnt, b.n_segs, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
     HIP_TRY(launch_tile_segments(b, true, ctx->stream));
+    HIP_TRY(launch_group_fills(b, b.cap_fills, ctx->stream));
     HIP_TRY(launch_propagate(b, ctx->stream));
     HIP_TRY(launch_entries(b, false, ctx->stream));
     HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
@@ -639,6 +641,7 @@ static void record_counts(pfh_ctx *ctx, const Plan &pl) {
     ctx->stats.alpha_tiles = ctx->h_counters[C_ALPHA];
     ctx->stats.entries = ctx->h_counters[C_ENTRIES];
     ctx->stats.guard_hits = ctx->h_counters[C_GUARD];
+    ctx->stats.alpha_entries = ctx->h_counters[C_ALPHA_ENTRIES];
     ctx->stats.tiles = (int64_t)pl.T;
     ctx->stats.columns = (int64_t)pl.W;
     ctx->stats.segments = (int64_t)pl.S;
@@ -677,8 +680,9 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
         ensure_sizes(ctx, pl, dump);
         b = batch_dev(ctx, pl, dump);
     }
-    if (dump) HIP_TRY(scan_u32(b.seg_cnt, b.seg_cnt, b.n_segs, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
+    HIP_TRY(scan_u32(b.seg_cnt, b.seg_cnt, b.n_segs, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
     HIP_TRY(launch_tile_segments(b, true, ctx->stream));
+    HIP_TRY(launch_group_fills(b, nf, ctx->stream));
     HIP_TRY(hipEventRecord(ev[3], ctx->stream));
     HIP_TRY(launch_propagate(b, ctx->stream));
     HIP_TRY(hipEventRecord(ev[4], ctx->stream));

@@ -41,7 +41,7 @@ class Stats(ctypes.Structure):
                 ("entries", ctypes.c_int64), ("fb_tiles", ctypes.c_int64), ("retries", ctypes.c_int64),
                 ("guard_hits", ctypes.c_int64), ("ms_total", ctypes.c_double), ("ms_stage", ctypes.c_double * 8),
                 ("capacity_fills", ctypes.c_int64), ("capacity_alpha", ctypes.c_int64),
-                ("capacity_entries", ctypes.c_int64)]
+                ("capacity_entries", ctypes.c_int64), ("alpha_entries", ctypes.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "ms_stage"}
