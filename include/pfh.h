@@ -74,9 +74,10 @@ typedef struct pfh_stats {
     int64_t retries;          /* capacity overflows that forced a re-run */
     int64_t guard_hits;       /* loop guards tripped (inputs on which the reference would not terminate) */
     double ms_total;          /* device time of the last pfh_rasterize (HIP events) */
-    double ms_stage[8];       /* plain mode: count, scan, emit, propagate, entries, sort, fill, composite */
+    double ms_stage[8];       /* plain mode: flatten, bin count, bin emit, propagate, entries, sort, fill, composite */
     int64_t capacity_fills, capacity_alpha, capacity_entries;
     int64_t alpha_entries;    /* draw entries that sample a coverage mask */
+    int64_t lines, capacity_lines; /* flattened chords and the chord buffer's capacity */
 } pfh_stats;
 
 /* Context on HIP device `device`.  Replaces Renderer's Device/Queue pair (core/renderer.cpp:13-37). */
@@ -121,10 +122,20 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
 
 int pfh_stats_get(pfh_ctx *ctx, pfh_stats *out);
 
-/* Sizing predictor weights (12 -> 64 -> 64 -> 2 MLP, FP32); see pfh_mlp_* below. */
+/*
+ * Sizing predictor: per path, an FP32 MLP 12 -> 64 -> 64 -> 3 predicting log1p of
+ * (flattened chords, fills, alpha tiles).  weights = mean[12], inv_std[12] (feature
+ * normalisation) followed by the 5187 MLP parameters (layout in pfh_mlp_*).  safety >= 1
+ * scales the predicted capacities.  weights == NULL removes the predictor.
+ */
 int pfh_set_predictor(pfh_ctx *ctx, const float *weights, int64_t n_floats, float safety);
 
-/* Device kernels of the sizing MLP, exposed for training and gradient checks (device pointers). */
+/*
+ * Device kernels of the sizing MLP, exposed for training and gradient checks (device pointers).
+ * Parameters (5187 floats): W1[12][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3];
+ * d_hidden holds n x 128 activations; d_out / d_target n x 3; d_loss one float (mean squared error).
+ * pfh_mlp_sgd applies one Adam step.
+ */
 int pfh_mlp_forward(pfh_ctx *ctx, const float *d_params, const float *d_x, int64_t n, float *d_hidden,
                     float *d_out);
 int pfh_mlp_backward(pfh_ctx *ctx, const float *d_params, const float *d_x, int64_t n, const float *d_hidden,

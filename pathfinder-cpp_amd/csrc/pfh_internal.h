@@ -25,6 +25,7 @@ enum Counter {
     C_OVERFLOW = 3,    // bit 0 fills, bit 1 alpha tiles, bit 2 entries exceeded capacity
     C_GUARD = 4,       // loop guards tripped
     C_ALPHA_ENTRIES = 5,  // draw entries that sample a mask
+    C_LINES = 6,       // flattened chords (lines) in the batch
     C_COUNT = 8
 };
 
@@ -61,7 +62,7 @@ struct BatchDev {
     const uint32_t *col_bases;
     const uint32_t *fb_bases;
     uint32_t n_segs, n_tiles, n_cols, n_fb;
-    uint32_t cap_fills, cap_alpha, cap_entries;
+    uint32_t cap_fills, cap_alpha, cap_entries, cap_lines;
     int32_t *tile_delta;          // [T] backdrop delta (int8 arithmetic, stored widened)
     uint32_t *tile_cnt;           // [T] fills per tile, then emit cursor
     uint32_t *tile_fill_off;      // [T + 1] exclusive scan of fill counts
@@ -69,7 +70,10 @@ struct BatchDev {
     int8_t *tile_bd;              // [T] propagated backdrop
     uint32_t *tile_entry;         // [T + 1] entry flag, then exclusive scan
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
-    uint32_t *seg_cnt;            // [S + 1] fills per segment, scanned (dump order)
+    uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
+    float4 *lines;                // [L] flattened chords (from, to) in emission order
+    uint32_t *line_seg;           // [L] batch segment index of each chord
+    uint32_t *line_cnt;           // [L + 1] fills per chord, then exclusive scan (emission order)
     uint2 *fills;                 // [F] grouped by tile: (x0 | y0 << 16, x1 | y1 << 16)
     uint4 *dump_fills;            // [F] emission order: (x0|y0<<16, x1|y1<<16, global tile, 0)
     uint4 *alpha_rec;             // [A] (fill begin, fill end, backdrop | rule << 8, tile)

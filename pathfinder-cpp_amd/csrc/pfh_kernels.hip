@@ -1,9 +1,10 @@
 // gfx950 kernels of the rasterization pipeline.
 //
 // Stage map (reference GLSL compute stage -> kernel here):
-//   dice.comp + bin.comp        -> k_tile_segments<COUNT/EMIT>: the hybrid tiler's
-//                                  flatten (core/d3d9/tiler.cpp:284-315) and tile walk
-//                                  (tiler.cpp:131-279) per segment, exact float order
+//   dice.comp                   -> k_flatten<COUNT/EMIT>: the hybrid tiler's recursive
+//                                  flatten (core/d3d9/tiler.cpp:284-315), chords in DFS order
+//   bin.comp                    -> k_bin<COUNT/EMIT>: its tile walk per chord
+//                                  (tiler.cpp:131-279), exact float order, + k_group_fills
 //   bound.comp                  -> tile arrays cleared by k_fill_u32 (pure write stream)
 //   propagate.comp              -> k_propagate (prepare_tiles, tiler.cpp:369-439) + z-buffer
 //   sort.comp                   -> k_entries_* + stable LSD radix sort by framebuffer tile
@@ -70,7 +71,7 @@ __device__ __forceinline__ bool flat_quad(const Seg4 &s) {
 }
 
 // ---------------------------------------------------------------------------
-// Flatten + tile walk per segment (hybrid semantics)
+// Tile walk of one chord (hybrid semantics)
 // ---------------------------------------------------------------------------
 
 template <bool EMIT>
@@ -79,9 +80,15 @@ struct Walker {
     float vb_l, vb_r, vb_b;
     int32_t tl, tt, tr, tb, tw, th;
     uint32_t tbase, cbase;
-    const BatchDev *b;
-    uint32_t count;          // fills emitted by this segment so far
-    uint32_t dump_base;      // first emission slot of this segment (EMIT + dump)
+    uint32_t *tile_cnt;
+    int32_t *tile_delta, *col_bd;
+    uint4 *out;              // EMIT with dumps: fills in emission order
+    uint2 *fills;            // EMIT: fills grouped by tile
+    const uint32_t *tile_fill_off;
+    uint32_t cap_fills;
+    uint32_t pend_tile, pend_n;  // COUNT: pending increments of one tile
+    uint32_t count;          // fills emitted by this chord so far
+    uint32_t out_base;       // first emission slot of this chord (EMIT)
     uint32_t guard;
 
     // core/d3d9/object_builder.cpp:19-64 add_fill
@@ -97,13 +104,31 @@ struct Walker {
         if (q0 == q2) return;
         uint32_t t = tbase + (uint32_t)((ty - tt) * tw + (tx - tl));
         if (!EMIT) {
-            atomicAdd(&b->tile_cnt[t], 1u);
+            // consecutive fills of a chord mostly share a tile: count them locally
+            if (t != pend_tile) {
+                if (pend_n) atomicAdd(&tile_cnt[pend_tile], pend_n);
+                pend_tile = t;
+                pend_n = 0;
+            }
+            pend_n++;
         } else {
-            // emission order; k_group_fills moves each fill into its tile's range
-            uint32_t d = dump_base + count;
-            if (d < b->cap_fills) b->dump_fills[d] = make_uint4(q0 | (q1 << 16), q2 | (q3 << 16), t, 0u);
+            // straight into the tile's range; order inside a tile is free (coverage is
+            // accumulated in fixed point)
+            const uint32_t packed_x = q0 | (q1 << 16), packed_y = q2 | (q3 << 16);
+            uint32_t slot = atomicSub(&tile_cnt[t], 1u) - 1u;
+            uint32_t pos = tile_fill_off[t] + slot;
+            if (pos < cap_fills) fills[pos] = make_uint2(packed_x, packed_y);
+            if (out) {
+                uint32_t d = out_base + count;
+                if (d < cap_fills) out[d] = make_uint4(packed_x, packed_y, t, 0u);
+            }
         }
         count++;
+    }
+
+    __device__ __forceinline__ void flush() {
+        if (!EMIT && pend_n) atomicAdd(&tile_cnt[pend_tile], pend_n);
+        pend_n = 0;
     }
 
     // core/d3d9/object_builder.cpp:96-114 adjust_alpha_tile_backdrop
@@ -112,10 +137,10 @@ struct Walker {
         int ox = tx - tl, oy = ty - tt;
         if (ox < 0 || ox >= tw || oy >= th) return;
         if (oy < 0) {
-            atomicAdd(&b->col_bd[cbase + ox], delta);
+            atomicAdd(&col_bd[cbase + ox], delta);
             return;
         }
-        atomicAdd(&b->tile_delta[tbase + (uint32_t)(oy * tw + ox)], delta);
+        atomicAdd(&tile_delta[tbase + (uint32_t)(oy * tw + ox)], delta);
     }
 
     __device__ __forceinline__ static uint32_t outcode(float2 p, float l, float t, float r, float bb) {
@@ -175,7 +200,8 @@ struct Walker {
         float2 cur = from;
         int cx = fx, cy = fy;
         int last = 0;  // 0 none, 1 X, 2 Y
-        int64_t lim = llabs((int64_t)ex - fx) + llabs((int64_t)ey - fy) + 2;
+        int64_t dx = (int64_t)ex - fx, dy = (int64_t)ey - fy;
+        int64_t lim = (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy) + 2;
         for (int64_t it = 0;; it++) {
             if (it > lim) {
                 guard++;
@@ -208,120 +234,155 @@ struct Walker {
             last = next;
         }
     }
-
-    // core/d3d9/tiler.cpp:284-315 process_segment, depth first, first half first;
-    // core/data/segment.cpp:42-126 split_cubic / split_quadratic at t = 0.5.
-    // The explicit stack holds pending right halves without their start point: a
-    // pending half starts exactly where the previously flattened piece ended (splits
-    // keep their endpoints bit-for-bit), so only (c0, c1, p1, depth) are stored.
-    __device__ void segment(const Seg4 &root, uint32_t kind) {
-        if (kind == SEG_LINE) {
-            line(root.p0, root.p1);
-            return;
-        }
-        float stk[MAX_FLATTEN_DEPTH + 1][6];
-        int sdepth[MAX_FLATTEN_DEPTH + 1];
-        int sp = 0;
-        const bool quad = kind == SEG_QUAD;
-        Seg4 s = root;
-        int d = 0;
-        while (true) {
-            bool flat = quad ? flat_quad(s) : flat_cubic(s);
-            if (flat || d >= MAX_FLATTEN_DEPTH) {
-                if (!flat) guard++;
-                line(s.p0, s.p1);
-                if (sp == 0) break;
-                sp--;
-                const float2 start = s.p1;
-                s.p0 = start;
-                s.c0 = make_float2(stk[sp][0], stk[sp][1]);
-                s.c1 = make_float2(stk[sp][2], stk[sp][3]);
-                s.p1 = make_float2(stk[sp][4], stk[sp][5]);
-                d = sdepth[sp];
-                continue;
-            }
-            const float t = 0.5f;
-            Seg4 a, bseg;
-            if (quad) {
-                float2 qa = lerp2(s.p0, s.c0, t), qb = lerp2(s.c0, s.p1, t), qc = lerp2(qa, qb, t);
-                a = Seg4{s.p0, qa, qa, qc};
-                bseg = Seg4{qc, qb, qb, s.p1};
-            } else {
-                float2 p01 = lerp2(s.p0, s.c0, t), p12 = lerp2(s.c0, s.c1, t), p23 = lerp2(s.c1, s.p1, t);
-                float2 p012 = lerp2(p01, p12, t), p123 = lerp2(p12, p23, t);
-                float2 p0123 = lerp2(p012, p123, t);
-                a = Seg4{s.p0, p01, p012, p0123};
-                bseg = Seg4{p0123, p123, p23, s.p1};
-            }
-            stk[sp][0] = bseg.c0.x;
-            stk[sp][1] = bseg.c0.y;
-            stk[sp][2] = bseg.c1.x;
-            stk[sp][3] = bseg.c1.y;
-            stk[sp][4] = bseg.p1.x;
-            stk[sp][5] = bseg.p1.y;
-            sdepth[sp] = d + 1;
-            sp++;
-            s = a;
-            d = d + 1;
-        }
-    }
 };
 
+// ---------------------------------------------------------------------------
+// Flatten: one thread per segment, chords out in depth-first order
+// ---------------------------------------------------------------------------
+//
+// core/d3d9/tiler.cpp:284-315 process_segment (depth first, first half first) and
+// core/data/segment.cpp:42-126 split_cubic / split_quadratic at t = 0.5.  The explicit
+// stack holds pending right halves without their start point: a pending half starts
+// exactly where the previously flattened piece ended (splits keep their endpoints
+// bit-for-bit), so only (c0, c1, p1, depth) are stored.
 template <bool EMIT>
-__global__ __launch_bounds__(256) void k_tile_segments(BatchDev b) {
+__global__ __launch_bounds__(256) void k_flatten(BatchDev b) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= b.n_segs) return;
     int k = find_scene(b.seg_bases, b.n_scenes, g);
     const SceneDev &sc = b.scenes[k];
-    uint32_t s = g - sc.seg_base;
-    uint2 sg = sc.segs[s];
-    uint32_t path = sc.seg_path[s];
-    int4 rect = sc.path_rect[path];
-    Walker<EMIT> w;
-    w.vb_l = sc.vb_l;
-    w.vb_r = sc.vb_r;
-    w.vb_b = sc.vb_b;
-    w.tl = rect.x;
-    w.tt = rect.y;
-    w.tr = rect.z;
-    w.tb = rect.w;
-    w.tw = rect.z - rect.x;
-    w.th = rect.w - rect.y;
-    w.tbase = sc.tile_base + sc.path_tile_off[path];
-    w.cbase = sc.col_base + sc.path_col_off[path];
-    w.b = &b;
-    w.count = 0;
-    w.guard = 0;
-    w.dump_base = EMIT ? b.seg_cnt[g] : 0u;
-    // With no columns nothing can land (add_fill culls, adjust_alpha_tile_backdrop ignores
-    // x outside [0, width)); with columns but no rows, above-rect backdrops still accumulate.
-    if (w.tw <= 0) {
-        if (!EMIT) b.seg_cnt[g] = 0;
-        return;
+    uint32_t s_local = g - sc.seg_base;
+    uint2 sg = sc.segs[s_local];
+    int4 rect = sc.path_rect[sc.seg_path[s_local]];
+    uint32_t n = 0, guard = 0;
+    uint32_t base = EMIT ? b.seg_cnt[g] : 0u;
+    // With no tile columns nothing can land (add_fill culls, adjust_alpha_tile_backdrop
+    // ignores x outside [0, width)); with columns but no rows, above-rect backdrops still do.
+    if (rect.z - rect.x > 0) {
+        const float2 *P = sc.pts + sg.x;
+        Seg4 s;
+        s.p0 = P[0];
+        if (sg.y == SEG_LINE) {
+            s.p1 = P[1];
+        } else if (sg.y == SEG_QUAD) {
+            s.c0 = P[1];
+            s.c1 = P[1];
+            s.p1 = P[2];
+        } else {
+            s.c0 = P[1];
+            s.c1 = P[2];
+            s.p1 = P[3];
+        }
+        // A non-finite point makes every flattened piece non-finite, and process_line_segment
+        // drops those (tiler.cpp:133-136): the segment contributes nothing.
+        bool finite = isfinite(s.p0.x) && isfinite(s.p0.y) && isfinite(s.p1.x) && isfinite(s.p1.y);
+        if (sg.y != SEG_LINE) finite = finite && isfinite(s.c0.x) && isfinite(s.c0.y);
+        if (sg.y == SEG_CUBIC) finite = finite && isfinite(s.c1.x) && isfinite(s.c1.y);
+        if (finite && sg.y == SEG_LINE) {
+            if (EMIT && base < b.cap_lines) {
+                b.lines[base] = make_float4(s.p0.x, s.p0.y, s.p1.x, s.p1.y);
+                b.line_seg[base] = g;
+            }
+            n = 1;
+        } else if (finite) {
+            float stk[MAX_FLATTEN_DEPTH + 1][6];
+            int sdepth[MAX_FLATTEN_DEPTH + 1];
+            int sp = 0, d = 0;
+            const bool quad = sg.y == SEG_QUAD;
+            while (true) {
+                bool flat = quad ? flat_quad(s) : flat_cubic(s);
+                if (flat || d >= MAX_FLATTEN_DEPTH) {
+                    if (!flat) guard++;
+                    if (EMIT && base + n < b.cap_lines) {
+                        b.lines[base + n] = make_float4(s.p0.x, s.p0.y, s.p1.x, s.p1.y);
+                        b.line_seg[base + n] = g;
+                    }
+                    n++;
+                    if (sp == 0) break;
+                    sp--;
+                    s.p0 = s.p1;
+                    s.c0 = make_float2(stk[sp][0], stk[sp][1]);
+                    s.c1 = make_float2(stk[sp][2], stk[sp][3]);
+                    s.p1 = make_float2(stk[sp][4], stk[sp][5]);
+                    d = sdepth[sp];
+                    continue;
+                }
+                const float t = 0.5f;
+                Seg4 a, bseg;
+                if (quad) {
+                    float2 qa = lerp2(s.p0, s.c0, t), qb = lerp2(s.c0, s.p1, t), qc = lerp2(qa, qb, t);
+                    a = Seg4{s.p0, qa, qa, qc};
+                    bseg = Seg4{qc, qb, qb, s.p1};
+                } else {
+                    float2 p01 = lerp2(s.p0, s.c0, t), p12 = lerp2(s.c0, s.c1, t), p23 = lerp2(s.c1, s.p1, t);
+                    float2 p012 = lerp2(p01, p12, t), p123 = lerp2(p12, p23, t);
+                    float2 p0123 = lerp2(p012, p123, t);
+                    a = Seg4{s.p0, p01, p012, p0123};
+                    bseg = Seg4{p0123, p123, p23, s.p1};
+                }
+                stk[sp][0] = bseg.c0.x;
+                stk[sp][1] = bseg.c0.y;
+                stk[sp][2] = bseg.c1.x;
+                stk[sp][3] = bseg.c1.y;
+                stk[sp][4] = bseg.p1.x;
+                stk[sp][5] = bseg.p1.y;
+                sdepth[sp] = d + 1;
+                sp++;
+                s = a;
+                d++;
+            }
+        }
     }
-    const float2 *P = sc.pts + sg.x;
-    Seg4 root;
-    root.p0 = P[0];
-    if (sg.y == SEG_LINE) {
-        root.p1 = P[1];
-    } else if (sg.y == SEG_QUAD) {
-        root.c0 = P[1];
-        root.c1 = P[1];
-        root.p1 = P[2];
-    } else {
-        root.c0 = P[1];
-        root.c1 = P[2];
-        root.p1 = P[3];
-    }
-    // A non-finite point makes every flattened piece non-finite, and process_line_segment
-    // drops those (tiler.cpp:133-136): skip the segment outright.
-    bool finite = isfinite(root.p0.x) && isfinite(root.p0.y) && isfinite(root.p1.x) && isfinite(root.p1.y);
-    if (sg.y != SEG_LINE) finite = finite && isfinite(root.c0.x) && isfinite(root.c0.y);
-    if (sg.y == SEG_CUBIC) finite = finite && isfinite(root.c1.x) && isfinite(root.c1.y);
-    if (finite) w.segment(root, sg.y);
     if (!EMIT) {
-        b.seg_cnt[g] = w.count;
-        if (w.guard) atomicAdd(&b.counters[C_GUARD], w.guard);
+        b.seg_cnt[g] = n;
+        if (guard) atomicAdd(&b.counters[C_GUARD], guard);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bin: one thread per chord, fills out in tile-walk order
+// ---------------------------------------------------------------------------
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_bin(BatchDev b) {
+    const uint32_t nl = min(b.counters[C_LINES], b.cap_lines);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+        const uint32_t g = b.line_seg[i];
+        const float4 ln = b.lines[i];
+        int k = find_scene(b.seg_bases, b.n_scenes, g);
+        const SceneDev &sc = b.scenes[k];
+        const uint32_t path = sc.seg_path[g - sc.seg_base];
+        const int4 rect = sc.path_rect[path];
+        Walker<EMIT> w;
+        w.vb_l = sc.vb_l;
+        w.vb_r = sc.vb_r;
+        w.vb_b = sc.vb_b;
+        w.tl = rect.x;
+        w.tt = rect.y;
+        w.tr = rect.z;
+        w.tb = rect.w;
+        w.tw = rect.z - rect.x;
+        w.th = rect.w - rect.y;
+        w.tbase = sc.tile_base + sc.path_tile_off[path];
+        w.cbase = sc.col_base + sc.path_col_off[path];
+        w.tile_cnt = b.tile_cnt;
+        w.tile_delta = b.tile_delta;
+        w.col_bd = b.col_bd;
+        w.out = b.dump ? b.dump_fills : nullptr;
+        w.fills = b.fills;
+        w.tile_fill_off = b.tile_fill_off;
+        w.cap_fills = b.cap_fills;
+        w.pend_tile = 0xffffffffu;
+        w.pend_n = 0;
+        w.count = 0;
+        w.guard = 0;
+        w.out_base = (EMIT && b.dump) ? b.line_cnt[i] : 0u;
+        w.line(make_float2(ln.x, ln.y), make_float2(ln.z, ln.w));
+        w.flush();
+        if (!EMIT) {
+            b.line_cnt[i] = w.count;
+            if (w.guard) atomicAdd(&b.counters[C_GUARD], w.guard);
+        }
     }
 }
 
@@ -463,6 +524,20 @@ struct StoreTileOff {
             counters[C_FILLS] = v.x;
             counters[C_ALPHA] = v.y;
         }
+    }
+};
+struct LoadU32Counted {
+    const uint32_t *p;
+    const uint32_t *n_ptr;
+    __device__ uint32_t operator()(uint32_t i) const { return i < *n_ptr ? p[i] : 0u; }
+};
+struct StoreCounted {
+    uint32_t *p, *counters;
+    int counter;
+    uint32_t n;
+    __device__ void operator()(uint32_t i, uint32_t v) const {
+        p[i] = v;
+        if (i == n && counter >= 0) counters[counter] = v;
     }
 };
 struct StoreEntryOff {
@@ -661,6 +736,16 @@ hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp
     return scan_impl<uint32_t>(LoadU32{in}, StoreU32{out}, n, tmp, s);
 }
 
+hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
+                        uint32_t *tmp, hipStream_t s) {
+    return scan_impl<uint32_t>(LoadU32Counted{inout, n_ptr}, StoreCounted{inout, counters, counter, cap}, cap, tmp, s);
+}
+
+// Exclusive scan in place with the total written to counters[counter].
+hipError_t scan_total(uint32_t *inout, uint32_t n, uint32_t *counters, int counter, uint32_t *tmp, hipStream_t s) {
+    return scan_impl<uint32_t>(LoadU32{inout}, StoreCounted{inout, counters, counter, n}, n, tmp, s);
+}
+
 hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, uint32_t *tmp, hipStream_t s) {
     return scan_impl<uint32_t>(LoadU32{flags_inout}, StoreEntryOff{flags_inout, counters, n}, n, tmp, s);
 }
@@ -670,12 +755,25 @@ hipError_t scan_tiles(const BatchDev &b, void *tmp, hipStream_t s) {
                          b.n_tiles, reinterpret_cast<U2 *>(tmp), s);
 }
 
-hipError_t launch_tile_segments(const BatchDev &b, bool emit, hipStream_t s) {
+hipError_t launch_flatten(const BatchDev &b, bool emit, hipStream_t s) {
     if (b.n_segs == 0) return hipSuccess;
-    if (emit) hipLaunchKernelGGL(k_tile_segments<true>, dim3(blocks_for(b.n_segs, 256)), dim3(256), 0, s, b);
-    else hipLaunchKernelGGL(k_tile_segments<false>, dim3(blocks_for(b.n_segs, 256)), dim3(256), 0, s, b);
+    if (emit) hipLaunchKernelGGL(k_flatten<true>, dim3(blocks_for(b.n_segs, 256)), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL(k_flatten<false>, dim3(blocks_for(b.n_segs, 256)), dim3(256), 0, s, b);
     return hipGetLastError();
 }
+
+// grid_lines: chords the launch is sized for (exact in plain mode, capacity in batched mode)
+hipError_t launch_bin(const BatchDev &b, bool emit, uint32_t grid_lines, hipStream_t s) {
+    if (grid_lines == 0) return hipSuccess;
+    uint32_t nb = std::min<uint32_t>(blocks_for(grid_lines, 256), 65535);
+    if (emit) hipLaunchKernelGGL(k_bin<true>, dim3(nb), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL(k_bin<false>, dim3(nb), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+// Exclusive scan of per-chord fill counts in place; total -> counter slot `counter`.
+hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
+                        uint32_t *tmp, hipStream_t s);
 
 hipError_t launch_group_fills(const BatchDev &b, uint32_t grid_fills, hipStream_t s) {
     if (grid_fills == 0) return hipSuccess;

@@ -1,18 +1,19 @@
-// FP32 sizing predictor: MLP 12 -> 64 -> 64 -> 2 (ReLU), trained on (path features ->
-// log1p(fill count), log1p(alpha tile count)) pairs that the count pass emits.  It
+// FP32 sizing predictor: MLP 12 -> 64 -> 64 -> 3 (ReLU), trained on (path features ->
+// log1p(chord count), log1p(fill count), log1p(alpha tile count)) triples that the
+// flatten and bin count passes emit.  It
 // replaces the reference's "count, read back, grow to 2^k, retry" sizing
 // (core/d3d11/renderer.cpp:537-577, 707-724, 834-845) for batched launches; an
 // under-prediction costs a retry, never a wrong pixel.  Tiny matrices: plain FP32
 // VALU kernels, no MFMA.
 //
-// Parameter layout (5122 floats): W1[12][64], b1[64], W2[64][64], b2[64], W3[64][2], b3[2]
+// Parameter layout (5187 floats): W1[12][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3]
 // (row-major, W[in][out]).  Hidden activations per sample: h1[64], h2[64].
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace pfh {
 
-constexpr int NI = 12, NH = 64, NO = 2;
+constexpr int NI = 12, NH = 64, NO = 3;
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NI * NH, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
               OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH * NO, N_PARAMS = OFF_B3 + NO;
 
@@ -36,24 +37,26 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ P
         for (int i = 0; i < NI; i++) a += xi[i] * w[OFF_W1 + i * NH + j];
         h1[j] = fmaxf(a, 0.0f);
     }
-    float o0 = w[OFF_B3], o1 = w[OFF_B3 + 1];
+    float o[NO];
+#pragma unroll
+    for (int q = 0; q < NO; q++) o[q] = w[OFF_B3 + q];
     for (int j = 0; j < NH; j++) {
         float a = w[OFF_B2 + j];
 #pragma unroll 8
         for (int i = 0; i < NH; i++) a += h1[i] * w[OFF_W2 + i * NH + j];
         float h2 = fmaxf(a, 0.0f);
         hidden[s * 2 * NH + NH + j] = h2;
-        o0 += h2 * w[OFF_W3 + j * NO + 0];
-        o1 += h2 * w[OFF_W3 + j * NO + 1];
+#pragma unroll
+        for (int q = 0; q < NO; q++) o[q] += h2 * w[OFF_W3 + j * NO + q];
     }
 #pragma unroll 8
     for (int j = 0; j < NH; j++) hidden[s * 2 * NH + j] = h1[j];
-    out[s * NO + 0] = o0;
-    out[s * NO + 1] = o1;
+#pragma unroll
+    for (int q = 0; q < NO; q++) out[s * NO + q] = o[q];
 }
 
 // Per-sample output and hidden gradients of the mean squared error
-// L = 1/(n*NO) * sum (out - target)^2; writes g[s] = (dout[2], dh2[64], dh1[64]).
+// L = 1/(n*NO) * sum (out - target)^2; writes g[s] = (dout[NO], dh2[64], dh1[64]).
 __global__ __launch_bounds__(256) void k_mlp_backward_act(const float *__restrict__ P, int64_t n,
                                                           const float *__restrict__ hidden,
                                                           const float *__restrict__ out,
@@ -69,17 +72,23 @@ __global__ __launch_bounds__(256) void k_mlp_backward_act(const float *__restric
     float l = 0.0f;
     if (s < n) {
         const float scale = 2.0f / (float)(n * NO);
-        float e0 = out[s * NO] - target[s * NO], e1 = out[s * NO + 1] - target[s * NO + 1];
-        l = e0 * e0 + e1 * e1;
-        float d0 = scale * e0, d1 = scale * e1;
+        float d[NO];
         float *gs = g + s * (NO + 2 * NH);
-        gs[0] = d0;
-        gs[1] = d1;
+#pragma unroll
+        for (int q = 0; q < NO; q++) {
+            float e = out[s * NO + q] - target[s * NO + q];
+            l += e * e;
+            d[q] = scale * e;
+            gs[q] = d[q];
+        }
         const float *h = hidden + s * 2 * NH;
         float dh2[NH];
 #pragma unroll 8
         for (int j = 0; j < NH; j++) {
-            float v = h[NH + j] > 0.0f ? d0 * w3[j * NO] + d1 * w3[j * NO + 1] : 0.0f;
+            float acc = 0.0f;
+#pragma unroll
+            for (int q = 0; q < NO; q++) acc += d[q] * w3[j * NO + q];
+            float v = h[NH + j] > 0.0f ? acc : 0.0f;
             dh2[j] = v;
             gs[NO + j] = v;
         }

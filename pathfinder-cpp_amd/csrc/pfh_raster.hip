@@ -44,53 +44,106 @@ __device__ __forceinline__ float apply_rule(float c, uint32_t even_odd) {
     return fminf(fmaxf(1.0f - fabsf(1.0f - m), 0.0f), 1.0f);
 }
 
+// Adds one fill's contribution to pixel column `col` of a tile's difference array
+// (column stride 1, row stride 16 words); LDS adds, since several lanes may share a column.
+__device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, int32_t *diff) {
+    const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), y0 = (float)(fl.x >> 16) * (1.0f / 256.0f);
+    const float x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f), y1 = (float)(fl.y >> 16) * (1.0f / 256.0f);
+    const float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
+    const float lo = fmaxf((float)col, xl), hi = fminf((float)(col + 1), xr);
+    if (!(hi > lo)) return;
+    const float sign = x0 > x1 ? 1.0f : -1.0f;
+    const float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
+    const float slope = (yr - yl) * __builtin_amdgcn_rcpf(xr - xl);
+    const float ya = yl + (lo - xl) * slope, yb = yl + (hi - xl) * slope;
+    const float w = hi - lo;
+    const int r0 = max(0, min(15, (int)floorf(fminf(ya, yb))));
+    const int r1 = max(0, min(15, (int)floorf(fmaxf(ya, yb))));
+    int32_t prev = 0;
+    for (int r = r0; r <= r1; r++) {
+        const float rb = (float)(r + 1);
+        const int32_t v = (int32_t)rintf(sign * w * avg_clamp01(rb - ya, rb - yb) * scale);
+        atomicAdd(diff + r * 16 + col, v - prev);
+        prev = v;
+    }
+    atomicAdd(diff + (r1 + 1) * 16 + col, (int32_t)rintf(sign * w * scale) - prev);
+}
+
+__device__ __forceinline__ uint32_t wave_max_of_groups(uint32_t v) {
+    // v is uniform within each 16-lane group: the wave maximum is the max of lanes 0, 16, 32, 48
+    uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    uint32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return max(max(a, b), max(c, d));
+}
+
+// A 16-lane group per alpha tile.  Fills arrive 16 at a time (one per lane); each
+// fill covers a run of pixel columns, and the group's (fill, column) pairs are
+// enumerated with a prefix sum so every lane works on a pair that contributes.
 __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     // 17 rows per tile slot (272 words): tile slots g and g+1 land on opposite bank halves
     __shared__ int32_t diff[FILL_TILES_PER_BLOCK][17][16];
+    __shared__ uint2 sfill[FILL_TILES_PER_BLOCK][16];
+    __shared__ uint32_t scum[FILL_TILES_PER_BLOCK][16];
     __shared__ uint32_t out8[FILL_TILES_PER_BLOCK][64];
-    const int g = threadIdx.x >> 4;     // tile slot in the block
-    const int col = threadIdx.x & 15;   // pixel column owned by this lane
+    const int g = threadIdx.x >> 4;   // tile slot in the block
+    const int li = threadIdx.x & 15;  // lane in the tile's group (= pixel column for the output)
     const uint32_t a = blockIdx.x * FILL_TILES_PER_BLOCK + g;
     const uint32_t na = min(b.counters[C_ALPHA], b.cap_alpha);
-    if (a >= na) return;  // no block-wide barriers below: lanes of one tile stay within one wave
+    if (blockIdx.x * FILL_TILES_PER_BLOCK >= na) return;  // whole block past the last tile
+    const bool valid = a < na;
+    // all lanes stay to the end: loops below have wave-uniform bounds and use shuffles
 #pragma unroll
-    for (int r = 0; r < 17; r++) diff[g][r][col] = 0;
-    const uint4 rec = b.alpha_rec[a];
+    for (int r = 0; r < 17; r++) diff[g][r][li] = 0;
+    uint4 rec = make_uint4(0, 0, 0, 0);
+    if (valid) rec = b.alpha_rec[a];
+    // After a fill-capacity overflow a tile's range can start past the buffer (rec.x >=
+    // cap_fills): clamp both ends so the count never wraps (the batch is then retried).
+    const uint32_t f0 = valid ? min(rec.x, b.cap_fills) : 0u;
+    const uint32_t f1 = valid ? min(rec.y, b.cap_fills) : 0u;
+    const uint32_t nf = f1 > f0 ? f1 - f0 : 0u;
     const float scale = (float)(1 << FIX_SHIFT);
-    const float cx0 = (float)col, cx1 = (float)(col + 1);
-    const uint32_t f1 = min(rec.y, b.cap_fills);
-    const int group_base = (threadIdx.x & 63) & ~15;
-    // fills arrive 16 at a time with one coalesced load (one per lane of the tile),
-    // then are broadcast across the tile's lanes
-    for (uint32_t fc = rec.x; fc < f1; fc += 16) {
-        uint2 mine = make_uint2(0, 0);
-        if (fc + col < f1) mine = b.fills[fc + col];
-        const uint32_t nchunk = min(16u, f1 - fc);
-        for (uint32_t j = 0; j < nchunk; j++) {
-            uint2 fl;
-            fl.x = (uint32_t)__shfl((int)mine.x, group_base + (int)j, 64);
-            fl.y = (uint32_t)__shfl((int)mine.y, group_base + (int)j, 64);
-            const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), y0 = (float)(fl.x >> 16) * (1.0f / 256.0f);
-            const float x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f), y1 = (float)(fl.y >> 16) * (1.0f / 256.0f);
-            const float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
-            const float lo = fmaxf(cx0, xl), hi = fminf(cx1, xr);
-            if (!(hi > lo)) continue;
-            const float sign = x0 > x1 ? 1.0f : -1.0f;
-            const float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
-            const float slope = (yr - yl) * __builtin_amdgcn_rcpf(xr - xl);
-            const float ya = yl + (lo - xl) * slope, yb = yl + (hi - xl) * slope;
-            const float w = hi - lo;
-            const int r0 = max(0, min(15, (int)floorf(fminf(ya, yb))));
-            const int r1 = max(0, min(15, (int)floorf(fmaxf(ya, yb))));
-            int32_t prev = 0;
-            for (int r = r0; r <= r1; r++) {
-                const float rb = (float)(r + 1);
-                const int32_t v = (int32_t)rintf(sign * w * avg_clamp01(rb - ya, rb - yb) * scale);
-                diff[g][r][col] += v - prev;
-                prev = v;
-            }
-            diff[g][r1 + 1][col] += (int32_t)rintf(sign * w * scale) - prev;
+    const uint32_t chunks = wave_max_of_groups((nf + 15) >> 4);
+    for (uint32_t ch = 0; ch < chunks; ch++) {
+        const uint32_t i = ch * 16 + li;
+        uint2 fl = make_uint2(0, 0);
+        uint32_t cnt = 0;
+        if (i < nf) {
+            fl = b.fills[f0 + i];
+            const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f);
+            const int c0 = (int)floorf(fminf(x0, x1)), c1 = (int)ceilf(fmaxf(x0, x1)) - 1;
+            cnt = (uint32_t)max(0, c1 - c0 + 1);
         }
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)incl, d, 16);
+            if (li >= d) incl += o;
+        }
+        const uint32_t total = (uint32_t)__shfl((int)incl, 15, 16);
+        sfill[g][li] = fl;
+        scum[g][li] = incl;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t rounds = wave_max_of_groups((total + 15) >> 4);
+        for (uint32_t rd = 0; rd < rounds; rd++) {
+            const uint32_t q = rd * 16 + li;
+            if (q < total) {
+                // the fill owning pair q: first j with scum[j] > q
+                int j = 0;
+#pragma unroll
+                for (int st = 8; st > 0; st >>= 1)
+                    if (scum[g][j + st - 1] <= q) j += st;
+                const uint2 fj = sfill[g][j];
+                const uint32_t before = j ? scum[g][j - 1] : 0u;
+                const float x0 = (float)(fj.x & 0xffffu) * (1.0f / 256.0f), x1 = (float)(fj.y & 0xffffu) * (1.0f / 256.0f);
+                const int col = (int)floorf(fminf(x0, x1)) + (int)(q - before);
+                accumulate_fill(fj, col, scale, &diff[g][0][0]);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     const float bd = (float)(int8_t)(rec.z & 0xffu);
     const uint32_t rule = (rec.z >> 8) & 1u;
@@ -99,18 +152,18 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     uint8_t *bytes = reinterpret_cast<uint8_t *>(&out8[g][0]);
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        run += diff[g][r][col];
+        run += diff[g][r][li];
         const float c = bd + (float)run * inv;
-        bytes[r * 16 + col] = (uint8_t)rintf(apply_rule(c, rule) * 255.0f);
+        bytes[r * 16 + li] = (uint8_t)rintf(apply_rule(c, rule) * 255.0f);
     }
-    // each lane stores one 16-byte row of its tile: 256 B per tile, coalesced.  The
-    // 16 lanes of a tile are one wave's contiguous lanes: order their LDS bytes first.
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    // each lane stores one 16-byte row of its tile: 256 B per tile, coalesced
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint4 row = reinterpret_cast<const uint4 *>(&out8[g][0])[col];
-    reinterpret_cast<uint4 *>(b.masks + (uint64_t)a * 256)[col] = row;
+    if (valid) {
+        const uint4 row = reinterpret_cast<const uint4 *>(&out8[g][0])[li];
+        reinterpret_cast<uint4 *>(b.masks + (uint64_t)a * 256)[li] = row;
+    }
 }
 
 // ---------------------------------------------------------------------------

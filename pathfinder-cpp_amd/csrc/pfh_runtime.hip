@@ -22,7 +22,11 @@ size_t scan_temp_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp, hipStream_t s);
 hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, uint32_t *tmp, hipStream_t s);
 hipError_t scan_tiles(const BatchDev &b, void *tmp, hipStream_t s);
-hipError_t launch_tile_segments(const BatchDev &b, bool emit, hipStream_t s);
+hipError_t launch_flatten(const BatchDev &b, bool emit, hipStream_t s);
+hipError_t launch_bin(const BatchDev &b, bool emit, uint32_t grid_lines, hipStream_t s);
+hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
+                        uint32_t *tmp, hipStream_t s);
+hipError_t scan_total(uint32_t *inout, uint32_t n, uint32_t *counters, int counter, uint32_t *tmp, hipStream_t s);
 hipError_t launch_propagate(const BatchDev &b, hipStream_t s);
 hipError_t launch_group_fills(const BatchDev &b, uint32_t grid_fills, hipStream_t s);
 hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s);
@@ -111,19 +115,20 @@ struct SceneHost {
     uint32_t seg_base = 0, tile_base = 0, col_base = 0, fb_base = 0;
     // exact counts of the last run (capacity planning)
     bool counted = false;
-    uint32_t last_fills = 0, last_alpha = 0, last_entries = 0;
-    float pred_fills = -1, pred_alpha = -1;
+    uint32_t last_fills = 0, last_alpha = 0, last_entries = 0, last_lines = 0;
+    float pred_fills = -1, pred_alpha = -1, pred_lines = -1;
 };
 
 struct GraphKey {
     std::vector<int32_t> ids;
     uint32_t flags;
-    uint32_t cap_f, cap_a, cap_e;
+    uint32_t cap_f, cap_a, cap_e, cap_l;
     bool operator<(const GraphKey &o) const {
         if (ids != o.ids) return ids < o.ids;
         if (flags != o.flags) return flags < o.flags;
         if (cap_f != o.cap_f) return cap_f < o.cap_f;
         if (cap_a != o.cap_a) return cap_a < o.cap_a;
+        if (cap_l != o.cap_l) return cap_l < o.cap_l;
         return cap_e < o.cap_e;
     }
 };
@@ -138,7 +143,7 @@ struct pfh_ctx {
     std::vector<std::unique_ptr<SceneHost>> scenes;
     // batch work buffers
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_entry, col_bd,
-        seg_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_cnt, keys, vals, keys_alt, vals_alt, entry_rec,
+        seg_cnt, lines, line_seg, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_cnt, keys, vals, keys_alt, vals_alt, entry_rec,
         counters, scan_tmp, sort_tmp;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
@@ -391,13 +396,13 @@ static SceneHost &get_scene(pfh_ctx *ctx, int32_t id) {
 static void predict_scene(const pfh_ctx *ctx, SceneHost &s) {
     const std::vector<float> &w = ctx->predictor;
     if (w.empty()) {
-        s.pred_fills = s.pred_alpha = -1;
+        s.pred_fills = s.pred_alpha = s.pred_lines = -1;
         return;
     }
-    // layout: normalisation (mean[12], inv_std[12]), W1[12x64], b1[64], W2[64x64], b2[64], W3[64x2], b3[2]
+    // layout: normalisation (mean[12], inv_std[12]), W1[12x64], b1[64], W2[64x64], b2[64], W3[64x3], b3[3]
     const float *mean = w.data(), *istd = mean + 12, *W1 = istd + 12, *b1 = W1 + 12 * 64, *W2 = b1 + 64,
-                *b2 = W2 + 64 * 64, *W3 = b2 + 64, *b3 = W3 + 64 * 2;
-    double sf = 0, sa = 0;
+                *b2 = W2 + 64 * 64, *W3 = b2 + 64, *b3 = W3 + 64 * 3;
+    double sl = 0, sf = 0, sa = 0;
     float x[12], h1[64], h2[64];
     for (uint32_t p = 0; p < s.n_paths; p++) {
         for (int i = 0; i < 12; i++) x[i] = (s.features[p * 12 + i] - mean[i]) * istd[i];
@@ -411,15 +416,18 @@ static void predict_scene(const pfh_ctx *ctx, SceneHost &s) {
             for (int i = 0; i < 64; i++) a += h1[i] * W2[i * 64 + j];
             h2[j] = a > 0 ? a : 0;
         }
-        float o0 = b3[0], o1 = b3[1];
+        float o0 = b3[0], o1 = b3[1], o2 = b3[2];
         for (int i = 0; i < 64; i++) {
-            o0 += h2[i] * W3[i * 2 + 0];
-            o1 += h2[i] * W3[i * 2 + 1];
+            o0 += h2[i] * W3[i * 3 + 0];
+            o1 += h2[i] * W3[i * 3 + 1];
+            o2 += h2[i] * W3[i * 3 + 2];
         }
-        // targets are log1p(count)
-        sf += std::expm1(std::max(0.0f, o0));
-        sa += std::expm1(std::max(0.0f, o1));
+        // targets are log1p(count): chords, fills, alpha tiles
+        sl += std::expm1(std::max(0.0f, o0));
+        sf += std::expm1(std::max(0.0f, o1));
+        sa += std::expm1(std::max(0.0f, o2));
     }
+    s.pred_lines = (float)sl;
     s.pred_fills = (float)sf;
     s.pred_alpha = (float)sa;
 }
@@ -431,7 +439,7 @@ static void predict_scene(const pfh_ctx *ctx, SceneHost &s) {
 struct Plan {
     std::vector<int32_t> ids;
     uint64_t S = 0, T = 0, W = 0, FB = 0;
-    uint32_t cap_f = 0, cap_a = 0, cap_e = 0;
+    uint32_t cap_f = 0, cap_a = 0, cap_e = 0, cap_l = 0;
     int key_bits = 1;
 };
 
@@ -446,13 +454,16 @@ static void ensure_sizes(pfh_ctx *ctx, const Plan &pl, bool dump) {
     ctx->tile_entry.ensure(4 * pl.T + 8);
     ctx->col_bd.ensure(4 * pl.W + 4);
     ctx->seg_cnt.ensure(4 * pl.S + 8);
+    ctx->lines.ensure(16ull * pl.cap_l);
+    ctx->line_seg.ensure(4ull * pl.cap_l);
+    ctx->line_cnt.ensure(4ull * pl.cap_l + 8);
     ctx->zbuf.ensure(4 * pl.FB + 4);
     ctx->fb_cnt.ensure(4 * pl.FB + 8);
     ctx->counters.ensure(4 * C_COUNT);
-    uint64_t scan_n = std::max<uint64_t>({pl.T + 1, pl.S + 1, pl.FB + 1, 1});
+    uint64_t scan_n = std::max<uint64_t>({pl.T + 1, pl.S + 1, pl.FB + 1, (uint64_t)pl.cap_l + 1, 1});
     ctx->scan_tmp.ensure(16 * scan_temp_elems(scan_n) + 64);
     ctx->fills.ensure(8ull * pl.cap_f);
-    ctx->dump_fills.ensure(16ull * pl.cap_f);
+    if (dump) ctx->dump_fills.ensure(16ull * pl.cap_f);
     ctx->alpha_rec.ensure(16ull * pl.cap_a);
     ctx->masks.ensure(256ull * pl.cap_a);
     ctx->keys.ensure(4ull * pl.cap_e);
@@ -565,6 +576,7 @@ static BatchDev batch_dev(pfh_ctx *ctx, const Plan &pl, bool dump) {
     b.cap_fills = pl.cap_f;
     b.cap_alpha = pl.cap_a;
     b.cap_entries = pl.cap_e;
+    b.cap_lines = pl.cap_l;
     b.tile_delta = ctx->tile_delta.as<int32_t>();
     b.tile_cnt = ctx->tile_cnt.as<uint32_t>();
     b.tile_fill_off = ctx->tile_fill_off.as<uint32_t>();
@@ -573,8 +585,11 @@ static BatchDev batch_dev(pfh_ctx *ctx, const Plan &pl, bool dump) {
     b.tile_entry = ctx->tile_entry.as<uint32_t>();
     b.col_bd = ctx->col_bd.as<int32_t>();
     b.seg_cnt = ctx->seg_cnt.as<uint32_t>();
+    b.lines = ctx->lines.as<float4>();
+    b.line_seg = ctx->line_seg.as<uint32_t>();
+    b.line_cnt = ctx->line_cnt.as<uint32_t>();
     b.fills = ctx->fills.as<uint2>();
-    b.dump_fills = ctx->dump_fills.as<uint4>();
+    b.dump_fills = dump ? ctx->dump_fills.as<uint4>() : nullptr;
     b.alpha_rec = ctx->alpha_rec.as<uint4>();
     b.masks = ctx->masks.as<uint8_t>();
     b.zbuf = ctx->zbuf.as<uint32_t>();
@@ -600,18 +615,25 @@ static void stage_clear(pfh_ctx *ctx, const BatchDev &b) {
     HIP_TRY(launch_fill_u32(b.counters, 0, C_COUNT, s));
 }
 
+// flatten (count, scan, emit) and bin count: chords and exact fill / alpha-tile counts
 static void stage_count(pfh_ctx *ctx, const BatchDev &b) {
-    HIP_TRY(launch_tile_segments(b, false, ctx->stream));
-    HIP_TRY(scan_tiles(b, ctx->scan_tmp.ptr, ctx->stream));
+    hipStream_t s = ctx->stream;
+    uint32_t *tmp = ctx->scan_tmp.as<uint32_t>();
+    HIP_TRY(launch_flatten(b, false, s));
+    HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp, s));
+    HIP_TRY(launch_flatten(b, true, s));
+    HIP_TRY(launch_bin(b, false, b.cap_lines, s));
+    HIP_TRY(scan_tiles(b, ctx->scan_tmp.ptr, s));
 }
 
 static void stage_emit(pfh_ctx *ctx, const BatchDev &b) {
-    HIP_TRY(scan_u32(b.seg_cnt, b.seg_cnt, b.n_segs, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
-    HIP_TRY(launch_tile_segments(b, true, ctx->stream));
-    HIP_TRY(launch_group_fills(b, b.cap_fills, ctx->stream));
-    HIP_TRY(launch_propagate(b, ctx->stream));
-    HIP_TRY(launch_entries(b, false, ctx->stream));
-    HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
+    hipStream_t s = ctx->stream;
+    uint32_t *tmp = ctx->scan_tmp.as<uint32_t>();
+    if (b.dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, b.cap_lines, b.counters, -1, tmp, s));
+    HIP_TRY(launch_bin(b, true, b.cap_lines, s));
+    HIP_TRY(launch_propagate(b, s));
+    HIP_TRY(launch_entries(b, false, s));
+    HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp, s));
 }
 
 static void stage_finish(pfh_ctx *ctx, const BatchDev &b, const Plan &pl) {
@@ -642,6 +664,8 @@ static void record_counts(pfh_ctx *ctx, const Plan &pl) {
     ctx->stats.entries = ctx->h_counters[C_ENTRIES];
     ctx->stats.guard_hits = ctx->h_counters[C_GUARD];
     ctx->stats.alpha_entries = ctx->h_counters[C_ALPHA_ENTRIES];
+    ctx->stats.lines = ctx->h_counters[C_LINES];
+    ctx->stats.capacity_lines = pl.cap_l;
     ctx->stats.tiles = (int64_t)pl.T;
     ctx->stats.columns = (int64_t)pl.W;
     ctx->stats.segments = (int64_t)pl.S;
@@ -660,19 +684,32 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     write_tables(ctx, pl, (flags & PFH_LOAD_DST) != 0);
     pl.cap_f = clamp_cap(ctx->fills.bytes / 8);
     pl.cap_a = clamp_cap(ctx->alpha_rec.bytes / 16);
+    pl.cap_l = clamp_cap(ctx->lines.bytes / 16);
     pl.cap_e = clamp_cap(std::max<uint64_t>(pl.T, 1));
     ensure_sizes(ctx, pl, dump);
     upload_tables(ctx, pl);
     BatchDev b = batch_dev(ctx, pl, dump);
+    hipStream_t st = ctx->stream;
+    uint32_t *tmp = ctx->scan_tmp.as<uint32_t>();
     hipEvent_t ev[9];
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-    HIP_TRY(hipEventRecord(ev[0], ctx->stream));
+    HIP_TRY(hipEventRecord(ev[0], st));
     stage_clear(ctx, b);
-    HIP_TRY(launch_tile_segments(b, false, ctx->stream));
-    HIP_TRY(hipEventRecord(ev[1], ctx->stream));
-    HIP_TRY(scan_tiles(b, ctx->scan_tmp.ptr, ctx->stream));
-    read_counters(ctx);  // fill / alpha counts (renderer.cpp:707-724, 834-845)
-    HIP_TRY(hipEventRecord(ev[2], ctx->stream));
+    HIP_TRY(launch_flatten(b, false, st));
+    HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp, st));
+    read_counters(ctx);  // chord count (renderer.cpp:707-724)
+    uint32_t nl = ctx->h_counters[C_LINES];
+    if (nl > pl.cap_l) {
+        pl.cap_l = clamp_cap(nl);
+        ensure_sizes(ctx, pl, dump);
+        b = batch_dev(ctx, pl, dump);
+    }
+    HIP_TRY(launch_flatten(b, true, st));
+    HIP_TRY(hipEventRecord(ev[1], st));
+    HIP_TRY(launch_bin(b, false, nl, st));
+    HIP_TRY(scan_tiles(b, ctx->scan_tmp.ptr, st));
+    read_counters(ctx);  // fill / alpha counts (renderer.cpp:834-845)
+    HIP_TRY(hipEventRecord(ev[2], st));
     uint32_t nf = ctx->h_counters[C_FILLS], na = ctx->h_counters[C_ALPHA];
     if (nf > pl.cap_f || na > pl.cap_a) {
         pl.cap_f = clamp_cap(std::max<uint64_t>(nf, pl.cap_f));
@@ -680,40 +717,32 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
         ensure_sizes(ctx, pl, dump);
         b = batch_dev(ctx, pl, dump);
     }
-    HIP_TRY(scan_u32(b.seg_cnt, b.seg_cnt, b.n_segs, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
-    HIP_TRY(launch_tile_segments(b, true, ctx->stream));
-    HIP_TRY(launch_group_fills(b, nf, ctx->stream));
-    HIP_TRY(hipEventRecord(ev[3], ctx->stream));
-    HIP_TRY(launch_propagate(b, ctx->stream));
-    HIP_TRY(hipEventRecord(ev[4], ctx->stream));
-    HIP_TRY(launch_entries(b, false, ctx->stream));
-    HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
-    read_counters(ctx);  // alpha-tile / entry counts (renderer.cpp:943-952)
-    HIP_TRY(launch_entries(b, true, ctx->stream));
-    HIP_TRY(hipEventRecord(ev[5], ctx->stream));
-    HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
+    if (dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, nl, b.counters, -1, tmp, st));
+    HIP_TRY(launch_bin(b, true, nl, st));
+    HIP_TRY(hipEventRecord(ev[3], st));
+    HIP_TRY(launch_propagate(b, st));
+    HIP_TRY(hipEventRecord(ev[4], st));
+    HIP_TRY(launch_entries(b, false, st));
+    HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp, st));
+    HIP_TRY(launch_entries(b, true, st));
+    HIP_TRY(hipEventRecord(ev[5], st));
+    HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, tmp, st));
     bool alt = false;
     HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
-                       ctx->sort_tmp.as<uint32_t>(), &alt, ctx->stream));
+                       ctx->sort_tmp.as<uint32_t>(), &alt, st));
     ctx->last_sorted_alt = alt;
-    HIP_TRY(hipEventRecord(ev[6], ctx->stream));
-    HIP_TRY(launch_fill_masks(b, ctx->h_counters[C_ALPHA], ctx->stream));
-    HIP_TRY(hipEventRecord(ev[7], ctx->stream));
-    HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, ctx->stream));
-    HIP_TRY(hipEventRecord(ev[8], ctx->stream));
+    HIP_TRY(hipEventRecord(ev[6], st));
+    HIP_TRY(launch_fill_masks(b, na, st));
+    HIP_TRY(hipEventRecord(ev[7], st));
+    HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, st));
+    HIP_TRY(hipEventRecord(ev[8], st));
     read_counters(ctx);
-    ctx->stats.ms_stage[0] += elapsed_ms(ctx, ev[0], ev[1]);
-    ctx->stats.ms_stage[1] += elapsed_ms(ctx, ev[1], ev[2]);
-    ctx->stats.ms_stage[2] += elapsed_ms(ctx, ev[2], ev[3]);
-    ctx->stats.ms_stage[3] += elapsed_ms(ctx, ev[3], ev[4]);
-    ctx->stats.ms_stage[4] += elapsed_ms(ctx, ev[4], ev[5]);
-    ctx->stats.ms_stage[5] += elapsed_ms(ctx, ev[5], ev[6]);
-    ctx->stats.ms_stage[6] += elapsed_ms(ctx, ev[6], ev[7]);
-    ctx->stats.ms_stage[7] += elapsed_ms(ctx, ev[7], ev[8]);
+    for (int i = 0; i < 8; i++) ctx->stats.ms_stage[i] += elapsed_ms(ctx, ev[i], ev[i + 1]);
     for (auto &e : ev) (void)hipEventDestroy(e);
     record_counts(ctx, pl);
     SceneHost &s = get_scene(ctx, id);
     s.counted = true;
+    s.last_lines = nl;
     s.last_fills = ctx->h_counters[C_FILLS];
     s.last_alpha = ctx->h_counters[C_ALPHA];
     s.last_entries = ctx->h_counters[C_ENTRIES];
@@ -730,15 +759,17 @@ static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint3
     write_tables(ctx, pl, (flags & PFH_LOAD_DST) != 0);
     // Capacities: entries are bounded by tiles exactly; fills and alpha tiles come from
     // the predictor, else from the last exact counts, else from a count pass read back.
-    uint64_t pf = 0, pa = 0;
+    uint64_t pf = 0, pa = 0, plc = 0;
     bool known = true;
     for (int32_t id : pl.ids) {
         SceneHost &s = get_scene(ctx, id);
         if ((flags & PFH_PREDICT) && !ctx->predictor.empty()) {
             if (s.pred_fills < 0) predict_scene(ctx, s);
+            plc += (uint64_t)std::ceil(s.pred_lines * ctx->predictor_safety) + 64;
             pf += (uint64_t)std::ceil(s.pred_fills * ctx->predictor_safety) + 64;
             pa += (uint64_t)std::ceil(s.pred_alpha * ctx->predictor_safety) + 16;
         } else if (s.counted) {
+            plc += s.last_lines;
             pf += s.last_fills;
             pa += s.last_alpha;
         } else {
@@ -746,14 +777,23 @@ static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint3
         }
     }
     pl.cap_e = clamp_cap(std::max<uint64_t>(pl.T, 1));
-    for (int attempt = 0; attempt < 3; attempt++) {
+    for (int attempt = 0; attempt < 4; attempt++) {
         if (!known) {
-            // size from an eager count pass (one read back), as the plain pipeline does
+            // size from an eager count pass (read back), as the plain pipeline does
+            pl.cap_l = clamp_cap(ctx->lines.bytes / 16);
             pl.cap_f = clamp_cap(ctx->fills.bytes / 8);
             pl.cap_a = clamp_cap(ctx->alpha_rec.bytes / 16);
             ensure_sizes(ctx, pl, dump);
             upload_tables(ctx, pl);
             BatchDev b = batch_dev(ctx, pl, dump);
+            stage_clear(ctx, b);
+            HIP_TRY(launch_flatten(b, false, ctx->stream));
+            HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
+            read_counters(ctx);
+            plc = ctx->h_counters[C_LINES];
+            pl.cap_l = clamp_cap(std::max<uint64_t>(plc, pl.cap_l));
+            ensure_sizes(ctx, pl, dump);
+            b = batch_dev(ctx, pl, dump);
             stage_clear(ctx, b);
             stage_count(ctx, b);
             read_counters(ctx);
@@ -761,17 +801,19 @@ static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint3
             pa = ctx->h_counters[C_ALPHA];
             known = true;
         }
+        pl.cap_l = clamp_cap(std::max<uint64_t>(plc, (uint64_t)(ctx->lines.bytes / 16)));
         pl.cap_f = clamp_cap(std::max<uint64_t>(pf, (uint64_t)(ctx->fills.bytes / 8)));
         pl.cap_a = clamp_cap(std::max<uint64_t>(pa, (uint64_t)(ctx->alpha_rec.bytes / 16)));
         ensure_sizes(ctx, pl, dump);
         // ensure() rounds to 2^k: use the full allocation
+        pl.cap_l = clamp_cap(std::min<uint64_t>(ctx->lines.bytes / 16, ctx->line_seg.bytes / 4));
         pl.cap_f = clamp_cap(ctx->fills.bytes / 8);
         pl.cap_a = clamp_cap(std::min<uint64_t>(ctx->alpha_rec.bytes / 16, ctx->masks.bytes / 256));
         upload_tables(ctx, pl);
         BatchDev b = batch_dev(ctx, pl, dump);
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
         if (use_graph) {
-            GraphKey key{pl.ids, flags & ~(uint32_t)PFH_PREDICT, pl.cap_f, pl.cap_a, pl.cap_e};
+            GraphKey key{pl.ids, flags & ~(uint32_t)PFH_PREDICT, pl.cap_f, pl.cap_a, pl.cap_e, pl.cap_l};
             auto it = ctx->graphs.find(key);
             if (it == ctx->graphs.end()) {
                 if (ctx->graphs.size() > 32) {
@@ -808,8 +850,8 @@ static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint3
         }
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         read_counters(ctx);
-        uint32_t nf = ctx->h_counters[C_FILLS], na = ctx->h_counters[C_ALPHA];
-        if (nf <= pl.cap_f && na <= pl.cap_a) {
+        uint32_t nl = ctx->h_counters[C_LINES], nf = ctx->h_counters[C_FILLS], na = ctx->h_counters[C_ALPHA];
+        if (nl <= pl.cap_l && nf <= pl.cap_f && na <= pl.cap_a) {
             ctx->stats.ms_total = elapsed_ms(ctx, ctx->ev0, ctx->ev1);
             record_counts(ctx, pl);
             for (int32_t id : pl.ids) {
@@ -818,10 +860,12 @@ static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint3
             }
             return;
         }
-        // under-prediction: a retry with the exact counts, never a wrong pixel
+        // under-prediction: a retry with the exact counts, never a wrong pixel.  When the
+        // chord buffer overflowed, the fill and alpha counts are partial: keep growing.
         ctx->stats.retries++;
-        pf = nf;
-        pa = na;
+        plc = std::max<uint64_t>(plc, nl);
+        pf = std::max<uint64_t>(pf, nl > pl.cap_l ? 2ull * nf : nf);
+        pa = std::max<uint64_t>(pa, nl > pl.cap_l ? 2ull * na : na);
     }
     throw PfhError{PFH_E_OVERFLOW, "capacity still exceeded after retries"};
 }
@@ -879,7 +923,8 @@ void pfh_destroy(pfh_ctx *ctx) {
             b->release();
     }
     for (DevBuf *b : {&ctx->scene_table, &ctx->bases, &ctx->tile_delta, &ctx->tile_cnt, &ctx->tile_fill_off,
-                      &ctx->tile_alpha_idx, &ctx->tile_bd, &ctx->tile_entry, &ctx->col_bd, &ctx->seg_cnt, &ctx->fills,
+                      &ctx->tile_alpha_idx, &ctx->tile_bd, &ctx->tile_entry, &ctx->col_bd, &ctx->seg_cnt, &ctx->lines,
+                      &ctx->line_seg, &ctx->line_cnt, &ctx->fills,
                       &ctx->dump_fills, &ctx->alpha_rec, &ctx->masks, &ctx->zbuf, &ctx->fb_cnt, &ctx->keys, &ctx->vals,
                       &ctx->keys_alt, &ctx->vals_alt, &ctx->entry_rec, &ctx->counters, &ctx->scan_tmp, &ctx->sort_tmp,
                       &ctx->mlp_params, &ctx->mlp_x, &ctx->mlp_h, &ctx->mlp_out})
@@ -986,10 +1031,12 @@ int pfh_dump_counts(pfh_ctx *ctx, int32_t scene_id, int64_t *counts) {
         SceneHost &s = get_scene(ctx, scene_id);
         if (!s.dumped) throw PfhError{PFH_E_STATE, "scene was not rasterized with PFH_DUMP"};
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        uint32_t f0 = 0, f1 = 0, a0 = 0, a1 = 0, e0 = 0, e1 = 0;
+        uint32_t f0 = 0, f1 = 0, a0 = 0, a1 = 0, e0 = 0, e1 = 0, l0 = 0, l1 = 0;
         uint32_t sb = s.seg_base, tb = s.tile_base, fb = s.fb_base, nfb = (uint32_t)(s.fb_w * s.fb_h);
-        HIP_TRY(hipMemcpy(&f0, ctx->seg_cnt.as<uint32_t>() + sb, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&f1, ctx->seg_cnt.as<uint32_t>() + sb + s.n_segs, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&l0, ctx->seg_cnt.as<uint32_t>() + sb, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&l1, ctx->seg_cnt.as<uint32_t>() + sb + s.n_segs, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&f0, ctx->line_cnt.as<uint32_t>() + l0, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&f1, ctx->line_cnt.as<uint32_t>() + l1, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&a0, ctx->tile_alpha_idx.as<uint32_t>() + tb, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&a1, ctx->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&e0, ctx->fb_cnt.as<uint32_t>() + fb, 4, hipMemcpyDeviceToHost));
@@ -1019,22 +1066,28 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
         auto d2h = [](void *dst, const void *src, size_t bytes) {
             if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
         };
-        std::vector<uint32_t> seg_off(s.n_segs + 1);
-        d2h(seg_off.data(), ctx->seg_cnt.as<uint32_t>() + sb, 4 * (s.n_segs + 1));
+        // chords of a path start at its first segment's chord offset; fills of a chord at
+        // that chord's scanned fill offset
+        std::vector<uint32_t> seg_line(s.n_segs + 1);
+        d2h(seg_line.data(), ctx->seg_cnt.as<uint32_t>() + sb, 4 * (s.n_segs + 1));
+        uint32_t l0 = seg_line[0], l1 = seg_line[s.n_segs];
+        std::vector<uint32_t> line_fill(l1 - l0 + 1);
+        d2h(line_fill.data(), ctx->line_cnt.as<uint32_t>() + l0, 4 * (size_t)(l1 - l0 + 1));
         for (uint32_t p = 0; p < s.n_paths; p++) std::memcpy(path_rect + 4 * p, &s.path_rect[p], 16);
         for (uint32_t p = 0; p <= s.n_paths; p++) {
             tile_off[p] = s.path_tile_off[p];
             col_off[p] = s.path_col_off[p];
-            fill_off[p] = (int64_t)seg_off[s.path_seg_off[p]] - seg_off[0];
+            uint32_t first_line = seg_line[s.path_seg_off[p]] - l0;
+            fill_off[p] = (int64_t)line_fill[first_line] - line_fill[0];
         }
         d2h(tile_backdrop, ctx->tile_bd.as<int8_t>() + tb, s.n_tiles);
         std::vector<uint32_t> foff(s.n_tiles + 1);
         d2h(foff.data(), ctx->tile_fill_off.as<uint32_t>() + tb, 4 * (s.n_tiles + 1));
         for (uint32_t t = 0; t < s.n_tiles; t++) tile_alpha[t] = foff[t + 1] > foff[t] ? 1 : 0;
         d2h(col_sum, ctx->col_bd.as<int32_t>() + cb, 4 * s.n_cols);
-        uint32_t nf = seg_off[s.n_segs] - seg_off[0];
+        uint32_t nf = line_fill[l1 - l0] - line_fill[0];
         std::vector<uint4> df(nf);
-        d2h(df.data(), ctx->dump_fills.as<uint4>() + seg_off[0], 16 * (size_t)nf);
+        d2h(df.data(), ctx->dump_fills.as<uint4>() + line_fill[0], 16 * (size_t)nf);
         uint32_t p = 0;
         for (uint32_t i = 0; i < nf; i++) {
             while (p + 1 <= s.n_paths && (int64_t)i >= fill_off[p + 1]) p++;
@@ -1096,7 +1149,7 @@ int pfh_set_predictor(pfh_ctx *ctx, const float *weights, int64_t n_floats, floa
         if (!(safety >= 1.0f)) throw PfhError{PFH_E_ARG, "safety factor must be >= 1"};
         ctx->predictor_safety = safety;
         for (auto &s : ctx->scenes)
-            if (s) s->pred_fills = s->pred_alpha = -1;
+            if (s) s->pred_fills = s->pred_alpha = s->pred_lines = -1;
     });
 }
 
@@ -1112,7 +1165,7 @@ int pfh_mlp_backward(pfh_ctx *ctx, const float *d_params, const float *d_x, int6
                      const float *d_out, const float *d_target, float *d_grad, float *d_loss) {
     if (!ctx || !d_params || !d_x || !d_hidden || !d_out || !d_target || !d_grad || !d_loss || n < 0) return PFH_E_ARG;
     return run_guarded(ctx, [&] {
-        ctx->mlp_x.ensure(sizeof(float) * (size_t)std::max<int64_t>(n, 1) * 130);
+        ctx->mlp_x.ensure(sizeof(float) * (size_t)std::max<int64_t>(n, 1) * 131);
         HIP_TRY(mlp_backward(d_params, d_x, n, d_hidden, d_out, d_target, d_grad, d_loss, ctx->mlp_x.as<float>(),
                              ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -22,7 +22,8 @@ PFH_LOAD_DST = 0x8
 PFH_NO_GRAPH = 0x10
 
 N_FEATURES = 12
-MLP_PARAMS = 5122
+MLP_OUTPUTS = 3
+MLP_PARAMS = 12 * 64 + 64 + 64 * 64 + 64 + 64 * MLP_OUTPUTS + MLP_OUTPUTS
 PREDICTOR_FLOATS = 24 + MLP_PARAMS
 
 _P = ctypes.c_void_p
@@ -41,7 +42,8 @@ class Stats(ctypes.Structure):
                 ("entries", ctypes.c_int64), ("fb_tiles", ctypes.c_int64), ("retries", ctypes.c_int64),
                 ("guard_hits", ctypes.c_int64), ("ms_total", ctypes.c_double), ("ms_stage", ctypes.c_double * 8),
                 ("capacity_fills", ctypes.c_int64), ("capacity_alpha", ctypes.c_int64),
-                ("capacity_entries", ctypes.c_int64), ("alpha_entries", ctypes.c_int64)]
+                ("capacity_entries", ctypes.c_int64), ("alpha_entries", ctypes.c_int64), ("lines", ctypes.c_int64),
+                ("capacity_lines", ctypes.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "ms_stage"}
