@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pathfinder-cpp_amd"))
 
 import pfh  # noqa: E402
+import predictor  # noqa: E402
 import scenegen  # noqa: E402
 
 METRIC = "paths/sec + Mpix/sec composited (batched vs plain), ≤1/255 alpha err"
@@ -100,6 +101,11 @@ def main():
     print(f"[bench] {args.config}: {len(scenes)} scene(s), {n_paths} paths, {pixels / 1e6:.1f} Mpix", flush=True)
 
     ctx = pfh.Context(0)
+    batched_flags = pfh.PFH_BATCHED
+    weights = predictor.load()
+    if weights is not None:
+        ctx.set_predictor(weights, safety=1.25)
+        batched_flags |= pfh.PFH_PREDICT
     ids = [ctx.upload(s) for s in scenes]
     print("[bench] uploaded", flush=True)
 
@@ -121,9 +127,8 @@ def main():
 
     # batched: all scenes in one pass, replayed from a HIP graph
     for _ in range(args.warmup):
-        ctx.rasterize(ids, pfh.PFH_BATCHED)
-    st_b = ctx.stats()
-    batched_s = time_steps(ctx, ids, pfh.PFH_BATCHED, args.steps)
+        ctx.rasterize(ids, batched_flags)
+    batched_s = time_steps(ctx, ids, batched_flags, args.steps)
     st = ctx.stats()
     print(f"[bench] batched {batched_s * 1e3:.3f} ms/step", flush=True)
 
@@ -151,14 +156,15 @@ def main():
         "metric": METRIC, "value": round(n_paths / batched_s, 1), "unit": "paths/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(batched_s * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": workload, "mode": "batched, graph replay", "scenes": len(scenes), "paths": n_paths,
+        "config": {"workload": workload, "mode": "batched, graph replay" + (
+            ", predictor-sized" if batched_flags & pfh.PFH_PREDICT else ""), "scenes": len(scenes), "paths": n_paths,
                    "pixels": pixels},
         "mpix_per_s": round(pixels / batched_s / 1e6, 1),
         "plain_paths_per_s": round(n_paths / plain_s, 1), "plain_ms_per_step": round(plain_s * 1e3, 4),
         "batched_over_plain": round(plain_s / batched_s, 3),
         "plain_stage_ms": {k: round(v, 4) for k, v in zip(names, stage_ms)},
-        "counts": {k: st[k] for k in ("segments", "tiles", "fills", "alpha_tiles", "entries", "alpha_entries",
-                                      "fb_tiles", "retries")},
+        "counts": {k: st[k] for k in ("segments", "lines", "tiles", "fills", "alpha_tiles", "entries",
+                                      "alpha_entries", "fb_tiles", "retries", "host_syncs")},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -78,6 +78,7 @@ typedef struct pfh_stats {
     int64_t capacity_fills, capacity_alpha, capacity_entries;
     int64_t alpha_entries;    /* draw entries that sample a coverage mask */
     int64_t lines, capacity_lines; /* flattened chords and the chord buffer's capacity */
+    int64_t host_syncs;       /* device->host round trips during the last pfh_rasterize */
 } pfh_stats;
 
 /* Context on HIP device `device`.  Replaces Renderer's Device/Queue pair (core/renderer.cpp:13-37). */
@@ -123,25 +124,37 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
 int pfh_stats_get(pfh_ctx *ctx, pfh_stats *out);
 
 /*
- * Sizing predictor: per path, an FP32 MLP 12 -> 64 -> 64 -> 3 predicting log1p of
- * (flattened chords, fills, alpha tiles).  weights = mean[12], inv_std[12] (feature
- * normalisation) followed by the 5187 MLP parameters (layout in pfh_mlp_*).  safety >= 1
- * scales the predicted capacities.  weights == NULL removes the predictor.
+ * Sizing predictor (replaces the count / read back / grow-to-2^k / retry sizing of
+ * core/d3d11/renderer.cpp:537-577, 707-724, 834-845 for batched launches): per path, an
+ * FP32 MLP 12 -> 64 -> 64 -> 3 predicting log1p of (flattened chords, fills, alpha tiles).
+ * weights = mean[12], inv_std[12] (feature normalisation) followed by the 5187 MLP
+ * parameters (layout at pfh_mlp_eval).  safety >= 1 scales the predicted capacities.
+ * weights == NULL removes the predictor.
  */
 int pfh_set_predictor(pfh_ctx *ctx, const float *weights, int64_t n_floats, float safety);
+/* Predicted (chords, fills, alpha tiles) of an uploaded scene (-1 without a predictor). */
+int pfh_scene_prediction(pfh_ctx *ctx, int32_t scene_id, float *out3);
 
 /*
- * Device kernels of the sizing MLP, exposed for training and gradient checks (device pointers).
- * Parameters (5187 floats): W1[12][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3];
- * d_hidden holds n x 128 activations; d_out / d_target n x 3; d_loss one float (mean squared error).
- * pfh_mlp_sgd applies one Adam step.
+ * Exact per-path sizes of the last PFH_DUMP rasterize of a scene: flattened chords,
+ * fills and alpha tiles — the training targets of the sizing predictor.
  */
-int pfh_mlp_forward(pfh_ctx *ctx, const float *d_params, const float *d_x, int64_t n, float *d_hidden,
-                    float *d_out);
-int pfh_mlp_backward(pfh_ctx *ctx, const float *d_params, const float *d_x, int64_t n, const float *d_hidden,
-                     const float *d_out, const float *d_target, float *d_grad, float *d_loss);
-int pfh_mlp_sgd(pfh_ctx *ctx, float *d_params, float *d_m, float *d_v, const float *d_grad, int64_t n_params,
-                float lr, int32_t step);
+int pfh_path_counts(pfh_ctx *ctx, int32_t scene_id, int32_t *lines, int32_t *fills, int32_t *alpha_tiles);
+
+/*
+ * The sizing MLP's HIP kernels on host arrays (the context stages them on the device).
+ * params: 5187 floats, W1[12][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3] (W[in][out]);
+ * x: n x 12 (normalised features); out / target: n x 3.
+ *   pfh_mlp_eval:  out = MLP(x)
+ *   pfh_mlp_grad:  gradient of the mean squared error over all n x 3 outputs, and the loss
+ *   pfh_mlp_train: `epochs` of Adam over shuffled mini-batches; params updated in place;
+ *                  loss_history[epochs] receives the full-data loss after each epoch.
+ */
+int pfh_mlp_eval(pfh_ctx *ctx, const float *params, const float *x, int64_t n, float *out);
+int pfh_mlp_grad(pfh_ctx *ctx, const float *params, const float *x, const float *target, int64_t n, float *grad,
+                 float *loss);
+int pfh_mlp_train(pfh_ctx *ctx, const float *x, const float *target, int64_t n, int32_t epochs, int32_t batch,
+                  float lr, uint64_t seed, float *params, float *loss_history);
 
 #ifdef __cplusplus
 }

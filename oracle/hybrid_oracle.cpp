@@ -65,6 +65,7 @@ struct PathTiling {
     std::vector<int32_t> col;       // per column: running sum (above-rect part during tiling, final after)
     std::vector<int32_t> col_above; // per column: above-rect sum before propagation
     std::vector<OFill> fills;       // emission order
+    uint32_t lines = 0;             // chords handed to process_line_segment
     uint32_t guard_hits = 0;        // loop guards tripped (reference would not terminate)
 };
 
@@ -318,6 +319,7 @@ struct Builder {
     void process_segment(const Seg &s, int depth) {
         const float tol = 1.0f; // FLATTENING_TOLERANCE, tiler.cpp:15
         if (s.kind == K_LINE) {
+            pt->lines++;
             process_line(s.p0, s.p1);
             return;
         }
@@ -326,6 +328,7 @@ struct Builder {
         // depth 16 (the flatness measure shrinks 16x per halving).
         if (flat || depth >= 16) {
             if (!flat) pt->guard_hits++;
+            pt->lines++;
             process_line(s.p0, s.p1);
             return;
         }
@@ -399,7 +402,10 @@ static void tile_path(const Scene &sc, int p, PathTiling &pt) {
     bld.tb = pt.rect[3];
     bld.pt = &pt;
     std::vector<Seg> segs;
-    for (int c = sc.path_contour_start[p]; c < sc.path_contour_start[p + 1]; c++) {
+    // With no tile columns nothing can land: add_fill culls every tile and
+    // adjust_alpha_tile_backdrop ignores x outside [0, width).
+    int first_contour = w > 0 ? sc.path_contour_start[p] : sc.path_contour_start[p + 1];
+    for (int c = first_contour; c < sc.path_contour_start[p + 1]; c++) {
         int c0 = sc.contour_start[c], c1 = sc.contour_start[c + 1];
         segs.clear();
         contour_segments((const V2 *)(sc.points + 2 * c0), sc.point_flags + c0, c1 - c0, sc.contour_closed[c] != 0,
@@ -568,6 +574,11 @@ void pfo_counts(void *h, int64_t *counts) {
     counts[5] = (int64_t)r->paths.size();
     counts[6] = a;
     counts[7] = g;
+}
+
+void pfo_path_lines(void *h, int32_t *out) {
+    auto *r = (Result *)h;
+    for (size_t p = 0; p < r->paths.size(); p++) out[p] = (int32_t)r->paths[p].lines;
 }
 
 double pfo_tiling_seconds(void *h) {

@@ -46,6 +46,7 @@ def load_oracle():
         lib.pfo_counts.argtypes = [_P, _P]
         lib.pfo_fetch.argtypes = [_P] * 17
         lib.pfo_free.argtypes = [_P]
+        lib.pfo_path_lines.argtypes = [_P, _P]
         lib.pfo_tiling_seconds.restype = ctypes.c_double
         lib.pfo_tiling_seconds.argtypes = [_P]
         lib.pfo_fill_area.restype = ctypes.c_double
@@ -117,6 +118,8 @@ def run_oracle(scene, zmode: int = 1) -> dict:
                   _ptr(out["entry_tile"]))
     out["n_alpha"] = int(counts[6])
     out["guard_hits"] = int(counts[7])
+    out["path_lines"] = np.zeros(scene.n_paths, np.int32)
+    lib.pfo_path_lines(h, _ptr(out["path_lines"]))
     out["tiling_seconds"] = lib.pfo_tiling_seconds(h)
     lib.pfo_free(h)
     return out

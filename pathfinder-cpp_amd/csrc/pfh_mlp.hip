@@ -19,6 +19,12 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NI * NH, OFF_W2 = OFF_B1 + NH, OFF_B
 
 int64_t mlp_param_count() { return N_PARAMS; }
 
+__global__ void k_zero(float *p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+
 __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ x,
                                                      int64_t n, float *__restrict__ hidden, float *__restrict__ out) {
     __shared__ float w[N_PARAMS];
@@ -154,9 +160,60 @@ __global__ __launch_bounds__(256) void k_mlp_outer(const float *__restrict__ A, 
     }
 }
 
-__global__ void k_zero(float *p, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
+
+// Sizing inference: per path forward pass, expm1 of the (log1p) outputs clamped at 0,
+// summed over the paths into sums[NO] (float atomics per block).
+__global__ __launch_bounds__(256) void k_mlp_predict_sum(const float *__restrict__ P, const float *__restrict__ x,
+                                                         int64_t n, float *__restrict__ sums) {
+    __shared__ float w[N_PARAMS];
+    __shared__ float red[4][NO];
+    for (int i = threadIdx.x; i < N_PARAMS; i += blockDim.x) w[i] = P[i];
+    __syncthreads();
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float o[NO];
+#pragma unroll
+    for (int q = 0; q < NO; q++) o[q] = 0.0f;
+    if (s < n) {
+        float xi[NI];
+#pragma unroll
+        for (int i = 0; i < NI; i++) xi[i] = x[s * NI + i];
+        float h1[NH];
+#pragma unroll 4
+        for (int j = 0; j < NH; j++) {
+            float a = w[OFF_B1 + j];
+#pragma unroll
+            for (int i = 0; i < NI; i++) a += xi[i] * w[OFF_W1 + i * NH + j];
+            h1[j] = fmaxf(a, 0.0f);
+        }
+#pragma unroll
+        for (int q = 0; q < NO; q++) o[q] = w[OFF_B3 + q];
+        for (int j = 0; j < NH; j++) {
+            float a = w[OFF_B2 + j];
+#pragma unroll 8
+            for (int i = 0; i < NH; i++) a += h1[i] * w[OFF_W2 + i * NH + j];
+            const float h2 = fmaxf(a, 0.0f);
+#pragma unroll
+            for (int q = 0; q < NO; q++) o[q] += h2 * w[OFF_W3 + j * NO + q];
+        }
+#pragma unroll
+        for (int q = 0; q < NO; q++) o[q] = expm1f(fmaxf(o[q], 0.0f));
+    }
+#pragma unroll
+    for (int q = 0; q < NO; q++) {
+        float v = o[q];
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NO) atomicAdd(&sums[threadIdx.x], red[0][threadIdx.x] + red[1][threadIdx.x] +
+                                                            red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+hipError_t mlp_predict_sum(const float *params, const float *x, int64_t n, float *sums, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(32), 0, s, sums, (int64_t)NO);
+    if (n > 0)
+        hipLaunchKernelGGL(k_mlp_predict_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, params, x, n, sums);
+    return hipGetLastError();
 }
 
 hipError_t mlp_forward(const float *params, const float *x, int64_t n, float *hidden, float *out, hipStream_t s) {

@@ -41,6 +41,7 @@ hipError_t mlp_backward(const float *params, const float *x, int64_t n, const fl
 hipError_t mlp_adam(float *params, float *m, float *v, const float *grad, int64_t n, float lr, int step,
                     hipStream_t s);
 int64_t mlp_param_count();
+hipError_t mlp_predict_sum(const float *params, const float *x, int64_t n, float *sums, hipStream_t s);
 }  // namespace pfh
 
 using namespace pfh;
@@ -60,6 +61,9 @@ struct PfhError {
         if (_e != hipSuccess) throw PfhError{PFH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
     } while (0)
 
+// Bumped whenever a device buffer is (re)allocated: captured graphs hold raw pointers.
+static uint64_t g_alloc_generation = 0;
+
 // Grow-only device buffer (the reference grows buffers to 2^k, core/d3d11/renderer.cpp:719-724).
 struct DevBuf {
     void *ptr = nullptr;
@@ -71,6 +75,7 @@ struct DevBuf {
         if (ptr) HIP_TRY(hipFree(ptr));
         ptr = nullptr;
         bytes = 0;
+        g_alloc_generation++;
         hipError_t e = hipMalloc(&ptr, cap);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -106,6 +111,7 @@ struct SceneHost {
     std::vector<uint32_t> path_tile_off, path_col_off, path_seg_off, path_rgba;
     std::vector<uint8_t> path_rule;
     std::vector<float> features;
+    DevBuf d_features;
     DevBuf d_pts, d_segs, d_seg_path, d_path_rect, d_path_tile_off, d_path_col_off, d_col_path, d_path_rgba,
         d_path_rule, d_out_owned;
     uint8_t *d_out = nullptr;
@@ -152,6 +158,7 @@ struct pfh_ctx {
     size_t h_bases_cap = 0;
     bool last_sorted_alt = false;
     std::map<GraphKey, hipGraphExec_t> graphs;
+    uint64_t graphs_generation = 0;  // g_alloc_generation the cached graphs were captured under
     // predictor
     std::vector<float> predictor;
     float predictor_safety = 1.25f;
@@ -256,6 +263,8 @@ static void path_features(const SceneHost &s, const std::vector<float2> &pts, co
     f[10] = std::log(side);
     f[11] = (float)s.path_rule[p];
 }
+
+static void predict_scene(pfh_ctx *ctx, SceneHost &s);
 
 static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
     validate_desc(d);
@@ -366,6 +375,7 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
     for (int p = 0; p < d->n_paths; p++) path_features(*s, pts, segs, (uint32_t)p, &s->features[(size_t)p * N_FEATURES]);
 
     HIP_TRY(hipSetDevice(ctx->device));
+    upload(s->d_features, s->features);
     upload(s->d_pts, pts);
     upload(s->d_segs, segs);
     upload(s->d_seg_path, seg_path);
@@ -379,6 +389,7 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
     s->d_out_owned.ensure((size_t)s->out_pitch * s->height);
     s->d_out = s->d_out_owned.as<uint8_t>();
     s->live = true;
+    predict_scene(ctx, *s);
     ctx->scenes.push_back(std::move(s));
     return (int)ctx->scenes.size() - 1;
 }
@@ -390,46 +401,25 @@ static SceneHost &get_scene(pfh_ctx *ctx, int32_t id) {
 }
 
 // ---------------------------------------------------------------------------
-// Predictor (host evaluation for sizing; the device kernels train it)
+// Predictor: device inference at upload / set-predictor time, never inside a rasterize
 // ---------------------------------------------------------------------------
 
-static void predict_scene(const pfh_ctx *ctx, SceneHost &s) {
-    const std::vector<float> &w = ctx->predictor;
-    if (w.empty()) {
+static void predict_scene(pfh_ctx *ctx, SceneHost &s) {
+    if (ctx->predictor.empty()) {
         s.pred_fills = s.pred_alpha = s.pred_lines = -1;
         return;
     }
-    // layout: normalisation (mean[12], inv_std[12]), W1[12x64], b1[64], W2[64x64], b2[64], W3[64x3], b3[3]
-    const float *mean = w.data(), *istd = mean + 12, *W1 = istd + 12, *b1 = W1 + 12 * 64, *W2 = b1 + 64,
-                *b2 = W2 + 64 * 64, *W3 = b2 + 64, *b3 = W3 + 64 * 3;
-    double sl = 0, sf = 0, sa = 0;
-    float x[12], h1[64], h2[64];
-    for (uint32_t p = 0; p < s.n_paths; p++) {
-        for (int i = 0; i < 12; i++) x[i] = (s.features[p * 12 + i] - mean[i]) * istd[i];
-        for (int j = 0; j < 64; j++) {
-            float a = b1[j];
-            for (int i = 0; i < 12; i++) a += x[i] * W1[i * 64 + j];
-            h1[j] = a > 0 ? a : 0;
-        }
-        for (int j = 0; j < 64; j++) {
-            float a = b2[j];
-            for (int i = 0; i < 64; i++) a += h1[i] * W2[i * 64 + j];
-            h2[j] = a > 0 ? a : 0;
-        }
-        float o0 = b3[0], o1 = b3[1], o2 = b3[2];
-        for (int i = 0; i < 64; i++) {
-            o0 += h2[i] * W3[i * 3 + 0];
-            o1 += h2[i] * W3[i * 3 + 1];
-            o2 += h2[i] * W3[i * 3 + 2];
-        }
-        // targets are log1p(count): chords, fills, alpha tiles
-        sl += std::expm1(std::max(0.0f, o0));
-        sf += std::expm1(std::max(0.0f, o1));
-        sa += std::expm1(std::max(0.0f, o2));
+    float sums[3] = {0, 0, 0};
+    if (s.n_paths > 0) {
+        ctx->mlp_out.ensure(16);
+        HIP_TRY(mlp_predict_sum(ctx->mlp_params.as<float>(), s.d_features.as<float>(), s.n_paths,
+                                ctx->mlp_out.as<float>(), ctx->stream));
+        HIP_TRY(hipMemcpyAsync(sums, ctx->mlp_out.ptr, 12, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
-    s.pred_lines = (float)sl;
-    s.pred_fills = (float)sf;
-    s.pred_alpha = (float)sa;
+    s.pred_lines = sums[0];
+    s.pred_fills = sums[1];
+    s.pred_alpha = sums[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -472,6 +462,22 @@ static void ensure_sizes(pfh_ctx *ctx, const Plan &pl, bool dump) {
     ctx->vals_alt.ensure(4ull * pl.cap_e);
     ctx->entry_rec.ensure(16ull * pl.cap_e);
     ctx->sort_tmp.ensure(4 * radix_temp_elems(pl.cap_e) + 64);
+}
+
+// Capacities the current allocations hold, by the same per-buffer sizes ensure_sizes asks
+// for, so that ensure_sizes on these capacities never reallocates (captured graphs stay valid).
+struct Caps {
+    uint32_t l, f, a;
+};
+static Caps held_caps(const pfh_ctx *ctx, bool dump) {
+    auto fit = [](size_t bytes, size_t per, size_t extra) -> uint64_t {
+        return bytes > extra ? (bytes - extra) / per : 0;
+    };
+    uint64_t l = std::min({fit(ctx->lines.bytes, 16, 0), fit(ctx->line_seg.bytes, 4, 0), fit(ctx->line_cnt.bytes, 4, 8)});
+    uint64_t f = fit(ctx->fills.bytes, 8, 0);
+    if (dump) f = std::min(f, fit(ctx->dump_fills.bytes, 16, 0));
+    uint64_t a = std::min(fit(ctx->alpha_rec.bytes, 16, 0), fit(ctx->masks.bytes, 256, 0));
+    return Caps{clamp_cap(l), clamp_cap(f), clamp_cap(a)};
 }
 
 static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
@@ -648,6 +654,7 @@ static void stage_finish(pfh_ctx *ctx, const BatchDev &b, const Plan &pl) {
 }
 
 static void read_counters(pfh_ctx *ctx) {
+    ctx->stats.host_syncs++;
     HIP_TRY(hipMemcpyAsync(ctx->h_counters, ctx->counters.ptr, 4 * C_COUNT, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
 }
@@ -682,21 +689,23 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     Plan pl;
     pl.ids = {id};
     write_tables(ctx, pl, (flags & PFH_LOAD_DST) != 0);
-    pl.cap_f = clamp_cap(ctx->fills.bytes / 8);
-    pl.cap_a = clamp_cap(ctx->alpha_rec.bytes / 16);
-    pl.cap_l = clamp_cap(ctx->lines.bytes / 16);
+    Caps held = held_caps(ctx, dump);
+    pl.cap_f = held.f;
+    pl.cap_a = held.a;
+    pl.cap_l = held.l;
     pl.cap_e = clamp_cap(std::max<uint64_t>(pl.T, 1));
     ensure_sizes(ctx, pl, dump);
     upload_tables(ctx, pl);
     BatchDev b = batch_dev(ctx, pl, dump);
     hipStream_t st = ctx->stream;
-    uint32_t *tmp = ctx->scan_tmp.as<uint32_t>();
+    // scratch is re-read at every use: ensure_sizes may reallocate it when a count grows
+    auto tmp = [ctx]() { return ctx->scan_tmp.as<uint32_t>(); };
     hipEvent_t ev[9];
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[0], st));
     stage_clear(ctx, b);
     HIP_TRY(launch_flatten(b, false, st));
-    HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp, st));
+    HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp(), st));
     read_counters(ctx);  // chord count (renderer.cpp:707-724)
     uint32_t nl = ctx->h_counters[C_LINES];
     if (nl > pl.cap_l) {
@@ -717,16 +726,16 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
         ensure_sizes(ctx, pl, dump);
         b = batch_dev(ctx, pl, dump);
     }
-    if (dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, nl, b.counters, -1, tmp, st));
+    if (dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, nl, b.counters, -1, tmp(), st));
     HIP_TRY(launch_bin(b, true, nl, st));
     HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(launch_propagate(b, st));
     HIP_TRY(hipEventRecord(ev[4], st));
     HIP_TRY(launch_entries(b, false, st));
-    HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp, st));
+    HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp(), st));
     HIP_TRY(launch_entries(b, true, st));
     HIP_TRY(hipEventRecord(ev[5], st));
-    HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, tmp, st));
+    HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, tmp(), st));
     bool alt = false;
     HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
                        ctx->sort_tmp.as<uint32_t>(), &alt, st));
@@ -764,7 +773,6 @@ static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint3
     for (int32_t id : pl.ids) {
         SceneHost &s = get_scene(ctx, id);
         if ((flags & PFH_PREDICT) && !ctx->predictor.empty()) {
-            if (s.pred_fills < 0) predict_scene(ctx, s);
             plc += (uint64_t)std::ceil(s.pred_lines * ctx->predictor_safety) + 64;
             pf += (uint64_t)std::ceil(s.pred_fills * ctx->predictor_safety) + 64;
             pa += (uint64_t)std::ceil(s.pred_alpha * ctx->predictor_safety) + 16;
@@ -780,9 +788,10 @@ static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint3
     for (int attempt = 0; attempt < 4; attempt++) {
         if (!known) {
             // size from an eager count pass (read back), as the plain pipeline does
-            pl.cap_l = clamp_cap(ctx->lines.bytes / 16);
-            pl.cap_f = clamp_cap(ctx->fills.bytes / 8);
-            pl.cap_a = clamp_cap(ctx->alpha_rec.bytes / 16);
+            Caps held = held_caps(ctx, dump);
+            pl.cap_l = held.l;
+            pl.cap_f = held.f;
+            pl.cap_a = held.a;
             ensure_sizes(ctx, pl, dump);
             upload_tables(ctx, pl);
             BatchDev b = batch_dev(ctx, pl, dump);
@@ -801,19 +810,33 @@ static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint3
             pa = ctx->h_counters[C_ALPHA];
             known = true;
         }
-        pl.cap_l = clamp_cap(std::max<uint64_t>(plc, (uint64_t)(ctx->lines.bytes / 16)));
-        pl.cap_f = clamp_cap(std::max<uint64_t>(pf, (uint64_t)(ctx->fills.bytes / 8)));
-        pl.cap_a = clamp_cap(std::max<uint64_t>(pa, (uint64_t)(ctx->alpha_rec.bytes / 16)));
+        Caps held = held_caps(ctx, dump);
+        pl.cap_l = clamp_cap(std::max<uint64_t>(plc, held.l));
+        pl.cap_f = clamp_cap(std::max<uint64_t>(pf, held.f));
+        pl.cap_a = clamp_cap(std::max<uint64_t>(pa, held.a));
         ensure_sizes(ctx, pl, dump);
-        // ensure() rounds to 2^k: use the full allocation
-        pl.cap_l = clamp_cap(std::min<uint64_t>(ctx->lines.bytes / 16, ctx->line_seg.bytes / 4));
-        pl.cap_f = clamp_cap(ctx->fills.bytes / 8);
-        pl.cap_a = clamp_cap(std::min<uint64_t>(ctx->alpha_rec.bytes / 16, ctx->masks.bytes / 256));
+        // ensure() rounds to 2^k: use the full allocation, then size the buffers that
+        // depend on it (scan scratch) for exactly those capacities, so that a repeat call
+        // asks for nothing new and replays its graph
+        for (int fix = 0; fix < 4; fix++) {
+            held = held_caps(ctx, dump);
+            if (fix > 0 && held.l == pl.cap_l && held.f == pl.cap_f && held.a == pl.cap_a) break;
+            pl.cap_l = held.l;
+            pl.cap_f = held.f;
+            pl.cap_a = held.a;
+            ensure_sizes(ctx, pl, dump);
+        }
         upload_tables(ctx, pl);
         BatchDev b = batch_dev(ctx, pl, dump);
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
         if (use_graph) {
             GraphKey key{pl.ids, flags & ~(uint32_t)PFH_PREDICT, pl.cap_f, pl.cap_a, pl.cap_e, pl.cap_l};
+            if (ctx->graphs_generation != g_alloc_generation) {
+                // some buffer moved since these graphs were captured: never replay them
+                for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+                ctx->graphs.clear();
+                ctx->graphs_generation = g_alloc_generation;
+            }
             auto it = ctx->graphs.find(key);
             if (it == ctx->graphs.end()) {
                 if (ctx->graphs.size() > 32) {
@@ -918,7 +941,7 @@ void pfh_destroy(pfh_ctx *ctx) {
     for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto &s : ctx->scenes) {
         if (!s) continue;
-        for (DevBuf *b : {&s->d_pts, &s->d_segs, &s->d_seg_path, &s->d_path_rect, &s->d_path_tile_off,
+        for (DevBuf *b : {&s->d_features, &s->d_pts, &s->d_segs, &s->d_seg_path, &s->d_path_rect, &s->d_path_tile_off,
                           &s->d_path_col_off, &s->d_col_path, &s->d_path_rgba, &s->d_path_rule, &s->d_out_owned})
             b->release();
     }
@@ -952,7 +975,7 @@ int pfh_scene_release(pfh_ctx *ctx, int32_t scene_id) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
         ctx->graphs.clear();
-        for (DevBuf *b : {&s.d_pts, &s.d_segs, &s.d_seg_path, &s.d_path_rect, &s.d_path_tile_off, &s.d_path_col_off,
+        for (DevBuf *b : {&s.d_features, &s.d_pts, &s.d_segs, &s.d_seg_path, &s.d_path_rect, &s.d_path_tile_off, &s.d_path_col_off,
                           &s.d_col_path, &s.d_path_rgba, &s.d_path_rule, &s.d_out_owned})
             b->release();
         ctx->scenes[scene_id].reset();
@@ -973,6 +996,7 @@ int pfh_rasterize(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fl
         HIP_TRY(hipSetDevice(ctx->device));
         for (int32_t i = 0; i < n; i++) get_scene(ctx, scene_ids[i]);
         if (n == 0) return;
+        ctx->stats.host_syncs = 0;
         ctx->stats.paths = 0;
         for (int32_t i = 0; i < n; i++) ctx->stats.paths += get_scene(ctx, scene_ids[i]).n_paths;
         if (flags & PFH_BATCHED) {
@@ -1139,46 +1163,195 @@ int pfh_stats_get(pfh_ctx *ctx, pfh_stats *out) {
 int pfh_set_predictor(pfh_ctx *ctx, const float *weights, int64_t n_floats, float safety) {
     if (!ctx) return PFH_E_ARG;
     return run_guarded(ctx, [&] {
-        int64_t need = 24 + mlp_param_count();
+        const int64_t np = mlp_param_count();
+        if (!(safety >= 1.0f)) throw PfhError{PFH_E_ARG, "safety factor must be >= 1"};
         if (weights == nullptr) {
             ctx->predictor.clear();
         } else {
-            if (n_floats != need) throw PfhError{PFH_E_ARG, "predictor needs " + std::to_string(need) + " floats"};
+            if (n_floats != 24 + np) throw PfhError{PFH_E_ARG, "predictor needs " + std::to_string(24 + np) + " floats"};
             ctx->predictor.assign(weights, weights + n_floats);
+            // fold the feature normalisation (x - mean) * inv_std into the first layer
+            std::vector<float> p(weights + 24, weights + 24 + np);
+            const float *mean = weights, *istd = weights + 12;
+            float *W1 = p.data(), *b1 = p.data() + 12 * 64;
+            for (int j = 0; j < 64; j++) {
+                double shift = 0;
+                for (int i = 0; i < 12; i++) {
+                    shift += (double)mean[i] * istd[i] * W1[i * 64 + j];
+                    W1[i * 64 + j] *= istd[i];
+                }
+                b1[j] = (float)(b1[j] - shift);
+            }
+            ctx->mlp_params.ensure(4 * np);
+            HIP_TRY(hipMemcpy(ctx->mlp_params.ptr, p.data(), 4 * np, hipMemcpyHostToDevice));
         }
-        if (!(safety >= 1.0f)) throw PfhError{PFH_E_ARG, "safety factor must be >= 1"};
         ctx->predictor_safety = safety;
         for (auto &s : ctx->scenes)
-            if (s) s->pred_fills = s->pred_alpha = s->pred_lines = -1;
+            if (s && s->live) predict_scene(ctx, *s);
     });
 }
 
-int pfh_mlp_forward(pfh_ctx *ctx, const float *d_params, const float *d_x, int64_t n, float *d_hidden, float *d_out) {
-    if (!ctx || !d_params || !d_x || !d_hidden || !d_out || n < 0) return PFH_E_ARG;
+/* Predicted (chords, fills, alpha tiles) of an uploaded scene, or -1 without a predictor. */
+int pfh_scene_prediction(pfh_ctx *ctx, int32_t scene_id, float *out3) {
+    if (!ctx || !out3) return PFH_E_ARG;
     return run_guarded(ctx, [&] {
-        HIP_TRY(mlp_forward(d_params, d_x, n, d_hidden, d_out, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        SceneHost &s = get_scene(ctx, scene_id);
+        out3[0] = s.pred_lines;
+        out3[1] = s.pred_fills;
+        out3[2] = s.pred_alpha;
     });
 }
 
-int pfh_mlp_backward(pfh_ctx *ctx, const float *d_params, const float *d_x, int64_t n, const float *d_hidden,
-                     const float *d_out, const float *d_target, float *d_grad, float *d_loss) {
-    if (!ctx || !d_params || !d_x || !d_hidden || !d_out || !d_target || !d_grad || !d_loss || n < 0) return PFH_E_ARG;
+int pfh_path_counts(pfh_ctx *ctx, int32_t scene_id, int32_t *lines, int32_t *fills, int32_t *alpha_tiles) {
+    if (!ctx || !lines || !fills || !alpha_tiles) return PFH_E_ARG;
     return run_guarded(ctx, [&] {
-        ctx->mlp_x.ensure(sizeof(float) * (size_t)std::max<int64_t>(n, 1) * 131);
-        HIP_TRY(mlp_backward(d_params, d_x, n, d_hidden, d_out, d_target, d_grad, d_loss, ctx->mlp_x.as<float>(),
-                             ctx->stream));
+        SceneHost &s = get_scene(ctx, scene_id);
+        if (!s.dumped) throw PfhError{PFH_E_STATE, "scene was not rasterized with PFH_DUMP"};
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<uint32_t> seg_line(s.n_segs + 1);
+        HIP_TRY(hipMemcpy(seg_line.data(), ctx->seg_cnt.as<uint32_t>() + s.seg_base, 4 * (s.n_segs + 1),
+                          hipMemcpyDeviceToHost));
+        uint32_t l0 = seg_line[0], l1 = seg_line[s.n_segs];
+        std::vector<uint32_t> line_fill(l1 - l0 + 1);
+        HIP_TRY(hipMemcpy(line_fill.data(), ctx->line_cnt.as<uint32_t>() + l0, 4 * (size_t)(l1 - l0 + 1),
+                          hipMemcpyDeviceToHost));
+        std::vector<uint32_t> fill_off(s.n_tiles + 1);
+        HIP_TRY(hipMemcpy(fill_off.data(), ctx->tile_fill_off.as<uint32_t>() + s.tile_base, 4 * (s.n_tiles + 1),
+                          hipMemcpyDeviceToHost));
+        for (uint32_t p = 0; p < s.n_paths; p++) {
+            uint32_t a = seg_line[s.path_seg_off[p]] - l0, e = seg_line[s.path_seg_off[p + 1]] - l0;
+            lines[p] = (int32_t)(e - a);
+            fills[p] = (int32_t)(line_fill[e] - line_fill[a]);
+            int32_t na = 0;
+            for (uint32_t t = s.path_tile_off[p]; t < s.path_tile_off[p + 1]; t++) na += fill_off[t + 1] > fill_off[t];
+            alpha_tiles[p] = na;
+        }
     });
 }
 
-int pfh_mlp_sgd(pfh_ctx *ctx, float *d_params, float *d_m, float *d_v, const float *d_grad, int64_t n_params,
-                float lr, int32_t step) {
-    if (!ctx || !d_params || !d_m || !d_v || !d_grad) return PFH_E_ARG;
+namespace {
+struct MlpDevEval {
+    DevBuf params, x, h, out;
+};
+}  // namespace
+
+int pfh_mlp_eval(pfh_ctx *ctx, const float *params, const float *x, int64_t n, float *out) {
+    if (!ctx || !params || (n > 0 && (!x || !out)) || n < 0) return PFH_E_ARG;
     return run_guarded(ctx, [&] {
-        if (n_params != mlp_param_count()) throw PfhError{PFH_E_ARG, "wrong parameter count"};
-        HIP_TRY(mlp_adam(d_params, d_m, d_v, d_grad, n_params, lr, step, ctx->stream));
+        const int64_t np = mlp_param_count();
+        if (n == 0) return;
+        MlpDevEval d;
+        d.params.ensure(4 * np);
+        d.x.ensure(4 * n * 12);
+        d.h.ensure(4 * n * 128);
+        d.out.ensure(4 * n * 3);
+        HIP_TRY(hipMemcpy(d.params.ptr, params, 4 * np, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.x.ptr, x, 4 * n * 12, hipMemcpyHostToDevice));
+        HIP_TRY(mlp_forward(d.params.as<float>(), d.x.as<float>(), n, d.h.as<float>(), d.out.as<float>(),
+                            ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(out, d.out.ptr, 4 * n * 3, hipMemcpyDeviceToHost));
+        for (DevBuf *bf : {&d.params, &d.x, &d.h, &d.out}) bf->release();
+    });
+}
+
+namespace {
+struct MlpDev {
+    DevBuf params, m, v, grad, loss, x, t, h, out, scratch;
+};
+static void mlp_grad_dev(pfh_ctx *ctx, MlpDev &d, const float *x, const float *t, int64_t n) {
+    HIP_TRY(mlp_forward(d.params.as<float>(), x, n, d.h.as<float>(), d.out.as<float>(), ctx->stream));
+    HIP_TRY(mlp_backward(d.params.as<float>(), x, n, d.h.as<float>(), d.out.as<float>(), t, d.grad.as<float>(),
+                         d.loss.as<float>(), d.scratch.as<float>(), ctx->stream));
+}
+}  // namespace
+
+int pfh_mlp_grad(pfh_ctx *ctx, const float *params, const float *x, const float *target, int64_t n, float *grad,
+                 float *loss) {
+    if (!ctx || !params || !x || !target || !grad || !loss || n <= 0) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        const int64_t np = mlp_param_count();
+        MlpDev d;
+        d.params.ensure(4 * np);
+        d.grad.ensure(4 * np);
+        d.loss.ensure(4);
+        d.x.ensure(4 * n * 12);
+        d.t.ensure(4 * n * 3);
+        d.h.ensure(4 * n * 128);
+        d.out.ensure(4 * n * 3);
+        d.scratch.ensure(4 * n * 131);
+        HIP_TRY(hipMemcpy(d.params.ptr, params, 4 * np, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.x.ptr, x, 4 * n * 12, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.t.ptr, target, 4 * n * 3, hipMemcpyHostToDevice));
+        mlp_grad_dev(ctx, d, d.x.as<float>(), d.t.as<float>(), n);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(grad, d.grad.ptr, 4 * np, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(loss, d.loss.ptr, 4, hipMemcpyDeviceToHost));
+        for (DevBuf *bf : {&d.params, &d.grad, &d.loss, &d.x, &d.t, &d.h, &d.out, &d.scratch}) bf->release();
+    });
+}
+
+int pfh_mlp_train(pfh_ctx *ctx, const float *x, const float *target, int64_t n, int32_t epochs, int32_t batch,
+                  float lr, uint64_t seed, float *params, float *loss_history) {
+    if (!ctx || !x || !target || !params || n <= 0 || epochs < 0 || batch <= 0) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        const int64_t np = mlp_param_count();
+        const int64_t bs = std::min<int64_t>(batch, n);
+        MlpDev d;
+        d.params.ensure(4 * np);
+        d.m.ensure(4 * np);
+        d.v.ensure(4 * np);
+        d.grad.ensure(4 * np);
+        d.loss.ensure(4);
+        d.x.ensure(4 * n * 12);
+        d.t.ensure(4 * n * 3);
+        d.h.ensure(4 * n * 128);
+        d.out.ensure(4 * n * 3);
+        d.scratch.ensure(4 * n * 131);
+        HIP_TRY(hipMemcpy(d.params.ptr, params, 4 * np, hipMemcpyHostToDevice));
+        std::vector<float> zeros(np, 0.0f);
+        HIP_TRY(hipMemcpy(d.m.ptr, zeros.data(), 4 * np, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.v.ptr, zeros.data(), 4 * np, hipMemcpyHostToDevice));
+        // shuffled copies of the data per epoch (host permutation, one upload per epoch)
+        std::vector<float> xs((size_t)n * 12), ts((size_t)n * 3);
+        std::vector<int64_t> perm(n);
+        for (int64_t i = 0; i < n; i++) perm[i] = i;
+        uint64_t st = seed ? seed : 0x9E3779B97F4A7C15ull;
+        auto rnd = [&]() {  // splitmix64
+            st += 0x9E3779B97F4A7C15ull;
+            uint64_t z = st;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        int step = 0;
+        for (int32_t ep = 0; ep < epochs; ep++) {
+            for (int64_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[rnd() % (uint64_t)(i + 1)]);
+            for (int64_t i = 0; i < n; i++) {
+                std::memcpy(&xs[(size_t)i * 12], x + perm[i] * 12, 48);
+                std::memcpy(&ts[(size_t)i * 3], target + perm[i] * 3, 12);
+            }
+            HIP_TRY(hipMemcpy(d.x.ptr, xs.data(), 4 * n * 12, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(d.t.ptr, ts.data(), 4 * n * 3, hipMemcpyHostToDevice));
+            for (int64_t b0 = 0; b0 + bs <= n; b0 += bs) {
+                mlp_grad_dev(ctx, d, d.x.as<float>() + b0 * 12, d.t.as<float>() + b0 * 3, bs);
+                HIP_TRY(mlp_adam(d.params.as<float>(), d.m.as<float>(), d.v.as<float>(), d.grad.as<float>(), np, lr,
+                                 ++step, ctx->stream));
+            }
+            if (loss_history) {
+                HIP_TRY(mlp_forward(d.params.as<float>(), d.x.as<float>(), n, d.h.as<float>(), d.out.as<float>(),
+                                    ctx->stream));
+                HIP_TRY(mlp_backward(d.params.as<float>(), d.x.as<float>(), n, d.h.as<float>(), d.out.as<float>(),
+                                     d.t.as<float>(), d.grad.as<float>(), d.loss.as<float>(), d.scratch.as<float>(),
+                                     ctx->stream));
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                HIP_TRY(hipMemcpy(&loss_history[ep], d.loss.ptr, 4, hipMemcpyDeviceToHost));
+            }
+        }
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(params, d.params.ptr, 4 * np, hipMemcpyDeviceToHost));
+        for (DevBuf *bf : {&d.params, &d.m, &d.v, &d.grad, &d.loss, &d.x, &d.t, &d.h, &d.out, &d.scratch})
+            bf->release();
     });
 }
 

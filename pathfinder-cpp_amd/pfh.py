@@ -43,7 +43,7 @@ class Stats(ctypes.Structure):
                 ("guard_hits", ctypes.c_int64), ("ms_total", ctypes.c_double), ("ms_stage", ctypes.c_double * 8),
                 ("capacity_fills", ctypes.c_int64), ("capacity_alpha", ctypes.c_int64),
                 ("capacity_entries", ctypes.c_int64), ("alpha_entries", ctypes.c_int64), ("lines", ctypes.c_int64),
-                ("capacity_lines", ctypes.c_int64)]
+                ("capacity_lines", ctypes.c_int64), ("host_syncs", ctypes.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "ms_stage"}
@@ -53,8 +53,8 @@ class Stats(ctypes.Structure):
 
 EXPORTS = ("pfh_create", "pfh_destroy", "pfh_last_error", "pfh_scene_upload", "pfh_scene_release",
            "pfh_scene_features", "pfh_rasterize", "pfh_scene_set_target", "pfh_scene_target", "pfh_readback_rgba8",
-           "pfh_dump_counts", "pfh_dump", "pfh_stats_get", "pfh_set_predictor", "pfh_mlp_forward",
-           "pfh_mlp_backward", "pfh_mlp_sgd")
+           "pfh_dump_counts", "pfh_dump", "pfh_stats_get", "pfh_set_predictor", "pfh_scene_prediction", "pfh_path_counts",
+           "pfh_mlp_eval", "pfh_mlp_grad", "pfh_mlp_train")
 
 _lib = None
 
@@ -86,9 +86,11 @@ def load(path: str = LIB_PATH):
         "pfh_dump": (ctypes.c_int, [_P, i32] + [_P] * 14),
         "pfh_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
         "pfh_set_predictor": (ctypes.c_int, [_P, _P, i64, f32]),
-        "pfh_mlp_forward": (ctypes.c_int, [_P, _P, _P, i64, _P, _P]),
-        "pfh_mlp_backward": (ctypes.c_int, [_P, _P, _P, i64, _P, _P, _P, _P, _P]),
-        "pfh_mlp_sgd": (ctypes.c_int, [_P, _P, _P, _P, _P, i64, f32, i32]),
+        "pfh_scene_prediction": (ctypes.c_int, [_P, i32, _P]),
+        "pfh_path_counts": (ctypes.c_int, [_P, i32, _P, _P, _P]),
+        "pfh_mlp_eval": (ctypes.c_int, [_P, _P, _P, i64, _P]),
+        "pfh_mlp_grad": (ctypes.c_int, [_P, _P, _P, _P, i64, _P, _P]),
+        "pfh_mlp_train": (ctypes.c_int, [_P, _P, _P, i64, i32, i32, f32, ctypes.c_uint64, _P, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -208,15 +210,42 @@ class Context:
         w = np.ascontiguousarray(weights, np.float32)
         self._check(self.lib.pfh_set_predictor(self.h, _ptr(w), int(w.shape[0]), safety), "pfh_set_predictor")
 
-    # MLP kernels on device pointers (torch tensors or raw addresses)
-    def mlp_forward(self, params, x, n, hidden, out):
-        self._check(self.lib.pfh_mlp_forward(self.h, _P(params), _P(x), int(n), _P(hidden), _P(out)),
-                    "pfh_mlp_forward")
+    def prediction(self, sid: int) -> np.ndarray:
+        """Predicted (chords, fills, alpha tiles) of an uploaded scene (-1 without a predictor)."""
+        out = np.zeros(3, np.float32)
+        self._check(self.lib.pfh_scene_prediction(self.h, sid, _ptr(out)), "pfh_scene_prediction")
+        return out
 
-    def mlp_backward(self, params, x, n, hidden, out, target, grad, loss):
-        self._check(self.lib.pfh_mlp_backward(self.h, _P(params), _P(x), int(n), _P(hidden), _P(out), _P(target),
-                                              _P(grad), _P(loss)), "pfh_mlp_backward")
+    def path_counts(self, sid: int):
+        """Exact per-path (chords, fills, alpha tiles) of the last PFH_DUMP rasterize."""
+        n = self._keep[sid][2]
+        out = [np.zeros(n, np.int32) for _ in range(3)]
+        self._check(self.lib.pfh_path_counts(self.h, sid, *[_ptr(a) for a in out]), "pfh_path_counts")
+        return np.stack(out, axis=1)
 
-    def mlp_adam(self, params, m, v, grad, n_params, lr, step):
-        self._check(self.lib.pfh_mlp_sgd(self.h, _P(params), _P(m), _P(v), _P(grad), int(n_params), float(lr),
-                                         int(step)), "pfh_mlp_sgd")
+    # the sizing MLP's HIP kernels on host arrays
+    def mlp_eval(self, params, x):
+        params = np.ascontiguousarray(params, np.float32)
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, N_FEATURES)
+        out = np.zeros((x.shape[0], MLP_OUTPUTS), np.float32)
+        self._check(self.lib.pfh_mlp_eval(self.h, _ptr(params), _ptr(x), x.shape[0], _ptr(out)), "pfh_mlp_eval")
+        return out
+
+    def mlp_grad(self, params, x, target):
+        params = np.ascontiguousarray(params, np.float32)
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, N_FEATURES)
+        target = np.ascontiguousarray(target, np.float32).reshape(-1, MLP_OUTPUTS)
+        grad = np.zeros(MLP_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self.lib.pfh_mlp_grad(self.h, _ptr(params), _ptr(x), _ptr(target), x.shape[0], _ptr(grad),
+                                          _ptr(loss)), "pfh_mlp_grad")
+        return grad, float(loss[0])
+
+    def mlp_train(self, params, x, target, epochs=10, batch=4096, lr=1e-3, seed=1):
+        params = np.ascontiguousarray(params, np.float32).copy()
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, N_FEATURES)
+        target = np.ascontiguousarray(target, np.float32).reshape(-1, MLP_OUTPUTS)
+        hist = np.zeros(max(epochs, 1), np.float32)
+        self._check(self.lib.pfh_mlp_train(self.h, _ptr(x), _ptr(target), x.shape[0], int(epochs), int(batch),
+                                           float(lr), int(seed), _ptr(params), _ptr(hist)), "pfh_mlp_train")
+        return params, hist[:epochs]

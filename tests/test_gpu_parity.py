@@ -118,3 +118,60 @@ def test_undersized_prediction_retries_without_wrong_pixels(ctx):
     for sid, img in zip(ids, ref):
         np.testing.assert_array_equal(fresh.readback(sid), img)
     fresh.close()
+
+
+def test_plain_fresh_context_more_chords_than_tiles():
+    # Small canvas, many small paths: the chord count exceeds the tile count, so the
+    # plain pipeline grows its chord buffer (and the scan scratch sized by it) after
+    # the first read-back.  Every later scan must use the regrown scratch.
+    scene = scenegen.blob_scene(77, 384, 384, 900, r_min=4.0, r_max=40.0)
+    o = oracle.run_oracle(scene)
+    assert int(o["path_lines"].sum()) > 2 * int(o["tile_off"][-1])
+    fresh = pfh.Context(0)
+    check_scene(fresh, scene, name="chords>tiles fresh")
+    fresh.close()
+
+
+def test_graph_replay_after_buffers_regrow():
+    # A captured graph holds raw buffer addresses; a larger rasterize in between
+    # regrows the buffers, and the replay of the first scene set must not use the old ones.
+    fresh = pfh.Context(0)
+    small = scenegen.batch_scenes(n_scenes=2, seed=9, side_max=384, paths_max=120)
+    big = scenegen.blob_scene(10, 2048, 2048, 3000, r_max=512.0)
+    ids = [fresh.upload(s) for s in small]
+    fresh.rasterize(ids, pfh.PFH_BATCHED)
+    first = [fresh.readback(i) for i in ids]
+    bid = fresh.upload(big)
+    fresh.rasterize([bid], pfh.PFH_BATCHED)
+    big_img = fresh.readback(bid)
+    for _ in range(2):
+        fresh.rasterize(ids, pfh.PFH_BATCHED)
+        for i, img in zip(ids, first):
+            np.testing.assert_array_equal(fresh.readback(i), img)
+    fresh.rasterize([bid], pfh.PFH_PLAIN)
+    np.testing.assert_array_equal(fresh.readback(bid), big_img)
+    fresh.close()
+
+
+def test_predicted_capacities_above_allocation_replay_graph():
+    # Plain first (buffers sized to the exact counts, rounded to 2^k), then a predictor
+    # whose totals exceed those allocations: the first batched call regrows the buffers
+    # and captures a graph; repeat calls must ask for nothing new and replay it.
+    scene = scenegen.blob_scene(31, 768, 768, 600, r_max=300.0)
+    fresh = pfh.Context(0)
+    sid = fresh.upload(scene)
+    fresh.rasterize([sid], pfh.PFH_PLAIN)
+    want = fresh.readback(sid)
+    st = fresh.stats()
+    totals = np.array([st["lines"], st["fills"], st["alpha_tiles"]], np.float64)
+    w = np.zeros(pfh.PREDICTOR_FLOATS, np.float32)
+    w[12:24] = 1.0
+    w[-3:] = np.log1p(2.5 * totals / scene.n_paths)  # output bias only: 2.5x the exact totals
+    fresh.set_predictor(w, safety=1.0)
+    for _ in range(4):
+        fresh.rasterize([sid], pfh.PFH_BATCHED | pfh.PFH_PREDICT)
+        st = fresh.stats()
+        assert st["retries"] == 0 and st["host_syncs"] == 1
+        assert st["capacity_lines"] >= 2 * totals[0]
+        np.testing.assert_array_equal(fresh.readback(sid), want)
+    fresh.close()
