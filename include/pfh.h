@@ -74,7 +74,8 @@ typedef struct pfh_stats {
     int64_t retries;          /* capacity overflows that forced a re-run */
     int64_t guard_hits;       /* loop guards tripped (inputs on which the reference would not terminate) */
     double ms_total;          /* device time of the last pfh_rasterize (HIP events) */
-    double ms_stage[8];       /* plain mode: flatten, bin count, bin emit, propagate, entries, sort, fill, composite */
+    double ms_stage[8];       /* plain mode: flatten, bin count, bin emit, propagate, entries, sort, fill, composite;
+                                 batched mode: [6] fill and [7] composite kernels of the last launch, others 0 */
     int64_t capacity_fills, capacity_alpha, capacity_entries;
     int64_t alpha_entries;    /* draw entries that sample a coverage mask */
     int64_t lines, capacity_lines; /* flattened chords and the chord buffer's capacity */
@@ -155,6 +156,37 @@ int pfh_mlp_grad(pfh_ctx *ctx, const float *params, const float *x, const float 
                  float *loss);
 int pfh_mlp_train(pfh_ctx *ctx, const float *x, const float *target, int64_t n, int32_t epochs, int32_t batch,
                   float lr, uint64_t seed, float *params, float *loss_history);
+
+/*
+ * Request loop (continuous batching).  No counterpart in the reference, which draws one scene per
+ * Canvas::draw (core/canvas.cpp:557-567) with blocking submits per stage (gpu/vk/queue.cpp:29-34).
+ * Requests name uploaded (resident) scenes.  Each of `slots` workspaces (own HIP stream, own work
+ * buffers) takes the queued requests whose predicted device bytes fit `memory_budget_bytes` beside
+ * the launches in flight (at most max_batch per launch, arrival order), launches them as one batched
+ * pass and returns at once; requests arriving meanwhile join the next launch of whichever slot frees
+ * first, so launches overlap on the device.  pfh_server_submit only queues; the loop advances (retires
+ * finished launches, admits queued requests) in pfh_server_poll, which the caller calls in its loop.  A scene is in at most one launch at a time; a request
+ * larger than the budget runs alone once nothing else is in flight.  flags: PFH_PREDICT, PFH_NO_GRAPH.
+ * Latency = submit to the poll that observes completion.  Results are the scene's target, as after
+ * pfh_rasterize.  pfh_scene_release refuses a scene with queued or running requests.
+ */
+typedef struct pfh_server_stats {
+    int64_t submitted, completed, queued, in_flight;
+    int64_t launches, retries;
+    int64_t max_concurrent_launches;  /* slots seen busy at once */
+    double mean_batch;                /* requests per launch */
+    double latency_ms_p50, latency_ms_p99, latency_ms_mean, latency_ms_max;
+    int64_t reserved_bytes, peak_reserved_bytes, budget_bytes;
+    int64_t device_bytes;             /* bytes the slots' work buffers hold */
+} pfh_server_stats;
+
+int pfh_server_start(pfh_ctx *ctx, int32_t slots, int64_t memory_budget_bytes, int32_t max_batch, uint32_t flags);
+int pfh_server_submit(pfh_ctx *ctx, int32_t scene_id, int64_t *request_id);
+/* Advances the loop; waits up to wait_ms for a completion; returns up to `capacity` completed ids. */
+int pfh_server_poll(pfh_ctx *ctx, double wait_ms, int64_t *done_ids, int32_t capacity, int32_t *n_done);
+int pfh_server_stats_get(pfh_ctx *ctx, pfh_server_stats *out);
+/* Runs every queued request to completion, then frees the slots. */
+int pfh_server_stop(pfh_ctx *ctx);
 
 #ifdef __cplusplus
 }

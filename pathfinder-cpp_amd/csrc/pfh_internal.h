@@ -15,6 +15,15 @@ constexpr int MAX_FLATTEN_DEPTH = 16;
 constexpr int FIX_SHIFT = 20;            // fixed-point coverage accumulation: 2^-20 px^2 units
 constexpr uint32_t NO_ALPHA = 0xffffffffu;
 
+// Binning ownership classes (a path's tiles and columns belong to it alone, so the wave or
+// workgroup that bins a path can count in LDS and store plainly).  A path goes to the wave
+// class when its tile rect and column count fit one wave's LDS slice, else to the group class,
+// which uses LDS up to BIN_GROUP_TILES / BIN_GROUP_COLS and global atomics beyond.
+constexpr int BIN_WAVE_TILES = 512;
+constexpr int BIN_WAVE_COLS = 256;
+constexpr int BIN_GROUP_TILES = 4608;
+constexpr int BIN_GROUP_COLS = 1024;
+
 enum SegKind : uint32_t { SEG_LINE = 1, SEG_QUAD = 2, SEG_CUBIC = 3 };
 
 // Counters written by the device pipeline (one block per batch).
@@ -40,11 +49,15 @@ struct SceneDev {
     const uint32_t *path_col_off;   // [n_paths + 1] local column offsets
     const uint32_t *col_path;     // local column -> local path
     const uint32_t *path_rgba;    // r | g << 8 | b << 16 | a << 24
+    const uint32_t *path_seg_off; // [n_paths + 1] local segment offsets
+    const uint32_t *bin_paths;    // local path indices: wave class, then group class
     const uint8_t *path_rule;     // 0 nonzero, 1 even-odd
     float vb_l, vb_t, vb_r, vb_b; // view box (right edge rounded up to 16)
     int32_t fb_x0, fb_y0, fb_w, fb_h;
     int32_t width, height;
     uint32_t n_paths, n_segs, n_tiles, n_cols, n_fb;
+    uint32_t n_bin_wave, n_bin_group;  // paths in each binning class
+    uint32_t bin_wave_base, bin_group_base;  // first item of this scene in the batch's class lists
     uint32_t seg_base, path_base, tile_base, col_base, fb_base;
     uint8_t *out;                 // RGBA8 target
     int64_t out_pitch;
@@ -61,10 +74,13 @@ struct BatchDev {
     const uint32_t *tile_bases;
     const uint32_t *col_bases;
     const uint32_t *fb_bases;
+    const uint32_t *bin_wave_bases;   // [n_scenes + 1] wave-class items per scene, scanned
+    const uint32_t *bin_group_bases;  // [n_scenes + 1] group-class items per scene, scanned
     uint32_t n_segs, n_tiles, n_cols, n_fb;
+    uint32_t n_bin_wave, n_bin_group;
     uint32_t cap_fills, cap_alpha, cap_entries, cap_lines;
     int32_t *tile_delta;          // [T] backdrop delta (int8 arithmetic, stored widened)
-    uint32_t *tile_cnt;           // [T] fills per tile, then emit cursor
+    uint32_t *tile_cnt;           // [T] fills per tile (global-atomic paths: then emit cursor)
     uint32_t *tile_fill_off;      // [T + 1] exclusive scan of fill counts
     uint32_t *tile_alpha_idx;     // [T + 1] exclusive scan of alpha flags
     int8_t *tile_bd;              // [T] propagated backdrop

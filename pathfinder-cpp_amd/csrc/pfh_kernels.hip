@@ -3,9 +3,10 @@
 // Stage map (reference GLSL compute stage -> kernel here):
 //   dice.comp                   -> k_flatten<COUNT/EMIT>: the hybrid tiler's recursive
 //                                  flatten (core/d3d9/tiler.cpp:284-315), chords in DFS order
-//   bin.comp                    -> k_bin<COUNT/EMIT>: its tile walk per chord
-//                                  (tiler.cpp:131-279), exact float order, + k_group_fills
-//   bound.comp                  -> tile arrays cleared by k_fill_u32 (pure write stream)
+//   bin.comp                    -> k_bin_wave / k_bin_group <COUNT/EMIT>: the tile walk of
+//                                  each chord (tiler.cpp:131-279), exact float order, per path
+//                                  with LDS counters; also writes every tile of the path's rect
+//   bound.comp                  -> (the bin count pass initialises each path's tiles)
 //   propagate.comp              -> k_propagate (prepare_tiles, tiler.cpp:369-439) + z-buffer
 //   sort.comp                   -> k_entries_* + stable LSD radix sort by framebuffer tile
 //   fill.comp / fill.frag       -> k_fill_masks: analytic box-filtered area, fixed-point
@@ -74,7 +75,11 @@ __device__ __forceinline__ bool flat_quad(const Seg4 &s) {
 // Tile walk of one chord (hybrid semantics)
 // ---------------------------------------------------------------------------
 
-template <bool EMIT>
+// The walk's side effects go either to global counters (atomics) or, when one wave or
+// workgroup owns the whole path, to LDS arrays indexed by the tile's position in the path's
+// rect (LDS = true): l_cnt holds fill counts (COUNT) or the tiles' next free fill slots
+// (EMIT), l_delta the backdrop deltas and l_col the above-rect column backdrops.
+template <bool EMIT, bool LDS = false>
 struct Walker {
     // scene / path state
     float vb_l, vb_r, vb_b;
@@ -82,6 +87,8 @@ struct Walker {
     uint32_t tbase, cbase;
     uint32_t *tile_cnt;
     int32_t *tile_delta, *col_bd;
+    uint32_t *l_cnt;
+    int32_t *l_delta, *l_col;
     uint4 *out;              // EMIT with dumps: fills in emission order
     uint2 *fills;            // EMIT: fills grouped by tile
     const uint32_t *tile_fill_off;
@@ -90,6 +97,11 @@ struct Walker {
     uint32_t count;          // fills emitted by this chord so far
     uint32_t out_base;       // first emission slot of this chord (EMIT)
     uint32_t guard;
+
+    __device__ __forceinline__ void count_add(uint32_t j, uint32_t n) {
+        if (LDS) atomicAdd(&l_cnt[j], n);
+        else atomicAdd(&tile_cnt[tbase + j], n);
+    }
 
     // core/d3d9/object_builder.cpp:19-64 add_fill
     __device__ __forceinline__ void add_fill(float2 from, float2 to, int tx, int ty) {
@@ -102,12 +114,12 @@ struct Walker {
         uint32_t q2 = (uint32_t)rintf(sse_min(sse_max(v2, 0.0f), 4095.0f));
         uint32_t q3 = (uint32_t)rintf(sse_min(sse_max(v3, 0.0f), 4095.0f));
         if (q0 == q2) return;
-        uint32_t t = tbase + (uint32_t)((ty - tt) * tw + (tx - tl));
+        uint32_t j = (uint32_t)((ty - tt) * tw + (tx - tl));  // tile within the path's rect
         if (!EMIT) {
             // consecutive fills of a chord mostly share a tile: count them locally
-            if (t != pend_tile) {
-                if (pend_n) atomicAdd(&tile_cnt[pend_tile], pend_n);
-                pend_tile = t;
+            if (j != pend_tile) {
+                if (pend_n) count_add(pend_tile, pend_n);
+                pend_tile = j;
                 pend_n = 0;
             }
             pend_n++;
@@ -115,19 +127,24 @@ struct Walker {
             // straight into the tile's range; order inside a tile is free (coverage is
             // accumulated in fixed point)
             const uint32_t packed_x = q0 | (q1 << 16), packed_y = q2 | (q3 << 16);
-            uint32_t slot = atomicSub(&tile_cnt[t], 1u) - 1u;
-            uint32_t pos = tile_fill_off[t] + slot;
+            uint32_t pos;
+            if (LDS) {
+                pos = atomicAdd(&l_cnt[j], 1u);
+            } else {
+                uint32_t slot = atomicSub(&tile_cnt[tbase + j], 1u) - 1u;
+                pos = tile_fill_off[tbase + j] + slot;
+            }
             if (pos < cap_fills) fills[pos] = make_uint2(packed_x, packed_y);
             if (out) {
                 uint32_t d = out_base + count;
-                if (d < cap_fills) out[d] = make_uint4(packed_x, packed_y, t, 0u);
+                if (d < cap_fills) out[d] = make_uint4(packed_x, packed_y, tbase + j, 0u);
             }
         }
         count++;
     }
 
     __device__ __forceinline__ void flush() {
-        if (!EMIT && pend_n) atomicAdd(&tile_cnt[pend_tile], pend_n);
+        if (!EMIT && pend_n) count_add(pend_tile, pend_n);
         pend_n = 0;
     }
 
@@ -137,10 +154,12 @@ struct Walker {
         int ox = tx - tl, oy = ty - tt;
         if (ox < 0 || ox >= tw || oy >= th) return;
         if (oy < 0) {
-            atomicAdd(&col_bd[cbase + ox], delta);
+            if (LDS) atomicAdd(&l_col[ox], delta);
+            else atomicAdd(&col_bd[cbase + ox], delta);
             return;
         }
-        atomicAdd(&tile_delta[tbase + (uint32_t)(oy * tw + ox)], delta);
+        if (LDS) atomicAdd(&l_delta[oy * tw + ox], delta);
+        else atomicAdd(&tile_delta[tbase + (uint32_t)(oy * tw + ox)], delta);
     }
 
     __device__ __forceinline__ static uint32_t outcode(float2 p, float l, float t, float r, float bb) {
@@ -340,7 +359,169 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b) {
 }
 
 // ---------------------------------------------------------------------------
-// Bin: one thread per chord, fills out in tile-walk order
+// Bin: per path, one wave (small rects) or one workgroup (large rects)
+// ---------------------------------------------------------------------------
+//
+// The chords of a path are contiguous (flatten emits in segment order, segments are in
+// path order), and every tile and column of the path's rect belongs to it alone.  So the
+// owner counts fills and accumulates backdrop deltas in LDS with LDS atomics and stores
+// the whole rect once (COUNT), then hands out fill slots from LDS cursors seeded with the
+// tiles' scanned offsets (EMIT).  No global atomics, and the work arrays need no clearing.
+
+// Chord range [c0, c1) of local path p of scene sc (seg_cnt holds the scanned offsets).
+__device__ __forceinline__ void path_chords(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t &c0,
+                                            uint32_t &c1) {
+    const uint32_t g0 = sc.seg_base + sc.path_seg_off[p], g1 = sc.seg_base + sc.path_seg_off[p + 1];
+    const uint32_t total = b.counters[C_LINES];
+    c0 = g0 < b.n_segs ? b.seg_cnt[g0] : total;
+    c1 = g1 < b.n_segs ? b.seg_cnt[g1] : total;
+    c0 = min(c0, b.cap_lines);
+    c1 = min(c1, b.cap_lines);
+}
+
+template <bool EMIT, bool LDS>
+__device__ __forceinline__ void walk_chords(const BatchDev &b, const SceneDev &sc, const int4 &rect, uint32_t tbase,
+                                            uint32_t cbase, uint32_t *l_cnt, int32_t *l_delta, int32_t *l_col,
+                                            uint32_t c0, uint32_t c1, uint32_t first, uint32_t stride) {
+    uint32_t guard = 0;
+    for (uint32_t i = c0 + first; i < c1; i += stride) {
+        const float4 ln = b.lines[i];
+        Walker<EMIT, LDS> w;
+        w.vb_l = sc.vb_l;
+        w.vb_r = sc.vb_r;
+        w.vb_b = sc.vb_b;
+        w.tl = rect.x;
+        w.tt = rect.y;
+        w.tr = rect.z;
+        w.tb = rect.w;
+        w.tw = rect.z - rect.x;
+        w.th = rect.w - rect.y;
+        w.tbase = tbase;
+        w.cbase = cbase;
+        w.tile_cnt = b.tile_cnt;
+        w.tile_delta = b.tile_delta;
+        w.col_bd = b.col_bd;
+        w.l_cnt = l_cnt;
+        w.l_delta = l_delta;
+        w.l_col = l_col;
+        w.out = b.dump ? b.dump_fills : nullptr;
+        w.fills = b.fills;
+        w.tile_fill_off = b.tile_fill_off;
+        w.cap_fills = b.cap_fills;
+        w.pend_tile = 0xffffffffu;
+        w.pend_n = 0;
+        w.count = 0;
+        w.guard = 0;
+        w.out_base = (EMIT && b.dump) ? b.line_cnt[i] : 0u;
+        w.line(make_float2(ln.x, ln.y), make_float2(ln.z, ln.w));
+        w.flush();
+        if (!EMIT && b.dump) b.line_cnt[i] = w.count;  // fills per chord: canonical dump order only
+        guard += w.guard;
+    }
+    if (!EMIT && guard) atomicAdd(&b.counters[C_GUARD], guard);
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave class: 4 paths per 256-thread block, one per wave, each with its own LDS slice.
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
+    __shared__ uint32_t s_cnt[4][BIN_WAVE_TILES];
+    __shared__ int32_t s_delta[4][EMIT ? 1 : BIN_WAVE_TILES];
+    __shared__ int32_t s_col[4][EMIT ? 1 : BIN_WAVE_COLS];
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t item = blockIdx.x * 4 + wv;
+    if (item >= b.n_bin_wave) return;  // whole wave: no block-wide barriers below
+    const int k = find_scene(b.bin_wave_bases, b.n_scenes, item);
+    const SceneDev &sc = b.scenes[k];
+    const uint32_t p = sc.bin_paths[item - b.bin_wave_bases[k]];
+    const int4 rect = sc.path_rect[p];
+    const uint32_t tw = (uint32_t)max(rect.z - rect.x, 0), th = (uint32_t)max(rect.w - rect.y, 0);
+    const uint32_t area = tw * th;
+    if (area > (uint32_t)BIN_WAVE_TILES || tw > (uint32_t)BIN_WAVE_COLS) return;  // not this class (host sorts)
+    const uint32_t tbase = sc.tile_base + sc.path_tile_off[p], cbase = sc.col_base + sc.path_col_off[p];
+    uint32_t *l_cnt = s_cnt[wv];
+    int32_t *l_delta = s_delta[wv], *l_col = s_col[wv];
+    if (EMIT) {
+        for (uint32_t j = lane; j < area; j += 64) l_cnt[j] = b.tile_fill_off[tbase + j];
+    } else {
+        for (uint32_t j = lane; j < area; j += 64) {
+            l_cnt[j] = 0;
+            l_delta[j] = 0;
+        }
+        for (uint32_t j = lane; j < tw; j += 64) l_col[j] = 0;
+    }
+    wave_sync();
+    uint32_t c0, c1;
+    path_chords(b, sc, p, c0, c1);
+    walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, l_cnt, l_delta, l_col, c0, c1, lane, 64);
+    if (!EMIT) {
+        wave_sync();
+        for (uint32_t j = lane; j < area; j += 64) {
+            b.tile_cnt[tbase + j] = l_cnt[j];
+            b.tile_delta[tbase + j] = l_delta[j];
+        }
+        for (uint32_t j = lane; j < tw; j += 64) b.col_bd[cbase + j] = l_col[j];
+    }
+}
+
+// Group class: one path per 256-thread block; LDS when the rect fits, else global atomics
+// on the path's own (first zeroed) tiles.
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
+    __shared__ uint32_t s_cnt[BIN_GROUP_TILES];
+    __shared__ int32_t s_delta[EMIT ? 1 : BIN_GROUP_TILES];
+    __shared__ int32_t s_col[EMIT ? 1 : BIN_GROUP_COLS];
+    const uint32_t item = blockIdx.x;
+    if (item >= b.n_bin_group) return;  // uniform over the block
+    const int k = find_scene(b.bin_group_bases, b.n_scenes, item);
+    const SceneDev &sc = b.scenes[k];
+    const uint32_t p = sc.bin_paths[sc.n_bin_wave + (item - b.bin_group_bases[k])];
+    const int4 rect = sc.path_rect[p];
+    const uint32_t tw = (uint32_t)max(rect.z - rect.x, 0), th = (uint32_t)max(rect.w - rect.y, 0);
+    const uint32_t area = tw * th;
+    const uint32_t tbase = sc.tile_base + sc.path_tile_off[p], cbase = sc.col_base + sc.path_col_off[p];
+    const bool lds = area <= (uint32_t)BIN_GROUP_TILES && tw <= (uint32_t)BIN_GROUP_COLS;
+    uint32_t c0, c1;
+    path_chords(b, sc, p, c0, c1);
+    if (lds) {
+        if (EMIT) {
+            for (uint32_t j = threadIdx.x; j < area; j += 256) s_cnt[j] = b.tile_fill_off[tbase + j];
+        } else {
+            for (uint32_t j = threadIdx.x; j < area; j += 256) {
+                s_cnt[j] = 0;
+                s_delta[j] = 0;
+            }
+            for (uint32_t j = threadIdx.x; j < tw; j += 256) s_col[j] = 0;
+        }
+        __syncthreads();
+        walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, s_cnt, s_delta, s_col, c0, c1, threadIdx.x, 256);
+        if (!EMIT) {
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < area; j += 256) {
+                b.tile_cnt[tbase + j] = s_cnt[j];
+                b.tile_delta[tbase + j] = s_delta[j];
+            }
+            for (uint32_t j = threadIdx.x; j < tw; j += 256) b.col_bd[cbase + j] = s_col[j];
+        }
+    } else {
+        if (!EMIT) {
+            for (uint32_t j = threadIdx.x; j < area; j += 256) {
+                b.tile_cnt[tbase + j] = 0;
+                b.tile_delta[tbase + j] = 0;
+            }
+            for (uint32_t j = threadIdx.x; j < tw; j += 256) b.col_bd[cbase + j] = 0;
+            __threadfence();  // the zeros reach L2 before any thread's atomics
+            __syncthreads();
+        }
+        walk_chords<EMIT, false>(b, sc, rect, tbase, cbase, nullptr, nullptr, nullptr, c0, c1, threadIdx.x, 256);
+    }
+}
+
 // ---------------------------------------------------------------------------
 
 template <bool EMIT>
@@ -386,17 +567,6 @@ __global__ __launch_bounds__(256) void k_bin(BatchDev b) {
     }
 }
 
-// Fills in emission order -> grouped by tile (tile_fill_off ranges).  Within a tile
-// the order is arbitrary: coverage is accumulated in fixed point.
-__global__ __launch_bounds__(256) void k_group_fills(BatchDev b) {
-    uint32_t n = min(b.counters[C_FILLS], b.cap_fills);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint4 e = b.dump_fills[i];
-        uint32_t slot = atomicSub(&b.tile_cnt[e.z], 1u) - 1u;
-        uint32_t pos = b.tile_fill_off[e.z] + slot;
-        if (pos < b.cap_fills) b.fills[pos] = make_uint2(e.x, e.y);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Device-wide exclusive scan (reduce / scan block sums / apply), 4096 items per block
@@ -763,22 +933,17 @@ hipError_t launch_flatten(const BatchDev &b, bool emit, hipStream_t s) {
 }
 
 // grid_lines: chords the launch is sized for (exact in plain mode, capacity in batched mode)
-hipError_t launch_bin(const BatchDev &b, bool emit, uint32_t grid_lines, hipStream_t s) {
-    if (grid_lines == 0) return hipSuccess;
-    uint32_t nb = std::min<uint32_t>(blocks_for(grid_lines, 256), 65535);
-    if (emit) hipLaunchKernelGGL(k_bin<true>, dim3(nb), dim3(256), 0, s, b);
-    else hipLaunchKernelGGL(k_bin<false>, dim3(nb), dim3(256), 0, s, b);
-    return hipGetLastError();
-}
-
-// Exclusive scan of per-chord fill counts in place; total -> counter slot `counter`.
-hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
-                        uint32_t *tmp, hipStream_t s);
-
-hipError_t launch_group_fills(const BatchDev &b, uint32_t grid_fills, hipStream_t s) {
-    if (grid_fills == 0) return hipSuccess;
-    uint32_t nb = std::min<uint32_t>(blocks_for(grid_fills, 256), 16384);
-    hipLaunchKernelGGL(k_group_fills, dim3(nb), dim3(256), 0, s, b);
+hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s) {
+    if (b.n_bin_wave) {
+        const dim3 g(blocks_for(b.n_bin_wave, 4));
+        if (emit) hipLaunchKernelGGL(k_bin_wave<true>, g, dim3(256), 0, s, b);
+        else hipLaunchKernelGGL(k_bin_wave<false>, g, dim3(256), 0, s, b);
+    }
+    if (b.n_bin_group) {
+        const dim3 g(b.n_bin_group);
+        if (emit) hipLaunchKernelGGL(k_bin_group<true>, g, dim3(256), 0, s, b);
+        else hipLaunchKernelGGL(k_bin_group<false>, g, dim3(256), 0, s, b);
+    }
     return hipGetLastError();
 }
 

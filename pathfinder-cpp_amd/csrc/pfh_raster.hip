@@ -87,9 +87,11 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     __shared__ uint32_t out8[FILL_TILES_PER_BLOCK][64];
     const int g = threadIdx.x >> 4;   // tile slot in the block
     const int li = threadIdx.x & 15;  // lane in the tile's group (= pixel column for the output)
-    const uint32_t a = blockIdx.x * FILL_TILES_PER_BLOCK + g;
     const uint32_t na = min(b.counters[C_ALPHA], b.cap_alpha);
-    if (blockIdx.x * FILL_TILES_PER_BLOCK >= na) return;  // whole block past the last tile
+    // grid-stride over groups of 16 tiles: the grid is sized from the capacity, the count
+    // is only known on the device; the loop bound is uniform over the block
+    for (uint32_t first = blockIdx.x * FILL_TILES_PER_BLOCK; first < na; first += gridDim.x * FILL_TILES_PER_BLOCK) {
+    const uint32_t a = first + g;
     const bool valid = a < na;
     // all lanes stay to the end: loops below have wave-uniform bounds and use shuffles
 #pragma unroll
@@ -163,6 +165,11 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     if (valid) {
         const uint4 row = reinterpret_cast<const uint4 *>(&out8[g][0])[li];
         reinterpret_cast<uint4 *>(b.masks + (uint64_t)a * 256)[li] = row;
+    }
+    // the next group reuses this slot's LDS
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -316,7 +323,9 @@ static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)(
 
 hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, hipStream_t s) {
     if (grid_alpha == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_masks, dim3(blocks_for(grid_alpha, FILL_TILES_PER_BLOCK)), dim3(256), 0, s, b);
+    // at most 8192 blocks (32 per CU): each loops over its share of the tiles
+    const uint32_t nb = std::min<uint32_t>(blocks_for(grid_alpha, FILL_TILES_PER_BLOCK), 8192u);
+    hipLaunchKernelGGL(k_fill_masks, dim3(nb), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

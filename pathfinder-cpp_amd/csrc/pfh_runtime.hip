@@ -4,13 +4,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pfh.h"
@@ -23,12 +26,11 @@ hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp
 hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, uint32_t *tmp, hipStream_t s);
 hipError_t scan_tiles(const BatchDev &b, void *tmp, hipStream_t s);
 hipError_t launch_flatten(const BatchDev &b, bool emit, hipStream_t s);
-hipError_t launch_bin(const BatchDev &b, bool emit, uint32_t grid_lines, hipStream_t s);
+hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s);
 hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
                         uint32_t *tmp, hipStream_t s);
 hipError_t scan_total(uint32_t *inout, uint32_t n, uint32_t *counters, int counter, uint32_t *tmp, hipStream_t s);
 hipError_t launch_propagate(const BatchDev &b, hipStream_t s);
-hipError_t launch_group_fills(const BatchDev &b, uint32_t grid_fills, hipStream_t s);
 hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s);
 size_t radix_temp_elems(uint64_t cap);
 hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, const uint32_t *n_ptr,
@@ -113,7 +115,8 @@ struct SceneHost {
     std::vector<float> features;
     DevBuf d_features;
     DevBuf d_pts, d_segs, d_seg_path, d_path_rect, d_path_tile_off, d_path_col_off, d_col_path, d_path_rgba,
-        d_path_rule, d_out_owned;
+        d_path_rule, d_path_seg_off, d_bin_paths, d_out_owned;
+    uint32_t n_bin_wave = 0, n_bin_group = 0;  // paths binned by one wave / by one workgroup
     uint8_t *d_out = nullptr;
     int64_t out_pitch = 0;
     // placement in the last rasterized batch (for dumps)
@@ -139,18 +142,20 @@ struct GraphKey {
     }
 };
 
-}  // namespace
+struct Server;
 
-struct pfh_ctx {
-    int device = 0;
+// Device working set of one launch: work buffers, stream, events and captured graphs.
+// The context's own workspace serves pfh_rasterize; the request loop owns one per slot so
+// that launches in different slots overlap on the device.
+struct Workspace {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string err;
-    std::vector<std::unique_ptr<SceneHost>> scenes;
-    // batch work buffers
+    // around the fill and composite kernels of a batched launch; recorded as event nodes
+    // inside captured graphs, so replays time those kernels too
+    hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_entry, col_bd,
-        seg_cnt, lines, line_seg, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_cnt, keys, vals, keys_alt, vals_alt, entry_rec,
-        counters, scan_tmp, sort_tmp;
+        seg_cnt, lines, line_seg, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_cnt, keys, vals, keys_alt,
+        vals_alt, entry_rec, counters, scan_tmp, sort_tmp;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
     size_t h_table_cap = 0;
@@ -159,11 +164,65 @@ struct pfh_ctx {
     bool last_sorted_alt = false;
     std::map<GraphKey, hipGraphExec_t> graphs;
     uint64_t graphs_generation = 0;  // g_alloc_generation the cached graphs were captured under
+
+    void create() {
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&ev0));
+        HIP_TRY(hipEventCreate(&ev1));
+        for (auto &e : kev) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipHostMalloc((void **)&h_counters, 4 * C_COUNT));
+    }
+    void drop_graphs() {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
+    }
+    std::vector<DevBuf *> all_buffers() {
+        return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd,
+                &tile_entry, &col_bd, &seg_cnt, &lines, &line_seg, &line_cnt, &fills, &dump_fills, &alpha_rec,
+                &masks, &zbuf, &fb_cnt, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
+                &sort_tmp};
+    }
+    size_t device_bytes() {
+        size_t n = 0;
+        for (DevBuf *b : all_buffers()) n += b->bytes;
+        return n;
+    }
+    void release_buffers() {
+        drop_graphs();
+        for (DevBuf *b : all_buffers()) b->release();
+    }
+    void destroy() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        release_buffers();
+        if (h_counters) (void)hipHostFree(h_counters);
+        if (h_table) (void)hipHostFree(h_table);
+        if (h_bases) (void)hipHostFree(h_bases);
+        h_counters = nullptr;
+        h_table = nullptr;
+        h_bases = nullptr;
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        for (auto &e : kev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+}  // namespace
+
+struct pfh_ctx {
+    int device = 0;
+    std::string err;
+    std::vector<std::unique_ptr<SceneHost>> scenes;
+    Workspace main;
+    Workspace *w = &main;  // workspace the pipeline functions enqueue into
     // predictor
     std::vector<float> predictor;
     float predictor_safety = 1.25f;
     DevBuf mlp_params, mlp_x, mlp_h, mlp_out;
     pfh_stats stats{};
+    std::unique_ptr<Server> server;  // request loop, when started
 };
 
 namespace {
@@ -371,6 +430,18 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
     s->n_segs = (uint32_t)segs.size();
     s->n_tiles = (uint32_t)tiles;
     s->n_cols = (uint32_t)cols;
+    // binning classes by what the path owns: its rect's tiles and its columns
+    std::vector<uint32_t> bin_wave, bin_group;
+    for (int p = 0; p < d->n_paths; p++) {
+        const int4 &r = s->path_rect[p];
+        const uint64_t w = (uint64_t)std::max(0, r.z - r.x), h = (uint64_t)std::max(0, r.w - r.y);
+        if (w == 0) continue;  // no columns: nothing is binned (k_flatten emits no chords)
+        if (w * h <= (uint64_t)BIN_WAVE_TILES && w <= (uint64_t)BIN_WAVE_COLS) bin_wave.push_back((uint32_t)p);
+        else bin_group.push_back((uint32_t)p);
+    }
+    s->n_bin_wave = (uint32_t)bin_wave.size();
+    s->n_bin_group = (uint32_t)bin_group.size();
+    bin_wave.insert(bin_wave.end(), bin_group.begin(), bin_group.end());
     s->features.resize((size_t)d->n_paths * N_FEATURES);
     for (int p = 0; p < d->n_paths; p++) path_features(*s, pts, segs, (uint32_t)p, &s->features[(size_t)p * N_FEATURES]);
 
@@ -385,6 +456,8 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
     upload(s->d_col_path, col_path);
     upload(s->d_path_rgba, s->path_rgba);
     upload(s->d_path_rule, s->path_rule);
+    upload(s->d_path_seg_off, s->path_seg_off);
+    upload(s->d_bin_paths, bin_wave);
     s->out_pitch = (int64_t)s->width * 4;
     s->d_out_owned.ensure((size_t)s->out_pitch * s->height);
     s->d_out = s->d_out_owned.as<uint8_t>();
@@ -413,9 +486,9 @@ static void predict_scene(pfh_ctx *ctx, SceneHost &s) {
     if (s.n_paths > 0) {
         ctx->mlp_out.ensure(16);
         HIP_TRY(mlp_predict_sum(ctx->mlp_params.as<float>(), s.d_features.as<float>(), s.n_paths,
-                                ctx->mlp_out.as<float>(), ctx->stream));
-        HIP_TRY(hipMemcpyAsync(sums, ctx->mlp_out.ptr, 12, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+                                ctx->mlp_out.as<float>(), ctx->w->stream));
+        HIP_TRY(hipMemcpyAsync(sums, ctx->mlp_out.ptr, 12, hipMemcpyDeviceToHost, ctx->w->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->w->stream));
     }
     s.pred_lines = sums[0];
     s.pred_fills = sums[1];
@@ -428,40 +501,40 @@ static void predict_scene(pfh_ctx *ctx, SceneHost &s) {
 
 struct Plan {
     std::vector<int32_t> ids;
-    uint64_t S = 0, T = 0, W = 0, FB = 0;
+    uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0;
     uint32_t cap_f = 0, cap_a = 0, cap_e = 0, cap_l = 0;
     int key_bits = 1;
 };
 
 static uint32_t clamp_cap(uint64_t v) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(v, 64), 0xF0000000ull); }
 
-static void ensure_sizes(pfh_ctx *ctx, const Plan &pl, bool dump) {
-    ctx->tile_delta.ensure(4 * pl.T + 4);
-    ctx->tile_cnt.ensure(4 * pl.T + 4);
-    ctx->tile_fill_off.ensure(4 * pl.T + 8);
-    ctx->tile_alpha_idx.ensure(4 * pl.T + 8);
-    ctx->tile_bd.ensure(pl.T + 4);
-    ctx->tile_entry.ensure(4 * pl.T + 8);
-    ctx->col_bd.ensure(4 * pl.W + 4);
-    ctx->seg_cnt.ensure(4 * pl.S + 8);
-    ctx->lines.ensure(16ull * pl.cap_l);
-    ctx->line_seg.ensure(4ull * pl.cap_l);
-    ctx->line_cnt.ensure(4ull * pl.cap_l + 8);
-    ctx->zbuf.ensure(4 * pl.FB + 4);
-    ctx->fb_cnt.ensure(4 * pl.FB + 8);
-    ctx->counters.ensure(4 * C_COUNT);
+static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
+    w.tile_delta.ensure(4 * pl.T + 4);
+    w.tile_cnt.ensure(4 * pl.T + 4);
+    w.tile_fill_off.ensure(4 * pl.T + 8);
+    w.tile_alpha_idx.ensure(4 * pl.T + 8);
+    w.tile_bd.ensure(pl.T + 4);
+    w.tile_entry.ensure(4 * pl.T + 8);
+    w.col_bd.ensure(4 * pl.W + 4);
+    w.seg_cnt.ensure(4 * pl.S + 8);
+    w.lines.ensure(16ull * pl.cap_l);
+    w.line_seg.ensure(4ull * pl.cap_l);
+    w.line_cnt.ensure(4ull * pl.cap_l + 8);
+    w.zbuf.ensure(4 * pl.FB + 4);
+    w.fb_cnt.ensure(4 * pl.FB + 8);
+    w.counters.ensure(4 * C_COUNT);
     uint64_t scan_n = std::max<uint64_t>({pl.T + 1, pl.S + 1, pl.FB + 1, (uint64_t)pl.cap_l + 1, 1});
-    ctx->scan_tmp.ensure(16 * scan_temp_elems(scan_n) + 64);
-    ctx->fills.ensure(8ull * pl.cap_f);
-    if (dump) ctx->dump_fills.ensure(16ull * pl.cap_f);
-    ctx->alpha_rec.ensure(16ull * pl.cap_a);
-    ctx->masks.ensure(256ull * pl.cap_a);
-    ctx->keys.ensure(4ull * pl.cap_e);
-    ctx->vals.ensure(4ull * pl.cap_e);
-    ctx->keys_alt.ensure(4ull * pl.cap_e);
-    ctx->vals_alt.ensure(4ull * pl.cap_e);
-    ctx->entry_rec.ensure(16ull * pl.cap_e);
-    ctx->sort_tmp.ensure(4 * radix_temp_elems(pl.cap_e) + 64);
+    w.scan_tmp.ensure(16 * scan_temp_elems(scan_n) + 64);
+    w.fills.ensure(8ull * pl.cap_f);
+    if (dump) w.dump_fills.ensure(16ull * pl.cap_f);
+    w.alpha_rec.ensure(16ull * pl.cap_a);
+    w.masks.ensure(256ull * pl.cap_a);
+    w.keys.ensure(4ull * pl.cap_e);
+    w.vals.ensure(4ull * pl.cap_e);
+    w.keys_alt.ensure(4ull * pl.cap_e);
+    w.vals_alt.ensure(4ull * pl.cap_e);
+    w.entry_rec.ensure(16ull * pl.cap_e);
+    w.sort_tmp.ensure(4 * radix_temp_elems(pl.cap_e) + 64);
 }
 
 // Capacities the current allocations hold, by the same per-buffer sizes ensure_sizes asks
@@ -469,33 +542,36 @@ static void ensure_sizes(pfh_ctx *ctx, const Plan &pl, bool dump) {
 struct Caps {
     uint32_t l, f, a;
 };
-static Caps held_caps(const pfh_ctx *ctx, bool dump) {
+static Caps held_caps(Workspace &w, bool dump) {
     auto fit = [](size_t bytes, size_t per, size_t extra) -> uint64_t {
         return bytes > extra ? (bytes - extra) / per : 0;
     };
-    uint64_t l = std::min({fit(ctx->lines.bytes, 16, 0), fit(ctx->line_seg.bytes, 4, 0), fit(ctx->line_cnt.bytes, 4, 8)});
-    uint64_t f = fit(ctx->fills.bytes, 8, 0);
-    if (dump) f = std::min(f, fit(ctx->dump_fills.bytes, 16, 0));
-    uint64_t a = std::min(fit(ctx->alpha_rec.bytes, 16, 0), fit(ctx->masks.bytes, 256, 0));
+    uint64_t l = std::min({fit(w.lines.bytes, 16, 0), fit(w.line_seg.bytes, 4, 0), fit(w.line_cnt.bytes, 4, 8)});
+    uint64_t f = fit(w.fills.bytes, 8, 0);
+    if (dump) f = std::min(f, fit(w.dump_fills.bytes, 16, 0));
+    uint64_t a = std::min(fit(w.alpha_rec.bytes, 16, 0), fit(w.masks.bytes, 256, 0));
     return Caps{clamp_cap(l), clamp_cap(f), clamp_cap(a)};
 }
 
+// base table: segment, tile, column, fb-tile, wave-class and group-class item offsets per scene
+constexpr size_t N_BASES = 6;
+
 static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
     size_t n = pl.ids.size();
-    if (ctx->h_table_cap < n) {
-        if (ctx->h_table) (void)hipHostFree(ctx->h_table);
-        HIP_TRY(hipHostMalloc((void **)&ctx->h_table, sizeof(SceneDev) * n));
-        ctx->h_table_cap = n;
+    if (ctx->w->h_table_cap < n) {
+        if (ctx->w->h_table) (void)hipHostFree(ctx->w->h_table);
+        HIP_TRY(hipHostMalloc((void **)&ctx->w->h_table, sizeof(SceneDev) * n));
+        ctx->w->h_table_cap = n;
     }
-    if (ctx->h_bases_cap < 4 * (n + 1)) {
-        if (ctx->h_bases) (void)hipHostFree(ctx->h_bases);
-        HIP_TRY(hipHostMalloc((void **)&ctx->h_bases, sizeof(uint32_t) * 4 * (n + 1)));
-        ctx->h_bases_cap = 4 * (n + 1);
+    if (ctx->w->h_bases_cap < N_BASES * (n + 1)) {
+        if (ctx->w->h_bases) (void)hipHostFree(ctx->w->h_bases);
+        HIP_TRY(hipHostMalloc((void **)&ctx->w->h_bases, sizeof(uint32_t) * N_BASES * (n + 1)));
+        ctx->w->h_bases_cap = N_BASES * (n + 1);
     }
-    uint64_t S = 0, T = 0, W = 0, FB = 0;
+    uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0;
     for (size_t k = 0; k < n; k++) {
         SceneHost &s = get_scene(ctx, pl.ids[k]);
-        SceneDev &d = ctx->h_table[k];
+        SceneDev &d = ctx->w->h_table[k];
         d.pts = s.d_pts.as<float2>();
         d.segs = s.d_segs.as<uint2>();
         d.seg_path = s.d_seg_path.as<uint32_t>();
@@ -504,6 +580,12 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.path_col_off = s.d_path_col_off.as<uint32_t>();
         d.col_path = s.d_col_path.as<uint32_t>();
         d.path_rgba = s.d_path_rgba.as<uint32_t>();
+        d.path_seg_off = s.d_path_seg_off.as<uint32_t>();
+        d.bin_paths = s.d_bin_paths.as<uint32_t>();
+        d.n_bin_wave = s.n_bin_wave;
+        d.n_bin_group = s.n_bin_group;
+        d.bin_wave_base = (uint32_t)BW;
+        d.bin_group_base = (uint32_t)BG;
         d.path_rule = s.d_path_rule.as<uint8_t>();
         d.vb_l = s.vb[0];
         d.vb_t = s.vb[1];
@@ -533,21 +615,29 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         s.tile_base = d.tile_base;
         s.col_base = d.col_base;
         s.fb_base = d.fb_base;
-        ctx->h_bases[0 * (n + 1) + k] = (uint32_t)S;
-        ctx->h_bases[1 * (n + 1) + k] = (uint32_t)T;
-        ctx->h_bases[2 * (n + 1) + k] = (uint32_t)W;
-        ctx->h_bases[3 * (n + 1) + k] = (uint32_t)FB;
+        ctx->w->h_bases[0 * (n + 1) + k] = (uint32_t)S;
+        ctx->w->h_bases[1 * (n + 1) + k] = (uint32_t)T;
+        ctx->w->h_bases[2 * (n + 1) + k] = (uint32_t)W;
+        ctx->w->h_bases[3 * (n + 1) + k] = (uint32_t)FB;
+        ctx->w->h_bases[4 * (n + 1) + k] = (uint32_t)BW;
+        ctx->w->h_bases[5 * (n + 1) + k] = (uint32_t)BG;
         S += s.n_segs;
         T += s.n_tiles;
         W += s.n_cols;
         FB += d.n_fb;
+        BW += s.n_bin_wave;
+        BG += s.n_bin_group;
         if (S >= 0xF0000000ull || T >= 0xF0000000ull || W >= 0xF0000000ull || FB >= 0xF0000000ull)
             throw PfhError{PFH_E_LIMIT, "batch exceeds 2^32 segments/tiles"};
     }
-    ctx->h_bases[0 * (n + 1) + n] = (uint32_t)S;
-    ctx->h_bases[1 * (n + 1) + n] = (uint32_t)T;
-    ctx->h_bases[2 * (n + 1) + n] = (uint32_t)W;
-    ctx->h_bases[3 * (n + 1) + n] = (uint32_t)FB;
+    ctx->w->h_bases[0 * (n + 1) + n] = (uint32_t)S;
+    ctx->w->h_bases[1 * (n + 1) + n] = (uint32_t)T;
+    ctx->w->h_bases[2 * (n + 1) + n] = (uint32_t)W;
+    ctx->w->h_bases[3 * (n + 1) + n] = (uint32_t)FB;
+    ctx->w->h_bases[4 * (n + 1) + n] = (uint32_t)BW;
+    ctx->w->h_bases[5 * (n + 1) + n] = (uint32_t)BG;
+    pl.BW = BW;
+    pl.BG = BG;
     pl.S = S;
     pl.T = T;
     pl.W = W;
@@ -556,25 +646,29 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
     while ((1ull << pl.key_bits) < FB) pl.key_bits++;
 }
 
-static void upload_tables(pfh_ctx *ctx, const Plan &pl) {
+static void upload_tables(Workspace &w, const Plan &pl) {
     size_t n = pl.ids.size();
-    ctx->scene_table.ensure(sizeof(SceneDev) * n);
-    ctx->bases.ensure(sizeof(uint32_t) * 4 * (n + 1));
-    HIP_TRY(hipMemcpyAsync(ctx->scene_table.ptr, ctx->h_table, sizeof(SceneDev) * n, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->bases.ptr, ctx->h_bases, sizeof(uint32_t) * 4 * (n + 1), hipMemcpyHostToDevice,
-                           ctx->stream));
+    w.scene_table.ensure(sizeof(SceneDev) * n);
+    w.bases.ensure(sizeof(uint32_t) * N_BASES * (n + 1));
+    HIP_TRY(hipMemcpyAsync(w.scene_table.ptr, w.h_table, sizeof(SceneDev) * n, hipMemcpyHostToDevice, w.stream));
+    HIP_TRY(hipMemcpyAsync(w.bases.ptr, w.h_bases, sizeof(uint32_t) * N_BASES * (n + 1), hipMemcpyHostToDevice,
+                           w.stream));
 }
 
-static BatchDev batch_dev(pfh_ctx *ctx, const Plan &pl, bool dump) {
+static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     BatchDev b{};
     size_t n = pl.ids.size();
-    b.scenes = ctx->scene_table.as<SceneDev>();
+    b.scenes = w.scene_table.as<SceneDev>();
     b.n_scenes = (int32_t)n;
-    const uint32_t *bases = ctx->bases.as<uint32_t>();
+    const uint32_t *bases = w.bases.as<uint32_t>();
     b.seg_bases = bases;
     b.tile_bases = bases + (n + 1);
     b.col_bases = bases + 2 * (n + 1);
     b.fb_bases = bases + 3 * (n + 1);
+    b.bin_wave_bases = bases + 4 * (n + 1);
+    b.bin_group_bases = bases + 5 * (n + 1);
+    b.n_bin_wave = (uint32_t)pl.BW;
+    b.n_bin_group = (uint32_t)pl.BG;
     b.n_segs = (uint32_t)pl.S;
     b.n_tiles = (uint32_t)pl.T;
     b.n_cols = (uint32_t)pl.W;
@@ -583,80 +677,82 @@ static BatchDev batch_dev(pfh_ctx *ctx, const Plan &pl, bool dump) {
     b.cap_alpha = pl.cap_a;
     b.cap_entries = pl.cap_e;
     b.cap_lines = pl.cap_l;
-    b.tile_delta = ctx->tile_delta.as<int32_t>();
-    b.tile_cnt = ctx->tile_cnt.as<uint32_t>();
-    b.tile_fill_off = ctx->tile_fill_off.as<uint32_t>();
-    b.tile_alpha_idx = ctx->tile_alpha_idx.as<uint32_t>();
-    b.tile_bd = ctx->tile_bd.as<int8_t>();
-    b.tile_entry = ctx->tile_entry.as<uint32_t>();
-    b.col_bd = ctx->col_bd.as<int32_t>();
-    b.seg_cnt = ctx->seg_cnt.as<uint32_t>();
-    b.lines = ctx->lines.as<float4>();
-    b.line_seg = ctx->line_seg.as<uint32_t>();
-    b.line_cnt = ctx->line_cnt.as<uint32_t>();
-    b.fills = ctx->fills.as<uint2>();
-    b.dump_fills = dump ? ctx->dump_fills.as<uint4>() : nullptr;
-    b.alpha_rec = ctx->alpha_rec.as<uint4>();
-    b.masks = ctx->masks.as<uint8_t>();
-    b.zbuf = ctx->zbuf.as<uint32_t>();
-    b.fb_cnt = ctx->fb_cnt.as<uint32_t>();
-    b.keys = ctx->keys.as<uint32_t>();
-    b.vals = ctx->vals.as<uint32_t>();
-    b.keys_alt = ctx->keys_alt.as<uint32_t>();
-    b.vals_alt = ctx->vals_alt.as<uint32_t>();
-    b.entry_rec = ctx->entry_rec.as<uint4>();
-    b.counters = ctx->counters.as<uint32_t>();
+    b.tile_delta = w.tile_delta.as<int32_t>();
+    b.tile_cnt = w.tile_cnt.as<uint32_t>();
+    b.tile_fill_off = w.tile_fill_off.as<uint32_t>();
+    b.tile_alpha_idx = w.tile_alpha_idx.as<uint32_t>();
+    b.tile_bd = w.tile_bd.as<int8_t>();
+    b.tile_entry = w.tile_entry.as<uint32_t>();
+    b.col_bd = w.col_bd.as<int32_t>();
+    b.seg_cnt = w.seg_cnt.as<uint32_t>();
+    b.lines = w.lines.as<float4>();
+    b.line_seg = w.line_seg.as<uint32_t>();
+    b.line_cnt = w.line_cnt.as<uint32_t>();
+    b.fills = w.fills.as<uint2>();
+    b.dump_fills = dump ? w.dump_fills.as<uint4>() : nullptr;
+    b.alpha_rec = w.alpha_rec.as<uint4>();
+    b.masks = w.masks.as<uint8_t>();
+    b.zbuf = w.zbuf.as<uint32_t>();
+    b.fb_cnt = w.fb_cnt.as<uint32_t>();
+    b.keys = w.keys.as<uint32_t>();
+    b.vals = w.vals.as<uint32_t>();
+    b.keys_alt = w.keys_alt.as<uint32_t>();
+    b.vals_alt = w.vals_alt.as<uint32_t>();
+    b.entry_rec = w.entry_rec.as<uint4>();
+    b.counters = w.counters.as<uint32_t>();
     b.dump = dump ? 1 : 0;
     return b;
 }
 
-// Stage groups; every launch goes to ctx->stream so the batched sequence can be captured.
-static void stage_clear(pfh_ctx *ctx, const BatchDev &b) {
-    hipStream_t s = ctx->stream;
-    HIP_TRY(launch_fill_u32((uint32_t *)b.tile_delta, 0, b.n_tiles, s));
-    HIP_TRY(launch_fill_u32(b.tile_cnt, 0, b.n_tiles, s));
-    HIP_TRY(launch_fill_u32((uint32_t *)b.col_bd, 0, b.n_cols, s));
+// Stage groups; every launch goes to ctx->w->stream so the batched sequence can be captured.
+static void stage_clear(Workspace &w, const BatchDev &b) {
+    hipStream_t s = w.stream;
+    // tile_delta / tile_cnt / col_bd need no clearing: the bin count pass stores every
+    // tile and column of every binned path (paths with no columns own none)
     HIP_TRY(launch_fill_u32(b.zbuf, 0, b.n_fb, s));
     HIP_TRY(launch_fill_u32(b.fb_cnt, 0, (uint64_t)b.n_fb + 1, s));
     HIP_TRY(launch_fill_u32(b.counters, 0, C_COUNT, s));
 }
 
 // flatten (count, scan, emit) and bin count: chords and exact fill / alpha-tile counts
-static void stage_count(pfh_ctx *ctx, const BatchDev &b) {
-    hipStream_t s = ctx->stream;
-    uint32_t *tmp = ctx->scan_tmp.as<uint32_t>();
+static void stage_count(Workspace &w, const BatchDev &b) {
+    hipStream_t s = w.stream;
+    uint32_t *tmp = w.scan_tmp.as<uint32_t>();
     HIP_TRY(launch_flatten(b, false, s));
     HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp, s));
     HIP_TRY(launch_flatten(b, true, s));
-    HIP_TRY(launch_bin(b, false, b.cap_lines, s));
-    HIP_TRY(scan_tiles(b, ctx->scan_tmp.ptr, s));
+    HIP_TRY(launch_bin(b, false, s));
+    HIP_TRY(scan_tiles(b, w.scan_tmp.ptr, s));
 }
 
-static void stage_emit(pfh_ctx *ctx, const BatchDev &b) {
-    hipStream_t s = ctx->stream;
-    uint32_t *tmp = ctx->scan_tmp.as<uint32_t>();
+static void stage_emit(Workspace &w, const BatchDev &b) {
+    hipStream_t s = w.stream;
+    uint32_t *tmp = w.scan_tmp.as<uint32_t>();
     if (b.dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, b.cap_lines, b.counters, -1, tmp, s));
-    HIP_TRY(launch_bin(b, true, b.cap_lines, s));
+    HIP_TRY(launch_bin(b, true, s));
     HIP_TRY(launch_propagate(b, s));
     HIP_TRY(launch_entries(b, false, s));
     HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp, s));
 }
 
-static void stage_finish(pfh_ctx *ctx, const BatchDev &b, const Plan &pl) {
-    HIP_TRY(launch_entries(b, true, ctx->stream));
-    HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
+static void stage_finish(Workspace &w, const BatchDev &b, const Plan &pl) {
+    HIP_TRY(launch_entries(b, true, w.stream));
+    HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, w.scan_tmp.as<uint32_t>(), w.stream));
     bool alt = false;
     HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
-                       ctx->sort_tmp.as<uint32_t>(), &alt, ctx->stream));
-    ctx->last_sorted_alt = alt;
-    HIP_TRY(launch_fill_masks(b, b.cap_alpha, ctx->stream));
-    HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, ctx->stream));
+                       w.sort_tmp.as<uint32_t>(), &alt, w.stream));
+    w.last_sorted_alt = alt;
+    HIP_TRY(hipEventRecordWithFlags(w.kev[0], w.stream, hipEventRecordExternal));
+    HIP_TRY(launch_fill_masks(b, b.cap_alpha, w.stream));
+    HIP_TRY(hipEventRecordWithFlags(w.kev[1], w.stream, hipEventRecordExternal));
+    HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, w.stream));
+    HIP_TRY(hipEventRecordWithFlags(w.kev[2], w.stream, hipEventRecordExternal));
 }
 
 static void read_counters(pfh_ctx *ctx) {
     ctx->stats.host_syncs++;
-    HIP_TRY(hipMemcpyAsync(ctx->h_counters, ctx->counters.ptr, 4 * C_COUNT, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->w->h_counters, ctx->w->counters.ptr, 4 * C_COUNT, hipMemcpyDeviceToHost, ctx->w->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->w->stream));
 }
 
 static double elapsed_ms(pfh_ctx *ctx, hipEvent_t a, hipEvent_t b) {
@@ -666,12 +762,12 @@ static double elapsed_ms(pfh_ctx *ctx, hipEvent_t a, hipEvent_t b) {
 }
 
 static void record_counts(pfh_ctx *ctx, const Plan &pl) {
-    ctx->stats.fills = ctx->h_counters[C_FILLS];
-    ctx->stats.alpha_tiles = ctx->h_counters[C_ALPHA];
-    ctx->stats.entries = ctx->h_counters[C_ENTRIES];
-    ctx->stats.guard_hits = ctx->h_counters[C_GUARD];
-    ctx->stats.alpha_entries = ctx->h_counters[C_ALPHA_ENTRIES];
-    ctx->stats.lines = ctx->h_counters[C_LINES];
+    ctx->stats.fills = ctx->w->h_counters[C_FILLS];
+    ctx->stats.alpha_tiles = ctx->w->h_counters[C_ALPHA];
+    ctx->stats.entries = ctx->w->h_counters[C_ENTRIES];
+    ctx->stats.guard_hits = ctx->w->h_counters[C_GUARD];
+    ctx->stats.alpha_entries = ctx->w->h_counters[C_ALPHA_ENTRIES];
+    ctx->stats.lines = ctx->w->h_counters[C_LINES];
     ctx->stats.capacity_lines = pl.cap_l;
     ctx->stats.tiles = (int64_t)pl.T;
     ctx->stats.columns = (int64_t)pl.W;
@@ -689,45 +785,45 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     Plan pl;
     pl.ids = {id};
     write_tables(ctx, pl, (flags & PFH_LOAD_DST) != 0);
-    Caps held = held_caps(ctx, dump);
+    Caps held = held_caps(*ctx->w, dump);
     pl.cap_f = held.f;
     pl.cap_a = held.a;
     pl.cap_l = held.l;
     pl.cap_e = clamp_cap(std::max<uint64_t>(pl.T, 1));
-    ensure_sizes(ctx, pl, dump);
-    upload_tables(ctx, pl);
-    BatchDev b = batch_dev(ctx, pl, dump);
-    hipStream_t st = ctx->stream;
+    ensure_sizes(*ctx->w, pl, dump);
+    upload_tables(*ctx->w, pl);
+    BatchDev b = batch_dev(*ctx->w, pl, dump);
+    hipStream_t st = ctx->w->stream;
     // scratch is re-read at every use: ensure_sizes may reallocate it when a count grows
-    auto tmp = [ctx]() { return ctx->scan_tmp.as<uint32_t>(); };
+    auto tmp = [ctx]() { return ctx->w->scan_tmp.as<uint32_t>(); };
     hipEvent_t ev[9];
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[0], st));
-    stage_clear(ctx, b);
+    stage_clear(*ctx->w, b);
     HIP_TRY(launch_flatten(b, false, st));
     HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp(), st));
     read_counters(ctx);  // chord count (renderer.cpp:707-724)
-    uint32_t nl = ctx->h_counters[C_LINES];
+    uint32_t nl = ctx->w->h_counters[C_LINES];
     if (nl > pl.cap_l) {
         pl.cap_l = clamp_cap(nl);
-        ensure_sizes(ctx, pl, dump);
-        b = batch_dev(ctx, pl, dump);
+        ensure_sizes(*ctx->w, pl, dump);
+        b = batch_dev(*ctx->w, pl, dump);
     }
     HIP_TRY(launch_flatten(b, true, st));
     HIP_TRY(hipEventRecord(ev[1], st));
-    HIP_TRY(launch_bin(b, false, nl, st));
-    HIP_TRY(scan_tiles(b, ctx->scan_tmp.ptr, st));
+    HIP_TRY(launch_bin(b, false, st));
+    HIP_TRY(scan_tiles(b, ctx->w->scan_tmp.ptr, st));
     read_counters(ctx);  // fill / alpha counts (renderer.cpp:834-845)
     HIP_TRY(hipEventRecord(ev[2], st));
-    uint32_t nf = ctx->h_counters[C_FILLS], na = ctx->h_counters[C_ALPHA];
+    uint32_t nf = ctx->w->h_counters[C_FILLS], na = ctx->w->h_counters[C_ALPHA];
     if (nf > pl.cap_f || na > pl.cap_a) {
         pl.cap_f = clamp_cap(std::max<uint64_t>(nf, pl.cap_f));
         pl.cap_a = clamp_cap(std::max<uint64_t>(na, pl.cap_a));
-        ensure_sizes(ctx, pl, dump);
-        b = batch_dev(ctx, pl, dump);
+        ensure_sizes(*ctx->w, pl, dump);
+        b = batch_dev(*ctx->w, pl, dump);
     }
     if (dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, nl, b.counters, -1, tmp(), st));
-    HIP_TRY(launch_bin(b, true, nl, st));
+    HIP_TRY(launch_bin(b, true, st));
     HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(launch_propagate(b, st));
     HIP_TRY(hipEventRecord(ev[4], st));
@@ -738,8 +834,8 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, tmp(), st));
     bool alt = false;
     HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
-                       ctx->sort_tmp.as<uint32_t>(), &alt, st));
-    ctx->last_sorted_alt = alt;
+                       ctx->w->sort_tmp.as<uint32_t>(), &alt, st));
+    ctx->w->last_sorted_alt = alt;
     HIP_TRY(hipEventRecord(ev[6], st));
     HIP_TRY(launch_fill_masks(b, na, st));
     HIP_TRY(hipEventRecord(ev[7], st));
@@ -752,147 +848,360 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     SceneHost &s = get_scene(ctx, id);
     s.counted = true;
     s.last_lines = nl;
-    s.last_fills = ctx->h_counters[C_FILLS];
-    s.last_alpha = ctx->h_counters[C_ALPHA];
-    s.last_entries = ctx->h_counters[C_ENTRIES];
+    s.last_fills = ctx->w->h_counters[C_FILLS];
+    s.last_alpha = ctx->w->h_counters[C_ALPHA];
+    s.last_entries = ctx->w->h_counters[C_ENTRIES];
     s.dumped = dump;
 }
 
 // Batched: every scene in one pass, no host round trip inside; the launch sequence is
-// captured once per (scene set, capacities, flags) and replayed from a HIP graph.
-static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint32_t flags) {
-    bool dump = (flags & PFH_DUMP) != 0;
-    bool use_graph = (flags & PFH_NO_GRAPH) == 0;
+// captured once per (scene set, capacities, flags) and replayed from a HIP graph.  Split in
+// three so that the request loop can keep launches of several workspaces in flight:
+// batched_prepare (capacities), batched_enqueue (asynchronous), batched_check (after the
+// launch's counters have arrived: accept, or grow and go again).
+struct BatchedLaunch {
     Plan pl;
+    uint32_t flags = 0;
+    bool dump = false, use_graph = true;
+    uint64_t plc = 0, pf = 0, pa = 0;  // requested chord / fill / alpha-tile capacities
+    int attempt = 0;
+};
+
+static void batched_prepare(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint32_t flags, BatchedLaunch &bl) {
+    bl = BatchedLaunch{};
+    bl.flags = flags;
+    bl.dump = (flags & PFH_DUMP) != 0;
+    bl.use_graph = (flags & PFH_NO_GRAPH) == 0;
+    Plan &pl = bl.pl;
     pl.ids.assign(ids, ids + n);
     write_tables(ctx, pl, (flags & PFH_LOAD_DST) != 0);
-    // Capacities: entries are bounded by tiles exactly; fills and alpha tiles come from
-    // the predictor, else from the last exact counts, else from a count pass read back.
-    uint64_t pf = 0, pa = 0, plc = 0;
+    // Capacities: entries are bounded by tiles exactly; chords, fills and alpha tiles come
+    // from the predictor, else from the last exact counts, else from a count pass read back.
     bool known = true;
     for (int32_t id : pl.ids) {
         SceneHost &s = get_scene(ctx, id);
         if ((flags & PFH_PREDICT) && !ctx->predictor.empty()) {
-            plc += (uint64_t)std::ceil(s.pred_lines * ctx->predictor_safety) + 64;
-            pf += (uint64_t)std::ceil(s.pred_fills * ctx->predictor_safety) + 64;
-            pa += (uint64_t)std::ceil(s.pred_alpha * ctx->predictor_safety) + 16;
+            bl.plc += (uint64_t)std::ceil(s.pred_lines * ctx->predictor_safety) + 64;
+            bl.pf += (uint64_t)std::ceil(s.pred_fills * ctx->predictor_safety) + 64;
+            bl.pa += (uint64_t)std::ceil(s.pred_alpha * ctx->predictor_safety) + 16;
         } else if (s.counted) {
-            plc += s.last_lines;
-            pf += s.last_fills;
-            pa += s.last_alpha;
+            bl.plc += s.last_lines;
+            bl.pf += s.last_fills;
+            bl.pa += s.last_alpha;
         } else {
             known = false;
         }
     }
     pl.cap_e = clamp_cap(std::max<uint64_t>(pl.T, 1));
-    for (int attempt = 0; attempt < 4; attempt++) {
-        if (!known) {
-            // size from an eager count pass (read back), as the plain pipeline does
-            Caps held = held_caps(ctx, dump);
-            pl.cap_l = held.l;
-            pl.cap_f = held.f;
-            pl.cap_a = held.a;
-            ensure_sizes(ctx, pl, dump);
-            upload_tables(ctx, pl);
-            BatchDev b = batch_dev(ctx, pl, dump);
-            stage_clear(ctx, b);
-            HIP_TRY(launch_flatten(b, false, ctx->stream));
-            HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, ctx->scan_tmp.as<uint32_t>(), ctx->stream));
-            read_counters(ctx);
-            plc = ctx->h_counters[C_LINES];
-            pl.cap_l = clamp_cap(std::max<uint64_t>(plc, pl.cap_l));
-            ensure_sizes(ctx, pl, dump);
-            b = batch_dev(ctx, pl, dump);
-            stage_clear(ctx, b);
-            stage_count(ctx, b);
-            read_counters(ctx);
-            pf = ctx->h_counters[C_FILLS];
-            pa = ctx->h_counters[C_ALPHA];
-            known = true;
-        }
-        Caps held = held_caps(ctx, dump);
-        pl.cap_l = clamp_cap(std::max<uint64_t>(plc, held.l));
-        pl.cap_f = clamp_cap(std::max<uint64_t>(pf, held.f));
-        pl.cap_a = clamp_cap(std::max<uint64_t>(pa, held.a));
-        ensure_sizes(ctx, pl, dump);
-        // ensure() rounds to 2^k: use the full allocation, then size the buffers that
-        // depend on it (scan scratch) for exactly those capacities, so that a repeat call
-        // asks for nothing new and replays its graph
-        for (int fix = 0; fix < 4; fix++) {
-            held = held_caps(ctx, dump);
-            if (fix > 0 && held.l == pl.cap_l && held.f == pl.cap_f && held.a == pl.cap_a) break;
-            pl.cap_l = held.l;
-            pl.cap_f = held.f;
-            pl.cap_a = held.a;
-            ensure_sizes(ctx, pl, dump);
-        }
-        upload_tables(ctx, pl);
-        BatchDev b = batch_dev(ctx, pl, dump);
-        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        if (use_graph) {
-            GraphKey key{pl.ids, flags & ~(uint32_t)PFH_PREDICT, pl.cap_f, pl.cap_a, pl.cap_e, pl.cap_l};
-            if (ctx->graphs_generation != g_alloc_generation) {
-                // some buffer moved since these graphs were captured: never replay them
-                for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
-                ctx->graphs.clear();
-                ctx->graphs_generation = g_alloc_generation;
-            }
-            auto it = ctx->graphs.find(key);
-            if (it == ctx->graphs.end()) {
-                if (ctx->graphs.size() > 32) {
-                    for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
-                    ctx->graphs.clear();
-                }
-                hipGraph_t graph;
-                HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                try {
-                    stage_clear(ctx, b);
-                    stage_count(ctx, b);
-                    stage_emit(ctx, b);
-                    stage_finish(ctx, b, pl);
-                } catch (...) {
-                    hipGraph_t dummy;
-                    (void)hipStreamEndCapture(ctx->stream, &dummy);
-                    throw;
-                }
-                HIP_TRY(hipStreamEndCapture(ctx->stream, &graph));
-                hipGraphExec_t exec;
-                HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                (void)hipGraphDestroy(graph);
-                it = ctx->graphs.emplace(key, exec).first;
-            } else {
-                // the sort's result buffer parity depends only on the key width
-                ctx->last_sorted_alt = ((pl.key_bits + 7) / 8) % 2 == 1;
-            }
-            HIP_TRY(hipGraphLaunch(it->second, ctx->stream));
-        } else {
-            stage_clear(ctx, b);
-            stage_count(ctx, b);
-            stage_emit(ctx, b);
-            stage_finish(ctx, b, pl);
-        }
-        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    if (!known) {
+        // size from an eager count pass (read back), as the plain pipeline does
+        Workspace &w = *ctx->w;
+        Caps held = held_caps(w, bl.dump);
+        pl.cap_l = held.l;
+        pl.cap_f = held.f;
+        pl.cap_a = held.a;
+        ensure_sizes(w, pl, bl.dump);
+        upload_tables(w, pl);
+        BatchDev b = batch_dev(w, pl, bl.dump);
+        stage_clear(w, b);
+        HIP_TRY(launch_flatten(b, false, w.stream));
+        HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, w.scan_tmp.as<uint32_t>(), w.stream));
         read_counters(ctx);
-        uint32_t nl = ctx->h_counters[C_LINES], nf = ctx->h_counters[C_FILLS], na = ctx->h_counters[C_ALPHA];
-        if (nl <= pl.cap_l && nf <= pl.cap_f && na <= pl.cap_a) {
-            ctx->stats.ms_total = elapsed_ms(ctx, ctx->ev0, ctx->ev1);
-            record_counts(ctx, pl);
-            for (int32_t id : pl.ids) {
-                SceneHost &s = get_scene(ctx, id);
-                s.dumped = dump;
-            }
-            return;
-        }
-        // under-prediction: a retry with the exact counts, never a wrong pixel.  When the
-        // chord buffer overflowed, the fill and alpha counts are partial: keep growing.
-        ctx->stats.retries++;
-        plc = std::max<uint64_t>(plc, nl);
-        pf = std::max<uint64_t>(pf, nl > pl.cap_l ? 2ull * nf : nf);
-        pa = std::max<uint64_t>(pa, nl > pl.cap_l ? 2ull * na : na);
+        bl.plc = w.h_counters[C_LINES];
+        pl.cap_l = clamp_cap(std::max<uint64_t>(bl.plc, pl.cap_l));
+        ensure_sizes(w, pl, bl.dump);
+        b = batch_dev(w, pl, bl.dump);
+        stage_clear(w, b);
+        stage_count(w, b);
+        read_counters(ctx);
+        bl.pf = w.h_counters[C_FILLS];
+        bl.pa = w.h_counters[C_ALPHA];
     }
-    throw PfhError{PFH_E_OVERFLOW, "capacity still exceeded after retries"};
 }
 
+// Enqueues one attempt on the current workspace's stream and the copy of its counters;
+// returns without waiting.  ev1 marks the end of the attempt.
+static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
+    Workspace &w = *ctx->w;
+    Plan &pl = bl.pl;
+    Caps held = held_caps(w, bl.dump);
+    pl.cap_l = clamp_cap(std::max<uint64_t>(bl.plc, held.l));
+    pl.cap_f = clamp_cap(std::max<uint64_t>(bl.pf, held.f));
+    pl.cap_a = clamp_cap(std::max<uint64_t>(bl.pa, held.a));
+    ensure_sizes(w, pl, bl.dump);
+    // ensure() rounds to 2^k: use the full allocation, then size the buffers that depend
+    // on it (scan scratch) for exactly those capacities, so that a repeat call asks for
+    // nothing new and replays its graph
+    for (int fix = 0; fix < 4; fix++) {
+        held = held_caps(w, bl.dump);
+        if (fix > 0 && held.l == pl.cap_l && held.f == pl.cap_f && held.a == pl.cap_a) break;
+        pl.cap_l = held.l;
+        pl.cap_f = held.f;
+        pl.cap_a = held.a;
+        ensure_sizes(w, pl, bl.dump);
+    }
+    upload_tables(w, pl);
+    BatchDev b = batch_dev(w, pl, bl.dump);
+    HIP_TRY(hipEventRecord(w.ev0, w.stream));
+    if (bl.use_graph) {
+        GraphKey key{pl.ids, bl.flags & ~(uint32_t)PFH_PREDICT, pl.cap_f, pl.cap_a, pl.cap_e, pl.cap_l};
+        if (w.graphs_generation != g_alloc_generation) {
+            // some buffer moved since these graphs were captured: never replay them
+            w.drop_graphs();
+            w.graphs_generation = g_alloc_generation;
+        }
+        auto it = w.graphs.find(key);
+        if (it == w.graphs.end()) {
+            if (w.graphs.size() > 32) w.drop_graphs();
+            hipGraph_t graph;
+            HIP_TRY(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                stage_clear(w, b);
+                stage_count(w, b);
+                stage_emit(w, b);
+                stage_finish(w, b, pl);
+            } catch (...) {
+                hipGraph_t dummy;
+                (void)hipStreamEndCapture(w.stream, &dummy);
+                throw;
+            }
+            HIP_TRY(hipStreamEndCapture(w.stream, &graph));
+            hipGraphExec_t exec;
+            HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            it = w.graphs.emplace(key, exec).first;
+        } else {
+            // the sort's result buffer parity depends only on the key width
+            w.last_sorted_alt = ((pl.key_bits + 7) / 8) % 2 == 1;
+        }
+        HIP_TRY(hipGraphLaunch(it->second, w.stream));
+    } else {
+        stage_clear(w, b);
+        stage_count(w, b);
+        stage_emit(w, b);
+        stage_finish(w, b, pl);
+    }
+    HIP_TRY(hipMemcpyAsync(w.h_counters, w.counters.ptr, 4 * C_COUNT, hipMemcpyDeviceToHost, w.stream));
+    HIP_TRY(hipEventRecord(w.ev1, w.stream));
+}
+
+// After the attempt's counters have arrived (ev1 complete): true when every capacity held;
+// otherwise the requested capacities grow to the exact counts for another attempt.
+static bool batched_check(pfh_ctx *ctx, BatchedLaunch &bl) {
+    Workspace &w = *ctx->w;
+    Plan &pl = bl.pl;
+    uint32_t nl = w.h_counters[C_LINES], nf = w.h_counters[C_FILLS], na = w.h_counters[C_ALPHA];
+    if (nl <= pl.cap_l && nf <= pl.cap_f && na <= pl.cap_a) return true;
+    // under-prediction: a retry with the exact counts, never a wrong pixel.  When the
+    // chord buffer overflowed, the fill and alpha counts are partial: keep growing.
+    ctx->stats.retries++;
+    bl.plc = std::max<uint64_t>(bl.plc, nl);
+    bl.pf = std::max<uint64_t>(bl.pf, nl > pl.cap_l ? 2ull * nf : nf);
+    bl.pa = std::max<uint64_t>(bl.pa, nl > pl.cap_l ? 2ull * na : na);
+    if (++bl.attempt >= 4) throw PfhError{PFH_E_OVERFLOW, "capacity still exceeded after retries"};
+    return false;
+}
+
+static void batched_accept(pfh_ctx *ctx, BatchedLaunch &bl) {
+    Workspace &w = *ctx->w;
+    ctx->stats.ms_total = elapsed_ms(ctx, w.ev0, w.ev1);
+    for (double &m : ctx->stats.ms_stage) m = 0;
+    ctx->stats.ms_stage[6] = elapsed_ms(ctx, w.kev[0], w.kev[1]);
+    ctx->stats.ms_stage[7] = elapsed_ms(ctx, w.kev[1], w.kev[2]);
+    record_counts(ctx, bl.pl);
+    for (int32_t id : bl.pl.ids) {
+        SceneHost &s = get_scene(ctx, id);
+        s.dumped = bl.dump;
+    }
+}
+
+static void rasterize_batched(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint32_t flags) {
+    BatchedLaunch bl;
+    batched_prepare(ctx, ids, n, flags, bl);
+    while (true) {
+        batched_enqueue(ctx, bl);
+        ctx->stats.host_syncs++;
+        HIP_TRY(hipEventSynchronize(ctx->w->ev1));
+        if (batched_check(ctx, bl)) break;
+    }
+    batched_accept(ctx, bl);
+}
+
+
+// ---------------------------------------------------------------------------
+// Request loop: continuous batching over several workspaces ("slots")
+// ---------------------------------------------------------------------------
+//
+// Requests name resident scenes.  Each free slot takes the longest prefix-compatible set of
+// queued requests whose predicted device bytes fit the memory budget beside the launches
+// already in flight, launches them as one batched pass on its own stream and returns to the
+// caller at once; requests that arrive meanwhile queue and join the next launch of whichever
+// slot frees first, so work of different launches overlaps on the device.  A scene is in at
+// most one launch at a time (its framebuffer is the launch's output).
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    int64_t id = 0;
+    int32_t scene = 0;
+    uint64_t bytes = 0;  // predicted device bytes of this scene's share of a launch
+    double t_submit = 0, t_start = 0;
+};
+
+struct Slot {
+    Workspace ws;
+    bool busy = false;
+    std::vector<Request> reqs;
+    BatchedLaunch bl;
+    uint64_t reserved = 0;
+};
+
+struct Server {
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::deque<Request> queue;
+    std::deque<int64_t> done;  // completed, not yet returned by pfh_server_poll
+    std::vector<double> latency_ms;
+    std::vector<int32_t> scene_busy;  // per scene id: in a launch
+    uint64_t budget = 0, reserved = 0, peak_reserved = 0;
+    int32_t max_batch = 64;
+    uint32_t flags = 0;
+    int64_t next_id = 1, submitted = 0, completed = 0, launches = 0, retries = 0, launched_requests = 0;
+    int64_t max_concurrent = 0;
+};
+
+// Device bytes one scene adds to a launch, from its resident sizes and its predicted (or last
+// exact) chord / fill / alpha-tile counts; per-element sizes are those ensure_sizes asks for.
+static uint64_t scene_launch_bytes(const pfh_ctx *ctx, const SceneHost &s) {
+    double L, F, A;
+    if (!ctx->predictor.empty() && s.pred_lines >= 0) {
+        L = s.pred_lines * ctx->predictor_safety;
+        F = s.pred_fills * ctx->predictor_safety;
+        A = s.pred_alpha * ctx->predictor_safety;
+    } else if (s.counted) {
+        L = s.last_lines;
+        F = s.last_fills;
+        A = s.last_alpha;
+    } else {  // unknown: generous bounds from the resident sizes
+        L = 16.0 * s.n_segs;
+        F = 2.0 * s.n_tiles + L;
+        A = s.n_tiles;
+    }
+    const double T = s.n_tiles, W = s.n_cols, S = s.n_segs, FB = (double)s.fb_w * s.fb_h;
+    const double per_tile = 4 + 4 + 8 + 1 + 8 + 4 * 4 + 16;  // counts, offsets, backdrop, entries, sort, records
+    double b = T * per_tile + 4 * W + 8 * S + 8 * FB + 24 * L + 8 * F + (16 + 256) * A;
+    return (uint64_t)b + 4096;
+}
+
+static Server &server_of(pfh_ctx *ctx) {
+    if (!ctx->server) throw PfhError{PFH_E_STATE, "request loop not started (pfh_server_start)"};
+    return *ctx->server;
+}
+
+static void drop_all_graphs(pfh_ctx *ctx) {
+    ctx->main.drop_graphs();
+    if (ctx->server)
+        for (auto &sl : ctx->server->slots) sl->ws.drop_graphs();
+}
+
+// One turn of the loop: retire finished launches (or re-launch them after an overflow),
+// then fill free slots from the queue.
+static void server_pump(pfh_ctx *ctx) {
+    Server &sv = server_of(ctx);
+    struct Restore {
+        pfh_ctx *c;
+        ~Restore() { c->w = &c->main; }
+    } restore{ctx};
+    for (auto &slp : sv.slots) {
+        Slot &sl = *slp;
+        if (!sl.busy) continue;
+        hipError_t q = hipEventQuery(sl.ws.ev1);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) throw PfhError{PFH_E_HIP, std::string("request launch: ") + hipGetErrorString(q)};
+        ctx->w = &sl.ws;
+        if (!batched_check(ctx, sl.bl)) {
+            sv.retries++;
+            batched_enqueue(ctx, sl.bl);
+            continue;
+        }
+        double t = now_s();
+        for (const Request &r : sl.reqs) {
+            sv.latency_ms.push_back((t - r.t_submit) * 1e3);
+            sv.done.push_back(r.id);
+            sv.scene_busy[r.scene] = 0;
+        }
+        sv.completed += (int64_t)sl.reqs.size();
+        sl.reqs.clear();
+        sv.reserved -= sl.reserved;
+        sl.reserved = 0;
+        sl.busy = false;
+    }
+    ctx->w = &ctx->main;
+    for (auto &slp : sv.slots) {
+        Slot &sl = *slp;
+        if (sl.busy || sv.queue.empty()) continue;
+        bool others_busy = false;
+        for (auto &o : sv.slots) others_busy |= o->busy;
+        std::vector<Request> batch;
+        uint64_t bytes = 0;
+        for (auto it = sv.queue.begin(); it != sv.queue.end() && (int32_t)batch.size() < sv.max_batch;) {
+            bool fits = sv.reserved + bytes + it->bytes <= sv.budget || (batch.empty() && !others_busy);
+            bool dup = sv.scene_busy[it->scene] != 0;
+            for (const Request &r : batch) dup |= r.scene == it->scene;
+            if (!fits) break;  // keep arrival order for admission
+            if (dup) {
+                ++it;
+                continue;
+            }
+            bytes += it->bytes;
+            batch.push_back(*it);
+            it = sv.queue.erase(it);
+        }
+        if (batch.empty()) break;
+        std::vector<int32_t> ids;
+        double t = now_s();
+        for (Request &r : batch) {
+            ids.push_back(r.scene);
+            r.t_start = t;
+            sv.scene_busy[r.scene] = 1;
+        }
+        ctx->w = &sl.ws;
+        batched_prepare(ctx, ids.data(), (int32_t)ids.size(), sv.flags | PFH_BATCHED, sl.bl);
+        batched_enqueue(ctx, sl.bl);
+        ctx->w = &ctx->main;
+        sl.reqs = std::move(batch);
+        sl.reserved = bytes;
+        sl.busy = true;
+        sv.reserved += bytes;
+        sv.peak_reserved = std::max(sv.peak_reserved, sv.reserved);
+        sv.launches++;
+        sv.launched_requests += (int64_t)ids.size();
+        int64_t busy = 0;
+        for (auto &o : sv.slots) busy += o->busy;
+        sv.max_concurrent = std::max(sv.max_concurrent, busy);
+    }
+}
+
+static bool server_idle(const Server &sv) {
+    if (!sv.queue.empty()) return false;
+    for (auto &sl : sv.slots)
+        if (sl->busy) return false;
+    return true;
+}
+
+static void server_destroy(pfh_ctx *ctx) {
+    if (!ctx->server) return;
+    for (auto &sl : ctx->server->slots) sl->ws.destroy();
+    ctx->server.reset();
+}
+
+static double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0;
+    size_t k = (size_t)std::min<double>(v.size() - 1, std::floor(q * (v.size() - 1) + 0.5));
+    std::nth_element(v.begin(), v.begin() + k, v.end());
+    return v[k];
+}
 
 static int run_guarded(pfh_ctx *ctx, const std::function<void()> &f) {
     try {
@@ -920,10 +1229,7 @@ int pfh_create(int device, pfh_ctx **out) {
         HIP_TRY(hipGetDeviceCount(&n));
         if (device < 0 || device >= n) throw PfhError{PFH_E_ARG, "no HIP device " + std::to_string(device)};
         HIP_TRY(hipSetDevice(device));
-        HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreate(&ctx->ev0));
-        HIP_TRY(hipEventCreate(&ctx->ev1));
-        HIP_TRY(hipHostMalloc((void **)&ctx->h_counters, 4 * C_COUNT));
+        ctx->main.create();
     });
     if (rc != PFH_OK) {
         fprintf(stderr, "pfh_create: %s\n", ctx->err.c_str());
@@ -937,27 +1243,15 @@ int pfh_create(int device, pfh_ctx **out) {
 void pfh_destroy(pfh_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    server_destroy(ctx);
+    ctx->main.destroy();
     for (auto &s : ctx->scenes) {
         if (!s) continue;
         for (DevBuf *b : {&s->d_features, &s->d_pts, &s->d_segs, &s->d_seg_path, &s->d_path_rect, &s->d_path_tile_off,
-                          &s->d_path_col_off, &s->d_col_path, &s->d_path_rgba, &s->d_path_rule, &s->d_out_owned})
+                          &s->d_path_col_off, &s->d_col_path, &s->d_path_rgba, &s->d_path_rule, &s->d_path_seg_off, &s->d_bin_paths, &s->d_out_owned})
             b->release();
     }
-    for (DevBuf *b : {&ctx->scene_table, &ctx->bases, &ctx->tile_delta, &ctx->tile_cnt, &ctx->tile_fill_off,
-                      &ctx->tile_alpha_idx, &ctx->tile_bd, &ctx->tile_entry, &ctx->col_bd, &ctx->seg_cnt, &ctx->lines,
-                      &ctx->line_seg, &ctx->line_cnt, &ctx->fills,
-                      &ctx->dump_fills, &ctx->alpha_rec, &ctx->masks, &ctx->zbuf, &ctx->fb_cnt, &ctx->keys, &ctx->vals,
-                      &ctx->keys_alt, &ctx->vals_alt, &ctx->entry_rec, &ctx->counters, &ctx->scan_tmp, &ctx->sort_tmp,
-                      &ctx->mlp_params, &ctx->mlp_x, &ctx->mlp_h, &ctx->mlp_out})
-        b->release();
-    if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
-    if (ctx->h_table) (void)hipHostFree(ctx->h_table);
-    if (ctx->h_bases) (void)hipHostFree(ctx->h_bases);
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    for (DevBuf *b : {&ctx->mlp_params, &ctx->mlp_x, &ctx->mlp_h, &ctx->mlp_out}) b->release();
     delete ctx;
 }
 
@@ -972,11 +1266,17 @@ int pfh_scene_release(pfh_ctx *ctx, int32_t scene_id) {
     if (!ctx) return PFH_E_ARG;
     return run_guarded(ctx, [&] {
         SceneHost &s = get_scene(ctx, scene_id);
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
-        ctx->graphs.clear();
+        if (ctx->server) {
+            Server &sv = *ctx->server;
+            bool queued = false;
+            for (const Request &r : sv.queue) queued |= r.scene == scene_id;
+            if (queued || ((size_t)scene_id < sv.scene_busy.size() && sv.scene_busy[scene_id]))
+                throw PfhError{PFH_E_STATE, "scene has requests in the request loop"};
+        }
+        HIP_TRY(hipStreamSynchronize(ctx->w->stream));
+        drop_all_graphs(ctx);
         for (DevBuf *b : {&s.d_features, &s.d_pts, &s.d_segs, &s.d_seg_path, &s.d_path_rect, &s.d_path_tile_off, &s.d_path_col_off,
-                          &s.d_col_path, &s.d_path_rgba, &s.d_path_rule, &s.d_out_owned})
+                          &s.d_col_path, &s.d_path_rgba, &s.d_path_rule, &s.d_path_seg_off, &s.d_bin_paths, &s.d_out_owned})
             b->release();
         ctx->scenes[scene_id].reset();
     });
@@ -1003,11 +1303,11 @@ int pfh_rasterize(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fl
             rasterize_batched(ctx, scene_ids, n, flags);
         } else {
             for (double &m : ctx->stats.ms_stage) m = 0;
-            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+            HIP_TRY(hipEventRecord(ctx->w->ev0, ctx->w->stream));
             for (int32_t i = 0; i < n; i++) rasterize_plain(ctx, scene_ids[i], flags);
-            HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-            HIP_TRY(hipEventSynchronize(ctx->ev1));
-            ctx->stats.ms_total = elapsed_ms(ctx, ctx->ev0, ctx->ev1);
+            HIP_TRY(hipEventRecord(ctx->w->ev1, ctx->w->stream));
+            HIP_TRY(hipEventSynchronize(ctx->w->ev1));
+            ctx->stats.ms_total = elapsed_ms(ctx, ctx->w->ev0, ctx->w->ev1);
         }
     });
 }
@@ -1024,8 +1324,7 @@ int pfh_scene_set_target(pfh_ctx *ctx, int32_t scene_id, void *device_rgba8, int
         if (pitch_bytes < (int64_t)s.width * 4 || (pitch_bytes & 3)) throw PfhError{PFH_E_ARG, "bad pitch"};
         s.d_out = reinterpret_cast<uint8_t *>(device_rgba8);
         s.out_pitch = pitch_bytes;
-        for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
-        ctx->graphs.clear();
+        drop_all_graphs(ctx);
     });
 }
 
@@ -1043,7 +1342,7 @@ int pfh_readback_rgba8(pfh_ctx *ctx, int32_t scene_id, void *dst, int64_t pitch_
     return run_guarded(ctx, [&] {
         SceneHost &s = get_scene(ctx, scene_id);
         if (pitch_bytes < (int64_t)s.width * 4) throw PfhError{PFH_E_ARG, "bad pitch"};
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->w->stream));
         HIP_TRY(hipMemcpy2D(dst, (size_t)pitch_bytes, s.d_out, (size_t)s.out_pitch, (size_t)s.width * 4,
                             (size_t)s.height, hipMemcpyDeviceToHost));
     });
@@ -1054,17 +1353,17 @@ int pfh_dump_counts(pfh_ctx *ctx, int32_t scene_id, int64_t *counts) {
     return run_guarded(ctx, [&] {
         SceneHost &s = get_scene(ctx, scene_id);
         if (!s.dumped) throw PfhError{PFH_E_STATE, "scene was not rasterized with PFH_DUMP"};
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->w->stream));
         uint32_t f0 = 0, f1 = 0, a0 = 0, a1 = 0, e0 = 0, e1 = 0, l0 = 0, l1 = 0;
         uint32_t sb = s.seg_base, tb = s.tile_base, fb = s.fb_base, nfb = (uint32_t)(s.fb_w * s.fb_h);
-        HIP_TRY(hipMemcpy(&l0, ctx->seg_cnt.as<uint32_t>() + sb, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&l1, ctx->seg_cnt.as<uint32_t>() + sb + s.n_segs, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&f0, ctx->line_cnt.as<uint32_t>() + l0, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&f1, ctx->line_cnt.as<uint32_t>() + l1, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&a0, ctx->tile_alpha_idx.as<uint32_t>() + tb, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&a1, ctx->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&e0, ctx->fb_cnt.as<uint32_t>() + fb, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&e1, ctx->fb_cnt.as<uint32_t>() + fb + nfb, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&l0, ctx->w->seg_cnt.as<uint32_t>() + sb, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&l1, ctx->w->seg_cnt.as<uint32_t>() + sb + s.n_segs, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&f0, ctx->w->line_cnt.as<uint32_t>() + l0, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&f1, ctx->w->line_cnt.as<uint32_t>() + l1, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&a0, ctx->w->tile_alpha_idx.as<uint32_t>() + tb, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&a1, ctx->w->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&e0, ctx->w->fb_cnt.as<uint32_t>() + fb, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&e1, ctx->w->fb_cnt.as<uint32_t>() + fb + nfb, 4, hipMemcpyDeviceToHost));
         counts[0] = s.n_tiles;
         counts[1] = s.n_cols;
         counts[2] = f1 - f0;
@@ -1072,7 +1371,7 @@ int pfh_dump_counts(pfh_ctx *ctx, int32_t scene_id, int64_t *counts) {
         counts[4] = nfb;
         counts[5] = s.n_paths;
         counts[6] = a1 - a0;
-        counts[7] = ctx->h_counters[C_GUARD];
+        counts[7] = ctx->w->h_counters[C_GUARD];
     });
 }
 
@@ -1084,7 +1383,7 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
     return run_guarded(ctx, [&] {
         SceneHost &s = get_scene(ctx, scene_id);
         if (!s.dumped) throw PfhError{PFH_E_STATE, "scene was not rasterized with PFH_DUMP"};
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->w->stream));
         uint32_t sb = s.seg_base, tb = s.tile_base, cb = s.col_base, fbb = s.fb_base;
         uint32_t nfb = (uint32_t)(s.fb_w * s.fb_h);
         auto d2h = [](void *dst, const void *src, size_t bytes) {
@@ -1093,10 +1392,10 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
         // chords of a path start at its first segment's chord offset; fills of a chord at
         // that chord's scanned fill offset
         std::vector<uint32_t> seg_line(s.n_segs + 1);
-        d2h(seg_line.data(), ctx->seg_cnt.as<uint32_t>() + sb, 4 * (s.n_segs + 1));
+        d2h(seg_line.data(), ctx->w->seg_cnt.as<uint32_t>() + sb, 4 * (s.n_segs + 1));
         uint32_t l0 = seg_line[0], l1 = seg_line[s.n_segs];
         std::vector<uint32_t> line_fill(l1 - l0 + 1);
-        d2h(line_fill.data(), ctx->line_cnt.as<uint32_t>() + l0, 4 * (size_t)(l1 - l0 + 1));
+        d2h(line_fill.data(), ctx->w->line_cnt.as<uint32_t>() + l0, 4 * (size_t)(l1 - l0 + 1));
         for (uint32_t p = 0; p < s.n_paths; p++) std::memcpy(path_rect + 4 * p, &s.path_rect[p], 16);
         for (uint32_t p = 0; p <= s.n_paths; p++) {
             tile_off[p] = s.path_tile_off[p];
@@ -1104,14 +1403,14 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
             uint32_t first_line = seg_line[s.path_seg_off[p]] - l0;
             fill_off[p] = (int64_t)line_fill[first_line] - line_fill[0];
         }
-        d2h(tile_backdrop, ctx->tile_bd.as<int8_t>() + tb, s.n_tiles);
+        d2h(tile_backdrop, ctx->w->tile_bd.as<int8_t>() + tb, s.n_tiles);
         std::vector<uint32_t> foff(s.n_tiles + 1);
-        d2h(foff.data(), ctx->tile_fill_off.as<uint32_t>() + tb, 4 * (s.n_tiles + 1));
+        d2h(foff.data(), ctx->w->tile_fill_off.as<uint32_t>() + tb, 4 * (s.n_tiles + 1));
         for (uint32_t t = 0; t < s.n_tiles; t++) tile_alpha[t] = foff[t + 1] > foff[t] ? 1 : 0;
-        d2h(col_sum, ctx->col_bd.as<int32_t>() + cb, 4 * s.n_cols);
+        d2h(col_sum, ctx->w->col_bd.as<int32_t>() + cb, 4 * s.n_cols);
         uint32_t nf = line_fill[l1 - l0] - line_fill[0];
         std::vector<uint4> df(nf);
-        d2h(df.data(), ctx->dump_fills.as<uint4>() + line_fill[0], 16 * (size_t)nf);
+        d2h(df.data(), ctx->w->dump_fills.as<uint4>() + line_fill[0], 16 * (size_t)nf);
         uint32_t p = 0;
         for (uint32_t i = 0; i < nf; i++) {
             while (p + 1 <= s.n_paths && (int64_t)i >= fill_off[p + 1]) p++;
@@ -1122,10 +1421,10 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
             fill_tile[i] = df[i].z - tb - s.path_tile_off[p];
         }
         std::vector<uint32_t> fboff(nfb + 1);
-        d2h(fboff.data(), ctx->fb_cnt.as<uint32_t>() + fbb, 4 * (nfb + 1));
+        d2h(fboff.data(), ctx->w->fb_cnt.as<uint32_t>() + fbb, 4 * (nfb + 1));
         uint32_t ne = fboff[nfb] - fboff[0];
         std::vector<uint32_t> sv(ne);
-        const uint32_t *sorted = ctx->last_sorted_alt ? ctx->vals_alt.as<uint32_t>() : ctx->vals.as<uint32_t>();
+        const uint32_t *sorted = ctx->w->last_sorted_alt ? ctx->w->vals_alt.as<uint32_t>() : ctx->w->vals.as<uint32_t>();
         d2h(sv.data(), sorted + fboff[0], 4 * (size_t)ne);
         uint32_t elo = 0xffffffffu, ehi = 0;
         for (uint32_t v : sv) {
@@ -1134,22 +1433,22 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
         }
         if (ne) {
             std::vector<uint4> recs(ehi - elo);
-            d2h(recs.data(), ctx->entry_rec.as<uint4>() + elo, 16 * (size_t)(ehi - elo));
+            d2h(recs.data(), ctx->w->entry_rec.as<uint4>() + elo, 16 * (size_t)(ehi - elo));
             for (uint32_t i = 0; i < ne; i++) {
                 entry_fbtile[i] = recs[sv[i] - elo].w;
                 entry_path[i] = recs[sv[i] - elo].z;
             }
         }
-        d2h(zbuf, ctx->zbuf.as<uint32_t>() + fbb, 4 * (size_t)nfb);
+        d2h(zbuf, ctx->w->zbuf.as<uint32_t>() + fbb, 4 * (size_t)nfb);
         fb_rect[0] = s.fb_x0;
         fb_rect[1] = s.fb_y0;
         fb_rect[2] = s.fb_x0 + s.fb_w;
         fb_rect[3] = s.fb_y0 + s.fb_h;
         if (masks) {
             uint32_t a0 = 0, a1 = 0;
-            d2h(&a0, ctx->tile_alpha_idx.as<uint32_t>() + tb, 4);
-            d2h(&a1, ctx->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4);
-            d2h(masks, ctx->masks.as<uint8_t>() + 256ull * a0, 256ull * (a1 - a0));
+            d2h(&a0, ctx->w->tile_alpha_idx.as<uint32_t>() + tb, 4);
+            d2h(&a1, ctx->w->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4);
+            d2h(masks, ctx->w->masks.as<uint8_t>() + 256ull * a0, 256ull * (a1 - a0));
         }
     });
 }
@@ -1207,16 +1506,16 @@ int pfh_path_counts(pfh_ctx *ctx, int32_t scene_id, int32_t *lines, int32_t *fil
     return run_guarded(ctx, [&] {
         SceneHost &s = get_scene(ctx, scene_id);
         if (!s.dumped) throw PfhError{PFH_E_STATE, "scene was not rasterized with PFH_DUMP"};
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->w->stream));
         std::vector<uint32_t> seg_line(s.n_segs + 1);
-        HIP_TRY(hipMemcpy(seg_line.data(), ctx->seg_cnt.as<uint32_t>() + s.seg_base, 4 * (s.n_segs + 1),
+        HIP_TRY(hipMemcpy(seg_line.data(), ctx->w->seg_cnt.as<uint32_t>() + s.seg_base, 4 * (s.n_segs + 1),
                           hipMemcpyDeviceToHost));
         uint32_t l0 = seg_line[0], l1 = seg_line[s.n_segs];
         std::vector<uint32_t> line_fill(l1 - l0 + 1);
-        HIP_TRY(hipMemcpy(line_fill.data(), ctx->line_cnt.as<uint32_t>() + l0, 4 * (size_t)(l1 - l0 + 1),
+        HIP_TRY(hipMemcpy(line_fill.data(), ctx->w->line_cnt.as<uint32_t>() + l0, 4 * (size_t)(l1 - l0 + 1),
                           hipMemcpyDeviceToHost));
         std::vector<uint32_t> fill_off(s.n_tiles + 1);
-        HIP_TRY(hipMemcpy(fill_off.data(), ctx->tile_fill_off.as<uint32_t>() + s.tile_base, 4 * (s.n_tiles + 1),
+        HIP_TRY(hipMemcpy(fill_off.data(), ctx->w->tile_fill_off.as<uint32_t>() + s.tile_base, 4 * (s.n_tiles + 1),
                           hipMemcpyDeviceToHost));
         for (uint32_t p = 0; p < s.n_paths; p++) {
             uint32_t a = seg_line[s.path_seg_off[p]] - l0, e = seg_line[s.path_seg_off[p + 1]] - l0;
@@ -1248,8 +1547,8 @@ int pfh_mlp_eval(pfh_ctx *ctx, const float *params, const float *x, int64_t n, f
         HIP_TRY(hipMemcpy(d.params.ptr, params, 4 * np, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.x.ptr, x, 4 * n * 12, hipMemcpyHostToDevice));
         HIP_TRY(mlp_forward(d.params.as<float>(), d.x.as<float>(), n, d.h.as<float>(), d.out.as<float>(),
-                            ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+                            ctx->w->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->w->stream));
         HIP_TRY(hipMemcpy(out, d.out.ptr, 4 * n * 3, hipMemcpyDeviceToHost));
         for (DevBuf *bf : {&d.params, &d.x, &d.h, &d.out}) bf->release();
     });
@@ -1260,9 +1559,9 @@ struct MlpDev {
     DevBuf params, m, v, grad, loss, x, t, h, out, scratch;
 };
 static void mlp_grad_dev(pfh_ctx *ctx, MlpDev &d, const float *x, const float *t, int64_t n) {
-    HIP_TRY(mlp_forward(d.params.as<float>(), x, n, d.h.as<float>(), d.out.as<float>(), ctx->stream));
+    HIP_TRY(mlp_forward(d.params.as<float>(), x, n, d.h.as<float>(), d.out.as<float>(), ctx->w->stream));
     HIP_TRY(mlp_backward(d.params.as<float>(), x, n, d.h.as<float>(), d.out.as<float>(), t, d.grad.as<float>(),
-                         d.loss.as<float>(), d.scratch.as<float>(), ctx->stream));
+                         d.loss.as<float>(), d.scratch.as<float>(), ctx->w->stream));
 }
 }  // namespace
 
@@ -1284,7 +1583,7 @@ int pfh_mlp_grad(pfh_ctx *ctx, const float *params, const float *x, const float 
         HIP_TRY(hipMemcpy(d.x.ptr, x, 4 * n * 12, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.t.ptr, target, 4 * n * 3, hipMemcpyHostToDevice));
         mlp_grad_dev(ctx, d, d.x.as<float>(), d.t.as<float>(), n);
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->w->stream));
         HIP_TRY(hipMemcpy(grad, d.grad.ptr, 4 * np, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(loss, d.loss.ptr, 4, hipMemcpyDeviceToHost));
         for (DevBuf *bf : {&d.params, &d.grad, &d.loss, &d.x, &d.t, &d.h, &d.out, &d.scratch}) bf->release();
@@ -1336,22 +1635,123 @@ int pfh_mlp_train(pfh_ctx *ctx, const float *x, const float *target, int64_t n, 
             for (int64_t b0 = 0; b0 + bs <= n; b0 += bs) {
                 mlp_grad_dev(ctx, d, d.x.as<float>() + b0 * 12, d.t.as<float>() + b0 * 3, bs);
                 HIP_TRY(mlp_adam(d.params.as<float>(), d.m.as<float>(), d.v.as<float>(), d.grad.as<float>(), np, lr,
-                                 ++step, ctx->stream));
+                                 ++step, ctx->w->stream));
             }
             if (loss_history) {
                 HIP_TRY(mlp_forward(d.params.as<float>(), d.x.as<float>(), n, d.h.as<float>(), d.out.as<float>(),
-                                    ctx->stream));
+                                    ctx->w->stream));
                 HIP_TRY(mlp_backward(d.params.as<float>(), d.x.as<float>(), n, d.h.as<float>(), d.out.as<float>(),
                                      d.t.as<float>(), d.grad.as<float>(), d.loss.as<float>(), d.scratch.as<float>(),
-                                     ctx->stream));
-                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                                     ctx->w->stream));
+                HIP_TRY(hipStreamSynchronize(ctx->w->stream));
                 HIP_TRY(hipMemcpy(&loss_history[ep], d.loss.ptr, 4, hipMemcpyDeviceToHost));
             }
         }
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->w->stream));
         HIP_TRY(hipMemcpy(params, d.params.ptr, 4 * np, hipMemcpyDeviceToHost));
         for (DevBuf *bf : {&d.params, &d.m, &d.v, &d.grad, &d.loss, &d.x, &d.t, &d.h, &d.out, &d.scratch})
             bf->release();
+    });
+}
+
+int pfh_server_start(pfh_ctx *ctx, int32_t slots, int64_t memory_budget_bytes, int32_t max_batch, uint32_t flags) {
+    if (!ctx) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        if (ctx->server) throw PfhError{PFH_E_STATE, "request loop already started"};
+        if (slots < 1 || slots > 3) throw PfhError{PFH_E_ARG, "slots must be 1..3 (one HIP stream each)"};
+        if (memory_budget_bytes <= 0 || max_batch < 1) throw PfhError{PFH_E_ARG, "budget and max_batch must be positive"};
+        if (flags & (PFH_DUMP | PFH_LOAD_DST)) throw PfhError{PFH_E_ARG, "PFH_DUMP / PFH_LOAD_DST are not served"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        auto sv = std::make_unique<Server>();
+        sv->budget = (uint64_t)memory_budget_bytes;
+        sv->max_batch = max_batch;
+        sv->flags = flags & (PFH_PREDICT | PFH_NO_GRAPH);
+        for (int32_t i = 0; i < slots; i++) {
+            sv->slots.push_back(std::make_unique<Slot>());
+            sv->slots.back()->ws.create();
+        }
+        ctx->server = std::move(sv);
+    });
+}
+
+int pfh_server_submit(pfh_ctx *ctx, int32_t scene_id, int64_t *request_id) {
+    if (!ctx || !request_id) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        Server &sv = server_of(ctx);
+        SceneHost &s = get_scene(ctx, scene_id);
+        if (sv.scene_busy.size() < ctx->scenes.size()) sv.scene_busy.resize(ctx->scenes.size(), 0);
+        Request r;
+        r.id = sv.next_id++;
+        r.scene = scene_id;
+        r.bytes = scene_launch_bytes(ctx, s);
+        r.t_submit = now_s();
+        sv.queue.push_back(r);
+        sv.submitted++;
+        *request_id = r.id;
+    });
+}
+
+int pfh_server_poll(pfh_ctx *ctx, double wait_ms, int64_t *done_ids, int32_t capacity, int32_t *n_done) {
+    if (!ctx || !n_done || capacity < 0 || (capacity > 0 && !done_ids)) return PFH_E_ARG;
+    *n_done = 0;
+    return run_guarded(ctx, [&] {
+        Server &sv = server_of(ctx);
+        HIP_TRY(hipSetDevice(ctx->device));
+        const double until = now_s() + std::max(0.0, wait_ms) * 1e-3;
+        while (true) {
+            server_pump(ctx);
+            if (!sv.done.empty() || server_idle(sv) || now_s() >= until) break;
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+        int32_t k = 0;
+        while (k < capacity && !sv.done.empty()) {
+            done_ids[k++] = sv.done.front();
+            sv.done.pop_front();
+        }
+        *n_done = k;
+    });
+}
+
+int pfh_server_stats_get(pfh_ctx *ctx, pfh_server_stats *out) {
+    if (!ctx || !out) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        Server &sv = server_of(ctx);
+        pfh_server_stats st{};
+        st.submitted = sv.submitted;
+        st.completed = sv.completed;
+        st.queued = (int64_t)sv.queue.size();
+        for (auto &sl : sv.slots) st.in_flight += (int64_t)sl->reqs.size();
+        st.launches = sv.launches;
+        st.retries = sv.retries;
+        st.max_concurrent_launches = sv.max_concurrent;
+        st.mean_batch = sv.launches ? (double)sv.launched_requests / (double)sv.launches : 0.0;
+        st.latency_ms_p50 = percentile(sv.latency_ms, 0.50);
+        st.latency_ms_p99 = percentile(sv.latency_ms, 0.99);
+        double sum = 0, mx = 0;
+        for (double v : sv.latency_ms) {
+            sum += v;
+            mx = std::max(mx, v);
+        }
+        st.latency_ms_mean = sv.latency_ms.empty() ? 0.0 : sum / (double)sv.latency_ms.size();
+        st.latency_ms_max = mx;
+        st.reserved_bytes = (int64_t)sv.reserved;
+        st.peak_reserved_bytes = (int64_t)sv.peak_reserved;
+        st.budget_bytes = (int64_t)sv.budget;
+        for (auto &sl : sv.slots) st.device_bytes += (int64_t)sl->ws.device_bytes();
+        *out = st;
+    });
+}
+
+int pfh_server_stop(pfh_ctx *ctx) {
+    if (!ctx) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        if (!ctx->server) return;
+        HIP_TRY(hipSetDevice(ctx->device));
+        while (!server_idle(*ctx->server)) {
+            server_pump(ctx);
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+        server_destroy(ctx);
     });
 }
 

@@ -51,10 +51,24 @@ class Stats(ctypes.Structure):
         return d
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("queued", ctypes.c_int64),
+                ("in_flight", ctypes.c_int64), ("launches", ctypes.c_int64), ("retries", ctypes.c_int64),
+                ("max_concurrent_launches", ctypes.c_int64), ("mean_batch", ctypes.c_double),
+                ("latency_ms_p50", ctypes.c_double), ("latency_ms_p99", ctypes.c_double),
+                ("latency_ms_mean", ctypes.c_double), ("latency_ms_max", ctypes.c_double),
+                ("reserved_bytes", ctypes.c_int64), ("peak_reserved_bytes", ctypes.c_int64),
+                ("budget_bytes", ctypes.c_int64), ("device_bytes", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EXPORTS = ("pfh_create", "pfh_destroy", "pfh_last_error", "pfh_scene_upload", "pfh_scene_release",
            "pfh_scene_features", "pfh_rasterize", "pfh_scene_set_target", "pfh_scene_target", "pfh_readback_rgba8",
            "pfh_dump_counts", "pfh_dump", "pfh_stats_get", "pfh_set_predictor", "pfh_scene_prediction", "pfh_path_counts",
-           "pfh_mlp_eval", "pfh_mlp_grad", "pfh_mlp_train")
+           "pfh_mlp_eval", "pfh_mlp_grad", "pfh_mlp_train", "pfh_server_start", "pfh_server_submit",
+           "pfh_server_poll", "pfh_server_stats_get", "pfh_server_stop")
 
 _lib = None
 
@@ -91,6 +105,11 @@ def load(path: str = LIB_PATH):
         "pfh_mlp_eval": (ctypes.c_int, [_P, _P, _P, i64, _P]),
         "pfh_mlp_grad": (ctypes.c_int, [_P, _P, _P, _P, i64, _P, _P]),
         "pfh_mlp_train": (ctypes.c_int, [_P, _P, _P, i64, i32, i32, f32, ctypes.c_uint64, _P, _P]),
+        "pfh_server_start": (ctypes.c_int, [_P, i32, i64, i32, ctypes.c_uint32]),
+        "pfh_server_submit": (ctypes.c_int, [_P, i32, ctypes.POINTER(i64)]),
+        "pfh_server_poll": (ctypes.c_int, [_P, ctypes.c_double, _P, i32, ctypes.POINTER(i32)]),
+        "pfh_server_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
+        "pfh_server_stop": (ctypes.c_int, [_P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -202,6 +221,31 @@ class Context:
         self._check(self.lib.pfh_dump(self.h, sid, *[_ptr(out[k]) for k in keys]), "pfh_dump")
         out["guard_hits"] = int(counts[7])
         return out
+
+    # request loop (continuous batching)
+    def server_start(self, slots: int = 2, memory_budget: int = 8 << 30, max_batch: int = 64,
+                     flags: int = PFH_PREDICT | PFH_NO_GRAPH):
+        self._check(self.lib.pfh_server_start(self.h, slots, int(memory_budget), max_batch, flags), "pfh_server_start")
+
+    def submit(self, sid: int) -> int:
+        rid = ctypes.c_int64()
+        self._check(self.lib.pfh_server_submit(self.h, sid, ctypes.byref(rid)), "pfh_server_submit")
+        return rid.value
+
+    def poll(self, wait_ms: float = 0.0, capacity: int = 1024) -> list:
+        ids = np.zeros(capacity, np.int64)
+        n = ctypes.c_int32()
+        self._check(self.lib.pfh_server_poll(self.h, float(wait_ms), _ptr(ids), capacity, ctypes.byref(n)),
+                    "pfh_server_poll")
+        return [int(x) for x in ids[:n.value]]
+
+    def server_stats(self) -> dict:
+        st = ServerStats()
+        self._check(self.lib.pfh_server_stats_get(self.h, ctypes.byref(st)), "pfh_server_stats_get")
+        return st.as_dict()
+
+    def server_stop(self):
+        self._check(self.lib.pfh_server_stop(self.h), "pfh_server_stop")
 
     def set_predictor(self, weights, safety: float = 1.25):
         if weights is None:

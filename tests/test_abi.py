@@ -3,6 +3,7 @@
 import ctypes
 import os
 import re
+import subprocess
 
 import pfh
 
@@ -30,3 +31,24 @@ def test_create_rejects_bad_device_without_crashing():
     assert lib.pfh_create(-1, ctypes.byref(h)) != 0
     assert not h.value
     assert lib.pfh_rasterize(None, None, 0, 0) == -1
+
+
+def test_struct_layouts_match_header(tmp_path):
+    # sizes and field offsets of the ABI structs, as a C compiler lays them out from the header
+    fields = {"pfh_stats": pfh.Stats, "pfh_server_stats": pfh.ServerStats, "pfh_scene_desc": pfh.SceneDesc}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "pfh.h"', "int main(void) {"]
+    for cname, py in fields.items():
+        lines.append(f'    printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'    printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.splitlines())
+    for cname, py in fields.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"
