@@ -153,6 +153,23 @@ def cpu_baseline(scenes, budget_s=15.0, threads=4):
             "host_nproc": os.cpu_count()}
 
 
+def copy_calibration(mbytes):
+    """One device-to-device copy of a known size through the HIP runtime, so that rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE of the runtime's copy kernel calibrate those counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mbytes) << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copied {n} bytes device to device", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +179,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="main workload only (no single_4k / serving)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-mb", type=int, default=0, help="also run one D2D copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: scenes shard across GPUs with no exchange (see DESIGN.md); "
@@ -223,6 +241,8 @@ def main():
         print("[bench] serving done", flush=True)
 
     out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(scenes, budget_s=args.cpu_budget)
+    if args.copy_mb:
+        copy_calibration(args.copy_mb)
     print(json.dumps(out), flush=True)
     ctx.close()
 

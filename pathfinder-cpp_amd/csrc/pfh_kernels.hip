@@ -263,11 +263,21 @@ struct Walker {
 // core/data/segment.cpp:42-126 split_cubic / split_quadratic at t = 0.5.  The explicit
 // stack holds pending right halves without their start point: a pending half starts
 // exactly where the previously flattened piece ended (splits keep their endpoints
-// bit-for-bit), so only (c0, c1, p1, depth) are stored.
+// bit-for-bit), so only (c0, c1, p1, depth) are stored.  The stack's first
+// FLAT_LDS_LEVELS levels live in LDS, level-major with one float per lane per row so the
+// lanes of a wave hit distinct banks; deeper levels (the benchmark generators never reach
+// them: curves there flatten within depth 6) fall back to a private array.  A private
+// stack alone is placed in scratch, where a divergent wave's narrow stores make the
+// cache write back whole lines: several times the chord output in HBM writes.
+constexpr int FLAT_LDS_LEVELS = 6;
+
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_flatten(BatchDev b) {
+    __shared__ float fs[FLAT_LDS_LEVELS][6][256];
+    __shared__ uint8_t fd[FLAT_LDS_LEVELS][256];
+    const uint32_t tid = threadIdx.x;
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= b.n_segs) return;
+    if (g >= b.n_segs) return;  // no barriers below: each lane uses only its own LDS column
     int k = find_scene(b.seg_bases, b.n_scenes, g);
     const SceneDev &sc = b.scenes[k];
     uint32_t s_local = g - sc.seg_base;
@@ -304,8 +314,8 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b) {
             }
             n = 1;
         } else if (finite) {
-            float stk[MAX_FLATTEN_DEPTH + 1][6];
-            int sdepth[MAX_FLATTEN_DEPTH + 1];
+            float stk[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS][6];
+            int sdepth[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS];
             int sp = 0, d = 0;
             const bool quad = sg.y == SEG_QUAD;
             while (true) {
@@ -320,10 +330,18 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b) {
                     if (sp == 0) break;
                     sp--;
                     s.p0 = s.p1;
-                    s.c0 = make_float2(stk[sp][0], stk[sp][1]);
-                    s.c1 = make_float2(stk[sp][2], stk[sp][3]);
-                    s.p1 = make_float2(stk[sp][4], stk[sp][5]);
-                    d = sdepth[sp];
+                    if (sp < FLAT_LDS_LEVELS) {
+                        s.c0 = make_float2(fs[sp][0][tid], fs[sp][1][tid]);
+                        s.c1 = make_float2(fs[sp][2][tid], fs[sp][3][tid]);
+                        s.p1 = make_float2(fs[sp][4][tid], fs[sp][5][tid]);
+                        d = fd[sp][tid];
+                    } else {
+                        const int q = sp - FLAT_LDS_LEVELS;
+                        s.c0 = make_float2(stk[q][0], stk[q][1]);
+                        s.c1 = make_float2(stk[q][2], stk[q][3]);
+                        s.p1 = make_float2(stk[q][4], stk[q][5]);
+                        d = sdepth[q];
+                    }
                     continue;
                 }
                 const float t = 0.5f;
@@ -339,13 +357,24 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b) {
                     a = Seg4{s.p0, p01, p012, p0123};
                     bseg = Seg4{p0123, p123, p23, s.p1};
                 }
-                stk[sp][0] = bseg.c0.x;
-                stk[sp][1] = bseg.c0.y;
-                stk[sp][2] = bseg.c1.x;
-                stk[sp][3] = bseg.c1.y;
-                stk[sp][4] = bseg.p1.x;
-                stk[sp][5] = bseg.p1.y;
-                sdepth[sp] = d + 1;
+                if (sp < FLAT_LDS_LEVELS) {
+                    fs[sp][0][tid] = bseg.c0.x;
+                    fs[sp][1][tid] = bseg.c0.y;
+                    fs[sp][2][tid] = bseg.c1.x;
+                    fs[sp][3][tid] = bseg.c1.y;
+                    fs[sp][4][tid] = bseg.p1.x;
+                    fs[sp][5][tid] = bseg.p1.y;
+                    fd[sp][tid] = (uint8_t)(d + 1);
+                } else {
+                    const int q = sp - FLAT_LDS_LEVELS;
+                    stk[q][0] = bseg.c0.x;
+                    stk[q][1] = bseg.c0.y;
+                    stk[q][2] = bseg.c1.x;
+                    stk[q][3] = bseg.c1.y;
+                    stk[q][4] = bseg.p1.x;
+                    stk[q][5] = bseg.p1.y;
+                    sdepth[q] = d + 1;
+                }
                 sp++;
                 s = a;
                 d++;
@@ -614,13 +643,14 @@ __device__ T block_exclusive_scan(T v, T &total) {
     return base + excl_in_wave;
 }
 
+// Order does not matter for the block sum: each pass reads one coalesced row of 256 items.
 template <typename T, typename Load>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Load ld, uint32_t n, T *block_sums) {
-    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
     T s = zero_of<T>();
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) {
-        uint64_t j = base + i;
+        const uint64_t j = base + (uint64_t)i * SCAN_THREADS;
         if (j < n) s = s + ld((uint32_t)j);
     }
     T total;
@@ -643,25 +673,45 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(T *block_sums, uin
     if (threadIdx.x == 0) block_sums[nb] = carry;
 }
 
+// Item s of a tile sits at LDS word s + s / 16: striped rows (lane-consecutive) and blocked
+// runs (16 per thread, threads 17 words apart) are both bank-conflict free.
+__device__ __forceinline__ uint32_t scan_pad(uint32_t s) { return s + (s >> 4); }
+
+// Loads and stores are striped (one coalesced row of 256 items at a time) and transposed
+// through LDS to the blocked order the per-thread running sum needs.
 template <typename T, typename Load, typename Store>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Load ld, Store st, uint32_t n, const T *block_sums,
                                                              uint32_t nb) {
-    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    __shared__ T tile[SCAN_TILE + SCAN_TILE / 16];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint32_t si = (uint32_t)i * SCAN_THREADS + tid;
+        const uint64_t j = base + si;
+        tile[scan_pad(si)] = j < n ? ld((uint32_t)j) : zero_of<T>();
+    }
+    __syncthreads();
     T vals[SCAN_ITEMS];
     T s = zero_of<T>();
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) {
-        uint64_t j = base + i;
-        vals[i] = j < n ? ld((uint32_t)j) : zero_of<T>();
+        vals[i] = tile[scan_pad(tid * SCAN_ITEMS + (uint32_t)i)];
         s = s + vals[i];
     }
     T total;
-    T run = block_exclusive_scan<T>(s, total) + block_sums[blockIdx.x];
+    T run = block_exclusive_scan<T>(s, total) + block_sums[blockIdx.x];  // its barriers order the reads above
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) {
-        uint64_t j = base + i;
-        if (j < n) st((uint32_t)j, run);
+        tile[scan_pad(tid * SCAN_ITEMS + (uint32_t)i)] = run;
         run = run + vals[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint32_t si = (uint32_t)i * SCAN_THREADS + tid;
+        const uint64_t j = base + si;
+        if (j < n) st((uint32_t)j, tile[scan_pad(si)]);
     }
     if (blockIdx.x == nb - 1 && threadIdx.x == 0) st(n, block_sums[nb]);
 }

@@ -27,6 +27,7 @@ namespace pfh {
 // 16 tiles per 256-thread block; each lane owns its column in LDS, so no atomics.
 
 constexpr int FILL_TILES_PER_BLOCK = 16;
+constexpr int MASK_PREFETCH = 8;  // composite: coverage masks loaded ahead of their blend
 
 __device__ __forceinline__ float avg_clamp01(float ua, float ub) {
     float a = fminf(ua, ub), bb = fmaxf(ua, ub);
@@ -264,37 +265,60 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         const uint64_t alpha_mask = valid_mask & ~solid_mask;
         const uint64_t run_end = solid_mask & ~(solid_mask >> 1);
         uint64_t events = alpha_mask | run_end;
+        const uint32_t *mask_words = reinterpret_cast<const uint32_t *>(b.masks);
         while (events) {
-            const int j = __builtin_ctzll(events);
-            events &= events - 1;
-            if ((alpha_mask >> j) & 1ull) {
-                const uint32_t aj = __builtin_amdgcn_readlane(ai, j);
-                const uint32_t cj = __builtin_amdgcn_readlane(c, j);
-                const float ja = (float)(cj >> 24) * (1.0f / 255.0f);
-                if (ja == 0.0f) continue;
-                const float jr = (float)(cj & 255u) * (1.0f / 255.0f);
-                const float jg = (float)((cj >> 8) & 255u) * (1.0f / 255.0f);
-                const float jb = (float)((cj >> 16) & 255u) * (1.0f / 255.0f);
-                const uint32_t m = reinterpret_cast<const uint32_t *>(b.masks + (uint64_t)aj * 256)[lane];
+            // Issue the masks of the next MASK_PREFETCH alpha events together (their loads are
+            // in flight at once), then blend every event up to the last of them in order.
+            uint32_t mq[MASK_PREFETCH];
+            uint64_t am = events & alpha_mask;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const float sa = ja * (float)((m >> (8 * q)) & 255u) * (1.0f / 255.0f);
-                    const float k1 = 1.0f - sa;
-                    r[q] = jr * sa + r[q] * k1;
-                    g[q] = jg * sa + g[q] * k1;
-                    bl[q] = jb * sa + bl[q] * k1;
-                    al[q] = sa + al[q] * k1;
+            for (int u = 0; u < MASK_PREFETCH; u++) {
+                mq[u] = 0;
+                if (am) {
+                    const int j = __builtin_ctzll(am);
+                    am &= am - 1;
+                    const uint32_t aj = __builtin_amdgcn_readlane(ai, j);
+                    const uint32_t cj = __builtin_amdgcn_readlane(c, j);
+                    if ((cj >> 24) != 0u) mq[u] = mask_words[(uint64_t)aj * 64 + lane];
                 }
-            } else {
-                const float fr = __shfl(pr, j, 64), fg = __shfl(pg, j, 64), fb = __shfl(pb, j, 64),
-                            fa = __shfl(pa, j, 64);
-                const float k1 = 1.0f - fa;
+            }
+            // events before the first alpha event not prefetched (all of them if none is left)
+            const uint64_t upto = am ? ((1ull << __builtin_ctzll(am)) - 1ull) : ~0ull;
+            uint64_t chunk = events & upto;
+            events &= ~upto;
+            while (chunk) {
+                const int j = __builtin_ctzll(chunk);
+                chunk &= chunk - 1;
+                if ((alpha_mask >> j) & 1ull) {
+                    const uint32_t m = mq[0];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    r[q] = fr + r[q] * k1;
-                    g[q] = fg + g[q] * k1;
-                    bl[q] = fb + bl[q] * k1;
-                    al[q] = fa + al[q] * k1;
+                    for (int u = 0; u + 1 < MASK_PREFETCH; u++) mq[u] = mq[u + 1];
+                    const uint32_t cj = __builtin_amdgcn_readlane(c, j);
+                    const float ja = (float)(cj >> 24) * (1.0f / 255.0f);
+                    if (ja == 0.0f) continue;
+                    const float jr = (float)(cj & 255u) * (1.0f / 255.0f);
+                    const float jg = (float)((cj >> 8) & 255u) * (1.0f / 255.0f);
+                    const float jb = (float)((cj >> 16) & 255u) * (1.0f / 255.0f);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const float sa = ja * (float)((m >> (8 * q)) & 255u) * (1.0f / 255.0f);
+                        const float k1 = 1.0f - sa;
+                        r[q] = jr * sa + r[q] * k1;
+                        g[q] = jg * sa + g[q] * k1;
+                        bl[q] = jb * sa + bl[q] * k1;
+                        al[q] = sa + al[q] * k1;
+                    }
+                } else {
+                    const float fr = __shfl(pr, j, 64), fg = __shfl(pg, j, 64), fb = __shfl(pb, j, 64),
+                                fa = __shfl(pa, j, 64);
+                    const float k1 = 1.0f - fa;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        r[q] = fr + r[q] * k1;
+                        g[q] = fg + g[q] * k1;
+                        bl[q] = fb + bl[q] * k1;
+                        al[q] = fa + al[q] * k1;
+                    }
                 }
             }
         }
