@@ -659,15 +659,27 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Load ld, uint32_t 
 }
 
 // Single block: exclusive scan of nb block sums in place; block_sums[nb] = grand total.
+// Each thread takes SCAN_ITEMS consecutive sums per round (the array is small and stays in
+// cache), so a round covers SCAN_TILE sums.
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(T *block_sums, uint32_t nb) {
     T carry = zero_of<T>();
-    for (uint32_t start = 0; start < nb; start += SCAN_THREADS) {
-        uint32_t i = start + threadIdx.x;
-        T v = i < nb ? block_sums[i] : zero_of<T>();
+    for (uint32_t start = 0; start < nb; start += SCAN_TILE) {
+        const uint32_t first = start + threadIdx.x * SCAN_ITEMS;
+        T vals[SCAN_ITEMS];
+        T s = zero_of<T>();
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            vals[i] = first + i < nb ? block_sums[first + i] : zero_of<T>();
+            s = s + vals[i];
+        }
         T total;
-        T ex = block_exclusive_scan<T>(v, total);
-        if (i < nb) block_sums[i] = carry + ex;
+        T run = carry + block_exclusive_scan<T>(s, total);
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            if (first + i < nb) block_sums[first + i] = run;
+            run = run + vals[i];
+        }
         carry = carry + total;
     }
     if (threadIdx.x == 0) block_sums[nb] = carry;

@@ -23,11 +23,15 @@ namespace pfh {
 // difference array and one prefix sum give the coverage.  Accumulation is fixed
 // point (2^-20): the result does not depend on the order fills arrive in.
 //
-// Layout: 16 lanes per alpha tile (one per pixel column), 4 tiles per wave,
-// 16 tiles per 256-thread block; each lane owns its column in LDS, so no atomics.
+// Layout: 16 lanes per alpha tile (one per pixel column), 4 tiles per wave, 16 tiles per
+// 256-thread block.  A tile's difference array is column-major in LDS (each column's 17
+// rows padded to 20 words, so a lane zeroes and reads its column with 16-byte accesses),
+// and the coverage mask is stored column-major too: lane c writes column c's 16 bytes with
+// one 16-byte store (byte 16 c + r is pixel (c, r)).
 
 constexpr int FILL_TILES_PER_BLOCK = 16;
-constexpr int MASK_PREFETCH = 8;  // composite: coverage masks loaded ahead of their blend
+constexpr int FILL_COL_WORDS = 20;                          // rows 0..16 used
+constexpr int FILL_GROUP_WORDS = 16 * FILL_COL_WORDS + 8;   // pad between tiles of a wave
 
 __device__ __forceinline__ float avg_clamp01(float ua, float ub) {
     float a = fminf(ua, ub), bb = fmaxf(ua, ub);
@@ -39,14 +43,19 @@ __device__ __forceinline__ float avg_clamp01(float ua, float ub) {
     return (full + mid) * __builtin_amdgcn_rcpf(du);
 }
 
-__device__ __forceinline__ float apply_rule(float c, uint32_t even_odd) {
-    if (!even_odd) return fminf(fabsf(c), 1.0f);
-    float m = c - 2.0f * floorf(c * 0.5f);  // GLSL mod(c, 2)
-    return fminf(fmaxf(1.0f - fabsf(1.0f - m), 0.0f), 1.0f);
+
+// Inclusive prefix sum within each 16-lane row of the wave (DPP row shifts: no LDS).
+__device__ __forceinline__ uint32_t row16_inclusive_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    return v;
 }
 
+
 // Adds one fill's contribution to pixel column `col` of a tile's difference array
-// (column stride 1, row stride 16 words); LDS adds, since several lanes may share a column.
+// (FILL_COL_WORDS words per column); LDS adds, since several lanes may share a column.
 __device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, int32_t *diff) {
     const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), y0 = (float)(fl.x >> 16) * (1.0f / 256.0f);
     const float x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f), y1 = (float)(fl.y >> 16) * (1.0f / 256.0f);
@@ -64,113 +73,130 @@ __device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, 
     for (int r = r0; r <= r1; r++) {
         const float rb = (float)(r + 1);
         const int32_t v = (int32_t)rintf(sign * w * avg_clamp01(rb - ya, rb - yb) * scale);
-        atomicAdd(diff + r * 16 + col, v - prev);
+        atomicAdd(diff + col * FILL_COL_WORDS + r, v - prev);
         prev = v;
     }
-    atomicAdd(diff + (r1 + 1) * 16 + col, (int32_t)rintf(sign * w * scale) - prev);
+    atomicAdd(diff + col * FILL_COL_WORDS + r1 + 1, (int32_t)rintf(sign * w * scale) - prev);
 }
 
-__device__ __forceinline__ uint32_t wave_max_of_groups(uint32_t v) {
-    // v is uniform within each 16-lane group: the wave maximum is the max of lanes 0, 16, 32, 48
-    uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
-    uint32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
-    return max(max(a, b), max(c, d));
+
+// Inclusive prefix sum across the 64-lane wave: DPP within rows, then the row totals.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    v = row16_inclusive_sum(v);
+    const uint32_t t0 = __builtin_amdgcn_readlane(v, 15), t1 = __builtin_amdgcn_readlane(v, 31);
+    const uint32_t t2 = __builtin_amdgcn_readlane(v, 47);
+    const uint32_t row = (threadIdx.x & 63) >> 4;
+    return v + (row == 0 ? 0u : (row == 1 ? t0 : (row == 2 ? t0 + t1 : t0 + t1 + t2)));
 }
 
-// A 16-lane group per alpha tile.  Fills arrive 16 at a time (one per lane); each
-// fill covers a run of pixel columns, and the group's (fill, column) pairs are
-// enumerated with a prefix sum so every lane works on a pair that contributes.
+// Coverage of one pixel from its fixed-point winding (2^-FIX_SHIFT units) under the fill rule
+// (shaders/d3d11/fill.comp:134-140), as an 8-bit value rounded to nearest.
+__device__ __forceinline__ uint32_t coverage_byte(int32_t w, uint32_t even_odd) {
+    const int32_t one = 1 << FIX_SHIFT;
+    const int32_t nz = min(abs(w), one);
+    const int32_t m = w & (2 * one - 1);  // w mod 2 (two's complement: also for negative w)
+    const int32_t eo = one - abs(one - m);
+    const uint32_t c = (uint32_t)(even_odd ? eo : nz);
+    return (c * 255u + (uint32_t)(one >> 1)) >> FIX_SHIFT;
+}
+
+// A wave takes 4 consecutive alpha tiles (16 lanes, one pixel column each, per tile).  Their
+// fills form one pooled list of up to 64 per chunk; the (fill, pixel column) pairs of a
+// chunk are enumerated with a wave-wide prefix sum, so every lane works on a pair that
+// contributes whatever the mix of light and heavy tiles in the wave.
 __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
-    // 17 rows per tile slot (272 words): tile slots g and g+1 land on opposite bank halves
-    __shared__ int32_t diff[FILL_TILES_PER_BLOCK][17][16];
-    __shared__ uint2 sfill[FILL_TILES_PER_BLOCK][16];
-    __shared__ uint32_t scum[FILL_TILES_PER_BLOCK][16];
-    __shared__ uint32_t out8[FILL_TILES_PER_BLOCK][64];
-    const int g = threadIdx.x >> 4;   // tile slot in the block
-    const int li = threadIdx.x & 15;  // lane in the tile's group (= pixel column for the output)
+    __shared__ __align__(16) int32_t diff[FILL_TILES_PER_BLOCK * FILL_GROUP_WORDS];
+    __shared__ uint2 sfill[4][64];
+    __shared__ uint32_t scum[4][64];
+    __shared__ uint32_t sslot[4][64];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t slot = lane >> 4, li = lane & 15;  // tile slot in the wave, pixel column
+    int32_t *dwave = diff + wv * 4 * FILL_GROUP_WORDS;
+    int32_t *dcol = dwave + slot * FILL_GROUP_WORDS + li * FILL_COL_WORDS;
     const uint32_t na = min(b.counters[C_ALPHA], b.cap_alpha);
+    const float scale = (float)(1 << FIX_SHIFT);
     // grid-stride over groups of 16 tiles: the grid is sized from the capacity, the count
     // is only known on the device; the loop bound is uniform over the block
     for (uint32_t first = blockIdx.x * FILL_TILES_PER_BLOCK; first < na; first += gridDim.x * FILL_TILES_PER_BLOCK) {
-    const uint32_t a = first + g;
-    const bool valid = a < na;
-    // all lanes stay to the end: loops below have wave-uniform bounds and use shuffles
+        const uint32_t a = first + wv * 4 + slot;
+        const bool valid = a < na;
 #pragma unroll
-    for (int r = 0; r < 17; r++) diff[g][r][li] = 0;
-    uint4 rec = make_uint4(0, 0, 0, 0);
-    if (valid) rec = b.alpha_rec[a];
-    // After a fill-capacity overflow a tile's range can start past the buffer (rec.x >=
-    // cap_fills): clamp both ends so the count never wraps (the batch is then retried).
-    const uint32_t f0 = valid ? min(rec.x, b.cap_fills) : 0u;
-    const uint32_t f1 = valid ? min(rec.y, b.cap_fills) : 0u;
-    const uint32_t nf = f1 > f0 ? f1 - f0 : 0u;
-    const float scale = (float)(1 << FIX_SHIFT);
-    const uint32_t chunks = wave_max_of_groups((nf + 15) >> 4);
-    for (uint32_t ch = 0; ch < chunks; ch++) {
-        const uint32_t i = ch * 16 + li;
-        uint2 fl = make_uint2(0, 0);
-        uint32_t cnt = 0;
-        if (i < nf) {
-            fl = b.fills[f0 + i];
-            const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f);
-            const int c0 = (int)floorf(fminf(x0, x1)), c1 = (int)ceilf(fmaxf(x0, x1)) - 1;
-            cnt = (uint32_t)max(0, c1 - c0 + 1);
+        for (int k = 0; k < FILL_COL_WORDS / 4; k++) reinterpret_cast<uint4 *>(dcol)[k] = make_uint4(0, 0, 0, 0);
+        const uint4 rec = valid ? b.alpha_rec[a] : make_uint4(0, 0, 0, 0);
+        // After a fill-capacity overflow a tile's range can start past the buffer: clamp both
+        // ends so no count wraps (the batch is then retried).
+        const uint32_t f0 = min(rec.x, b.cap_fills);
+        const uint32_t f1 = max(f0, min(rec.y, b.cap_fills));
+        // the 4 tiles' ranges (uniform over the wave), concatenated
+        const uint32_t s0 = __builtin_amdgcn_readlane(f0, 0), n0 = __builtin_amdgcn_readlane(f1, 0) - s0;
+        const uint32_t s1 = __builtin_amdgcn_readlane(f0, 16), n1 = __builtin_amdgcn_readlane(f1, 16) - s1;
+        const uint32_t s2 = __builtin_amdgcn_readlane(f0, 32), n2 = __builtin_amdgcn_readlane(f1, 32) - s2;
+        const uint32_t s3 = __builtin_amdgcn_readlane(f0, 48), n3 = __builtin_amdgcn_readlane(f1, 48) - s3;
+        const uint32_t c1 = n0, c2 = c1 + n1, c3 = c2 + n2, ntot = c3 + n3;
+        for (uint32_t base = 0; base < ntot; base += 64) {
+            const uint32_t k = base + lane;
+            uint2 fl = make_uint2(0, 0);
+            uint32_t cnt = 0, fs = 0;
+            if (k < ntot) {
+                fs = (k >= c1 ? 1u : 0u) + (k >= c2 ? 1u : 0u) + (k >= c3 ? 1u : 0u);
+                const uint32_t addr = fs == 0 ? s0 + k : (fs == 1 ? s1 + (k - c1) : (fs == 2 ? s2 + (k - c2) : s3 + (k - c3)));
+                fl = b.fills[addr];
+                const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f);
+                const int cl = (int)floorf(fminf(x0, x1)), cr = (int)ceilf(fmaxf(x0, x1)) - 1;
+                cnt = (uint32_t)max(0, cr - cl + 1);
+            }
+            const uint32_t incl = wave_inclusive_sum(cnt);
+            const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+            sfill[wv][lane] = fl;
+            scum[wv][lane] = incl;
+            sslot[wv][lane] = fs;
+            // the zeroed columns (first chunk) and this chunk's fills are visible to the wave
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t q0 = 0; q0 < total; q0 += 64) {
+                const uint32_t q = q0 + lane;
+                if (q < total) {
+                    // the fill owning pair q: first j with scum[j] > q
+                    uint32_t j = 0;
+#pragma unroll
+                    for (uint32_t st = 32; st > 0; st >>= 1)
+                        if (scum[wv][j + st - 1] <= q) j += st;
+                    const uint2 fj = sfill[wv][j];
+                    const uint32_t before = j ? scum[wv][j - 1] : 0u;
+                    const float x0 = (float)(fj.x & 0xffffu) * (1.0f / 256.0f), x1 = (float)(fj.y & 0xffffu) * (1.0f / 256.0f);
+                    const int col = (int)floorf(fminf(x0, x1)) + (int)(q - before);
+                    accumulate_fill(fj, col, scale, dwave + sslot[wv][j] * FILL_GROUP_WORDS);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        uint32_t incl = cnt;
-#pragma unroll
-        for (int d = 1; d < 16; d <<= 1) {
-            const uint32_t o = (uint32_t)__shfl_up((int)incl, d, 16);
-            if (li >= d) incl += o;
+        if (ntot == 0) {  // nothing accumulated: the zeroing still has to land first
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        const uint32_t total = (uint32_t)__shfl((int)incl, 15, 16);
-        sfill[g][li] = fl;
-        scum[g][li] = incl;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t rounds = wave_max_of_groups((total + 15) >> 4);
-        for (uint32_t rd = 0; rd < rounds; rd++) {
-            const uint32_t q = rd * 16 + li;
-            if (q < total) {
-                // the fill owning pair q: first j with scum[j] > q
-                int j = 0;
+        const uint32_t rule = (rec.z >> 8) & 1u;
+        int32_t run = (int32_t)(int8_t)(rec.z & 0xffu) * (1 << FIX_SHIFT);  // backdrop, exact
+        uint32_t packed[4] = {0, 0, 0, 0};
 #pragma unroll
-                for (int st = 8; st > 0; st >>= 1)
-                    if (scum[g][j + st - 1] <= q) j += st;
-                const uint2 fj = sfill[g][j];
-                const uint32_t before = j ? scum[g][j - 1] : 0u;
-                const float x0 = (float)(fj.x & 0xffffu) * (1.0f / 256.0f), x1 = (float)(fj.y & 0xffffu) * (1.0f / 256.0f);
-                const int col = (int)floorf(fminf(x0, x1)) + (int)(q - before);
-                accumulate_fill(fj, col, scale, &diff[g][0][0]);
+        for (int k = 0; k < 4; k++) {
+            const uint4 v = reinterpret_cast<const uint4 *>(dcol)[k];
+            const int32_t d[4] = {(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                run += d[r];
+                packed[k] |= coverage_byte(run, rule) << (8 * r);
             }
         }
+        if (valid)
+            reinterpret_cast<uint4 *>(b.masks + (uint64_t)a * 256)[li] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+        // the next group of tiles reuses this wave's LDS
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    const float bd = (float)(int8_t)(rec.z & 0xffu);
-    const uint32_t rule = (rec.z >> 8) & 1u;
-    const float inv = 1.0f / scale;
-    int32_t run = 0;
-    uint8_t *bytes = reinterpret_cast<uint8_t *>(&out8[g][0]);
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        run += diff[g][r][li];
-        const float c = bd + (float)run * inv;
-        bytes[r * 16 + li] = (uint8_t)rintf(apply_rule(c, rule) * 255.0f);
-    }
-    // each lane stores one 16-byte row of its tile: 256 B per tile, coalesced
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (valid) {
-        const uint4 row = reinterpret_cast<const uint4 *>(&out8[g][0])[li];
-        reinterpret_cast<uint4 *>(b.masks + (uint64_t)a * 256)[li] = row;
-    }
-    // the next group reuses this slot's LDS
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -204,15 +230,16 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     const SceneDev &sc = b.scenes[k];
     const uint32_t fl = f - sc.fb_base;
     const int tx = sc.fb_x0 + (int)(fl % (uint32_t)sc.fb_w), ty = sc.fb_y0 + (int)(fl / (uint32_t)sc.fb_w);
-    const int row = lane >> 2, col = (lane & 3) * 4;
-    const int y = ty * TILE + row, x0 = tx * TILE + col;
+    // a lane owns 4 vertically adjacent pixels: column lane / 4, rows 4 (lane % 4) .. + 3,
+    // matching the column-major coverage masks (one 32-bit mask word per lane)
+    const int x = tx * TILE + (lane >> 2), y0 = ty * TILE + (lane & 3) * 4;
     float r[4], g[4], bl[4], al[4];
-    if (sc.load_dst && y < sc.height) {
+    if (sc.load_dst && x < sc.width) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             uint32_t px = 0;
-            if (x0 + j < sc.width)
-                px = *reinterpret_cast<const uint32_t *>(sc.out + (int64_t)y * sc.out_pitch + (int64_t)(x0 + j) * 4);
+            if (y0 + j < sc.height)
+                px = *reinterpret_cast<const uint32_t *>(sc.out + (int64_t)(y0 + j) * sc.out_pitch + (int64_t)x * 4);
             r[j] = (float)(px & 255u) * (1.0f / 255.0f);
             g[j] = (float)((px >> 8) & 255u) * (1.0f / 255.0f);
             bl[j] = (float)((px >> 16) & 255u) * (1.0f / 255.0f);
@@ -265,66 +292,42 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         const uint64_t alpha_mask = valid_mask & ~solid_mask;
         const uint64_t run_end = solid_mask & ~(solid_mask >> 1);
         uint64_t events = alpha_mask | run_end;
-        const uint32_t *mask_words = reinterpret_cast<const uint32_t *>(b.masks);
         while (events) {
-            // Issue the masks of the next MASK_PREFETCH alpha events together (their loads are
-            // in flight at once), then blend every event up to the last of them in order.
-            uint32_t mq[MASK_PREFETCH];
-            uint64_t am = events & alpha_mask;
+            const int j = __builtin_ctzll(events);
+            events &= events - 1;
+            if ((alpha_mask >> j) & 1ull) {
+                const uint32_t aj = __builtin_amdgcn_readlane(ai, j);
+                const uint32_t cj = __builtin_amdgcn_readlane(c, j);
+                const float ja = (float)(cj >> 24) * (1.0f / 255.0f);
+                if (ja == 0.0f) continue;
+                const float jr = (float)(cj & 255u) * (1.0f / 255.0f);
+                const float jg = (float)((cj >> 8) & 255u) * (1.0f / 255.0f);
+                const float jb = (float)((cj >> 16) & 255u) * (1.0f / 255.0f);
+                const uint32_t m = reinterpret_cast<const uint32_t *>(b.masks + (uint64_t)aj * 256)[lane];
 #pragma unroll
-            for (int u = 0; u < MASK_PREFETCH; u++) {
-                mq[u] = 0;
-                if (am) {
-                    const int j = __builtin_ctzll(am);
-                    am &= am - 1;
-                    const uint32_t aj = __builtin_amdgcn_readlane(ai, j);
-                    const uint32_t cj = __builtin_amdgcn_readlane(c, j);
-                    if ((cj >> 24) != 0u) mq[u] = mask_words[(uint64_t)aj * 64 + lane];
+                for (int q = 0; q < 4; q++) {
+                    const float sa = ja * (float)((m >> (8 * q)) & 255u) * (1.0f / 255.0f);
+                    const float k1 = 1.0f - sa;
+                    r[q] = jr * sa + r[q] * k1;
+                    g[q] = jg * sa + g[q] * k1;
+                    bl[q] = jb * sa + bl[q] * k1;
+                    al[q] = sa + al[q] * k1;
                 }
-            }
-            // events before the first alpha event not prefetched (all of them if none is left)
-            const uint64_t upto = am ? ((1ull << __builtin_ctzll(am)) - 1ull) : ~0ull;
-            uint64_t chunk = events & upto;
-            events &= ~upto;
-            while (chunk) {
-                const int j = __builtin_ctzll(chunk);
-                chunk &= chunk - 1;
-                if ((alpha_mask >> j) & 1ull) {
-                    const uint32_t m = mq[0];
+            } else {
+                const float fr = __shfl(pr, j, 64), fg = __shfl(pg, j, 64), fb = __shfl(pb, j, 64),
+                            fa = __shfl(pa, j, 64);
+                const float k1 = 1.0f - fa;
 #pragma unroll
-                    for (int u = 0; u + 1 < MASK_PREFETCH; u++) mq[u] = mq[u + 1];
-                    const uint32_t cj = __builtin_amdgcn_readlane(c, j);
-                    const float ja = (float)(cj >> 24) * (1.0f / 255.0f);
-                    if (ja == 0.0f) continue;
-                    const float jr = (float)(cj & 255u) * (1.0f / 255.0f);
-                    const float jg = (float)((cj >> 8) & 255u) * (1.0f / 255.0f);
-                    const float jb = (float)((cj >> 16) & 255u) * (1.0f / 255.0f);
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const float sa = ja * (float)((m >> (8 * q)) & 255u) * (1.0f / 255.0f);
-                        const float k1 = 1.0f - sa;
-                        r[q] = jr * sa + r[q] * k1;
-                        g[q] = jg * sa + g[q] * k1;
-                        bl[q] = jb * sa + bl[q] * k1;
-                        al[q] = sa + al[q] * k1;
-                    }
-                } else {
-                    const float fr = __shfl(pr, j, 64), fg = __shfl(pg, j, 64), fb = __shfl(pb, j, 64),
-                                fa = __shfl(pa, j, 64);
-                    const float k1 = 1.0f - fa;
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        r[q] = fr + r[q] * k1;
-                        g[q] = fg + g[q] * k1;
-                        bl[q] = fb + bl[q] * k1;
-                        al[q] = fa + al[q] * k1;
-                    }
+                for (int q = 0; q < 4; q++) {
+                    r[q] = fr + r[q] * k1;
+                    g[q] = fg + g[q] * k1;
+                    bl[q] = fb + bl[q] * k1;
+                    al[q] = fa + al[q] * k1;
                 }
             }
         }
     }
-    if (y >= sc.height) return;
-    uint8_t *orow = sc.out + (int64_t)y * sc.out_pitch;
+    if (x >= sc.width) return;
     uint32_t px[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -334,13 +337,9 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         const uint32_t pa = (uint32_t)rintf(fminf(fmaxf(al[q], 0.0f), 1.0f) * 255.0f);
         px[q] = pr | (pg << 8) | (pb << 16) | (pa << 24);
     }
-    if (x0 + 3 < sc.width && ((reinterpret_cast<uintptr_t>(orow) & 15) == 0)) {
-        reinterpret_cast<uint4 *>(orow)[x0 >> 2] = make_uint4(px[0], px[1], px[2], px[3]);
-    } else {
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            if (x0 + q < sc.width) reinterpret_cast<uint32_t *>(orow)[x0 + q] = px[q];
-    }
+    for (int q = 0; q < 4; q++)
+        if (y0 + q < sc.height) *reinterpret_cast<uint32_t *>(sc.out + (int64_t)(y0 + q) * sc.out_pitch + (int64_t)x * 4) = px[q];
 }
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }

@@ -1449,6 +1449,14 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
             d2h(&a0, ctx->w->tile_alpha_idx.as<uint32_t>() + tb, 4);
             d2h(&a1, ctx->w->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4);
             d2h(masks, ctx->w->masks.as<uint8_t>() + 256ull * a0, 256ull * (a1 - a0));
+            // device masks are column-major (byte 16 x + y); the dump is row-major (16 y + x)
+            uint8_t t[256];
+            for (uint64_t k = 0; k < (uint64_t)(a1 - a0); k++) {
+                uint8_t *m = masks + 256 * k;
+                for (int xx = 0; xx < 16; xx++)
+                    for (int yy = 0; yy < 16; yy++) t[16 * yy + xx] = m[16 * xx + yy];
+                memcpy(m, t, 256);
+            }
         }
     });
 }
