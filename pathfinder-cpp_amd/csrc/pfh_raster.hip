@@ -33,15 +33,6 @@ constexpr int FILL_TILES_PER_BLOCK = 16;
 constexpr int FILL_COL_WORDS = 20;                          // rows 0..16 used
 constexpr int FILL_GROUP_WORDS = 16 * FILL_COL_WORDS + 8;   // pad between tiles of a wave
 
-__device__ __forceinline__ float avg_clamp01(float ua, float ub) {
-    float a = fminf(ua, ub), bb = fmaxf(ua, ub);
-    float du = bb - a;
-    if (du <= 1e-6f) return fminf(fmaxf(0.5f * (a + bb), 0.0f), 1.0f);
-    float full = bb > 1.0f ? bb - fmaxf(a, 1.0f) : 0.0f;
-    float l2 = fmaxf(a, 0.0f), h2 = fminf(bb, 1.0f);
-    float mid = h2 > l2 ? (h2 - l2) * (h2 + l2) * 0.5f : 0.0f;
-    return (full + mid) * __builtin_amdgcn_rcpf(du);
-}
 
 
 // Inclusive prefix sum within each 16-lane row of the wave (DPP row shifts: no LDS).
@@ -56,29 +47,40 @@ __device__ __forceinline__ uint32_t row16_inclusive_sum(uint32_t v) {
 
 // Adds one fill's contribution to pixel column `col` of a tile's difference array
 // (FILL_COL_WORDS words per column); LDS adds, since several lanes may share a column.
+// Over the column's x-window [lo, hi] (width w) the fill runs linearly between heights ymin
+// and ymax.  With G(t) = ∫ max(t - y(x), 0) dx over the window (0 below ymin, w (t - ymean)
+// above ymax, w (t - ymin)^2 / (2 (ymax - ymin)) between), the covered area of pixel row r is
+// F(r + 1) = ∫ clamp(r + 1 - y(x), 0, 1) dx = G(r + 1) - G(r): one G per row boundary.
 __device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, int32_t *diff) {
     const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), y0 = (float)(fl.x >> 16) * (1.0f / 256.0f);
     const float x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f), y1 = (float)(fl.y >> 16) * (1.0f / 256.0f);
     const float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
     const float lo = fmaxf((float)col, xl), hi = fminf((float)(col + 1), xr);
     if (!(hi > lo)) return;
-    const float sign = x0 > x1 ? 1.0f : -1.0f;
+    const float ks = x0 > x1 ? scale : -scale;  // sign of the fill, in fixed-point units
     const float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
     const float slope = (yr - yl) * __builtin_amdgcn_rcpf(xr - xl);
     const float ya = yl + (lo - xl) * slope, yb = yl + (hi - xl) * slope;
     const float w = hi - lo;
-    const int r0 = max(0, min(15, (int)floorf(fminf(ya, yb))));
-    const int r1 = max(0, min(15, (int)floorf(fmaxf(ya, yb))));
+    const float ymin = fminf(ya, yb), ymax = fmaxf(ya, yb);
+    const float half_sum = 0.5f * (ymin + ymax);
+    const float k2 = w * 0.5f * __builtin_amdgcn_rcpf(ymax - ymin);  // used only when ymin < t < ymax
+    const int r0 = max(0, min(15, (int)floorf(ymin)));
+    const int r1 = max(0, min(15, (int)floorf(ymax)));
     int32_t prev = 0;
+    float g_prev = 0.0f;  // G(r0) = 0: r0 <= ymin
+    int32_t *cd = diff + col * FILL_COL_WORDS;
     for (int r = r0; r <= r1; r++) {
-        const float rb = (float)(r + 1);
-        const int32_t v = (int32_t)rintf(sign * w * avg_clamp01(rb - ya, rb - yb) * scale);
-        atomicAdd(diff + col * FILL_COL_WORDS + r, v - prev);
+        const float t = (float)(r + 1);
+        const float a = t - ymin;
+        const float g = t >= ymax ? w * (t - half_sum) : (a > 0.0f ? k2 * a * a : 0.0f);
+        const int32_t v = (int32_t)rintf(ks * (g - g_prev));
+        atomicAdd(cd + r, v - prev);
         prev = v;
+        g_prev = g;
     }
-    atomicAdd(diff + col * FILL_COL_WORDS + r1 + 1, (int32_t)rintf(sign * w * scale) - prev);
+    atomicAdd(cd + r1 + 1, (int32_t)rintf(ks * w) - prev);
 }
-
 
 // Inclusive prefix sum across the 64-lane wave: DPP within rows, then the row totals.
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
@@ -98,6 +100,23 @@ __device__ __forceinline__ uint32_t coverage_byte(int32_t w, uint32_t even_odd) 
     const int32_t eo = one - abs(one - m);
     const uint32_t c = (uint32_t)(even_odd ? eo : nz);
     return (c * 255u + (uint32_t)(one >> 1)) >> FIX_SHIFT;
+}
+
+// Prefix of a column's 16 difference values on top of the backdrop, then coverage bytes packed
+// 4 per word (RULE 0 nonzero, 1 even-odd, 2 per lane from `rule`).
+template <int RULE>
+__device__ __forceinline__ void resolve_column(int32_t run, const int32_t (&dv)[16], uint32_t rule,
+                                               uint32_t (&packed)[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            run += dv[4 * k + r];
+            word |= coverage_byte(run, RULE == 2 ? rule : (uint32_t)RULE) << (8 * r);
+        }
+        packed[k] = word;
+    }
 }
 
 // A wave takes 4 consecutive alpha tiles (16 lanes, one pixel column each, per tile).  Their
@@ -179,18 +198,22 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         const uint32_t rule = (rec.z >> 8) & 1u;
-        int32_t run = (int32_t)(int8_t)(rec.z & 0xffu) * (1 << FIX_SHIFT);  // backdrop, exact
-        uint32_t packed[4] = {0, 0, 0, 0};
+        int32_t dv[16];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const uint4 v = reinterpret_cast<const uint4 *>(dcol)[k];
-            const int32_t d[4] = {(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                run += d[r];
-                packed[k] |= coverage_byte(run, rule) << (8 * r);
-            }
+            dv[4 * k] = (int32_t)v.x;
+            dv[4 * k + 1] = (int32_t)v.y;
+            dv[4 * k + 2] = (int32_t)v.z;
+            dv[4 * k + 3] = (int32_t)v.w;
         }
+        const int32_t bd_fix = (int32_t)(int8_t)(rec.z & 0xffu) * (1 << FIX_SHIFT);  // backdrop, exact
+        uint32_t packed[4];
+        // the 4 tiles of a wave nearly always belong to one path: branch on a wave-uniform rule
+        const uint64_t eo_mask = __ballot(rule != 0);
+        if (eo_mask == 0) resolve_column<0>(bd_fix, dv, rule, packed);
+        else if (eo_mask == ~0ull) resolve_column<1>(bd_fix, dv, rule, packed);
+        else resolve_column<2>(bd_fix, dv, rule, packed);
         if (valid)
             reinterpret_cast<uint4 *>(b.masks + (uint64_t)a * 256)[li] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
         // the next group of tiles reuses this wave's LDS
@@ -213,6 +236,30 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
 // premultiply) and BlendState::from_over (gpu/base.h:83-93); layers accumulate in
 // fp32 and the framebuffer is stored once as RGBA8 (premultiplied).
 
+// One step of the segmented inclusive scan of premultiplied colours under "over": the value
+// from the DPP source lane lies below this lane's accumulated run unless a run starts (or a
+// non-solid entry sits) inside this lane's span; lanes without a source are left alone.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void over_step(bool has_src, float &pr, float &pg, float &pb, float &pa, bool &flag) {
+    auto dpp = [](float v) {
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+    };
+    const float qr = dpp(pr), qg = dpp(pg), qb = dpp(pb), qa = dpp(pa);
+    const bool qf = __builtin_amdgcn_update_dpp(0, flag ? 1 : 0, CTRL, ROW_MASK, 0xf, false) != 0;
+    if (has_src && !flag) {
+        const float k1 = 1.0f - pa;
+        pr = pr + qr * k1;
+        pg = pg + qg * k1;
+        pb = pb + qb * k1;
+        pa = pa + qa * k1;
+    }
+    if (has_src) flag = flag || qf;
+}
+
+__device__ __forceinline__ float lane_f(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
 __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *sorted_vals) {
     const int lane = threadIdx.x & 63;
     const uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -234,12 +281,12 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     // matching the column-major coverage masks (one 32-bit mask word per lane)
     const int x = tx * TILE + (lane >> 2), y0 = ty * TILE + (lane & 3) * 4;
     float r[4], g[4], bl[4], al[4];
+    uint8_t *const cptr = sc.out + (int64_t)y0 * sc.out_pitch + (int64_t)x * 4;  // this lane's top pixel
     if (sc.load_dst && x < sc.width) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             uint32_t px = 0;
-            if (y0 + j < sc.height)
-                px = *reinterpret_cast<const uint32_t *>(sc.out + (int64_t)(y0 + j) * sc.out_pitch + (int64_t)x * 4);
+            if (y0 + j < sc.height) px = *reinterpret_cast<const uint32_t *>(cptr + j * sc.out_pitch);
             r[j] = (float)(px & 255u) * (1.0f / 255.0f);
             g[j] = (float)((px >> 8) & 255u) * (1.0f / 255.0f);
             bl[j] = (float)((px >> 16) & 255u) * (1.0f / 255.0f);
@@ -272,21 +319,15 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         // a run starts at a solid lane whose predecessor is not solid
         bool head = solid && (lane == 0 || !((solid_mask >> (lane - 1)) & 1ull));
         bool flag = head || !solid;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const float qr = __shfl_up(pr, d, 64), qg = __shfl_up(pg, d, 64), qb = __shfl_up(pb, d, 64),
-                        qa = __shfl_up(pa, d, 64);
-            const bool qf = __shfl_up((int)flag, d, 64);
-            if (lane >= d && !flag) {
-                // the entry on top is this lane's accumulated colour; the lower one is q
-                const float k1 = 1.0f - pa;
-                pr = pr + qr * k1;
-                pg = pg + qg * k1;
-                pb = pb + qb * k1;
-                pa = pa + qa * k1;
-            }
-            if (lane >= d) flag = flag || qf;
-        }
+        // DPP steps: shifts by 1, 2, 4, 8 within each 16-lane row, then the row totals
+        // (lane 15 into row 1 and lane 47 into row 3, then lane 31 into rows 2 and 3)
+        const int row = lane >> 4, rl = lane & 15;
+        over_step<0x111, 0xf>(rl >= 1, pr, pg, pb, pa, flag);
+        over_step<0x112, 0xf>(rl >= 2, pr, pg, pb, pa, flag);
+        over_step<0x114, 0xf>(rl >= 4, pr, pg, pb, pa, flag);
+        over_step<0x118, 0xf>(rl >= 8, pr, pg, pb, pa, flag);
+        over_step<0x142, 0xa>(row == 1 || row == 3, pr, pg, pb, pa, flag);
+        over_step<0x143, 0xc>(row >= 2, pr, pg, pb, pa, flag);
         // events: alpha entries, and the last entry of each solid run
         const uint64_t valid_mask = __ballot(valid);
         const uint64_t alpha_mask = valid_mask & ~solid_mask;
@@ -314,8 +355,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
                     al[q] = sa + al[q] * k1;
                 }
             } else {
-                const float fr = __shfl(pr, j, 64), fg = __shfl(pg, j, 64), fb = __shfl(pb, j, 64),
-                            fa = __shfl(pa, j, 64);
+                const float fr = lane_f(pr, j), fg = lane_f(pg, j), fb = lane_f(pb, j), fa = lane_f(pa, j);
                 const float k1 = 1.0f - fa;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
@@ -339,7 +379,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     }
 #pragma unroll
     for (int q = 0; q < 4; q++)
-        if (y0 + q < sc.height) *reinterpret_cast<uint32_t *>(sc.out + (int64_t)(y0 + q) * sc.out_pitch + (int64_t)x * 4) = px[q];
+        if (y0 + q < sc.height) *reinterpret_cast<uint32_t *>(cptr + q * sc.out_pitch) = px[q];
 }
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
