@@ -894,47 +894,74 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t *keys, co
     hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Scatter of one 8-bit digit pass.  The block's 4096 items are first ranked stably by digit
+// into LDS (16 rows of 256: a ballot multisplit gives each item's rank among equal digits in
+// its wave, earlier waves and rows are added from per-digit counters), then written out
+// in digit order: runs of equal digits land at consecutive global positions, so stores
+// coalesce instead of scattering one item per digit per row.  offs is the exclusive scan of
+// the [digit][block] histogram: the block's count of digit d is the difference of its
+// entry and the next one.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *keys, const uint32_t *vals,
                                                            uint32_t *keys_out, uint32_t *vals_out,
                                                            const uint32_t *n_ptr, uint32_t cap, int shift,
                                                            const uint32_t *offs, uint32_t nblocks) {
-    __shared__ uint32_t run[256];
+    __shared__ uint32_t skey[RS_TILE], sval[RS_TILE];
+    __shared__ uint32_t run[256];     // items of each digit placed so far (local)
+    __shared__ uint32_t lstart[256];  // local start of each digit
+    __shared__ uint32_t goff[256];    // global start of each digit's run from this block
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t n = min(*n_ptr, cap);
-    run[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
-    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    const uint32_t n = min(*n_ptr, cap);
+    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    if (base >= n) return;  // uniform over the block
+    const uint32_t valid_n = (uint32_t)min<uint64_t>(RS_TILE, n - base);
+    {
+        const uint64_t h = (uint64_t)threadIdx.x * nblocks + blockIdx.x;
+        const uint32_t o = offs[h];
+        goff[threadIdx.x] = o;
+        uint32_t total;
+        lstart[threadIdx.x] = block_exclusive_scan<uint32_t>(offs[h + 1] - o, total);
+        run[threadIdx.x] = 0;
+    }
+    __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int r = 0; r < RS_ITEMS; r++) {
         for (int q = threadIdx.x; q < (RS_THREADS / 64) * 256; q += RS_THREADS) (&wcnt[0][0])[q] = 0;
         __syncthreads();
-        uint64_t j = base + (uint64_t)r * RS_THREADS + threadIdx.x;
-        bool valid = j < n;
-        uint32_t key = valid ? keys[j] : 0u;
-        uint32_t val = valid ? vals[j] : 0u;
-        uint32_t d = (key >> shift) & 255u;
+        const uint32_t li = (uint32_t)r * RS_THREADS + threadIdx.x;
+        const bool valid = li < valid_n;
+        const uint32_t key = valid ? keys[base + li] : 0u;
+        const uint32_t val = valid ? vals[base + li] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
         // lanes holding the same digit (multisplit peer mask)
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; bit++) {
-            uint64_t m = __ballot((d >> bit) & 1u);
+            const uint64_t m = __ballot((d >> bit) & 1u);
             peers &= ((d >> bit) & 1u) ? m : ~m;
         }
-        uint32_t rank = __popcll(peers & lt_mask);
-        uint32_t cnt = __popcll(peers);
+        const uint32_t rank = __popcll(peers & lt_mask);
+        const uint32_t cnt = __popcll(peers);
         if (valid && rank == cnt - 1) wcnt[wave][d] = cnt;
         __syncthreads();
         if (valid) {
-            uint32_t pos = run[d] + rank;
+            uint32_t pos = lstart[d] + run[d] + rank;
             for (int w = 0; w < wave; w++) pos += wcnt[w][d];
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            skey[pos] = key;
+            sval[pos] = val;
         }
         __syncthreads();
         uint32_t add = 0;
         for (int w = 0; w < RS_THREADS / 64; w++) add += wcnt[w][threadIdx.x];
         run[threadIdx.x] += add;
         __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < valid_n; i += RS_THREADS) {
+        const uint32_t key = skey[i];
+        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t g = goff[d] + (i - lstart[d]);
+        keys_out[g] = key;
+        vals_out[g] = sval[i];
     }
 }
 
