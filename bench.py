@@ -6,7 +6,8 @@ log-U[256, 4096] px and log-U[10, 20 000] closed paths each (scenegen.batch_scen
 every scene from outlines resident in HBM to its RGBA8 framebuffer in HBM:
 
 * batched: all 256 scenes in one pass (flatten, bin, propagate, sort, fill, composite),
-  buffers sized by the learned predictor, replayed from one HIP graph -> ``value``;
+  buffers sized by the learned predictor, launch schedule chosen by the autotuner (once,
+  before timing), replayed from one HIP graph -> ``value``;
 * plain: one scene at a time, stage by stage, exact sizes read back between stages (the
   reference renderer's structure) -> ``plain_paths_per_s``, ``batched_over_plain``.
 
@@ -89,6 +90,9 @@ def measure(ctx, ids, flags, steps, warmup, plain_steps):
             stage_ms += np.array(ctx.stats()["ms_stage"])
     plain_s = (time.perf_counter() - t0) / plain_steps
     stage_ms /= plain_steps
+    # launch schedule for this workload (once, outside the timed region)
+    sched, ms0, ms1 = ctx.autotune(ids, flags & pfh.PFH_PREDICT, reps=3)
+    tuning = {"schedule": sched, "default_ms": round(ms0, 4), "tuned_ms": round(ms1, 4)}
     for _ in range(warmup):
         ctx.rasterize(ids, flags)
     fill_ms = comp_ms = 0.0
@@ -99,7 +103,7 @@ def measure(ctx, ids, flags, steps, warmup, plain_steps):
         fill_ms += ms[6]
         comp_ms += ms[7]
     batched_s = (time.perf_counter() - t0) / steps
-    return plain_s, stage_ms, batched_s, fill_ms / steps, comp_ms / steps, ctx.stats()
+    return plain_s, stage_ms, batched_s, fill_ms / steps, comp_ms / steps, ctx.stats(), tuning
 
 
 def serve(ctx, ids, qps, n_requests, seed, slots=2, budget=64 << 30, max_batch=32):
@@ -201,8 +205,8 @@ def main():
     print("[bench] uploaded", flush=True)
 
     plain_steps = max(1, args.steps // 4) if args.config == "batch" else args.steps
-    plain_s, stage_ms, batched_s, fill_ms, comp_ms, st = measure(ctx, ids, flags, args.steps, args.warmup,
-                                                                 plain_steps)
+    plain_s, stage_ms, batched_s, fill_ms, comp_ms, st, tuning = measure(ctx, ids, flags, args.steps, args.warmup,
+                                                                         plain_steps)
     print(f"[bench] plain {plain_s * 1e3:.3f} ms/step, batched {batched_s * 1e3:.3f} ms/step", flush=True)
     names = ["flatten", "bin_count", "bin_emit", "propagate", "entries", "sort", "fill", "composite"]
     out = {
@@ -219,17 +223,19 @@ def main():
         "counts": {k: st[k] for k in ("segments", "lines", "tiles", "fills", "alpha_tiles", "entries",
                                       "alpha_entries", "fb_tiles", "retries", "host_syncs")},
         "roofline": roofline(st, fill_ms, comp_ms, pixels, n_paths),
+        "autotune": tuning,
     }
 
     if not args.no_extra and args.config == "batch":
         one = scenegen.single_scene_4k()
         sid = ctx.upload(one)
-        p_s, s_ms, b_s, f_ms, c_ms, st1 = measure(ctx, [sid], flags, 20, 3, 20)
+        p_s, s_ms, b_s, f_ms, c_ms, st1, tune1 = measure(ctx, [sid], flags, 20, 3, 20)
         out["single_4k"] = {"workload": WORKLOADS["single"], "paths_per_s": round(one.n_paths / b_s, 1),
                             "ms_per_step": round(b_s * 1e3, 4), "plain_ms_per_step": round(p_s * 1e3, 4),
                             "mpix_per_s": round(one.width * one.height / b_s / 1e6, 1),
                             "plain_stage_ms": {k: round(v, 4) for k, v in zip(names, s_ms)},
-                            "roofline": roofline(st1, f_ms, c_ms, one.width * one.height, one.n_paths)}
+                            "roofline": roofline(st1, f_ms, c_ms, one.width * one.height, one.n_paths),
+                            "autotune": tune1}
         ctx.release(sid)
         print(f"[bench] single_4k batched {b_s * 1e3:.3f} ms/step", flush=True)
         # serving: request rate as a fraction of the batched throughput in scenes/s

@@ -158,6 +158,26 @@ int pfh_mlp_train(pfh_ctx *ctx, const float *x, const float *target, int64_t n, 
                   float lr, uint64_t seed, float *params, float *loss_history);
 
 /*
+ * Launch schedule of the batched pipeline (no counterpart in the reference, whose compute stages
+ * run at fixed local sizes, e.g. shaders/d3d11/fill.comp, tile.comp).  Every schedule produces bit-identical results;
+ * pfh_autotune picks one for a set of scenes by timing graph replays.
+ */
+typedef struct pfh_schedule {
+    int32_t fill_grid;        /* max workgroups of the coverage-mask kernel (it loops beyond) */
+    int32_t composite_block;  /* threads per composite workgroup: 64, 128 or 256 (a wave per 16x16 tile) */
+    int32_t column_block;     /* threads per workgroup of the per-column propagate / entries kernels */
+    int32_t xcd_order;        /* bit 0 fill, bit 1 composite: workgroups regrouped so that each of the
+                                 8 XCDs (own L2) works on one contiguous range of tiles */
+} pfh_schedule;
+int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out);
+int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in);
+/* Coordinate descent from the current schedule: each parameter's candidates timed as the median
+ * device time of `reps` replays of the batched launch of these scenes; the context keeps the
+ * fastest.  ms_start / ms_tuned (may be NULL): time of the starting and of the chosen schedule. */
+int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t flags, int32_t reps, pfh_schedule *out,
+                 double *ms_start, double *ms_tuned);
+
+/*
  * Request loop (continuous batching).  No counterpart in the reference, which draws one scene per
  * Canvas::draw (core/canvas.cpp:557-567) with blocking submits per stage (gpu/vk/queue.cpp:29-34).
  * Requests name uploaded (resident) scenes.  Each of `slots` workspaces (own HIP stream, own work

@@ -26,6 +26,22 @@ constexpr int BIN_GROUP_COLS = 1024;
 
 enum SegKind : uint32_t { SEG_LINE = 1, SEG_QUAD = 2, SEG_CUBIC = 3 };
 
+// Launch schedule (chosen by pfh_autotune; every schedule gives bit-identical results).
+struct Schedule {
+    uint32_t fill_grid = 8192;        // max workgroups of k_fill_masks (grid-stride beyond)
+    uint32_t composite_block = 256;   // threads per k_composite workgroup (a wave per fb tile)
+    uint32_t column_block = 256;      // threads per workgroup of k_propagate / k_entries
+    uint32_t xcd_order = 0;           // bit 0 fill, bit 1 composite: workgroups grouped per XCD
+};
+
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs (each with its own L2).
+// Logical block of hardware block b when the grid's blocks are regrouped so that each XCD
+// processes one contiguous range of logical blocks (a bijection on [0, nb)).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 // Counters written by the device pipeline (one block per batch).
 enum Counter {
     C_FILLS = 0,       // total fills (count pass)
@@ -101,6 +117,7 @@ struct BatchDev {
     uint4 *entry_rec;             // [E] (alpha index | NO_ALPHA, rgba (alpha 0 if covers nothing), local path, fb)
     uint32_t *counters;           // [C_COUNT]
     int32_t dump;
+    uint32_t xcd_order;           // Schedule::xcd_order
 };
 
 }  // namespace pfh

@@ -1036,16 +1036,18 @@ hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_propagate(const BatchDev &b, hipStream_t s) {
+hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s) {
     if (b.n_cols == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_propagate, dim3(blocks_for(b.n_cols, 256)), dim3(256), 0, s, b);
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_propagate, dim3(blocks_for(b.n_cols, block)), dim3(block), 0, s, b);
     return hipGetLastError();
 }
 
-hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s) {
+hipError_t launch_entries(const BatchDev &b, bool write, uint32_t block, hipStream_t s) {
     if (b.n_cols == 0) return hipSuccess;
-    if (write) hipLaunchKernelGGL(k_entries<true>, dim3(blocks_for(b.n_cols, 256)), dim3(256), 0, s, b);
-    else hipLaunchKernelGGL(k_entries<false>, dim3(blocks_for(b.n_cols, 256)), dim3(256), 0, s, b);
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    if (write) hipLaunchKernelGGL(k_entries<true>, dim3(blocks_for(b.n_cols, block)), dim3(block), 0, s, b);
+    else hipLaunchKernelGGL(k_entries<false>, dim3(blocks_for(b.n_cols, block)), dim3(block), 0, s, b);
     return hipGetLastError();
 }
 

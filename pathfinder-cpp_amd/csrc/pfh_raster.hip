@@ -136,7 +136,8 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     const float scale = (float)(1 << FIX_SHIFT);
     // grid-stride over groups of 16 tiles: the grid is sized from the capacity, the count
     // is only known on the device; the loop bound is uniform over the block
-    for (uint32_t first = blockIdx.x * FILL_TILES_PER_BLOCK; first < na; first += gridDim.x * FILL_TILES_PER_BLOCK) {
+    const uint32_t lb = (b.xcd_order & 1u) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    for (uint32_t first = lb * FILL_TILES_PER_BLOCK; first < na; first += gridDim.x * FILL_TILES_PER_BLOCK) {
         const uint32_t a = first + wv * 4 + slot;
         const bool valid = a < na;
 #pragma unroll
@@ -262,7 +263,8 @@ __device__ __forceinline__ float lane_f(float v, int j) {
 
 __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *sorted_vals) {
     const int lane = threadIdx.x & 63;
-    const uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lb = (b.xcd_order & 2u) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t f = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (f >= b.n_fb) return;
     int k = 0;
     {
@@ -384,17 +386,18 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
-hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, hipStream_t s) {
+hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, uint32_t max_blocks, hipStream_t s) {
     if (grid_alpha == 0) return hipSuccess;
-    // at most 8192 blocks (32 per CU): each loops over its share of the tiles
-    const uint32_t nb = std::min<uint32_t>(blocks_for(grid_alpha, FILL_TILES_PER_BLOCK), 8192u);
+    // at most max_blocks workgroups: each loops over its share of the tiles
+    const uint32_t nb = std::min<uint32_t>(blocks_for(grid_alpha, FILL_TILES_PER_BLOCK), std::max(max_blocks, 1u));
     hipLaunchKernelGGL(k_fill_masks, dim3(nb), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
-hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, hipStream_t s) {
+hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, uint32_t block, hipStream_t s) {
     if (b.n_fb == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_composite, dim3(blocks_for(b.n_fb, 4)), dim3(256), 0, s, b, sorted_vals);
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_composite, dim3(blocks_for(b.n_fb, block / 64)), dim3(block), 0, s, b, sorted_vals);
     return hipGetLastError();
 }
 

@@ -30,13 +30,13 @@ hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s);
 hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
                         uint32_t *tmp, hipStream_t s);
 hipError_t scan_total(uint32_t *inout, uint32_t n, uint32_t *counters, int counter, uint32_t *tmp, hipStream_t s);
-hipError_t launch_propagate(const BatchDev &b, hipStream_t s);
-hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s);
+hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s);
+hipError_t launch_entries(const BatchDev &b, bool write, uint32_t block, hipStream_t s);
 size_t radix_temp_elems(uint64_t cap);
 hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, const uint32_t *n_ptr,
                       uint32_t cap, int bits, uint32_t *tmp, bool *result_alt, hipStream_t s);
-hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, hipStream_t s);
-hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, hipStream_t s);
+hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, uint32_t max_blocks, hipStream_t s);
+hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, uint32_t block, hipStream_t s);
 hipError_t mlp_forward(const float *params, const float *x, int64_t n, float *hidden, float *out, hipStream_t s);
 hipError_t mlp_backward(const float *params, const float *x, int64_t n, const float *hidden, const float *out,
                         const float *target, float *grad, float *loss, float *scratch, hipStream_t s);
@@ -132,8 +132,11 @@ struct GraphKey {
     std::vector<int32_t> ids;
     uint32_t flags;
     uint32_t cap_f, cap_a, cap_e, cap_l;
+    uint32_t sched[4];  // the launch schedule the graph was captured with
     bool operator<(const GraphKey &o) const {
         if (ids != o.ids) return ids < o.ids;
+        for (int i = 0; i < 4; i++)
+            if (sched[i] != o.sched[i]) return sched[i] < o.sched[i];
         if (flags != o.flags) return flags < o.flags;
         if (cap_f != o.cap_f) return cap_f < o.cap_f;
         if (cap_a != o.cap_a) return cap_a < o.cap_a;
@@ -222,6 +225,7 @@ struct pfh_ctx {
     float predictor_safety = 1.25f;
     DevBuf mlp_params, mlp_x, mlp_h, mlp_out;
     pfh_stats stats{};
+    Schedule sched;                  // launch schedule (pfh_set_schedule / pfh_autotune)
     std::unique_ptr<Server> server;  // request loop, when started
 };
 
@@ -504,6 +508,7 @@ struct Plan {
     uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0;
     uint32_t cap_f = 0, cap_a = 0, cap_e = 0, cap_l = 0;
     int key_bits = 1;
+    Schedule sched;
 };
 
 static uint32_t clamp_cap(uint64_t v) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(v, 64), 0xF0000000ull); }
@@ -557,6 +562,7 @@ static Caps held_caps(Workspace &w, bool dump) {
 constexpr size_t N_BASES = 6;
 
 static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
+    pl.sched = ctx->sched;
     size_t n = pl.ids.size();
     if (ctx->w->h_table_cap < n) {
         if (ctx->w->h_table) (void)hipHostFree(ctx->w->h_table);
@@ -701,6 +707,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.entry_rec = w.entry_rec.as<uint4>();
     b.counters = w.counters.as<uint32_t>();
     b.dump = dump ? 1 : 0;
+    b.xcd_order = pl.sched.xcd_order;
     return b;
 }
 
@@ -725,27 +732,27 @@ static void stage_count(Workspace &w, const BatchDev &b) {
     HIP_TRY(scan_tiles(b, w.scan_tmp.ptr, s));
 }
 
-static void stage_emit(Workspace &w, const BatchDev &b) {
+static void stage_emit(Workspace &w, const BatchDev &b, const Plan &pl) {
     hipStream_t s = w.stream;
     uint32_t *tmp = w.scan_tmp.as<uint32_t>();
     if (b.dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, b.cap_lines, b.counters, -1, tmp, s));
     HIP_TRY(launch_bin(b, true, s));
-    HIP_TRY(launch_propagate(b, s));
-    HIP_TRY(launch_entries(b, false, s));
+    HIP_TRY(launch_propagate(b, pl.sched.column_block, s));
+    HIP_TRY(launch_entries(b, false, pl.sched.column_block, s));
     HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp, s));
 }
 
 static void stage_finish(Workspace &w, const BatchDev &b, const Plan &pl) {
-    HIP_TRY(launch_entries(b, true, w.stream));
+    HIP_TRY(launch_entries(b, true, pl.sched.column_block, w.stream));
     HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, w.scan_tmp.as<uint32_t>(), w.stream));
     bool alt = false;
     HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
                        w.sort_tmp.as<uint32_t>(), &alt, w.stream));
     w.last_sorted_alt = alt;
     HIP_TRY(hipEventRecordWithFlags(w.kev[0], w.stream, hipEventRecordExternal));
-    HIP_TRY(launch_fill_masks(b, b.cap_alpha, w.stream));
+    HIP_TRY(launch_fill_masks(b, b.cap_alpha, pl.sched.fill_grid, w.stream));
     HIP_TRY(hipEventRecordWithFlags(w.kev[1], w.stream, hipEventRecordExternal));
-    HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, w.stream));
+    HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, pl.sched.composite_block, w.stream));
     HIP_TRY(hipEventRecordWithFlags(w.kev[2], w.stream, hipEventRecordExternal));
 }
 
@@ -825,11 +832,11 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     if (dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, nl, b.counters, -1, tmp(), st));
     HIP_TRY(launch_bin(b, true, st));
     HIP_TRY(hipEventRecord(ev[3], st));
-    HIP_TRY(launch_propagate(b, st));
+    HIP_TRY(launch_propagate(b, pl.sched.column_block, st));
     HIP_TRY(hipEventRecord(ev[4], st));
-    HIP_TRY(launch_entries(b, false, st));
+    HIP_TRY(launch_entries(b, false, pl.sched.column_block, st));
     HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp(), st));
-    HIP_TRY(launch_entries(b, true, st));
+    HIP_TRY(launch_entries(b, true, pl.sched.column_block, st));
     HIP_TRY(hipEventRecord(ev[5], st));
     HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, tmp(), st));
     bool alt = false;
@@ -837,9 +844,9 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
                        ctx->w->sort_tmp.as<uint32_t>(), &alt, st));
     ctx->w->last_sorted_alt = alt;
     HIP_TRY(hipEventRecord(ev[6], st));
-    HIP_TRY(launch_fill_masks(b, na, st));
+    HIP_TRY(launch_fill_masks(b, na, pl.sched.fill_grid, st));
     HIP_TRY(hipEventRecord(ev[7], st));
-    HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, st));
+    HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, pl.sched.composite_block, st));
     HIP_TRY(hipEventRecord(ev[8], st));
     read_counters(ctx);
     for (int i = 0; i < 8; i++) ctx->stats.ms_stage[i] += elapsed_ms(ctx, ev[i], ev[i + 1]);
@@ -944,7 +951,13 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
     BatchDev b = batch_dev(w, pl, bl.dump);
     HIP_TRY(hipEventRecord(w.ev0, w.stream));
     if (bl.use_graph) {
-        GraphKey key{pl.ids, bl.flags & ~(uint32_t)PFH_PREDICT, pl.cap_f, pl.cap_a, pl.cap_e, pl.cap_l};
+        GraphKey key{pl.ids,
+                     bl.flags & ~(uint32_t)PFH_PREDICT,
+                     pl.cap_f,
+                     pl.cap_a,
+                     pl.cap_e,
+                     pl.cap_l,
+                     {pl.sched.fill_grid, pl.sched.composite_block, pl.sched.column_block, pl.sched.xcd_order}};
         if (w.graphs_generation != g_alloc_generation) {
             // some buffer moved since these graphs were captured: never replay them
             w.drop_graphs();
@@ -958,7 +971,7 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
             try {
                 stage_clear(w, b);
                 stage_count(w, b);
-                stage_emit(w, b);
+                stage_emit(w, b, pl);
                 stage_finish(w, b, pl);
             } catch (...) {
                 hipGraph_t dummy;
@@ -978,7 +991,7 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
     } else {
         stage_clear(w, b);
         stage_count(w, b);
-        stage_emit(w, b);
+        stage_emit(w, b, pl);
         stage_finish(w, b, pl);
     }
     HIP_TRY(hipMemcpyAsync(w.h_counters, w.counters.ptr, 4 * C_COUNT, hipMemcpyDeviceToHost, w.stream));
@@ -1659,6 +1672,83 @@ int pfh_mlp_train(pfh_ctx *ctx, const float *x, const float *target, int64_t n, 
         HIP_TRY(hipMemcpy(params, d.params.ptr, 4 * np, hipMemcpyDeviceToHost));
         for (DevBuf *bf : {&d.params, &d.m, &d.v, &d.grad, &d.loss, &d.x, &d.t, &d.h, &d.out, &d.scratch})
             bf->release();
+    });
+}
+
+static bool valid_schedule(const pfh_schedule &s) {
+    auto blk = [](int32_t v) { return v == 64 || v == 128 || v == 256; };
+    return s.fill_grid >= 1 && s.fill_grid <= (1 << 20) && blk(s.composite_block) && blk(s.column_block) &&
+           s.xcd_order >= 0 && s.xcd_order <= 3;
+}
+
+int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out) {
+    if (!ctx || !out) return PFH_E_ARG;
+    out->fill_grid = (int32_t)ctx->sched.fill_grid;
+    out->composite_block = (int32_t)ctx->sched.composite_block;
+    out->column_block = (int32_t)ctx->sched.column_block;
+    out->xcd_order = (int32_t)ctx->sched.xcd_order;
+    return PFH_OK;
+}
+
+int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in) {
+    if (!ctx || !in) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        if (!valid_schedule(*in)) throw PfhError{PFH_E_ARG, "invalid schedule"};
+        ctx->sched.fill_grid = (uint32_t)in->fill_grid;
+        ctx->sched.composite_block = (uint32_t)in->composite_block;
+        ctx->sched.column_block = (uint32_t)in->column_block;
+        ctx->sched.xcd_order = (uint32_t)in->xcd_order;
+    });
+}
+
+// Coordinate descent over the schedule, one parameter at a time from the current schedule,
+// each candidate timed as the median device time of `reps` graph replays of the batched
+// launch of these scenes (capture excluded).  Keeps the fastest; results never change.
+int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t flags, int32_t reps, pfh_schedule *out,
+                 double *ms_start, double *ms_tuned) {
+    if (!ctx || !scene_ids || n <= 0 || reps < 1 || reps > 100 || !out) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        HIP_TRY(hipSetDevice(ctx->device));
+        for (int32_t i = 0; i < n; i++) get_scene(ctx, scene_ids[i]);
+        const uint32_t f = (flags & (PFH_PREDICT | PFH_LOAD_DST)) | PFH_BATCHED;
+        auto measure = [&]() {
+            rasterize_batched(ctx, scene_ids, n, f);  // capture (or re-capture) and settle sizes
+            std::vector<double> t;
+            for (int32_t r = 0; r < reps; r++) {
+                rasterize_batched(ctx, scene_ids, n, f);
+                t.push_back(ctx->stats.ms_total);
+            }
+            std::sort(t.begin(), t.end());
+            return t[t.size() / 2];
+        };
+        Schedule best = ctx->sched;
+        double best_ms = measure();
+        if (ms_start) *ms_start = best_ms;
+        const std::vector<uint32_t> cand[4] = {{1024, 2048, 4096, 8192, 16384, 32768}, {64, 128, 256}, {64, 128, 256},
+                                              {0, 1, 2, 3}};
+        for (int param = 0; param < 4; param++) {
+            for (uint32_t v : cand[param]) {
+                Schedule trial = best;
+                uint32_t *field = param == 0 ? &trial.fill_grid
+                                  : param == 1 ? &trial.composite_block
+                                  : param == 2 ? &trial.column_block
+                                               : &trial.xcd_order;
+                if (*field == v) continue;
+                *field = v;
+                ctx->sched = trial;
+                const double ms = measure();
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = trial;
+                }
+            }
+        }
+        ctx->sched = best;
+        out->fill_grid = (int32_t)best.fill_grid;
+        out->composite_block = (int32_t)best.composite_block;
+        out->column_block = (int32_t)best.column_block;
+        out->xcd_order = (int32_t)best.xcd_order;
+        if (ms_tuned) *ms_tuned = best_ms;
     });
 }
 

@@ -64,11 +64,20 @@ class ServerStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("fill_grid", ctypes.c_int32), ("composite_block", ctypes.c_int32),
+                ("column_block", ctypes.c_int32), ("xcd_order", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EXPORTS = ("pfh_create", "pfh_destroy", "pfh_last_error", "pfh_scene_upload", "pfh_scene_release",
            "pfh_scene_features", "pfh_rasterize", "pfh_scene_set_target", "pfh_scene_target", "pfh_readback_rgba8",
            "pfh_dump_counts", "pfh_dump", "pfh_stats_get", "pfh_set_predictor", "pfh_scene_prediction", "pfh_path_counts",
            "pfh_mlp_eval", "pfh_mlp_grad", "pfh_mlp_train", "pfh_server_start", "pfh_server_submit",
-           "pfh_server_poll", "pfh_server_stats_get", "pfh_server_stop")
+           "pfh_server_poll", "pfh_server_stats_get", "pfh_server_stop", "pfh_get_schedule",
+           "pfh_set_schedule", "pfh_autotune")
 
 _lib = None
 
@@ -110,6 +119,10 @@ def load(path: str = LIB_PATH):
         "pfh_server_poll": (ctypes.c_int, [_P, ctypes.c_double, _P, i32, ctypes.POINTER(i32)]),
         "pfh_server_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
         "pfh_server_stop": (ctypes.c_int, [_P]),
+        "pfh_get_schedule": (ctypes.c_int, [_P, ctypes.POINTER(Schedule)]),
+        "pfh_set_schedule": (ctypes.c_int, [_P, ctypes.POINTER(Schedule)]),
+        "pfh_autotune": (ctypes.c_int, [_P, _P, i32, ctypes.c_uint32, i32, ctypes.POINTER(Schedule),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -221,6 +234,23 @@ class Context:
         self._check(self.lib.pfh_dump(self.h, sid, *[_ptr(out[k]) for k in keys]), "pfh_dump")
         out["guard_hits"] = int(counts[7])
         return out
+
+    # launch schedule
+    def schedule(self) -> dict:
+        sc = Schedule()
+        self._check(self.lib.pfh_get_schedule(self.h, ctypes.byref(sc)), "pfh_get_schedule")
+        return sc.as_dict()
+
+    def set_schedule(self, **kw):
+        sc = Schedule(**{**self.schedule(), **kw})
+        self._check(self.lib.pfh_set_schedule(self.h, ctypes.byref(sc)), "pfh_set_schedule")
+
+    def autotune(self, ids, flags: int = PFH_PREDICT, reps: int = 5):
+        arr = np.ascontiguousarray(ids, np.int32)
+        sc, a, b = Schedule(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.pfh_autotune(self.h, _ptr(arr), len(arr), flags, reps, ctypes.byref(sc), ctypes.byref(a),
+                                          ctypes.byref(b)), "pfh_autotune")
+        return sc.as_dict(), a.value, b.value
 
     # request loop (continuous batching)
     def server_start(self, slots: int = 2, memory_budget: int = 8 << 30, max_batch: int = 64,

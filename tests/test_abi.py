@@ -35,7 +35,8 @@ def test_create_rejects_bad_device_without_crashing():
 
 def test_struct_layouts_match_header(tmp_path):
     # sizes and field offsets of the ABI structs, as a C compiler lays them out from the header
-    fields = {"pfh_stats": pfh.Stats, "pfh_server_stats": pfh.ServerStats, "pfh_scene_desc": pfh.SceneDesc}
+    fields = {"pfh_stats": pfh.Stats, "pfh_server_stats": pfh.ServerStats, "pfh_scene_desc": pfh.SceneDesc,
+              "pfh_schedule": pfh.Schedule}
     lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "pfh.h"', "int main(void) {"]
     for cname, py in fields.items():
         lines.append(f'    printf("{cname} %zu\\n", sizeof({cname}));')
