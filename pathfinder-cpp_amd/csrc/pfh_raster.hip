@@ -307,7 +307,10 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
             const uint4 rec = b.entry_rec[sorted_vals[i]];
             ai = rec.x;
             c = rec.y;
-            if (ai != NO_ALPHA && ai >= b.cap_alpha) c = 0;  // capacity overflow: the run is retried
+            if (ai != NO_ALPHA && ai >= b.cap_alpha) {  // capacity overflow: the run is retried
+                c = 0;                                  // blends nothing
+                ai = 0;                                 // and reads a mask that exists
+            }
         }
         const float ca = (float)(c >> 24) * (1.0f / 255.0f);
         const float cr = (float)(c & 255u) * (1.0f / 255.0f);
@@ -339,17 +342,17 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
             const int j = __builtin_ctzll(events);
             events &= events - 1;
             if ((alpha_mask >> j) & 1ull) {
+                // no early out for alpha 0: blending with zero coverage leaves the pixel as it is
                 const uint32_t aj = __builtin_amdgcn_readlane(ai, j);
                 const uint32_t cj = __builtin_amdgcn_readlane(c, j);
-                const float ja = (float)(cj >> 24) * (1.0f / 255.0f);
-                if (ja == 0.0f) continue;
+                const float ja = (float)(cj >> 24) * (1.0f / (255.0f * 255.0f));  // alpha x mask scale
                 const float jr = (float)(cj & 255u) * (1.0f / 255.0f);
                 const float jg = (float)((cj >> 8) & 255u) * (1.0f / 255.0f);
                 const float jb = (float)((cj >> 16) & 255u) * (1.0f / 255.0f);
                 const uint32_t m = reinterpret_cast<const uint32_t *>(b.masks + (uint64_t)aj * 256)[lane];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const float sa = ja * (float)((m >> (8 * q)) & 255u) * (1.0f / 255.0f);
+                    const float sa = ja * (float)((m >> (8 * q)) & 255u);
                     const float k1 = 1.0f - sa;
                     r[q] = jr * sa + r[q] * k1;
                     g[q] = jg * sa + g[q] * k1;
