@@ -77,8 +77,9 @@ def roofline(st, fill_ms, comp_ms, pixels, n_paths):
                       "k_composite_gbs": round(comp_gbs, 1), "k_composite_ms": round(comp_ms, 4)}}
 
 
-def measure(ctx, ids, flags, steps, warmup, plain_steps):
-    """Plain and batched timings of one workload; batched kernel times from in-graph events."""
+def measure(ctx, ids, flags, steps, warmup, plain_steps, schedule=None):
+    """Plain and batched timings of one workload; batched kernel times from in-graph events.
+    The launch schedule is autotuned for the workload, or ``schedule`` when given."""
     for _ in range(max(1, warmup)):
         for sid in ids:
             ctx.rasterize([sid], pfh.PFH_PLAIN)
@@ -91,8 +92,12 @@ def measure(ctx, ids, flags, steps, warmup, plain_steps):
     plain_s = (time.perf_counter() - t0) / plain_steps
     stage_ms /= plain_steps
     # launch schedule for this workload (once, outside the timed region)
-    sched, ms0, ms1 = ctx.autotune(ids, flags & pfh.PFH_PREDICT, reps=3)
-    tuning = {"schedule": sched, "default_ms": round(ms0, 4), "tuned_ms": round(ms1, 4)}
+    if schedule is None:
+        sched, ms0, ms1 = ctx.autotune(ids, flags & pfh.PFH_PREDICT, reps=3)
+        tuning = {"schedule": sched, "default_ms": round(ms0, 4), "tuned_ms": round(ms1, 4)}
+    else:
+        ctx.set_schedule(**schedule)
+        tuning = {"schedule": ctx.schedule(), "tuned": False}
     for _ in range(warmup):
         ctx.rasterize(ids, flags)
     fill_ms = comp_ms = 0.0
@@ -184,7 +189,12 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="main workload only (no single_4k / serving)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-mb", type=int, default=0, help="also run one D2D copy of this size (counter calibration)")
+    ap.add_argument("--schedule", default=None,
+                    help="launch schedule instead of autotuning, e.g. fill_grid=32768,composite_block=64")
     args = ap.parse_args()
+    schedule = None
+    if args.schedule:
+        schedule = {k: int(v) for k, v in (kv.split("=") for kv in args.schedule.split(","))}
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: scenes shard across GPUs with no exchange (see DESIGN.md); "
               f"--gpus {args.gpus} is not supported in this build", file=sys.stderr)
@@ -206,7 +216,7 @@ def main():
 
     plain_steps = max(1, args.steps // 4) if args.config == "batch" else args.steps
     plain_s, stage_ms, batched_s, fill_ms, comp_ms, st, tuning = measure(ctx, ids, flags, args.steps, args.warmup,
-                                                                         plain_steps)
+                                                                         plain_steps, schedule)
     print(f"[bench] plain {plain_s * 1e3:.3f} ms/step, batched {batched_s * 1e3:.3f} ms/step", flush=True)
     names = ["flatten", "bin_count", "bin_emit", "propagate", "entries", "sort", "fill", "composite"]
     out = {
@@ -229,7 +239,7 @@ def main():
     if not args.no_extra and args.config == "batch":
         one = scenegen.single_scene_4k()
         sid = ctx.upload(one)
-        p_s, s_ms, b_s, f_ms, c_ms, st1, tune1 = measure(ctx, [sid], flags, 20, 3, 20)
+        p_s, s_ms, b_s, f_ms, c_ms, st1, tune1 = measure(ctx, [sid], flags, 20, 3, 20, schedule)
         out["single_4k"] = {"workload": WORKLOADS["single"], "paths_per_s": round(one.n_paths / b_s, 1),
                             "ms_per_step": round(b_s * 1e3, 4), "plain_ms_per_step": round(p_s * 1e3, 4),
                             "mpix_per_s": round(one.width * one.height / b_s / 1e6, 1),

@@ -168,6 +168,8 @@ typedef struct pfh_schedule {
     int32_t column_block;     /* threads per workgroup of the per-column propagate / entries kernels */
     int32_t xcd_order;        /* bit 0 fill, bit 1 composite: workgroups regrouped so that each of the
                                  8 XCDs (own L2) works on one contiguous range of tiles */
+    int32_t flatten_segs;     /* segments each flatten wave works through (multiple of 64; 0: from the
+                                 segment count) */
 } pfh_schedule;
 int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out);
 int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in);

@@ -32,6 +32,7 @@ struct Schedule {
     uint32_t composite_block = 256;   // threads per k_composite workgroup (a wave per fb tile)
     uint32_t column_block = 256;      // threads per workgroup of k_propagate / k_entries
     uint32_t xcd_order = 0;           // bit 0 fill, bit 1 composite: workgroups grouped per XCD
+    uint32_t flatten_segs = 0;        // segments per k_flatten wave (multiple of 64), 0: from the count
 };
 
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs (each with its own L2).
@@ -104,7 +105,6 @@ struct BatchDev {
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
     float4 *lines;                // [L] flattened chords (from, to) in emission order
-    uint32_t *line_seg;           // [L] batch segment index of each chord
     uint32_t *line_cnt;           // [L + 1] fills per chord, then exclusive scan (emission order)
     uint2 *fills;                 // [F] grouped by tile: (x0 | y0 << 16, x1 | y1 << 16)
     uint4 *dump_fills;            // [F] emission order: (x0|y0<<16, x1|y1<<16, global tile, 0)

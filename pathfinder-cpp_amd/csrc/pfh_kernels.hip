@@ -271,63 +271,119 @@ struct Walker {
 // cache write back whole lines: several times the chord output in HBM writes.
 constexpr int FLAT_LDS_LEVELS = 6;
 
+// Lanes are scheduled dynamically within a wave: the wave owns a contiguous range of
+// segments, each iteration every lane takes one depth-first step of its segment, and a lane
+// whose segment is done takes the next one of the range (a wave-uniform counter, ranks by
+// ballot).  So a wave no longer runs as long as its deepest curve while the lanes holding
+// lines and shallow curves idle.  Segment data is loaded 64 segments at a time, one per lane
+// (one dependent load chain per 64 segments, coalesced), and handed to the lane that takes
+// the segment by lane shuffles.
 template <bool EMIT>
-__global__ __launch_bounds__(256) void k_flatten(BatchDev b) {
+__global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_wave) {
     __shared__ float fs[FLAT_LDS_LEVELS][6][256];
     __shared__ uint8_t fd[FLAT_LDS_LEVELS][256];
-    const uint32_t tid = threadIdx.x;
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= b.n_segs) return;  // no barriers below: each lane uses only its own LDS column
-    int k = find_scene(b.seg_bases, b.n_scenes, g);
-    const SceneDev &sc = b.scenes[k];
-    uint32_t s_local = g - sc.seg_base;
-    uint2 sg = sc.segs[s_local];
-    int4 rect = sc.path_rect[sc.seg_path[s_local]];
-    uint32_t n = 0, guard = 0;
-    uint32_t base = EMIT ? b.seg_cnt[g] : 0u;
-    // With no tile columns nothing can land (add_fill culls, adjust_alpha_tile_backdrop
-    // ignores x outside [0, width)); with columns but no rows, above-rect backdrops still do.
-    if (rect.z - rect.x > 0) {
-        const float2 *P = sc.pts + sg.x;
-        Seg4 s;
-        s.p0 = P[0];
-        if (sg.y == SEG_LINE) {
-            s.p1 = P[1];
-        } else if (sg.y == SEG_QUAD) {
-            s.c0 = P[1];
-            s.c1 = P[1];
-            s.p1 = P[2];
-        } else {
-            s.c0 = P[1];
-            s.c1 = P[2];
-            s.p1 = P[3];
-        }
-        // A non-finite point makes every flattened piece non-finite, and process_line_segment
-        // drops those (tiler.cpp:133-136): the segment contributes nothing.
-        bool finite = isfinite(s.p0.x) && isfinite(s.p0.y) && isfinite(s.p1.x) && isfinite(s.p1.y);
-        if (sg.y != SEG_LINE) finite = finite && isfinite(s.c0.x) && isfinite(s.c0.y);
-        if (sg.y == SEG_CUBIC) finite = finite && isfinite(s.c1.x) && isfinite(s.c1.y);
-        if (finite && sg.y == SEG_LINE) {
-            if (EMIT && base < b.cap_lines) {
-                b.lines[base] = make_float4(s.p0.x, s.p0.y, s.p1.x, s.p1.y);
-                b.line_seg[base] = g;
-            }
-            n = 1;
-        } else if (finite) {
-            float stk[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS][6];
-            int sdepth[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS];
-            int sp = 0, d = 0;
-            const bool quad = sg.y == SEG_QUAD;
-            while (true) {
-                bool flat = quad ? flat_quad(s) : flat_cubic(s);
-                if (flat || d >= MAX_FLATTEN_DEPTH) {
-                    if (!flat) guard++;
-                    if (EMIT && base + n < b.cap_lines) {
-                        b.lines[base + n] = make_float4(s.p0.x, s.p0.y, s.p1.x, s.p1.y);
-                        b.line_seg[base + n] = g;
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint64_t w0 = ((uint64_t)blockIdx.x * 4 + (tid >> 6)) * segs_per_wave;
+    if (w0 >= b.n_segs) return;  // whole wave; each lane uses only its own LDS column
+    const uint32_t w1 = (uint32_t)min<uint64_t>(w0 + segs_per_wave, b.n_segs);
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    float stk[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS][6];
+    int sdepth[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS];
+    // scene of the next segment to load: segment indices only grow, so it is advanced, not searched
+    int k = find_scene(b.seg_bases, b.n_scenes, (uint32_t)w0);
+    uint32_t next = (uint32_t)w0;           // first segment not yet handed out (wave-uniform)
+    uint32_t batch = (uint32_t)w0, batch_end = (uint32_t)w0;  // staged segments [batch, batch_end)
+    float2 q0 = {}, q1 = {}, q2 = {}, q3 = {};  // staged segment batch + lane: points
+    uint32_t qkind = 0, qbase = 0;              // its kind (0: emits nothing) and chord offset
+    Seg4 s;
+    uint32_t g = 0, kind = 0, n = 0, base = 0, guard = 0;
+    int d = 0, sp = 0;
+    bool active = false;
+    while (true) {
+        const uint64_t want = __ballot(!active);
+        if (want && next < w1) {  // wave-uniform
+            if (next >= batch_end) {
+                // stage the next 64 segments, one per lane
+                batch = next;
+                batch_end = min(next + 64, w1);
+                const uint32_t gl = batch + lane;
+                int kl = k;
+                qkind = 0;
+                if (gl < batch_end) {
+                    while (gl >= b.seg_bases[kl + 1]) kl++;
+                    const SceneDev &sc = b.scenes[kl];
+                    const uint32_t s_local = gl - sc.seg_base;
+                    const uint2 sg = sc.segs[s_local];
+                    const int4 rect = sc.path_rect[sc.seg_path[s_local]];
+                    qbase = EMIT ? b.seg_cnt[gl] : 0u;
+                    // With no tile columns nothing can land (add_fill culls, adjust_alpha_tile_backdrop
+                    // ignores x outside [0, width)); with columns but no rows, above-rect backdrops still do.
+                    if (rect.z - rect.x > 0) {
+                        const float2 *P = sc.pts + sg.x;
+                        q0 = P[0];
+                        if (sg.y == SEG_LINE) {
+                            q3 = P[1];
+                        } else if (sg.y == SEG_QUAD) {
+                            q1 = P[1];
+                            q2 = P[1];
+                            q3 = P[2];
+                        } else {
+                            q1 = P[1];
+                            q2 = P[2];
+                            q3 = P[3];
+                        }
+                        // A non-finite point makes every flattened piece non-finite, and
+                        // process_line_segment drops those (tiler.cpp:133-136): nothing from it.
+                        bool finite = isfinite(q0.x) && isfinite(q0.y) && isfinite(q3.x) && isfinite(q3.y);
+                        if (sg.y != SEG_LINE) finite = finite && isfinite(q1.x) && isfinite(q1.y);
+                        if (sg.y == SEG_CUBIC) finite = finite && isfinite(q2.x) && isfinite(q2.y);
+                        if (finite) qkind = sg.y;
                     }
-                    n++;
-                    if (sp == 0) break;
+                }
+                k = __shfl(kl, (int)(batch_end - 1 - batch));
+            }
+            // idle lanes take the next staged segments, in lane order
+            const uint32_t cand = next + (uint32_t)__popcll(want & lt_mask);
+            next = min(next + (uint32_t)__popcll(want), batch_end);
+            const int src = (int)((cand - batch) & 63);
+            const float2 t0 = make_float2(__shfl(q0.x, src), __shfl(q0.y, src));
+            const float2 t1 = make_float2(__shfl(q1.x, src), __shfl(q1.y, src));
+            const float2 t2 = make_float2(__shfl(q2.x, src), __shfl(q2.y, src));
+            const float2 t3 = make_float2(__shfl(q3.x, src), __shfl(q3.y, src));
+            const uint32_t tk = __shfl(qkind, src);
+            const uint32_t tb = EMIT ? __shfl(qbase, src) : 0u;
+            if (!active && cand < batch_end) {
+                g = cand;
+                s = Seg4{t0, t1, t2, t3};
+                kind = tk;
+                base = tb;
+                n = 0;
+                d = 0;
+                sp = 0;
+                active = true;
+            }
+        }
+        if (__ballot(active) == 0) break;  // every lane idle: the range is done
+        if (!active) continue;
+        bool done = false;
+        if (kind == 0) {
+            done = true;
+        } else if (kind == SEG_LINE) {
+            if (EMIT && base < b.cap_lines) b.lines[base] = make_float4(s.p0.x, s.p0.y, s.p1.x, s.p1.y);
+            n = 1;
+            done = true;
+        } else {
+            // one depth-first step: a flat piece is emitted (then the next pending half is
+            // popped), otherwise it is split at t = 0.5 and the right half pushed
+            const bool quad = kind == SEG_QUAD;
+            const bool flat = quad ? flat_quad(s) : flat_cubic(s);
+            if (flat || d >= MAX_FLATTEN_DEPTH) {
+                if (!flat) guard++;
+                if (EMIT && base + n < b.cap_lines) b.lines[base + n] = make_float4(s.p0.x, s.p0.y, s.p1.x, s.p1.y);
+                n++;
+                if (sp == 0) {
+                    done = true;
+                } else {
                     sp--;
                     s.p0 = s.p1;
                     if (sp < FLAT_LDS_LEVELS) {
@@ -342,8 +398,8 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b) {
                         s.p1 = make_float2(stk[q][4], stk[q][5]);
                         d = sdepth[q];
                     }
-                    continue;
                 }
+            } else {
                 const float t = 0.5f;
                 Seg4 a, bseg;
                 if (quad) {
@@ -380,11 +436,12 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b) {
                 d++;
             }
         }
+        if (done) {
+            if (!EMIT) b.seg_cnt[g] = n;
+            active = false;
+        }
     }
-    if (!EMIT) {
-        b.seg_cnt[g] = n;
-        if (guard) atomicAdd(&b.counters[C_GUARD], guard);
-    }
+    if (!EMIT && guard) atomicAdd(&b.counters[C_GUARD], guard);
 }
 
 // ---------------------------------------------------------------------------
@@ -553,48 +610,6 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
 
 // ---------------------------------------------------------------------------
 
-template <bool EMIT>
-__global__ __launch_bounds__(256) void k_bin(BatchDev b) {
-    const uint32_t nl = min(b.counters[C_LINES], b.cap_lines);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
-        const uint32_t g = b.line_seg[i];
-        const float4 ln = b.lines[i];
-        int k = find_scene(b.seg_bases, b.n_scenes, g);
-        const SceneDev &sc = b.scenes[k];
-        const uint32_t path = sc.seg_path[g - sc.seg_base];
-        const int4 rect = sc.path_rect[path];
-        Walker<EMIT> w;
-        w.vb_l = sc.vb_l;
-        w.vb_r = sc.vb_r;
-        w.vb_b = sc.vb_b;
-        w.tl = rect.x;
-        w.tt = rect.y;
-        w.tr = rect.z;
-        w.tb = rect.w;
-        w.tw = rect.z - rect.x;
-        w.th = rect.w - rect.y;
-        w.tbase = sc.tile_base + sc.path_tile_off[path];
-        w.cbase = sc.col_base + sc.path_col_off[path];
-        w.tile_cnt = b.tile_cnt;
-        w.tile_delta = b.tile_delta;
-        w.col_bd = b.col_bd;
-        w.out = b.dump ? b.dump_fills : nullptr;
-        w.fills = b.fills;
-        w.tile_fill_off = b.tile_fill_off;
-        w.cap_fills = b.cap_fills;
-        w.pend_tile = 0xffffffffu;
-        w.pend_n = 0;
-        w.count = 0;
-        w.guard = 0;
-        w.out_base = (EMIT && b.dump) ? b.line_cnt[i] : 0u;
-        w.line(make_float2(ln.x, ln.y), make_float2(ln.z, ln.w));
-        w.flush();
-        if (!EMIT) {
-            b.line_cnt[i] = w.count;
-            if (w.guard) atomicAdd(&b.counters[C_GUARD], w.guard);
-        }
-    }
-}
 
 
 // ---------------------------------------------------------------------------
@@ -1014,10 +1029,13 @@ hipError_t scan_tiles(const BatchDev &b, void *tmp, hipStream_t s) {
                          b.n_tiles, reinterpret_cast<U2 *>(tmp), s);
 }
 
-hipError_t launch_flatten(const BatchDev &b, bool emit, hipStream_t s) {
+// segs_per_wave: segments each wave works through (a multiple of 64, at least 64)
+hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, hipStream_t s) {
     if (b.n_segs == 0) return hipSuccess;
-    if (emit) hipLaunchKernelGGL(k_flatten<true>, dim3(blocks_for(b.n_segs, 256)), dim3(256), 0, s, b);
-    else hipLaunchKernelGGL(k_flatten<false>, dim3(blocks_for(b.n_segs, 256)), dim3(256), 0, s, b);
+    if (segs_per_wave < 64 || segs_per_wave % 64) return hipErrorInvalidValue;
+    const dim3 grid(blocks_for(blocks_for(b.n_segs, segs_per_wave), 4));
+    if (emit) hipLaunchKernelGGL(k_flatten<true>, grid, dim3(256), 0, s, b, segs_per_wave);
+    else hipLaunchKernelGGL(k_flatten<false>, grid, dim3(256), 0, s, b, segs_per_wave);
     return hipGetLastError();
 }
 

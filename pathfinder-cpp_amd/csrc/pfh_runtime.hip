@@ -25,7 +25,7 @@ size_t scan_temp_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp, hipStream_t s);
 hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, uint32_t *tmp, hipStream_t s);
 hipError_t scan_tiles(const BatchDev &b, void *tmp, hipStream_t s);
-hipError_t launch_flatten(const BatchDev &b, bool emit, hipStream_t s);
+hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, hipStream_t s);
 hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s);
 hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
                         uint32_t *tmp, hipStream_t s);
@@ -132,10 +132,10 @@ struct GraphKey {
     std::vector<int32_t> ids;
     uint32_t flags;
     uint32_t cap_f, cap_a, cap_e, cap_l;
-    uint32_t sched[4];  // the launch schedule the graph was captured with
+    uint32_t sched[5];  // the launch schedule the graph was captured with
     bool operator<(const GraphKey &o) const {
         if (ids != o.ids) return ids < o.ids;
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 5; i++)
             if (sched[i] != o.sched[i]) return sched[i] < o.sched[i];
         if (flags != o.flags) return flags < o.flags;
         if (cap_f != o.cap_f) return cap_f < o.cap_f;
@@ -157,7 +157,7 @@ struct Workspace {
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_entry, col_bd,
-        seg_cnt, lines, line_seg, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_cnt, keys, vals, keys_alt,
+        seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_cnt, keys, vals, keys_alt,
         vals_alt, entry_rec, counters, scan_tmp, sort_tmp;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
@@ -181,7 +181,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd,
-                &tile_entry, &col_bd, &seg_cnt, &lines, &line_seg, &line_cnt, &fills, &dump_fills, &alpha_rec,
+                &tile_entry, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
                 &masks, &zbuf, &fb_cnt, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
                 &sort_tmp};
     }
@@ -511,6 +511,14 @@ struct Plan {
     Schedule sched;
 };
 
+// Segments per flatten wave: the schedule's choice, or enough for ~4 waves per SIMD (1024 SIMDs),
+// 128 to 2048 (a wave balances its lanes over its whole range).
+static uint32_t flatten_segs(const Plan &pl) {
+    if (pl.sched.flatten_segs) return pl.sched.flatten_segs;
+    const uint64_t per = (pl.S + 4096 * 64 - 1) / (4096 * 64);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(64 * per, 128), 2048);
+}
+
 static uint32_t clamp_cap(uint64_t v) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(v, 64), 0xF0000000ull); }
 
 static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
@@ -523,7 +531,6 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.col_bd.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
     w.lines.ensure(16ull * pl.cap_l);
-    w.line_seg.ensure(4ull * pl.cap_l);
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
     w.zbuf.ensure(4 * pl.FB + 4);
     w.fb_cnt.ensure(4 * pl.FB + 8);
@@ -551,7 +558,7 @@ static Caps held_caps(Workspace &w, bool dump) {
     auto fit = [](size_t bytes, size_t per, size_t extra) -> uint64_t {
         return bytes > extra ? (bytes - extra) / per : 0;
     };
-    uint64_t l = std::min({fit(w.lines.bytes, 16, 0), fit(w.line_seg.bytes, 4, 0), fit(w.line_cnt.bytes, 4, 8)});
+    uint64_t l = std::min({fit(w.lines.bytes, 16, 0), fit(w.line_cnt.bytes, 4, 8)});
     uint64_t f = fit(w.fills.bytes, 8, 0);
     if (dump) f = std::min(f, fit(w.dump_fills.bytes, 16, 0));
     uint64_t a = std::min(fit(w.alpha_rec.bytes, 16, 0), fit(w.masks.bytes, 256, 0));
@@ -692,7 +699,6 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.col_bd = w.col_bd.as<int32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
     b.lines = w.lines.as<float4>();
-    b.line_seg = w.line_seg.as<uint32_t>();
     b.line_cnt = w.line_cnt.as<uint32_t>();
     b.fills = w.fills.as<uint2>();
     b.dump_fills = dump ? w.dump_fills.as<uint4>() : nullptr;
@@ -722,12 +728,12 @@ static void stage_clear(Workspace &w, const BatchDev &b) {
 }
 
 // flatten (count, scan, emit) and bin count: chords and exact fill / alpha-tile counts
-static void stage_count(Workspace &w, const BatchDev &b) {
+static void stage_count(Workspace &w, const BatchDev &b, const Plan &pl) {
     hipStream_t s = w.stream;
     uint32_t *tmp = w.scan_tmp.as<uint32_t>();
-    HIP_TRY(launch_flatten(b, false, s));
+    HIP_TRY(launch_flatten(b, false, flatten_segs(pl), s));
     HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp, s));
-    HIP_TRY(launch_flatten(b, true, s));
+    HIP_TRY(launch_flatten(b, true, flatten_segs(pl), s));
     HIP_TRY(launch_bin(b, false, s));
     HIP_TRY(scan_tiles(b, w.scan_tmp.ptr, s));
 }
@@ -807,7 +813,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[0], st));
     stage_clear(*ctx->w, b);
-    HIP_TRY(launch_flatten(b, false, st));
+    HIP_TRY(launch_flatten(b, false, flatten_segs(pl), st));
     HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp(), st));
     read_counters(ctx);  // chord count (renderer.cpp:707-724)
     uint32_t nl = ctx->w->h_counters[C_LINES];
@@ -816,7 +822,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
         ensure_sizes(*ctx->w, pl, dump);
         b = batch_dev(*ctx->w, pl, dump);
     }
-    HIP_TRY(launch_flatten(b, true, st));
+    HIP_TRY(launch_flatten(b, true, flatten_segs(pl), st));
     HIP_TRY(hipEventRecord(ev[1], st));
     HIP_TRY(launch_bin(b, false, st));
     HIP_TRY(scan_tiles(b, ctx->w->scan_tmp.ptr, st));
@@ -911,7 +917,7 @@ static void batched_prepare(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint32_
         upload_tables(w, pl);
         BatchDev b = batch_dev(w, pl, bl.dump);
         stage_clear(w, b);
-        HIP_TRY(launch_flatten(b, false, w.stream));
+        HIP_TRY(launch_flatten(b, false, flatten_segs(pl), w.stream));
         HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, w.scan_tmp.as<uint32_t>(), w.stream));
         read_counters(ctx);
         bl.plc = w.h_counters[C_LINES];
@@ -919,7 +925,7 @@ static void batched_prepare(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint32_
         ensure_sizes(w, pl, bl.dump);
         b = batch_dev(w, pl, bl.dump);
         stage_clear(w, b);
-        stage_count(w, b);
+        stage_count(w, b, pl);
         read_counters(ctx);
         bl.pf = w.h_counters[C_FILLS];
         bl.pa = w.h_counters[C_ALPHA];
@@ -957,7 +963,8 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
                      pl.cap_a,
                      pl.cap_e,
                      pl.cap_l,
-                     {pl.sched.fill_grid, pl.sched.composite_block, pl.sched.column_block, pl.sched.xcd_order}};
+                     {pl.sched.fill_grid, pl.sched.composite_block, pl.sched.column_block, pl.sched.xcd_order,
+                      pl.sched.flatten_segs}};
         if (w.graphs_generation != g_alloc_generation) {
             // some buffer moved since these graphs were captured: never replay them
             w.drop_graphs();
@@ -970,7 +977,7 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
             HIP_TRY(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
             try {
                 stage_clear(w, b);
-                stage_count(w, b);
+                stage_count(w, b, pl);
                 stage_emit(w, b, pl);
                 stage_finish(w, b, pl);
             } catch (...) {
@@ -990,7 +997,7 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
         HIP_TRY(hipGraphLaunch(it->second, w.stream));
     } else {
         stage_clear(w, b);
-        stage_count(w, b);
+        stage_count(w, b, pl);
         stage_emit(w, b, pl);
         stage_finish(w, b, pl);
     }
@@ -1678,7 +1685,8 @@ int pfh_mlp_train(pfh_ctx *ctx, const float *x, const float *target, int64_t n, 
 static bool valid_schedule(const pfh_schedule &s) {
     auto blk = [](int32_t v) { return v == 64 || v == 128 || v == 256; };
     return s.fill_grid >= 1 && s.fill_grid <= (1 << 20) && blk(s.composite_block) && blk(s.column_block) &&
-           s.xcd_order >= 0 && s.xcd_order <= 3;
+           s.xcd_order >= 0 && s.xcd_order <= 3 && s.flatten_segs >= 0 && s.flatten_segs <= (1 << 16) &&
+           s.flatten_segs % 64 == 0;
 }
 
 int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out) {
@@ -1687,6 +1695,7 @@ int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out) {
     out->composite_block = (int32_t)ctx->sched.composite_block;
     out->column_block = (int32_t)ctx->sched.column_block;
     out->xcd_order = (int32_t)ctx->sched.xcd_order;
+    out->flatten_segs = (int32_t)ctx->sched.flatten_segs;
     return PFH_OK;
 }
 
@@ -1698,6 +1707,7 @@ int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in) {
         ctx->sched.composite_block = (uint32_t)in->composite_block;
         ctx->sched.column_block = (uint32_t)in->column_block;
         ctx->sched.xcd_order = (uint32_t)in->xcd_order;
+        ctx->sched.flatten_segs = (uint32_t)in->flatten_segs;
     });
 }
 
@@ -1724,15 +1734,16 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         Schedule best = ctx->sched;
         double best_ms = measure();
         if (ms_start) *ms_start = best_ms;
-        const std::vector<uint32_t> cand[4] = {{1024, 2048, 4096, 8192, 16384, 32768}, {64, 128, 256}, {64, 128, 256},
-                                              {0, 1, 2, 3}};
-        for (int param = 0; param < 4; param++) {
+        const std::vector<uint32_t> cand[5] = {{1024, 2048, 4096, 8192, 16384, 32768}, {64, 128, 256}, {64, 128, 256},
+                                              {0, 1, 2, 3}, {0, 128, 256, 512, 1024, 2048}};
+        for (int param = 0; param < 5; param++) {
             for (uint32_t v : cand[param]) {
                 Schedule trial = best;
                 uint32_t *field = param == 0 ? &trial.fill_grid
                                   : param == 1 ? &trial.composite_block
                                   : param == 2 ? &trial.column_block
-                                               : &trial.xcd_order;
+                                  : param == 3 ? &trial.xcd_order
+                                               : &trial.flatten_segs;
                 if (*field == v) continue;
                 *field = v;
                 ctx->sched = trial;
@@ -1748,6 +1759,7 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         out->composite_block = (int32_t)best.composite_block;
         out->column_block = (int32_t)best.column_block;
         out->xcd_order = (int32_t)best.xcd_order;
+        out->flatten_segs = (int32_t)best.flatten_segs;
         if (ms_tuned) *ms_tuned = best_ms;
     });
 }
