@@ -43,6 +43,31 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
     return x * q + min(x, r) + (b >> 3);
 }
 
+// Scene holding batch item i: the largest k in [0, n) with bases[k] <= i, where bases holds
+// the n + 1 scanned per-scene item offsets (bases[0] = 0 <= i < bases[n]).  i is the same in
+// every lane and all 64 lanes are active: a 64-ary search, one vector load and one ballot per
+// round (one round up to 64 scenes, two up to 4096), in place of a chain of dependent loads.
+__device__ __forceinline__ int find_scene_wave(const uint32_t *bases, int n, uint32_t i) {
+    const int lane = (int)(threadIdx.x & 63);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int step = (hi - lo + 64) >> 6;  // candidates lo, lo + step, ... cover [lo, hi]
+        const int cand = lo + lane * step;
+        const uint64_t ok = __ballot(cand <= hi && bases[cand] <= i);  // a prefix of the lanes, lane 0 set
+        lo += (63 - __clzll(ok)) * step;
+        hi = min(hi, lo + step - 1);
+    }
+    return lo;
+}
+
+// Scene of each lane's item, when no lane's item is below lane 0's (all 64 lanes active):
+// lane 0's scene, then a short walk over the scene boundaries inside the wave's items.
+__device__ __forceinline__ int find_scene_lanes(const uint32_t *bases, int n, uint32_t i) {
+    int k = find_scene_wave(bases, n, __builtin_amdgcn_readfirstlane(i));
+    while (bases[k + 1] <= i) k++;
+    return k;
+}
+
 // Counters written by the device pipeline (one block per batch).
 enum Counter {
     C_FILLS = 0,       // total fills (count pass)

@@ -22,16 +22,6 @@ namespace pfh {
 // helpers
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ int find_scene(const uint32_t *bases, int n, uint32_t i) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (bases[mid] <= i) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 __global__ void k_fill_u32(uint32_t *p, uint32_t value, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -290,7 +280,7 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
     float stk[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS][6];
     int sdepth[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS];
     // scene of the next segment to load: segment indices only grow, so it is advanced, not searched
-    int k = find_scene(b.seg_bases, b.n_scenes, (uint32_t)w0);
+    int k = find_scene_wave(b.seg_bases, b.n_scenes, (uint32_t)w0);
     uint32_t next = (uint32_t)w0;           // first segment not yet handed out (wave-uniform)
     uint32_t batch = (uint32_t)w0, batch_end = (uint32_t)w0;  // staged segments [batch, batch_end)
     float2 q0 = {}, q1 = {}, q2 = {}, q3 = {};  // staged segment batch + lane: points
@@ -522,7 +512,7 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wv;
     if (item >= b.n_bin_wave) return;  // whole wave: no block-wide barriers below
-    const int k = find_scene(b.bin_wave_bases, b.n_scenes, item);
+    const int k = find_scene_wave(b.bin_wave_bases, b.n_scenes, item);
     const SceneDev &sc = b.scenes[k];
     const uint32_t p = sc.bin_paths[item - b.bin_wave_bases[k]];
     const int4 rect = sc.path_rect[p];
@@ -564,7 +554,7 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
     __shared__ int32_t s_col[EMIT ? 1 : BIN_GROUP_COLS];
     const uint32_t item = blockIdx.x;
     if (item >= b.n_bin_group) return;  // uniform over the block
-    const int k = find_scene(b.bin_group_bases, b.n_scenes, item);
+    const int k = find_scene_wave(b.bin_group_bases, b.n_scenes, item);
     const SceneDev &sc = b.scenes[k];
     const uint32_t p = sc.bin_paths[sc.n_bin_wave + (item - b.bin_group_bases[k])];
     const int4 rect = sc.path_rect[p];
@@ -802,8 +792,9 @@ struct StoreEntryOff {
 
 __global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane searches (lanes past the end with the last column), then those leave
+    int k = find_scene_lanes(b.col_bases, b.n_scenes, min(c, b.n_cols - 1));
     if (c >= b.n_cols) return;
-    int k = find_scene(b.col_bases, b.n_scenes, c);
     const SceneDev &sc = b.scenes[k];
     uint32_t cl = c - sc.col_base;
     uint32_t p = sc.col_path[cl];
@@ -839,8 +830,8 @@ template <bool WRITE>
 __global__ __launch_bounds__(256) void k_entries(BatchDev b) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n_alpha_entries = 0;
+    const int k = find_scene_lanes(b.col_bases, b.n_scenes, min(c, b.n_cols - 1));
     if (c < b.n_cols) {
-        int k = find_scene(b.col_bases, b.n_scenes, c);
         const SceneDev &sc = b.scenes[k];
         uint32_t cl = c - sc.col_base;
         uint32_t p = sc.col_path[cl];

@@ -266,16 +266,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     const uint32_t lb = (b.xcd_order & 2u) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t f = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (f >= b.n_fb) return;
-    int k = 0;
-    {
-        int lo = 0, hi = b.n_scenes - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (b.fb_bases[mid] <= f) lo = mid;
-            else hi = mid - 1;
-        }
-        k = lo;
-    }
+    const int k = find_scene_wave(b.fb_bases, b.n_scenes, f);
     const SceneDev &sc = b.scenes[k];
     const uint32_t fl = f - sc.fb_base;
     const int tx = sc.fb_x0 + (int)(fl % (uint32_t)sc.fb_w), ty = sc.fb_y0 + (int)(fl / (uint32_t)sc.fb_w);
@@ -313,13 +304,14 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
             }
         }
         const float ca = (float)(c >> 24) * (1.0f / 255.0f);
-        const float cr = (float)(c & 255u) * (1.0f / 255.0f);
-        const float cg = (float)((c >> 8) & 255u) * (1.0f / 255.0f);
-        const float cb = (float)((c >> 16) & 255u) * (1.0f / 255.0f);
         const bool valid = (uint32_t)lane < cnt;
         const bool solid = valid && ai == NO_ALPHA;
-        // premultiplied colour of a solid entry; fold runs with an inclusive segmented scan
-        float pr = cr * ca, pg = cg * ca, pb = cb * ca, pa = ca;
+        // premultiplied colour; an alpha entry's is also scaled by 1/255 so that it multiplies
+        // the raw mask byte.  Runs of solid entries are folded by an inclusive segmented scan.
+        const float sc_a = solid ? ca : ca * (1.0f / 255.0f);
+        const float kc = sc_a * (1.0f / 255.0f);
+        float pr = (float)(c & 255u) * kc, pg = (float)((c >> 8) & 255u) * kc, pb = (float)((c >> 16) & 255u) * kc;
+        float pa = sc_a;
         const uint64_t solid_mask = __ballot(solid);
         // a run starts at a solid lane whose predecessor is not solid
         bool head = solid && (lane == 0 || !((solid_mask >> (lane - 1)) & 1ull));
@@ -338,36 +330,39 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         const uint64_t alpha_mask = valid_mask & ~solid_mask;
         const uint64_t run_end = solid_mask & ~(solid_mask >> 1);
         uint64_t events = alpha_mask | run_end;
+        // the mask of the next alpha entry is loaded one alpha entry ahead, so its latency
+        // overlaps the blend of the current one (and of the solid runs between them)
+        const uint32_t *const mask_words = reinterpret_cast<const uint32_t *>(b.masks) + lane;
+        uint64_t pending = alpha_mask;
+        uint32_t m_next = 0;
+        if (pending) m_next = mask_words[(uint64_t)__builtin_amdgcn_readlane(ai, __builtin_ctzll(pending)) * 64];
         while (events) {
             const int j = __builtin_ctzll(events);
             events &= events - 1;
+            const float fr = lane_f(pr, j), fg = lane_f(pg, j), fb = lane_f(pb, j), fa = lane_f(pa, j);
             if ((alpha_mask >> j) & 1ull) {
                 // no early out for alpha 0: blending with zero coverage leaves the pixel as it is
-                const uint32_t aj = __builtin_amdgcn_readlane(ai, j);
-                const uint32_t cj = __builtin_amdgcn_readlane(c, j);
-                const float ja = (float)(cj >> 24) * (1.0f / (255.0f * 255.0f));  // alpha x mask scale
-                const float jr = (float)(cj & 255u) * (1.0f / 255.0f);
-                const float jg = (float)((cj >> 8) & 255u) * (1.0f / 255.0f);
-                const float jb = (float)((cj >> 16) & 255u) * (1.0f / 255.0f);
-                const uint32_t m = reinterpret_cast<const uint32_t *>(b.masks + (uint64_t)aj * 256)[lane];
+                const uint32_t m = m_next;
+                pending &= pending - 1;
+                if (pending)
+                    m_next = mask_words[(uint64_t)__builtin_amdgcn_readlane(ai, __builtin_ctzll(pending)) * 64];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const float sa = ja * (float)((m >> (8 * q)) & 255u);
-                    const float k1 = 1.0f - sa;
-                    r[q] = jr * sa + r[q] * k1;
-                    g[q] = jg * sa + g[q] * k1;
-                    bl[q] = jb * sa + bl[q] * k1;
-                    al[q] = sa + al[q] * k1;
+                    const float mq = (float)((m >> (8 * q)) & 255u);
+                    const float k1 = fmaf(-fa, mq, 1.0f);
+                    r[q] = fmaf(fr, mq, r[q] * k1);
+                    g[q] = fmaf(fg, mq, g[q] * k1);
+                    bl[q] = fmaf(fb, mq, bl[q] * k1);
+                    al[q] = fmaf(fa, mq, al[q] * k1);
                 }
             } else {
-                const float fr = lane_f(pr, j), fg = lane_f(pg, j), fb = lane_f(pb, j), fa = lane_f(pa, j);
                 const float k1 = 1.0f - fa;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    r[q] = fr + r[q] * k1;
-                    g[q] = fg + g[q] * k1;
-                    bl[q] = fb + bl[q] * k1;
-                    al[q] = fa + al[q] * k1;
+                    r[q] = fmaf(r[q], k1, fr);
+                    g[q] = fmaf(g[q], k1, fg);
+                    bl[q] = fmaf(bl[q], k1, fb);
+                    al[q] = fmaf(al[q], k1, fa);
                 }
             }
         }
