@@ -68,6 +68,14 @@ __device__ __forceinline__ int find_scene_lanes(const uint32_t *bases, int n, ui
     return k;
 }
 
+// What a tile draws, from its fills and propagated backdrop (written by propagate).
+enum TileStatus : uint8_t {
+    TILE_EMPTY = 0,  // no fills, zero backdrop: no draw entry
+    TILE_ALPHA = 1,  // has fills: a coverage mask
+    TILE_SOLID = 2,  // no fills, backdrop covers the whole tile
+    TILE_CLEAR = 3,  // no fills, even-odd rule with an even nonzero backdrop: an entry that covers nothing
+};
+
 // Counters written by the device pipeline (one block per batch).
 enum Counter {
     C_FILLS = 0,       // total fills (count pass)
@@ -125,7 +133,8 @@ struct BatchDev {
     uint32_t *tile_cnt;           // [T] fills per tile (global-atomic paths: then emit cursor)
     uint32_t *tile_fill_off;      // [T + 1] exclusive scan of fill counts
     uint32_t *tile_alpha_idx;     // [T + 1] exclusive scan of alpha flags
-    int8_t *tile_bd;              // [T] propagated backdrop
+    int8_t *tile_bd;              // [T] propagated backdrop (written with dumps only)
+    uint8_t *tile_st;             // [T] tile status (TileStatus)
     uint32_t *tile_entry;         // [T + 1] entry flag, then exclusive scan
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan

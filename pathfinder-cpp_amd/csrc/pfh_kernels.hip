@@ -789,36 +789,82 @@ struct StoreEntryOff {
 // ---------------------------------------------------------------------------
 // Backdrop propagation down each path column (tiler.cpp:369-439) + z-buffer
 // ---------------------------------------------------------------------------
+//
+// The column kernels walk a path column top to bottom, one thread per column.  Their
+// loads do not depend on earlier rows (only the running backdrop does, in registers), so
+// rows are taken COL_ROWS at a time: every load of the group is issued before the first
+// is used, and a wave waits on memory once per group instead of once per row.
+
+constexpr int COL_ROWS = 4;
+
+// Where a tile of a path column lands in its scene's framebuffer-tile grid.
+struct ColumnWalk {
+    uint32_t p, rgba, rule, tbase;
+    int w, h, x, fb_row0;  // fb_row0: fb tile index (scene-local) of row 0
+    uint32_t fb_base, fb_w;
+    __device__ ColumnWalk(const BatchDev &b, int k, uint32_t c) {
+        const SceneDev &sc = b.scenes[k];
+        const uint32_t cl = c - sc.col_base;
+        p = sc.col_path[cl];
+        const int4 r = sc.path_rect[p];
+        w = r.z - r.x;
+        h = r.w - r.y;
+        x = (int)(cl - sc.path_col_off[p]);
+        tbase = sc.tile_base + sc.path_tile_off[p];
+        rgba = sc.path_rgba[p];
+        rule = sc.path_rule[p];
+        fb_base = sc.fb_base;
+        fb_w = (uint32_t)sc.fb_w;
+        fb_row0 = (r.y - sc.fb_y0) * sc.fb_w + (r.x + x - sc.fb_x0);
+    }
+    __device__ uint32_t tile(int row) const { return tbase + (uint32_t)(row * w + x); }
+    __device__ uint32_t fb_local(int row) const { return (uint32_t)(fb_row0 + row * (int)fb_w); }
+};
 
 __global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     // every lane searches (lanes past the end with the last column), then those leave
     int k = find_scene_lanes(b.col_bases, b.n_scenes, min(c, b.n_cols - 1));
     if (c >= b.n_cols) return;
-    const SceneDev &sc = b.scenes[k];
-    uint32_t cl = c - sc.col_base;
-    uint32_t p = sc.col_path[cl];
-    int4 r = sc.path_rect[p];
-    int w = r.z - r.x, h = r.w - r.y;
-    int x = (int)(cl - sc.path_col_off[p]);
-    uint32_t tbase = sc.tile_base + sc.path_tile_off[p];
-    uint32_t rgba = sc.path_rgba[p];
-    uint32_t rule = sc.path_rule[p];
-    bool opaque = (rgba >> 24) == 255u;
+    const ColumnWalk cw(b, k, c);
+    const bool opaque = (cw.rgba >> 24) == 255u;
     int32_t running = b.col_bd[c];
-    for (int row = 0; row < h; row++) {
-        uint32_t t = tbase + (uint32_t)(row * w + x);
-        int8_t bd = (int8_t)running;
-        running += (int8_t)b.tile_delta[t];
-        b.tile_bd[t] = bd;
-        uint32_t f0 = b.tile_fill_off[t], f1 = b.tile_fill_off[t + 1];
-        if (f1 > f0) {
-            uint32_t a = b.tile_alpha_idx[t];
-            if (a < b.cap_alpha) b.alpha_rec[a] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd | (rule << 8), t);
-        } else if (opaque && bd != 0 && !(rule && (bd & 1) == 0)) {
-            int ty = r.y + row, tx = r.x + x;
-            uint32_t fb = sc.fb_base + (uint32_t)((ty - sc.fb_y0) * sc.fb_w + (tx - sc.fb_x0));
-            atomicMax(&b.zbuf[fb], p);
+    for (int row0 = 0; row0 < cw.h; row0 += COL_ROWS) {
+        int32_t delta[COL_ROWS];
+        uint32_t f0[COL_ROWS], f1[COL_ROWS];
+#pragma unroll
+        for (int i = 0; i < COL_ROWS; i++) {
+            delta[i] = 0;
+            f0[i] = f1[i] = 0;
+            if (row0 + i < cw.h) {
+                const uint32_t t = cw.tile(row0 + i);
+                delta[i] = b.tile_delta[t];
+                f0[i] = b.tile_fill_off[t];
+                f1[i] = b.tile_fill_off[t + 1];
+            }
+        }
+        int8_t bd[COL_ROWS];
+        uint32_t ai[COL_ROWS];
+#pragma unroll
+        for (int i = 0; i < COL_ROWS; i++) {
+            bd[i] = (int8_t)running;
+            running += (int8_t)delta[i];
+            ai[i] = 0;
+            if (row0 + i < cw.h && f1[i] > f0[i]) ai[i] = b.tile_alpha_idx[cw.tile(row0 + i)];
+        }
+#pragma unroll
+        for (int i = 0; i < COL_ROWS; i++) {
+            if (row0 + i >= cw.h) break;
+            const uint32_t t = cw.tile(row0 + i);
+            const bool alpha = f1[i] > f0[i];
+            const bool even = cw.rule && (bd[i] & 1) == 0;
+            b.tile_st[t] = alpha ? TILE_ALPHA : (bd[i] == 0 ? TILE_EMPTY : (even ? TILE_CLEAR : TILE_SOLID));
+            if (b.dump) b.tile_bd[t] = bd[i];
+            if (alpha) {
+                if (ai[i] < b.cap_alpha) b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (cw.rule << 8), t);
+            } else if (opaque && bd[i] != 0 && !even) {
+                atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row0 + i)], cw.p);
+            }
         }
     }
     b.col_bd[c] = running;
@@ -832,41 +878,55 @@ __global__ __launch_bounds__(256) void k_entries(BatchDev b) {
     uint32_t n_alpha_entries = 0;
     const int k = find_scene_lanes(b.col_bases, b.n_scenes, min(c, b.n_cols - 1));
     if (c < b.n_cols) {
-        const SceneDev &sc = b.scenes[k];
-        uint32_t cl = c - sc.col_base;
-        uint32_t p = sc.col_path[cl];
-        int4 r = sc.path_rect[p];
-        int w = r.z - r.x, h = r.w - r.y;
-        int x = (int)(cl - sc.path_col_off[p]);
-        uint32_t tbase = sc.tile_base + sc.path_tile_off[p];
-        uint32_t rgba = sc.path_rgba[p];
-        uint32_t rule = sc.path_rule[p];
-        for (int row = 0; row < h; row++) {
-            uint32_t t = tbase + (uint32_t)(row * w + x);
-            int ty = r.y + row, tx = r.x + x;
-            uint32_t fbl = (uint32_t)((ty - sc.fb_y0) * sc.fb_w + (tx - sc.fb_x0));
-            uint32_t fb = sc.fb_base + fbl;
+        const ColumnWalk cw(b, k, c);
+        for (int row0 = 0; row0 < cw.h; row0 += COL_ROWS) {
             if (!WRITE) {
-                bool alpha = b.tile_fill_off[t + 1] > b.tile_fill_off[t];
-                int8_t bd = b.tile_bd[t];
-                bool keep = (alpha || bd != 0) && (int32_t)p >= (int32_t)b.zbuf[fb];
-                b.tile_entry[t] = keep ? 1u : 0u;
-            } else {
-                uint32_t e = b.tile_entry[t];
-                if (b.tile_entry[t + 1] == e) continue;
-                if (e >= b.cap_entries) continue;
-                bool alpha = b.tile_fill_off[t + 1] > b.tile_fill_off[t];
-                uint32_t ai = alpha ? b.tile_alpha_idx[t] : NO_ALPHA;
-                uint32_t eff = rgba;
-                if (!alpha) {
-                    int8_t bd = b.tile_bd[t];
-                    if (rule && (bd & 1) == 0) eff &= 0x00ffffffu;  // even-odd, even winding: covers nothing
+                uint32_t st[COL_ROWS], z[COL_ROWS];
+#pragma unroll
+                for (int i = 0; i < COL_ROWS; i++) {
+                    st[i] = TILE_EMPTY;
+                    z[i] = 0;
+                    if (row0 + i < cw.h) {
+                        st[i] = b.tile_st[cw.tile(row0 + i)];
+                        z[i] = b.zbuf[cw.fb_base + cw.fb_local(row0 + i)];
+                    }
                 }
-                b.keys[e] = fb;
-                b.vals[e] = e;
-                b.entry_rec[e] = make_uint4(ai, eff, p, fbl);
-                atomicAdd(&b.fb_cnt[fb], 1u);
-                n_alpha_entries += alpha ? 1u : 0u;
+#pragma unroll
+                for (int i = 0; i < COL_ROWS; i++)
+                    if (row0 + i < cw.h)
+                        b.tile_entry[cw.tile(row0 + i)] = (st[i] != TILE_EMPTY && (int32_t)cw.p >= (int32_t)z[i]) ? 1u : 0u;
+            } else {
+                uint32_t e0[COL_ROWS], e1[COL_ROWS], st[COL_ROWS];
+#pragma unroll
+                for (int i = 0; i < COL_ROWS; i++) {
+                    e0[i] = e1[i] = 0;
+                    st[i] = TILE_EMPTY;
+                    if (row0 + i < cw.h) {
+                        const uint32_t t = cw.tile(row0 + i);
+                        e0[i] = b.tile_entry[t];
+                        e1[i] = b.tile_entry[t + 1];
+                        st[i] = b.tile_st[t];
+                    }
+                }
+                uint32_t ai[COL_ROWS];
+#pragma unroll
+                for (int i = 0; i < COL_ROWS; i++) {
+                    ai[i] = NO_ALPHA;
+                    if (e1[i] != e0[i] && st[i] == TILE_ALPHA) ai[i] = b.tile_alpha_idx[cw.tile(row0 + i)];
+                }
+#pragma unroll
+                for (int i = 0; i < COL_ROWS; i++) {
+                    const uint32_t e = e0[i];
+                    if (e1[i] == e || e >= b.cap_entries) continue;  // no entry, or past the capacity
+                    const uint32_t fbl = cw.fb_local(row0 + i), fb = cw.fb_base + fbl;
+                    // even-odd solid tile with an even winding covers nothing: zero alpha
+                    const uint32_t eff = st[i] == TILE_CLEAR ? (cw.rgba & 0x00ffffffu) : cw.rgba;
+                    b.keys[e] = fb;
+                    b.vals[e] = e;
+                    b.entry_rec[e] = make_uint4(ai[i], eff, cw.p, fbl);
+                    atomicAdd(&b.fb_cnt[fb], 1u);
+                    n_alpha_entries += st[i] == TILE_ALPHA ? 1u : 0u;
+                }
             }
         }
     }

@@ -156,7 +156,7 @@ struct Workspace {
     // around the fill and composite kernels of a batched launch; recorded as event nodes
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
-    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_entry, col_bd,
+    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, tile_entry, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_cnt, keys, vals, keys_alt,
         vals_alt, entry_rec, counters, scan_tmp, sort_tmp;
     uint32_t *h_counters = nullptr;  // pinned
@@ -180,7 +180,7 @@ struct Workspace {
         graphs.clear();
     }
     std::vector<DevBuf *> all_buffers() {
-        return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd,
+        return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
                 &tile_entry, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
                 &masks, &zbuf, &fb_cnt, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
                 &sort_tmp};
@@ -526,7 +526,8 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.tile_cnt.ensure(4 * pl.T + 4);
     w.tile_fill_off.ensure(4 * pl.T + 8);
     w.tile_alpha_idx.ensure(4 * pl.T + 8);
-    w.tile_bd.ensure(pl.T + 4);
+    if (dump) w.tile_bd.ensure(pl.T + 4);
+    w.tile_st.ensure(pl.T + 4);
     w.tile_entry.ensure(4 * pl.T + 8);
     w.col_bd.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
@@ -694,7 +695,8 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.tile_cnt = w.tile_cnt.as<uint32_t>();
     b.tile_fill_off = w.tile_fill_off.as<uint32_t>();
     b.tile_alpha_idx = w.tile_alpha_idx.as<uint32_t>();
-    b.tile_bd = w.tile_bd.as<int8_t>();
+    b.tile_bd = dump ? w.tile_bd.as<int8_t>() : nullptr;
+    b.tile_st = w.tile_st.as<uint8_t>();
     b.tile_entry = w.tile_entry.as<uint32_t>();
     b.col_bd = w.col_bd.as<int32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
@@ -1109,7 +1111,7 @@ static uint64_t scene_launch_bytes(const pfh_ctx *ctx, const SceneHost &s) {
         A = s.n_tiles;
     }
     const double T = s.n_tiles, W = s.n_cols, S = s.n_segs, FB = (double)s.fb_w * s.fb_h;
-    const double per_tile = 4 + 4 + 8 + 1 + 8 + 4 * 4 + 16;  // counts, offsets, backdrop, entries, sort, records
+    const double per_tile = 4 + 4 + 8 + 1 + 8 + 4 * 4 + 16;  // counts, offsets, status, entries, sort, records
     double b = T * per_tile + 4 * W + 8 * S + 8 * FB + 24 * L + 8 * F + (16 + 256) * A;
     return (uint64_t)b + 4096;
 }
