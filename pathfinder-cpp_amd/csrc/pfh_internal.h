@@ -145,7 +145,7 @@ struct BatchDev {
     uint4 *alpha_rec;             // [A] (fill begin, fill end, backdrop | rule << 8, tile)
     uint8_t *masks;               // [A * 256]
     uint32_t *zbuf;               // [FB]
-    uint32_t *fb_cnt;             // [FB + 1] entries per framebuffer tile, then offsets
+    uint2 *fb_range;              // [FB] [start, end) of each framebuffer tile's entries in sort order
     uint32_t *keys, *vals;        // [E] sort keys / values
     uint32_t *keys_alt, *vals_alt;
     uint4 *entry_rec;             // [E] (alpha index | NO_ALPHA, rgba (alpha 0 if covers nothing), local path, fb)

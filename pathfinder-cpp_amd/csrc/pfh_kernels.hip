@@ -921,10 +921,8 @@ __global__ __launch_bounds__(256) void k_entries(BatchDev b) {
                     const uint32_t fbl = cw.fb_local(row0 + i), fb = cw.fb_base + fbl;
                     // even-odd solid tile with an even winding covers nothing: zero alpha
                     const uint32_t eff = st[i] == TILE_CLEAR ? (cw.rgba & 0x00ffffffu) : cw.rgba;
-                    b.keys[e] = fb;
-                    b.vals[e] = e;
+                    b.keys[e] = fb;  // its value, e, is generated by the sort's first pass
                     b.entry_rec[e] = make_uint4(ai[i], eff, cw.p, fbl);
-                    atomicAdd(&b.fb_cnt[fb], 1u);
                     n_alpha_entries += st[i] == TILE_ALPHA ? 1u : 0u;
                 }
             }
@@ -966,7 +964,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t *keys, co
 // in digit order: runs of equal digits land at consecutive global positions, so stores
 // coalesce instead of scattering one item per digit per row.  offs is the exclusive scan of
 // the [digit][block] histogram: the block's count of digit d is the difference of its
-// entry and the next one.
+// entry and the next one.  With no vals, an item's value is its input index.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *keys, const uint32_t *vals,
                                                            uint32_t *keys_out, uint32_t *vals_out,
                                                            const uint32_t *n_ptr, uint32_t cap, int shift,
@@ -997,7 +995,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *keys,
         const uint32_t li = (uint32_t)r * RS_THREADS + threadIdx.x;
         const bool valid = li < valid_n;
         const uint32_t key = valid ? keys[base + li] : 0u;
-        const uint32_t val = valid ? vals[base + li] : 0u;
+        const uint32_t val = !valid ? 0u : (vals ? vals[base + li] : (uint32_t)(base + li));
         const uint32_t d = (key >> shift) & 255u;
         // lanes holding the same digit (multisplit peer mask)
         uint64_t peers = __ballot(valid);
@@ -1120,10 +1118,52 @@ hipError_t launch_entries(const BatchDev &b, bool write, uint32_t block, hipStre
     return hipGetLastError();
 }
 
+// Range [start, end) of every framebuffer tile's run in the sorted keys, marked at the run
+// boundaries: item i starts its key's run when the key before differs and ends it when the
+// key after differs.  Tiles with no entries keep the cleared (0, 0).
+constexpr int FB_BOUNDS_ITEMS = 4;  // items per thread per step, all loads issued first
+
+__global__ __launch_bounds__(256) void k_fb_bounds(const uint32_t *keys, const uint32_t *n_ptr, uint32_t cap,
+                                                   uint32_t n_fb, uint2 *range) {
+    const uint32_t n = min(*n_ptr, cap);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += FB_BOUNDS_ITEMS * stride) {
+        uint32_t prev[FB_BOUNDS_ITEMS], key[FB_BOUNDS_ITEMS], next[FB_BOUNDS_ITEMS];
+#pragma unroll
+        for (int j = 0; j < FB_BOUNDS_ITEMS; j++) {
+            const uint64_t i = i0 + j * stride;
+            key[j] = n_fb;  // marks nothing
+            prev[j] = next[j] = n_fb + 1;
+            if (i < n) {
+                key[j] = keys[i];
+                if (i > 0) prev[j] = keys[i - 1];
+                if (i + 1 < n) next[j] = keys[i + 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < FB_BOUNDS_ITEMS; j++) {
+            const uint64_t i = i0 + j * stride;
+            if (key[j] >= n_fb) continue;  // past the count (keys of the entry pass are below n_fb)
+            if (prev[j] != key[j]) range[key[j]].x = (uint32_t)i;
+            if (next[j] != key[j]) range[key[j]].y = (uint32_t)(i + 1);
+        }
+    }
+}
+
+hipError_t launch_fb_bounds(const uint32_t *sorted_keys, const uint32_t *n_ptr, uint32_t cap, uint32_t n_fb,
+                            uint2 *range, hipStream_t s) {
+    // the count is known on the device only: a grid for the capacity, strided
+    const uint32_t nb = std::min<uint32_t>(blocks_for(cap, 256 * FB_BOUNDS_ITEMS), 16384);
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fb_bounds, dim3(nb), dim3(256), 0, s, sorted_keys, n_ptr, cap, n_fb, range);
+    return hipGetLastError();
+}
+
 size_t radix_temp_elems(uint64_t cap) { return 256ull * blocks_for(cap, RS_TILE) + 1 + scan_temp_elems(256ull * blocks_for(cap, RS_TILE)); }
 
-// Sorts (keys, vals)[0 .. *n_ptr) by the low `bits` bits of the key; returns the buffer
-// pair holding the result via *result_alt (false: keys/vals, true: keys_alt/vals_alt).
+// Sorts keys[0 .. *n_ptr), with values starting as the item indices (vals is not read), by
+// the low `bits` bits of the key, stably; returns the buffer pair holding the result via
+// *result_alt (false: keys/vals, true: keys_alt/vals_alt).
 hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, const uint32_t *n_ptr,
                       uint32_t cap, int bits, uint32_t *tmp, bool *result_alt, hipStream_t s) {
     uint32_t nb = blocks_for(cap, RS_TILE);
@@ -1134,7 +1174,7 @@ hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32
     bool alt = false;
     for (int shift = 0; shift < bits; shift += 8) {
         const uint32_t *ki = alt ? keys_alt : keys;
-        const uint32_t *vi = alt ? vals_alt : vals;
+        const uint32_t *vi = shift == 0 ? nullptr : (alt ? vals_alt : vals);  // values start as the indices
         uint32_t *ko = alt ? keys : keys_alt;
         uint32_t *vo = alt ? vals : vals_alt;
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, ki, n_ptr, cap, shift, hist, nb);

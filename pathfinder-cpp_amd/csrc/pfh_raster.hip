@@ -289,7 +289,8 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
 #pragma unroll
         for (int j = 0; j < 4; j++) r[j] = g[j] = bl[j] = al[j] = 0.0f;
     }
-    uint32_t e0 = b.fb_cnt[f], e1 = min(b.fb_cnt[f + 1], b.cap_entries);
+    const uint2 range = b.fb_range[f];
+    const uint32_t e0 = range.x, e1 = min(range.y, b.cap_entries);
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t i = base + lane;
         const uint32_t cnt = min(64u, e1 - base);

@@ -32,6 +32,8 @@ hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, ui
 hipError_t scan_total(uint32_t *inout, uint32_t n, uint32_t *counters, int counter, uint32_t *tmp, hipStream_t s);
 hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s);
 hipError_t launch_entries(const BatchDev &b, bool write, uint32_t block, hipStream_t s);
+hipError_t launch_fb_bounds(const uint32_t *sorted_keys, const uint32_t *n_ptr, uint32_t cap, uint32_t n_fb,
+                            uint2 *range, hipStream_t s);
 size_t radix_temp_elems(uint64_t cap);
 hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, const uint32_t *n_ptr,
                       uint32_t cap, int bits, uint32_t *tmp, bool *result_alt, hipStream_t s);
@@ -157,7 +159,7 @@ struct Workspace {
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, tile_entry, col_bd,
-        seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_cnt, keys, vals, keys_alt,
+        seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, counters, scan_tmp, sort_tmp;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
@@ -182,7 +184,7 @@ struct Workspace {
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
                 &tile_entry, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
-                &masks, &zbuf, &fb_cnt, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
+                &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
                 &sort_tmp};
     }
     size_t device_bytes() {
@@ -534,7 +536,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.lines.ensure(16ull * pl.cap_l);
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
     w.zbuf.ensure(4 * pl.FB + 4);
-    w.fb_cnt.ensure(4 * pl.FB + 8);
+    w.fb_range.ensure(8 * pl.FB + 8);
     w.counters.ensure(4 * C_COUNT);
     uint64_t scan_n = std::max<uint64_t>({pl.T + 1, pl.S + 1, pl.FB + 1, (uint64_t)pl.cap_l + 1, 1});
     w.scan_tmp.ensure(16 * scan_temp_elems(scan_n) + 64);
@@ -707,7 +709,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.alpha_rec = w.alpha_rec.as<uint4>();
     b.masks = w.masks.as<uint8_t>();
     b.zbuf = w.zbuf.as<uint32_t>();
-    b.fb_cnt = w.fb_cnt.as<uint32_t>();
+    b.fb_range = w.fb_range.as<uint2>();
     b.keys = w.keys.as<uint32_t>();
     b.vals = w.vals.as<uint32_t>();
     b.keys_alt = w.keys_alt.as<uint32_t>();
@@ -725,7 +727,7 @@ static void stage_clear(Workspace &w, const BatchDev &b) {
     // tile_delta / tile_cnt / col_bd need no clearing: the bin count pass stores every
     // tile and column of every binned path (paths with no columns own none)
     HIP_TRY(launch_fill_u32(b.zbuf, 0, b.n_fb, s));
-    HIP_TRY(launch_fill_u32(b.fb_cnt, 0, (uint64_t)b.n_fb + 1, s));
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.fb_range), 0, 2 * (uint64_t)b.n_fb, s));
     HIP_TRY(launch_fill_u32(b.counters, 0, C_COUNT, s));
 }
 
@@ -752,11 +754,12 @@ static void stage_emit(Workspace &w, const BatchDev &b, const Plan &pl) {
 
 static void stage_finish(Workspace &w, const BatchDev &b, const Plan &pl) {
     HIP_TRY(launch_entries(b, true, pl.sched.column_block, w.stream));
-    HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, w.scan_tmp.as<uint32_t>(), w.stream));
     bool alt = false;
     HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
                        w.sort_tmp.as<uint32_t>(), &alt, w.stream));
     w.last_sorted_alt = alt;
+    HIP_TRY(launch_fb_bounds(alt ? b.keys_alt : b.keys, b.counters + C_ENTRIES, b.cap_entries, b.n_fb, b.fb_range,
+                             w.stream));
     HIP_TRY(hipEventRecordWithFlags(w.kev[0], w.stream, hipEventRecordExternal));
     HIP_TRY(launch_fill_masks(b, b.cap_alpha, pl.sched.fill_grid, w.stream));
     HIP_TRY(hipEventRecordWithFlags(w.kev[1], w.stream, hipEventRecordExternal));
@@ -846,11 +849,11 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp(), st));
     HIP_TRY(launch_entries(b, true, pl.sched.column_block, st));
     HIP_TRY(hipEventRecord(ev[5], st));
-    HIP_TRY(scan_u32(b.fb_cnt, b.fb_cnt, b.n_fb, tmp(), st));
     bool alt = false;
     HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
                        ctx->w->sort_tmp.as<uint32_t>(), &alt, st));
     ctx->w->last_sorted_alt = alt;
+    HIP_TRY(launch_fb_bounds(alt ? b.keys_alt : b.keys, b.counters + C_ENTRIES, b.cap_entries, b.n_fb, b.fb_range, st));
     HIP_TRY(hipEventRecord(ev[6], st));
     HIP_TRY(launch_fill_masks(b, na, pl.sched.fill_grid, st));
     HIP_TRY(hipEventRecord(ev[7], st));
@@ -1112,7 +1115,7 @@ static uint64_t scene_launch_bytes(const pfh_ctx *ctx, const SceneHost &s) {
     }
     const double T = s.n_tiles, W = s.n_cols, S = s.n_segs, FB = (double)s.fb_w * s.fb_h;
     const double per_tile = 4 + 4 + 8 + 1 + 8 + 4 * 4 + 16;  // counts, offsets, status, entries, sort, records
-    double b = T * per_tile + 4 * W + 8 * S + 8 * FB + 24 * L + 8 * F + (16 + 256) * A;
+    double b = T * per_tile + 4 * W + 8 * S + 12 * FB + 24 * L + 8 * F + (16 + 256) * A;
     return (uint64_t)b + 4096;
 }
 
@@ -1377,15 +1380,16 @@ int pfh_dump_counts(pfh_ctx *ctx, int32_t scene_id, int64_t *counts) {
         if (!s.dumped) throw PfhError{PFH_E_STATE, "scene was not rasterized with PFH_DUMP"};
         HIP_TRY(hipStreamSynchronize(ctx->w->stream));
         uint32_t f0 = 0, f1 = 0, a0 = 0, a1 = 0, e0 = 0, e1 = 0, l0 = 0, l1 = 0;
-        uint32_t sb = s.seg_base, tb = s.tile_base, fb = s.fb_base, nfb = (uint32_t)(s.fb_w * s.fb_h);
+        uint32_t sb = s.seg_base, tb = s.tile_base, nfb = (uint32_t)(s.fb_w * s.fb_h);
         HIP_TRY(hipMemcpy(&l0, ctx->w->seg_cnt.as<uint32_t>() + sb, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&l1, ctx->w->seg_cnt.as<uint32_t>() + sb + s.n_segs, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&f0, ctx->w->line_cnt.as<uint32_t>() + l0, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&f1, ctx->w->line_cnt.as<uint32_t>() + l1, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&a0, ctx->w->tile_alpha_idx.as<uint32_t>() + tb, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&a1, ctx->w->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&e0, ctx->w->fb_cnt.as<uint32_t>() + fb, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&e1, ctx->w->fb_cnt.as<uint32_t>() + fb + nfb, 4, hipMemcpyDeviceToHost));
+        // a scene's entries keep their index range through the sort (keys are grouped by scene)
+        HIP_TRY(hipMemcpy(&e0, ctx->w->tile_entry.as<uint32_t>() + tb, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&e1, ctx->w->tile_entry.as<uint32_t>() + tb + s.n_tiles, 4, hipMemcpyDeviceToHost));
         counts[0] = s.n_tiles;
         counts[1] = s.n_cols;
         counts[2] = f1 - f0;
@@ -1442,12 +1446,14 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
             fill_seg[4 * i + 3] = (uint16_t)(df[i].y >> 16);
             fill_tile[i] = df[i].z - tb - s.path_tile_off[p];
         }
-        std::vector<uint32_t> fboff(nfb + 1);
-        d2h(fboff.data(), ctx->w->fb_cnt.as<uint32_t>() + fbb, 4 * (nfb + 1));
-        uint32_t ne = fboff[nfb] - fboff[0];
+        // a scene's entries keep their index range through the sort (keys are grouped by scene)
+        uint32_t ebase = 0, eend = 0;
+        d2h(&ebase, ctx->w->tile_entry.as<uint32_t>() + tb, 4);
+        d2h(&eend, ctx->w->tile_entry.as<uint32_t>() + tb + s.n_tiles, 4);
+        uint32_t ne = eend - ebase;
         std::vector<uint32_t> sv(ne);
         const uint32_t *sorted = ctx->w->last_sorted_alt ? ctx->w->vals_alt.as<uint32_t>() : ctx->w->vals.as<uint32_t>();
-        d2h(sv.data(), sorted + fboff[0], 4 * (size_t)ne);
+        d2h(sv.data(), sorted + ebase, 4 * (size_t)ne);
         uint32_t elo = 0xffffffffu, ehi = 0;
         for (uint32_t v : sv) {
             elo = std::min(elo, v);
