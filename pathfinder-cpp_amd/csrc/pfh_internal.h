@@ -146,6 +146,7 @@ struct BatchDev {
     uint8_t *masks;               // [A * 256]
     uint32_t *zbuf;               // [FB]
     uint2 *fb_range;              // [FB] [start, end) of each framebuffer tile's entries in sort order
+    const uint32_t *ones_mask;    // [64] an all-ones coverage mask row (full coverage)
     uint32_t *keys, *vals;        // [E] sort keys / values
     uint32_t *keys_alt, *vals_alt;
     uint4 *entry_rec;             // [E] (alpha index | NO_ALPHA, rgba (alpha 0 if covers nothing), local path, fb)

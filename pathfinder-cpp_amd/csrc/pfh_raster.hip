@@ -228,14 +228,19 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
 // Composite
 // ---------------------------------------------------------------------------
 //
-// One wave per framebuffer tile, 4 pixels of one row per lane.  Entries arrive in
-// draw order (ascending path index).  Premultiplied src-over is associative, so a
-// run of consecutive solid entries (whose coverage is the same 0 or 1 for every
-// pixel of the tile) is folded into one premultiplied colour by a segmented scan
-// across the wave; the per-pixel loop then blends only alpha entries and folded
-// runs.  The colour model follows shaders/d3d9/tile.frag:590-653 (mask alpha,
-// premultiply) and BlendState::from_over (gpu/base.h:83-93); layers accumulate in
-// fp32 and the framebuffer is stored once as RGBA8 (premultiplied).
+// One wave per framebuffer tile, 4 pixels of one column per lane.  Entries arrive in
+// draw order (ascending path index) and are composited front to back: from the top
+// entry down, each adds its premultiplied colour weighted by the transmittance T left
+// by the entries above it, and T shrinks by its coverage alpha.  Per pixel and entry
+// that is 6 operations for a masked entry (coverage, weight, 3 colours, T) where
+// back-to-front "over" needs 10.  The framebuffer's alpha is 1 - T at the end (over a
+// loaded destination: the destination weighted by T is added).  Premultiplied src-over is
+// associative, so a run of consecutive solid entries (whose coverage is the same 0 or 1
+// for every pixel of the tile) is folded into one premultiplied colour by a segmented scan
+// across the wave, and the per-pixel loop blends only masked entries and folded runs.
+// The colour model follows shaders/d3d9/tile.frag:590-653 (mask alpha, premultiply) and
+// BlendState::from_over (gpu/base.h:83-93); layers accumulate in fp32 and the framebuffer
+// is stored once as RGBA8 (premultiplied).
 
 // One step of the segmented inclusive scan of premultiplied colours under "over": the value
 // from the DPP source lane lies below this lane's accumulated run unless a run starts (or a
@@ -273,25 +278,16 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     // a lane owns 4 vertically adjacent pixels: column lane / 4, rows 4 (lane % 4) .. + 3,
     // matching the column-major coverage masks (one 32-bit mask word per lane)
     const int x = tx * TILE + (lane >> 2), y0 = ty * TILE + (lane & 3) * 4;
-    float r[4], g[4], bl[4], al[4];
-    uint8_t *const cptr = sc.out + (int64_t)y0 * sc.out_pitch + (int64_t)x * 4;  // this lane's top pixel
-    if (sc.load_dst && x < sc.width) {
+    float r[4], g[4], bl[4], tr[4];  // accumulated premultiplied colour, transmittance
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            uint32_t px = 0;
-            if (y0 + j < sc.height) px = *reinterpret_cast<const uint32_t *>(cptr + j * sc.out_pitch);
-            r[j] = (float)(px & 255u) * (1.0f / 255.0f);
-            g[j] = (float)((px >> 8) & 255u) * (1.0f / 255.0f);
-            bl[j] = (float)((px >> 16) & 255u) * (1.0f / 255.0f);
-            al[j] = (float)(px >> 24) * (1.0f / 255.0f);
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < 4; j++) r[j] = g[j] = bl[j] = al[j] = 0.0f;
+    for (int q = 0; q < 4; q++) {
+        r[q] = g[q] = bl[q] = 0.0f;
+        tr[q] = 1.0f;
     }
     const uint2 range = b.fb_range[f];
     const uint32_t e0 = range.x, e1 = min(range.y, b.cap_entries);
-    for (uint32_t base = e0; base < e1; base += 64) {
+    // chunks of 64 entries from the top one down
+    for (uint32_t base = e0 + (e1 > e0 ? (e1 - e0 - 1) / 64 * 64 : 0); e1 > e0; base -= 64) {
         const uint32_t i = base + lane;
         const uint32_t cnt = min(64u, e1 - base);
         uint32_t ai = NO_ALPHA, c = 0;
@@ -307,7 +303,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         const float ca = (float)(c >> 24) * (1.0f / 255.0f);
         const bool valid = (uint32_t)lane < cnt;
         const bool solid = valid && ai == NO_ALPHA;
-        // premultiplied colour; an alpha entry's is also scaled by 1/255 so that it multiplies
+        // premultiplied colour; a masked entry's is also scaled by 1/255 so that it multiplies
         // the raw mask byte.  Runs of solid entries are folded by an inclusive segmented scan.
         const float sc_a = solid ? ca : ca * (1.0f / 255.0f);
         const float kc = sc_a * (1.0f / 255.0f);
@@ -326,49 +322,77 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         over_step<0x118, 0xf>(rl >= 8, pr, pg, pb, pa, flag);
         over_step<0x142, 0xa>(row == 1 || row == 3, pr, pg, pb, pa, flag);
         over_step<0x143, 0xc>(row >= 2, pr, pg, pb, pa, flag);
-        // events: alpha entries, and the last entry of each solid run
+        // events, taken from the top: masked entries, and the last (top) entry of each solid
+        // run, which holds the run's folded colour.  Every event blends the same way: a solid
+        // run samples an all-ones mask row, and its colour is scaled by 1/255 like a masked
+        // entry's, so the loop has a single path.
+        if (solid) {
+            pr *= 1.0f / 255.0f;
+            pg *= 1.0f / 255.0f;
+            pb *= 1.0f / 255.0f;
+            pa *= 1.0f / 255.0f;
+        }
         const uint64_t valid_mask = __ballot(valid);
         const uint64_t alpha_mask = valid_mask & ~solid_mask;
         const uint64_t run_end = solid_mask & ~(solid_mask >> 1);
         uint64_t events = alpha_mask | run_end;
-        // the mask of the next alpha entry is loaded one alpha entry ahead, so its latency
-        // overlaps the blend of the current one (and of the solid runs between them)
-        const uint32_t *const mask_words = reinterpret_cast<const uint32_t *>(b.masks) + lane;
-        uint64_t pending = alpha_mask;
-        uint32_t m_next = 0;
-        if (pending) m_next = mask_words[(uint64_t)__builtin_amdgcn_readlane(ai, __builtin_ctzll(pending)) * 64];
-        while (events) {
-            const int j = __builtin_ctzll(events);
-            events &= events - 1;
+        // this lane's mask word for event j (ai is NO_ALPHA on solid lanes)
+        auto mask_word = [&](int j) -> uint32_t {
+            const uint32_t idx = __builtin_amdgcn_readlane(ai, j);
+            const uint32_t *row_words = idx == NO_ALPHA ? b.ones_mask : reinterpret_cast<const uint32_t *>(b.masks) + (uint64_t)idx * 64;
+            return row_words[lane];
+        };
+        auto blend = [&](int j, uint32_t m) {
             const float fr = lane_f(pr, j), fg = lane_f(pg, j), fb = lane_f(pb, j), fa = lane_f(pa, j);
-            if ((alpha_mask >> j) & 1ull) {
-                // no early out for alpha 0: blending with zero coverage leaves the pixel as it is
-                const uint32_t m = m_next;
-                pending &= pending - 1;
-                if (pending)
-                    m_next = mask_words[(uint64_t)__builtin_amdgcn_readlane(ai, __builtin_ctzll(pending)) * 64];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const float mq = (float)((m >> (8 * q)) & 255u);
-                    const float k1 = fmaf(-fa, mq, 1.0f);
-                    r[q] = fmaf(fr, mq, r[q] * k1);
-                    g[q] = fmaf(fg, mq, g[q] * k1);
-                    bl[q] = fmaf(fb, mq, bl[q] * k1);
-                    al[q] = fmaf(fa, mq, al[q] * k1);
-                }
-            } else {
-                const float k1 = 1.0f - fa;
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    r[q] = fmaf(r[q], k1, fr);
-                    g[q] = fmaf(g[q], k1, fg);
-                    bl[q] = fmaf(bl[q], k1, fb);
-                    al[q] = fmaf(al[q], k1, fa);
-                }
+            for (int q = 0; q < 4; q++) {
+                const float wq = tr[q] * (float)((m >> (8 * q)) & 255u);
+                r[q] = fmaf(wq, fr, r[q]);
+                g[q] = fmaf(wq, fg, g[q]);
+                bl[q] = fmaf(wq, fb, bl[q]);
+                tr[q] = fmaf(-wq, fa, tr[q]);
             }
+        };
+        // The next event's mask word is loaded while the current one blends.  Two registers
+        // take turns (the loop is unrolled by two) so that no copy waits for a load, and the
+        // body has no exit between its halves, so the loads stay ahead of the blends: an odd
+        // count ends with a blend of coverage 0, which changes nothing.
+        uint32_t ma = events ? mask_word(63 - __clzll(events)) : 0u, mb = 0;
+        while (events) {
+            const int j = 63 - __clzll(events);
+            events &= ~(1ull << j);
+            const bool has_next = events != 0;
+            const int jn = has_next ? 63 - __clzll(events) : j;
+            mb = mask_word(jn);
+            __builtin_amdgcn_sched_barrier(0);  // keep each load ahead of the blend it overlaps
+            blend(j, ma);
+            events &= ~(1ull << jn);
+            __builtin_amdgcn_sched_barrier(0);
+            ma = mask_word(events ? 63 - __clzll(events) : jn);
+            __builtin_amdgcn_sched_barrier(0);
+            blend(jn, has_next ? mb : 0u);
+            __builtin_amdgcn_sched_barrier(0);
         }
+        if (base == e0) break;
     }
     if (x >= sc.width) return;
+    uint8_t *const cptr = sc.out + (int64_t)y0 * sc.out_pitch + (int64_t)x * 4;  // this lane's top pixel
+    float al[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) al[q] = 1.0f - tr[q];
+    if (sc.load_dst) {
+        // the destination lies below every entry: it shows through the remaining transmittance
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t px = 0;
+            if (y0 + q < sc.height) px = *reinterpret_cast<const uint32_t *>(cptr + q * sc.out_pitch);
+            const float wq = tr[q] * (1.0f / 255.0f);
+            r[q] = fmaf(wq, (float)(px & 255u), r[q]);
+            g[q] = fmaf(wq, (float)((px >> 8) & 255u), g[q]);
+            bl[q] = fmaf(wq, (float)((px >> 16) & 255u), bl[q]);
+            al[q] = fmaf(wq, (float)(px >> 24), al[q]);
+        }
+    }
     uint32_t px[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {

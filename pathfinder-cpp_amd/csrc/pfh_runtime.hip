@@ -160,7 +160,7 @@ struct Workspace {
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, tile_entry, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_range, keys, vals, keys_alt,
-        vals_alt, entry_rec, counters, scan_tmp, sort_tmp;
+        vals_alt, entry_rec, counters, scan_tmp, sort_tmp, ones_mask;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
     size_t h_table_cap = 0;
@@ -185,7 +185,7 @@ struct Workspace {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
                 &tile_entry, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
-                &sort_tmp};
+                &sort_tmp, &ones_mask};
     }
     size_t device_bytes() {
         size_t n = 0;
@@ -537,6 +537,10 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
     w.zbuf.ensure(4 * pl.FB + 4);
     w.fb_range.ensure(8 * pl.FB + 8);
+    if (!w.ones_mask.ptr) {
+        w.ones_mask.ensure(256);
+        HIP_TRY(launch_fill_u32(w.ones_mask.as<uint32_t>(), 0xffffffffu, 64, w.stream));
+    }
     w.counters.ensure(4 * C_COUNT);
     uint64_t scan_n = std::max<uint64_t>({pl.T + 1, pl.S + 1, pl.FB + 1, (uint64_t)pl.cap_l + 1, 1});
     w.scan_tmp.ensure(16 * scan_temp_elems(scan_n) + 64);
@@ -710,6 +714,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.masks = w.masks.as<uint8_t>();
     b.zbuf = w.zbuf.as<uint32_t>();
     b.fb_range = w.fb_range.as<uint2>();
+    b.ones_mask = w.ones_mask.as<uint32_t>();
     b.keys = w.keys.as<uint32_t>();
     b.vals = w.vals.as<uint32_t>();
     b.keys_alt = w.keys_alt.as<uint32_t>();
