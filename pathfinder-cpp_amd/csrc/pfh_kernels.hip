@@ -871,7 +871,11 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
 }
 
 // Draw entries: tiles with a mask or a nonzero backdrop that survive the z-cull
-// (core/d3d9/scene_builder.cpp:57-79, shaders/d3d9/tile.vert:103-118).
+// (core/d3d9/scene_builder.cpp:57-79, shaders/d3d9/tile.vert:103-118).  The count pass
+// stores one count per path column; after their scan the write pass numbers a column's
+// entries from its offset, top to bottom.  Entries of a path precede those of the next
+// (columns are in path order), which is all the sort by framebuffer tile needs to keep
+// draw order: a path has at most one tile per framebuffer tile.
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_entries(BatchDev b) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -879,54 +883,47 @@ __global__ __launch_bounds__(256) void k_entries(BatchDev b) {
     const int k = find_scene_lanes(b.col_bases, b.n_scenes, min(c, b.n_cols - 1));
     if (c < b.n_cols) {
         const ColumnWalk cw(b, k, c);
+        uint32_t e = WRITE ? b.col_entry[c] : 0u;  // WRITE: next entry index; count: entries so far
         for (int row0 = 0; row0 < cw.h; row0 += COL_ROWS) {
+            uint32_t st[COL_ROWS], z[COL_ROWS];
+#pragma unroll
+            for (int i = 0; i < COL_ROWS; i++) {
+                st[i] = TILE_EMPTY;
+                z[i] = 0;
+                if (row0 + i < cw.h) {
+                    st[i] = b.tile_st[cw.tile(row0 + i)];
+                    z[i] = b.zbuf[cw.fb_base + cw.fb_local(row0 + i)];
+                }
+            }
+            bool keep[COL_ROWS];
+#pragma unroll
+            for (int i = 0; i < COL_ROWS; i++) keep[i] = st[i] != TILE_EMPTY && (int32_t)cw.p >= (int32_t)z[i];
             if (!WRITE) {
-                uint32_t st[COL_ROWS], z[COL_ROWS];
 #pragma unroll
-                for (int i = 0; i < COL_ROWS; i++) {
-                    st[i] = TILE_EMPTY;
-                    z[i] = 0;
-                    if (row0 + i < cw.h) {
-                        st[i] = b.tile_st[cw.tile(row0 + i)];
-                        z[i] = b.zbuf[cw.fb_base + cw.fb_local(row0 + i)];
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < COL_ROWS; i++)
-                    if (row0 + i < cw.h)
-                        b.tile_entry[cw.tile(row0 + i)] = (st[i] != TILE_EMPTY && (int32_t)cw.p >= (int32_t)z[i]) ? 1u : 0u;
+                for (int i = 0; i < COL_ROWS; i++) e += keep[i] ? 1u : 0u;
             } else {
-                uint32_t e0[COL_ROWS], e1[COL_ROWS], st[COL_ROWS];
-#pragma unroll
-                for (int i = 0; i < COL_ROWS; i++) {
-                    e0[i] = e1[i] = 0;
-                    st[i] = TILE_EMPTY;
-                    if (row0 + i < cw.h) {
-                        const uint32_t t = cw.tile(row0 + i);
-                        e0[i] = b.tile_entry[t];
-                        e1[i] = b.tile_entry[t + 1];
-                        st[i] = b.tile_st[t];
-                    }
-                }
                 uint32_t ai[COL_ROWS];
 #pragma unroll
                 for (int i = 0; i < COL_ROWS; i++) {
                     ai[i] = NO_ALPHA;
-                    if (e1[i] != e0[i] && st[i] == TILE_ALPHA) ai[i] = b.tile_alpha_idx[cw.tile(row0 + i)];
+                    if (keep[i] && st[i] == TILE_ALPHA) ai[i] = b.tile_alpha_idx[cw.tile(row0 + i)];
                 }
 #pragma unroll
                 for (int i = 0; i < COL_ROWS; i++) {
-                    const uint32_t e = e0[i];
-                    if (e1[i] == e || e >= b.cap_entries) continue;  // no entry, or past the capacity
-                    const uint32_t fbl = cw.fb_local(row0 + i), fb = cw.fb_base + fbl;
-                    // even-odd solid tile with an even winding covers nothing: zero alpha
-                    const uint32_t eff = st[i] == TILE_CLEAR ? (cw.rgba & 0x00ffffffu) : cw.rgba;
-                    b.keys[e] = fb;  // its value, e, is generated by the sort's first pass
-                    b.entry_rec[e] = make_uint4(ai[i], eff, cw.p, fbl);
-                    n_alpha_entries += st[i] == TILE_ALPHA ? 1u : 0u;
+                    if (!keep[i]) continue;
+                    if (e < b.cap_entries) {
+                        const uint32_t fbl = cw.fb_local(row0 + i);
+                        // even-odd solid tile with an even winding covers nothing: zero alpha
+                        const uint32_t eff = st[i] == TILE_CLEAR ? (cw.rgba & 0x00ffffffu) : cw.rgba;
+                        b.keys[e] = cw.fb_base + fbl;  // its value, e, is generated by the sort's first pass
+                        b.entry_rec[e] = make_uint4(ai[i], eff, cw.p, fbl);
+                        n_alpha_entries += st[i] == TILE_ALPHA ? 1u : 0u;
+                    }
+                    e++;
                 }
             }
         }
+        if (!WRITE) b.col_entry[c] = e;
     }
     if (WRITE) {
         for (int d = 32; d > 0; d >>= 1) n_alpha_entries += __shfl_xor(n_alpha_entries, d, 64);

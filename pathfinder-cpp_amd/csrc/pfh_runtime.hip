@@ -158,7 +158,7 @@ struct Workspace {
     // around the fill and composite kernels of a batched launch; recorded as event nodes
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
-    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, tile_entry, col_bd,
+    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, col_entry, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, counters, scan_tmp, sort_tmp, ones_mask;
     uint32_t *h_counters = nullptr;  // pinned
@@ -183,7 +183,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &tile_entry, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
+                &col_entry, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
                 &sort_tmp, &ones_mask};
     }
@@ -530,7 +530,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.tile_alpha_idx.ensure(4 * pl.T + 8);
     if (dump) w.tile_bd.ensure(pl.T + 4);
     w.tile_st.ensure(pl.T + 4);
-    w.tile_entry.ensure(4 * pl.T + 8);
+    w.col_entry.ensure(4 * pl.W + 8);
     w.col_bd.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
     w.lines.ensure(16ull * pl.cap_l);
@@ -703,7 +703,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.tile_alpha_idx = w.tile_alpha_idx.as<uint32_t>();
     b.tile_bd = dump ? w.tile_bd.as<int8_t>() : nullptr;
     b.tile_st = w.tile_st.as<uint8_t>();
-    b.tile_entry = w.tile_entry.as<uint32_t>();
+    b.col_entry = w.col_entry.as<uint32_t>();
     b.col_bd = w.col_bd.as<int32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
     b.lines = w.lines.as<float4>();
@@ -754,7 +754,7 @@ static void stage_emit(Workspace &w, const BatchDev &b, const Plan &pl) {
     HIP_TRY(launch_bin(b, true, s));
     HIP_TRY(launch_propagate(b, pl.sched.column_block, s));
     HIP_TRY(launch_entries(b, false, pl.sched.column_block, s));
-    HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp, s));
+    HIP_TRY(scan_entries(b.col_entry, b.n_cols, b.counters, tmp, s));
 }
 
 static void stage_finish(Workspace &w, const BatchDev &b, const Plan &pl) {
@@ -851,7 +851,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     HIP_TRY(launch_propagate(b, pl.sched.column_block, st));
     HIP_TRY(hipEventRecord(ev[4], st));
     HIP_TRY(launch_entries(b, false, pl.sched.column_block, st));
-    HIP_TRY(scan_entries(b.tile_entry, b.n_tiles, b.counters, tmp(), st));
+    HIP_TRY(scan_entries(b.col_entry, b.n_cols, b.counters, tmp(), st));
     HIP_TRY(launch_entries(b, true, pl.sched.column_block, st));
     HIP_TRY(hipEventRecord(ev[5], st));
     bool alt = false;
@@ -1119,8 +1119,8 @@ static uint64_t scene_launch_bytes(const pfh_ctx *ctx, const SceneHost &s) {
         A = s.n_tiles;
     }
     const double T = s.n_tiles, W = s.n_cols, S = s.n_segs, FB = (double)s.fb_w * s.fb_h;
-    const double per_tile = 4 + 4 + 8 + 1 + 8 + 4 * 4 + 16;  // counts, offsets, status, entries, sort, records
-    double b = T * per_tile + 4 * W + 8 * S + 12 * FB + 24 * L + 8 * F + (16 + 256) * A;
+    const double per_tile = 4 + 4 + 8 + 1 + 4 + 4 * 4 + 16;  // counts, offsets, status, sort, records
+    double b = T * per_tile + 8 * W + 8 * S + 12 * FB + 24 * L + 8 * F + (16 + 256) * A;
     return (uint64_t)b + 4096;
 }
 
@@ -1393,8 +1393,8 @@ int pfh_dump_counts(pfh_ctx *ctx, int32_t scene_id, int64_t *counts) {
         HIP_TRY(hipMemcpy(&a0, ctx->w->tile_alpha_idx.as<uint32_t>() + tb, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&a1, ctx->w->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4, hipMemcpyDeviceToHost));
         // a scene's entries keep their index range through the sort (keys are grouped by scene)
-        HIP_TRY(hipMemcpy(&e0, ctx->w->tile_entry.as<uint32_t>() + tb, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&e1, ctx->w->tile_entry.as<uint32_t>() + tb + s.n_tiles, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&e0, ctx->w->col_entry.as<uint32_t>() + s.col_base, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&e1, ctx->w->col_entry.as<uint32_t>() + s.col_base + s.n_cols, 4, hipMemcpyDeviceToHost));
         counts[0] = s.n_tiles;
         counts[1] = s.n_cols;
         counts[2] = f1 - f0;
@@ -1453,8 +1453,8 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
         }
         // a scene's entries keep their index range through the sort (keys are grouped by scene)
         uint32_t ebase = 0, eend = 0;
-        d2h(&ebase, ctx->w->tile_entry.as<uint32_t>() + tb, 4);
-        d2h(&eend, ctx->w->tile_entry.as<uint32_t>() + tb + s.n_tiles, 4);
+        d2h(&ebase, ctx->w->col_entry.as<uint32_t>() + cb, 4);
+        d2h(&eend, ctx->w->col_entry.as<uint32_t>() + cb + s.n_cols, 4);
         uint32_t ne = eend - ebase;
         std::vector<uint32_t> sv(ne);
         const uint32_t *sorted = ctx->w->last_sorted_alt ? ctx->w->vals_alt.as<uint32_t>() : ctx->w->vals.as<uint32_t>();
