@@ -126,7 +126,7 @@ struct BatchDev {
     const uint32_t *fb_bases;
     const uint32_t *bin_wave_bases;   // [n_scenes + 1] wave-class items per scene, scanned
     const uint32_t *bin_group_bases;  // [n_scenes + 1] group-class items per scene, scanned
-    uint32_t n_segs, n_tiles, n_cols, n_fb;
+    uint32_t n_segs, n_tiles, n_cols, n_fb, n_paths;
     uint32_t n_bin_wave, n_bin_group;
     uint32_t cap_fills, cap_alpha, cap_entries, cap_lines;
     int32_t *tile_delta;          // [T] backdrop delta (int8 arithmetic, stored widened)
@@ -136,6 +136,7 @@ struct BatchDev {
     int8_t *tile_bd;              // [T] propagated backdrop (written with dumps only)
     uint8_t *tile_st;             // [T] tile status (TileStatus)
     uint32_t *col_entry;          // [W + 1] draw entries per column, then exclusive scan
+    uint2 *path_tot;              // [P + 1] {fills, alpha tiles} per batch path, then exclusive scan
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
     float4 *lines;                // [L] flattened chords (from, to) in emission order

@@ -122,7 +122,8 @@ struct Walker {
                 pos = atomicAdd(&l_cnt[j], 1u);
             } else {
                 uint32_t slot = atomicSub(&tile_cnt[tbase + j], 1u) - 1u;
-                pos = tile_fill_off[tbase + j] + slot;
+                // written by this block in this launch: read from L2
+                pos = __hip_atomic_load(&tile_fill_off[tbase + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + slot;
             }
             if (pos < cap_fills) fills[pos] = make_uint2(packed_x, packed_y);
             if (out) {
@@ -435,6 +436,52 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
 }
 
 // ---------------------------------------------------------------------------
+// Device-wide exclusive scan (reduce / scan block sums / apply), 4096 items per block
+// ---------------------------------------------------------------------------
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+struct U2 {
+    uint32_t x, y;
+    __device__ __forceinline__ U2 operator+(const U2 &o) const { return U2{x + o.x, y + o.y}; }
+};
+
+__device__ __forceinline__ uint32_t shfl_up_val(uint32_t v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ U2 shfl_up_val(U2 v, int d) { return U2{__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64)}; }
+
+template <typename T>
+__device__ __forceinline__ T zero_of() { return T{}; }
+
+// Exclusive scan of one value per thread across a 256-thread block; returns the block total.
+template <typename T>
+__device__ T block_exclusive_scan(T v, T &total) {
+    __shared__ T wave_tot[SCAN_THREADS / 64];
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = shfl_up_val(inc, d);
+        if (lane >= d) inc = inc + o;
+    }
+    if (lane == 63) wave_tot[wave] = inc;
+    __syncthreads();
+    T base = zero_of<T>();
+    T all = zero_of<T>();
+#pragma unroll
+    for (int w = 0; w < SCAN_THREADS / 64; w++) {
+        if (w < wave) base = base + wave_tot[w];
+        all = all + wave_tot[w];
+    }
+    __syncthreads();
+    total = all;
+    T excl_in_wave = shfl_up_val(inc, 1);
+    if (lane == 0) excl_in_wave = zero_of<T>();
+    return base + excl_in_wave;
+}
+
+// ---------------------------------------------------------------------------
 // Bin: per path, one wave (small rects) or one workgroup (large rects)
 // ---------------------------------------------------------------------------
 //
@@ -503,6 +550,80 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Per-path totals and offsets.  The count pass stores each path's {fills, alpha tiles}
+// (path_tot, batch path order); after their scan the emit pass turns its rect's tile counts
+// into the tiles' fill offsets and alpha-tile indices (a local scan, row-major), writes
+// them, and uses the fill offsets as its LDS cursors.  The batch-wide order of tiles is
+// path order then row-major within the rect, so this equals a scan over all tiles.
+
+__device__ __forceinline__ U2 wave_sum_u2(U2 v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        v.x += __shfl_xor(v.x, d, 64);
+        v.y += __shfl_xor(v.y, d, 64);
+    }
+    return v;
+}
+
+// Inclusive prefix sums of (count, count > 0) across the wave.
+__device__ __forceinline__ U2 wave_inclusive_u2(uint32_t c) {
+    U2 v{c, c > 0 ? 1u : 0u};
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const U2 o = shfl_up_val(v, d);
+        if (lane >= d) v = v + o;
+    }
+    return v;
+}
+
+// Wave: tile offsets of a rect of `area` tiles (counts in cnt_src, global) from the path's
+// scanned base; offsets go to the tile arrays and, when given, to the LDS cursors.
+__device__ __forceinline__ void wave_tile_offsets(const BatchDev &b, uint32_t tbase, uint32_t area, U2 base,
+                                                  uint32_t *l_cur) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t j0 = 0; j0 < area; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const uint32_t c = j < area ? b.tile_cnt[tbase + j] : 0u;
+        const U2 inc = wave_inclusive_u2(c);
+        if (j < area) {
+            const uint32_t off = base.x + inc.x - c, ai = base.y + inc.y - (c > 0 ? 1u : 0u);
+            b.tile_fill_off[tbase + j] = off;
+            b.tile_alpha_idx[tbase + j] = ai;
+            if (l_cur) l_cur[j] = off;
+        }
+        base = base + U2{__shfl(inc.x, 63, 64), __shfl(inc.y, 63, 64)};
+    }
+}
+
+// Block (256 threads) version, called by every thread of the block.
+__device__ __forceinline__ void block_tile_offsets(const BatchDev &b, uint32_t tbase, uint32_t area, U2 base,
+                                                   uint32_t *l_cur) {
+    for (uint32_t j0 = 0; j0 < area; j0 += 256) {
+        const uint32_t j = j0 + threadIdx.x;
+        const uint32_t c = j < area ? b.tile_cnt[tbase + j] : 0u;
+        U2 tot;
+        const U2 ex = block_exclusive_scan<U2>(U2{c, c > 0 ? 1u : 0u}, tot);
+        if (j < area) {
+            b.tile_fill_off[tbase + j] = base.x + ex.x;
+            b.tile_alpha_idx[tbase + j] = base.y + ex.y;
+            if (l_cur) l_cur[j] = base.x + ex.x;
+        }
+        base = base + tot;
+    }
+}
+
+// Block sum over the block's threads; `part` is 4 slots of LDS the block no longer reads.
+__device__ __forceinline__ U2 block_sum_u2(U2 v, U2 *part) {
+    v = wave_sum_u2(v);
+    __syncthreads();  // part may alias data read just before
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const U2 t = part[0] + part[1] + part[2] + part[3];
+    __syncthreads();
+    return t;
+}
+
 // Wave class: 4 paths per 256-thread block, one per wave, each with its own LDS slice.
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
@@ -522,8 +643,10 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
     const uint32_t tbase = sc.tile_base + sc.path_tile_off[p], cbase = sc.col_base + sc.path_col_off[p];
     uint32_t *l_cnt = s_cnt[wv];
     int32_t *l_delta = s_delta[wv], *l_col = s_col[wv];
+    const uint32_t gp = sc.path_base + p;
     if (EMIT) {
-        for (uint32_t j = lane; j < area; j += 64) l_cnt[j] = b.tile_fill_off[tbase + j];
+        const uint2 base = b.path_tot[gp];
+        wave_tile_offsets(b, tbase, area, U2{base.x, base.y}, l_cnt);
     } else {
         for (uint32_t j = lane; j < area; j += 64) {
             l_cnt[j] = 0;
@@ -537,11 +660,16 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
     walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, l_cnt, l_delta, l_col, c0, c1, lane, 64);
     if (!EMIT) {
         wave_sync();
+        U2 tot{0, 0};
         for (uint32_t j = lane; j < area; j += 64) {
-            b.tile_cnt[tbase + j] = l_cnt[j];
+            const uint32_t c = l_cnt[j];
+            b.tile_cnt[tbase + j] = c;
             b.tile_delta[tbase + j] = l_delta[j];
+            tot = tot + U2{c, c > 0 ? 1u : 0u};
         }
         for (uint32_t j = lane; j < tw; j += 64) b.col_bd[cbase + j] = l_col[j];
+        tot = wave_sum_u2(tot);
+        if (lane == 0) b.path_tot[gp] = make_uint2(tot.x, tot.y);
     }
 }
 
@@ -562,11 +690,13 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
     const uint32_t area = tw * th;
     const uint32_t tbase = sc.tile_base + sc.path_tile_off[p], cbase = sc.col_base + sc.path_col_off[p];
     const bool lds = area <= (uint32_t)BIN_GROUP_TILES && tw <= (uint32_t)BIN_GROUP_COLS;
+    const uint32_t gp = sc.path_base + p;
     uint32_t c0, c1;
     path_chords(b, sc, p, c0, c1);
     if (lds) {
         if (EMIT) {
-            for (uint32_t j = threadIdx.x; j < area; j += 256) s_cnt[j] = b.tile_fill_off[tbase + j];
+            const uint2 base = b.path_tot[gp];
+            block_tile_offsets(b, tbase, area, U2{base.x, base.y}, s_cnt);
         } else {
             for (uint32_t j = threadIdx.x; j < area; j += 256) {
                 s_cnt[j] = 0;
@@ -578,13 +708,24 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
         walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, s_cnt, s_delta, s_col, c0, c1, threadIdx.x, 256);
         if (!EMIT) {
             __syncthreads();
+            U2 tot{0, 0};
             for (uint32_t j = threadIdx.x; j < area; j += 256) {
-                b.tile_cnt[tbase + j] = s_cnt[j];
+                const uint32_t c = s_cnt[j];
+                b.tile_cnt[tbase + j] = c;
                 b.tile_delta[tbase + j] = s_delta[j];
+                tot = tot + U2{c, c > 0 ? 1u : 0u};
             }
             for (uint32_t j = threadIdx.x; j < tw; j += 256) b.col_bd[cbase + j] = s_col[j];
+            tot = block_sum_u2(tot, reinterpret_cast<U2 *>(s_col));
+            if (threadIdx.x == 0) b.path_tot[gp] = make_uint2(tot.x, tot.y);
         }
     } else {
+        if (EMIT) {
+            const uint2 base = b.path_tot[gp];
+            block_tile_offsets(b, tbase, area, U2{base.x, base.y}, nullptr);
+            __threadfence();  // the offsets reach L2 before any thread reads them back
+            __syncthreads();
+        }
         if (!EMIT) {
             for (uint32_t j = threadIdx.x; j < area; j += 256) {
                 b.tile_cnt[tbase + j] = 0;
@@ -595,58 +736,23 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
             __syncthreads();
         }
         walk_chords<EMIT, false>(b, sc, rect, tbase, cbase, nullptr, nullptr, nullptr, c0, c1, threadIdx.x, 256);
+        if (!EMIT) {
+            __threadfence();  // every thread's atomics are done before the counts are summed
+            __syncthreads();
+            U2 tot{0, 0};
+            for (uint32_t j = threadIdx.x; j < area; j += 256) {
+                const uint32_t c = __hip_atomic_load(&b.tile_cnt[tbase + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                tot = tot + U2{c, c > 0 ? 1u : 0u};
+            }
+            tot = block_sum_u2(tot, reinterpret_cast<U2 *>(s_col));  // s_col is unused on this path
+            if (threadIdx.x == 0) b.path_tot[gp] = make_uint2(tot.x, tot.y);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 
 
-
-// ---------------------------------------------------------------------------
-// Device-wide exclusive scan (reduce / scan block sums / apply), 4096 items per block
-// ---------------------------------------------------------------------------
-
-constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 16;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-
-struct U2 {
-    uint32_t x, y;
-    __device__ __forceinline__ U2 operator+(const U2 &o) const { return U2{x + o.x, y + o.y}; }
-};
-
-__device__ __forceinline__ uint32_t shfl_up_val(uint32_t v, int d) { return __shfl_up(v, d, 64); }
-__device__ __forceinline__ U2 shfl_up_val(U2 v, int d) { return U2{__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64)}; }
-
-template <typename T>
-__device__ __forceinline__ T zero_of() { return T{}; }
-
-// Exclusive scan of one value per thread across a 256-thread block; returns the block total.
-template <typename T>
-__device__ T block_exclusive_scan(T v, T &total) {
-    __shared__ T wave_tot[SCAN_THREADS / 64];
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    T inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        T o = shfl_up_val(inc, d);
-        if (lane >= d) inc = inc + o;
-    }
-    if (lane == 63) wave_tot[wave] = inc;
-    __syncthreads();
-    T base = zero_of<T>();
-    T all = zero_of<T>();
-#pragma unroll
-    for (int w = 0; w < SCAN_THREADS / 64; w++) {
-        if (w < wave) base = base + wave_tot[w];
-        all = all + wave_tot[w];
-    }
-    __syncthreads();
-    total = all;
-    T excl_in_wave = shfl_up_val(inc, 1);
-    if (lane == 0) excl_in_wave = zero_of<T>();
-    return base + excl_in_wave;
-}
 
 // Order does not matter for the block sum: each pass reads one coalesced row of 256 items.
 template <typename T, typename Load>
@@ -744,20 +850,22 @@ struct StoreU32 {
     __device__ void operator()(uint32_t i, uint32_t v) const { p[i] = v; }
 };
 // Tiles: (fill count, has fills) -> fill offsets, alpha indices; totals -> counters.
-struct LoadTileCnt {
-    const uint32_t *cnt;
+struct LoadPathTot {
+    const uint2 *tot;
     __device__ U2 operator()(uint32_t i) const {
-        uint32_t c = cnt[i];
-        return U2{c, c > 0 ? 1u : 0u};
+        const uint2 v = tot[i];
+        return U2{v.x, v.y};
     }
 };
-struct StoreTileOff {
-    uint32_t *fill_off, *alpha_idx, *counters;
+struct StorePathOff {
+    uint2 *off;
+    uint32_t *fill_end, *alpha_end, *counters;
     uint32_t n;
     __device__ void operator()(uint32_t i, U2 v) const {
-        fill_off[i] = v.x;
-        alpha_idx[i] = v.y;
+        off[i] = make_uint2(v.x, v.y);
         if (i == n) {
+            *fill_end = v.x;
+            *alpha_end = v.y;
             counters[C_FILLS] = v.x;
             counters[C_ALPHA] = v.y;
         }
@@ -1070,9 +1178,13 @@ hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, u
     return scan_impl<uint32_t>(LoadU32{flags_inout}, StoreEntryOff{flags_inout, counters, n}, n, tmp, s);
 }
 
-hipError_t scan_tiles(const BatchDev &b, void *tmp, hipStream_t s) {
-    return scan_impl<U2>(LoadTileCnt{b.tile_cnt}, StoreTileOff{b.tile_fill_off, b.tile_alpha_idx, b.counters, b.n_tiles},
-                         b.n_tiles, reinterpret_cast<U2 *>(tmp), s);
+// Exclusive scan of the per-path {fills, alpha tiles} in place (path_tot[P] = totals), with
+// the totals also in the counters and at the end of the tile offset arrays.
+hipError_t scan_paths(const BatchDev &b, void *tmp, hipStream_t s) {
+    return scan_impl<U2>(LoadPathTot{b.path_tot},
+                         StorePathOff{b.path_tot, b.tile_fill_off + b.n_tiles, b.tile_alpha_idx + b.n_tiles, b.counters,
+                                      b.n_paths},
+                         b.n_paths, reinterpret_cast<U2 *>(tmp), s);
 }
 
 // segs_per_wave: segments each wave works through (a multiple of 64, at least 64)

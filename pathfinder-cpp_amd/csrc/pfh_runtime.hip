@@ -24,7 +24,7 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t 
 size_t scan_temp_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp, hipStream_t s);
 hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, uint32_t *tmp, hipStream_t s);
-hipError_t scan_tiles(const BatchDev &b, void *tmp, hipStream_t s);
+hipError_t scan_paths(const BatchDev &b, void *tmp, hipStream_t s);
 hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, hipStream_t s);
 hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s);
 hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
@@ -158,7 +158,7 @@ struct Workspace {
     // around the fill and composite kernels of a batched launch; recorded as event nodes
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
-    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, col_entry, col_bd,
+    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, col_entry, path_tot, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, counters, scan_tmp, sort_tmp, ones_mask;
     uint32_t *h_counters = nullptr;  // pinned
@@ -183,7 +183,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &col_entry, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
+                &col_entry, &path_tot, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
                 &sort_tmp, &ones_mask};
     }
@@ -507,7 +507,7 @@ static void predict_scene(pfh_ctx *ctx, SceneHost &s) {
 
 struct Plan {
     std::vector<int32_t> ids;
-    uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0;
+    uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0, P = 0;
     uint32_t cap_f = 0, cap_a = 0, cap_e = 0, cap_l = 0;
     int key_bits = 1;
     Schedule sched;
@@ -531,6 +531,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     if (dump) w.tile_bd.ensure(pl.T + 4);
     w.tile_st.ensure(pl.T + 4);
     w.col_entry.ensure(4 * pl.W + 8);
+    w.path_tot.ensure(8 * pl.P + 8);
     w.col_bd.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
     w.lines.ensure(16ull * pl.cap_l);
@@ -542,7 +543,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
         HIP_TRY(launch_fill_u32(w.ones_mask.as<uint32_t>(), 0xffffffffu, 64, w.stream));
     }
     w.counters.ensure(4 * C_COUNT);
-    uint64_t scan_n = std::max<uint64_t>({pl.T + 1, pl.S + 1, pl.FB + 1, (uint64_t)pl.cap_l + 1, 1});
+    uint64_t scan_n = std::max<uint64_t>({pl.T + 1, pl.S + 1, pl.FB + 1, pl.P + 1, pl.W + 1, (uint64_t)pl.cap_l + 1, 1});
     w.scan_tmp.ensure(16 * scan_temp_elems(scan_n) + 64);
     w.fills.ensure(8ull * pl.cap_f);
     if (dump) w.dump_fills.ensure(16ull * pl.cap_f);
@@ -588,7 +589,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         HIP_TRY(hipHostMalloc((void **)&ctx->w->h_bases, sizeof(uint32_t) * N_BASES * (n + 1)));
         ctx->w->h_bases_cap = N_BASES * (n + 1);
     }
-    uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0;
+    uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0, P = 0;
     for (size_t k = 0; k < n; k++) {
         SceneHost &s = get_scene(ctx, pl.ids[k]);
         SceneDev &d = ctx->w->h_table[k];
@@ -623,7 +624,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.n_cols = s.n_cols;
         d.n_fb = (uint32_t)(s.fb_w * s.fb_h);
         d.seg_base = (uint32_t)S;
-        d.path_base = 0;
+        d.path_base = (uint32_t)P;
         d.tile_base = (uint32_t)T;
         d.col_base = (uint32_t)W;
         d.fb_base = (uint32_t)FB;
@@ -647,6 +648,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         FB += d.n_fb;
         BW += s.n_bin_wave;
         BG += s.n_bin_group;
+        P += s.n_paths;
         if (S >= 0xF0000000ull || T >= 0xF0000000ull || W >= 0xF0000000ull || FB >= 0xF0000000ull)
             throw PfhError{PFH_E_LIMIT, "batch exceeds 2^32 segments/tiles"};
     }
@@ -662,6 +664,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
     pl.T = T;
     pl.W = W;
     pl.FB = FB;
+    pl.P = P;
     pl.key_bits = 1;
     while ((1ull << pl.key_bits) < FB) pl.key_bits++;
 }
@@ -704,6 +707,8 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.tile_bd = dump ? w.tile_bd.as<int8_t>() : nullptr;
     b.tile_st = w.tile_st.as<uint8_t>();
     b.col_entry = w.col_entry.as<uint32_t>();
+    b.path_tot = w.path_tot.as<uint2>();
+    b.n_paths = (uint32_t)pl.P;
     b.col_bd = w.col_bd.as<int32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
     b.lines = w.lines.as<float4>();
@@ -732,6 +737,8 @@ static void stage_clear(Workspace &w, const BatchDev &b) {
     // tile_delta / tile_cnt / col_bd need no clearing: the bin count pass stores every
     // tile and column of every binned path (paths with no columns own none)
     HIP_TRY(launch_fill_u32(b.zbuf, 0, b.n_fb, s));
+    // paths with no columns are not binned: their totals stay zero
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.path_tot), 0, 2 * (uint64_t)b.n_paths, s));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.fb_range), 0, 2 * (uint64_t)b.n_fb, s));
     HIP_TRY(launch_fill_u32(b.counters, 0, C_COUNT, s));
 }
@@ -744,7 +751,7 @@ static void stage_count(Workspace &w, const BatchDev &b, const Plan &pl) {
     HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp, s));
     HIP_TRY(launch_flatten(b, true, flatten_segs(pl), s));
     HIP_TRY(launch_bin(b, false, s));
-    HIP_TRY(scan_tiles(b, w.scan_tmp.ptr, s));
+    HIP_TRY(scan_paths(b, w.scan_tmp.ptr, s));
 }
 
 static void stage_emit(Workspace &w, const BatchDev &b, const Plan &pl) {
@@ -835,7 +842,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     HIP_TRY(launch_flatten(b, true, flatten_segs(pl), st));
     HIP_TRY(hipEventRecord(ev[1], st));
     HIP_TRY(launch_bin(b, false, st));
-    HIP_TRY(scan_tiles(b, ctx->w->scan_tmp.ptr, st));
+    HIP_TRY(scan_paths(b, ctx->w->scan_tmp.ptr, st));
     read_counters(ctx);  // fill / alpha counts (renderer.cpp:834-845)
     HIP_TRY(hipEventRecord(ev[2], st));
     uint32_t nf = ctx->w->h_counters[C_FILLS], na = ctx->w->h_counters[C_ALPHA];
