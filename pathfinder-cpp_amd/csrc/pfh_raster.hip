@@ -314,14 +314,20 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         bool head = solid && (lane == 0 || !((solid_mask >> (lane - 1)) & 1ull));
         bool flag = head || !solid;
         // DPP steps: shifts by 1, 2, 4, 8 within each 16-lane row, then the row totals
-        // (lane 15 into row 1 and lane 47 into row 3, then lane 31 into rows 2 and 3)
+        // (lane 15 into row 1 and lane 47 into row 3, then lane 31 into rows 2 and 3).  A step
+        // changes nothing unless a run is longer than its shift (within a row) or crosses the
+        // row boundary it bridges, so only those steps run (runs are mostly 1-3 entries).
         const int row = lane >> 4, rl = lane & 15;
-        over_step<0x111, 0xf>(rl >= 1, pr, pg, pb, pa, flag);
-        over_step<0x112, 0xf>(rl >= 2, pr, pg, pb, pa, flag);
-        over_step<0x114, 0xf>(rl >= 4, pr, pg, pb, pa, flag);
-        over_step<0x118, 0xf>(rl >= 8, pr, pg, pb, pa, flag);
-        over_step<0x142, 0xa>(row == 1 || row == 3, pr, pg, pb, pa, flag);
-        over_step<0x143, 0xc>(row >= 2, pr, pg, pb, pa, flag);
+        const uint64_t run2 = solid_mask & (solid_mask >> 1);  // bit i: lanes i, i + 1 both solid
+        const uint64_t run3 = run2 & (solid_mask >> 2);
+        const uint64_t run5 = run3 & (run2 >> 2) & (solid_mask >> 4);
+        const uint64_t run9 = run5 & (run5 >> 4);
+        if (run2) over_step<0x111, 0xf>(rl >= 1, pr, pg, pb, pa, flag);
+        if (run3) over_step<0x112, 0xf>(rl >= 2, pr, pg, pb, pa, flag);
+        if (run5) over_step<0x114, 0xf>(rl >= 4, pr, pg, pb, pa, flag);
+        if (run9) over_step<0x118, 0xf>(rl >= 8, pr, pg, pb, pa, flag);
+        if (run2 & ((1ull << 15) | (1ull << 47))) over_step<0x142, 0xa>(row == 1 || row == 3, pr, pg, pb, pa, flag);
+        if (run2 & (1ull << 31)) over_step<0x143, 0xc>(row >= 2, pr, pg, pb, pa, flag);
         // events, taken from the top: masked entries, and the last (top) entry of each solid
         // run, which holds the run's folded colour.  Every event blends the same way: a solid
         // run samples an all-ones mask row, and its colour is scaled by 1/255 like a masked
