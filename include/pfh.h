@@ -165,7 +165,7 @@ int pfh_mlp_train(pfh_ctx *ctx, const float *x, const float *target, int64_t n, 
 typedef struct pfh_schedule {
     int32_t fill_grid;        /* max workgroups of the coverage-mask kernel (it loops beyond) */
     int32_t composite_block;  /* threads per composite workgroup: 64, 128 or 256 (a wave per 16x16 tile) */
-    int32_t column_block;     /* threads per workgroup of the per-column propagate / entries kernels */
+    int32_t column_block;     /* threads per workgroup of the per-column propagate kernel */
     int32_t xcd_order;        /* bit 0 fill, bit 1 composite: workgroups regrouped so that each of the
                                  8 XCDs (own L2) works on one contiguous range of tiles */
     int32_t flatten_segs;     /* segments each flatten wave works through (multiple of 64; 0: from the

@@ -135,7 +135,7 @@ struct BatchDev {
     uint32_t *tile_alpha_idx;     // [T + 1] exclusive scan of alpha flags
     int8_t *tile_bd;              // [T] propagated backdrop (written with dumps only)
     uint8_t *tile_st;             // [T] tile status (TileStatus)
-    uint32_t *col_entry;          // [W + 1] draw entries per column, then exclusive scan
+    uint2 *path_ent;              // [P + 1] {draw entries, alpha entries} per batch path, then exclusive scan
     uint2 *path_tot;              // [P + 1] {fills, alpha tiles} per batch path, then exclusive scan
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan

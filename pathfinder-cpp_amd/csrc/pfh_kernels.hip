@@ -886,11 +886,15 @@ struct StoreCounted {
     }
 };
 struct StoreEntryOff {
-    uint32_t *off, *counters;
+    uint2 *off;
+    uint32_t *counters;
     uint32_t n;
-    __device__ void operator()(uint32_t i, uint32_t v) const {
-        off[i] = v;
-        if (i == n) counters[C_ENTRIES] = v;
+    __device__ void operator()(uint32_t i, U2 v) const {
+        off[i] = make_uint2(v.x, v.y);
+        if (i == n) {
+            counters[C_ENTRIES] = v.x;
+            counters[C_ALPHA_ENTRIES] = v.y;
+        }
     }
 };
 
@@ -979,64 +983,105 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
 }
 
 // Draw entries: tiles with a mask or a nonzero backdrop that survive the z-cull
-// (core/d3d9/scene_builder.cpp:57-79, shaders/d3d9/tile.vert:103-118).  The count pass
-// stores one count per path column; after their scan the write pass numbers a column's
-// entries from its offset, top to bottom.  Entries of a path precede those of the next
-// (columns are in path order), which is all the sort by framebuffer tile needs to keep
-// draw order: a path has at most one tile per framebuffer tile.
-template <bool WRITE>
-__global__ __launch_bounds__(256) void k_entries(BatchDev b) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n_alpha_entries = 0;
-    const int k = find_scene_lanes(b.col_bases, b.n_scenes, min(c, b.n_cols - 1));
-    if (c < b.n_cols) {
-        const ColumnWalk cw(b, k, c);
-        uint32_t e = WRITE ? b.col_entry[c] : 0u;  // WRITE: next entry index; count: entries so far
-        for (int row0 = 0; row0 < cw.h; row0 += COL_ROWS) {
-            uint32_t st[COL_ROWS], z[COL_ROWS];
-#pragma unroll
-            for (int i = 0; i < COL_ROWS; i++) {
-                st[i] = TILE_EMPTY;
-                z[i] = 0;
-                if (row0 + i < cw.h) {
-                    st[i] = b.tile_st[cw.tile(row0 + i)];
-                    z[i] = b.zbuf[cw.fb_base + cw.fb_local(row0 + i)];
-                }
-            }
-            bool keep[COL_ROWS];
-#pragma unroll
-            for (int i = 0; i < COL_ROWS; i++) keep[i] = st[i] != TILE_EMPTY && (int32_t)cw.p >= (int32_t)z[i];
-            if (!WRITE) {
-#pragma unroll
-                for (int i = 0; i < COL_ROWS; i++) e += keep[i] ? 1u : 0u;
-            } else {
-                uint32_t ai[COL_ROWS];
-#pragma unroll
-                for (int i = 0; i < COL_ROWS; i++) {
-                    ai[i] = NO_ALPHA;
-                    if (keep[i] && st[i] == TILE_ALPHA) ai[i] = b.tile_alpha_idx[cw.tile(row0 + i)];
-                }
-#pragma unroll
-                for (int i = 0; i < COL_ROWS; i++) {
-                    if (!keep[i]) continue;
-                    if (e < b.cap_entries) {
-                        const uint32_t fbl = cw.fb_local(row0 + i);
-                        // even-odd solid tile with an even winding covers nothing: zero alpha
-                        const uint32_t eff = st[i] == TILE_CLEAR ? (cw.rgba & 0x00ffffffu) : cw.rgba;
-                        b.keys[e] = cw.fb_base + fbl;  // its value, e, is generated by the sort's first pass
-                        b.entry_rec[e] = make_uint4(ai[i], eff, cw.p, fbl);
-                        n_alpha_entries += st[i] == TILE_ALPHA ? 1u : 0u;
-                    }
-                    e++;
-                }
-            }
+// (core/d3d9/scene_builder.cpp:57-79, shaders/d3d9/tile.vert:103-118), per path like the
+// bin kernels (a wave per small rect, a block per large one) with the threads striding over
+// the rect's tiles in row-major order.  The count pass stores each path's entry count;
+// after their scan the write pass numbers the path's entries in row-major order by ballot
+// rank, so consecutive threads write consecutive entries.  Entries of a path precede those
+// of the next, which is all the sort by framebuffer tile needs to keep draw order: a path
+// has at most one tile per framebuffer tile.
+
+struct EntryTile {
+    bool keep;
+    uint32_t st, fbl;
+};
+
+// Tile j (row-major) of a path's rect: status and survival of the z-cull.
+__device__ __forceinline__ EntryTile entry_tile(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t tbase,
+                                                int w, float inv_w, int fb_row0, uint32_t j) {
+    // row = j / w from a float estimate (off by at most one below 2^24 tiles), then corrected
+    uint32_t row = (uint32_t)((float)j * inv_w);
+    if (row * (uint32_t)w > j) row--;
+    else if ((row + 1) * (uint32_t)w <= j) row++;
+    const uint32_t x = j - row * (uint32_t)w;
+    EntryTile et;
+    et.st = b.tile_st[tbase + j];
+    et.fbl = (uint32_t)(fb_row0 + (int)row * sc.fb_w + (int)x);
+    const uint32_t z = b.zbuf[sc.fb_base + et.fbl];
+    et.keep = et.st != TILE_EMPTY && (int32_t)p >= (int32_t)z;
+    return et;
+}
+
+__device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t rgba,
+                                            uint32_t tbase, uint32_t j, const EntryTile &et, uint32_t e) {
+    if (e >= b.cap_entries) return;  // past the capacity: the batch is retried
+    const uint32_t ai = et.st == TILE_ALPHA ? b.tile_alpha_idx[tbase + j] : NO_ALPHA;
+    // even-odd solid tile with an even winding covers nothing: zero alpha
+    const uint32_t eff = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
+    b.keys[e] = sc.fb_base + et.fbl;  // its value, e, is generated by the sort's first pass
+    b.entry_rec[e] = make_uint4(ai, eff, p, et.fbl);
+}
+
+// THREADS 64: one wave per path (its own item); 256: one block per path.
+template <bool WRITE, int THREADS>
+__device__ __forceinline__ void path_entries(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t gp) {
+    const uint32_t t = THREADS == 64 ? (threadIdx.x & 63) : threadIdx.x, lane = threadIdx.x & 63;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int4 r = sc.path_rect[p];
+    const int w = max(r.z - r.x, 0), h = max(r.w - r.y, 0);
+    const uint32_t area = (uint32_t)w * (uint32_t)h;
+    const uint32_t tbase = sc.tile_base + sc.path_tile_off[p];
+    const int fb_row0 = (r.y - sc.fb_y0) * sc.fb_w + (r.x - sc.fb_x0);
+    const uint32_t rgba = sc.path_rgba[p];
+    const float inv_w = w > 0 ? 1.0f / (float)w : 0.0f;
+    uint32_t e = WRITE ? b.path_ent[gp].x : 0u, n_alpha = 0;
+    for (uint32_t j0 = 0; j0 < area; j0 += THREADS) {  // uniform over the wave / block
+        const uint32_t j = j0 + t;
+        EntryTile et{false, TILE_EMPTY, 0};
+        if (j < area) et = entry_tile(b, sc, p, tbase, w, inv_w, fb_row0, j);
+        if (THREADS == 64) {
+            const uint64_t m = __ballot(et.keep);
+            if (WRITE && et.keep) write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask));
+            if (!WRITE) n_alpha += (uint32_t)__popcll(__ballot(et.keep && et.st == TILE_ALPHA));
+            e += (uint32_t)__popcll(m);
+        } else {
+            uint32_t total;
+            const uint32_t rank = block_exclusive_scan<uint32_t>(et.keep ? 1u : 0u, total);
+            if (WRITE && et.keep) write_entry(b, sc, p, rgba, tbase, j, et, e + rank);
+            if (!WRITE) n_alpha += et.keep && et.st == TILE_ALPHA ? 1u : 0u;
+            e += total;
         }
-        if (!WRITE) b.col_entry[c] = e;
     }
-    if (WRITE) {
-        for (int d = 32; d > 0; d >>= 1) n_alpha_entries += __shfl_xor(n_alpha_entries, d, 64);
-        if ((threadIdx.x & 63) == 0 && n_alpha_entries) atomicAdd(&b.counters[C_ALPHA_ENTRIES], n_alpha_entries);
+    if (!WRITE) {
+        if (THREADS == 256) {  // the per-thread alpha counts: a block sum (the last scan's LDS is free)
+            for (int d = 32; d > 0; d >>= 1) n_alpha += __shfl_xor(n_alpha, d, 64);
+            __shared__ uint32_t part[4];
+            if (lane == 0) part[threadIdx.x >> 6] = n_alpha;
+            __syncthreads();
+            n_alpha = part[0] + part[1] + part[2] + part[3];
+        }
+        if (t == 0) b.path_ent[gp] = make_uint2(e, n_alpha);
     }
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_entries_wave(BatchDev b) {
+    const uint32_t item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= b.n_bin_wave) return;  // whole wave
+    const int k = find_scene_wave(b.bin_wave_bases, b.n_scenes, item);
+    const SceneDev &sc = b.scenes[k];
+    const uint32_t p = sc.bin_paths[item - b.bin_wave_bases[k]];
+    path_entries<WRITE, 64>(b, sc, p, sc.path_base + p);
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_entries_group(BatchDev b) {
+    const uint32_t item = blockIdx.x;
+    if (item >= b.n_bin_group) return;  // uniform over the block
+    const int k = find_scene_wave(b.bin_group_bases, b.n_scenes, item);
+    const SceneDev &sc = b.scenes[k];
+    const uint32_t p = sc.bin_paths[sc.n_bin_wave + (item - b.bin_group_bases[k])];
+    path_entries<WRITE, 256>(b, sc, p, sc.path_base + p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1174,8 +1219,9 @@ hipError_t scan_total(uint32_t *inout, uint32_t n, uint32_t *counters, int count
     return scan_impl<uint32_t>(LoadU32{inout}, StoreCounted{inout, counters, counter, n}, n, tmp, s);
 }
 
-hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, uint32_t *tmp, hipStream_t s) {
-    return scan_impl<uint32_t>(LoadU32{flags_inout}, StoreEntryOff{flags_inout, counters, n}, n, tmp, s);
+// Exclusive scan of the per-path {entries, alpha entries} in place, totals to the counters.
+hipError_t scan_entries(uint2 *inout, uint32_t n, uint32_t *counters, void *tmp, hipStream_t s) {
+    return scan_impl<U2>(LoadPathTot{inout}, StoreEntryOff{inout, counters, n}, n, reinterpret_cast<U2 *>(tmp), s);
 }
 
 // Exclusive scan of the per-path {fills, alpha tiles} in place (path_tot[P] = totals), with
@@ -1219,11 +1265,17 @@ hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_entries(const BatchDev &b, bool write, uint32_t block, hipStream_t s) {
-    if (b.n_cols == 0) return hipSuccess;
-    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
-    if (write) hipLaunchKernelGGL(k_entries<true>, dim3(blocks_for(b.n_cols, block)), dim3(block), 0, s, b);
-    else hipLaunchKernelGGL(k_entries<false>, dim3(blocks_for(b.n_cols, block)), dim3(block), 0, s, b);
+hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s) {
+    if (b.n_bin_wave) {
+        const dim3 g(blocks_for(b.n_bin_wave, 4));
+        if (write) hipLaunchKernelGGL(k_entries_wave<true>, g, dim3(256), 0, s, b);
+        else hipLaunchKernelGGL(k_entries_wave<false>, g, dim3(256), 0, s, b);
+    }
+    if (b.n_bin_group) {
+        const dim3 g(b.n_bin_group);
+        if (write) hipLaunchKernelGGL(k_entries_group<true>, g, dim3(256), 0, s, b);
+        else hipLaunchKernelGGL(k_entries_group<false>, g, dim3(256), 0, s, b);
+    }
     return hipGetLastError();
 }
 

@@ -23,7 +23,7 @@ namespace pfh {
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t st);
 size_t scan_temp_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp, hipStream_t s);
-hipError_t scan_entries(uint32_t *flags_inout, uint32_t n, uint32_t *counters, uint32_t *tmp, hipStream_t s);
+hipError_t scan_entries(uint2 *inout, uint32_t n, uint32_t *counters, void *tmp, hipStream_t s);
 hipError_t scan_paths(const BatchDev &b, void *tmp, hipStream_t s);
 hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, hipStream_t s);
 hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s);
@@ -31,7 +31,7 @@ hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, ui
                         uint32_t *tmp, hipStream_t s);
 hipError_t scan_total(uint32_t *inout, uint32_t n, uint32_t *counters, int counter, uint32_t *tmp, hipStream_t s);
 hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s);
-hipError_t launch_entries(const BatchDev &b, bool write, uint32_t block, hipStream_t s);
+hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s);
 hipError_t launch_fb_bounds(const uint32_t *sorted_keys, const uint32_t *n_ptr, uint32_t cap, uint32_t n_fb,
                             uint2 *range, hipStream_t s);
 size_t radix_temp_elems(uint64_t cap);
@@ -123,7 +123,7 @@ struct SceneHost {
     int64_t out_pitch = 0;
     // placement in the last rasterized batch (for dumps)
     bool dumped = false;
-    uint32_t seg_base = 0, tile_base = 0, col_base = 0, fb_base = 0;
+    uint32_t seg_base = 0, tile_base = 0, col_base = 0, fb_base = 0, path_base = 0;
     // exact counts of the last run (capacity planning)
     bool counted = false;
     uint32_t last_fills = 0, last_alpha = 0, last_entries = 0, last_lines = 0;
@@ -158,7 +158,7 @@ struct Workspace {
     // around the fill and composite kernels of a batched launch; recorded as event nodes
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
-    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, col_entry, path_tot, col_bd,
+    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, path_tot, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, counters, scan_tmp, sort_tmp, ones_mask;
     uint32_t *h_counters = nullptr;  // pinned
@@ -183,7 +183,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &col_entry, &path_tot, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
+                &path_ent, &path_tot, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
                 &sort_tmp, &ones_mask};
     }
@@ -530,7 +530,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.tile_alpha_idx.ensure(4 * pl.T + 8);
     if (dump) w.tile_bd.ensure(pl.T + 4);
     w.tile_st.ensure(pl.T + 4);
-    w.col_entry.ensure(4 * pl.W + 8);
+    w.path_ent.ensure(8 * pl.P + 8);
     w.path_tot.ensure(8 * pl.P + 8);
     w.col_bd.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
@@ -633,6 +633,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.load_dst = load_dst ? 1u : 0u;
         d.pad = 0;
         s.seg_base = d.seg_base;
+        s.path_base = d.path_base;
         s.tile_base = d.tile_base;
         s.col_base = d.col_base;
         s.fb_base = d.fb_base;
@@ -706,7 +707,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.tile_alpha_idx = w.tile_alpha_idx.as<uint32_t>();
     b.tile_bd = dump ? w.tile_bd.as<int8_t>() : nullptr;
     b.tile_st = w.tile_st.as<uint8_t>();
-    b.col_entry = w.col_entry.as<uint32_t>();
+    b.path_ent = w.path_ent.as<uint2>();
     b.path_tot = w.path_tot.as<uint2>();
     b.n_paths = (uint32_t)pl.P;
     b.col_bd = w.col_bd.as<int32_t>();
@@ -737,8 +738,9 @@ static void stage_clear(Workspace &w, const BatchDev &b) {
     // tile_delta / tile_cnt / col_bd need no clearing: the bin count pass stores every
     // tile and column of every binned path (paths with no columns own none)
     HIP_TRY(launch_fill_u32(b.zbuf, 0, b.n_fb, s));
-    // paths with no columns are not binned: their totals stay zero
+    // paths with no columns are not binned: their totals and entry counts stay zero
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.path_tot), 0, 2 * (uint64_t)b.n_paths, s));
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.path_ent), 0, 2 * (uint64_t)b.n_paths, s));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.fb_range), 0, 2 * (uint64_t)b.n_fb, s));
     HIP_TRY(launch_fill_u32(b.counters, 0, C_COUNT, s));
 }
@@ -760,12 +762,12 @@ static void stage_emit(Workspace &w, const BatchDev &b, const Plan &pl) {
     if (b.dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, b.cap_lines, b.counters, -1, tmp, s));
     HIP_TRY(launch_bin(b, true, s));
     HIP_TRY(launch_propagate(b, pl.sched.column_block, s));
-    HIP_TRY(launch_entries(b, false, pl.sched.column_block, s));
-    HIP_TRY(scan_entries(b.col_entry, b.n_cols, b.counters, tmp, s));
+    HIP_TRY(launch_entries(b, false, s));
+    HIP_TRY(scan_entries(b.path_ent, b.n_paths, b.counters, w.scan_tmp.ptr, s));
 }
 
 static void stage_finish(Workspace &w, const BatchDev &b, const Plan &pl) {
-    HIP_TRY(launch_entries(b, true, pl.sched.column_block, w.stream));
+    HIP_TRY(launch_entries(b, true, w.stream));
     bool alt = false;
     HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
                        w.sort_tmp.as<uint32_t>(), &alt, w.stream));
@@ -857,9 +859,9 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(launch_propagate(b, pl.sched.column_block, st));
     HIP_TRY(hipEventRecord(ev[4], st));
-    HIP_TRY(launch_entries(b, false, pl.sched.column_block, st));
-    HIP_TRY(scan_entries(b.col_entry, b.n_cols, b.counters, tmp(), st));
-    HIP_TRY(launch_entries(b, true, pl.sched.column_block, st));
+    HIP_TRY(launch_entries(b, false, st));
+    HIP_TRY(scan_entries(b.path_ent, b.n_paths, b.counters, ctx->w->scan_tmp.ptr, st));
+    HIP_TRY(launch_entries(b, true, st));
     HIP_TRY(hipEventRecord(ev[5], st));
     bool alt = false;
     HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
@@ -1126,8 +1128,8 @@ static uint64_t scene_launch_bytes(const pfh_ctx *ctx, const SceneHost &s) {
         A = s.n_tiles;
     }
     const double T = s.n_tiles, W = s.n_cols, S = s.n_segs, FB = (double)s.fb_w * s.fb_h;
-    const double per_tile = 4 + 4 + 8 + 1 + 4 + 4 * 4 + 16;  // counts, offsets, status, sort, records
-    double b = T * per_tile + 8 * W + 8 * S + 12 * FB + 24 * L + 8 * F + (16 + 256) * A;
+    const double per_tile = 4 + 4 + 8 + 1 + 4 * 4 + 16;  // counts, offsets, status, sort, records
+    double b = T * per_tile + 4 * W + 16 * s.n_paths + 8 * S + 12 * FB + 24 * L + 8 * F + (16 + 256) * A;
     return (uint64_t)b + 4096;
 }
 
@@ -1400,8 +1402,8 @@ int pfh_dump_counts(pfh_ctx *ctx, int32_t scene_id, int64_t *counts) {
         HIP_TRY(hipMemcpy(&a0, ctx->w->tile_alpha_idx.as<uint32_t>() + tb, 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&a1, ctx->w->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4, hipMemcpyDeviceToHost));
         // a scene's entries keep their index range through the sort (keys are grouped by scene)
-        HIP_TRY(hipMemcpy(&e0, ctx->w->col_entry.as<uint32_t>() + s.col_base, 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&e1, ctx->w->col_entry.as<uint32_t>() + s.col_base + s.n_cols, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&e0, ctx->w->path_ent.as<uint2>() + s.path_base, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&e1, ctx->w->path_ent.as<uint2>() + s.path_base + s.n_paths, 4, hipMemcpyDeviceToHost));
         counts[0] = s.n_tiles;
         counts[1] = s.n_cols;
         counts[2] = f1 - f0;
@@ -1460,8 +1462,8 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
         }
         // a scene's entries keep their index range through the sort (keys are grouped by scene)
         uint32_t ebase = 0, eend = 0;
-        d2h(&ebase, ctx->w->col_entry.as<uint32_t>() + cb, 4);
-        d2h(&eend, ctx->w->col_entry.as<uint32_t>() + cb + s.n_cols, 4);
+        d2h(&ebase, ctx->w->path_ent.as<uint2>() + s.path_base, 4);
+        d2h(&eend, ctx->w->path_ent.as<uint2>() + s.path_base + s.n_paths, 4);
         uint32_t ne = eend - ebase;
         std::vector<uint32_t> sv(ne);
         const uint32_t *sorted = ctx->w->last_sorted_alt ? ctx->w->vals_alt.as<uint32_t>() : ctx->w->vals.as<uint32_t>();
