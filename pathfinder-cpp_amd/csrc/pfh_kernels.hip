@@ -1034,31 +1034,45 @@ __device__ __forceinline__ void path_entries(const BatchDev &b, const SceneDev &
     const int fb_row0 = (r.y - sc.fb_y0) * sc.fb_w + (r.x - sc.fb_x0);
     const uint32_t rgba = sc.path_rgba[p];
     const float inv_w = w > 0 ? 1.0f / (float)w : 0.0f;
-    uint32_t e = WRITE ? b.path_ent[gp].x : 0u, n_alpha = 0;
+    uint32_t e = WRITE ? b.path_ent[gp].x : 0u, n_alpha = 0;  // count pass: this wave's counts
+    __shared__ uint32_t wave_tot[2][4];  // block version, write pass: per-wave counts, double-buffered
+    const uint32_t wv = threadIdx.x >> 6;
+    uint32_t parity = 0;
     for (uint32_t j0 = 0; j0 < area; j0 += THREADS) {  // uniform over the wave / block
         const uint32_t j = j0 + t;
         EntryTile et{false, TILE_EMPTY, 0};
         if (j < area) et = entry_tile(b, sc, p, tbase, w, inv_w, fb_row0, j);
-        if (THREADS == 64) {
-            const uint64_t m = __ballot(et.keep);
-            if (WRITE && et.keep) write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask));
-            if (!WRITE) n_alpha += (uint32_t)__popcll(__ballot(et.keep && et.st == TILE_ALPHA));
+        const uint64_t m = __ballot(et.keep);
+        if (!WRITE) {
+            e += (uint32_t)__popcll(m);
+            n_alpha += (uint32_t)__popcll(__ballot(et.keep && et.st == TILE_ALPHA));
+        } else if (THREADS == 64) {
+            if (et.keep) write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask));
             e += (uint32_t)__popcll(m);
         } else {
-            uint32_t total;
-            const uint32_t rank = block_exclusive_scan<uint32_t>(et.keep ? 1u : 0u, total);
-            if (WRITE && et.keep) write_entry(b, sc, p, rgba, tbase, j, et, e + rank);
-            if (!WRITE) n_alpha += et.keep && et.st == TILE_ALPHA ? 1u : 0u;
+            // entries before this wave's in the chunk: the earlier waves' counts.  A slot is
+            // rewritten two chunks later, after a barrier every wave passes once done reading it.
+            if (lane == 0) wave_tot[parity][wv] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+#pragma unroll
+            for (uint32_t v = 0; v < 4; v++) {
+                const uint32_t c = wave_tot[parity][v];
+                before += v < wv ? c : 0u;
+                total += c;
+            }
+            if (et.keep) write_entry(b, sc, p, rgba, tbase, j, et, e + before + (uint32_t)__popcll(m & lt_mask));
             e += total;
+            parity ^= 1u;
         }
     }
     if (!WRITE) {
-        if (THREADS == 256) {  // the per-thread alpha counts: a block sum (the last scan's LDS is free)
-            for (int d = 32; d > 0; d >>= 1) n_alpha += __shfl_xor(n_alpha, d, 64);
-            __shared__ uint32_t part[4];
-            if (lane == 0) part[threadIdx.x >> 6] = n_alpha;
+        if (THREADS == 256) {  // the waves' counts: a block sum
+            __shared__ uint2 part[4];
+            if (lane == 0) part[wv] = make_uint2(e, n_alpha);
             __syncthreads();
-            n_alpha = part[0] + part[1] + part[2] + part[3];
+            e = part[0].x + part[1].x + part[2].x + part[3].x;
+            n_alpha = part[0].y + part[1].y + part[2].y + part[3].y;
         }
         if (t == 0) b.path_ent[gp] = make_uint2(e, n_alpha);
     }
