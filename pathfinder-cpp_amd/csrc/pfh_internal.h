@@ -28,7 +28,7 @@ enum SegKind : uint32_t { SEG_LINE = 1, SEG_QUAD = 2, SEG_CUBIC = 3 };
 
 // Launch schedule (chosen by pfh_autotune; every schedule gives bit-identical results).
 struct Schedule {
-    uint32_t fill_grid = 8192;        // max workgroups of k_fill_masks (grid-stride beyond)
+    uint32_t fill_grid = 16384;       // max workgroups of k_fill_masks (grid-stride beyond)
     uint32_t composite_block = 256;   // threads per k_composite workgroup (a wave per fb tile)
     uint32_t column_block = 256;      // threads per workgroup of k_propagate / k_entries
     uint32_t xcd_order = 0;           // bit 0 fill, bit 1 composite: workgroups grouped per XCD

@@ -13,7 +13,7 @@ def test_every_schedule_gives_identical_pixels():
     ctx = pfh.Context(0)
     scenes = scenegen.batch_scenes(n_scenes=5, seed=808, side_max=1536, paths_max=2000)
     ids = [ctx.upload(s) for s in scenes]
-    assert ctx.schedule() == {"fill_grid": 8192, "composite_block": 256, "column_block": 256, "xcd_order": 0,
+    assert ctx.schedule() == {"fill_grid": 16384, "composite_block": 256, "column_block": 256, "xcd_order": 0,
                               "flatten_segs": 0}
     ctx.rasterize(ids, pfh.PFH_BATCHED)
     want = [ctx.readback(i) for i in ids]
