@@ -50,7 +50,12 @@ __device__ __forceinline__ uint32_t row16_inclusive_sum(uint32_t v) {
 // Over the column's x-window [lo, hi] (width w) the fill runs linearly between heights ymin
 // and ymax.  With G(t) = ∫ max(t - y(x), 0) dx over the window (0 below ymin, w (t - ymean)
 // above ymax, w (t - ymin)^2 / (2 (ymax - ymin)) between), the covered area of pixel row r is
-// F(r + 1) = ∫ clamp(r + 1 - y(x), 0, 1) dx = G(r + 1) - G(r): one G per row boundary.
+// F(r) = ∫ clamp(r + 1 - y(x), 0, 1) dx = G(r + 1) - G(r).  The array holds F's differences
+// down the column in fixed point: rows r0 = floor(ymin) .. r1 = floor(ymax) are crossed, rows
+// below get the full w.  Inside the crossed span F grows by the same 2 w / (2 (ymax - ymin))
+// from row to row (G is quadratic there), so only the first two rows, the last one and the
+// full width are evaluated, and the rows between add one constant: a pair costs the same
+// whatever its height.  The running sum is exact at r0, r0 + 1, r1 and below.
 __device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, int32_t *diff) {
     const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), y0 = (float)(fl.x >> 16) * (1.0f / 256.0f);
     const float x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f), y1 = (float)(fl.y >> 16) * (1.0f / 256.0f);
@@ -67,19 +72,23 @@ __device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, 
     const float k2 = w * 0.5f * __builtin_amdgcn_rcpf(ymax - ymin);  // used only when ymin < t < ymax
     const int r0 = max(0, min(15, (int)floorf(ymin)));
     const int r1 = max(0, min(15, (int)floorf(ymax)));
-    int32_t prev = 0;
-    float g_prev = 0.0f;  // G(r0) = 0: r0 <= ymin
+    const int n = r1 - r0;
+    auto G = [&](int ti) {  // G(0 .. r0) = 0: r0 <= ymin
+        const float t = (float)ti, a = t - ymin;
+        return t >= ymax ? w * (t - half_sum) : (a > 0.0f ? k2 * a * a : 0.0f);
+    };
+    const float g1 = G(r0 + 1), g2 = G(r0 + 2), ga = G(r1), gb = G(r1 + 1);
+    const int32_t full = (int32_t)rintf(ks * w);
+    const int32_t v0 = (int32_t)rintf(ks * g1);          // F(r0)
+    const int32_t v1 = (int32_t)rintf(ks * (g2 - g1));   // F(r0 + 1), when n >= 1
+    const int32_t vd = (int32_t)rintf(ks * (gb - ga));   // F(r1), when n >= 2
+    const int32_t dm = (int32_t)rintf(ks * (2.0f * k2));  // F(r) - F(r - 1) for r0 + 2 <= r <= r1 - 1
     int32_t *cd = diff + col * FILL_COL_WORDS;
-    for (int r = r0; r <= r1; r++) {
-        const float t = (float)(r + 1);
-        const float a = t - ymin;
-        const float g = t >= ymax ? w * (t - half_sum) : (a > 0.0f ? k2 * a * a : 0.0f);
-        const int32_t v = (int32_t)rintf(ks * (g - g_prev));
-        atomicAdd(cd + r, v - prev);
-        prev = v;
-        g_prev = g;
-    }
-    atomicAdd(cd + r1 + 1, (int32_t)rintf(ks * w) - prev);
+    atomicAdd(cd + r0, v0);
+    atomicAdd(cd + r0 + 1, (n == 0 ? full : v1) - v0);
+    for (int r = r0 + 2; r < r1; r++) atomicAdd(cd + r, dm);
+    if (n >= 2) atomicAdd(cd + r1, vd - (v1 + (n - 2) * dm));
+    if (n >= 1) atomicAdd(cd + r1 + 1, full - (n == 1 ? v1 : vd));
 }
 
 // Inclusive prefix sum across the 64-lane wave: DPP within rows, then the row totals.
