@@ -115,6 +115,15 @@ struct SceneDev {
     uint32_t pad;
 };
 
+// A binned path, resolved once per launch (after the chord scan) so that the per-path
+// kernels start from one load instead of a chain of dependent ones.
+struct BinItem {
+    int4 rect;                 // tile rect (l, t, r, b)
+    uint32_t tbase, cbase;     // batch index of its first tile / column
+    uint32_t c0, c1;           // its chords [c0, c1)
+    uint32_t p, gp, k, pad;    // local path, batch path, scene
+};
+
 // Batch-wide work arrays (device pointers).  Sizes: S segments, T tiles,
 // W columns, FB framebuffer tiles, F/A/E capacities for fills, alpha tiles, entries.
 struct BatchDev {
@@ -137,6 +146,7 @@ struct BatchDev {
     uint8_t *tile_st;             // [T] tile status (TileStatus)
     uint2 *path_ent;              // [P + 1] {draw entries, alpha entries} per batch path, then exclusive scan
     uint2 *path_tot;              // [P + 1] {fills, alpha tiles} per batch path, then exclusive scan
+    BinItem *bin_items;           // [n_bin_wave + n_bin_group] wave class, then group class
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
     float4 *lines;                // [L] flattened chords (from, to) in emission order

@@ -624,6 +624,29 @@ __device__ __forceinline__ U2 block_sum_u2(U2 v, U2 *part) {
     return t;
 }
 
+// One descriptor per binned path: thread per item, class GROUP 0 (wave) or 1 (group).
+template <int GROUP>
+__global__ __launch_bounds__(256) void k_bin_items(BatchDev b) {
+    const uint32_t n = GROUP ? b.n_bin_group : b.n_bin_wave;
+    const uint32_t *bases = GROUP ? b.bin_group_bases : b.bin_wave_bases;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane searches (lanes past the end with the last item), then those leave
+    const int k = find_scene_lanes(bases, b.n_scenes, min(i, n - 1));
+    if (i >= n) return;
+    const SceneDev &sc = b.scenes[k];
+    const uint32_t p = sc.bin_paths[(GROUP ? sc.n_bin_wave : 0u) + (i - bases[k])];
+    BinItem it;
+    it.rect = sc.path_rect[p];
+    it.tbase = sc.tile_base + sc.path_tile_off[p];
+    it.cbase = sc.col_base + sc.path_col_off[p];
+    path_chords(b, sc, p, it.c0, it.c1);
+    it.p = p;
+    it.gp = sc.path_base + p;
+    it.k = (uint32_t)k;
+    it.pad = 0;
+    b.bin_items[(GROUP ? b.n_bin_wave : 0u) + i] = it;
+}
+
 // Wave class: 4 paths per 256-thread block, one per wave, each with its own LDS slice.
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
@@ -633,17 +656,16 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wv;
     if (item >= b.n_bin_wave) return;  // whole wave: no block-wide barriers below
-    const int k = find_scene_wave(b.bin_wave_bases, b.n_scenes, item);
-    const SceneDev &sc = b.scenes[k];
-    const uint32_t p = sc.bin_paths[item - b.bin_wave_bases[k]];
-    const int4 rect = sc.path_rect[p];
+    const BinItem it = b.bin_items[item];
+    const SceneDev &sc = b.scenes[it.k];
+    const int4 rect = it.rect;
     const uint32_t tw = (uint32_t)max(rect.z - rect.x, 0), th = (uint32_t)max(rect.w - rect.y, 0);
     const uint32_t area = tw * th;
     if (area > (uint32_t)BIN_WAVE_TILES || tw > (uint32_t)BIN_WAVE_COLS) return;  // not this class (host sorts)
-    const uint32_t tbase = sc.tile_base + sc.path_tile_off[p], cbase = sc.col_base + sc.path_col_off[p];
+    const uint32_t tbase = it.tbase, cbase = it.cbase;
     uint32_t *l_cnt = s_cnt[wv];
     int32_t *l_delta = s_delta[wv], *l_col = s_col[wv];
-    const uint32_t gp = sc.path_base + p;
+    const uint32_t gp = it.gp;
     if (EMIT) {
         const uint2 base = b.path_tot[gp];
         wave_tile_offsets(b, tbase, area, U2{base.x, base.y}, l_cnt);
@@ -655,9 +677,7 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
         for (uint32_t j = lane; j < tw; j += 64) l_col[j] = 0;
     }
     wave_sync();
-    uint32_t c0, c1;
-    path_chords(b, sc, p, c0, c1);
-    walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, l_cnt, l_delta, l_col, c0, c1, lane, 64);
+    walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, l_cnt, l_delta, l_col, it.c0, it.c1, lane, 64);
     if (!EMIT) {
         wave_sync();
         U2 tot{0, 0};
@@ -682,17 +702,15 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
     __shared__ int32_t s_col[EMIT ? 1 : BIN_GROUP_COLS];
     const uint32_t item = blockIdx.x;
     if (item >= b.n_bin_group) return;  // uniform over the block
-    const int k = find_scene_wave(b.bin_group_bases, b.n_scenes, item);
-    const SceneDev &sc = b.scenes[k];
-    const uint32_t p = sc.bin_paths[sc.n_bin_wave + (item - b.bin_group_bases[k])];
-    const int4 rect = sc.path_rect[p];
+    const BinItem it = b.bin_items[b.n_bin_wave + item];
+    const SceneDev &sc = b.scenes[it.k];
+    const int4 rect = it.rect;
     const uint32_t tw = (uint32_t)max(rect.z - rect.x, 0), th = (uint32_t)max(rect.w - rect.y, 0);
     const uint32_t area = tw * th;
-    const uint32_t tbase = sc.tile_base + sc.path_tile_off[p], cbase = sc.col_base + sc.path_col_off[p];
+    const uint32_t tbase = it.tbase, cbase = it.cbase;
     const bool lds = area <= (uint32_t)BIN_GROUP_TILES && tw <= (uint32_t)BIN_GROUP_COLS;
-    const uint32_t gp = sc.path_base + p;
-    uint32_t c0, c1;
-    path_chords(b, sc, p, c0, c1);
+    const uint32_t gp = it.gp;
+    const uint32_t c0 = it.c0, c1 = it.c1;
     if (lds) {
         if (EMIT) {
             const uint2 base = b.path_tot[gp];
@@ -1024,13 +1042,14 @@ __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &s
 
 // THREADS 64: one wave per path (its own item); 256: one block per path.
 template <bool WRITE, int THREADS>
-__device__ __forceinline__ void path_entries(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t gp) {
+__device__ __forceinline__ void path_entries(const BatchDev &b, const SceneDev &sc, const BinItem &it) {
     const uint32_t t = THREADS == 64 ? (threadIdx.x & 63) : threadIdx.x, lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const int4 r = sc.path_rect[p];
+    const uint32_t p = it.p, gp = it.gp;
+    const int4 r = it.rect;
     const int w = max(r.z - r.x, 0), h = max(r.w - r.y, 0);
     const uint32_t area = (uint32_t)w * (uint32_t)h;
-    const uint32_t tbase = sc.tile_base + sc.path_tile_off[p];
+    const uint32_t tbase = it.tbase;
     const int fb_row0 = (r.y - sc.fb_y0) * sc.fb_w + (r.x - sc.fb_x0);
     const uint32_t rgba = sc.path_rgba[p];
     const float inv_w = w > 0 ? 1.0f / (float)w : 0.0f;
@@ -1082,20 +1101,16 @@ template <bool WRITE>
 __global__ __launch_bounds__(256) void k_entries_wave(BatchDev b) {
     const uint32_t item = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (item >= b.n_bin_wave) return;  // whole wave
-    const int k = find_scene_wave(b.bin_wave_bases, b.n_scenes, item);
-    const SceneDev &sc = b.scenes[k];
-    const uint32_t p = sc.bin_paths[item - b.bin_wave_bases[k]];
-    path_entries<WRITE, 64>(b, sc, p, sc.path_base + p);
+    const BinItem it = b.bin_items[item];
+    path_entries<WRITE, 64>(b, b.scenes[it.k], it);
 }
 
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_entries_group(BatchDev b) {
     const uint32_t item = blockIdx.x;
     if (item >= b.n_bin_group) return;  // uniform over the block
-    const int k = find_scene_wave(b.bin_group_bases, b.n_scenes, item);
-    const SceneDev &sc = b.scenes[k];
-    const uint32_t p = sc.bin_paths[sc.n_bin_wave + (item - b.bin_group_bases[k])];
-    path_entries<WRITE, 256>(b, sc, p, sc.path_base + p);
+    const BinItem it = b.bin_items[b.n_bin_wave + item];
+    path_entries<WRITE, 256>(b, b.scenes[it.k], it);
 }
 
 // ---------------------------------------------------------------------------
@@ -1258,6 +1273,12 @@ hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, 
 }
 
 // grid_lines: chords the launch is sized for (exact in plain mode, capacity in batched mode)
+hipError_t launch_bin_items(const BatchDev &b, hipStream_t s) {
+    if (b.n_bin_wave) hipLaunchKernelGGL(k_bin_items<0>, dim3(blocks_for(b.n_bin_wave, 256)), dim3(256), 0, s, b);
+    if (b.n_bin_group) hipLaunchKernelGGL(k_bin_items<1>, dim3(blocks_for(b.n_bin_group, 256)), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s) {
     if (b.n_bin_wave) {
         const dim3 g(blocks_for(b.n_bin_wave, 4));

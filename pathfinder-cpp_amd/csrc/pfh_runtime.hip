@@ -32,6 +32,7 @@ hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, ui
 hipError_t scan_total(uint32_t *inout, uint32_t n, uint32_t *counters, int counter, uint32_t *tmp, hipStream_t s);
 hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s);
 hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s);
+hipError_t launch_bin_items(const BatchDev &b, hipStream_t s);
 hipError_t launch_fb_bounds(const uint32_t *sorted_keys, const uint32_t *n_ptr, uint32_t cap, uint32_t n_fb,
                             uint2 *range, hipStream_t s);
 size_t radix_temp_elems(uint64_t cap);
@@ -158,7 +159,7 @@ struct Workspace {
     // around the fill and composite kernels of a batched launch; recorded as event nodes
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
-    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, path_tot, col_bd,
+    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, path_tot, bin_items, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, counters, scan_tmp, sort_tmp, ones_mask;
     uint32_t *h_counters = nullptr;  // pinned
@@ -183,7 +184,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &path_ent, &path_tot, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
+                &path_ent, &path_tot, &bin_items, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
                 &sort_tmp, &ones_mask};
     }
@@ -532,6 +533,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.tile_st.ensure(pl.T + 4);
     w.path_ent.ensure(8 * pl.P + 8);
     w.path_tot.ensure(8 * pl.P + 8);
+    w.bin_items.ensure(sizeof(BinItem) * (pl.BW + pl.BG) + 64);
     w.col_bd.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
     w.lines.ensure(16ull * pl.cap_l);
@@ -709,6 +711,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.tile_st = w.tile_st.as<uint8_t>();
     b.path_ent = w.path_ent.as<uint2>();
     b.path_tot = w.path_tot.as<uint2>();
+    b.bin_items = w.bin_items.as<BinItem>();
     b.n_paths = (uint32_t)pl.P;
     b.col_bd = w.col_bd.as<int32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
@@ -752,6 +755,7 @@ static void stage_count(Workspace &w, const BatchDev &b, const Plan &pl) {
     HIP_TRY(launch_flatten(b, false, flatten_segs(pl), s));
     HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp, s));
     HIP_TRY(launch_flatten(b, true, flatten_segs(pl), s));
+    HIP_TRY(launch_bin_items(b, s));
     HIP_TRY(launch_bin(b, false, s));
     HIP_TRY(scan_paths(b, w.scan_tmp.ptr, s));
 }
@@ -843,6 +847,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     }
     HIP_TRY(launch_flatten(b, true, flatten_segs(pl), st));
     HIP_TRY(hipEventRecord(ev[1], st));
+    HIP_TRY(launch_bin_items(b, st));
     HIP_TRY(launch_bin(b, false, st));
     HIP_TRY(scan_paths(b, ctx->w->scan_tmp.ptr, st));
     read_counters(ctx);  // fill / alpha counts (renderer.cpp:834-845)
