@@ -121,7 +121,9 @@ struct BinItem {
     int4 rect;                 // tile rect (l, t, r, b)
     uint32_t tbase, cbase;     // batch index of its first tile / column
     uint32_t c0, c1;           // its chords [c0, c1)
-    uint32_t p, gp, k, pad;    // local path, batch path, scene
+    uint32_t p, gp, k;         // local path, batch path, scene
+    uint32_t rgba, rule;       // paint, fill rule
+    uint32_t pad[3];
 };
 
 // Batch-wide work arrays (device pointers).  Sizes: S segments, T tiles,
@@ -147,6 +149,7 @@ struct BatchDev {
     uint2 *path_ent;              // [P + 1] {draw entries, alpha entries} per batch path, then exclusive scan
     uint2 *path_tot;              // [P + 1] {fills, alpha tiles} per batch path, then exclusive scan
     BinItem *bin_items;           // [n_bin_wave + n_bin_group] wave class, then group class
+    uint32_t *col_item;           // [W] the bin item owning each column
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
     float4 *lines;                // [L] flattened chords (from, to) in emission order

@@ -643,8 +643,13 @@ __global__ __launch_bounds__(256) void k_bin_items(BatchDev b) {
     it.p = p;
     it.gp = sc.path_base + p;
     it.k = (uint32_t)k;
-    it.pad = 0;
-    b.bin_items[(GROUP ? b.n_bin_wave : 0u) + i] = it;
+    it.rgba = sc.path_rgba[p];
+    it.rule = sc.path_rule[p];
+    it.pad[0] = it.pad[1] = it.pad[2] = 0;
+    const uint32_t gi = (GROUP ? b.n_bin_wave : 0u) + i;
+    b.bin_items[gi] = it;
+    const uint32_t tw = (uint32_t)max(it.rect.z - it.rect.x, 0);
+    for (uint32_t j = 0; j < tw; j++) b.col_item[it.cbase + j] = gi;
 }
 
 // Wave class: 4 paths per 256-thread block, one per wave, each with its own LDS slice.
@@ -932,17 +937,18 @@ struct ColumnWalk {
     uint32_t p, rgba, rule, tbase;
     int w, h, x, fb_row0;  // fb_row0: fb tile index (scene-local) of row 0
     uint32_t fb_base, fb_w;
-    __device__ ColumnWalk(const BatchDev &b, int k, uint32_t c) {
-        const SceneDev &sc = b.scenes[k];
-        const uint32_t cl = c - sc.col_base;
-        p = sc.col_path[cl];
-        const int4 r = sc.path_rect[p];
+    // column c of the batch, through its bin item (every column belongs to a binned path)
+    __device__ ColumnWalk(const BatchDev &b, uint32_t c) {
+        const BinItem it = b.bin_items[b.col_item[c]];
+        const SceneDev &sc = b.scenes[it.k];
+        const int4 r = it.rect;
+        p = it.p;
         w = r.z - r.x;
         h = r.w - r.y;
-        x = (int)(cl - sc.path_col_off[p]);
-        tbase = sc.tile_base + sc.path_tile_off[p];
-        rgba = sc.path_rgba[p];
-        rule = sc.path_rule[p];
+        x = (int)(c - it.cbase);
+        tbase = it.tbase;
+        rgba = it.rgba;
+        rule = it.rule;
         fb_base = sc.fb_base;
         fb_w = (uint32_t)sc.fb_w;
         fb_row0 = (r.y - sc.fb_y0) * sc.fb_w + (r.x + x - sc.fb_x0);
@@ -953,10 +959,8 @@ struct ColumnWalk {
 
 __global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    // every lane searches (lanes past the end with the last column), then those leave
-    int k = find_scene_lanes(b.col_bases, b.n_scenes, min(c, b.n_cols - 1));
     if (c >= b.n_cols) return;
-    const ColumnWalk cw(b, k, c);
+    const ColumnWalk cw(b, c);
     const bool opaque = (cw.rgba >> 24) == 255u;
     int32_t running = b.col_bd[c];
     for (int row0 = 0; row0 < cw.h; row0 += COL_ROWS) {
@@ -1051,7 +1055,7 @@ __device__ __forceinline__ void path_entries(const BatchDev &b, const SceneDev &
     const uint32_t area = (uint32_t)w * (uint32_t)h;
     const uint32_t tbase = it.tbase;
     const int fb_row0 = (r.y - sc.fb_y0) * sc.fb_w + (r.x - sc.fb_x0);
-    const uint32_t rgba = sc.path_rgba[p];
+    const uint32_t rgba = it.rgba;
     const float inv_w = w > 0 ? 1.0f / (float)w : 0.0f;
     uint32_t e = WRITE ? b.path_ent[gp].x : 0u, n_alpha = 0;  // count pass: this wave's counts
     __shared__ uint32_t wave_tot[2][4];  // block version, write pass: per-wave counts, double-buffered

@@ -159,7 +159,7 @@ struct Workspace {
     // around the fill and composite kernels of a batched launch; recorded as event nodes
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
-    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, path_tot, bin_items, col_bd,
+    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, path_tot, bin_items, col_item, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, counters, scan_tmp, sort_tmp, ones_mask;
     uint32_t *h_counters = nullptr;  // pinned
@@ -184,7 +184,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &path_ent, &path_tot, &bin_items, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
+                &path_ent, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
                 &sort_tmp, &ones_mask};
     }
@@ -534,6 +534,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.path_ent.ensure(8 * pl.P + 8);
     w.path_tot.ensure(8 * pl.P + 8);
     w.bin_items.ensure(sizeof(BinItem) * (pl.BW + pl.BG) + 64);
+    w.col_item.ensure(4 * pl.W + 4);
     w.col_bd.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
     w.lines.ensure(16ull * pl.cap_l);
@@ -712,6 +713,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.path_ent = w.path_ent.as<uint2>();
     b.path_tot = w.path_tot.as<uint2>();
     b.bin_items = w.bin_items.as<BinItem>();
+    b.col_item = w.col_item.as<uint32_t>();
     b.n_paths = (uint32_t)pl.P;
     b.col_bd = w.col_bd.as<int32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
