@@ -1125,20 +1125,63 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t *keys, const uint32_t *n_ptr, uint32_t cap,
-                                                        int shift, uint32_t *hist, uint32_t nblocks) {
+// The sort runs per scene: a scene's entries are contiguous and its keys lie in its own range
+// of framebuffer tiles, so sorting each scene's entries by their scene-local tile index
+// (16 bits for canvases up to 4096^2) sorts the whole batch.  Sort blocks never straddle
+// scenes: scene k's entries [seg_e[k], seg_e[k + 1]) (clamped to the capacity) take blocks
+// [seg_b[k], seg_b[k + 1]) of RS_TILE items, and the digit histogram is laid out
+// [scene][digit][block of the scene], so its exclusive scan places every item inside its
+// scene's range, ordered by digit, then block, then rank.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_segments(BatchDev b, uint32_t *seg_e, uint32_t *seg_b) {
+    uint32_t carry = 0;
+    for (int k0 = 0; k0 <= b.n_scenes; k0 += RS_THREADS) {
+        const int k = k0 + (int)threadIdx.x;
+        uint32_t e = 0, e_next = 0;
+        if (k <= b.n_scenes) {
+            const uint32_t total = min(b.counters[C_ENTRIES], b.cap_entries);
+            e = k < b.n_scenes ? min(b.path_ent[b.scenes[k].path_base].x, total) : total;
+            e_next = k + 1 < b.n_scenes ? min(b.path_ent[b.scenes[k + 1].path_base].x, total) : total;
+            seg_e[k] = e;
+        }
+        const uint32_t nbk = k < b.n_scenes ? (e_next - e + RS_TILE - 1) / RS_TILE : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<uint32_t>(nbk, tot);
+        if (k <= b.n_scenes) seg_b[k] = carry + ex;
+        carry += tot;
+    }
+}
+
+// This block's scene and item range; false past the last block.
+struct RsBlock {
+    int k;
+    uint32_t lb, nbk, hbase, e0, e1, fb_base;
+};
+__device__ __forceinline__ bool rs_block(const BatchDev &b, const uint32_t *seg_e, const uint32_t *seg_b, RsBlock &rb) {
+    if (blockIdx.x >= seg_b[b.n_scenes]) return false;
+    rb.k = find_scene_wave(seg_b, b.n_scenes, blockIdx.x);
+    rb.lb = blockIdx.x - seg_b[rb.k];
+    rb.nbk = seg_b[rb.k + 1] - seg_b[rb.k];
+    rb.hbase = 256u * seg_b[rb.k];
+    rb.e0 = seg_e[rb.k] + rb.lb * RS_TILE;
+    rb.e1 = min(rb.e0 + RS_TILE, seg_e[rb.k + 1]);
+    rb.fb_base = b.scenes[rb.k].fb_base;
+    return true;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(BatchDev b, const uint32_t *keys, const uint32_t *seg_e,
+                                                        const uint32_t *seg_b, int shift, uint32_t *hist) {
     __shared__ uint32_t h[256];
+    RsBlock rb;
+    if (!rs_block(b, seg_e, seg_b, rb)) return;  // uniform over the block
     h[threadIdx.x] = 0;
     __syncthreads();
-    uint32_t n = min(*n_ptr, cap);
-    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
 #pragma unroll 4
     for (int i = 0; i < RS_ITEMS; i++) {
-        uint64_t j = base + (uint64_t)i * RS_THREADS + threadIdx.x;
-        if (j < n) atomicAdd(&h[(keys[j] >> shift) & 255u], 1u);
+        const uint32_t j = rb.e0 + (uint32_t)i * RS_THREADS + threadIdx.x;
+        if (j < rb.e1) atomicAdd(&h[((keys[j] - rb.fb_base) >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    hist[rb.hbase + threadIdx.x * rb.nbk + rb.lb] = h[threadIdx.x];
 }
 
 // Scatter of one 8-bit digit pass.  The block's 4096 items are first ranked stably by digit
@@ -1148,22 +1191,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t *keys, co
 // coalesce instead of scattering one item per digit per row.  offs is the exclusive scan of
 // the [digit][block] histogram: the block's count of digit d is the difference of its
 // entry and the next one.  With no vals, an item's value is its input index.
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *keys, const uint32_t *vals,
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(BatchDev b, const uint32_t *keys, const uint32_t *vals,
                                                            uint32_t *keys_out, uint32_t *vals_out,
-                                                           const uint32_t *n_ptr, uint32_t cap, int shift,
-                                                           const uint32_t *offs, uint32_t nblocks) {
+                                                           const uint32_t *seg_e, const uint32_t *seg_b, int shift,
+                                                           const uint32_t *offs) {
     __shared__ uint32_t skey[RS_TILE], sval[RS_TILE];
     __shared__ uint32_t run[256];     // items of each digit placed so far (local)
     __shared__ uint32_t lstart[256];  // local start of each digit
     __shared__ uint32_t goff[256];    // global start of each digit's run from this block
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t n = min(*n_ptr, cap);
-    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
-    if (base >= n) return;  // uniform over the block
-    const uint32_t valid_n = (uint32_t)min<uint64_t>(RS_TILE, n - base);
+    RsBlock rb;
+    if (!rs_block(b, seg_e, seg_b, rb)) return;  // uniform over the block
+    const uint32_t base = rb.e0;
+    const uint32_t valid_n = rb.e1 - rb.e0;
     {
-        const uint64_t h = (uint64_t)threadIdx.x * nblocks + blockIdx.x;
+        const uint32_t h = rb.hbase + threadIdx.x * rb.nbk + rb.lb;
         const uint32_t o = offs[h];
         goff[threadIdx.x] = o;
         uint32_t total;
@@ -1179,7 +1222,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *keys,
         const bool valid = li < valid_n;
         const uint32_t key = valid ? keys[base + li] : 0u;
         const uint32_t val = !valid ? 0u : (vals ? vals[base + li] : (uint32_t)(base + li));
-        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t d = ((key - rb.fb_base) >> shift) & 255u;
         // lanes holding the same digit (multisplit peer mask)
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -1205,7 +1248,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *keys,
     }
     for (uint32_t i = threadIdx.x; i < valid_n; i += RS_THREADS) {
         const uint32_t key = skey[i];
-        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t d = ((key - rb.fb_base) >> shift) & 255u;
         const uint32_t g = goff[d] + (i - lstart[d]);
         keys_out[g] = key;
         vals_out[g] = sval[i];
@@ -1359,28 +1402,38 @@ hipError_t launch_fb_bounds(const uint32_t *sorted_keys, const uint32_t *n_ptr, 
     return hipGetLastError();
 }
 
-size_t radix_temp_elems(uint64_t cap) { return 256ull * blocks_for(cap, RS_TILE) + 1 + scan_temp_elems(256ull * blocks_for(cap, RS_TILE)); }
+// Sort blocks for a capacity of `cap` entries over `n_scenes` scenes (one partial block each).
+static uint32_t rs_max_blocks(uint64_t cap, uint32_t n_scenes) { return blocks_for(cap, RS_TILE) + n_scenes + 1; }
 
-// Sorts keys[0 .. *n_ptr), with values starting as the item indices (vals is not read), by
-// the low `bits` bits of the key, stably; returns the buffer pair holding the result via
-// *result_alt (false: keys/vals, true: keys_alt/vals_alt).
-hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, const uint32_t *n_ptr,
-                      uint32_t cap, int bits, uint32_t *tmp, bool *result_alt, hipStream_t s) {
-    uint32_t nb = blocks_for(cap, RS_TILE);
-    if (nb == 0) nb = 1;
-    uint32_t nh = 256 * nb;
-    uint32_t *hist = tmp;
-    uint32_t *scan_tmp = tmp + nh + 1;
+size_t radix_temp_elems(uint64_t cap, uint32_t n_scenes) {
+    const uint64_t nh = 256ull * rs_max_blocks(cap, n_scenes);
+    return 2ull * (n_scenes + 1) + nh + 1 + scan_temp_elems(nh);
+}
+
+// Sorts the batch's entry keys (b.keys[0 .. entries)), with values starting as the item
+// indices (vals is not read), scene by scene by the low `bits` bits of the scene-local
+// framebuffer tile, stably; returns the buffer pair holding the result via *result_alt
+// (false: keys/vals, true: keys_alt/vals_alt).
+hipError_t radix_sort(const BatchDev &b, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                      int bits, uint32_t *tmp, bool *result_alt, hipStream_t s) {
+    const uint32_t n_sc = (uint32_t)b.n_scenes;
+    const uint32_t nb = rs_max_blocks(b.cap_entries, n_sc);
+    const uint32_t nh = 256 * nb;
+    uint32_t *seg_e = tmp, *seg_b = tmp + (n_sc + 1);
+    uint32_t *hist = tmp + 2 * (n_sc + 1);
+    uint32_t *scan_tmp = hist + nh + 1;
+    hipLaunchKernelGGL(k_rs_segments, dim3(1), dim3(RS_THREADS), 0, s, b, seg_e, seg_b);
     bool alt = false;
     for (int shift = 0; shift < bits; shift += 8) {
         const uint32_t *ki = alt ? keys_alt : keys;
         const uint32_t *vi = shift == 0 ? nullptr : (alt ? vals_alt : vals);  // values start as the indices
         uint32_t *ko = alt ? keys : keys_alt;
         uint32_t *vo = alt ? vals : vals_alt;
-        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, ki, n_ptr, cap, shift, hist, nb);
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, b, ki, seg_e, seg_b, shift, hist);
+        // the scan covers the capacity's blocks; entries past this launch's blocks are never read
         hipError_t e = scan_u32(hist, hist, nh, scan_tmp, s);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n_ptr, cap, shift, hist, nb);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, s, b, ki, vi, ko, vo, seg_e, seg_b, shift, hist);
         alt = !alt;
     }
     *result_alt = alt;

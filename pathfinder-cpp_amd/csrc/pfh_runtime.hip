@@ -35,9 +35,9 @@ hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s);
 hipError_t launch_bin_items(const BatchDev &b, hipStream_t s);
 hipError_t launch_fb_bounds(const uint32_t *sorted_keys, const uint32_t *n_ptr, uint32_t cap, uint32_t n_fb,
                             uint2 *range, hipStream_t s);
-size_t radix_temp_elems(uint64_t cap);
-hipError_t radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, const uint32_t *n_ptr,
-                      uint32_t cap, int bits, uint32_t *tmp, bool *result_alt, hipStream_t s);
+size_t radix_temp_elems(uint64_t cap, uint32_t n_scenes);
+hipError_t radix_sort(const BatchDev &b, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                      int bits, uint32_t *tmp, bool *result_alt, hipStream_t s);
 hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, uint32_t max_blocks, hipStream_t s);
 hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, uint32_t block, hipStream_t s);
 hipError_t mlp_forward(const float *params, const float *x, int64_t n, float *hidden, float *out, hipStream_t s);
@@ -557,7 +557,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.keys_alt.ensure(4ull * pl.cap_e);
     w.vals_alt.ensure(4ull * pl.cap_e);
     w.entry_rec.ensure(16ull * pl.cap_e);
-    w.sort_tmp.ensure(4 * radix_temp_elems(pl.cap_e) + 64);
+    w.sort_tmp.ensure(4 * radix_temp_elems(pl.cap_e, (uint32_t)pl.ids.size()) + 64);
 }
 
 // Capacities the current allocations hold, by the same per-buffer sizes ensure_sizes asks
@@ -669,8 +669,11 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
     pl.W = W;
     pl.FB = FB;
     pl.P = P;
+    // the sort runs per scene on scene-local framebuffer tiles
+    uint64_t max_fb = 1;
+    for (size_t k = 0; k < n; k++) max_fb = std::max<uint64_t>(max_fb, ctx->w->h_table[k].n_fb);
     pl.key_bits = 1;
-    while ((1ull << pl.key_bits) < FB) pl.key_bits++;
+    while ((1ull << pl.key_bits) < max_fb) pl.key_bits++;
 }
 
 static void upload_tables(Workspace &w, const Plan &pl) {
@@ -775,8 +778,8 @@ static void stage_emit(Workspace &w, const BatchDev &b, const Plan &pl) {
 static void stage_finish(Workspace &w, const BatchDev &b, const Plan &pl) {
     HIP_TRY(launch_entries(b, true, w.stream));
     bool alt = false;
-    HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
-                       w.sort_tmp.as<uint32_t>(), &alt, w.stream));
+    HIP_TRY(radix_sort(b, b.keys, b.vals, b.keys_alt, b.vals_alt, pl.key_bits, w.sort_tmp.as<uint32_t>(), &alt,
+                       w.stream));
     w.last_sorted_alt = alt;
     HIP_TRY(launch_fb_bounds(alt ? b.keys_alt : b.keys, b.counters + C_ENTRIES, b.cap_entries, b.n_fb, b.fb_range,
                              w.stream));
@@ -871,8 +874,8 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     HIP_TRY(launch_entries(b, true, st));
     HIP_TRY(hipEventRecord(ev[5], st));
     bool alt = false;
-    HIP_TRY(radix_sort(b.keys, b.vals, b.keys_alt, b.vals_alt, b.counters + C_ENTRIES, b.cap_entries, pl.key_bits,
-                       ctx->w->sort_tmp.as<uint32_t>(), &alt, st));
+    HIP_TRY(radix_sort(b, b.keys, b.vals, b.keys_alt, b.vals_alt, pl.key_bits, ctx->w->sort_tmp.as<uint32_t>(), &alt,
+                       st));
     ctx->w->last_sorted_alt = alt;
     HIP_TRY(launch_fb_bounds(alt ? b.keys_alt : b.keys, b.counters + C_ENTRIES, b.cap_entries, b.n_fb, b.fb_range, st));
     HIP_TRY(hipEventRecord(ev[6], st));
