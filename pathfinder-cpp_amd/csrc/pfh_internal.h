@@ -163,7 +163,8 @@ struct BatchDev {
     const uint32_t *ones_mask;    // [64] an all-ones coverage mask row (full coverage)
     uint32_t *keys, *vals;        // [E] sort keys / values
     uint32_t *keys_alt, *vals_alt;
-    uint4 *entry_rec;             // [E] (alpha index | NO_ALPHA, rgba (alpha 0 if covers nothing), local path, fb)
+    uint2 *entry_rec;             // [E] (alpha index | NO_ALPHA, rgba (alpha 0 if covers nothing))
+    uint2 *entry_meta;            // [E] with dumps only: (local path, scene-local fb tile)
     uint32_t *counters;           // [C_COUNT]
     int32_t dump;
     uint32_t xcd_order;           // Schedule::xcd_order

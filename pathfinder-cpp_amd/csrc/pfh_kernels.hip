@@ -1041,7 +1041,8 @@ __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &s
     // even-odd solid tile with an even winding covers nothing: zero alpha
     const uint32_t eff = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
     b.keys[e] = sc.fb_base + et.fbl;  // its value, e, is generated by the sort's first pass
-    b.entry_rec[e] = make_uint4(ai, eff, p, et.fbl);
+    b.entry_rec[e] = make_uint2(ai, eff);
+    if (b.dump) b.entry_meta[e] = make_uint2(p, et.fbl);
 }
 
 // THREADS 64: one wave per path (its own item); 256: one block per path.

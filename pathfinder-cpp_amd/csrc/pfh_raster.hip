@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         const uint32_t cnt = min(64u, e1 - base);
         uint32_t ai = NO_ALPHA, c = 0;
         if (i < e1) {
-            const uint4 rec = b.entry_rec[sorted_vals[i]];
+            const uint2 rec = b.entry_rec[sorted_vals[i]];
             ai = rec.x;
             c = rec.y;
             if (ai != NO_ALPHA && ai >= b.cap_alpha) {  // capacity overflow: the run is retried

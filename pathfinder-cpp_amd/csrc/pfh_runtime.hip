@@ -161,7 +161,7 @@ struct Workspace {
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, path_tot, bin_items, col_item, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_range, keys, vals, keys_alt,
-        vals_alt, entry_rec, counters, scan_tmp, sort_tmp, ones_mask;
+        vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp, ones_mask;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
     size_t h_table_cap = 0;
@@ -185,7 +185,7 @@ struct Workspace {
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
                 &path_ent, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
-                &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &counters, &scan_tmp,
+                &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &entry_meta, &counters, &scan_tmp,
                 &sort_tmp, &ones_mask};
     }
     size_t device_bytes() {
@@ -556,7 +556,8 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.vals.ensure(4ull * pl.cap_e);
     w.keys_alt.ensure(4ull * pl.cap_e);
     w.vals_alt.ensure(4ull * pl.cap_e);
-    w.entry_rec.ensure(16ull * pl.cap_e);
+    w.entry_rec.ensure(8ull * pl.cap_e);
+    if (dump) w.entry_meta.ensure(8ull * pl.cap_e);
     w.sort_tmp.ensure(4 * radix_temp_elems(pl.cap_e, (uint32_t)pl.ids.size()) + 64);
 }
 
@@ -733,7 +734,8 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.vals = w.vals.as<uint32_t>();
     b.keys_alt = w.keys_alt.as<uint32_t>();
     b.vals_alt = w.vals_alt.as<uint32_t>();
-    b.entry_rec = w.entry_rec.as<uint4>();
+    b.entry_rec = w.entry_rec.as<uint2>();
+    b.entry_meta = dump ? w.entry_meta.as<uint2>() : nullptr;
     b.counters = w.counters.as<uint32_t>();
     b.dump = dump ? 1 : 0;
     b.xcd_order = pl.sched.xcd_order;
@@ -1138,7 +1140,7 @@ static uint64_t scene_launch_bytes(const pfh_ctx *ctx, const SceneHost &s) {
         A = s.n_tiles;
     }
     const double T = s.n_tiles, W = s.n_cols, S = s.n_segs, FB = (double)s.fb_w * s.fb_h;
-    const double per_tile = 4 + 4 + 8 + 1 + 4 * 4 + 16;  // counts, offsets, status, sort, records
+    const double per_tile = 4 + 4 + 8 + 1 + 4 * 4 + 8;  // counts, offsets, status, sort, records
     double b = T * per_tile + 4 * W + 16 * s.n_paths + 8 * S + 12 * FB + 24 * L + 8 * F + (16 + 256) * A;
     return (uint64_t)b + 4096;
 }
@@ -1484,11 +1486,11 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
             ehi = std::max(ehi, v + 1);
         }
         if (ne) {
-            std::vector<uint4> recs(ehi - elo);
-            d2h(recs.data(), ctx->w->entry_rec.as<uint4>() + elo, 16 * (size_t)(ehi - elo));
+            std::vector<uint2> meta(ehi - elo);
+            d2h(meta.data(), ctx->w->entry_meta.as<uint2>() + elo, 8 * (size_t)(ehi - elo));
             for (uint32_t i = 0; i < ne; i++) {
-                entry_fbtile[i] = recs[sv[i] - elo].w;
-                entry_path[i] = recs[sv[i] - elo].z;
+                entry_fbtile[i] = meta[sv[i] - elo].y;
+                entry_path[i] = meta[sv[i] - elo].x;
             }
         }
         d2h(zbuf, ctx->w->zbuf.as<uint32_t>() + fbb, 4 * (size_t)nfb);
