@@ -305,11 +305,10 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
                     const SceneDev &sc = b.scenes[kl];
                     const uint32_t s_local = gl - sc.seg_base;
                     const uint2 sg = sc.segs[s_local];
-                    const int4 rect = sc.path_rect[sc.seg_path[s_local]];
                     qbase = EMIT ? b.seg_cnt[gl] : 0u;
-                    // With no tile columns nothing can land (add_fill culls, adjust_alpha_tile_backdrop
-                    // ignores x outside [0, width)); with columns but no rows, above-rect backdrops still do.
-                    if (rect.z - rect.x > 0) {
+                    // kind 0: a segment of a path without tile columns, from which nothing can land
+                    // (with columns but no rows, above-rect backdrops still do: those are kept)
+                    if (sg.y != 0) {
                         const float2 *P = sc.pts + sg.x;
                         q0 = P[0];
                         if (sg.y == SEG_LINE) {

@@ -452,10 +452,18 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
     s->features.resize((size_t)d->n_paths * N_FEATURES);
     for (int p = 0; p < d->n_paths; p++) path_features(*s, pts, segs, (uint32_t)p, &s->features[(size_t)p * N_FEATURES]);
 
+    // On the device, segments of paths without tile columns carry kind 0: nothing can land
+    // from them (add_fill culls, adjust_alpha_tile_backdrop ignores x outside [0, width)), so
+    // the flattener skips them without looking up their path.
+    std::vector<uint2> segs_dev = segs;
+    for (size_t i = 0; i < segs_dev.size(); i++) {
+        const int4 &r = s->path_rect[seg_path[i]];
+        if (r.z - r.x <= 0) segs_dev[i].y = 0;
+    }
     HIP_TRY(hipSetDevice(ctx->device));
     upload(s->d_features, s->features);
     upload(s->d_pts, pts);
-    upload(s->d_segs, segs);
+    upload(s->d_segs, segs_dev);
     upload(s->d_seg_path, seg_path);
     upload(s->d_path_rect, s->path_rect);
     upload(s->d_path_tile_off, s->path_tile_off);
