@@ -408,14 +408,15 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
             al[q] = fmaf(wq, (float)(px >> 24), al[q]);
         }
     }
+    // v_cvt_pk_u8_f32 rounds to nearest even and saturates to [0, 255] (NaN to 0), i.e. the
+    // byte is rint(clamp(v, 0, 1) * 255), written straight into its place in the word
     uint32_t px[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const uint32_t pr = (uint32_t)rintf(fminf(fmaxf(r[q], 0.0f), 1.0f) * 255.0f);
-        const uint32_t pg = (uint32_t)rintf(fminf(fmaxf(g[q], 0.0f), 1.0f) * 255.0f);
-        const uint32_t pb = (uint32_t)rintf(fminf(fmaxf(bl[q], 0.0f), 1.0f) * 255.0f);
-        const uint32_t pa = (uint32_t)rintf(fminf(fmaxf(al[q], 0.0f), 1.0f) * 255.0f);
-        px[q] = pr | (pg << 8) | (pb << 16) | (pa << 24);
+        uint32_t w = __builtin_amdgcn_cvt_pk_u8_f32(r[q] * 255.0f, 0, 0u);
+        w = __builtin_amdgcn_cvt_pk_u8_f32(g[q] * 255.0f, 1, w);
+        w = __builtin_amdgcn_cvt_pk_u8_f32(bl[q] * 255.0f, 2, w);
+        px[q] = __builtin_amdgcn_cvt_pk_u8_f32(al[q] * 255.0f, 3, w);
     }
 #pragma unroll
     for (int q = 0; q < 4; q++)
