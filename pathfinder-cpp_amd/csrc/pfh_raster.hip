@@ -100,31 +100,44 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
     return v + (row == 0 ? 0u : (row == 1 ? t0 : (row == 2 ? t0 + t1 : t0 + t1 + t2)));
 }
 
+// v_med3_i32: the median of three.
+__device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // Coverage of one pixel from its fixed-point winding (2^-FIX_SHIFT units) under the fill rule
-// (shaders/d3d11/fill.comp:134-140), as an 8-bit value rounded to nearest.
-__device__ __forceinline__ uint32_t coverage_byte(int32_t w, uint32_t even_odd) {
+// (shaders/d3d11/fill.comp:134-140), rounded to nearest 8 bits and left in bits 24..31:
+// (c * 255 + 2^19) >> 20 == (c * 4080 + 2^23) >> 24 for 0 <= c <= 2^20 (fits 32 bits).
+// Nonzero: min(|w|, 1) = med3(w, -w, 1); even-odd: 1 - |1 - (w mod 2)| = min(m, 2 - m).
+__device__ __forceinline__ uint32_t coverage_hi(int32_t w, uint32_t even_odd) {
     const int32_t one = 1 << FIX_SHIFT;
-    const int32_t nz = min(abs(w), one);
+    const int32_t nz = med3_i32(w, -w, one);
     const int32_t m = w & (2 * one - 1);  // w mod 2 (two's complement: also for negative w)
-    const int32_t eo = one - abs(one - m);
+    const int32_t eo = min(m, 2 * one - m);
     const uint32_t c = (uint32_t)(even_odd ? eo : nz);
-    return (c * 255u + (uint32_t)(one >> 1)) >> FIX_SHIFT;
+    return c * 4080u + (1u << 23);
 }
 
 // Prefix of a column's 16 difference values on top of the backdrop, then coverage bytes packed
-// 4 per word (RULE 0 nonzero, 1 even-odd, 2 per lane from `rule`).
+// 4 per word (RULE 0 nonzero, 1 even-odd, 2 per lane from `rule`).  v_perm_b32 gathers the
+// top bytes: selector bytes 0-3 pick the second operand's bytes, 4-7 the first's, 0x0c gives 0.
 template <int RULE>
 __device__ __forceinline__ void resolve_column(int32_t run, const int32_t (&dv)[16], uint32_t rule,
                                                uint32_t (&packed)[4]) {
+    const uint32_t eo = RULE == 2 ? rule : (uint32_t)RULE;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        uint32_t word = 0;
+        uint32_t hi[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             run += dv[4 * k + r];
-            word |= coverage_byte(run, RULE == 2 ? rule : (uint32_t)RULE) << (8 * r);
+            hi[r] = coverage_hi(run, eo);
         }
-        packed[k] = word;
+        const uint32_t lo2 = __builtin_amdgcn_perm(hi[1], hi[0], 0x0c0c0703u);  // bytes 0, 1
+        const uint32_t hi2 = __builtin_amdgcn_perm(hi[3], hi[2], 0x07030c0cu);  // bytes 2, 3
+        packed[k] = lo2 | hi2;
     }
 }
 
