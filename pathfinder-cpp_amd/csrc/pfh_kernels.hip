@@ -226,11 +226,14 @@ struct Walker {
             if (cx == ex && cy == ey) next = 0;
             float2 npos = f2add(from, f2muls(vec, nt));  // LineSegmentF::sample
             add_fill(cur, npos, cx, cy);
-            float2 corner = make_float2((float)cx * 16.0f, (float)cy * 16.0f);
-            if (sty < 0 && next == 2) add_fill(npos, corner, cx, cy);
-            else if (sty > 0 && last == 2) add_fill(corner, cur, cx, cy);
-            if (stx < 0 && last == 1) adjust(cx, cy, 1);
-            else if (stx > 0 && next == 1) adjust(cx, cy, -1);
+            // the auxiliary fill along the tile's left edge: (npos, corner) when stepping up out of
+            // the tile, (corner, cur) when having stepped down into it; one call site for both, so
+            // lanes walking up and down do not run two copies
+            const float2 corner = make_float2((float)cx * 16.0f, (float)cy * 16.0f);
+            const bool up = sty < 0 && next == 2, down = sty > 0 && last == 2;
+            if (up || down) add_fill(up ? npos : corner, up ? corner : cur, cx, cy);
+            // the backdrop step when leaving to the right or having entered from the left
+            if ((stx < 0 && last == 1) || (stx > 0 && next == 1)) adjust(cx, cy, stx < 0 ? 1 : -1);
             if (next == 1) {
                 tmx += tdx;
                 cx += stx;
