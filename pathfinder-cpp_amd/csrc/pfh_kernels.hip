@@ -176,12 +176,16 @@ struct Walker {
             if (of == 0 && ot == 0) return true;
             if ((of & ot) != 0) return false;
             bool clip_from = of > ot;
-            float2 p = clip_from ? from : to;
             uint32_t oc = clip_from ? of : ot;
-            if (oc & 1) p = make_float2(l, lerpc(from.y, to.y, (l - from.x) / (to.x - from.x)));
-            else if (oc & 2) p = make_float2(r, lerpc(from.y, to.y, (r - from.x) / (to.x - from.x)));
-            else if (oc & 4) p = make_float2(lerpc(from.x, to.x, (t - from.y) / (to.y - from.y)), t);
-            else if (oc & 8) p = make_float2(lerpc(from.x, to.x, (bb - from.y) / (to.y - from.y)), bb);
+            // the first edge of the outcode (left, right, top, bottom): a vertical edge fixes x and
+            // interpolates y, a horizontal one the reverse.  The operands are selected first so
+            // that one division serves every case (lanes clipping different edges run it once).
+            const bool vert = (oc & 3u) != 0;
+            const float edge = (oc & 1u) ? l : ((oc & 2u) ? r : ((oc & 4u) ? t : bb));
+            const float a0 = vert ? from.x : from.y, a1 = vert ? to.x : to.y;
+            const float b0 = vert ? from.y : from.x, b1 = vert ? to.y : to.x;
+            const float other = lerpc(b0, b1, (edge - a0) / (a1 - a0));
+            const float2 p = vert ? make_float2(edge, other) : make_float2(other, edge);
             if (clip_from) {
                 from = p;
                 of = outcode(p, l, t, r, bb);
