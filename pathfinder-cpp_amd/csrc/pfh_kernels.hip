@@ -397,19 +397,16 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
                     }
                 }
             } else {
+                // de Casteljau at t = 0.5, one path for both kinds (lanes holding quads and cubics
+                // run it once).  A quad is held as (p0, c, c, p1): p01 = lerp(p0, c) and
+                // p23 = lerp(c, p1) are its qa and qb, and taking them in place of p012 and p123
+                // makes p0123 = lerp(qa, qb), its midpoint: halves (p0, qa, qa, qc), (qc, qb, qb, p1).
                 const float t = 0.5f;
-                Seg4 a, bseg;
-                if (quad) {
-                    float2 qa = lerp2(s.p0, s.c0, t), qb = lerp2(s.c0, s.p1, t), qc = lerp2(qa, qb, t);
-                    a = Seg4{s.p0, qa, qa, qc};
-                    bseg = Seg4{qc, qb, qb, s.p1};
-                } else {
-                    float2 p01 = lerp2(s.p0, s.c0, t), p12 = lerp2(s.c0, s.c1, t), p23 = lerp2(s.c1, s.p1, t);
-                    float2 p012 = lerp2(p01, p12, t), p123 = lerp2(p12, p23, t);
-                    float2 p0123 = lerp2(p012, p123, t);
-                    a = Seg4{s.p0, p01, p012, p0123};
-                    bseg = Seg4{p0123, p123, p23, s.p1};
-                }
+                const float2 p01 = lerp2(s.p0, s.c0, t), p12 = lerp2(s.c0, s.c1, t), p23 = lerp2(s.c1, s.p1, t);
+                const float2 p012 = quad ? p01 : lerp2(p01, p12, t), p123 = quad ? p23 : lerp2(p12, p23, t);
+                const float2 p0123 = lerp2(p012, p123, t);
+                const Seg4 a{s.p0, p01, p012, p0123};
+                const Seg4 bseg{p0123, p123, p23, s.p1};
                 if (sp < FLAT_LDS_LEVELS) {
                     fs[sp][0][tid] = bseg.c0.x;
                     fs[sp][1][tid] = bseg.c0.y;
