@@ -52,8 +52,9 @@ WORKLOADS = {
 
 def fill_bytes(st):
     # 12 B per fill read (reference Fill layout), 8 B alpha-tile index + 4 B backdrop/ctrl
-    # per alpha tile, 256 B of u8 coverage written per 16x16 alpha tile.
-    return 12 * st["fills"] + 8 * st["alpha_tiles"] + 4 * st["alpha_tiles"] + 256 * st["alpha_tiles"]
+    # per alpha tile, 256 B of u8 coverage written per 16x16 alpha tile; over the alpha tiles
+    # the fill kernel rasterizes (those draw entries sample) and their fills.
+    return 12 * st["fills_masked"] + 8 * st["alpha_masked"] + 4 * st["alpha_masked"] + 256 * st["alpha_masked"]
 
 
 def composite_bytes(st, pixels, n_paths):
@@ -231,7 +232,8 @@ def main():
         "batched_over_plain": round(plain_s / batched_s, 3),
         "plain_stage_ms": {k: round(v, 4) for k, v in zip(names, stage_ms)},
         "counts": {k: st[k] for k in ("segments", "lines", "tiles", "fills", "alpha_tiles", "entries",
-                                      "alpha_entries", "fb_tiles", "retries", "host_syncs")},
+                                      "alpha_entries", "fills_masked", "alpha_masked", "fb_tiles", "retries",
+                                      "host_syncs")},
         "roofline": roofline(st, fill_ms, comp_ms, pixels, n_paths),
         "autotune": tuning,
     }

@@ -80,6 +80,8 @@ typedef struct pfh_stats {
     int64_t alpha_entries;    /* draw entries that sample a coverage mask */
     int64_t lines, capacity_lines; /* flattened chords and the chord buffer's capacity */
     int64_t host_syncs;       /* device->host round trips during the last pfh_rasterize */
+    int64_t fills_masked, alpha_masked; /* fills and alpha tiles the coverage pass rasterized: those of
+                                           alpha tiles that draw entries sample (all of them with PFH_DUMP) */
 } pfh_stats;
 
 /* Context on HIP device `device`.  Replaces Renderer's Device/Queue pair (core/renderer.cpp:13-37). */

@@ -85,8 +85,11 @@ enum Counter {
     C_GUARD = 4,       // loop guards tripped
     C_ALPHA_ENTRIES = 5,  // draw entries that sample a mask
     C_LINES = 6,       // flattened chords (lines) in the batch
-    C_COUNT = 8
+    C_DRAWN = 8,       // [C_DRAWN, C_DRAWN + DRAWN_SLOTS): fills the fill kernel read, spread over
+                       // slots (per block, by block index) so that no single address takes every add
+    C_COUNT = 8 + 64
 };
+constexpr int DRAWN_SLOTS = 64;
 
 // One scene of a batch as the kernels see it.  Scene-local arrays are resident
 // from upload; *_base fields place the scene inside the batch's work arrays.
@@ -157,6 +160,7 @@ struct BatchDev {
     uint2 *fills;                 // [F] grouped by tile: (x0 | y0 << 16, x1 | y1 << 16)
     uint4 *dump_fills;            // [F] emission order: (x0|y0<<16, x1|y1<<16, global tile, 0)
     uint4 *alpha_rec;             // [A] (fill begin, fill end, backdrop | rule << 8, tile)
+    uint32_t *alpha_vis;          // [A] alpha indices of the alpha entries (tiles that survive the z-cull)
     uint8_t *masks;               // [A * 256]
     uint32_t *zbuf;               // [FB]
     uint2 *fb_range;              // [FB] [start, end) of each framebuffer tile's entries in sort order

@@ -1037,10 +1037,14 @@ __device__ __forceinline__ EntryTile entry_tile(const BatchDev &b, const SceneDe
     return et;
 }
 
+// e: the entry's index; ea: its index among alpha entries (for an alpha tile).  The alpha
+// entries' tiles are listed in that order for the fill kernel, which rasterizes only them.
 __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t rgba,
-                                            uint32_t tbase, uint32_t j, const EntryTile &et, uint32_t e) {
+                                            uint32_t tbase, uint32_t j, const EntryTile &et, uint32_t e,
+                                            uint32_t ea) {
     if (e >= b.cap_entries) return;  // past the capacity: the batch is retried
     const uint32_t ai = et.st == TILE_ALPHA ? b.tile_alpha_idx[tbase + j] : NO_ALPHA;
+    if (ai != NO_ALPHA && ea < b.cap_alpha) b.alpha_vis[ea] = ai;
     // even-odd solid tile with an even winding covers nothing: zero alpha
     const uint32_t eff = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
     b.keys[e] = sc.fb_base + et.fbl;  // its value, e, is generated by the sort's first pass
@@ -1061,35 +1065,45 @@ __device__ __forceinline__ void path_entries(const BatchDev &b, const SceneDev &
     const int fb_row0 = (r.y - sc.fb_y0) * sc.fb_w + (r.x - sc.fb_x0);
     const uint32_t rgba = it.rgba;
     const float inv_w = w > 0 ? 1.0f / (float)w : 0.0f;
-    uint32_t e = WRITE ? b.path_ent[gp].x : 0u, n_alpha = 0;  // count pass: this wave's counts
-    __shared__ uint32_t wave_tot[2][4];  // block version, write pass: per-wave counts, double-buffered
+    // write pass: the path's first entry and first alpha entry; count pass: this wave's counts
+    const uint2 first = WRITE ? b.path_ent[gp] : make_uint2(0u, 0u);
+    uint32_t e = first.x, n_alpha = first.y;
+    __shared__ uint2 wave_tot[2][4];  // block version, write pass: per-wave counts, double-buffered
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t parity = 0;
     for (uint32_t j0 = 0; j0 < area; j0 += THREADS) {  // uniform over the wave / block
         const uint32_t j = j0 + t;
         EntryTile et{false, TILE_EMPTY, 0};
         if (j < area) et = entry_tile(b, sc, p, tbase, w, inv_w, fb_row0, j);
-        const uint64_t m = __ballot(et.keep);
+        const uint64_t m = __ballot(et.keep), ma = __ballot(et.keep && et.st == TILE_ALPHA);
         if (!WRITE) {
             e += (uint32_t)__popcll(m);
-            n_alpha += (uint32_t)__popcll(__ballot(et.keep && et.st == TILE_ALPHA));
+            n_alpha += (uint32_t)__popcll(ma);
         } else if (THREADS == 64) {
-            if (et.keep) write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask));
+            if (et.keep)
+                write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
+                            n_alpha + (uint32_t)__popcll(ma & lt_mask));
             e += (uint32_t)__popcll(m);
+            n_alpha += (uint32_t)__popcll(ma);
         } else {
             // entries before this wave's in the chunk: the earlier waves' counts.  A slot is
             // rewritten two chunks later, after a barrier every wave passes once done reading it.
-            if (lane == 0) wave_tot[parity][wv] = (uint32_t)__popcll(m);
+            if (lane == 0) wave_tot[parity][wv] = make_uint2((uint32_t)__popcll(m), (uint32_t)__popcll(ma));
             __syncthreads();
-            uint32_t before = 0, total = 0;
+            uint32_t before = 0, total = 0, before_a = 0, total_a = 0;
 #pragma unroll
             for (uint32_t v = 0; v < 4; v++) {
-                const uint32_t c = wave_tot[parity][v];
-                before += v < wv ? c : 0u;
-                total += c;
+                const uint2 c = wave_tot[parity][v];
+                before += v < wv ? c.x : 0u;
+                before_a += v < wv ? c.y : 0u;
+                total += c.x;
+                total_a += c.y;
             }
-            if (et.keep) write_entry(b, sc, p, rgba, tbase, j, et, e + before + (uint32_t)__popcll(m & lt_mask));
+            if (et.keep)
+                write_entry(b, sc, p, rgba, tbase, j, et, e + before + (uint32_t)__popcll(m & lt_mask),
+                            n_alpha + before_a + (uint32_t)__popcll(ma & lt_mask));
             e += total;
+            n_alpha += total_a;
             parity ^= 1u;
         }
     }

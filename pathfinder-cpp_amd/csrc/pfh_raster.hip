@@ -144,7 +144,10 @@ __device__ __forceinline__ void resolve_column(int32_t run, const int32_t (&dv)[
 // A wave takes 4 consecutive alpha tiles (16 lanes, one pixel column each, per tile).  Their
 // fills form one pooled list of up to 64 per chunk; the (fill, pixel column) pairs of a
 // chunk are enumerated with a wave-wide prefix sum, so every lane works on a pair that
-// contributes whatever the mix of light and heavy tiles in the wave.
+// contributes whatever the mix of light and heavy tiles in the wave.  The tiles are those
+// some draw entry samples, in the list the entry pass wrote: a mask hidden under an opaque
+// solid tile of a later path is never read, so it is not computed.  With dumps every alpha
+// tile is rasterized, in index order.
 __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     __shared__ __align__(16) int32_t diff[FILL_TILES_PER_BLOCK * FILL_GROUP_WORDS];
     __shared__ uint2 sfill[4][64];
@@ -154,17 +157,31 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     const uint32_t slot = lane >> 4, li = lane & 15;  // tile slot in the wave, pixel column
     int32_t *dwave = diff + wv * 4 * FILL_GROUP_WORDS;
     int32_t *dcol = dwave + slot * FILL_GROUP_WORDS + li * FILL_COL_WORDS;
-    const uint32_t na = min(b.counters[C_ALPHA], b.cap_alpha);
+    const bool listed = !b.dump;
+    const uint32_t na = min(b.counters[listed ? C_ALPHA_ENTRIES : C_ALPHA], b.cap_alpha);
     const float scale = (float)(1 << FIX_SHIFT);
+    uint32_t drawn = 0;  // fills this wave read (wave-uniform)
     // grid-stride over groups of 16 tiles: the grid is sized from the capacity, the count
     // is only known on the device; the loop bound is uniform over the block
     const uint32_t lb = (b.xcd_order & 1u) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    for (uint32_t first = lb * FILL_TILES_PER_BLOCK; first < na; first += gridDim.x * FILL_TILES_PER_BLOCK) {
-        const uint32_t a = first + wv * 4 + slot;
-        const bool valid = a < na;
+    const uint32_t stride = gridDim.x * FILL_TILES_PER_BLOCK, first0 = lb * FILL_TILES_PER_BLOCK;
+    const uint32_t mine = wv * 4 + slot;  // this lane's tile within a group
+    // The tile list and the tile records are read ahead (the record one group ahead, the list
+    // entry two), so a group starts with its record in registers and waits only for its fills.
+    // ~0u: no tile.
+    auto tile_at = [&](uint32_t idx) { return idx >= na ? ~0u : (listed ? b.alpha_vis[idx] : idx); };
+    auto rec_of = [&](uint32_t a) { return a < b.cap_alpha ? b.alpha_rec[a] : make_uint4(0, 0, 0, 0); };
+    uint32_t a_next = tile_at(first0 + mine), a_after = tile_at(first0 + stride + mine);
+    uint4 rec_next = rec_of(a_next);
+    for (uint32_t first = first0; first < na; first += stride) {
+        const uint32_t a = a_next;
+        const uint4 rec = rec_next;
+        const bool valid = a < b.cap_alpha;
+        rec_next = rec_of(a_after);
+        a_next = a_after;
+        a_after = tile_at(first + 2 * stride + mine);
 #pragma unroll
         for (int k = 0; k < FILL_COL_WORDS / 4; k++) reinterpret_cast<uint4 *>(dcol)[k] = make_uint4(0, 0, 0, 0);
-        const uint4 rec = valid ? b.alpha_rec[a] : make_uint4(0, 0, 0, 0);
         // After a fill-capacity overflow a tile's range can start past the buffer: clamp both
         // ends so no count wraps (the batch is then retried).
         const uint32_t f0 = min(rec.x, b.cap_fills);
@@ -175,6 +192,7 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
         const uint32_t s2 = __builtin_amdgcn_readlane(f0, 32), n2 = __builtin_amdgcn_readlane(f1, 32) - s2;
         const uint32_t s3 = __builtin_amdgcn_readlane(f0, 48), n3 = __builtin_amdgcn_readlane(f1, 48) - s3;
         const uint32_t c1 = n0, c2 = c1 + n1, c3 = c2 + n2, ntot = c3 + n3;
+        drawn += ntot;
         for (uint32_t base = 0; base < ntot; base += 64) {
             const uint32_t k = base + lane;
             uint2 fl = make_uint2(0, 0);
@@ -244,6 +262,12 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    // one add per block, spread over slots by block index (the loop above is uniform over the block)
+    __shared__ uint32_t sdrawn[4];
+    if (lane == 0) sdrawn[wv] = drawn;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicAdd(&b.counters[C_DRAWN + (blockIdx.x & (DRAWN_SLOTS - 1))], sdrawn[0] + sdrawn[1] + sdrawn[2] + sdrawn[3]);
 }
 
 // ---------------------------------------------------------------------------

@@ -160,7 +160,7 @@ struct Workspace {
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, path_tot, bin_items, col_item, col_bd,
-        seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, masks, zbuf, fb_range, keys, vals, keys_alt,
+        seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, alpha_vis, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp, ones_mask;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
@@ -184,7 +184,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &path_ent, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec,
+                &path_ent, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &entry_meta, &counters, &scan_tmp,
                 &sort_tmp, &ones_mask};
     }
@@ -559,6 +559,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.fills.ensure(8ull * pl.cap_f);
     if (dump) w.dump_fills.ensure(16ull * pl.cap_f);
     w.alpha_rec.ensure(16ull * pl.cap_a);
+    w.alpha_vis.ensure(4ull * pl.cap_a);
     w.masks.ensure(256ull * pl.cap_a);
     w.keys.ensure(4ull * pl.cap_e);
     w.vals.ensure(4ull * pl.cap_e);
@@ -581,7 +582,7 @@ static Caps held_caps(Workspace &w, bool dump) {
     uint64_t l = std::min({fit(w.lines.bytes, 16, 0), fit(w.line_cnt.bytes, 4, 8)});
     uint64_t f = fit(w.fills.bytes, 8, 0);
     if (dump) f = std::min(f, fit(w.dump_fills.bytes, 16, 0));
-    uint64_t a = std::min(fit(w.alpha_rec.bytes, 16, 0), fit(w.masks.bytes, 256, 0));
+    uint64_t a = std::min({fit(w.alpha_rec.bytes, 16, 0), fit(w.alpha_vis.bytes, 4, 0), fit(w.masks.bytes, 256, 0)});
     return Caps{clamp_cap(l), clamp_cap(f), clamp_cap(a)};
 }
 
@@ -734,6 +735,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.fills = w.fills.as<uint2>();
     b.dump_fills = dump ? w.dump_fills.as<uint4>() : nullptr;
     b.alpha_rec = w.alpha_rec.as<uint4>();
+    b.alpha_vis = w.alpha_vis.as<uint32_t>();
     b.masks = w.masks.as<uint8_t>();
     b.zbuf = w.zbuf.as<uint32_t>();
     b.fb_range = w.fb_range.as<uint2>();
@@ -812,8 +814,13 @@ static double elapsed_ms(pfh_ctx *ctx, hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-static void record_counts(pfh_ctx *ctx, const Plan &pl) {
+static void record_counts(pfh_ctx *ctx, const Plan &pl, bool dump) {
     ctx->stats.fills = ctx->w->h_counters[C_FILLS];
+    int64_t drawn = 0;
+    for (int i = 0; i < DRAWN_SLOTS; i++) drawn += ctx->w->h_counters[C_DRAWN + i];
+    ctx->stats.fills_masked = drawn;
+    // the fill kernel rasterizes the alpha tiles of draw entries (every alpha tile with dumps)
+    ctx->stats.alpha_masked = std::min<int64_t>(ctx->w->h_counters[dump ? C_ALPHA : C_ALPHA_ENTRIES], pl.cap_a);
     ctx->stats.alpha_tiles = ctx->w->h_counters[C_ALPHA];
     ctx->stats.entries = ctx->w->h_counters[C_ENTRIES];
     ctx->stats.guard_hits = ctx->w->h_counters[C_GUARD];
@@ -896,7 +903,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     read_counters(ctx);
     for (int i = 0; i < 8; i++) ctx->stats.ms_stage[i] += elapsed_ms(ctx, ev[i], ev[i + 1]);
     for (auto &e : ev) (void)hipEventDestroy(e);
-    record_counts(ctx, pl);
+    record_counts(ctx, pl, dump);
     SceneHost &s = get_scene(ctx, id);
     s.counted = true;
     s.last_lines = nl;
@@ -1067,7 +1074,7 @@ static void batched_accept(pfh_ctx *ctx, BatchedLaunch &bl) {
     for (double &m : ctx->stats.ms_stage) m = 0;
     ctx->stats.ms_stage[6] = elapsed_ms(ctx, w.kev[0], w.kev[1]);
     ctx->stats.ms_stage[7] = elapsed_ms(ctx, w.kev[1], w.kev[2]);
-    record_counts(ctx, bl.pl);
+    record_counts(ctx, bl.pl, bl.dump);
     for (int32_t id : bl.pl.ids) {
         SceneHost &s = get_scene(ctx, id);
         s.dumped = bl.dump;
@@ -1149,7 +1156,7 @@ static uint64_t scene_launch_bytes(const pfh_ctx *ctx, const SceneHost &s) {
     }
     const double T = s.n_tiles, W = s.n_cols, S = s.n_segs, FB = (double)s.fb_w * s.fb_h;
     const double per_tile = 4 + 4 + 8 + 1 + 4 * 4 + 8;  // counts, offsets, status, sort, records
-    double b = T * per_tile + 4 * W + 16 * s.n_paths + 8 * S + 12 * FB + 24 * L + 8 * F + (16 + 256) * A;
+    double b = T * per_tile + 4 * W + 16 * s.n_paths + 8 * S + 12 * FB + 24 * L + 8 * F + (16 + 4 + 256) * A;
     return (uint64_t)b + 4096;
 }
 

@@ -43,7 +43,8 @@ class Stats(ctypes.Structure):
                 ("guard_hits", ctypes.c_int64), ("ms_total", ctypes.c_double), ("ms_stage", ctypes.c_double * 8),
                 ("capacity_fills", ctypes.c_int64), ("capacity_alpha", ctypes.c_int64),
                 ("capacity_entries", ctypes.c_int64), ("alpha_entries", ctypes.c_int64), ("lines", ctypes.c_int64),
-                ("capacity_lines", ctypes.c_int64), ("host_syncs", ctypes.c_int64)]
+                ("capacity_lines", ctypes.c_int64), ("host_syncs", ctypes.c_int64),
+                ("fills_masked", ctypes.c_int64), ("alpha_masked", ctypes.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "ms_stage"}

@@ -97,6 +97,55 @@ def test_batched_equals_plain(ctx):
         ctx.release(sid)
 
 
+def sampled_alpha(model) -> tuple:
+    """(alpha tiles that draw entries sample, their fills) of the oracle's model."""
+    at = model["alpha"]
+    ai = at["alpha_index"][model["canon"]["entry_tile"]]
+    ai = ai[ai >= 0]
+    return int(ai.shape[0]), int((at["fill_off"][ai + 1] - at["fill_off"][ai]).sum())
+
+
+def test_fill_rasterizes_only_sampled_alpha_tiles(ctx):
+    """Without PFH_DUMP the coverage pass rasterizes only the alpha tiles some draw entry
+    samples (a mask under an opaque solid tile of a later path is never read): its counts
+    match the oracle's entries, and the pixels equal the dump run's, which rasterizes every
+    alpha tile, and the oracle's."""
+    scenes = [scenegen.cubic_scene(41, size=512, n_paths=64),
+              scenegen.blob_scene(42, 1024, 1024, 1500, alpha_min=0.95),
+              scenegen.blob_scene(43, 768, 768, 600, alpha_min=0.95)]
+    models = [oracle.full_model(s, zmode=1) for s in scenes]
+    sampled = [sampled_alpha(m) for m in models]
+    n_alpha = sum(int(m["alpha"]["tile"].shape[0]) for m in models)
+    n_fills = sum(int(m["canon"]["fill_seg"].shape[0]) for m in models)
+    assert sum(a for a, _ in sampled) < n_alpha  # the scenes do hide some alpha tiles
+    ids = [ctx.upload(s) for s in scenes]
+    for flags in (pfh.PFH_PLAIN, pfh.PFH_BATCHED | pfh.PFH_NO_GRAPH, pfh.PFH_BATCHED):
+        print(f"[sampled alpha] flags={flags}", flush=True)
+        runs = [[sid] for sid in ids] if flags == pfh.PFH_PLAIN else [ids]
+        listed_imgs = {}  # the run without dumps, which rasterizes the sampled tiles only
+        for dump in (False, True):
+            got_a = got_f = 0
+            for run in runs:
+                ctx.rasterize(run, flags | (pfh.PFH_DUMP if dump else 0))
+                st = ctx.stats()
+                got_a += st["alpha_masked"]
+                got_f += st["fills_masked"]
+            if dump:
+                assert (got_a, got_f) == (n_alpha, n_fills)
+            else:
+                assert (got_a, got_f) == (sum(a for a, _ in sampled), sum(f for _, f in sampled))
+            for sid, m in zip(ids, models):
+                img = ctx.readback(sid)
+                want = np.rint(np.clip(m["image"], 0.0, 1.0) * 255.0).astype(np.int64)
+                assert np.abs(img.astype(np.int64) - want).max() <= RGBA_TOL
+                if dump:
+                    np.testing.assert_array_equal(img, listed_imgs[sid])
+                else:
+                    listed_imgs[sid] = img
+    for sid in ids:
+        ctx.release(sid)
+
+
 def test_undersized_prediction_retries_without_wrong_pixels(ctx):
     scenes = scenegen.batch_scenes(n_scenes=3, seed=321, side_max=512, paths_max=300)
     ids = [ctx.upload(s) for s in scenes]
