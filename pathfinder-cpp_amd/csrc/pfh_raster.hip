@@ -312,6 +312,8 @@ __device__ __forceinline__ float lane_f(float v, int j) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
+constexpr int EV_GROUP = 4;  // blend events whose mask words are loaded together
+
 __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *sorted_vals) {
     const int lane = threadIdx.x & 63;
     const uint32_t lb = (b.xcd_order & 2u) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -388,9 +390,11 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         const uint64_t alpha_mask = valid_mask & ~solid_mask;
         const uint64_t run_end = solid_mask & ~(solid_mask >> 1);
         uint64_t events = alpha_mask | run_end;
-        // this lane's mask word for event j (ai is NO_ALPHA on solid lanes)
+        // this lane's mask word for event j (ai is NO_ALPHA on solid lanes; j < 0, an empty
+        // slot, reads the all-ones row: every slot loads, so no branch splits the loads)
         auto mask_word = [&](int j) -> uint32_t {
-            const uint32_t idx = __builtin_amdgcn_readlane(ai, j);
+            const uint32_t ij = __builtin_amdgcn_readlane(ai, max(j, 0));
+            const uint32_t idx = j >= 0 ? ij : NO_ALPHA;
             const uint32_t *row_words = idx == NO_ALPHA ? b.ones_mask : reinterpret_cast<const uint32_t *>(b.masks) + (uint64_t)idx * 64;
             return row_words[lane];
         };
@@ -405,25 +409,38 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
                 tr[q] = fmaf(-wq, fa, tr[q]);
             }
         };
-        // The next event's mask word is loaded while the current one blends.  Two registers
-        // take turns (the loop is unrolled by two) so that no copy waits for a load, and the
-        // body has no exit between its halves, so the loads stay ahead of the blends: an odd
-        // count ends with a blend of coverage 0, which changes nothing.
-        uint32_t ma = events ? mask_word(63 - __clzll(events)) : 0u, mb = 0;
-        while (events) {
-            const int j = 63 - __clzll(events);
-            events &= ~(1ull << j);
-            const bool has_next = events != 0;
-            const int jn = has_next ? 63 - __clzll(events) : j;
-            mb = mask_word(jn);
-            __builtin_amdgcn_sched_barrier(0);  // keep each load ahead of the blend it overlaps
-            blend(j, ma);
-            events &= ~(1ull << jn);
+        // Events go in groups of EV_GROUP from the top: the next group's mask words are loaded
+        // (EV_GROUP loads in flight) while the current group blends, so a wave waits on memory
+        // once per group.  Two register sets take turns (the loop is unrolled by two), so no
+        // copy waits for a load, and scheduling barriers keep the loads ahead of the blends.
+        // An event index < 0 is an empty slot: no blend.
+        auto take = [&](int (&js)[EV_GROUP], uint32_t (&ms)[EV_GROUP]) {
+#pragma unroll
+            for (int d = 0; d < EV_GROUP; d++) {
+                js[d] = events ? 63 - __clzll(events) : -1;
+                events &= events ? ~(1ull << js[d]) : 0ull;
+                ms[d] = mask_word(js[d]);
+            }
+        };
+        auto blend_group = [&](const int (&js)[EV_GROUP], const uint32_t (&ms)[EV_GROUP]) {
+#pragma unroll
+            for (int d = 0; d < EV_GROUP; d++)
+                if (js[d] >= 0) blend(js[d], ms[d]);
+        };
+        int ja[EV_GROUP], jb[EV_GROUP];
+        uint32_t ma[EV_GROUP], mb[EV_GROUP];
+        take(ja, ma);
+        while (true) {
+            take(jb, mb);
             __builtin_amdgcn_sched_barrier(0);
-            ma = mask_word(events ? 63 - __clzll(events) : jn);
+            blend_group(ja, ma);
             __builtin_amdgcn_sched_barrier(0);
-            blend(jn, has_next ? mb : 0u);
+            if (jb[0] < 0) break;
+            take(ja, ma);
             __builtin_amdgcn_sched_barrier(0);
+            blend_group(jb, mb);
+            __builtin_amdgcn_sched_barrier(0);
+            if (ja[0] < 0) break;
         }
         if (base == e0) break;
     }
