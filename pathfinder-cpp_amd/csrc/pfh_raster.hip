@@ -308,13 +308,11 @@ __device__ __forceinline__ void over_step(bool has_src, float &pr, float &pg, fl
     if (has_src) flag = flag || qf;
 }
 
-__device__ __forceinline__ float lane_f(float v, int j) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
 
-constexpr int EV_GROUP = 4;  // blend events whose mask words are loaded together
+constexpr int EV_GROUP = 3;  // blend events whose mask words are loaded together
 
 __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *sorted_vals) {
+    __shared__ float4 s_colour[4][64];  // per wave of the block: the chunk's premultiplied colours
     const int lane = threadIdx.x & 63;
     const uint32_t lb = (b.xcd_order & 2u) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t f = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -398,8 +396,18 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
             const uint32_t *row_words = idx == NO_ALPHA ? b.ones_mask : reinterpret_cast<const uint32_t *>(b.masks) + (uint64_t)idx * 64;
             return row_words[lane];
         };
-        auto blend = [&](int j, uint32_t m) {
-            const float fr = lane_f(pr, j), fg = lane_f(pg, j), fb = lane_f(pb, j), fa = lane_f(pa, j);
+        // The chunk's event colours go through LDS: a blend reads its event's colour with one
+        // broadcast LDS read instead of four lane reads, which would occupy the VALU.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        s_colour[threadIdx.x >> 6][lane] = make_float4(pr, pg, pb, pa);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const float4 *wcol = s_colour[threadIdx.x >> 6];
+        auto blend = [&](float4 cf, uint32_t m) {
+            const float fr = cf.x, fg = cf.y, fb = cf.z, fa = cf.w;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const float wq = tr[q] * (float)((m >> (8 * q)) & 255u);
@@ -423,9 +431,12 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
             }
         };
         auto blend_group = [&](const int (&js)[EV_GROUP], const uint32_t (&ms)[EV_GROUP]) {
+            float4 cf[EV_GROUP];
+#pragma unroll
+            for (int d = 0; d < EV_GROUP; d++) cf[d] = wcol[max(js[d], 0)];
 #pragma unroll
             for (int d = 0; d < EV_GROUP; d++)
-                if (js[d] >= 0) blend(js[d], ms[d]);
+                if (js[d] >= 0) blend(cf[d], ms[d]);
         };
         int ja[EV_GROUP], jb[EV_GROUP];
         uint32_t ma[EV_GROUP], mb[EV_GROUP];
