@@ -52,43 +52,50 @@ __device__ __forceinline__ uint32_t row16_inclusive_sum(uint32_t v) {
 // above ymax, w (t - ymin)^2 / (2 (ymax - ymin)) between), the covered area of pixel row r is
 // F(r) = ∫ clamp(r + 1 - y(x), 0, 1) dx = G(r + 1) - G(r).  The array holds F's differences
 // down the column in fixed point: rows r0 = floor(ymin) .. r1 = floor(ymax) are crossed, rows
-// below get the full w.  Inside the crossed span F grows by the same 2 w / (2 (ymax - ymin))
-// from row to row (G is quadratic there), so only the first two rows, the last one and the
-// full width are evaluated, and the rows between add one constant: a pair costs the same
-// whatever its height.  The running sum is exact at r0, r0 + 1, r1 and below.
-__device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, int32_t *diff) {
+// below get the full w.  Which form of G applies at each row boundary follows from n = r1 - r0
+// alone (r0 + 1 is past ymax only when n = 0, r0 + 2 only when n <= 1, r1 + 1 always), so no
+// lane branches on it.  Inside the crossed span F grows by the same 2 k2 from row to row (G is
+// quadratic there), so F(r0), F(r0 + 1), F(r1) and the full width are evaluated and the rows
+// between add one constant: a pair costs the same whatever its height.  Values are rounded to
+// the fixed-point grid once each and the differences are taken in integers, so the running sum
+// is exact at r0, r0 + 1, r1 and below.
+// v_cvt_rpi_i32_f32: floor(v + 0.5) as an integer, in one instruction.
+__device__ __forceinline__ int32_t round_fix(float v) {
+    int32_t r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+__device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, int32_t *tile_diff) {
     const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), y0 = (float)(fl.x >> 16) * (1.0f / 256.0f);
     const float x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f), y1 = (float)(fl.y >> 16) * (1.0f / 256.0f);
     const float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
     const float lo = fmaxf((float)col, xl), hi = fminf((float)(col + 1), xr);
-    if (!(hi > lo)) return;
+    if (!(hi > lo) || (uint32_t)col > 15u) return;
     const float ks = x0 > x1 ? scale : -scale;  // sign of the fill, in fixed-point units
     const float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
     const float slope = (yr - yl) * __builtin_amdgcn_rcpf(xr - xl);
     const float ya = yl + (lo - xl) * slope, yb = yl + (hi - xl) * slope;
-    const float w = hi - lo;
     const float ymin = fminf(ya, yb), ymax = fmaxf(ya, yb);
-    const float half_sum = 0.5f * (ymin + ymax);
-    const float k2 = w * 0.5f * __builtin_amdgcn_rcpf(ymax - ymin);  // used only when ymin < t < ymax
-    const int r0 = max(0, min(15, (int)floorf(ymin)));
-    const int r1 = max(0, min(15, (int)floorf(ymax)));
-    const int n = r1 - r0;
-    auto G = [&](int ti) {  // G(0 .. r0) = 0: r0 <= ymin
-        const float t = (float)ti, a = t - ymin;
-        return t >= ymax ? w * (t - half_sum) : (a > 0.0f ? k2 * a * a : 0.0f);
-    };
-    const float g1 = G(r0 + 1), g2 = G(r0 + 2), ga = G(r1), gb = G(r1 + 1);
-    const int32_t full = (int32_t)rintf(ks * w);
-    const int32_t v0 = (int32_t)rintf(ks * g1);          // F(r0)
-    const int32_t v1 = (int32_t)rintf(ks * (g2 - g1));   // F(r0 + 1), when n >= 1
-    const int32_t vd = (int32_t)rintf(ks * (gb - ga));   // F(r1), when n >= 2
-    const int32_t dm = (int32_t)rintf(ks * (2.0f * k2));  // F(r) - F(r - 1) for r0 + 2 <= r <= r1 - 1
-    int32_t *cd = diff + col * FILL_COL_WORDS;
+    const float ws = ks * (hi - lo);  // the full width, signed, in fixed-point units
+    const float fr0 = __builtin_amdgcn_fmed3f(floorf(ymin), 0.0f, 15.0f);
+    const float fr1 = __builtin_amdgcn_fmed3f(floorf(ymax), 0.0f, 15.0f);
+    const int r0 = (int)fr0, r1 = (int)fr1, n = r1 - r0;
+    // k2 is infinite when ymin == ymax; then n == 0 and only the linear form is selected
+    const float k2 = 0.5f * ws * __builtin_amdgcn_rcpf(ymax - ymin);
+    const float u0 = (fr0 - ymin) + 1.0f, u1 = fr1 - ymin;
+    const float gb = ws * ((fr1 + 1.0f) - 0.5f * (ymin + ymax));  // G(r1 + 1): linear form
+    const float f0 = n == 0 ? gb : k2 * u0 * u0;                    // F(r0) = G(r0 + 1)
+    const float fd = gb - k2 * u1 * u1;                              // F(r1), when n >= 1
+    const float f1 = n == 1 ? fd : k2 * (2.0f * u0 + 1.0f);          // F(r0 + 1), when n >= 1
+    const int32_t full = round_fix(ws), v0 = round_fix(f0), v1 = round_fix(f1), vd = round_fix(fd);
+    const int32_t dm = round_fix(2.0f * k2);  // F(r) - F(r - 1) for r0 + 2 <= r <= r1 - 1
+    int32_t *cd = tile_diff + __umul24((uint32_t)col, (uint32_t)FILL_COL_WORDS);
     atomicAdd(cd + r0, v0);
     atomicAdd(cd + r0 + 1, (n == 0 ? full : v1) - v0);
     for (int r = r0 + 2; r < r1; r++) atomicAdd(cd + r, dm);
     if (n >= 2) atomicAdd(cd + r1, vd - (v1 + (n - 2) * dm));
-    if (n >= 1) atomicAdd(cd + r1 + 1, full - (n == 1 ? v1 : vd));
+    if (n >= 1) atomicAdd(cd + r1 + 1, full - vd);  // n == 1: v1 == vd
 }
 
 // Inclusive prefix sum across the 64-lane wave: DPP within rows, then the row totals.
@@ -150,9 +157,10 @@ __device__ __forceinline__ void resolve_column(int32_t run, const int32_t (&dv)[
 // tile is rasterized, in index order.
 __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     __shared__ __align__(16) int32_t diff[FILL_TILES_PER_BLOCK * FILL_GROUP_WORDS];
-    __shared__ uint2 sfill[4][64];
-    __shared__ uint32_t scum[4][64];
-    __shared__ uint32_t sslot[4][64];
+    // a chunk's fills: (fill x word, fill y word, first pair, tile offset in the wave's LDS << 4
+    // | first pixel column); a pair window's marks: 1 where a fill's pairs start
+    __shared__ uint4 sfill[4][64];
+    __shared__ uint8_t smark[4][64];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t slot = lane >> 4, li = lane & 15;  // tile slot in the wave, pixel column
     int32_t *dwave = diff + wv * 4 * FILL_GROUP_WORDS;
@@ -196,7 +204,7 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
         for (uint32_t base = 0; base < ntot; base += 64) {
             const uint32_t k = base + lane;
             uint2 fl = make_uint2(0, 0);
-            uint32_t cnt = 0, fs = 0;
+            uint32_t cnt = 0, fs = 0, c0 = 0;
             if (k < ntot) {
                 fs = (k >= c1 ? 1u : 0u) + (k >= c2 ? 1u : 0u) + (k >= c3 ? 1u : 0u);
                 const uint32_t addr = fs == 0 ? s0 + k : (fs == 1 ? s1 + (k - c1) : (fs == 2 ? s2 + (k - c2) : s3 + (k - c3)));
@@ -204,34 +212,44 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
                 const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f);
                 const int cl = (int)floorf(fminf(x0, x1)), cr = (int)ceilf(fmaxf(x0, x1)) - 1;
                 cnt = (uint32_t)max(0, cr - cl + 1);
+                c0 = (uint32_t)cl & 15u;  // 8.8 fills lie in [0, 16): cl <= 15
             }
             const uint32_t incl = wave_inclusive_sum(cnt);
             const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-            sfill[wv][lane] = fl;
-            scum[wv][lane] = incl;
-            sslot[wv][lane] = fs;
-            // the zeroed columns (first chunk) and this chunk's fills are visible to the wave
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t start = incl - cnt;  // this fill's first pair
+            sfill[wv][lane] = make_uint4(fl.x, fl.y, start, ((fs * FILL_GROUP_WORDS) << 4) | c0);
+            // Pairs go in windows of 64, one per lane.  The fill owning pair q is the last one
+            // starting at or before q: marks at the window's fill starts, their ballot, and the
+            // count of marks up to this lane (plus the fills that started in earlier windows)
+            // give it.  Every fill covers at least one column (the bin stage drops fills with
+            // equal x ends), so no two fills start at the same pair.
+            uint32_t carry = 0;
             for (uint32_t q0 = 0; q0 < total; q0 += 64) {
+                smark[wv][lane] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (cnt != 0 && start >= q0 && start - q0 < 64u) smark[wv][start - q0] = 1;
+                // the marks, the chunk's fills and (first chunk) the zeroed columns are visible
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const uint64_t marks = __ballot(smark[wv][lane] != 0);
                 const uint32_t q = q0 + lane;
                 if (q < total) {
-                    // the fill owning pair q: first j with scum[j] > q
-                    uint32_t j = 0;
-#pragma unroll
-                    for (uint32_t st = 32; st > 0; st >>= 1)
-                        if (scum[wv][j + st - 1] <= q) j += st;
-                    const uint2 fj = sfill[wv][j];
-                    const uint32_t before = j ? scum[wv][j - 1] : 0u;
-                    const float x0 = (float)(fj.x & 0xffffu) * (1.0f / 256.0f), x1 = (float)(fj.y & 0xffffu) * (1.0f / 256.0f);
-                    const int col = (int)floorf(fminf(x0, x1)) + (int)(q - before);
-                    accumulate_fill(fj, col, scale, dwave + sslot[wv][j] * FILL_GROUP_WORDS);
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(marks >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)marks, 0u));
+                    const uint32_t j = min(carry + below + (uint32_t)((marks >> lane) & 1ull) - 1u, 63u);
+                    const uint4 f = sfill[wv][j];
+                    const int col = (int)(f.w & 15u) + (int)(q - f.z);
+                    accumulate_fill(make_uint2(f.x, f.y), col, scale, dwave + (f.w >> 4));
                 }
+                carry += (uint32_t)__popcll(marks);
+                // the next window rewrites the marks, the next chunk the fills
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         if (ntot == 0) {  // nothing accumulated: the zeroing still has to land first
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
