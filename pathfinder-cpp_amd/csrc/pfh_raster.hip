@@ -66,16 +66,31 @@ __device__ __forceinline__ int32_t round_fix(float v) {
     return r;
 }
 
-__device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, int32_t *tile_diff) {
+// A fill's geometry, decoded once per fill: x range [xl, xr], the height yl at xl, the slope,
+// and the sign (fill direction) carried in xl's sign bit (xl >= 0, so |x| restores it).
+struct __align__(16) FillGeom {
+    float xl_signed, xr, yl, slope;
+};
+
+__device__ __forceinline__ FillGeom fill_geom(uint2 fl) {
     const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), y0 = (float)(fl.x >> 16) * (1.0f / 256.0f);
     const float x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f), y1 = (float)(fl.y >> 16) * (1.0f / 256.0f);
     const float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
+    const float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
+    FillGeom g;
+    g.xl_signed = x0 > x1 ? xl : -xl;  // positive: the fill adds coverage
+    g.xr = xr;
+    g.yl = yl;
+    g.slope = (yr - yl) * __builtin_amdgcn_rcpf(xr - xl);
+    return g;
+}
+
+__device__ __forceinline__ void accumulate_fill(const FillGeom &g, int col, float scale, int32_t *tile_diff) {
+    const float xl = fabsf(g.xl_signed), xr = g.xr;
     const float lo = fmaxf((float)col, xl), hi = fminf((float)(col + 1), xr);
     if (!(hi > lo) || (uint32_t)col > 15u) return;
-    const float ks = x0 > x1 ? scale : -scale;  // sign of the fill, in fixed-point units
-    const float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
-    const float slope = (yr - yl) * __builtin_amdgcn_rcpf(xr - xl);
-    const float ya = yl + (lo - xl) * slope, yb = yl + (hi - xl) * slope;
+    const float ks = __builtin_copysignf(scale, g.xl_signed);  // sign of the fill, in fixed-point units
+    const float ya = g.yl + (lo - xl) * g.slope, yb = g.yl + (hi - xl) * g.slope;
     const float ymin = fminf(ya, yb), ymax = fmaxf(ya, yb);
     const float ws = ks * (hi - lo);  // the full width, signed, in fixed-point units
     const float fr0 = __builtin_amdgcn_fmed3f(floorf(ymin), 0.0f, 15.0f);
@@ -98,13 +113,13 @@ __device__ __forceinline__ void accumulate_fill(uint2 fl, int col, float scale, 
     if (n >= 1) atomicAdd(cd + r1 + 1, full - vd);  // n == 1: v1 == vd
 }
 
-// Inclusive prefix sum across the 64-lane wave: DPP within rows, then the row totals.
+// Inclusive prefix sum across the 64-lane wave: DPP within rows, then the row totals by row
+// broadcasts (lane 15 into rows 1 and 3, then lane 31 into rows 2 and 3).
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
     v = row16_inclusive_sum(v);
-    const uint32_t t0 = __builtin_amdgcn_readlane(v, 15), t1 = __builtin_amdgcn_readlane(v, 31);
-    const uint32_t t2 = __builtin_amdgcn_readlane(v, 47);
-    const uint32_t row = (threadIdx.x & 63) >> 4;
-    return v + (row == 0 ? 0u : (row == 1 ? t0 : (row == 2 ? t0 + t1 : t0 + t1 + t2)));
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
 }
 
 // v_med3_i32: the median of three.
@@ -157,9 +172,10 @@ __device__ __forceinline__ void resolve_column(int32_t run, const int32_t (&dv)[
 // tile is rasterized, in index order.
 __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     __shared__ __align__(16) int32_t diff[FILL_TILES_PER_BLOCK * FILL_GROUP_WORDS];
-    // a chunk's fills: (fill x word, fill y word, first pair, tile offset in the wave's LDS << 4
-    // | first pixel column); a pair window's marks: 1 where a fill's pairs start
-    __shared__ uint4 sfill[4][64];
+    // a chunk's fills: geometry, and (first pair | tile offset in the wave's LDS << 11 | first
+    // pixel column << 27); a pair window's marks: 1 where a fill's pairs start
+    __shared__ FillGeom sgeom[4][64];
+    __shared__ uint32_t smeta[4][64];
     __shared__ uint8_t smark[4][64];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t slot = lane >> 4, li = lane & 15;  // tile slot in the wave, pixel column
@@ -200,24 +216,32 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
         const uint32_t s2 = __builtin_amdgcn_readlane(f0, 32), n2 = __builtin_amdgcn_readlane(f1, 32) - s2;
         const uint32_t s3 = __builtin_amdgcn_readlane(f0, 48), n3 = __builtin_amdgcn_readlane(f1, 48) - s3;
         const uint32_t c1 = n0, c2 = c1 + n1, c3 = c2 + n2, ntot = c3 + n3;
+        const uint32_t o1 = s1 - c1, o2 = s2 - c2, o3 = s3 - c3;  // wrapping differences
         drawn += ntot;
         for (uint32_t base = 0; base < ntot; base += 64) {
             const uint32_t k = base + lane;
-            uint2 fl = make_uint2(0, 0);
             uint32_t cnt = 0, fs = 0, c0 = 0;
+            FillGeom g{0.0f, 0.0f, 0.0f, 0.0f};
             if (k < ntot) {
-                fs = (k >= c1 ? 1u : 0u) + (k >= c2 ? 1u : 0u) + (k >= c3 ? 1u : 0u);
-                const uint32_t addr = fs == 0 ? s0 + k : (fs == 1 ? s1 + (k - c1) : (fs == 2 ? s2 + (k - c2) : s3 + (k - c3)));
-                fl = b.fills[addr];
-                const float x0 = (float)(fl.x & 0xffffu) * (1.0f / 256.0f), x1 = (float)(fl.y & 0xffffu) * (1.0f / 256.0f);
-                const int cl = (int)floorf(fminf(x0, x1)), cr = (int)ceilf(fmaxf(x0, x1)) - 1;
-                cnt = (uint32_t)max(0, cr - cl + 1);
+                // fill k of the concatenated ranges: k minus its tile's first, plus that tile's start
+                const bool ge1 = k >= c1, ge2 = k >= c2, ge3 = k >= c3;
+                fs = (ge1 ? 1u : 0u) + (ge2 ? 1u : 0u) + (ge3 ? 1u : 0u);
+                uint32_t off = s0;
+                off = ge1 ? o1 : off;
+                off = ge2 ? o2 : off;
+                off = ge3 ? o3 : off;
+                const uint32_t addr = k + off;
+                g = fill_geom(b.fills[addr]);
+                const float xl = fabsf(g.xl_signed);
+                const int cl = (int)floorf(xl);
+                cnt = (uint32_t)max(0, (int)ceilf(g.xr) - cl);
                 c0 = (uint32_t)cl & 15u;  // 8.8 fills lie in [0, 16): cl <= 15
             }
             const uint32_t incl = wave_inclusive_sum(cnt);
             const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-            const uint32_t start = incl - cnt;  // this fill's first pair
-            sfill[wv][lane] = make_uint4(fl.x, fl.y, start, ((fs * FILL_GROUP_WORDS) << 4) | c0);
+            const uint32_t start = incl - cnt;  // this fill's first pair (< 64 * 17)
+            sgeom[wv][lane] = g;
+            smeta[wv][lane] = start | ((fs * FILL_GROUP_WORDS) << 11) | (c0 << 27);
             // Pairs go in windows of 64, one per lane.  The fill owning pair q is the last one
             // starting at or before q: marks at the window's fill starts, their ballot, and the
             // count of marks up to this lane (plus the fills that started in earlier windows)
@@ -240,9 +264,10 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
                     const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(marks >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)marks, 0u));
                     const uint32_t j = min(carry + below + (uint32_t)((marks >> lane) & 1ull) - 1u, 63u);
-                    const uint4 f = sfill[wv][j];
-                    const int col = (int)(f.w & 15u) + (int)(q - f.z);
-                    accumulate_fill(make_uint2(f.x, f.y), col, scale, dwave + (f.w >> 4));
+                    const FillGeom g = sgeom[wv][j];
+                    const uint32_t m = smeta[wv][j];
+                    const int col = (int)(m >> 27) + (int)(q - (m & 2047u));
+                    accumulate_fill(g, col, scale, dwave + ((m >> 11) & 2047u));
                 }
                 carry += (uint32_t)__popcll(marks);
                 // the next window rewrites the marks, the next chunk the fills
