@@ -161,10 +161,9 @@ struct BatchDev {
     uint4 *dump_fills;            // [F] emission order: (x0|y0<<16, x1|y1<<16, global tile, 0)
     uint4 *alpha_rec;             // [A] (fill begin, fill end, backdrop | rule << 8, tile)
     uint32_t *alpha_vis;          // [A] alpha indices of the alpha entries (tiles that survive the z-cull)
-    uint8_t *masks;               // [A * 256]
+    uint8_t *masks;               // [(A + 1) * 256]: row cap_alpha is all ones (written by the fill kernel)
     uint32_t *zbuf;               // [FB]
     uint2 *fb_range;              // [FB] [start, end) of each framebuffer tile's entries in sort order
-    const uint32_t *ones_mask;    // [64] an all-ones coverage mask row (full coverage)
     uint32_t *keys, *vals;        // [E] sort keys / values
     uint32_t *keys_alt, *vals_alt;
     uint2 *entry_rec;             // [E] (alpha index | NO_ALPHA, rgba (alpha 0 if covers nothing))

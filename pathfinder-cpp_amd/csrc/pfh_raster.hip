@@ -185,6 +185,9 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     const uint32_t na = min(b.counters[listed ? C_ALPHA_ENTRIES : C_ALPHA], b.cap_alpha);
     const float scale = (float)(1 << FIX_SHIFT);
     uint32_t drawn = 0;  // fills this wave read (wave-uniform)
+    // row cap_alpha of the masks is all ones: the composite's full-coverage row
+    if (blockIdx.x == 0 && wv == 0)
+        reinterpret_cast<uint32_t *>(b.masks)[(uint64_t)b.cap_alpha * 64 + lane] = 0xffffffffu;
     // grid-stride over groups of 16 tiles: the grid is sized from the capacity, the count
     // is only known on the device; the loop bound is uniform over the block
     const uint32_t lb = (b.xcd_order & 1u) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -367,6 +370,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     // a lane owns 4 vertically adjacent pixels: column lane / 4, rows 4 (lane % 4) .. + 3,
     // matching the column-major coverage masks (one 32-bit mask word per lane)
     const int x = tx * TILE + (lane >> 2), y0 = ty * TILE + (lane & 3) * 4;
+    uint32_t lane_off = (uint32_t)lane * 4u;  // this lane's word in a mask row
     float r[4], g[4], bl[4], tr[4];  // accumulated premultiplied colour, transmittance
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -379,19 +383,22 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     for (uint32_t base = e0 + (e1 > e0 ? (e1 - e0 - 1) / 64 * 64 : 0); e1 > e0; base -= 64) {
         const uint32_t i = base + lane;
         const uint32_t cnt = min(64u, e1 - base);
-        uint32_t ai = NO_ALPHA, c = 0;
+        // ai: the mask row this lane's entry samples; row cap_alpha is all ones (full
+        // coverage), sampled by solid entries and lanes past the chunk
+        uint32_t ai = b.cap_alpha, c = 0;
+        bool masked = false;
         if (i < e1) {
             const uint2 rec = b.entry_rec[sorted_vals[i]];
-            ai = rec.x;
             c = rec.y;
-            if (ai != NO_ALPHA && ai >= b.cap_alpha) {  // capacity overflow: the run is retried
-                c = 0;                                  // blends nothing
-                ai = 0;                                 // and reads a mask that exists
+            if (rec.x != NO_ALPHA) {
+                masked = true;
+                if (rec.x < b.cap_alpha) ai = rec.x;
+                else c = 0;  // capacity overflow (the run is retried): blends nothing
             }
         }
         const float ca = (float)(c >> 24) * (1.0f / 255.0f);
         const bool valid = (uint32_t)lane < cnt;
-        const bool solid = valid && ai == NO_ALPHA;
+        const bool solid = valid && !masked;
         // premultiplied colour; a masked entry's is also scaled by 1/255 so that it multiplies
         // the raw mask byte.  Runs of solid entries are folded by an inclusive segmented scan.
         const float sc_a = solid ? ca : ca * (1.0f / 255.0f);
@@ -431,13 +438,15 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         const uint64_t alpha_mask = valid_mask & ~solid_mask;
         const uint64_t run_end = solid_mask & ~(solid_mask >> 1);
         uint64_t events = alpha_mask | run_end;
-        // this lane's mask word for event j (ai is NO_ALPHA on solid lanes; j < 0, an empty
-        // slot, reads the all-ones row: every slot loads, so no branch splits the loads)
+        // this lane's mask word for event j; an empty slot (j = -1) loads lane 63's row, which
+        // exists, so every slot loads and no branch splits the loads
         auto mask_word = [&](int j) -> uint32_t {
-            const uint32_t ij = __builtin_amdgcn_readlane(ai, max(j, 0));
-            const uint32_t idx = j >= 0 ? ij : NO_ALPHA;
-            const uint32_t *row_words = idx == NO_ALPHA ? b.ones_mask : reinterpret_cast<const uint32_t *>(b.masks) + (uint64_t)idx * 64;
-            return row_words[lane];
+            const uint32_t idx = __builtin_amdgcn_readlane(ai, j & 63);
+            const uint8_t *row = b.masks + (uint64_t)idx * 256;  // uniform: a scalar base
+            // the lane's byte offset, made opaque in place, so that the load keeps the
+            // scalar-base form instead of a hoisted per-lane pointer plus a vector add per event
+            asm volatile("" : "+v"(lane_off));
+            return *reinterpret_cast<const uint32_t *>(row + lane_off);
         };
         // The chunk's event colours go through LDS: a blend reads its event's colour with one
         // broadcast LDS read instead of four lane reads, which would occupy the VALU.
@@ -468,8 +477,8 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         auto take = [&](int (&js)[EV_GROUP], uint32_t (&ms)[EV_GROUP]) {
 #pragma unroll
             for (int d = 0; d < EV_GROUP; d++) {
-                js[d] = events ? 63 - __clzll(events) : -1;
-                events &= events ? ~(1ull << js[d]) : 0ull;
+                js[d] = 63 - __clzll(events);  // -1 once none are left
+                events &= ~(1ull << (js[d] & 63));
                 ms[d] = mask_word(js[d]);
             }
         };
@@ -534,9 +543,9 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, uint32_t max_blocks, hipStream_t s) {
-    if (grid_alpha == 0) return hipSuccess;
-    // at most max_blocks workgroups: each loops over its share of the tiles
-    const uint32_t nb = std::min<uint32_t>(blocks_for(grid_alpha, FILL_TILES_PER_BLOCK), std::max(max_blocks, 1u));
+    // at least one workgroup (it writes the all-ones row), at most max_blocks: each loops over
+    // its share of the tiles
+    const uint32_t nb = std::max(1u, std::min<uint32_t>(blocks_for(grid_alpha, FILL_TILES_PER_BLOCK), max_blocks));
     hipLaunchKernelGGL(k_fill_masks, dim3(nb), dim3(256), 0, s, b);
     return hipGetLastError();
 }

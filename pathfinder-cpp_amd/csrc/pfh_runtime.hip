@@ -161,7 +161,7 @@ struct Workspace {
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, path_tot, bin_items, col_item, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, alpha_vis, masks, zbuf, fb_range, keys, vals, keys_alt,
-        vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp, ones_mask;
+        vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
     size_t h_table_cap = 0;
@@ -186,7 +186,7 @@ struct Workspace {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
                 &path_ent, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &entry_meta, &counters, &scan_tmp,
-                &sort_tmp, &ones_mask};
+                &sort_tmp};
     }
     size_t device_bytes() {
         size_t n = 0;
@@ -549,10 +549,6 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
     w.zbuf.ensure(4 * pl.FB + 4);
     w.fb_range.ensure(8 * pl.FB + 8);
-    if (!w.ones_mask.ptr) {
-        w.ones_mask.ensure(256);
-        HIP_TRY(launch_fill_u32(w.ones_mask.as<uint32_t>(), 0xffffffffu, 64, w.stream));
-    }
     w.counters.ensure(4 * C_COUNT);
     uint64_t scan_n = std::max<uint64_t>({pl.T + 1, pl.S + 1, pl.FB + 1, pl.P + 1, pl.W + 1, (uint64_t)pl.cap_l + 1, 1});
     w.scan_tmp.ensure(16 * scan_temp_elems(scan_n) + 64);
@@ -560,7 +556,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     if (dump) w.dump_fills.ensure(16ull * pl.cap_f);
     w.alpha_rec.ensure(16ull * pl.cap_a);
     w.alpha_vis.ensure(4ull * pl.cap_a);
-    w.masks.ensure(256ull * pl.cap_a);
+    w.masks.ensure(256ull * pl.cap_a + 256);  // + the all-ones row
     w.keys.ensure(4ull * pl.cap_e);
     w.vals.ensure(4ull * pl.cap_e);
     w.keys_alt.ensure(4ull * pl.cap_e);
@@ -582,7 +578,7 @@ static Caps held_caps(Workspace &w, bool dump) {
     uint64_t l = std::min({fit(w.lines.bytes, 16, 0), fit(w.line_cnt.bytes, 4, 8)});
     uint64_t f = fit(w.fills.bytes, 8, 0);
     if (dump) f = std::min(f, fit(w.dump_fills.bytes, 16, 0));
-    uint64_t a = std::min({fit(w.alpha_rec.bytes, 16, 0), fit(w.alpha_vis.bytes, 4, 0), fit(w.masks.bytes, 256, 0)});
+    uint64_t a = std::min({fit(w.alpha_rec.bytes, 16, 0), fit(w.alpha_vis.bytes, 4, 0), fit(w.masks.bytes, 256, 256)});
     return Caps{clamp_cap(l), clamp_cap(f), clamp_cap(a)};
 }
 
@@ -739,7 +735,6 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.masks = w.masks.as<uint8_t>();
     b.zbuf = w.zbuf.as<uint32_t>();
     b.fb_range = w.fb_range.as<uint2>();
-    b.ones_mask = w.ones_mask.as<uint32_t>();
     b.keys = w.keys.as<uint32_t>();
     b.vals = w.vals.as<uint32_t>();
     b.keys_alt = w.keys_alt.as<uint32_t>();
