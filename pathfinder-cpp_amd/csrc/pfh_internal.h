@@ -20,9 +20,10 @@ constexpr uint32_t NO_ALPHA = 0xffffffffu;
 // class when its tile rect and column count fit one wave's LDS slice, else to the group class,
 // which uses LDS up to BIN_GROUP_TILES / BIN_GROUP_COLS and global atomics beyond.
 constexpr int BIN_WAVE_TILES = 512;
-constexpr int BIN_WAVE_COLS = 256;
+constexpr int BIN_WAVE_COLS = 128;
 constexpr int BIN_GROUP_TILES = 4608;
-constexpr int BIN_GROUP_COLS = 1024;
+constexpr int BIN_GROUP_COLS = 512;
+constexpr int WALK_QUEUE = 128;          // per wave: chords that cross tile boundaries, waiting for a full wave
 
 enum SegKind : uint32_t { SEG_LINE = 1, SEG_QUAD = 2, SEG_CUBIC = 3 };
 

@@ -505,53 +505,105 @@ __device__ __forceinline__ void path_chords(const BatchDev &b, const SceneDev &s
     c1 = min(c1, b.cap_lines);
 }
 
-template <bool EMIT, bool LDS>
-__device__ __forceinline__ void walk_chords(const BatchDev &b, const SceneDev &sc, const int4 &rect, uint32_t tbase,
-                                            uint32_t cbase, uint32_t *l_cnt, int32_t *l_delta, int32_t *l_col,
-                                            uint32_t c0, uint32_t c1, uint32_t first, uint32_t stride) {
-    uint32_t guard = 0;
-    for (uint32_t i = c0 + first; i < c1; i += stride) {
-        const float4 ln = b.lines[i];
-        Walker<EMIT, LDS> w;
-        w.vb_l = sc.vb_l;
-        w.vb_r = sc.vb_r;
-        w.vb_b = sc.vb_b;
-        w.tl = rect.x;
-        w.tt = rect.y;
-        w.tr = rect.z;
-        w.tb = rect.w;
-        w.tw = rect.z - rect.x;
-        w.th = rect.w - rect.y;
-        w.tbase = tbase;
-        w.cbase = cbase;
-        w.tile_cnt = b.tile_cnt;
-        w.tile_delta = b.tile_delta;
-        w.col_bd = b.col_bd;
-        w.l_cnt = l_cnt;
-        w.l_delta = l_delta;
-        w.l_col = l_col;
-        w.out = b.dump ? b.dump_fills : nullptr;
-        w.fills = b.fills;
-        w.tile_fill_off = b.tile_fill_off;
-        w.cap_fills = b.cap_fills;
-        w.pend_tile = 0xffffffffu;
-        w.pend_n = 0;
-        w.count = 0;
-        w.guard = 0;
-        w.out_base = (EMIT && b.dump) ? b.line_cnt[i] : 0u;
-        w.line(make_float2(ln.x, ln.y), make_float2(ln.z, ln.w));
-        w.flush();
-        if (!EMIT && b.dump) b.line_cnt[i] = w.count;  // fills per chord: canonical dump order only
-        guard += w.guard;
-    }
-    if (!EMIT && guard) atomicAdd(&b.counters[C_GUARD], guard);
-}
-
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// The chords [c0, c1) of a path, thread `first` of `stride` taking every stride-th.  Most
+// chords lie inside one tile, where process_line_segment emits exactly one fill (from, from +
+// vec) and no auxiliary fill or backdrop step: the tile crossings it would compute are all at
+// t >= 1.  Those are emitted at once; the others (crossing a tile boundary) are queued per wave
+// in LDS (wq, WALK_QUEUE slots) and walked 64 at a time, so the DDA's divisions and steps run
+// with every lane on a chord that needs them.
+template <bool EMIT, bool LDS>
+__device__ __forceinline__ void walk_chords(const BatchDev &b, const SceneDev &sc, const int4 &rect, uint32_t tbase,
+                                            uint32_t cbase, uint32_t *l_cnt, int32_t *l_delta, int32_t *l_col,
+                                            uint32_t c0, uint32_t c1, uint32_t first, uint32_t stride,
+                                            uint32_t *wq) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t guard = 0;
+    Walker<EMIT, LDS> w;
+    w.vb_l = sc.vb_l;
+    w.vb_r = sc.vb_r;
+    w.vb_b = sc.vb_b;
+    w.tl = rect.x;
+    w.tt = rect.y;
+    w.tr = rect.z;
+    w.tb = rect.w;
+    w.tw = rect.z - rect.x;
+    w.th = rect.w - rect.y;
+    w.tbase = tbase;
+    w.cbase = cbase;
+    w.tile_cnt = b.tile_cnt;
+    w.tile_delta = b.tile_delta;
+    w.col_bd = b.col_bd;
+    w.l_cnt = l_cnt;
+    w.l_delta = l_delta;
+    w.l_col = l_col;
+    w.out = b.dump ? b.dump_fills : nullptr;
+    w.fills = b.fills;
+    w.tile_fill_off = b.tile_fill_off;
+    w.cap_fills = b.cap_fills;
+    auto begin = [&](uint32_t i) {
+        w.pend_tile = 0xffffffffu;
+        w.pend_n = 0;
+        w.count = 0;
+        w.guard = 0;
+        w.out_base = (EMIT && b.dump) ? b.line_cnt[i] : 0u;
+    };
+    auto end = [&](uint32_t i) {
+        w.flush();
+        if (!EMIT && b.dump) b.line_cnt[i] = w.count;  // fills per chord: canonical dump order only
+        guard += w.guard;
+    };
+    uint32_t qn = 0;  // queued chords (wave-uniform)
+    for (uint32_t base = c0; base < c1; base += stride) {  // uniform over the wave
+        const uint32_t i = base + first;
+        bool multi = false;
+        if (i < c1) {
+            const float4 ln = b.lines[i];
+            float2 from = make_float2(ln.x, ln.y), to = make_float2(ln.z, ln.w);
+            begin(i);
+            if (isfinite(from.x) && isfinite(from.y) && isfinite(to.x) && isfinite(to.y) && w.clip(from, to)) {
+                const float inv = 1.0f / 16.0f;
+                const int fx = (int)floorf(from.x * inv), fy = (int)floorf(from.y * inv);
+                const int ex = (int)floorf(to.x * inv), ey = (int)floorf(to.y * inv);
+                const float2 vec = f2sub(to, from);
+                // a -0 component makes a crossing time -inf: left to the full walk
+                const bool neg_zero = (vec.x == 0.0f && signbit(vec.x)) || (vec.y == 0.0f && signbit(vec.y));
+                if (fx == ex && fy == ey && !neg_zero) w.add_fill(from, f2add(from, vec), fx, fy);
+                else multi = true;
+            }
+            if (!multi) end(i);
+        }
+        const uint64_t m = __ballot(multi);
+        if (multi) wq[qn + (uint32_t)__popcll(m & lt_mask)] = i;
+        qn += (uint32_t)__popcll(m);
+        wave_sync();
+        if (qn >= 64) {  // at most 127 queued: one full batch
+            qn -= 64;
+            const uint32_t ci = wq[qn + lane];
+            wave_sync();  // read before the next chunk's chords overwrite the slots
+            const float4 ln = b.lines[ci];
+            begin(ci);
+            w.line(make_float2(ln.x, ln.y), make_float2(ln.z, ln.w));
+            end(ci);
+        }
+    }
+    if (lane < qn) {  // the rest of the queue
+        const uint32_t ci = wq[lane];
+        const float4 ln = b.lines[ci];
+        begin(ci);
+        w.line(make_float2(ln.x, ln.y), make_float2(ln.z, ln.w));
+        end(ci);
+    }
+    wave_sync();  // the queue is read before the caller reuses the wave's LDS
+    if (!EMIT && guard) atomicAdd(&b.counters[C_GUARD], guard);
+}
+
 
 // Per-path totals and offsets.  The count pass stores each path's {fills, alpha tiles}
 // (path_tot, batch path order); after their scan the emit pass turns its rect's tile counts
@@ -661,6 +713,7 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
     __shared__ uint32_t s_cnt[4][BIN_WAVE_TILES];
     __shared__ int32_t s_delta[4][EMIT ? 1 : BIN_WAVE_TILES];
     __shared__ int32_t s_col[4][EMIT ? 1 : BIN_WAVE_COLS];
+    __shared__ uint32_t s_queue[4][WALK_QUEUE];
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wv;
     if (item >= b.n_bin_wave) return;  // whole wave: no block-wide barriers below
@@ -685,7 +738,7 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
         for (uint32_t j = lane; j < tw; j += 64) l_col[j] = 0;
     }
     wave_sync();
-    walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, l_cnt, l_delta, l_col, it.c0, it.c1, lane, 64);
+    walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, l_cnt, l_delta, l_col, it.c0, it.c1, lane, 64, s_queue[wv]);
     if (!EMIT) {
         wave_sync();
         U2 tot{0, 0};
@@ -708,6 +761,7 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
     __shared__ uint32_t s_cnt[BIN_GROUP_TILES];
     __shared__ int32_t s_delta[EMIT ? 1 : BIN_GROUP_TILES];
     __shared__ int32_t s_col[EMIT ? 1 : BIN_GROUP_COLS];
+    __shared__ uint32_t s_queue[4][WALK_QUEUE];
     const uint32_t item = blockIdx.x;
     if (item >= b.n_bin_group) return;  // uniform over the block
     const BinItem it = b.bin_items[b.n_bin_wave + item];
@@ -731,7 +785,8 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
             for (uint32_t j = threadIdx.x; j < tw; j += 256) s_col[j] = 0;
         }
         __syncthreads();
-        walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, s_cnt, s_delta, s_col, c0, c1, threadIdx.x, 256);
+        walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, s_cnt, s_delta, s_col, c0, c1, threadIdx.x, 256,
+                                s_queue[threadIdx.x >> 6]);
         if (!EMIT) {
             __syncthreads();
             U2 tot{0, 0};
@@ -761,7 +816,8 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
             __threadfence();  // the zeros reach L2 before any thread's atomics
             __syncthreads();
         }
-        walk_chords<EMIT, false>(b, sc, rect, tbase, cbase, nullptr, nullptr, nullptr, c0, c1, threadIdx.x, 256);
+        walk_chords<EMIT, false>(b, sc, rect, tbase, cbase, nullptr, nullptr, nullptr, c0, c1, threadIdx.x, 256,
+                                 s_queue[threadIdx.x >> 6]);
         if (!EMIT) {
             __threadfence();  // every thread's atomics are done before the counts are summed
             __syncthreads();
