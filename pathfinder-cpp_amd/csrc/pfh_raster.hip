@@ -85,9 +85,22 @@ __device__ __forceinline__ FillGeom fill_geom(uint2 fl) {
     return g;
 }
 
+// v_max_f32 / v_min_f32 on operands that are never NaN here, without the quieting fminf and
+// fmaxf apply to values loaded from memory.
+__device__ __forceinline__ float max_f(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float min_f(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 __device__ __forceinline__ void accumulate_fill(const FillGeom &g, int col, float scale, int32_t *tile_diff) {
     const float xl = fabsf(g.xl_signed), xr = g.xr;
-    const float lo = fmaxf((float)col, xl), hi = fminf((float)(col + 1), xr);
+    const float lo = max_f((float)col, xl), hi = min_f((float)(col + 1), xr);
     if (!(hi > lo) || (uint32_t)col > 15u) return;
     const float ks = __builtin_copysignf(scale, g.xl_signed);  // sign of the fill, in fixed-point units
     const float ya = g.yl + (lo - xl) * g.slope, yb = g.yl + (hi - xl) * g.slope;
@@ -138,8 +151,8 @@ __device__ __forceinline__ uint32_t coverage_hi(int32_t w, uint32_t even_odd) {
     const int32_t nz = med3_i32(w, -w, one);
     const int32_t m = w & (2 * one - 1);  // w mod 2 (two's complement: also for negative w)
     const int32_t eo = min(m, 2 * one - m);
-    const uint32_t c = (uint32_t)(even_odd ? eo : nz);
-    return c * 4080u + (1u << 23);
+    const uint32_t c = (uint32_t)(even_odd ? eo : nz);  // <= 2^20: a 24-bit multiply-add
+    return __umul24(c, 4080u) + (1u << 23);
 }
 
 // Prefix of a column's 16 difference values on top of the backdrop, then coverage bytes packed
