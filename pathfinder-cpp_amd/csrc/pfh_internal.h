@@ -21,7 +21,7 @@ constexpr uint32_t NO_ALPHA = 0xffffffffu;
 // which uses LDS up to BIN_GROUP_TILES / BIN_GROUP_COLS and global atomics beyond.
 constexpr int BIN_WAVE_TILES = 512;
 constexpr int BIN_WAVE_COLS = 128;
-constexpr int BIN_GROUP_TILES = 4608;
+constexpr int BIN_GROUP_TILES = 4600;   // with the walk queues: 20 KB (emit) and 40 KB (count) of LDS
 constexpr int BIN_GROUP_COLS = 512;
 constexpr int WALK_QUEUE = 128;          // per wave: chords that cross tile boundaries, waiting for a full wave
 

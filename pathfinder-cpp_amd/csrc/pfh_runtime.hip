@@ -523,11 +523,11 @@ struct Plan {
 };
 
 // Segments per flatten wave: the schedule's choice, or enough for ~4 waves per SIMD (1024 SIMDs),
-// 128 to 2048 (a wave balances its lanes over its whole range).
+// 64 to 2048 (a wave balances its lanes over its whole range).
 static uint32_t flatten_segs(const Plan &pl) {
     if (pl.sched.flatten_segs) return pl.sched.flatten_segs;
     const uint64_t per = (pl.S + 4096 * 64 - 1) / (4096 * 64);
-    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(64 * per, 128), 2048);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(64 * per, 64), 2048);
 }
 
 static uint32_t clamp_cap(uint64_t v) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(v, 64), 0xF0000000ull); }
@@ -1779,7 +1779,7 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         double best_ms = measure();
         if (ms_start) *ms_start = best_ms;
         const std::vector<uint32_t> cand[5] = {{1024, 2048, 4096, 8192, 16384, 32768}, {64, 128, 256}, {64, 128, 256},
-                                              {0, 1, 2, 3}, {0, 128, 256, 512, 1024, 2048}};
+                                              {0, 1, 2, 3}, {0, 64, 128, 256, 512, 1024, 2048}};
         for (int param = 0; param < 5; param++) {
             for (uint32_t v : cand[param]) {
                 Schedule trial = best;
