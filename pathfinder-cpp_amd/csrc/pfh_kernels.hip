@@ -1259,21 +1259,23 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(BatchDev b, const uint32
 }
 
 // Scatter of one 8-bit digit pass.  The block's 4096 items are first ranked stably by digit
-// into LDS (16 rows of 256: a ballot multisplit gives each item's rank among equal digits in
-// its wave, earlier waves and rows are added from per-digit counters), then written out
-// in digit order: runs of equal digits land at consecutive global positions, so stores
-// coalesce instead of scattering one item per digit per row.  offs is the exclusive scan of
-// the [digit][block] histogram: the block's count of digit d is the difference of its
-// entry and the next one.  With no vals, an item's value is its input index.
+// into LDS, then written out in digit order: runs of equal digits land at consecutive global
+// positions, so stores coalesce instead of scattering one item per digit per row.  Each wave
+// ranks its own contiguous 1024 items (16 rows of 64) with wave-local digit counters: a ballot
+// multisplit gives an item's rank among equal digits of its row, the counter the items of
+// earlier rows; only then are the waves' counts combined per digit.  offs is the exclusive
+// scan of the [digit][block] histogram: the block's count of digit d is the difference of
+// its entry and the next one.  With no vals, an item's value is its input index.
+constexpr int RS_WAVE_ITEMS = RS_ITEMS * 64;
+
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(BatchDev b, const uint32_t *keys, const uint32_t *vals,
                                                            uint32_t *keys_out, uint32_t *vals_out,
                                                            const uint32_t *seg_e, const uint32_t *seg_b, int shift,
                                                            const uint32_t *offs) {
     __shared__ uint32_t skey[RS_TILE], sval[RS_TILE];
-    __shared__ uint32_t run[256];     // items of each digit placed so far (local)
     __shared__ uint32_t lstart[256];  // local start of each digit
     __shared__ uint32_t goff[256];    // global start of each digit's run from this block
-    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][256];  // per wave: digit counts, then the wave's base per digit
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     RsBlock rb;
     if (!rs_block(b, seg_e, seg_b, rb)) return;  // uniform over the block
@@ -1285,41 +1287,62 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(BatchDev b, const uin
         goff[threadIdx.x] = o;
         uint32_t total;
         lstart[threadIdx.x] = block_exclusive_scan<uint32_t>(offs[h + 1] - o, total);
-        run[threadIdx.x] = 0;
     }
+    for (int q = threadIdx.x; q < (RS_THREADS / 64) * 256; q += RS_THREADS) (&wcnt[0][0])[q] = 0;
     __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t *cnt = wcnt[wave];
+    uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+    const uint32_t w0 = (uint32_t)wave * RS_WAVE_ITEMS;
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        for (int q = threadIdx.x; q < (RS_THREADS / 64) * 256; q += RS_THREADS) (&wcnt[0][0])[q] = 0;
-        __syncthreads();
-        const uint32_t li = (uint32_t)r * RS_THREADS + threadIdx.x;
+        const uint32_t li = w0 + (uint32_t)r * 64 + (uint32_t)lane;
         const bool valid = li < valid_n;
-        const uint32_t key = valid ? keys[base + li] : 0u;
-        const uint32_t val = !valid ? 0u : (vals ? vals[base + li] : (uint32_t)(base + li));
-        const uint32_t d = ((key - rb.fb_base) >> shift) & 255u;
-        // lanes holding the same digit (multisplit peer mask)
+        key[r] = valid ? keys[base + li] : 0u;
+        val[r] = !valid ? 0u : (vals ? vals[base + li] : (uint32_t)(base + li));
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const uint32_t li = w0 + (uint32_t)r * 64 + (uint32_t)lane;
+        const bool valid = li < valid_n;
+        const uint32_t d = ((key[r] - rb.fb_base) >> shift) & 255u;
+        // lanes of this row holding the same digit (multisplit peer mask)
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; bit++) {
             const uint64_t m = __ballot((d >> bit) & 1u);
             peers &= ((d >> bit) & 1u) ? m : ~m;
         }
-        const uint32_t rank = __popcll(peers & lt_mask);
-        const uint32_t cnt = __popcll(peers);
-        if (valid && rank == cnt - 1) wcnt[wave][d] = cnt;
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = lstart[d] + run[d] + rank;
-            for (int w = 0; w < wave; w++) pos += wcnt[w][d];
-            skey[pos] = key;
-            sval[pos] = val;
-        }
-        __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < RS_THREADS / 64; w++) add += wcnt[w][threadIdx.x];
-        run[threadIdx.x] += add;
-        __syncthreads();
+        const uint32_t rk = __popcll(peers & lt_mask);
+        const uint32_t before = valid ? cnt[d] : 0u;  // items of digit d in earlier rows of this wave
+        rank[r] = before + rk;
+        // the last lane of each digit group advances the counter (after every lane's read: a
+        // wave's LDS operations complete in order)
+        if (valid && rk == (uint32_t)__popcll(peers) - 1u) cnt[d] = before + rk + 1u;
     }
+    __syncthreads();
+    {  // per digit: the counts of earlier waves
+        const uint32_t d = threadIdx.x;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; w++) {
+            const uint32_t c = wcnt[w][d];
+            wcnt[w][d] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const uint32_t li = w0 + (uint32_t)r * 64 + (uint32_t)lane;
+        if (li < valid_n) {
+            const uint32_t d = ((key[r] - rb.fb_base) >> shift) & 255u;
+            const uint32_t pos = lstart[d] + cnt[d] + rank[r];
+            skey[pos] = key[r];
+            sval[pos] = val[r];
+        }
+    }
+    __syncthreads();
     for (uint32_t i = threadIdx.x; i < valid_n; i += RS_THREADS) {
         const uint32_t key = skey[i];
         const uint32_t d = ((key - rb.fb_base) >> shift) & 255u;
