@@ -151,6 +151,7 @@ struct BatchDev {
     int8_t *tile_bd;              // [T] propagated backdrop (written with dumps only)
     uint8_t *tile_st;             // [T] tile status (TileStatus)
     uint2 *path_ent;              // [P + 1] {draw entries, alpha entries} per batch path, then exclusive scan
+    uint2 *ent_quarter;           // [4 n_bin_group] {draw entries, alpha entries} per quarter of a group-class rect
     uint2 *path_tot;              // [P + 1] {fills, alpha tiles} per batch path, then exclusive scan
     BinItem *bin_items;           // [n_bin_wave + n_bin_group] wave class, then group class
     uint32_t *col_item;           // [W] the bin item owning each column

@@ -1108,71 +1108,34 @@ __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &s
     if (b.dump) b.entry_meta[e] = make_uint2(p, et.fbl);
 }
 
-// THREADS 64: one wave per path (its own item); 256: one block per path.
-template <bool WRITE, int THREADS>
-__device__ __forceinline__ void path_entries(const BatchDev &b, const SceneDev &sc, const BinItem &it) {
-    const uint32_t t = THREADS == 64 ? (threadIdx.x & 63) : threadIdx.x, lane = threadIdx.x & 63;
+// One wave over the tiles [j0, j1) of a path's rect, row-major.  Count pass: returns the
+// range's {entries, alpha entries}.  Write pass: numbers them from `first` (entry, alpha entry)
+// by ballot rank.
+template <bool WRITE>
+__device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev &sc, const BinItem &it, uint32_t j0,
+                                               uint32_t j1, uint2 first) {
+    const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t p = it.p, gp = it.gp;
+    const uint32_t p = it.p;
     const int4 r = it.rect;
-    const int w = max(r.z - r.x, 0), h = max(r.w - r.y, 0);
-    const uint32_t area = (uint32_t)w * (uint32_t)h;
+    const int w = max(r.z - r.x, 0);
     const uint32_t tbase = it.tbase;
     const int fb_row0 = (r.y - sc.fb_y0) * sc.fb_w + (r.x - sc.fb_x0);
     const uint32_t rgba = it.rgba;
     const float inv_w = w > 0 ? 1.0f / (float)w : 0.0f;
-    // write pass: the path's first entry and first alpha entry; count pass: this wave's counts
-    const uint2 first = WRITE ? b.path_ent[gp] : make_uint2(0u, 0u);
     uint32_t e = first.x, n_alpha = first.y;
-    __shared__ uint2 wave_tot[2][4];  // block version, write pass: per-wave counts, double-buffered
-    const uint32_t wv = threadIdx.x >> 6;
-    uint32_t parity = 0;
-    for (uint32_t j0 = 0; j0 < area; j0 += THREADS) {  // uniform over the wave / block
-        const uint32_t j = j0 + t;
+    for (uint32_t jc = j0; jc < j1; jc += 64) {  // uniform over the wave
+        const uint32_t j = jc + lane;
         EntryTile et{false, TILE_EMPTY, 0};
-        if (j < area) et = entry_tile(b, sc, p, tbase, w, inv_w, fb_row0, j);
+        if (j < j1) et = entry_tile(b, sc, p, tbase, w, inv_w, fb_row0, j);
         const uint64_t m = __ballot(et.keep), ma = __ballot(et.keep && et.st == TILE_ALPHA);
-        if (!WRITE) {
-            e += (uint32_t)__popcll(m);
-            n_alpha += (uint32_t)__popcll(ma);
-        } else if (THREADS == 64) {
-            if (et.keep)
-                write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
-                            n_alpha + (uint32_t)__popcll(ma & lt_mask));
-            e += (uint32_t)__popcll(m);
-            n_alpha += (uint32_t)__popcll(ma);
-        } else {
-            // entries before this wave's in the chunk: the earlier waves' counts.  A slot is
-            // rewritten two chunks later, after a barrier every wave passes once done reading it.
-            if (lane == 0) wave_tot[parity][wv] = make_uint2((uint32_t)__popcll(m), (uint32_t)__popcll(ma));
-            __syncthreads();
-            uint32_t before = 0, total = 0, before_a = 0, total_a = 0;
-#pragma unroll
-            for (uint32_t v = 0; v < 4; v++) {
-                const uint2 c = wave_tot[parity][v];
-                before += v < wv ? c.x : 0u;
-                before_a += v < wv ? c.y : 0u;
-                total += c.x;
-                total_a += c.y;
-            }
-            if (et.keep)
-                write_entry(b, sc, p, rgba, tbase, j, et, e + before + (uint32_t)__popcll(m & lt_mask),
-                            n_alpha + before_a + (uint32_t)__popcll(ma & lt_mask));
-            e += total;
-            n_alpha += total_a;
-            parity ^= 1u;
-        }
+        if (WRITE && et.keep)
+            write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
+                        n_alpha + (uint32_t)__popcll(ma & lt_mask));
+        e += (uint32_t)__popcll(m);
+        n_alpha += (uint32_t)__popcll(ma);
     }
-    if (!WRITE) {
-        if (THREADS == 256) {  // the waves' counts: a block sum
-            __shared__ uint2 part[4];
-            if (lane == 0) part[wv] = make_uint2(e, n_alpha);
-            __syncthreads();
-            e = part[0].x + part[1].x + part[2].x + part[3].x;
-            n_alpha = part[0].y + part[1].y + part[2].y + part[3].y;
-        }
-        if (t == 0) b.path_ent[gp] = make_uint2(e, n_alpha);
-    }
+    return make_uint2(e, n_alpha);
 }
 
 template <bool WRITE>
@@ -1180,15 +1143,46 @@ __global__ __launch_bounds__(256) void k_entries_wave(BatchDev b) {
     const uint32_t item = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (item >= b.n_bin_wave) return;  // whole wave
     const BinItem it = b.bin_items[item];
-    path_entries<WRITE, 64>(b, b.scenes[it.k], it);
+    const int4 r = it.rect;
+    const uint32_t area = (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
+    const uint2 first = WRITE ? b.path_ent[it.gp] : make_uint2(0u, 0u);
+    const uint2 n = range_entries<WRITE>(b, b.scenes[it.k], it, 0, area, first);
+    if (!WRITE && (threadIdx.x & 63) == 0) b.path_ent[it.gp] = n;
 }
 
+// A block per large path: wave w takes the w-th quarter of the rect's tiles (row-major, in
+// whole 64-tile chunks), so the waves need no barrier.  The count pass keeps each quarter's
+// counts (ent_quarter) beside the path's total; the write pass starts each wave after the
+// quarters before it.
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_entries_group(BatchDev b) {
     const uint32_t item = blockIdx.x;
     if (item >= b.n_bin_group) return;  // uniform over the block
     const BinItem it = b.bin_items[b.n_bin_wave + item];
-    path_entries<WRITE, 256>(b, b.scenes[it.k], it);
+    const int4 r = it.rect;
+    const uint32_t area = (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t q = (area + 255) / 256 * 64;  // tiles per quarter, whole chunks
+    const uint32_t j0 = min(wv * q, area), j1 = min(j0 + q, area);
+    uint2 first = make_uint2(0u, 0u);
+    if (WRITE) {
+        first = b.path_ent[it.gp];
+        for (uint32_t v = 0; v < wv; v++) {
+            const uint2 c = b.ent_quarter[item * 4 + v];
+            first.x += c.x;
+            first.y += c.y;
+        }
+    }
+    const uint2 n = range_entries<WRITE>(b, b.scenes[it.k], it, j0, j1, first);
+    if (!WRITE) {
+        if (lane == 0) b.ent_quarter[item * 4 + wv] = n;
+        __shared__ uint2 part[4];
+        if (lane == 0) part[wv] = n;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            b.path_ent[it.gp] = make_uint2(part[0].x + part[1].x + part[2].x + part[3].x,
+                                           part[0].y + part[1].y + part[2].y + part[3].y);
+    }
 }
 
 // ---------------------------------------------------------------------------

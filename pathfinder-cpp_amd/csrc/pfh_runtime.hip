@@ -159,7 +159,7 @@ struct Workspace {
     // around the fill and composite kernels of a batched launch; recorded as event nodes
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
-    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, path_tot, bin_items, col_item, col_bd,
+    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, ent_quarter, path_tot, bin_items, col_item, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, alpha_vis, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp;
     uint32_t *h_counters = nullptr;  // pinned
@@ -184,7 +184,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &path_ent, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
+                &path_ent, &ent_quarter, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &entry_meta, &counters, &scan_tmp,
                 &sort_tmp};
     }
@@ -540,6 +540,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     if (dump) w.tile_bd.ensure(pl.T + 4);
     w.tile_st.ensure(pl.T + 4);
     w.path_ent.ensure(8 * pl.P + 8);
+    w.ent_quarter.ensure(32 * pl.BG + 32);
     w.path_tot.ensure(8 * pl.P + 8);
     w.bin_items.ensure(sizeof(BinItem) * (pl.BW + pl.BG) + 64);
     w.col_item.ensure(4 * pl.W + 4);
@@ -720,6 +721,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.tile_bd = dump ? w.tile_bd.as<int8_t>() : nullptr;
     b.tile_st = w.tile_st.as<uint8_t>();
     b.path_ent = w.path_ent.as<uint2>();
+    b.ent_quarter = w.ent_quarter.as<uint2>();
     b.path_tot = w.path_tot.as<uint2>();
     b.bin_items = w.bin_items.as<BinItem>();
     b.col_item = w.col_item.as<uint32_t>();
