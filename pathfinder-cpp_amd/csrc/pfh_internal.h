@@ -116,7 +116,7 @@ struct SceneDev {
     uint8_t *out;                 // RGBA8 target
     int64_t out_pitch;
     uint32_t load_dst;            // blend over existing target contents
-    uint32_t pad;
+    float fb_inv_w;               // 1 / fb_w (0 without framebuffer tiles)
 };
 
 // A binned path, resolved once per launch (after the chord scan) so that the per-path

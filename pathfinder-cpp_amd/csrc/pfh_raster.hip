@@ -378,8 +378,12 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     if (f >= b.n_fb) return;
     const int k = find_scene_wave(b.fb_bases, b.n_scenes, f);
     const SceneDev &sc = b.scenes[k];
-    const uint32_t fl = f - sc.fb_base;
-    const int tx = sc.fb_x0 + (int)(fl % (uint32_t)sc.fb_w), ty = sc.fb_y0 + (int)(fl / (uint32_t)sc.fb_w);
+    const uint32_t fl = f - sc.fb_base, fb_w = (uint32_t)sc.fb_w;
+    // fl / fb_w from a float estimate (off by at most one below 2^24 tiles), then corrected
+    uint32_t row = (uint32_t)((float)fl * sc.fb_inv_w);
+    if (row * fb_w > fl) row--;
+    else if ((row + 1) * fb_w <= fl) row++;
+    const int tx = sc.fb_x0 + (int)(fl - row * fb_w), ty = sc.fb_y0 + (int)row;
     // a lane owns 4 vertically adjacent pixels: column lane / 4, rows 4 (lane % 4) .. + 3,
     // matching the column-major coverage masks (one 32-bit mask word per lane)
     const int x = tx * TILE + (lane >> 2), y0 = ty * TILE + (lane & 3) * 4;

@@ -625,6 +625,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.fb_x0 = s.fb_x0;
         d.fb_y0 = s.fb_y0;
         d.fb_w = s.fb_w;
+        d.fb_inv_w = s.fb_w > 0 ? 1.0f / (float)s.fb_w : 0.0f;
         d.fb_h = s.fb_h;
         d.width = s.width;
         d.height = s.height;
@@ -641,7 +642,6 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.out = s.d_out;
         d.out_pitch = s.out_pitch;
         d.load_dst = load_dst ? 1u : 0u;
-        d.pad = 0;
         s.seg_base = d.seg_base;
         s.path_base = d.path_base;
         s.tile_base = d.tile_base;
