@@ -29,7 +29,7 @@ def ctx():
     c.close()
 
 
-def check_scene(ctx, scene, flags=pfh.PFH_PLAIN, name=""):
+def check_scene(ctx, scene, flags=pfh.PFH_PLAIN, name="", guard_hits=0):
     print(f"[parity] {name} paths={scene.n_paths} canvas={scene.width}x{scene.height} flags={flags}", flush=True)
     sid = ctx.upload(scene)
     ctx.rasterize([sid], flags | pfh.PFH_DUMP)
@@ -37,7 +37,7 @@ def check_scene(ctx, scene, flags=pfh.PFH_PLAIN, name=""):
     img = ctx.readback(sid)
     model = oracle.full_model(scene, zmode=1)
     canon = model["canon"]
-    assert got["guard_hits"] == canon["guard_hits"] == 0
+    assert got["guard_hits"] == canon["guard_hits"] == guard_hits
     for k in oracle.CANONICAL_KEYS:
         np.testing.assert_array_equal(got[k], canon[k], err_msg=f"{name}: {k}")
     if got["masks"].size:
@@ -65,6 +65,18 @@ def test_blob_scene_parity(ctx, seed, size, paths):
 def test_star_scene_parity(ctx):
     check_scene(ctx, scenegen.star_scene(seed=5, size=1024, n_stars=40, n_max=600, m_max=30, r_max=400.0),
                 name="stars")
+
+
+@pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED | pfh.PFH_NO_GRAPH])
+def test_negative_zero_direction_chords(ctx, flags):
+    """Chords whose direction has a -0 component: process_line_segment's crossing time is
+    -inf there (x / -0), its sample point NaN, and the walk ends at the loop guard.  The
+    kernels take such chords through the full walk (not the one-tile shortcut) and must
+    reproduce the reference's fills and guard count exactly."""
+    pts = np.array([[0.0, 10.0], [-0.0, 50.0], [30.0, 50.0], [30.0, 0.0], [10.0, -0.0]], np.float32)
+    scene = scenegen.Scene(64, 64, pts, np.zeros(5, np.uint8), np.array([0, 5]), np.ones(1, np.uint8),
+                           np.array([0, 1]), np.zeros(1, np.uint8), np.array([0xff3366cc], np.uint32))
+    check_scene(ctx, scene, flags=flags, name="-0 chords", guard_hits=2)
 
 
 def test_empty_and_degenerate_scenes(ctx):
