@@ -374,10 +374,17 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     __shared__ float4 s_colour[4][64];  // per wave of the block: the chunk's premultiplied colours
     const int lane = threadIdx.x & 63;
     const uint32_t lb = (b.xcd_order & 2u) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint32_t f = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // the tile and its scene are uniform over the wave: held in scalar registers, so the scene
+    // record and the tile's range come through scalar loads
+    const uint32_t f = __builtin_amdgcn_readfirstlane(lb * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (f >= b.n_fb) return;
-    const int k = find_scene_wave(b.fb_bases, b.n_scenes, f);
+    const int k = __builtin_amdgcn_readfirstlane(find_scene_wave(b.fb_bases, b.n_scenes, f));
     const SceneDev &sc = b.scenes[k];
+    // what the store at the end needs, read here (scalar loads) rather than after the blending
+    const int width = sc.width, height = sc.height;
+    uint8_t *const out = sc.out;
+    const int64_t pitch = sc.out_pitch;
+    const bool load_dst = sc.load_dst != 0;
     const uint32_t fl = f - sc.fb_base, fb_w = (uint32_t)sc.fb_w;
     // fl / fb_w from a float estimate (off by at most one below 2^24 tiles), then corrected
     uint32_t row = (uint32_t)((float)fl * sc.fb_inv_w);
@@ -524,17 +531,17 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         }
         if (base == e0) break;
     }
-    if (x >= sc.width) return;
-    uint8_t *const cptr = sc.out + (int64_t)y0 * sc.out_pitch + (int64_t)x * 4;  // this lane's top pixel
+    if (x >= width) return;
+    uint8_t *const cptr = out + (int64_t)y0 * pitch + (int64_t)x * 4;  // this lane's top pixel
     float al[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) al[q] = 1.0f - tr[q];
-    if (sc.load_dst) {
+    if (load_dst) {
         // the destination lies below every entry: it shows through the remaining transmittance
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             uint32_t px = 0;
-            if (y0 + q < sc.height) px = *reinterpret_cast<const uint32_t *>(cptr + q * sc.out_pitch);
+            if (y0 + q < height) px = *reinterpret_cast<const uint32_t *>(cptr + q * pitch);
             const float wq = tr[q] * (1.0f / 255.0f);
             r[q] = fmaf(wq, (float)(px & 255u), r[q]);
             g[q] = fmaf(wq, (float)((px >> 8) & 255u), g[q]);
@@ -554,7 +561,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     }
 #pragma unroll
     for (int q = 0; q < 4; q++)
-        if (y0 + q < sc.height) *reinterpret_cast<uint32_t *>(cptr + q * sc.out_pitch) = px[q];
+        if (y0 + q < height) *reinterpret_cast<uint32_t *>(cptr + q * pitch) = px[q];
 }
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
