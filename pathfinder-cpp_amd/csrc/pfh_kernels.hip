@@ -715,7 +715,7 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
     __shared__ int32_t s_col[4][EMIT ? 1 : BIN_WAVE_COLS];
     __shared__ uint32_t s_queue[4][WALK_QUEUE];
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t item = blockIdx.x * 4 + wv;
+    const uint32_t item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);  // uniform: scalar loads below
     if (item >= b.n_bin_wave) return;  // whole wave: no block-wide barriers below
     const BinItem it = b.bin_items[item];
     const SceneDev &sc = b.scenes[it.k];
@@ -1140,7 +1140,7 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
 
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_entries_wave(BatchDev b) {
-    const uint32_t item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));  // uniform
     if (item >= b.n_bin_wave) return;  // whole wave
     const BinItem it = b.bin_items[item];
     const int4 r = it.rect;
