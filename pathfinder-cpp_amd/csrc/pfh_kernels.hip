@@ -560,11 +560,17 @@ __device__ __forceinline__ void walk_chords(const BatchDev &b, const SceneDev &s
         guard += w.guard;
     };
     uint32_t qn = 0;  // queued chords (wave-uniform)
+    // count pass: the next chunk's chord is loaded while this one is processed (measured: the
+    // count passes gain ~5 %, the emit passes, whose stores also wait, lose ~1 %)
+    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 ln_next = !EMIT && c0 + first < c1 ? b.lines[c0 + first] : zero4;
     for (uint32_t base = c0; base < c1; base += stride) {  // uniform over the wave
         const uint32_t i = base + first;
+        const float4 ln_cur = ln_next;
+        if (!EMIT && base + stride + first < c1) ln_next = b.lines[base + stride + first];
         bool multi = false;
         if (i < c1) {
-            const float4 ln = b.lines[i];
+            const float4 ln = EMIT ? b.lines[i] : ln_cur;
             float2 from = make_float2(ln.x, ln.y), to = make_float2(ln.z, ln.w);
             begin(i);
             if (isfinite(from.x) && isfinite(from.y) && isfinite(to.x) && isfinite(to.y) && w.clip(from, to)) {
