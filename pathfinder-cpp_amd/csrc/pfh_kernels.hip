@@ -1083,22 +1083,6 @@ struct EntryTile {
     uint32_t st, fbl;
 };
 
-// Tile j (row-major) of a path's rect: status and survival of the z-cull.
-__device__ __forceinline__ EntryTile entry_tile(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t tbase,
-                                                int w, float inv_w, int fb_row0, uint32_t j) {
-    // row = j / w from a float estimate (off by at most one below 2^24 tiles), then corrected
-    uint32_t row = (uint32_t)((float)j * inv_w);
-    if (row * (uint32_t)w > j) row--;
-    else if ((row + 1) * (uint32_t)w <= j) row++;
-    const uint32_t x = j - row * (uint32_t)w;
-    EntryTile et;
-    et.st = b.tile_st[tbase + j];
-    et.fbl = (uint32_t)(fb_row0 + (int)row * sc.fb_w + (int)x);
-    const uint32_t z = b.zbuf[sc.fb_base + et.fbl];
-    et.keep = et.st != TILE_EMPTY && (int32_t)p >= (int32_t)z;
-    return et;
-}
-
 // e: the entry's index; ea: its index among alpha entries (for an alpha tile).  The alpha
 // entries' tiles are listed in that order for the fill kernel, which rasterizes only them.
 __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t rgba,
@@ -1130,10 +1114,31 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
     const uint32_t rgba = it.rgba;
     const float inv_w = w > 0 ? 1.0f / (float)w : 0.0f;
     uint32_t e = first.x, n_alpha = first.y;
+    // tile j's status, its framebuffer tile and that tile's z, loaded one chunk ahead; row = j / w
+    // from a float estimate (off by at most one below 2^24 tiles), then corrected
+    auto fetch = [&](uint32_t j, uint32_t &st, uint32_t &z, uint32_t &fbl) {
+        st = TILE_EMPTY;
+        z = 0;
+        fbl = 0;
+        if (j < j1) {
+            uint32_t row = (uint32_t)((float)j * inv_w);
+            if (row * (uint32_t)w > j) row--;
+            else if ((row + 1) * (uint32_t)w <= j) row++;
+            const uint32_t x = j - row * (uint32_t)w;
+            fbl = (uint32_t)(fb_row0 + (int)row * sc.fb_w + (int)x);
+            st = b.tile_st[tbase + j];
+            z = b.zbuf[sc.fb_base + fbl];
+        }
+    };
+    uint32_t n_st, n_z, n_fbl;
+    fetch(j0 + lane, n_st, n_z, n_fbl);
     for (uint32_t jc = j0; jc < j1; jc += 64) {  // uniform over the wave
         const uint32_t j = jc + lane;
-        EntryTile et{false, TILE_EMPTY, 0};
-        if (j < j1) et = entry_tile(b, sc, p, tbase, w, inv_w, fb_row0, j);
+        EntryTile et;
+        et.st = n_st;
+        et.fbl = n_fbl;
+        et.keep = n_st != TILE_EMPTY && (int32_t)p >= (int32_t)n_z;
+        fetch(jc + 64 + lane, n_st, n_z, n_fbl);
         const uint64_t m = __ballot(et.keep), ma = __ballot(et.keep && et.st == TILE_ALPHA);
         if (WRITE && et.keep)
             write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
