@@ -995,7 +995,7 @@ struct StoreEntryOff {
 // rows are taken COL_ROWS at a time: every load of the group is issued before the first
 // is used, and a wave waits on memory once per group instead of once per row.
 
-constexpr int COL_ROWS = 4;
+constexpr int COL_ROWS = 8;
 
 // Where a tile of a path column lands in its scene's framebuffer-tile grid.
 struct ColumnWalk {
