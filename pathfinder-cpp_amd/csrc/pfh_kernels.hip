@@ -263,11 +263,13 @@ struct Walker {
 // exactly where the previously flattened piece ended (splits keep their endpoints
 // bit-for-bit), so only (c0, c1, p1, depth) are stored.  The stack's first
 // FLAT_LDS_LEVELS levels live in LDS, level-major with one float per lane per row so the
-// lanes of a wave hit distinct banks; deeper levels (the benchmark generators never reach
-// them: curves there flatten within depth 6) fall back to a private array.  A private
-// stack alone is placed in scratch, where a divergent wave's narrow stores make the
+// lanes of a wave hit distinct banks; deeper levels (rare: curves of the benchmark
+// generators flatten within depth 6) fall back to a private array.  Four levels leave room
+// for 6 workgroups per CU (6 levels: 4, and both passes 12–15 % slower; 3 levels: the
+// private levels' traffic costs more).  A private stack alone is placed in scratch, where a
+// divergent wave's narrow stores make the
 // cache write back whole lines: several times the chord output in HBM writes.
-constexpr int FLAT_LDS_LEVELS = 6;
+constexpr int FLAT_LDS_LEVELS = 4;
 
 // Lanes are scheduled dynamically within a wave: the wave owns a contiguous range of
 // segments, each iteration every lane takes one depth-first step of its segment, and a lane
