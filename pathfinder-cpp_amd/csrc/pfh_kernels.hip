@@ -69,7 +69,10 @@ __device__ __forceinline__ bool flat_quad(const Seg4 &s) {
 // workgroup owns the whole path, to LDS arrays indexed by the tile's position in the path's
 // rect (LDS = true): l_cnt holds fill counts (COUNT) or the tiles' next free fill slots
 // (EMIT), l_delta the backdrop deltas and l_col the above-rect column backdrops.
-template <bool EMIT, bool LDS = false>
+// PACK: l_delta holds two tiles per word, 16 bits each biased by 0x8000, so that ±1 steps of
+// either half never carry into the other while |sum| < 2^15 (a chord steps a tile at most once:
+// callers use it for paths with fewer than 2^15 chords).
+template <bool EMIT, bool LDS = false, bool PACK = false>
 struct Walker {
     // scene / path state
     float vb_l, vb_r, vb_b;
@@ -149,8 +152,10 @@ struct Walker {
             else atomicAdd(&col_bd[cbase + ox], delta);
             return;
         }
-        if (LDS) atomicAdd(&l_delta[oy * tw + ox], delta);
-        else atomicAdd(&tile_delta[tbase + (uint32_t)(oy * tw + ox)], delta);
+        const uint32_t j = (uint32_t)(oy * tw + ox);
+        if (LDS && PACK) atomicAdd(reinterpret_cast<uint32_t *>(l_delta) + (j >> 1), (uint32_t)delta << (16 * (j & 1)));
+        else if (LDS) atomicAdd(&l_delta[j], delta);
+        else atomicAdd(&tile_delta[tbase + j], delta);
     }
 
     __device__ __forceinline__ static uint32_t outcode(float2 p, float l, float t, float r, float bb) {
@@ -519,7 +524,7 @@ __device__ __forceinline__ void wave_sync() {
 // t >= 1.  Those are emitted at once; the others (crossing a tile boundary) are queued per wave
 // in LDS (wq, WALK_QUEUE slots) and walked 64 at a time, so the DDA's divisions and steps run
 // with every lane on a chord that needs them.
-template <bool EMIT, bool LDS>
+template <bool EMIT, bool LDS, bool PACK = false>
 __device__ __forceinline__ void walk_chords(const BatchDev &b, const SceneDev &sc, const int4 &rect, uint32_t tbase,
                                             uint32_t cbase, uint32_t *l_cnt, int32_t *l_delta, int32_t *l_col,
                                             uint32_t c0, uint32_t c1, uint32_t first, uint32_t stride,
@@ -527,7 +532,7 @@ __device__ __forceinline__ void walk_chords(const BatchDev &b, const SceneDev &s
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint32_t guard = 0;
-    Walker<EMIT, LDS> w;
+    Walker<EMIT, LDS, PACK> w;
     w.vb_l = sc.vb_l;
     w.vb_r = sc.vb_r;
     w.vb_b = sc.vb_b;
@@ -767,7 +772,7 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
     __shared__ uint32_t s_cnt[BIN_GROUP_TILES];
-    __shared__ int32_t s_delta[EMIT ? 1 : BIN_GROUP_TILES];
+    __shared__ uint32_t s_delta[EMIT ? 1 : (BIN_GROUP_TILES + 1) / 2];  // packed, two tiles per word
     __shared__ int32_t s_col[EMIT ? 1 : BIN_GROUP_COLS];
     __shared__ uint32_t s_queue[4][WALK_QUEUE];
     const uint32_t item = blockIdx.x;
@@ -778,30 +783,30 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
     const uint32_t tw = (uint32_t)max(rect.z - rect.x, 0), th = (uint32_t)max(rect.w - rect.y, 0);
     const uint32_t area = tw * th;
     const uint32_t tbase = it.tbase, cbase = it.cbase;
-    const bool lds = area <= (uint32_t)BIN_GROUP_TILES && tw <= (uint32_t)BIN_GROUP_COLS;
     const uint32_t gp = it.gp;
     const uint32_t c0 = it.c0, c1 = it.c1;
+    // the packed LDS deltas hold sums below 2^15 in magnitude: at most one step per chord
+    const bool lds = area <= (uint32_t)BIN_GROUP_TILES && tw <= (uint32_t)BIN_GROUP_COLS &&
+                     (EMIT || c1 - c0 < (1u << 15));
     if (lds) {
         if (EMIT) {
             const uint2 base = b.path_tot[gp];
             block_tile_offsets(b, tbase, area, U2{base.x, base.y}, s_cnt);
         } else {
-            for (uint32_t j = threadIdx.x; j < area; j += 256) {
-                s_cnt[j] = 0;
-                s_delta[j] = 0;
-            }
+            for (uint32_t j = threadIdx.x; j < area; j += 256) s_cnt[j] = 0;
+            for (uint32_t j = threadIdx.x; j < (area + 1) / 2; j += 256) s_delta[j] = 0x80008000u;
             for (uint32_t j = threadIdx.x; j < tw; j += 256) s_col[j] = 0;
         }
         __syncthreads();
-        walk_chords<EMIT, true>(b, sc, rect, tbase, cbase, s_cnt, s_delta, s_col, c0, c1, threadIdx.x, 256,
-                                s_queue[threadIdx.x >> 6]);
+        walk_chords<EMIT, true, true>(b, sc, rect, tbase, cbase, s_cnt, reinterpret_cast<int32_t *>(s_delta), s_col,
+                                      c0, c1, threadIdx.x, 256, s_queue[threadIdx.x >> 6]);
         if (!EMIT) {
             __syncthreads();
             U2 tot{0, 0};
             for (uint32_t j = threadIdx.x; j < area; j += 256) {
                 const uint32_t c = s_cnt[j];
                 b.tile_cnt[tbase + j] = c;
-                b.tile_delta[tbase + j] = s_delta[j];
+                b.tile_delta[tbase + j] = (int32_t)((s_delta[j >> 1] >> (16 * (j & 1))) & 0xffffu) - 0x8000;
                 tot = tot + U2{c, c > 0 ? 1u : 0u};
             }
             for (uint32_t j = threadIdx.x; j < tw; j += 256) b.col_bd[cbase + j] = s_col[j];

@@ -79,6 +79,32 @@ def test_negative_zero_direction_chords(ctx, flags):
     check_scene(ctx, scene, flags=flags, name="-0 chords", guard_hits=2)
 
 
+def polygon_scene(size, polys):
+    """One path per (cx, cy, r, n_vertices, rgba): a closed n-gon of lines."""
+    pts, cstart, pstart = [], [0], [0]
+    for cx, cy, r, n, _ in polys:
+        a = np.arange(n) * (2 * np.pi / n)
+        pts.append(np.stack([cx + r * np.cos(a), cy + r * np.sin(a)], 1))
+        cstart.append(cstart[-1] + n)
+        pstart.append(pstart[-1] + 1)
+    pts = np.concatenate(pts).astype(np.float32)
+    k = len(polys)
+    return scenegen.Scene(size, size, pts, np.zeros(len(pts), np.uint8), np.array(cstart), np.ones(k, np.uint8),
+                          np.array(pstart), np.array([i % 2 for i in range(k)], np.uint8),
+                          np.array([p[4] for p in polys], np.uint32))
+
+
+@pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED | pfh.PFH_NO_GRAPH])
+def test_paths_past_the_lds_limits(ctx, flags):
+    """Paths the group-class kernels cannot bin in LDS: a rect of more than 4 600 tiles (count
+    and emit through global atomics) and a path of more than 2^15 chords (its backdrop sums
+    could overflow the packed 16-bit LDS deltas, so its count pass uses global atomics)."""
+    scene = polygon_scene(2048, [(1024.0, 1024.0, 900.0, 64, 0xc0ff8000),      # ~113 x 113 tiles
+                                 (700.0, 800.0, 300.0, 40000, 0x80336699),      # 40 000 chords
+                                 (1300.0, 1200.0, 200.0, 9, 0xff20c020)])
+    check_scene(ctx, scene, flags=flags, name="past LDS limits")
+
+
 def test_empty_and_degenerate_scenes(ctx):
     empty = scenegen.Scene(64, 48, np.zeros((0, 2), np.float32), np.zeros(0, np.uint8), np.array([0]),
                            np.zeros(0, np.uint8), np.array([0]), np.zeros(0, np.uint8), np.zeros(0, np.uint32))
