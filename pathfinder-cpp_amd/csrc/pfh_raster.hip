@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     // matching the column-major coverage masks (one 32-bit mask word per lane)
     const int x = tx * TILE + (lane >> 2), y0 = ty * TILE + (lane & 3) * 4;
     uint32_t lane_off = (uint32_t)lane * 4u;  // this lane's word in a mask row
-    float r[4], g[4], bl[4], tr[4];  // accumulated premultiplied colour, transmittance
+    float r[4], g[4], bl[4], tr[4];  // accumulated premultiplied colour (byte units), transmittance
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         r[q] = g[q] = bl[q] = 0.0f;
@@ -423,11 +423,11 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         const float ca = (float)(c >> 24) * (1.0f / 255.0f);
         const bool valid = (uint32_t)lane < cnt;
         const bool solid = valid && !masked;
-        // premultiplied colour; a masked entry's is also scaled by 1/255 so that it multiplies
+        // premultiplied colour, the colour channels in byte units (0..255) so that the store
+        // converts them as they are; a masked entry's is also scaled by 1/255 so that it multiplies
         // the raw mask byte.  Runs of solid entries are folded by an inclusive segmented scan.
         const float sc_a = solid ? ca : ca * (1.0f / 255.0f);
-        const float kc = sc_a * (1.0f / 255.0f);
-        float pr = (float)(c & 255u) * kc, pg = (float)((c >> 8) & 255u) * kc, pb = (float)((c >> 16) & 255u) * kc;
+        float pr = (float)(c & 255u) * sc_a, pg = (float)((c >> 8) & 255u) * sc_a, pb = (float)((c >> 16) & 255u) * sc_a;
         float pa = sc_a;
         const uint64_t solid_mask = __ballot(solid);
         // a run starts at a solid lane whose predecessor is not solid
@@ -535,29 +535,28 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     uint8_t *const cptr = out + (int64_t)y0 * pitch + (int64_t)x * 4;  // this lane's top pixel
     float al[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) al[q] = 1.0f - tr[q];
+    for (int q = 0; q < 4; q++) al[q] = fmaf(tr[q], -255.0f, 255.0f);  // byte units, like r, g, bl
     if (load_dst) {
         // the destination lies below every entry: it shows through the remaining transmittance
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             uint32_t px = 0;
             if (y0 + q < height) px = *reinterpret_cast<const uint32_t *>(cptr + q * pitch);
-            const float wq = tr[q] * (1.0f / 255.0f);
-            r[q] = fmaf(wq, (float)(px & 255u), r[q]);
-            g[q] = fmaf(wq, (float)((px >> 8) & 255u), g[q]);
-            bl[q] = fmaf(wq, (float)((px >> 16) & 255u), bl[q]);
-            al[q] = fmaf(wq, (float)(px >> 24), al[q]);
+            r[q] = fmaf(tr[q], (float)(px & 255u), r[q]);
+            g[q] = fmaf(tr[q], (float)((px >> 8) & 255u), g[q]);
+            bl[q] = fmaf(tr[q], (float)((px >> 16) & 255u), bl[q]);
+            al[q] = fmaf(tr[q], (float)(px >> 24), al[q]);
         }
     }
     // v_cvt_pk_u8_f32 rounds to nearest even and saturates to [0, 255] (NaN to 0), i.e. the
-    // byte is rint(clamp(v, 0, 1) * 255), written straight into its place in the word
+    // byte is rint(clamp(v, 0, 255)), written straight into its place in the word
     uint32_t px[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        uint32_t w = __builtin_amdgcn_cvt_pk_u8_f32(r[q] * 255.0f, 0, 0u);
-        w = __builtin_amdgcn_cvt_pk_u8_f32(g[q] * 255.0f, 1, w);
-        w = __builtin_amdgcn_cvt_pk_u8_f32(bl[q] * 255.0f, 2, w);
-        px[q] = __builtin_amdgcn_cvt_pk_u8_f32(al[q] * 255.0f, 3, w);
+        uint32_t w = __builtin_amdgcn_cvt_pk_u8_f32(r[q], 0, 0u);
+        w = __builtin_amdgcn_cvt_pk_u8_f32(g[q], 1, w);
+        w = __builtin_amdgcn_cvt_pk_u8_f32(bl[q], 2, w);
+        px[q] = __builtin_amdgcn_cvt_pk_u8_f32(al[q], 3, w);
     }
 #pragma unroll
     for (int q = 0; q < 4; q++)
