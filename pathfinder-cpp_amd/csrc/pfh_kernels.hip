@@ -1087,7 +1087,7 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
 
 struct EntryTile {
     bool keep;
-    uint32_t st, fbl;
+    uint32_t st, fbl, ai;  // ai: the alpha index of an alpha tile (write pass)
 };
 
 // e: the entry's index; ea: its index among alpha entries (for an alpha tile).  The alpha
@@ -1096,7 +1096,7 @@ __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &s
                                             uint32_t tbase, uint32_t j, const EntryTile &et, uint32_t e,
                                             uint32_t ea) {
     if (e >= b.cap_entries) return;  // past the capacity: the batch is retried
-    const uint32_t ai = et.st == TILE_ALPHA ? b.tile_alpha_idx[tbase + j] : NO_ALPHA;
+    const uint32_t ai = et.st == TILE_ALPHA ? et.ai : NO_ALPHA;
     if (ai != NO_ALPHA && ea < b.cap_alpha) b.alpha_vis[ea] = ai;
     // even-odd solid tile with an even winding covers nothing: zero alpha
     const uint32_t eff = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
@@ -1139,11 +1139,14 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
     };
     uint32_t n_st, n_z, n_fbl;
     fetch(j0 + lane, n_st, n_z, n_fbl);
+    // write pass: a chunk's alpha indices are loaded at the end of the chunk before it
+    uint32_t n_ai = WRITE && n_st == TILE_ALPHA ? b.tile_alpha_idx[tbase + j0 + lane] : NO_ALPHA;
     for (uint32_t jc = j0; jc < j1; jc += 64) {  // uniform over the wave
         const uint32_t j = jc + lane;
         EntryTile et;
         et.st = n_st;
         et.fbl = n_fbl;
+        et.ai = n_ai;
         et.keep = n_st != TILE_EMPTY && (int32_t)p >= (int32_t)n_z;
         fetch(jc + 64 + lane, n_st, n_z, n_fbl);
         const uint64_t m = __ballot(et.keep), ma = __ballot(et.keep && et.st == TILE_ALPHA);
@@ -1152,6 +1155,7 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
                         n_alpha + (uint32_t)__popcll(ma & lt_mask));
         e += (uint32_t)__popcll(m);
         n_alpha += (uint32_t)__popcll(ma);
+        if (WRITE) n_ai = n_st == TILE_ALPHA ? b.tile_alpha_idx[tbase + jc + 64 + lane] : NO_ALPHA;
     }
     return make_uint2(e, n_alpha);
 }
