@@ -26,6 +26,11 @@ constexpr int BIN_GROUP_COLS = 512;
 constexpr int WALK_QUEUE = 128;          // per wave: chords that cross tile boundaries, waiting for a full wave
 
 enum SegKind : uint32_t { SEG_LINE = 1, SEG_QUAD = 2, SEG_CUBIC = 3 };
+// Device segment word (.y): kind in the low byte (0: emits nothing), SEG_CHAIN_BREAK when the
+// segment does not start where the previous emitting segment of its path ended (bit for bit):
+// its chord points are then preceded by a NaN separator and its start point.
+constexpr uint32_t SEG_KIND_MASK = 0xffu;
+constexpr uint32_t SEG_CHAIN_BREAK = 0x100u;
 
 // Launch schedule (chosen by pfh_autotune; every schedule gives bit-identical results).
 struct Schedule {
@@ -157,7 +162,10 @@ struct BatchDev {
     uint32_t *col_item;           // [W] the bin item owning each column
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
-    float4 *lines;                // [L] flattened chords (from, to) in emission order
+    // [L] flattened chords as a point stream, in emission order: per chain of chords a NaN
+    // separator and the start point, then each chord's end point; chord i of a path's range
+    // [c0, c1) runs from point i to point i + 1 (a pair with a NaN end emits nothing)
+    float2 *lines;
     uint32_t *line_cnt;           // [L + 1] fills per chord, then exclusive scan (emission order)
     uint2 *fills;                 // [F] grouped by tile: (x0 | y0 << 16, x1 | y1 << 16)
     uint4 *dump_fills;            // [F] emission order: (x0|y0<<16, x1|y1<<16, global tile, 0)

@@ -302,6 +302,7 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
     uint32_t qkind = 0, qbase = 0;              // its kind (0: emits nothing) and chord offset
     Seg4 s;
     uint32_t g = 0, kind = 0, n = 0, base = 0, guard = 0;
+    bool brk = false;  // this segment starts a chord chain (a separator and its start point first)
     int d = 0, sp = 0;
     bool active = false;
     while (true) {
@@ -322,12 +323,13 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
                     qbase = EMIT ? b.seg_cnt[gl] : 0u;
                     // kind 0: a segment of a path without tile columns, from which nothing can land
                     // (with columns but no rows, above-rect backdrops still do: those are kept)
-                    if (sg.y != 0) {
+                    const uint32_t sk = sg.y & SEG_KIND_MASK;
+                    if (sk != 0) {
                         const float2 *P = sc.pts + sg.x;
                         q0 = P[0];
-                        if (sg.y == SEG_LINE) {
+                        if (sk == SEG_LINE) {
                             q3 = P[1];
-                        } else if (sg.y == SEG_QUAD) {
+                        } else if (sk == SEG_QUAD) {
                             q1 = P[1];
                             q2 = P[1];
                             q3 = P[2];
@@ -339,9 +341,9 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
                         // A non-finite point makes every flattened piece non-finite, and
                         // process_line_segment drops those (tiler.cpp:133-136): nothing from it.
                         bool finite = isfinite(q0.x) && isfinite(q0.y) && isfinite(q3.x) && isfinite(q3.y);
-                        if (sg.y != SEG_LINE) finite = finite && isfinite(q1.x) && isfinite(q1.y);
-                        if (sg.y == SEG_CUBIC) finite = finite && isfinite(q2.x) && isfinite(q2.y);
-                        if (finite) qkind = sg.y;
+                        if (sk != SEG_LINE) finite = finite && isfinite(q1.x) && isfinite(q1.y);
+                        if (sk == SEG_CUBIC) finite = finite && isfinite(q2.x) && isfinite(q2.y);
+                        if (finite) qkind = sg.y;  // kind and chain-break flag
                     }
                 }
                 k = __shfl(kl, (int)(batch_end - 1 - batch));
@@ -359,8 +361,14 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
             if (!active && cand < batch_end) {
                 g = cand;
                 s = Seg4{t0, t1, t2, t3};
-                kind = tk;
+                kind = tk & SEG_KIND_MASK;
+                brk = kind != 0 && (tk & SEG_CHAIN_BREAK) != 0;
                 base = tb;
+                if (EMIT && brk) {  // the chain's separator (a NaN point: its chords emit nothing) and start
+                    if (base < b.cap_lines) b.lines[base] = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+                    if (base + 1 < b.cap_lines) b.lines[base + 1] = s.p0;
+                    base += 2;
+                }
                 n = 0;
                 d = 0;
                 sp = 0;
@@ -373,7 +381,7 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
         if (kind == 0) {
             done = true;
         } else if (kind == SEG_LINE) {
-            if (EMIT && base < b.cap_lines) b.lines[base] = make_float4(s.p0.x, s.p0.y, s.p1.x, s.p1.y);
+            if (EMIT && base < b.cap_lines) b.lines[base] = s.p1;
             n = 1;
             done = true;
         } else {
@@ -383,7 +391,7 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
             const bool flat = quad ? flat_quad(s) : flat_cubic(s);
             if (flat || d >= MAX_FLATTEN_DEPTH) {
                 if (!flat) guard++;
-                if (EMIT && base + n < b.cap_lines) b.lines[base + n] = make_float4(s.p0.x, s.p0.y, s.p1.x, s.p1.y);
+                if (EMIT && base + n < b.cap_lines) b.lines[base + n] = s.p1;
                 n++;
                 if (sp == 0) {
                     done = true;
@@ -438,7 +446,7 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
             }
         }
         if (done) {
-            if (!EMIT) b.seg_cnt[g] = n;
+            if (!EMIT) b.seg_cnt[g] = n + (brk ? 2u : 0u);  // stream points: chords (+ separator, start)
             active = false;
         }
     }
@@ -567,17 +575,24 @@ __device__ __forceinline__ void walk_chords(const BatchDev &b, const SceneDev &s
         guard += w.guard;
     };
     uint32_t qn = 0;  // queued chords (wave-uniform)
+    // chord i: stream points i and i + 1 of the path's range (the last point pairs with a NaN:
+    // nothing is emitted, and the dump's per-chord count there is 0)
+    auto chord = [&](uint32_t i) {
+        const float2 a = b.lines[i];
+        const float2 e = i + 1 < c1 ? b.lines[i + 1] : make_float2(__builtin_nanf(""), __builtin_nanf(""));
+        return make_float4(a.x, a.y, e.x, e.y);
+    };
     // count pass: the next chunk's chord is loaded while this one is processed (measured: the
     // count passes gain ~5 %, the emit passes, whose stores also wait, lose ~1 %)
     const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float4 ln_next = !EMIT && c0 + first < c1 ? b.lines[c0 + first] : zero4;
+    float4 ln_next = !EMIT && c0 + first < c1 ? chord(c0 + first) : zero4;
     for (uint32_t base = c0; base < c1; base += stride) {  // uniform over the wave
         const uint32_t i = base + first;
         const float4 ln_cur = ln_next;
-        if (!EMIT && base + stride + first < c1) ln_next = b.lines[base + stride + first];
+        if (!EMIT && base + stride + first < c1) ln_next = chord(base + stride + first);
         bool multi = false;
         if (i < c1) {
-            const float4 ln = EMIT ? b.lines[i] : ln_cur;
+            const float4 ln = EMIT ? chord(i) : ln_cur;
             float2 from = make_float2(ln.x, ln.y), to = make_float2(ln.z, ln.w);
             begin(i);
             if (isfinite(from.x) && isfinite(from.y) && isfinite(to.x) && isfinite(to.y) && w.clip(from, to)) {
@@ -600,7 +615,7 @@ __device__ __forceinline__ void walk_chords(const BatchDev &b, const SceneDev &s
             qn -= 64;
             const uint32_t ci = wq[qn + lane];
             wave_sync();  // read before the next chunk's chords overwrite the slots
-            const float4 ln = b.lines[ci];
+            const float4 ln = chord(ci);
             begin(ci);
             w.line(make_float2(ln.x, ln.y), make_float2(ln.z, ln.w));
             end(ci);
@@ -608,7 +623,7 @@ __device__ __forceinline__ void walk_chords(const BatchDev &b, const SceneDev &s
     }
     if (lane < qn) {  // the rest of the queue
         const uint32_t ci = wq[lane];
-        const float4 ln = b.lines[ci];
+        const float4 ln = chord(ci);
         begin(ci);
         w.line(make_float2(ln.x, ln.y), make_float2(ln.z, ln.w));
         end(ci);

@@ -115,6 +115,8 @@ struct SceneHost {
     std::vector<int4> path_rect;
     std::vector<uint32_t> path_tile_off, path_col_off, path_seg_off, path_rgba;
     std::vector<uint8_t> path_rule;
+    std::vector<uint32_t> path_breaks;  // chord chains per path (two extra stream points each)
+    uint64_t n_breaks = 0;
     std::vector<float> features;
     DevBuf d_features;
     DevBuf d_pts, d_segs, d_seg_path, d_path_rect, d_path_tile_off, d_path_col_off, d_col_path, d_path_rgba,
@@ -454,11 +456,36 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
 
     // On the device, segments of paths without tile columns carry kind 0: nothing can land
     // from them (add_fill culls, adjust_alpha_tile_backdrop ignores x outside [0, width)), so
-    // the flattener skips them without looking up their path.
+    // the flattener skips them without looking up their path.  So do segments with a
+    // non-finite point (process_line_segment drops every piece of them, tiler.cpp:133-136).
+    // A segment that does not continue the previous emitting segment of its path bit for bit
+    // is marked SEG_CHAIN_BREAK: its chord points start with a separator and its start point.
     std::vector<uint2> segs_dev = segs;
+    s->path_breaks.assign(d->n_paths, 0);
+    s->n_breaks = 0;
+    auto seg_np = [](uint32_t k) { return k == SEG_LINE ? 2u : (k == SEG_QUAD ? 3u : 4u); };
+    auto bits = [](float2 v) { uint64_t u; std::memcpy(&u, &v, 8); return u; };
     for (size_t i = 0; i < segs_dev.size(); i++) {
-        const int4 &r = s->path_rect[seg_path[i]];
-        if (r.z - r.x <= 0) segs_dev[i].y = 0;
+        const uint32_t p = seg_path[i];
+        const int4 &r = s->path_rect[p];
+        const uint32_t np = seg_np(segs[i].y);
+        bool finite = true;
+        for (uint32_t q = 0; q < np; q++) finite = finite && std::isfinite(pts[segs[i].x + q].x) && std::isfinite(pts[segs[i].x + q].y);
+        if (r.z - r.x <= 0 || !finite) {
+            segs_dev[i].y = 0;
+            continue;
+        }
+        bool chained = false;
+        for (size_t j = i; j-- > 0 && seg_path[j] == p;) {  // the previous emitting segment of the path
+            if (segs_dev[j].y == 0) break;
+            chained = bits(pts[segs[j].x + seg_np(segs[j].y) - 1]) == bits(pts[segs[i].x]);
+            break;
+        }
+        if (!chained) {
+            segs_dev[i].y |= SEG_CHAIN_BREAK;
+            s->path_breaks[p]++;
+            s->n_breaks++;
+        }
     }
     HIP_TRY(hipSetDevice(ctx->device));
     upload(s->d_features, s->features);
@@ -517,6 +544,7 @@ static void predict_scene(pfh_ctx *ctx, SceneHost &s) {
 struct Plan {
     std::vector<int32_t> ids;
     uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0, P = 0;
+    uint64_t breaks = 0;  // chord chains of the batch (two stream points each besides the chords)
     uint32_t cap_f = 0, cap_a = 0, cap_e = 0, cap_l = 0;
     int key_bits = 1;
     Schedule sched;
@@ -546,7 +574,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.col_item.ensure(4 * pl.W + 4);
     w.col_bd.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
-    w.lines.ensure(16ull * pl.cap_l);
+    w.lines.ensure(8ull * pl.cap_l);
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
     w.zbuf.ensure(4 * pl.FB + 4);
     w.fb_range.ensure(8 * pl.FB + 8);
@@ -576,7 +604,7 @@ static Caps held_caps(Workspace &w, bool dump) {
     auto fit = [](size_t bytes, size_t per, size_t extra) -> uint64_t {
         return bytes > extra ? (bytes - extra) / per : 0;
     };
-    uint64_t l = std::min({fit(w.lines.bytes, 16, 0), fit(w.line_cnt.bytes, 4, 8)});
+    uint64_t l = std::min({fit(w.lines.bytes, 8, 0), fit(w.line_cnt.bytes, 4, 8)});
     uint64_t f = fit(w.fills.bytes, 8, 0);
     if (dump) f = std::min(f, fit(w.dump_fills.bytes, 16, 0));
     uint64_t a = std::min({fit(w.alpha_rec.bytes, 16, 0), fit(w.alpha_vis.bytes, 4, 0), fit(w.masks.bytes, 256, 256)});
@@ -654,6 +682,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         ctx->w->h_bases[4 * (n + 1) + k] = (uint32_t)BW;
         ctx->w->h_bases[5 * (n + 1) + k] = (uint32_t)BG;
         S += s.n_segs;
+        pl.breaks += s.n_breaks;
         T += s.n_tiles;
         W += s.n_cols;
         FB += d.n_fb;
@@ -728,7 +757,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.n_paths = (uint32_t)pl.P;
     b.col_bd = w.col_bd.as<int32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
-    b.lines = w.lines.as<float4>();
+    b.lines = w.lines.as<float2>();
     b.line_cnt = w.line_cnt.as<uint32_t>();
     b.fills = w.fills.as<uint2>();
     b.dump_fills = dump ? w.dump_fills.as<uint4>() : nullptr;
@@ -822,7 +851,8 @@ static void record_counts(pfh_ctx *ctx, const Plan &pl, bool dump) {
     ctx->stats.entries = ctx->w->h_counters[C_ENTRIES];
     ctx->stats.guard_hits = ctx->w->h_counters[C_GUARD];
     ctx->stats.alpha_entries = ctx->w->h_counters[C_ALPHA_ENTRIES];
-    ctx->stats.lines = ctx->w->h_counters[C_LINES];
+    // C_LINES counts stream points: two per chord chain besides one per chord
+    ctx->stats.lines = (int64_t)ctx->w->h_counters[C_LINES] - 2 * (int64_t)pl.breaks;
     ctx->stats.capacity_lines = pl.cap_l;
     ctx->stats.tiles = (int64_t)pl.T;
     ctx->stats.columns = (int64_t)pl.W;
@@ -937,7 +967,7 @@ static void batched_prepare(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint32_
     for (int32_t id : pl.ids) {
         SceneHost &s = get_scene(ctx, id);
         if ((flags & PFH_PREDICT) && !ctx->predictor.empty()) {
-            bl.plc += (uint64_t)std::ceil(s.pred_lines * ctx->predictor_safety) + 64;
+            bl.plc += (uint64_t)std::ceil(s.pred_lines * ctx->predictor_safety) + 2 * s.n_breaks + 64;
             bl.pf += (uint64_t)std::ceil(s.pred_fills * ctx->predictor_safety) + 64;
             bl.pa += (uint64_t)std::ceil(s.pred_alpha * ctx->predictor_safety) + 16;
         } else if (s.counted) {
@@ -1139,7 +1169,7 @@ struct Server {
 static uint64_t scene_launch_bytes(const pfh_ctx *ctx, const SceneHost &s) {
     double L, F, A;
     if (!ctx->predictor.empty() && s.pred_lines >= 0) {
-        L = s.pred_lines * ctx->predictor_safety;
+        L = s.pred_lines * ctx->predictor_safety + 2.0 * s.n_breaks;  // stream points
         F = s.pred_fills * ctx->predictor_safety;
         A = s.pred_alpha * ctx->predictor_safety;
     } else if (s.counted) {
@@ -1153,7 +1183,7 @@ static uint64_t scene_launch_bytes(const pfh_ctx *ctx, const SceneHost &s) {
     }
     const double T = s.n_tiles, W = s.n_cols, S = s.n_segs, FB = (double)s.fb_w * s.fb_h;
     const double per_tile = 4 + 4 + 8 + 1 + 4 * 4 + 8;  // counts, offsets, status, sort, records
-    double b = T * per_tile + 4 * W + 16 * s.n_paths + 8 * S + 12 * FB + 24 * L + 8 * F + (16 + 4 + 256) * A;
+    double b = T * per_tile + 4 * W + 16 * s.n_paths + 8 * S + 12 * FB + 12 * L + 8 * F + (16 + 4 + 256) * A;
     return (uint64_t)b + 4096;
 }
 
@@ -1593,7 +1623,7 @@ int pfh_path_counts(pfh_ctx *ctx, int32_t scene_id, int32_t *lines, int32_t *fil
                           hipMemcpyDeviceToHost));
         for (uint32_t p = 0; p < s.n_paths; p++) {
             uint32_t a = seg_line[s.path_seg_off[p]] - l0, e = seg_line[s.path_seg_off[p + 1]] - l0;
-            lines[p] = (int32_t)(e - a);
+            lines[p] = (int32_t)(e - a - 2 * s.path_breaks[p]);  // chords: the stream's points less two per chain
             fills[p] = (int32_t)(line_fill[e] - line_fill[a]);
             int32_t na = 0;
             for (uint32_t t = s.path_tile_off[p]; t < s.path_tile_off[p + 1]; t++) na += fill_off[t + 1] > fill_off[t];
