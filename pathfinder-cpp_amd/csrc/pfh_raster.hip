@@ -368,7 +368,7 @@ __device__ __forceinline__ void over_step(bool has_src, float &pr, float &pg, fl
 }
 
 
-constexpr int EV_GROUP = 3;  // blend events whose mask words are loaded together
+constexpr int EV_GROUP = 4;  // blend events whose mask words are loaded together
 
 __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *sorted_vals) {
     __shared__ float4 s_colour[4][64];  // per wave of the block: the chunk's premultiplied colours
