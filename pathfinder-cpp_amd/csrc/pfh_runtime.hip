@@ -1810,7 +1810,7 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         Schedule best = ctx->sched;
         double best_ms = measure();
         if (ms_start) *ms_start = best_ms;
-        const std::vector<uint32_t> cand[5] = {{1024, 2048, 4096, 8192, 16384, 32768}, {64, 128, 256}, {64, 128, 256},
+        const std::vector<uint32_t> cand[5] = {{1024, 2048, 4096, 8192, 16384, 32768, 65536}, {64, 128, 256}, {64, 128, 256},
                                               {0, 1, 2, 3}, {0, 64, 128, 256, 512, 1024, 2048}};
         for (int param = 0; param < 5; param++) {
             for (uint32_t v : cand[param]) {
