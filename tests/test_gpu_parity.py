@@ -105,6 +105,25 @@ def test_paths_past_the_lds_limits(ctx, flags):
     check_scene(ctx, scene, flags=flags, name="past LDS limits")
 
 
+@pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED | pfh.PFH_NO_GRAPH])
+def test_non_finite_points_and_chord_chains(ctx, flags):
+    """Segments with a NaN or infinite point emit nothing (process_line_segment drops every
+    piece), so the chord chain of their contour restarts after them; a contour that starts
+    where the previous one ended continues the chain.  Fills must match the reference's."""
+    nan, inf = np.nan, np.inf
+    pts = np.array([[10, 10], [60, 12], [nan, 40], [70, 70], [15, 60],   # a NaN mid-contour
+                    [15, 60], [40, 90], [5, 100],                        # starts where the last ended
+                    [80, 20], [120, 10], [130, 60], [90, 70],             # a cubic
+                    [100, 100], [120, 110], [90, 120],
+                    [20, 110], [inf, 115], [40, 125]], np.float32)       # an infinite point
+    flags_pt = np.zeros(len(pts), np.uint8)
+    flags_pt[9], flags_pt[10] = 1, 2
+    scene = scenegen.Scene(128, 128, pts, flags_pt, np.array([0, 5, 8, 12, 15, 18]), np.ones(5, np.uint8),
+                           np.array([0, 2, 4, 5]), np.array([0, 1, 0], np.uint8),
+                           np.array([0xff2040e0, 0xa0e04020, 0xc020c040], np.uint32))
+    check_scene(ctx, scene, flags=flags, name="non-finite points")
+
+
 def test_empty_and_degenerate_scenes(ctx):
     empty = scenegen.Scene(64, 48, np.zeros((0, 2), np.float32), np.zeros(0, np.uint8), np.array([0]),
                            np.zeros(0, np.uint8), np.array([0]), np.zeros(0, np.uint8), np.zeros(0, np.uint32))
