@@ -67,6 +67,15 @@ typedef struct pfh_scene_desc {
     const int32_t *path_contour_start;  /* [n_paths + 1] */
     const uint8_t *path_fill_rule;      /* [n_paths] PFH_FILL_* */
     const uint32_t *path_rgba;          /* [n_paths] */
+    /* Clip paths (Scene::clip_paths, core/scene.h:53-130; ClipPath::outline): the last n_clip_paths
+     * paths of the arrays above are clip paths, tiled like draw paths but never drawn.  path_clip[p]
+     * (NULL: no path is clipped) is the clip path of draw path p (DrawPath::clip_path) as an index
+     * among the clip paths, or -1; clip paths themselves take -1 (nested clips are undefined in the
+     * reference's hybrid tiler, core/d3d9/scene_builder.cpp:209-216, and rejected).  Resolution and
+     * mask combine follow core/d3d9/tiler.cpp:385-431 and shaders/d3d9/tile_clip_combine.frag:25-27:
+     * the clip's own fill rule is not used (its coverage enters as |winding sum|). */
+    int32_t n_clip_paths;
+    const int32_t *path_clip;           /* [n_paths] or NULL */
 } pfh_scene_desc;
 
 typedef struct pfh_stats {

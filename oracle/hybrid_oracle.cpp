@@ -67,6 +67,10 @@ struct PathTiling {
     std::vector<OFill> fills;       // emission order
     uint32_t lines = 0;             // chords handed to process_line_segment
     uint32_t guard_hits = 0;        // loop guards tripped (reference would not terminate)
+    // Clip resolution of a draw path with a clip path (tiler.cpp:385-431), per tile:
+    std::vector<uint8_t> combine;   // 1: both tiles have masks; the clip combine job applies comb_bd
+    std::vector<int8_t> comb_bd;    // the draw tile's own backdrop, handed to the combine job
+    std::vector<int64_t> clip_src;  // global tile index of the clip tile whose mask is used, or -1
 };
 
 struct Scene {
@@ -423,7 +427,44 @@ static void tile_path(const Scene &sc, int p, PathTiling &pt) {
     bld.prepare_tiles();
 }
 
+// core/d3d9/tiler.cpp:385-431 (the clip part of prepare_tiles), applied after the draw path's
+// own propagation, which it does not change: the column sums add the tile deltas read before it.
+// The clip tile at the same tile coordinates decides: both masks -> a combine job (the draw tile's
+// backdrop moves into the job, the tile keeps its mask); a solid draw tile over a clip mask -> the
+// tile takes the clip's mask and backdrop; a blank clip tile, or none, -> the tile is culled.
+static void resolve_clip(PathTiling &pt, const PathTiling &cp, int64_t clip_tile_base) {
+    int w = std::max(pt.rect[2] - pt.rect[0], 0), h = std::max(pt.rect[3] - pt.rect[1], 0);
+    int cw = std::max(cp.rect[2] - cp.rect[0], 0), ch = std::max(cp.rect[3] - cp.rect[1], 0);
+    pt.combine.assign((size_t)w * h, 0);
+    pt.comb_bd.assign((size_t)w * h, 0);
+    pt.clip_src.assign((size_t)w * h, -1);
+    for (int i = 0; i < w * h; i++) {
+        int cx = pt.rect[0] + i % w - cp.rect[0], cy = pt.rect[1] + i / w - cp.rect[1];
+        if (cx < 0 || cy < 0 || cx >= cw || cy >= ch) {  // outside the clip path's rect
+            pt.alpha[i] = 0;
+            pt.backdrop[i] = 0;
+            continue;
+        }
+        int ci = cy * cw + cx;
+        bool clip_alpha = cp.alpha[ci] != 0, draw_alpha = pt.alpha[i] != 0;
+        if (clip_alpha && draw_alpha) {
+            pt.combine[i] = 1;
+            pt.comb_bd[i] = pt.backdrop[i];
+            pt.clip_src[i] = clip_tile_base + ci;
+            pt.backdrop[i] = 0;
+        } else if (clip_alpha && pt.backdrop[i] != 0) {
+            pt.alpha[i] = 1;
+            pt.backdrop[i] = cp.backdrop[ci];
+            pt.clip_src[i] = clip_tile_base + ci;
+        } else if (!clip_alpha && cp.backdrop[ci] == 0) {
+            pt.alpha[i] = 0;
+            pt.backdrop[i] = 0;
+        }
+    }
+}
+
 struct Result {
+    int n_draw = 0;
     std::vector<PathTiling> paths;
     int32_t fb_rect[4];
     std::vector<uint32_t> zbuf;
@@ -507,15 +548,29 @@ double pfo_apply_rule(double c, int even_odd) {
 // (core/d3d9/scene_builder.cpp:57-79: any opaque solid tile with backdrop != 0
 // occludes, even an even-odd tile with even backdrop); zmode 1 = the fixed rule
 // (such even-odd tiles cover nothing and never occlude, shaders/d3d11/propagate.comp:190-198).
+//
+// Clip paths (Scene::clip_paths, core/scene.h) are the last n_clip paths; path_clip[p] (NULL: none)
+// names the clip path of draw path p by its index among them, or -1.  Clip paths are tiled like draw
+// paths (core/d3d9/scene_builder.cpp:200-225) but draw nothing themselves.
 void *pfo_run(int n_paths, const int32_t *path_contour_start, const uint8_t *path_fill_rule,
               const uint8_t *path_opaque, const int32_t *contour_start, const uint8_t *contour_closed,
-              const float *points, const uint8_t *point_flags, const float *view_box, int zmode) {
+              const float *points, const uint8_t *point_flags, const float *view_box, int zmode, int n_clip,
+              const int32_t *path_clip) {
     auto *res = new Result();
     Scene sc{n_paths, path_contour_start, path_fill_rule, contour_start, contour_closed, points, point_flags,
              scene_view_box(view_box)};
     res->paths.resize(n_paths);
+    const int n_draw = n_paths - n_clip;
+    res->n_draw = n_draw;
     auto t0 = std::chrono::steady_clock::now();
     for (int p = 0; p < n_paths; p++) tile_path(sc, p, res->paths[p]);
+    std::vector<int64_t> tile_base(n_paths + 1, 0);
+    for (int p = 0; p < n_paths; p++) tile_base[p + 1] = tile_base[p] + (int64_t)res->paths[p].backdrop.size();
+    for (int p = 0; path_clip && p < n_draw; p++) {
+        if (path_clip[p] < 0) continue;
+        int g = n_draw + path_clip[p];
+        resolve_clip(res->paths[p], res->paths[g], tile_base[g]);
+    }
     res->tiling_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
     // core/d3d9/scene_builder.cpp:18-95 — one batch (solid paints only), z over the view box.
@@ -527,8 +582,7 @@ void *pfo_run(int n_paths, const int32_t *path_contour_start, const uint8_t *pat
     int fw = res->fb_rect[2] - res->fb_rect[0], fh = res->fb_rect[3] - res->fb_rect[1];
     res->zbuf.assign((size_t)fw * fh, 0);
     std::vector<std::vector<std::pair<uint32_t, int64_t>>> lists((size_t)fw * fh);
-    int64_t tile_base = 0;
-    for (int p = 0; p < n_paths; p++) {
+    for (int p = 0; p < n_draw; p++) {
         PathTiling &pt = res->paths[p];
         int w = pt.rect[2] - pt.rect[0], h = pt.rect[3] - pt.rect[1];
         if (w < 0) w = 0;
@@ -537,12 +591,11 @@ void *pfo_run(int n_paths, const int32_t *path_contour_start, const uint8_t *pat
             int tx = pt.rect[0] + i % w, ty = pt.rect[1] + i / w;
             if (!pt.alpha[i] && pt.backdrop[i] == 0) continue;
             int64_t fb = (int64_t)(ty - res->fb_rect[1]) * fw + (tx - res->fb_rect[0]);
-            lists[fb].push_back({(uint32_t)p, tile_base + i});
+            lists[fb].push_back({(uint32_t)p, tile_base[p] + i});
             if (!path_opaque[p] || pt.alpha[i]) continue;
             if (zmode == 1 && path_fill_rule[p] && (pt.backdrop[i] % 2) == 0) continue;
             res->zbuf[fb] = std::max(res->zbuf[fb], (uint32_t)p);
         }
-        tile_base += (int64_t)w * h;
     }
     for (size_t fb = 0; fb < lists.size(); fb++) {
         for (auto &e : lists[fb]) {
@@ -628,6 +681,28 @@ void pfo_fetch(void *h, int32_t *path_rect, int64_t *tile_off, int64_t *col_off,
     }
     if (!r->zbuf.empty()) memcpy(zbuf, r->zbuf.data(), r->zbuf.size() * 4);
     memcpy(fb_rect, r->fb_rect, 16);
+}
+
+// Per global tile: clip combine flag, the backdrop handed to the combine job, and the clip
+// tile whose mask a draw tile uses (-1 where none).
+void pfo_fetch_clip(void *h, uint8_t *combine, int8_t *comb_bd, int64_t *clip_src) {
+    auto *r = (Result *)h;
+    int64_t t = 0;
+    for (auto &pt : r->paths) {
+        size_t n = pt.backdrop.size();
+        for (size_t i = 0; i < n; i++) {
+            bool has = !pt.combine.empty();
+            combine[t + i] = has ? pt.combine[i] : 0;
+            comb_bd[t + i] = has ? pt.comb_bd[i] : 0;
+            clip_src[t + i] = has ? pt.clip_src[i] : -1;
+        }
+        t += (int64_t)n;
+    }
+}
+
+// out[i] = the fill rule even_odd[i] applied to c[i].
+void pfo_apply_rule_array(int64_t n, const double *c, const uint8_t *even_odd, double *out) {
+    for (int64_t i = 0; i < n; i++) out[i] = apply_rule(c[i], even_odd[i]);
 }
 
 void pfo_free(void *h) {

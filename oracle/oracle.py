@@ -42,7 +42,9 @@ def load_oracle():
             build()
         lib = ctypes.CDLL(ORACLE_SO)
         lib.pfo_run.restype = _P
-        lib.pfo_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int]
+        lib.pfo_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, ctypes.c_int, _P]
+        lib.pfo_fetch_clip.argtypes = [_P] * 4
+        lib.pfo_apply_rule_array.argtypes = [ctypes.c_int64, _P, _P, _P]
         lib.pfo_counts.argtypes = [_P, _P]
         lib.pfo_fetch.argtypes = [_P] * 17
         lib.pfo_free.argtypes = [_P]
@@ -69,7 +71,8 @@ def load_ref():
     if _ref is None:
         lib = ctypes.CDLL(REF_SO)
         lib.pfref_run.restype = _P
-        lib.pfref_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, _P]
+        lib.pfref_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, _P, ctypes.c_int, _P]
+        lib.pfref_fetch_clip.argtypes = [_P] * 3
         lib.pfref_counts.argtypes = [_P, _P]
         lib.pfref_fetch.argtypes = [_P] * 14
         lib.pfref_free.argtypes = [_P]
@@ -83,6 +86,10 @@ def _scene_args(scene):
     return [ctypes.c_int(scene.n_paths), _ptr(scene.path_contour_start), _ptr(scene.path_fill_rule),
             _ptr(scene.path_opaque), _ptr(scene.contour_start), _ptr(scene.contour_closed),
             _ptr(scene.points), _ptr(scene.point_flags), _ptr(scene.view_box)]
+
+
+def _clip_args(scene):
+    return [ctypes.c_int(scene.n_clip_paths), _ptr(scene.path_clip)]
 
 
 def _alloc(counts):
@@ -106,7 +113,7 @@ def run_oracle(scene, zmode: int = 1) -> dict:
     zmode 0: the reference's z-buffer rule; 1: even-odd tiles with an even
     backdrop never occlude (the rule the HIP path implements)."""
     lib = load_oracle()
-    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(zmode))
+    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(zmode), *_clip_args(scene))
     counts = np.zeros(8, np.int64)
     lib.pfo_counts(h, _ptr(counts))
     out = _alloc(counts)
@@ -116,6 +123,10 @@ def run_oracle(scene, zmode: int = 1) -> dict:
     out["entry_tile"] = np.zeros(E, np.int64)
     lib.pfo_fetch(h, *[_ptr(out[k]) for k in CANONICAL_KEYS], _ptr(out["tile_delta"]), _ptr(out["col_above"]),
                   _ptr(out["entry_tile"]))
+    out["clip_combine"] = np.zeros(T, np.uint8)
+    out["clip_comb_bd"] = np.zeros(T, np.int8)
+    out["clip_src"] = np.zeros(T, np.int64)
+    lib.pfo_fetch_clip(h, _ptr(out["clip_combine"]), _ptr(out["clip_comb_bd"]), _ptr(out["clip_src"]))
     out["n_alpha"] = int(counts[6])
     out["guard_hits"] = int(counts[7])
     out["path_lines"] = np.zeros(scene.n_paths, np.int32)
@@ -129,11 +140,15 @@ def run_ref(scene, threads: int = 1) -> dict:
     """Canonical tiling of ``scene`` by the reference's own compiled Tiler."""
     lib = load_ref()
     el = np.zeros(1, np.float64)
-    h = lib.pfref_run(*_scene_args(scene), ctypes.c_int(threads), _ptr(el))
+    h = lib.pfref_run(*_scene_args(scene), ctypes.c_int(threads), _ptr(el), *_clip_args(scene))
     counts = np.zeros(6, np.int64)
     lib.pfref_counts(h, _ptr(counts))
     out = _alloc(counts)
     lib.pfref_fetch(h, *[_ptr(out[k]) for k in CANONICAL_KEYS])
+    T = out["tile_alpha"].shape[0]
+    out["clip_combine"] = np.zeros(T, np.uint8)
+    out["clip_comb_bd"] = np.zeros(T, np.int8)
+    lib.pfref_fetch_clip(h, _ptr(out["clip_combine"]), _ptr(out["clip_comb_bd"]))
     lib.pfref_free(h)
     out["tiling_seconds"] = float(el[0])
     return out
@@ -159,7 +174,10 @@ def alpha_tiles(canon: dict, scene) -> dict:
     fill_path = np.repeat(np.arange(scene.n_paths), np.diff(canon["fill_off"]))
     fill_global_tile = canon["tile_off"][fill_path] + canon["fill_tile"].astype(np.int64)
     fa = alpha_index[fill_global_tile]
-    order = np.argsort(fa, kind="stable")
+    # fills of tiles that clip resolution culled draw nothing (core/d3d9/tiler.cpp:425-428)
+    keep = fa >= 0
+    fa = fa[keep]
+    order = np.nonzero(keep)[0][np.argsort(fa, kind="stable")]
     counts = np.bincount(fa, minlength=alpha_tiles.shape[0]) if fa.size else np.zeros(alpha_tiles.shape[0], np.int64)
     off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     return dict(tile=alpha_tiles, alpha_index=alpha_index, fill_off=off,
@@ -198,11 +216,47 @@ def composite_fp64(canon: dict, at: dict, coverage: np.ndarray, scene) -> np.nda
     return out
 
 
+def apply_rule(c: np.ndarray, even_odd: np.ndarray) -> np.ndarray:
+    """The fill rule (nonzero 0, even-odd 1) applied elementwise to FP64 coverage sums."""
+    c = np.ascontiguousarray(c, np.float64)
+    eo = np.ascontiguousarray(np.broadcast_to(np.asarray(even_odd, np.uint8).reshape(
+        (-1,) + (1,) * (c.ndim - 1)), c.shape))
+    out = np.empty_like(c)
+    load_oracle().pfo_apply_rule_array(ctypes.c_int64(c.size), _ptr(c), _ptr(eo), _ptr(out))
+    return out
+
+
+def clipped_coverage(canon: dict, at: dict) -> np.ndarray:
+    """FP64 coverage of every alpha tile after clip resolution (core/d3d9/tiler.cpp:385-431) and the
+    clip combine pass (shaders/d3d9/tile_clip_combine.frag:25-27, then the draw path's fill rule in
+    shaders/d3d9/tile.frag):
+      * a tile with a combine job: rule(min(|own sum + own backdrop|, |clip tile's sum + clip backdrop|));
+      * a solid tile that took the clip tile's mask: rule(clip tile's sum + clip backdrop);
+      * any other alpha tile: rule(own sum + backdrop); a clip path's own tiles under the nonzero
+        rule, as their coverage enters the combine (abs()) whatever the clip's fill rule."""
+    raw = coverage_fp64(at, raw=True)
+    src = canon["clip_src"][at["tile"]]
+    comb = canon["clip_combine"][at["tile"]].astype(bool)
+    eff = raw.copy()
+    has = src >= 0
+    s_raw = np.abs(raw[at["alpha_index"][src[has]]])
+    d_raw = np.abs(raw[has] + canon["clip_comb_bd"][at["tile"][has]].astype(np.float64)[:, None, None])
+    eff[has] = np.where(comb[has][:, None, None], np.minimum(d_raw, s_raw), s_raw)
+    n_draw = canon["tile_off"].shape[0] - 1 - scene_n_clip(canon)
+    eo = np.where(at["path"] >= n_draw, 0, at["even_odd"]).astype(np.uint8)
+    return apply_rule(eff, eo)
+
+
+def scene_n_clip(canon: dict) -> int:
+    return int(canon.get("n_clip_paths", 0))
+
+
 def full_model(scene, zmode: int = 1) -> dict:
     """Canonical tiling, FP64 coverage and FP64 composite of a scene."""
     canon = run_oracle(scene, zmode)
+    canon["n_clip_paths"] = int(scene.n_clip_paths)
     at = alpha_tiles(canon, scene)
-    cov = coverage_fp64(at)
+    cov = clipped_coverage(canon, at) if scene.n_clip_paths else coverage_fp64(at)
     img = composite_fp64(canon, at, cov, scene)
     return dict(canon=canon, alpha=at, coverage=cov, image=img)
 

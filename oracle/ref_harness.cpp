@@ -46,6 +46,8 @@ struct RefResult {
     std::vector<uint32_t> entry_fbtile, entry_path;
     std::vector<uint32_t> zbuf;
     int32_t fb_rect[4] = {0, 0, 0, 0};
+    std::vector<uint8_t> combine;  // per tile: the reference queued a clip combine job for it
+    std::vector<int8_t> comb_bd;   // that job's dest_backdrop
 };
 
 Outline make_outline(const float *pts, const uint8_t *pflags, const int32_t *contour_start, const uint8_t *closed,
@@ -72,10 +74,17 @@ extern "C" {
 // threads > 1 reproduces SceneBuilderD3D9::build_paths_on_cpu's interleaved
 // split (core/d3d9/scene_builder.cpp:170-194); the canonical output does not
 // depend on it.  Returns an opaque handle, or nullptr.
+//
+// Clip paths: the last n_clip paths are Scene::clip_paths, tiled first as in
+// SceneBuilderD3D9::build_paths_on_cpu (core/d3d9/scene_builder.cpp:137-166, 200-225);
+// path_clip[p] (NULL: none) is draw path p's clip path index among them, or -1.  A clipped
+// draw path is tiled twice: without its clip for its fills and their tiles (clip resolution
+// changes only tiles, tiler.cpp:385-431, and can drop a tile's alpha id), and with it for the
+// resolved tiles and the clip combine jobs.  Scenes with clip paths are tiled on one thread.
 void *pfref_run(int n_paths, const int32_t *path_contour_start, const uint8_t *path_fill_rule,
                 const uint8_t *path_opaque, const int32_t *contour_start, const uint8_t *contour_closed,
                 const float *points, const uint8_t *point_flags, const float *view_box, int threads,
-                double *elapsed_s) {
+                double *elapsed_s, int n_clip, const int32_t *path_clip) {
     auto *res = new RefResult();
     Scene scene(0, RectF(view_box[0], view_box[1], view_box[2], view_box[3]));
     HarnessBuilder builder;
@@ -89,10 +98,25 @@ void *pfref_run(int n_paths, const int32_t *path_contour_start, const uint8_t *p
                                    path_contour_start[p + 1]);
     }
 
+    const int n_draw = n_paths - n_clip;
     std::vector<BuiltDrawPath> built(n_paths);
     std::vector<std::vector<Fill>> fills(n_paths);
+    std::vector<BuiltPath> built_clips(n_clip);
+    for (int q = 0; q < n_clip; q++) {
+        int g = n_draw + q;
+        TilingPathInfo info{};
+        info.type = TilingPathInfo::Type::Clip;
+        FillRule rule = path_fill_rule[g] ? FillRule::EvenOdd : FillRule::Winding;
+        Tiler tiler(builder, (uint32_t)q, outlines[g], rule, vb, nullptr, {}, info);
+        tiler.generate_tiles();
+        built_clips[q] = tiler.object_builder.built_path;
+        built[g].path = tiler.object_builder.built_path;
+        fills[g] = std::move(tiler.object_builder.fills);
+    }
+    std::vector<BuiltPath> resolved(n_draw);
+    if (n_clip > 0) threads = 1;
     auto work = [&](int begin, int stride) {
-        for (int p = begin; p < n_paths; p += stride) {
+        for (int p = begin; p < n_draw; p += stride) {
             TilingPathInfo info{};
             info.type = TilingPathInfo::Type::Draw;
             info.info.paint_id = 0;
@@ -105,6 +129,12 @@ void *pfref_run(int n_paths, const int32_t *path_contour_start, const uint8_t *p
             built[p].blend_mode = BlendMode::SrcOver;
             built[p].mask_fill_rule = info.info.fill_rule;
             fills[p] = std::move(tiler.object_builder.fills);
+            if (path_clip && path_clip[p] >= 0) {
+                auto clip_id = std::make_shared<uint32_t>((uint32_t)path_clip[p]);
+                Tiler clipped(builder, (uint32_t)p, outlines[p], info.info.fill_rule, vb, clip_id, built_clips, info);
+                clipped.generate_tiles();
+                resolved[p] = clipped.object_builder.built_path;
+            }
         }
     };
     auto t0 = std::chrono::steady_clock::now();
@@ -127,10 +157,22 @@ void *pfref_run(int n_paths, const int32_t *path_contour_start, const uint8_t *p
         const RectI &r = bp.data.tiles.rect;
         res->path_rect.insert(res->path_rect.end(), {r.left, r.top, r.right, r.bottom});
         const auto &tiles = bp.data.tiles.data;
-        std::vector<int64_t> alpha_to_local;
-        for (size_t i = 0; i < tiles.size(); i++) {
-            res->tile_backdrop.push_back(tiles[i].backdrop);
-            res->tile_alpha.push_back(tiles[i].alpha_tile_id.is_valid() ? 1 : 0);
+        const bool clipped = p < n_draw && path_clip && path_clip[p] >= 0;
+        const auto &out_tiles = clipped ? resolved[p].data.tiles.data : tiles;
+        for (size_t i = 0; i < out_tiles.size(); i++) {
+            res->tile_backdrop.push_back(out_tiles[i].backdrop);
+            res->tile_alpha.push_back(out_tiles[i].alpha_tile_id.is_valid() ? 1 : 0);
+            uint8_t comb = 0;
+            int8_t cbd = 0;
+            if (clipped && resolved[p].data.clip_tiles) {
+                const auto &ct = resolved[p].data.clip_tiles->data[i];
+                if (ct.dest_tile_id.is_valid() && ct.src_tile_id.is_valid()) {
+                    comb = 1;
+                    cbd = (int8_t)ct.dest_backdrop;
+                }
+            }
+            res->combine.push_back(comb);
+            res->comb_bd.push_back(cbd);
         }
         for (auto v : bp.data.backdrops) res->col_sum.push_back(v);
         for (const Fill &f : fills[p]) {
@@ -154,8 +196,11 @@ void *pfref_run(int n_paths, const int32_t *path_contour_start, const uint8_t *p
     }
 
     // Draw order and z-buffer through the reference's own batch builder.
-    if (n_paths > 0) {
-        auto batches = build_tile_batches_for_draw_path_display_item(scene, built, Range(0, (uint64_t)n_paths));
+    if (n_draw > 0) {
+        std::vector<BuiltDrawPath> draws(built.begin(), built.begin() + n_draw);
+        for (int p = 0; p < n_draw; p++)
+            if (path_clip && path_clip[p] >= 0) draws[p].path = resolved[p];
+        auto batches = build_tile_batches_for_draw_path_display_item(scene, draws, Range(0, (uint64_t)n_draw));
         if (!batches.empty()) {
             const auto &batch = batches[0];
             const RectI &zr = batch.z_buffer_data.rect;
@@ -254,6 +299,15 @@ void pfref_fetch(void *h, int32_t *path_rect, int64_t *tile_off, int64_t *col_of
     cp(entry_path, r->entry_path.data(), r->entry_path.size() * 4);
     cp(zbuf, r->zbuf.data(), r->zbuf.size() * 4);
     cp(fb_rect, r->fb_rect, 16);
+}
+
+// Per tile: whether the reference queued a clip combine job for it, and the job's dest_backdrop.
+void pfref_fetch_clip(void *h, uint8_t *combine, int8_t *comb_bd) {
+    auto *r = (RefResult *)h;
+    if (!r->combine.empty()) {
+        memcpy(combine, r->combine.data(), r->combine.size());
+        memcpy(comb_bd, r->comb_bd.data(), r->comb_bd.size());
+    }
 }
 
 void pfref_free(void *h) {

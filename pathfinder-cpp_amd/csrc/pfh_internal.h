@@ -15,6 +15,14 @@ constexpr int MAX_FLATTEN_DEPTH = 16;
 constexpr int FIX_SHIFT = 20;            // fixed-point coverage accumulation: 2^-20 px^2 units
 constexpr uint32_t NO_ALPHA = 0xffffffffu;
 
+// Clip reference of a path (SceneDev::path_clip, then BinItem::clip).  A draw path clipped by a
+// clip path holds that path's binning position: its index in the scene's wave-class list, or
+// CLIP_GROUP | its index in the group-class list (BinItem::clip: the batch bin item index).
+constexpr uint32_t CLIP_NONE = 0xffffffffu;     // a draw path without a clip path
+constexpr uint32_t CLIP_IS_CLIP = 0xfffffffeu;  // a clip path: tiled, never drawn
+constexpr uint32_t CLIP_EMPTY = 0xfffffffdu;    // clipped by a clip path with no tile columns: all culled
+constexpr uint32_t CLIP_GROUP = 0x80000000u;
+
 // Binning ownership classes (a path's tiles and columns belong to it alone, so the wave or
 // workgroup that bins a path can count in LDS and store plainly).  A path goes to the wave
 // class when its tile rect and column count fit one wave's LDS slice, else to the group class,
@@ -80,6 +88,8 @@ enum TileStatus : uint8_t {
     TILE_ALPHA = 1,  // has fills: a coverage mask
     TILE_SOLID = 2,  // no fills, backdrop covers the whole tile
     TILE_CLEAR = 3,  // no fills, even-odd rule with an even nonzero backdrop: an entry that covers nothing
+    TILE_CLIP_MASK = 4,  // no fills, nonzero backdrop, over a clip tile with fills: draws the clip's
+                         // coverage (mask row tile_clip_ai)
 };
 
 // Counters written by the device pipeline (one block per batch).
@@ -91,6 +101,7 @@ enum Counter {
     C_GUARD = 4,       // loop guards tripped
     C_ALPHA_ENTRIES = 5,  // draw entries that sample a mask
     C_LINES = 6,       // flattened chords (lines) in the batch
+    C_CLIP_JOBS = 7,   // clip combine jobs (alpha tiles over clip tiles with fills)
     C_DRAWN = 8,       // [C_DRAWN, C_DRAWN + DRAWN_SLOTS): fills the fill kernel read, spread over
                        // slots (per block, by block index) so that no single address takes every add
     C_COUNT = 8 + 64
@@ -111,6 +122,7 @@ struct SceneDev {
     const uint32_t *path_seg_off; // [n_paths + 1] local segment offsets
     const uint32_t *bin_paths;    // local path indices: wave class, then group class
     const uint8_t *path_rule;     // 0 nonzero, 1 even-odd
+    const uint32_t *path_clip;    // [n_paths] clip reference (CLIP_*), or null: no clip paths
     float vb_l, vb_t, vb_r, vb_b; // view box (right edge rounded up to 16)
     int32_t fb_x0, fb_y0, fb_w, fb_h;
     int32_t width, height;
@@ -132,7 +144,8 @@ struct BinItem {
     uint32_t c0, c1;           // its chords [c0, c1)
     uint32_t p, gp, k;         // local path, batch path, scene
     uint32_t rgba, rule;       // paint, fill rule
-    uint32_t pad[3];
+    uint32_t clip;             // CLIP_NONE / CLIP_IS_CLIP / CLIP_EMPTY, or the clip path's bin item
+    uint32_t pad[2];
 };
 
 // Batch-wide work arrays (device pointers).  Sizes: S segments, T tiles,
@@ -149,11 +162,15 @@ struct BatchDev {
     uint32_t n_segs, n_tiles, n_cols, n_fb, n_paths;
     uint32_t n_bin_wave, n_bin_group;
     uint32_t cap_fills, cap_alpha, cap_entries, cap_lines;
+    uint32_t cap_vis;             // alpha_vis capacity: cap_alpha, or with clip paths the larger of it and
+                                  // cap_entries (solid tiles of several paths can take one clip mask)
     int32_t *tile_delta;          // [T] backdrop delta (int8 arithmetic, stored widened)
     uint32_t *tile_cnt;           // [T] fills per tile (global-atomic paths: then emit cursor)
     uint32_t *tile_fill_off;      // [T + 1] exclusive scan of fill counts
     uint32_t *tile_alpha_idx;     // [T + 1] exclusive scan of alpha flags
-    int8_t *tile_bd;              // [T] propagated backdrop (written with dumps only)
+    int8_t *tile_bd;              // [T] propagated backdrop (written with dumps or clip paths only)
+    uint32_t *tile_clip_ai;       // [T] with clip paths: mask row of a TILE_CLIP_MASK tile
+    uint2 *clip_jobs;             // [A] with clip paths: (draw alpha index, clip alpha index) to combine
     uint8_t *tile_st;             // [T] tile status (TileStatus)
     uint2 *path_ent;              // [P + 1] {draw entries, alpha entries} per batch path, then exclusive scan
     uint2 *ent_quarter;           // [4 n_bin_group] {draw entries, alpha entries} per quarter of a group-class rect
@@ -170,7 +187,7 @@ struct BatchDev {
     uint2 *fills;                 // [F] grouped by tile: (x0 | y0 << 16, x1 | y1 << 16)
     uint4 *dump_fills;            // [F] emission order: (x0|y0<<16, x1|y1<<16, global tile, 0)
     uint4 *alpha_rec;             // [A] (fill begin, fill end, backdrop | rule << 8, tile)
-    uint32_t *alpha_vis;          // [A] alpha indices of the alpha entries (tiles that survive the z-cull)
+    uint32_t *alpha_vis;          // [cap_vis] mask rows of the alpha entries (tiles that survive the z-cull)
     uint8_t *masks;               // [(A + 1) * 256]: row cap_alpha is all ones (written by the fill kernel)
     uint32_t *zbuf;               // [FB]
     uint2 *fb_range;              // [FB] [start, end) of each framebuffer tile's entries in sort order
@@ -180,6 +197,7 @@ struct BatchDev {
     uint2 *entry_meta;            // [E] with dumps only: (local path, scene-local fb tile)
     uint32_t *counters;           // [C_COUNT]
     int32_t dump;
+    int32_t has_clips;            // some scene has clip paths: propagate in two phases, combine masks
     uint32_t xcd_order;           // Schedule::xcd_order
 };
 

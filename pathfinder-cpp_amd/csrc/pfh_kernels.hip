@@ -649,6 +649,8 @@ __device__ __forceinline__ U2 wave_sum_u2(U2 v) {
 }
 
 // Inclusive prefix sums of (count, count > 0) across the wave.
+__device__ __forceinline__ uint32_t alpha_rows(const BinItem &it) { return it.clip == CLIP_IS_CLIP ? 2u : 1u; }
+
 __device__ __forceinline__ U2 wave_inclusive_u2(uint32_t c) {
     U2 v{c, c > 0 ? 1u : 0u};
     const int lane = threadIdx.x & 63;
@@ -662,26 +664,28 @@ __device__ __forceinline__ U2 wave_inclusive_u2(uint32_t c) {
 
 // Wave: tile offsets of a rect of `area` tiles (counts in cnt_src, global) from the path's
 // scanned base; offsets go to the tile arrays and, when given, to the LDS cursors.
+// aw: mask rows per alpha tile (2 for a clip path's tiles, whose coverage is kept under both
+// fill rules; 1 otherwise).
 __device__ __forceinline__ void wave_tile_offsets(const BatchDev &b, uint32_t tbase, uint32_t area, U2 base,
-                                                  uint32_t *l_cur) {
+                                                  uint32_t *l_cur, uint32_t aw) {
     const uint32_t lane = threadIdx.x & 63;
     for (uint32_t j0 = 0; j0 < area; j0 += 64) {
         const uint32_t j = j0 + lane;
         const uint32_t c = j < area ? b.tile_cnt[tbase + j] : 0u;
         const U2 inc = wave_inclusive_u2(c);
         if (j < area) {
-            const uint32_t off = base.x + inc.x - c, ai = base.y + inc.y - (c > 0 ? 1u : 0u);
+            const uint32_t off = base.x + inc.x - c, ai = base.y + aw * (inc.y - (c > 0 ? 1u : 0u));
             b.tile_fill_off[tbase + j] = off;
             b.tile_alpha_idx[tbase + j] = ai;
             if (l_cur) l_cur[j] = off;
         }
-        base = base + U2{__shfl(inc.x, 63, 64), __shfl(inc.y, 63, 64)};
+        base = base + U2{__shfl(inc.x, 63, 64), aw * __shfl(inc.y, 63, 64)};
     }
 }
 
 // Block (256 threads) version, called by every thread of the block.
 __device__ __forceinline__ void block_tile_offsets(const BatchDev &b, uint32_t tbase, uint32_t area, U2 base,
-                                                   uint32_t *l_cur) {
+                                                   uint32_t *l_cur, uint32_t aw) {
     for (uint32_t j0 = 0; j0 < area; j0 += 256) {
         const uint32_t j = j0 + threadIdx.x;
         const uint32_t c = j < area ? b.tile_cnt[tbase + j] : 0u;
@@ -689,10 +693,10 @@ __device__ __forceinline__ void block_tile_offsets(const BatchDev &b, uint32_t t
         const U2 ex = block_exclusive_scan<U2>(U2{c, c > 0 ? 1u : 0u}, tot);
         if (j < area) {
             b.tile_fill_off[tbase + j] = base.x + ex.x;
-            b.tile_alpha_idx[tbase + j] = base.y + ex.y;
+            b.tile_alpha_idx[tbase + j] = base.y + aw * ex.y;
             if (l_cur) l_cur[j] = base.x + ex.x;
         }
-        base = base + tot;
+        base = base + U2{tot.x, aw * tot.y};
     }
 }
 
@@ -728,7 +732,11 @@ __global__ __launch_bounds__(256) void k_bin_items(BatchDev b) {
     it.k = (uint32_t)k;
     it.rgba = sc.path_rgba[p];
     it.rule = sc.path_rule[p];
-    it.pad[0] = it.pad[1] = it.pad[2] = 0;
+    uint32_t cr = sc.path_clip ? sc.path_clip[p] : CLIP_NONE;
+    if (cr < CLIP_EMPTY)
+        cr = (cr & CLIP_GROUP) ? b.n_bin_wave + sc.bin_group_base + (cr & ~CLIP_GROUP) : sc.bin_wave_base + cr;
+    it.clip = cr;
+    it.pad[0] = it.pad[1] = 0;
     const uint32_t gi = (GROUP ? b.n_bin_wave : 0u) + i;
     b.bin_items[gi] = it;
     const uint32_t tw = (uint32_t)max(it.rect.z - it.rect.x, 0);
@@ -757,7 +765,7 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
     const uint32_t gp = it.gp;
     if (EMIT) {
         const uint2 base = b.path_tot[gp];
-        wave_tile_offsets(b, tbase, area, U2{base.x, base.y}, l_cnt);
+        wave_tile_offsets(b, tbase, area, U2{base.x, base.y}, l_cnt, alpha_rows(it));
     } else {
         for (uint32_t j = lane; j < area; j += 64) {
             l_cnt[j] = 0;
@@ -778,7 +786,7 @@ __global__ __launch_bounds__(256) void k_bin_wave(BatchDev b) {
         }
         for (uint32_t j = lane; j < tw; j += 64) b.col_bd[cbase + j] = l_col[j];
         tot = wave_sum_u2(tot);
-        if (lane == 0) b.path_tot[gp] = make_uint2(tot.x, tot.y);
+        if (lane == 0) b.path_tot[gp] = make_uint2(tot.x, tot.y * alpha_rows(it));
     }
 }
 
@@ -806,7 +814,7 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
     if (lds) {
         if (EMIT) {
             const uint2 base = b.path_tot[gp];
-            block_tile_offsets(b, tbase, area, U2{base.x, base.y}, s_cnt);
+            block_tile_offsets(b, tbase, area, U2{base.x, base.y}, s_cnt, alpha_rows(it));
         } else {
             for (uint32_t j = threadIdx.x; j < area; j += 256) s_cnt[j] = 0;
             for (uint32_t j = threadIdx.x; j < (area + 1) / 2; j += 256) s_delta[j] = 0x80008000u;
@@ -826,12 +834,12 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
             }
             for (uint32_t j = threadIdx.x; j < tw; j += 256) b.col_bd[cbase + j] = s_col[j];
             tot = block_sum_u2(tot, reinterpret_cast<U2 *>(s_col));
-            if (threadIdx.x == 0) b.path_tot[gp] = make_uint2(tot.x, tot.y);
+            if (threadIdx.x == 0) b.path_tot[gp] = make_uint2(tot.x, tot.y * alpha_rows(it));
         }
     } else {
         if (EMIT) {
             const uint2 base = b.path_tot[gp];
-            block_tile_offsets(b, tbase, area, U2{base.x, base.y}, nullptr);
+            block_tile_offsets(b, tbase, area, U2{base.x, base.y}, nullptr, alpha_rows(it));
             __threadfence();  // the offsets reach L2 before any thread reads them back
             __syncthreads();
         }
@@ -855,7 +863,7 @@ __global__ __launch_bounds__(256) void k_bin_group(BatchDev b) {
                 tot = tot + U2{c, c > 0 ? 1u : 0u};
             }
             tot = block_sum_u2(tot, reinterpret_cast<U2 *>(s_col));  // s_col is unused on this path
-            if (threadIdx.x == 0) b.path_tot[gp] = make_uint2(tot.x, tot.y);
+            if (threadIdx.x == 0) b.path_tot[gp] = make_uint2(tot.x, tot.y * alpha_rows(it));
         }
     }
 }
@@ -1021,7 +1029,8 @@ constexpr int COL_ROWS = 8;
 
 // Where a tile of a path column lands in its scene's framebuffer-tile grid.
 struct ColumnWalk {
-    uint32_t p, rgba, rule, tbase;
+    uint32_t p, rgba, rule, tbase, clip;
+    int rx, ry;  // the path's rect origin (tiles)
     int w, h, x, fb_row0;  // fb_row0: fb tile index (scene-local) of row 0
     uint32_t fb_base, fb_w;
     // column c of the batch, through its bin item (every column belongs to a binned path)
@@ -1036,6 +1045,9 @@ struct ColumnWalk {
         tbase = it.tbase;
         rgba = it.rgba;
         rule = it.rule;
+        clip = it.clip;
+        rx = r.x;
+        ry = r.y;
         fb_base = sc.fb_base;
         fb_w = (uint32_t)sc.fb_w;
         fb_row0 = (r.y - sc.fb_y0) * sc.fb_w + (r.x + x - sc.fb_x0);
@@ -1044,10 +1056,37 @@ struct ColumnWalk {
     __device__ uint32_t fb_local(int row) const { return (uint32_t)(fb_row0 + row * (int)fb_w); }
 };
 
-__global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
+// Clip resolution of one draw tile (core/d3d9/tiler.cpp:385-431) against the clip path's tile at
+// the same tile coordinates, whose backdrop and fills are final (the clip paths' columns are
+// propagated by an earlier launch).  Returns the tile's status; bd becomes its backdrop.
+struct ClipTile {
+    bool inside, alpha;
+    int8_t bd;
+    uint32_t ai;
+};
+__device__ __forceinline__ ClipTile clip_tile_at(const BatchDev &b, const BinItem &ci, int tx, int ty) {
+    ClipTile r{false, false, 0, 0};
+    const int4 cr = ci.rect;
+    if (tx < cr.x || tx >= cr.z || ty < cr.y || ty >= cr.w) return r;
+    const uint32_t ct = ci.tbase + (uint32_t)((ty - cr.y) * (cr.z - cr.x) + (tx - cr.x));
+    r.inside = true;
+    r.alpha = b.tile_fill_off[ct + 1] > b.tile_fill_off[ct];
+    r.bd = b.tile_bd[ct];
+    r.ai = b.tile_alpha_idx[ct];
+    return r;
+}
+
+// phase 0: every column (no clip paths in the batch); 1: clip paths' columns only; 2: draw
+// paths' columns only, after phase 1 (a clipped draw tile reads its clip path's final tiles).
+__global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.n_cols) return;
     const ColumnWalk cw(b, c);
+    const bool is_clip = cw.clip == CLIP_IS_CLIP;
+    if ((phase == 1 && !is_clip) || (phase == 2 && is_clip)) return;
+    const bool clipped = cw.clip < CLIP_EMPTY || cw.clip == CLIP_EMPTY;
+    BinItem ci;
+    if (cw.clip < CLIP_EMPTY) ci = b.bin_items[cw.clip];
     const bool opaque = (cw.rgba >> 24) == 255u;
     int32_t running = b.col_bd[c];
     for (int row0 = 0; row0 < cw.h; row0 += COL_ROWS) {
@@ -1078,14 +1117,50 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b) {
             if (row0 + i >= cw.h) break;
             const uint32_t t = cw.tile(row0 + i);
             const bool alpha = f1[i] > f0[i];
+            if (is_clip) {
+                // a clip path draws nothing itself: its masks keep |sum + backdrop| under both
+                // rules (rows ai, ai + 1) for the draw tiles that take them over
+                b.tile_st[t] = alpha ? TILE_ALPHA : TILE_EMPTY;
+                b.tile_bd[t] = bd[i];
+                if (alpha && ai[i] + 1 < b.cap_alpha) {
+                    b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i], t);
+                    b.alpha_rec[ai[i] + 1] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (1u << 8), t);
+                }
+                continue;
+            }
+            if (alpha && ai[i] < b.cap_alpha)
+                b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (cw.rule << 8), t);
+            if (clipped) {
+                ClipTile ct{false, false, 0, 0};
+                if (cw.clip != CLIP_EMPTY) ct = clip_tile_at(b, ci, cw.rx + cw.x, cw.ry + row0 + i);
+                uint8_t st;
+                int8_t out_bd = 0;
+                if (!ct.inside || (!ct.alpha && ct.bd == 0)) {
+                    st = TILE_EMPTY;  // outside the clip's rect or over a blank clip tile: culled
+                } else if (ct.alpha && alpha) {
+                    // both masks: min(|own + backdrop|, |clip|) replaces the mask; backdrop 0
+                    st = TILE_ALPHA;
+                    const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
+                    if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai[i], ct.ai);
+                } else if (ct.alpha && bd[i] != 0) {
+                    st = TILE_CLIP_MASK;  // solid over a clip mask: the clip's mask and backdrop
+                    out_bd = ct.bd;
+                    b.tile_clip_ai[t] = ct.ai + cw.rule;
+                } else {
+                    // unchanged: an alpha tile, or a solid tile over a solid clip tile
+                    const bool even = cw.rule && (bd[i] & 1) == 0;
+                    st = alpha ? TILE_ALPHA : (bd[i] == 0 ? TILE_EMPTY : (even ? TILE_CLEAR : TILE_SOLID));
+                    out_bd = bd[i];
+                    if (st == TILE_SOLID && opaque) atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row0 + i)], cw.p);
+                }
+                b.tile_st[t] = st;
+                if (b.dump) b.tile_bd[t] = out_bd;
+                continue;
+            }
             const bool even = cw.rule && (bd[i] & 1) == 0;
             b.tile_st[t] = alpha ? TILE_ALPHA : (bd[i] == 0 ? TILE_EMPTY : (even ? TILE_CLEAR : TILE_SOLID));
             if (b.dump) b.tile_bd[t] = bd[i];
-            if (alpha) {
-                if (ai[i] < b.cap_alpha) b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (cw.rule << 8), t);
-            } else if (opaque && bd[i] != 0 && !even) {
-                atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row0 + i)], cw.p);
-            }
+            if (!alpha && opaque && bd[i] != 0 && !even) atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row0 + i)], cw.p);
         }
     }
     b.col_bd[c] = running;
@@ -1111,8 +1186,8 @@ __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &s
                                             uint32_t tbase, uint32_t j, const EntryTile &et, uint32_t e,
                                             uint32_t ea) {
     if (e >= b.cap_entries) return;  // past the capacity: the batch is retried
-    const uint32_t ai = et.st == TILE_ALPHA ? et.ai : NO_ALPHA;
-    if (ai != NO_ALPHA && ea < b.cap_alpha) b.alpha_vis[ea] = ai;
+    const uint32_t ai = (et.st == TILE_ALPHA || et.st == TILE_CLIP_MASK) ? et.ai : NO_ALPHA;
+    if (ai != NO_ALPHA && ea < b.cap_vis) b.alpha_vis[ea] = ai;
     // even-odd solid tile with an even winding covers nothing: zero alpha
     const uint32_t eff = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
     b.keys[e] = sc.fb_base + et.fbl;  // its value, e, is generated by the sort's first pass
@@ -1155,7 +1230,10 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
     uint32_t n_st, n_z, n_fbl;
     fetch(j0 + lane, n_st, n_z, n_fbl);
     // write pass: a chunk's alpha indices are loaded at the end of the chunk before it
-    uint32_t n_ai = WRITE && n_st == TILE_ALPHA ? b.tile_alpha_idx[tbase + j0 + lane] : NO_ALPHA;
+    auto alpha_of = [&](uint32_t st, uint32_t j) {
+        return st == TILE_ALPHA ? b.tile_alpha_idx[tbase + j] : (st == TILE_CLIP_MASK ? b.tile_clip_ai[tbase + j] : NO_ALPHA);
+    };
+    uint32_t n_ai = WRITE ? alpha_of(n_st, j0 + lane) : NO_ALPHA;
     for (uint32_t jc = j0; jc < j1; jc += 64) {  // uniform over the wave
         const uint32_t j = jc + lane;
         EntryTile et;
@@ -1164,13 +1242,13 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
         et.ai = n_ai;
         et.keep = n_st != TILE_EMPTY && (int32_t)p >= (int32_t)n_z;
         fetch(jc + 64 + lane, n_st, n_z, n_fbl);
-        const uint64_t m = __ballot(et.keep), ma = __ballot(et.keep && et.st == TILE_ALPHA);
+        const uint64_t m = __ballot(et.keep), ma = __ballot(et.keep && (et.st == TILE_ALPHA || et.st == TILE_CLIP_MASK));
         if (WRITE && et.keep)
             write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
                         n_alpha + (uint32_t)__popcll(ma & lt_mask));
         e += (uint32_t)__popcll(m);
         n_alpha += (uint32_t)__popcll(ma);
-        if (WRITE) n_ai = n_st == TILE_ALPHA ? b.tile_alpha_idx[tbase + jc + 64 + lane] : NO_ALPHA;
+        if (WRITE) n_ai = alpha_of(n_st, jc + 64 + lane);
     }
     return make_uint2(e, n_alpha);
 }
@@ -1181,7 +1259,8 @@ __global__ __launch_bounds__(256) void k_entries_wave(BatchDev b) {
     if (item >= b.n_bin_wave) return;  // whole wave
     const BinItem it = b.bin_items[item];
     const int4 r = it.rect;
-    const uint32_t area = (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
+    // a clip path draws nothing: no entries
+    const uint32_t area = it.clip == CLIP_IS_CLIP ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
     const uint2 first = WRITE ? b.path_ent[it.gp] : make_uint2(0u, 0u);
     const uint2 n = range_entries<WRITE>(b, b.scenes[it.k], it, 0, area, first);
     if (!WRITE && (threadIdx.x & 63) == 0) b.path_ent[it.gp] = n;
@@ -1197,7 +1276,7 @@ __global__ __launch_bounds__(256) void k_entries_group(BatchDev b) {
     if (item >= b.n_bin_group) return;  // uniform over the block
     const BinItem it = b.bin_items[b.n_bin_wave + item];
     const int4 r = it.rect;
-    const uint32_t area = (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
+    const uint32_t area = it.clip == CLIP_IS_CLIP ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t q = (area + 255) / 256 * 64;  // tiles per quarter, whole chunks
     const uint32_t j0 = min(wv * q, area), j1 = min(j0 + q, area);
@@ -1471,7 +1550,13 @@ hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s) {
 hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s) {
     if (b.n_cols == 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_propagate, dim3(blocks_for(b.n_cols, block)), dim3(block), 0, s, b);
+    const dim3 g(blocks_for(b.n_cols, block));
+    if (!b.has_clips) {
+        hipLaunchKernelGGL(k_propagate, g, dim3(block), 0, s, b, 0u);
+    } else {
+        hipLaunchKernelGGL(k_propagate, g, dim3(block), 0, s, b, 1u);
+        hipLaunchKernelGGL(k_propagate, g, dim3(block), 0, s, b, 2u);
+    }
     return hipGetLastError();
 }
 

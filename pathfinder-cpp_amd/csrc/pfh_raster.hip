@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_fill_masks(BatchDev b) {
     int32_t *dwave = diff + wv * 4 * FILL_GROUP_WORDS;
     int32_t *dcol = dwave + slot * FILL_GROUP_WORDS + li * FILL_COL_WORDS;
     const bool listed = !b.dump;
-    const uint32_t na = min(b.counters[listed ? C_ALPHA_ENTRIES : C_ALPHA], b.cap_alpha);
+    const uint32_t na = min(b.counters[listed ? C_ALPHA_ENTRIES : C_ALPHA], listed ? b.cap_vis : b.cap_alpha);
     const float scale = (float)(1 << FIX_SHIFT);
     uint32_t drawn = 0;  // fills this wave read (wave-uniform)
     // row cap_alpha of the masks is all ones: the composite's full-coverage row
@@ -563,6 +563,85 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         if (y0 + q < height) *reinterpret_cast<uint32_t *>(cptr + q * pitch) = px[q];
 }
 
+// ---------------------------------------------------------------------------
+// Clip combine (shaders/d3d9/tile_clip_combine.frag:25-27 and the draw path's fill rule in
+// shaders/d3d9/tile.frag): a draw tile with its own mask over a clip tile with a mask gets
+// rule(min(|own sum + own backdrop|, |clip sum + clip backdrop|)) per pixel.
+// ---------------------------------------------------------------------------
+
+// Mean of clamp(u, 0, 1) for u linear between ua and ub.
+__device__ __forceinline__ float mean_clamp01(float ua, float ub) {
+    const float a = fminf(ua, ub), c = fmaxf(ua, ub);
+    if (c - a <= 0.0f) return fminf(fmaxf(a, 0.0f), 1.0f);
+    const float full = c > 1.0f ? c - fmaxf(a, 1.0f) : 0.0f;
+    const float lo = fmaxf(a, 0.0f), hi = fminf(c, 1.0f);
+    const float mid = hi > lo ? (hi - lo) * (hi + lo) * 0.5f : 0.0f;
+    return (full + mid) / (c - a);
+}
+
+// Signed area of one 8.8 fill over pixel (px, py) of its tile: the box-filtered coverage
+// that the area LUT of shaders/d3d9/fill.frag:25-45 tabulates, evaluated directly.
+__device__ __forceinline__ float fill_area_at(uint2 fl, float px, float py) {
+    const float k = 1.0f / 256.0f;
+    const float x0 = (float)(fl.x & 0xffffu) * k, y0 = (float)(fl.x >> 16) * k;
+    const float x1 = (float)(fl.y & 0xffffu) * k, y1 = (float)(fl.y >> 16) * k;
+    if (x0 == x1) return 0.0f;
+    const float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
+    const float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
+    const float lo = fmaxf(px, xl), hi = fminf(px + 1.0f, xr);
+    if (hi <= lo) return 0.0f;
+    const float slope = (yr - yl) / (xr - xl);
+    const float ua = py + 1.0f - (yl + (lo - xl) * slope), ub = py + 1.0f - (yl + (hi - xl) * slope);
+    return (x0 > x1 ? 1.0f : -1.0f) * (hi - lo) * mean_clamp01(ua, ub);
+}
+
+__device__ __forceinline__ float fill_rule_f(float c, uint32_t even_odd) {
+    if (!even_odd) return fminf(fabsf(c), 1.0f);
+    const float m = c - 2.0f * floorf(c * 0.5f);
+    return fminf(fmaxf(1.0f - fabsf(1.0f - m), 0.0f), 1.0f);
+}
+
+// A wave per combine job (rare: only tiles where a clipped path's and its clip's masks meet),
+// after k_fill_masks, whose unclipped mask for the tile it replaces.  Each lane owns 4 pixels
+// of one column and walks both fill lists.
+__global__ __launch_bounds__(256) void k_fill_clipped(BatchDev b) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t n = min(b.counters[C_CLIP_JOBS], b.cap_alpha);
+    const float px = (float)(lane & 15u), py0 = (float)((lane >> 4) * 4u);
+    for (uint32_t j = blockIdx.x * 4 + (threadIdx.x >> 6); j < n; j += gridDim.x * 4) {  // uniform per wave
+        const uint2 job = b.clip_jobs[j];
+        if (job.x >= b.cap_alpha || job.y >= b.cap_alpha) continue;
+        const uint4 rd = b.alpha_rec[job.x], rc = b.alpha_rec[job.y];
+        float d[4], c[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            d[q] = (float)(int8_t)(rd.z & 0xffu);
+            c[q] = (float)(int8_t)(rc.z & 0xffu);
+        }
+        const uint32_t d0 = min(rd.x, b.cap_fills), d1 = max(d0, min(rd.y, b.cap_fills));
+        for (uint32_t f = d0; f < d1; f++) {
+            const uint2 fl = b.fills[f];
+#pragma unroll
+            for (int q = 0; q < 4; q++) d[q] += fill_area_at(fl, px, py0 + (float)q);
+        }
+        const uint32_t c0 = min(rc.x, b.cap_fills), c1 = max(c0, min(rc.y, b.cap_fills));
+        for (uint32_t f = c0; f < c1; f++) {
+            const uint2 fl = b.fills[f];
+#pragma unroll
+            for (int q = 0; q < 4; q++) c[q] += fill_area_at(fl, px, py0 + (float)q);
+        }
+        const uint32_t rule = (rd.z >> 8) & 1u;
+        uint32_t packed = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float v = fill_rule_f(fminf(fabsf(d[q]), fabsf(c[q])), rule);
+            packed |= (uint32_t)__float2int_rn(v * 255.0f) << (8 * q);
+        }
+        // column-major mask: byte 16 x + y
+        *reinterpret_cast<uint32_t *>(b.masks + (uint64_t)job.x * 256 + (lane & 15u) * 16 + (lane >> 4) * 4) = packed;
+    }
+}
+
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, uint32_t max_blocks, hipStream_t s) {
@@ -570,6 +649,12 @@ hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, uint32_t ma
     // its share of the tiles
     const uint32_t nb = std::max(1u, std::min<uint32_t>(blocks_for(grid_alpha, FILL_TILES_PER_BLOCK), max_blocks));
     hipLaunchKernelGGL(k_fill_masks, dim3(nb), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_clipped(const BatchDev &b, hipStream_t s) {
+    if (!b.has_clips) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_clipped, dim3(256), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

@@ -39,6 +39,7 @@ size_t radix_temp_elems(uint64_t cap, uint32_t n_scenes);
 hipError_t radix_sort(const BatchDev &b, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
                       int bits, uint32_t *tmp, bool *result_alt, hipStream_t s);
 hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, uint32_t max_blocks, hipStream_t s);
+hipError_t launch_fill_clipped(const BatchDev &b, hipStream_t s);
 hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, uint32_t block, hipStream_t s);
 hipError_t mlp_forward(const float *params, const float *x, int64_t n, float *hidden, float *out, hipStream_t s);
 hipError_t mlp_backward(const float *params, const float *x, int64_t n, const float *hidden, const float *out,
@@ -115,12 +116,14 @@ struct SceneHost {
     std::vector<int4> path_rect;
     std::vector<uint32_t> path_tile_off, path_col_off, path_seg_off, path_rgba;
     std::vector<uint8_t> path_rule;
+    uint32_t n_clip = 0;                // trailing clip paths
+    std::vector<uint32_t> path_clip;    // clip reference per path (CLIP_*)
     std::vector<uint32_t> path_breaks;  // chord chains per path (two extra stream points each)
     uint64_t n_breaks = 0;
     std::vector<float> features;
     DevBuf d_features;
     DevBuf d_pts, d_segs, d_seg_path, d_path_rect, d_path_tile_off, d_path_col_off, d_col_path, d_path_rgba,
-        d_path_rule, d_path_seg_off, d_bin_paths, d_out_owned;
+        d_path_rule, d_path_seg_off, d_bin_paths, d_out_owned, d_path_clip;
     uint32_t n_bin_wave = 0, n_bin_group = 0;  // paths binned by one wave / by one workgroup
     uint8_t *d_out = nullptr;
     int64_t out_pitch = 0;
@@ -163,7 +166,7 @@ struct Workspace {
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, ent_quarter, path_tot, bin_items, col_item, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, alpha_vis, masks, zbuf, fb_range, keys, vals, keys_alt,
-        vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp;
+        vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp, tile_clip_ai, clip_jobs;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
     size_t h_table_cap = 0;
@@ -188,7 +191,7 @@ struct Workspace {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
                 &path_ent, &ent_quarter, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &entry_meta, &counters, &scan_tmp,
-                &sort_tmp};
+                &sort_tmp, &tile_clip_ai, &clip_jobs};
     }
     size_t device_bytes() {
         size_t n = 0;
@@ -277,6 +280,18 @@ static void validate_desc(const pfh_scene_desc *d) {
             throw PfhError{PFH_E_ARG, "path_contour_start not monotone"};
     if ((int64_t)d->width * d->height > (int64_t)1 << 30) throw PfhError{PFH_E_LIMIT, "canvas larger than 2^30 pixels"};
     if (d->n_paths >= (1 << 24)) throw PfhError{PFH_E_LIMIT, "more than 2^24 paths in a scene"};
+    if (d->n_clip_paths < 0 || d->n_clip_paths > d->n_paths)
+        throw PfhError{PFH_E_ARG, "n_clip_paths must lie in [0, n_paths]"};
+    if (d->path_clip) {
+        const int n_draw = d->n_paths - d->n_clip_paths;
+        for (int p = 0; p < d->n_paths; p++) {
+            const int32_t c = d->path_clip[p];
+            if (c < -1 || c >= d->n_clip_paths) throw PfhError{PFH_E_ARG, "path_clip names no clip path"};
+            // The reference's hybrid tiler builds clip paths with no clip paths of their own to
+            // refer to (core/d3d9/scene_builder.cpp:209-216): nested clips are undefined there.
+            if (p >= n_draw && c != -1) throw PfhError{PFH_E_ARG, "nested clip paths are not supported"};
+        }
+    }
 }
 
 // Features for the sizing predictor, per path (see pfh_scene_features).
@@ -450,6 +465,19 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
     }
     s->n_bin_wave = (uint32_t)bin_wave.size();
     s->n_bin_group = (uint32_t)bin_group.size();
+    // clip references: a clip path's binning position (or CLIP_EMPTY without tile columns)
+    s->n_clip = (uint32_t)d->n_clip_paths;
+    s->path_clip.assign(d->n_paths, CLIP_NONE);
+    if (s->n_clip > 0) {
+        const int n_draw = d->n_paths - d->n_clip_paths;
+        std::vector<uint32_t> pos(d->n_paths, CLIP_EMPTY);
+        for (size_t i = 0; i < bin_wave.size(); i++) pos[bin_wave[i]] = (uint32_t)i;
+        for (size_t i = 0; i < bin_group.size(); i++) pos[bin_group[i]] = CLIP_GROUP | (uint32_t)i;
+        for (int p = 0; p < d->n_paths; p++) {
+            if (p >= n_draw) s->path_clip[p] = CLIP_IS_CLIP;
+            else if (d->path_clip && d->path_clip[p] >= 0) s->path_clip[p] = pos[n_draw + d->path_clip[p]];
+        }
+    }
     bin_wave.insert(bin_wave.end(), bin_group.begin(), bin_group.end());
     s->features.resize((size_t)d->n_paths * N_FEATURES);
     for (int p = 0; p < d->n_paths; p++) path_features(*s, pts, segs, (uint32_t)p, &s->features[(size_t)p * N_FEATURES]);
@@ -500,6 +528,7 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
     upload(s->d_path_rule, s->path_rule);
     upload(s->d_path_seg_off, s->path_seg_off);
     upload(s->d_bin_paths, bin_wave);
+    if (s->n_clip > 0) upload(s->d_path_clip, s->path_clip);
     s->out_pitch = (int64_t)s->width * 4;
     s->d_out_owned.ensure((size_t)s->out_pitch * s->height);
     s->d_out = s->d_out_owned.as<uint8_t>();
@@ -545,6 +574,7 @@ struct Plan {
     std::vector<int32_t> ids;
     uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0, P = 0;
     uint64_t breaks = 0;  // chord chains of the batch (two stream points each besides the chords)
+    bool clips = false;   // some scene has clip paths
     uint32_t cap_f = 0, cap_a = 0, cap_e = 0, cap_l = 0;
     int key_bits = 1;
     Schedule sched;
@@ -565,7 +595,11 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.tile_cnt.ensure(4 * pl.T + 4);
     w.tile_fill_off.ensure(4 * pl.T + 8);
     w.tile_alpha_idx.ensure(4 * pl.T + 8);
-    if (dump) w.tile_bd.ensure(pl.T + 4);
+    if (dump || pl.clips) w.tile_bd.ensure(pl.T + 4);
+    if (pl.clips) {
+        w.tile_clip_ai.ensure(4 * pl.T + 4);
+        w.clip_jobs.ensure(8ull * pl.cap_a + 8);
+    }
     w.tile_st.ensure(pl.T + 4);
     w.path_ent.ensure(8 * pl.P + 8);
     w.ent_quarter.ensure(32 * pl.BG + 32);
@@ -584,7 +618,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.fills.ensure(8ull * pl.cap_f);
     if (dump) w.dump_fills.ensure(16ull * pl.cap_f);
     w.alpha_rec.ensure(16ull * pl.cap_a);
-    w.alpha_vis.ensure(4ull * pl.cap_a);
+    w.alpha_vis.ensure(4ull * (pl.clips ? std::max(pl.cap_a, pl.cap_e) : pl.cap_a));
     w.masks.ensure(256ull * pl.cap_a + 256);  // + the all-ones row
     w.keys.ensure(4ull * pl.cap_e);
     w.vals.ensure(4ull * pl.cap_e);
@@ -646,6 +680,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.bin_wave_base = (uint32_t)BW;
         d.bin_group_base = (uint32_t)BG;
         d.path_rule = s.d_path_rule.as<uint8_t>();
+        d.path_clip = s.n_clip > 0 ? s.d_path_clip.as<uint32_t>() : nullptr;
         d.vb_l = s.vb[0];
         d.vb_t = s.vb[1];
         d.vb_r = s.vb[2];
@@ -683,6 +718,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         ctx->w->h_bases[5 * (n + 1) + k] = (uint32_t)BG;
         S += s.n_segs;
         pl.breaks += s.n_breaks;
+        pl.clips = pl.clips || s.n_clip > 0;
         T += s.n_tiles;
         W += s.n_cols;
         FB += d.n_fb;
@@ -743,11 +779,15 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.cap_alpha = pl.cap_a;
     b.cap_entries = pl.cap_e;
     b.cap_lines = pl.cap_l;
+    b.cap_vis = pl.clips ? std::max(pl.cap_a, pl.cap_e) : pl.cap_a;
     b.tile_delta = w.tile_delta.as<int32_t>();
     b.tile_cnt = w.tile_cnt.as<uint32_t>();
     b.tile_fill_off = w.tile_fill_off.as<uint32_t>();
     b.tile_alpha_idx = w.tile_alpha_idx.as<uint32_t>();
-    b.tile_bd = dump ? w.tile_bd.as<int8_t>() : nullptr;
+    b.tile_bd = (dump || pl.clips) ? w.tile_bd.as<int8_t>() : nullptr;
+    b.tile_clip_ai = pl.clips ? w.tile_clip_ai.as<uint32_t>() : nullptr;
+    b.clip_jobs = pl.clips ? w.clip_jobs.as<uint2>() : nullptr;
+    b.has_clips = pl.clips ? 1 : 0;
     b.tile_st = w.tile_st.as<uint8_t>();
     b.path_ent = w.path_ent.as<uint2>();
     b.ent_quarter = w.ent_quarter.as<uint2>();
@@ -823,6 +863,7 @@ static void stage_finish(Workspace &w, const BatchDev &b, const Plan &pl) {
                              w.stream));
     HIP_TRY(hipEventRecordWithFlags(w.kev[0], w.stream, hipEventRecordExternal));
     HIP_TRY(launch_fill_masks(b, b.cap_alpha, pl.sched.fill_grid, w.stream));
+    HIP_TRY(launch_fill_clipped(b, w.stream));
     HIP_TRY(hipEventRecordWithFlags(w.kev[1], w.stream, hipEventRecordExternal));
     HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, pl.sched.composite_block, w.stream));
     HIP_TRY(hipEventRecordWithFlags(w.kev[2], w.stream, hipEventRecordExternal));
@@ -924,6 +965,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     HIP_TRY(launch_fb_bounds(alt ? b.keys_alt : b.keys, b.counters + C_ENTRIES, b.cap_entries, b.n_fb, b.fb_range, st));
     HIP_TRY(hipEventRecord(ev[6], st));
     HIP_TRY(launch_fill_masks(b, na, pl.sched.fill_grid, st));
+    HIP_TRY(launch_fill_clipped(b, st));
     HIP_TRY(hipEventRecord(ev[7], st));
     HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, pl.sched.composite_block, st));
     HIP_TRY(hipEventRecord(ev[8], st));
@@ -1341,7 +1383,7 @@ void pfh_destroy(pfh_ctx *ctx) {
     for (auto &s : ctx->scenes) {
         if (!s) continue;
         for (DevBuf *b : {&s->d_features, &s->d_pts, &s->d_segs, &s->d_seg_path, &s->d_path_rect, &s->d_path_tile_off,
-                          &s->d_path_col_off, &s->d_col_path, &s->d_path_rgba, &s->d_path_rule, &s->d_path_seg_off, &s->d_bin_paths, &s->d_out_owned})
+                          &s->d_path_col_off, &s->d_col_path, &s->d_path_rgba, &s->d_path_rule, &s->d_path_seg_off, &s->d_bin_paths, &s->d_out_owned, &s->d_path_clip})
             b->release();
     }
     for (DevBuf *b : {&ctx->mlp_params, &ctx->mlp_x, &ctx->mlp_h, &ctx->mlp_out}) b->release();
@@ -1369,7 +1411,7 @@ int pfh_scene_release(pfh_ctx *ctx, int32_t scene_id) {
         HIP_TRY(hipStreamSynchronize(ctx->w->stream));
         drop_all_graphs(ctx);
         for (DevBuf *b : {&s.d_features, &s.d_pts, &s.d_segs, &s.d_seg_path, &s.d_path_rect, &s.d_path_tile_off, &s.d_path_col_off,
-                          &s.d_col_path, &s.d_path_rgba, &s.d_path_rule, &s.d_path_seg_off, &s.d_bin_paths, &s.d_out_owned})
+                          &s.d_col_path, &s.d_path_rgba, &s.d_path_rule, &s.d_path_seg_off, &s.d_bin_paths, &s.d_out_owned, &s.d_path_clip})
             b->release();
         ctx->scenes[scene_id].reset();
     });
@@ -1464,7 +1506,14 @@ int pfh_dump_counts(pfh_ctx *ctx, int32_t scene_id, int64_t *counts) {
         counts[3] = e1 - e0;
         counts[4] = nfb;
         counts[5] = s.n_paths;
-        counts[6] = a1 - a0;
+        // alpha tiles after clip resolution (a clip path's tiles hold two mask rows each)
+        std::vector<uint8_t> st(s.n_tiles);
+        if (s.n_tiles) HIP_TRY(hipMemcpy(st.data(), ctx->w->tile_st.as<uint8_t>() + tb, s.n_tiles, hipMemcpyDeviceToHost));
+        int64_t na = 0;
+        for (uint8_t v : st) na += (v == TILE_ALPHA || v == TILE_CLIP_MASK) ? 1 : 0;
+        (void)a0;
+        (void)a1;
+        counts[6] = na;
         counts[7] = ctx->w->h_counters[C_GUARD];
     });
 }
@@ -1498,9 +1547,10 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
             fill_off[p] = (int64_t)line_fill[first_line] - line_fill[0];
         }
         d2h(tile_backdrop, ctx->w->tile_bd.as<int8_t>() + tb, s.n_tiles);
-        std::vector<uint32_t> foff(s.n_tiles + 1);
-        d2h(foff.data(), ctx->w->tile_fill_off.as<uint32_t>() + tb, 4 * (s.n_tiles + 1));
-        for (uint32_t t = 0; t < s.n_tiles; t++) tile_alpha[t] = foff[t + 1] > foff[t] ? 1 : 0;
+        // a tile draws a mask when it has fills, or (clipped) when it took its clip tile's
+        std::vector<uint8_t> tst(s.n_tiles + 1);
+        d2h(tst.data(), ctx->w->tile_st.as<uint8_t>() + tb, s.n_tiles);
+        for (uint32_t t = 0; t < s.n_tiles; t++) tile_alpha[t] = (tst[t] == TILE_ALPHA || tst[t] == TILE_CLIP_MASK) ? 1 : 0;
         d2h(col_sum, ctx->w->col_bd.as<int32_t>() + cb, 4 * s.n_cols);
         uint32_t nf = line_fill[l1 - l0] - line_fill[0];
         std::vector<uint4> df(nf);
@@ -1541,17 +1591,25 @@ int pfh_dump(pfh_ctx *ctx, int32_t scene_id, int32_t *path_rect, int64_t *tile_o
         fb_rect[2] = s.fb_x0 + s.fb_w;
         fb_rect[3] = s.fb_y0 + s.fb_h;
         if (masks) {
-            uint32_t a0 = 0, a1 = 0;
-            d2h(&a0, ctx->w->tile_alpha_idx.as<uint32_t>() + tb, 4);
-            d2h(&a1, ctx->w->tile_alpha_idx.as<uint32_t>() + tb + s.n_tiles, 4);
-            d2h(masks, ctx->w->masks.as<uint8_t>() + 256ull * a0, 256ull * (a1 - a0));
-            // device masks are column-major (byte 16 x + y); the dump is row-major (16 y + x)
-            uint8_t t[256];
-            for (uint64_t k = 0; k < (uint64_t)(a1 - a0); k++) {
-                uint8_t *m = masks + 256 * k;
+            // the mask row of each alpha tile in (path, row, column) order: its own, or for a
+            // tile that took its clip tile's mask, that row (a clip path's own tiles: the first
+            // of their two rows); the scene's rows are [a0, a1)
+            std::vector<uint32_t> aidx(s.n_tiles + 1), cai(s.n_clip ? s.n_tiles : 0);
+            d2h(aidx.data(), ctx->w->tile_alpha_idx.as<uint32_t>() + tb, 4 * (s.n_tiles + 1));
+            if (s.n_clip) d2h(cai.data(), ctx->w->tile_clip_ai.as<uint32_t>() + tb, 4 * s.n_tiles);
+            const uint32_t a0 = aidx[0], a1 = aidx[s.n_tiles];
+            std::vector<uint8_t> rows(256ull * (a1 - a0));
+            if (a1 > a0) d2h(rows.data(), ctx->w->masks.as<uint8_t>() + 256ull * a0, rows.size());
+            uint64_t k = 0;
+            for (uint32_t t = 0; t < s.n_tiles; t++) {
+                if (!tile_alpha[t]) continue;
+                const uint32_t row = (tst[t] == TILE_CLIP_MASK ? cai[t] : aidx[t]) - a0;
+                if (row >= a1 - a0) throw PfhError{PFH_E_STATE, "mask row outside the scene's rows"};
+                // device masks are column-major (byte 16 x + y); the dump is row-major (16 y + x)
+                const uint8_t *m = rows.data() + 256ull * row;
+                uint8_t *o = masks + 256 * k++;
                 for (int xx = 0; xx < 16; xx++)
-                    for (int yy = 0; yy < 16; yy++) t[16 * yy + xx] = m[16 * xx + yy];
-                memcpy(m, t, 256);
+                    for (int yy = 0; yy < 16; yy++) o[16 * yy + xx] = m[16 * xx + yy];
             }
         }
     });

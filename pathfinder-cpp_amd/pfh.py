@@ -33,7 +33,8 @@ class SceneDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("view_box", ctypes.c_float * 4),
                 ("n_paths", ctypes.c_int32), ("n_contours", ctypes.c_int32), ("n_points", ctypes.c_int32),
                 ("points", _P), ("point_flags", _P), ("contour_start", _P), ("contour_closed", _P),
-                ("path_contour_start", _P), ("path_fill_rule", _P), ("path_rgba", _P)]
+                ("path_contour_start", _P), ("path_fill_rule", _P), ("path_rgba", _P),
+                ("n_clip_paths", ctypes.c_int32), ("path_clip", _P)]
 
 
 class Stats(ctypes.Structure):
@@ -180,6 +181,9 @@ class Context:
         d.path_contour_start = _ptr(scene.path_contour_start)
         d.path_fill_rule = _ptr(scene.path_fill_rule)
         d.path_rgba = _ptr(scene.path_rgba)
+        d.n_clip_paths = int(getattr(scene, "n_clip_paths", 0))
+        path_clip = getattr(scene, "path_clip", None)
+        d.path_clip = None if path_clip is None else _ptr(path_clip)
         sid = ctypes.c_int32()
         self._check(self.lib.pfh_scene_upload(self.h, ctypes.byref(d), ctypes.byref(sid)), "pfh_scene_upload")
         self._keep[sid.value] = (int(scene.width), int(scene.height), scene.n_paths)

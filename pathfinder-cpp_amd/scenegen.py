@@ -50,10 +50,16 @@ class Scene:
     path_fill_rule: np.ndarray
     path_rgba: np.ndarray
     view_box: np.ndarray = None
+    # Clip paths (Scene::clip_paths, core/scene.h): the last n_clip_paths paths; path_clip[p] is the
+    # clip path of draw path p as an index among them, or -1 (None: no path is clipped).
+    n_clip_paths: int = 0
+    path_clip: np.ndarray = None
 
     def __post_init__(self):
         if self.view_box is None:
             self.view_box = np.array([0, 0, self.width, self.height], np.float32)
+        if self.path_clip is not None:
+            self.path_clip = np.ascontiguousarray(self.path_clip, np.int32)
         self.points = np.ascontiguousarray(self.points, np.float32).reshape(-1, 2)
         self.point_flags = np.ascontiguousarray(self.point_flags, np.uint8)
         self.contour_start = np.ascontiguousarray(self.contour_start, np.int32)
@@ -73,7 +79,8 @@ class Scene:
         return ((self.path_rgba >> 24) == 255).astype(np.uint8)
 
     def subset(self, paths) -> "Scene":
-        """The scene restricted to the given draw paths (in the given order)."""
+        """The scene restricted to the given draw paths (in the given order); clip paths are dropped."""
+        assert self.path_clip is None or (self.path_clip < 0).all(), "subset of a scene with clipped paths"
         paths = np.asarray(paths, np.int64)
         pts, flg, cst, ccl, pcs = [], [], [0], [], [0]
         for p in paths:
@@ -205,6 +212,19 @@ def batch_scenes(n_scenes: int = 256, seed: int = BASE_SEED ^ 3, side_min: int =
     counts = np.rint(np.exp(rng.uniform(math.log(paths_min), math.log(paths_max), n_scenes))).astype(int)
     return [blob_scene(seed + k + 1, int(sides[k]), int(sides[k]), int(counts[k]),
                        r_max=min(512.0, sides[k] / 2.0)) for k in range(n_scenes)]
+
+
+def clip_scene(seed: int, size: int = 1024, n_draw: int = 200, n_clip: int = 8, clipped: float = 0.6,
+               r_max: float = 256.0) -> Scene:
+    """Blob draw paths, the last n_clip blobs of the generator serving as clip paths, each draw path
+    clipped by a random one of them with probability `clipped`."""
+    base = blob_scene(seed, size, size, n_draw + n_clip, r_max=r_max)
+    rng = _rng(seed ^ 0x5EED)
+    pc = rng.integers(0, n_clip, n_draw).astype(np.int32)
+    pc[rng.random(n_draw) >= clipped] = -1
+    base.n_clip_paths = n_clip
+    base.path_clip = np.concatenate([pc, np.full(n_clip, -1, np.int32)])
+    return base
 
 
 def star_scene(seed: int = BASE_SEED ^ 2, size: int = 4096, n_stars: int = 2000, n_min: int = 100,

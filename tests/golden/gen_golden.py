@@ -103,6 +103,46 @@ def edge_scenes():
     return s
 
 
+def with_clips(scene, n_clip, path_clip):
+    scene.n_clip_paths = n_clip
+    scene.path_clip = np.array(path_clip, np.int32)
+    return scene
+
+
+def ring(cx, cy, r0, r1, n=48):
+    """Annulus: outer contour clockwise, inner counter-clockwise (winding 0 in the hole)."""
+    a = np.arange(n) * (2 * math.pi / n)
+    outer = np.stack([cx + r1 * np.cos(a), cy + r1 * np.sin(a)], 1)
+    inner = np.stack([cx + r0 * np.cos(a), cy + r0 * np.sin(a)], 1)[::-1]
+    return [(outer, None, True), (inner, None, True)]
+
+
+def clip_scenes():
+    """Clip paths (core/d3d9/tiler.cpp:385-431): every resolution case — combine jobs (both
+    masks), solid draw tiles taking the clip's mask, culls outside the clip's rect and on blank
+    clip tiles (a ring's hole), untouched tiles under solid clip tiles — with nonzero and even-odd
+    draw paths, clip windings above 1, shared clips and unclipped paths between clipped ones."""
+    s = {}
+    s["clip_ring_over_rect"] = with_clips(make_scene(256, 256, [
+        ([rect(8, 8, 248, 248)], 0, HALF),
+        (ring(128, 128, 40, 110), 0, 0),
+    ]), 1, [0, -1])
+    s["clip_overlap_evenodd"] = with_clips(make_scene(256, 256, [
+        ([rect(20.5, 20.5, 230.25, 200.75)], 1, OPAQ),
+        ([star(128, 128, 100, 7, 3)], 1, HALF),
+        ([rect(0, 100, 256, 140)], 0, OPAQ),
+        ([rect(30, 30, 200, 200), rect(60, 60, 236, 236)], 0, 0),
+        ([star(140, 120, 90, 5, 2)], 1, 0),
+    ]), 2, [0, 1, -1, -1, -1])
+    s["clip_partly_outside"] = with_clips(make_scene(192, 160, [
+        ([([(-40, -20), (230, 10), (150, 190), (-10, 120)], None, True)], 0, HALF),
+        ([([(10, 10), (90, 30), (60, 80)], [0, 1, 0], True)], 0, OPAQ),
+        ([([(40, -60), (300, 40), (80, 150)], None, True)], 0, 0),
+    ]), 1, [0, 0, -1])
+    s["clip_random_blobs"] = scenegen.clip_scene(4242, size=512, n_draw=60, n_clip=5, r_max=160.0)
+    return s
+
+
 def random_scenes():
     s = {}
     for k in range(8):
@@ -118,7 +158,7 @@ SCENE_KEYS = ("points", "point_flags", "contour_start", "contour_closed", "path_
 
 def main():
     assert oracle.ref_available(), "build oracle/_ref first: oracle/build_ref.sh"
-    scenes = {**edge_scenes(), **random_scenes()}
+    scenes = {**edge_scenes(), **random_scenes(), **clip_scenes()}
     data = {}
     names = []
     for name, sc in scenes.items():
@@ -129,6 +169,11 @@ def main():
             data[f"{name}/scene/{k}"] = getattr(sc, k)
         for k in oracle.CANONICAL_KEYS:
             data[f"{name}/ref/{k}"] = ref[k]
+        if sc.n_clip_paths:
+            data[f"{name}/scene/n_clip_paths"] = np.array(sc.n_clip_paths, np.int32)
+            data[f"{name}/scene/path_clip"] = sc.path_clip
+            for k in ("clip_combine", "clip_comb_bd"):
+                data[f"{name}/ref/{k}"] = ref[k]
         print(f"{name:28s} paths {sc.n_paths:4d} tiles {ref['tile_alpha'].size:7d} fills {ref['fill_tile'].size:7d}"
               f" entries {ref['entry_path'].size:6d}")
     data["names"] = np.array(names)

@@ -18,6 +18,9 @@ def test_oracle_matches_reference_fixture(golden, name):
     assert got["guard_hits"] == 0
     for k in oracle.CANONICAL_KEYS:
         np.testing.assert_array_equal(got[k], ref[k], err_msg=f"{name}: {k}")
+    if scene.n_clip_paths:  # the reference's clip combine jobs (core/d3d9/tiler.cpp:393-403)
+        for k in ("clip_combine", "clip_comb_bd"):
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=f"{name}: {k}")
 
 
 def test_fixed_zrule_only_relaxes_evenodd_even(golden):

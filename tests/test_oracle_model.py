@@ -186,3 +186,81 @@ def test_blob_scene_canonical_invariants():
     gt = canon["tile_off"][fp] + canon["fill_tile"]
     assert canon["tile_alpha"][gt].all()
     assert (canon["fill_seg"][:, 0] != canon["fill_seg"][:, 2]).all()
+
+
+def _rect_path(x0, y0, x1, y1):
+    return np.array([[x0, y0], [x1, y0], [x1, y1], [x0, y1]], np.float32)
+
+
+def _rect_scene(w, h, rects, rules, n_clip, path_clip):
+    """One path per list of rects (its contours); the last n_clip paths are clip paths."""
+    pts, cst, pcs = [], [0], [0]
+    for contours in rects:
+        for r in contours:
+            pts.append(_rect_path(*r))
+            cst.append(cst[-1] + 4)
+        pcs.append(len(cst) - 1)
+    k = len(rects)
+    sc = scenegen.Scene(w, h, np.concatenate(pts), np.zeros(4 * (len(cst) - 1), np.uint8), np.array(cst),
+                        np.ones(len(cst) - 1, np.uint8), np.array(pcs), np.array(rules, np.uint8),
+                        np.full(k, 0xff00ff00, np.uint32))
+    sc.n_clip_paths = n_clip
+    sc.path_clip = np.array(path_clip, np.int32)
+    return sc
+
+
+def _box_area(w, h, r):
+    """Exact per-pixel area of pixel [x, x+1) x [y, y+1) inside rect r (the box-filter coverage)."""
+    x = np.arange(w, dtype=np.float64)
+    y = np.arange(h, dtype=np.float64)
+    ox = np.clip(np.minimum(x + 1, r[2]) - np.maximum(x, r[0]), 0, 1)
+    oy = np.clip(np.minimum(y + 1, r[3]) - np.maximum(y, r[1]), 0, 1)
+    return oy[:, None] * ox[None, :]
+
+
+def test_clip_model_is_min_of_box_coverages():
+    """A rect clipped by a rect: the clip combine gives min(draw area, clip area) per pixel, a
+    closed form in the box-filter areas (shaders/d3d9/tile_clip_combine.frag:25-27).  Edges off
+    the pixel grid so both masks are fractional where they meet; tiles outside the clip's rect
+    are culled, solid tiles inside take the clip's mask.  Tolerance 1/255: a fill's tile-local x
+    is clamped to 4095/256 (core/d3d9/object_builder.cpp:19-64), so an edge across a whole tile
+    leaves its last pixel column 1/256 short."""
+    d, c = (3.5, 4.25, 70.75, 50.5), (20.25, 10.5, 90.5, 40.75)
+    sc = _rect_scene(96, 64, [[d], [c]], [0, 0], 1, [0, -1])
+    m = oracle.full_model(sc)
+    got = oracle.path_coverage_image(sc, m, 0)
+    want = np.minimum(_box_area(96, 64, d), _box_area(96, 64, c))
+    assert np.abs(got - want).max() <= 1.0 / 255.0
+    assert np.abs(got - want).mean() < 1e-3
+    # the clip path draws nothing itself
+    assert (m["canon"]["entry_path"] == 0).all()
+
+
+def test_clip_model_windings_above_one():
+    """Clip winding 2 (two overlapping clip contours) under an even-odd draw path of winding 2, on
+    pixel-aligned rects.  Where the clip tile has a mask the combine gives
+    evenodd(min(|w_draw|, |w_clip|)); under a solid clip tile the draw tile passes unchanged,
+    evenodd(w_draw) (core/d3d9/tiler.cpp:385-431 combines only two masks).  The clip's own fill
+    rule does not matter."""
+    d = [(8, 8, 56, 56), (24, 24, 72, 72)]        # winding 2 in [24, 56)^2
+    c = [(16, 0, 64, 64), (32, 16, 80, 48)]       # winding 2 in [32, 64) x [16, 48)
+    eo = np.ones(1, np.uint8)
+    for clip_rule in (0, 1):
+        sc = _rect_scene(96, 96, [d, c], [1, clip_rule], 1, [0, -1])
+        m = oracle.full_model(sc)
+        got = oracle.path_coverage_image(sc, m, 0)
+        wd = sum(_box_area(96, 96, r) for r in d)
+        wc = sum(_box_area(96, 96, r) for r in c)
+        canon = m["canon"]
+        l, t, r, b = [int(v) for v in canon["path_rect"][1]]
+        t0 = int(canon["tile_off"][1])
+        clip_mask = np.zeros((96, 96), bool)
+        for i in range((r - l) * (b - t)):
+            tx, ty = l + i % (r - l), t + i // (r - l)
+            clip_mask[ty * 16:ty * 16 + 16, tx * 16:tx * 16 + 16] = canon["tile_alpha"][t0 + i] == 1
+        want = np.where(clip_mask, oracle.apply_rule(np.minimum(wd, wc), eo), oracle.apply_rule(wd, eo))
+        want[wc == 0] = 0.0
+        assert np.abs(got - want).max() <= 1.0 / 255.0, clip_rule  # 8.8 clamp, as above
+        both = (wd == 2) & (wc == 2) & clip_mask
+        assert both.any() and (got[both] <= 1.0 / 255.0).all()
+        assert ((wd == 2) & (wc == 1) & clip_mask).any()
