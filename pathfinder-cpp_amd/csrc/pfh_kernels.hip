@@ -1078,15 +1078,17 @@ __device__ __forceinline__ ClipTile clip_tile_at(const BatchDev &b, const BinIte
 
 // phase 0: every column (no clip paths in the batch); 1: clip paths' columns only; 2: draw
 // paths' columns only, after phase 1 (a clipped draw tile reads its clip path's final tiles).
+// CLIPS: the batch has clip paths (without them the kernel is the plain per-column walk).
+template <bool CLIPS>
 __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.n_cols) return;
     const ColumnWalk cw(b, c);
-    const bool is_clip = cw.clip == CLIP_IS_CLIP;
-    if ((phase == 1 && !is_clip) || (phase == 2 && is_clip)) return;
-    const bool clipped = cw.clip < CLIP_EMPTY || cw.clip == CLIP_EMPTY;
+    const bool is_clip = CLIPS && cw.clip == CLIP_IS_CLIP;
+    if (CLIPS && ((phase == 1 && !is_clip) || (phase == 2 && is_clip))) return;
+    const bool clipped = CLIPS && (cw.clip < CLIP_EMPTY || cw.clip == CLIP_EMPTY);
     BinItem ci;
-    if (cw.clip < CLIP_EMPTY) ci = b.bin_items[cw.clip];
+    if (CLIPS && cw.clip < CLIP_EMPTY) ci = b.bin_items[cw.clip];
     const bool opaque = (cw.rgba >> 24) == 255u;
     int32_t running = b.col_bd[c];
     for (int row0 = 0; row0 < cw.h; row0 += COL_ROWS) {
@@ -1117,7 +1119,7 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
             if (row0 + i >= cw.h) break;
             const uint32_t t = cw.tile(row0 + i);
             const bool alpha = f1[i] > f0[i];
-            if (is_clip) {
+            if (CLIPS && is_clip) {
                 // a clip path draws nothing itself: its masks keep |sum + backdrop| under both
                 // rules (rows ai, ai + 1) for the draw tiles that take them over
                 b.tile_st[t] = alpha ? TILE_ALPHA : TILE_EMPTY;
@@ -1128,9 +1130,9 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
                 }
                 continue;
             }
-            if (alpha && ai[i] < b.cap_alpha)
-                b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (cw.rule << 8), t);
-            if (clipped) {
+            if (CLIPS && clipped) {
+                if (alpha && ai[i] < b.cap_alpha)
+                    b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (cw.rule << 8), t);
                 ClipTile ct{false, false, 0, 0};
                 if (cw.clip != CLIP_EMPTY) ct = clip_tile_at(b, ci, cw.rx + cw.x, cw.ry + row0 + i);
                 uint8_t st;
@@ -1160,7 +1162,11 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
             const bool even = cw.rule && (bd[i] & 1) == 0;
             b.tile_st[t] = alpha ? TILE_ALPHA : (bd[i] == 0 ? TILE_EMPTY : (even ? TILE_CLEAR : TILE_SOLID));
             if (b.dump) b.tile_bd[t] = bd[i];
-            if (!alpha && opaque && bd[i] != 0 && !even) atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row0 + i)], cw.p);
+            if (alpha) {
+                if (ai[i] < b.cap_alpha) b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (cw.rule << 8), t);
+            } else if (opaque && bd[i] != 0 && !even) {
+                atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row0 + i)], cw.p);
+            }
         }
     }
     b.col_bd[c] = running;
@@ -1198,7 +1204,7 @@ __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &s
 // One wave over the tiles [j0, j1) of a path's rect, row-major.  Count pass: returns the
 // range's {entries, alpha entries}.  Write pass: numbers them from `first` (entry, alpha entry)
 // by ballot rank.
-template <bool WRITE>
+template <bool WRITE, bool CLIPS>
 __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev &sc, const BinItem &it, uint32_t j0,
                                                uint32_t j1, uint2 first) {
     const uint32_t lane = threadIdx.x & 63;
@@ -1231,6 +1237,7 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
     fetch(j0 + lane, n_st, n_z, n_fbl);
     // write pass: a chunk's alpha indices are loaded at the end of the chunk before it
     auto alpha_of = [&](uint32_t st, uint32_t j) {
+        if (!CLIPS) return st == TILE_ALPHA ? b.tile_alpha_idx[tbase + j] : NO_ALPHA;
         return st == TILE_ALPHA ? b.tile_alpha_idx[tbase + j] : (st == TILE_CLIP_MASK ? b.tile_clip_ai[tbase + j] : NO_ALPHA);
     };
     uint32_t n_ai = WRITE ? alpha_of(n_st, j0 + lane) : NO_ALPHA;
@@ -1242,7 +1249,8 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
         et.ai = n_ai;
         et.keep = n_st != TILE_EMPTY && (int32_t)p >= (int32_t)n_z;
         fetch(jc + 64 + lane, n_st, n_z, n_fbl);
-        const uint64_t m = __ballot(et.keep), ma = __ballot(et.keep && (et.st == TILE_ALPHA || et.st == TILE_CLIP_MASK));
+        const uint64_t m = __ballot(et.keep);
+        const uint64_t ma = __ballot(et.keep && (et.st == TILE_ALPHA || (CLIPS && et.st == TILE_CLIP_MASK)));
         if (WRITE && et.keep)
             write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
                         n_alpha + (uint32_t)__popcll(ma & lt_mask));
@@ -1253,16 +1261,16 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
     return make_uint2(e, n_alpha);
 }
 
-template <bool WRITE>
+template <bool WRITE, bool CLIPS>
 __global__ __launch_bounds__(256) void k_entries_wave(BatchDev b) {
     const uint32_t item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));  // uniform
     if (item >= b.n_bin_wave) return;  // whole wave
     const BinItem it = b.bin_items[item];
     const int4 r = it.rect;
     // a clip path draws nothing: no entries
-    const uint32_t area = it.clip == CLIP_IS_CLIP ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
+    const uint32_t area = CLIPS && it.clip == CLIP_IS_CLIP ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
     const uint2 first = WRITE ? b.path_ent[it.gp] : make_uint2(0u, 0u);
-    const uint2 n = range_entries<WRITE>(b, b.scenes[it.k], it, 0, area, first);
+    const uint2 n = range_entries<WRITE, CLIPS>(b, b.scenes[it.k], it, 0, area, first);
     if (!WRITE && (threadIdx.x & 63) == 0) b.path_ent[it.gp] = n;
 }
 
@@ -1270,13 +1278,13 @@ __global__ __launch_bounds__(256) void k_entries_wave(BatchDev b) {
 // whole 64-tile chunks), so the waves need no barrier.  The count pass keeps each quarter's
 // counts (ent_quarter) beside the path's total; the write pass starts each wave after the
 // quarters before it.
-template <bool WRITE>
+template <bool WRITE, bool CLIPS>
 __global__ __launch_bounds__(256) void k_entries_group(BatchDev b) {
     const uint32_t item = blockIdx.x;
     if (item >= b.n_bin_group) return;  // uniform over the block
     const BinItem it = b.bin_items[b.n_bin_wave + item];
     const int4 r = it.rect;
-    const uint32_t area = it.clip == CLIP_IS_CLIP ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
+    const uint32_t area = CLIPS && it.clip == CLIP_IS_CLIP ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t q = (area + 255) / 256 * 64;  // tiles per quarter, whole chunks
     const uint32_t j0 = min(wv * q, area), j1 = min(j0 + q, area);
@@ -1289,7 +1297,7 @@ __global__ __launch_bounds__(256) void k_entries_group(BatchDev b) {
             first.y += c.y;
         }
     }
-    const uint2 n = range_entries<WRITE>(b, b.scenes[it.k], it, j0, j1, first);
+    const uint2 n = range_entries<WRITE, CLIPS>(b, b.scenes[it.k], it, j0, j1, first);
     if (!WRITE) {
         if (lane == 0) b.ent_quarter[item * 4 + wv] = n;
         __shared__ uint2 part[4];
@@ -1552,25 +1560,31 @@ hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s) {
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     const dim3 g(blocks_for(b.n_cols, block));
     if (!b.has_clips) {
-        hipLaunchKernelGGL(k_propagate, g, dim3(block), 0, s, b, 0u);
+        hipLaunchKernelGGL(k_propagate<false>, g, dim3(block), 0, s, b, 0u);
     } else {
-        hipLaunchKernelGGL(k_propagate, g, dim3(block), 0, s, b, 1u);
-        hipLaunchKernelGGL(k_propagate, g, dim3(block), 0, s, b, 2u);
+        hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, 1u);
+        hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, 2u);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s) {
+template <bool CLIPS>
+static void launch_entries_t(const BatchDev &b, bool write, hipStream_t s) {
     if (b.n_bin_wave) {
         const dim3 g(blocks_for(b.n_bin_wave, 4));
-        if (write) hipLaunchKernelGGL(k_entries_wave<true>, g, dim3(256), 0, s, b);
-        else hipLaunchKernelGGL(k_entries_wave<false>, g, dim3(256), 0, s, b);
+        if (write) hipLaunchKernelGGL((k_entries_wave<true, CLIPS>), g, dim3(256), 0, s, b);
+        else hipLaunchKernelGGL((k_entries_wave<false, CLIPS>), g, dim3(256), 0, s, b);
     }
     if (b.n_bin_group) {
         const dim3 g(b.n_bin_group);
-        if (write) hipLaunchKernelGGL(k_entries_group<true>, g, dim3(256), 0, s, b);
-        else hipLaunchKernelGGL(k_entries_group<false>, g, dim3(256), 0, s, b);
+        if (write) hipLaunchKernelGGL((k_entries_group<true, CLIPS>), g, dim3(256), 0, s, b);
+        else hipLaunchKernelGGL((k_entries_group<false, CLIPS>), g, dim3(256), 0, s, b);
     }
+}
+
+hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s) {
+    if (b.has_clips) launch_entries_t<true>(b, write, s);
+    else launch_entries_t<false>(b, write, s);
     return hipGetLastError();
 }
 
