@@ -227,6 +227,33 @@ def clip_scene(seed: int, size: int = 1024, n_draw: int = 200, n_clip: int = 8, 
     return base
 
 
+def concat_paths(a: Scene, b: Scene) -> Scene:
+    """The paths of a followed by those of b, on a's canvas (clip information is not carried)."""
+    return Scene(a.width, a.height, np.concatenate([a.points, b.points]),
+                 np.concatenate([a.point_flags, b.point_flags]),
+                 np.concatenate([a.contour_start, b.contour_start[1:] + a.contour_start[-1]]),
+                 np.concatenate([a.contour_closed, b.contour_closed]),
+                 np.concatenate([a.path_contour_start, b.path_contour_start[1:] + a.path_contour_start[-1]]),
+                 np.concatenate([a.path_fill_rule, b.path_fill_rule]), np.concatenate([a.path_rgba, b.path_rgba]),
+                 a.view_box.copy())
+
+
+def stress_clip_scene(seed: int = BASE_SEED ^ 4, size: int = 4096, n_stars: int = 2000, n_max: int = 2000,
+                      n_clip: int = 200, clipped: float = 0.5) -> Scene:
+    """The stress workload's star polygons (heavy overlap, windings far above 1) under clip paths:
+    n_clip blob outlines as clip paths, each star clipped by a random one with probability
+    `clipped` (single-level clips; strokes reach the rasterizer as outlines like any path)."""
+    stars = star_scene(seed=seed, size=size, n_stars=n_stars, n_max=n_max)
+    clips = blob_scene(seed ^ 0xC11, size, size, n_clip, r_max=size / 4)
+    sc = concat_paths(stars, clips)
+    rng = _rng(seed ^ 0xC12)
+    pc = rng.integers(0, n_clip, n_stars).astype(np.int32)
+    pc[rng.random(n_stars) >= clipped] = -1
+    sc.n_clip_paths = n_clip
+    sc.path_clip = np.concatenate([pc, np.full(n_clip, -1, np.int32)])
+    return sc
+
+
 def star_scene(seed: int = BASE_SEED ^ 2, size: int = 4096, n_stars: int = 2000, n_min: int = 100,
                n_max: int = 2000, m_max: int = 50, r_min: float = 16.0, r_max: float = 512.0) -> Scene:
     """Star polygons {n/m} with gcd(n, m) = 1: centre winding +-m (sign by orientation)."""

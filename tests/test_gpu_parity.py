@@ -339,3 +339,11 @@ def test_nested_clip_paths_rejected(ctx):
     scene.path_clip[0] = 2  # no clip path 2
     with pytest.raises(pfh.PfhError, match="path_clip"):
         ctx.upload(scene)
+
+
+@pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED])
+def test_stress_stars_under_clip_paths(ctx, flags):
+    """The stress workload's shape at 2048²: heavily overlapping star polygons (windings far above
+    1, both fill rules) with half of them clipped by 40 blob clip paths."""
+    scene = scenegen.stress_clip_scene(7, size=2048, n_stars=300, n_max=600, n_clip=40)
+    check_scene(ctx, scene, flags=flags, name="stress stars + clips")
