@@ -45,6 +45,8 @@ WORKLOADS = {
     "batch": "256 scenes, square canvases of side log-U[256,4096] px, 10-20000 closed paths each (log-uniform), "
              "4-64 line/quad/cubic segments per path, radii log-U[4,min(512,side/2)] px, alpha U(0.2,1), "
              "nonzero + even-odd",
+    "stress_clips": "4096x4096 scene, 2000 star polygons of 100-2000 vertices (windings far above 1, both fill "
+                    "rules) under 200 blob clip paths (single level), half of the stars clipped",
     "single": "4096x4096 scene, 10000 closed paths of 4-64 line/quad/cubic segments, radii log-U[4,512] px, "
               "alpha U(0.2,1), nonzero + even-odd",
 }
@@ -250,6 +252,17 @@ def main():
                             "autotune": tune1}
         ctx.release(sid)
         print(f"[bench] single_4k batched {b_s * 1e3:.3f} ms/step", flush=True)
+        # stress shape with clip paths: two-phase propagate and the clip combine kernel
+        stress = scenegen.stress_clip_scene()
+        sid = ctx.upload(stress)
+        p_s, s_ms, b_s, f_ms, c_ms, st2, _ = measure(ctx, [sid], flags, 10, 2, 3, ctx.schedule())
+        out["stress_clips"] = {"workload": WORKLOADS["stress_clips"], "paths_per_s": round(stress.n_paths / b_s, 1),
+                               "ms_per_step": round(b_s * 1e3, 4), "plain_ms_per_step": round(p_s * 1e3, 4),
+                               "fill_ms": round(f_ms, 4), "composite_ms": round(c_ms, 4),
+                               "counts": {k: st2[k] for k in ("lines", "fills", "alpha_tiles", "entries",
+                                                              "alpha_entries", "retries")}}
+        ctx.release(sid)
+        print(f"[bench] stress_clips batched {b_s * 1e3:.3f} ms/step", flush=True)
         # serving: request rate as a fraction of the batched throughput in scenes/s
         capacity = len(ids) / batched_s
         out["serving"] = {"requests": "Poisson arrivals, each a uniformly drawn resident scene of the workload",
