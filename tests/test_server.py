@@ -115,3 +115,27 @@ def test_same_scene_twice_waits_for_itself(served):
         assert ctx.server_stats()["launches"] == 2
     finally:
         ctx.server_stop()
+
+
+def test_clip_scenes_through_the_loop(served):
+    """Scenes with clip paths mixed with clip-free ones in the request loop (predicted capacities,
+    eager launches): every result equals the scene rasterized alone."""
+    ctx, ids, alone = served
+    clip_ids = []
+    for k in range(3):
+        sid = ctx.upload(scenegen.clip_scene(600 + k, size=640 + 128 * k, n_draw=250, n_clip=6))
+        ctx.rasterize([sid], pfh.PFH_PLAIN)
+        alone[sid] = ctx.readback(sid)
+        clip_ids.append(sid)
+    ctx.server_start(slots=2, memory_budget=16 << 30, max_batch=4,
+                     flags=pfh.PFH_PREDICT | pfh.PFH_NO_GRAPH)
+    try:
+        rid_scene = {}
+        for sid in [clip_ids[0], ids[0], clip_ids[1], ids[1], clip_ids[2], ids[2]]:
+            rid_scene[ctx.submit(sid)] = sid
+        _drain(ctx, rid_scene, alone)
+        assert ctx.server_stats()["completed"] == 6
+    finally:
+        ctx.server_stop()
+    for sid in clip_ids:
+        ctx.release(sid)
