@@ -22,7 +22,9 @@ scenes through the continuous-batching request loop: p50/p99 latency and through
 sources, on the host's cores over a bounded sample of the same scenes; tiling only: the
 reference's raster half needs a graphics pipeline CDNA lacks).
 
-``--config single`` runs the 4096^2 scene as the main workload instead.
+``--config single`` runs the 4096^2 scene as the main workload instead; ``--config stress_clips`` the
+stress shape (4096^2, 2000 star polygons under 200 single-level clip paths; its cpu_baseline tiles
+every path without the clip resolution).
 """
 import argparse
 import json
@@ -187,7 +189,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", choices=["batch", "single"], default="batch")
+    ap.add_argument("--config", choices=["batch", "single", "stress_clips"], default="batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="main workload only (no single_4k / serving)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -203,7 +205,8 @@ def main():
               f"--gpus {args.gpus} is not supported in this build", file=sys.stderr)
         sys.exit(2)
 
-    scenes = scenegen.batch_scenes() if args.config == "batch" else [scenegen.single_scene_4k()]
+    scenes = (scenegen.batch_scenes() if args.config == "batch" else
+              [scenegen.single_scene_4k()] if args.config == "single" else [scenegen.stress_clip_scene()])
     n_paths = sum(s.n_paths for s in scenes)
     pixels = sum(s.width * s.height for s in scenes)
     print(f"[bench] {args.config}: {len(scenes)} scene(s), {n_paths} paths, {pixels / 1e6:.1f} Mpix", flush=True)

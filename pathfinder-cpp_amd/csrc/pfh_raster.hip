@@ -601,44 +601,55 @@ __device__ __forceinline__ float fill_rule_f(float c, uint32_t even_odd) {
     return fminf(fmaxf(1.0f - fabsf(1.0f - m), 0.0f), 1.0f);
 }
 
-// A wave per combine job (rare: only tiles where a clipped path's and its clip's masks meet),
-// after k_fill_masks, whose unclipped mask for the tile it replaces.  Each lane owns 4 pixels
-// of one column and walks both fill lists.
+// A workgroup per combine job (rare: only tiles where a clipped path's and its clip's masks
+// meet), after k_fill_masks, whose unclipped mask for the tile it replaces.  Each lane owns 4
+// pixels of one column; the 4 waves take every 4th fill of both lists and their partial sums
+// meet in LDS, so a long fill list costs a quarter of its length in serial steps.
 __global__ __launch_bounds__(256) void k_fill_clipped(BatchDev b) {
-    const uint32_t lane = threadIdx.x & 63;
+    __shared__ float part[2][4][4][64];  // (draw, clip) x wave x pixel row offset x lane
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t n = min(b.counters[C_CLIP_JOBS], b.cap_alpha);
     const float px = (float)(lane & 15u), py0 = (float)((lane >> 4) * 4u);
-    for (uint32_t j = blockIdx.x * 4 + (threadIdx.x >> 6); j < n; j += gridDim.x * 4) {  // uniform per wave
+    for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {  // uniform over the block
         const uint2 job = b.clip_jobs[j];
-        if (job.x >= b.cap_alpha || job.y >= b.cap_alpha) continue;
-        const uint4 rd = b.alpha_rec[job.x], rc = b.alpha_rec[job.y];
-        float d[4], c[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            d[q] = (float)(int8_t)(rd.z & 0xffu);
-            c[q] = (float)(int8_t)(rc.z & 0xffu);
-        }
+        const bool ok = job.x < b.cap_alpha && job.y < b.cap_alpha;
+        const uint4 rd = ok ? b.alpha_rec[job.x] : make_uint4(0, 0, 0, 0);
+        const uint4 rc = ok ? b.alpha_rec[job.y] : make_uint4(0, 0, 0, 0);
+        float d[4] = {0.0f, 0.0f, 0.0f, 0.0f}, c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         const uint32_t d0 = min(rd.x, b.cap_fills), d1 = max(d0, min(rd.y, b.cap_fills));
-        for (uint32_t f = d0; f < d1; f++) {
+        for (uint32_t f = d0 + wv; f < d1; f += 4) {
             const uint2 fl = b.fills[f];
 #pragma unroll
             for (int q = 0; q < 4; q++) d[q] += fill_area_at(fl, px, py0 + (float)q);
         }
         const uint32_t c0 = min(rc.x, b.cap_fills), c1 = max(c0, min(rc.y, b.cap_fills));
-        for (uint32_t f = c0; f < c1; f++) {
+        for (uint32_t f = c0 + wv; f < c1; f += 4) {
             const uint2 fl = b.fills[f];
 #pragma unroll
             for (int q = 0; q < 4; q++) c[q] += fill_area_at(fl, px, py0 + (float)q);
         }
-        const uint32_t rule = (rd.z >> 8) & 1u;
-        uint32_t packed = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float v = fill_rule_f(fminf(fabsf(d[q]), fabsf(c[q])), rule);
-            packed |= (uint32_t)__float2int_rn(v * 255.0f) << (8 * q);
+            part[0][wv][q][lane] = d[q];
+            part[1][wv][q][lane] = c[q];
         }
-        // column-major mask: byte 16 x + y
-        *reinterpret_cast<uint32_t *>(b.masks + (uint64_t)job.x * 256 + (lane & 15u) * 16 + (lane >> 4) * 4) = packed;
+        __syncthreads();
+        if (wv == 0 && ok) {
+            const uint32_t rule = (rd.z >> 8) & 1u;
+            uint32_t packed = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float sd = (float)(int8_t)(rd.z & 0xffu) + part[0][0][q][lane] + part[0][1][q][lane] +
+                                 part[0][2][q][lane] + part[0][3][q][lane];
+                const float sc = (float)(int8_t)(rc.z & 0xffu) + part[1][0][q][lane] + part[1][1][q][lane] +
+                                 part[1][2][q][lane] + part[1][3][q][lane];
+                const float v = fill_rule_f(fminf(fabsf(sd), fabsf(sc)), rule);
+                packed |= (uint32_t)__float2int_rn(v * 255.0f) << (8 * q);
+            }
+            // column-major mask: byte 16 x + y
+            *reinterpret_cast<uint32_t *>(b.masks + (uint64_t)job.x * 256 + (lane & 15u) * 16 + (lane >> 4) * 4) = packed;
+        }
+        __syncthreads();  // the next job rewrites the partial sums
     }
 }
 
@@ -654,7 +665,8 @@ hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, uint32_t ma
 
 hipError_t launch_fill_clipped(const BatchDev &b, hipStream_t s) {
     if (!b.has_clips) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_clipped, dim3(256), dim3(256), 0, s, b);
+    // the job count is only known on the device: the workgroups stride over it
+    hipLaunchKernelGGL(k_fill_clipped, dim3(2048), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
