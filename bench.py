@@ -201,8 +201,9 @@ def main():
     if args.schedule:
         schedule = {k: int(v) for k, v in (kv.split("=") for kv in args.schedule.split(","))}
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: scenes shard across GPUs with no exchange (see DESIGN.md); "
-              f"--gpus {args.gpus} is not supported in this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU: scenes would shard across GPUs with no exchange (one process "
+              f"per GPU, each rasterizing its own share of the scenes); --gpus {args.gpus} is not supported",
+              file=sys.stderr)
         sys.exit(2)
 
     scenes = (scenegen.batch_scenes() if args.config == "batch" else
