@@ -16,15 +16,15 @@ k_composite) in the timed batched steps: HIP events recorded as nodes of the rep
 on the library's stream bracket each kernel; achieved = algorithmic bytes / that time.
 
 Also reported: ``single_4k`` (one 4096x4096 scene of 10 000 paths, scenegen.single_scene_4k,
-the same two modes and roofline), ``serving`` (Poisson request streams over the resident
-scenes through the continuous-batching request loop: p50/p99 latency and throughput), and
-``cpu_baseline`` (the reference's own hybrid-mode tiler, oracle/_ref compiled from its
-sources, on the host's cores over a bounded sample of the same scenes; tiling only: the
-reference's raster half needs a graphics pipeline CDNA lacks).
+the same two modes and roofline), ``stress`` (scenegen.stress_scene: 4096x4096, 2 000 star
+polygons, 500 stroked cubic polylines, 200 clip paths nesting up to 4 deep), ``serving``
+(Poisson request streams over the resident scenes through the continuous-batching request loop:
+p50/p99 latency and throughput), and ``cpu_baseline`` (the reference's hybrid path on the host's
+cores over a stratified sample of the same scenes: its own Tiler, oracle/_ref compiled from its
+sources, and a restatement of its raster half; see cpu_baseline()).
 
-``--config single`` runs the 4096^2 scene as the main workload instead; ``--config stress_clips`` the
-stress shape (4096^2, 2000 star polygons under 200 single-level clip paths; its cpu_baseline tiles
-every path without the clip resolution).
+``--config single`` runs the 4096^2 scene as the main workload instead; ``--config stress`` the
+stress scene (its cpu_baseline is null: the CPU raster restatement does not resolve clip paths).
 """
 import argparse
 import json
@@ -47,8 +47,9 @@ WORKLOADS = {
     "batch": "256 scenes, square canvases of side log-U[256,4096] px, 10-20000 closed paths each (log-uniform), "
              "4-64 line/quad/cubic segments per path, radii log-U[4,min(512,side/2)] px, alpha U(0.2,1), "
              "nonzero + even-odd",
-    "stress_clips": "4096x4096 scene, 2000 star polygons of 100-2000 vertices (windings far above 1, both fill "
-                    "rules) under 200 blob clip paths (single level), half of the stars clipped",
+    "stress": "4096x4096 scene, 2000 star polygons of 100-2000 vertices (windings up to 50, both fill rules), "
+              "500 open polylines of 4-16 cubics stroked at width U(1,20) (caps and joins of every kind), 200 "
+              "clip paths (circles and rects) in nesting chains of depth 1-4, half of the draw paths clipped",
     "single": "4096x4096 scene, 10000 closed paths of 4-64 line/quad/cubic segments, radii log-U[4,512] px, "
               "alpha U(0.2,1), nonzero + even-odd",
 }
@@ -143,28 +144,55 @@ def serve(ctx, ids, qps, n_requests, seed, slots=2, budget=64 << 30, max_batch=3
     return out
 
 
-def cpu_baseline(scenes, budget_s=15.0, threads=4):
-    """The reference's tiler over the workload's scenes, in order, until about budget_s of CPU work."""
+def cpu_baseline(scenes, reps=5, n_sample=16, threads_ref=4):
+    """The reference's hybrid path on the host's cores over a stratified sample of the workload's
+    scenes (n_sample scenes at evenly spaced ranks of path count, the largest included), each leg
+    the median of `reps` timed runs after one warm-up:
+      (i)   tiling: the reference's own Tiler (oracle/_ref, compiled from its sources) on
+            threads_ref threads (PATHFINDER_THREADS, core/d3d9/scene_builder.cpp:13);
+      (ii)  the same on every host thread of this GPU's share (at most 16);
+      (iii) end to end: (ii) plus the raster half on as many threads, a restatement of the hybrid
+            fill and per-layer composite (oracle.raster_cpu; the reference runs it as GPU raster
+            shaders, which CDNA lacks), into RGBA8.
+    `value` is leg (iii) in paths/s; Mpix/s beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # bench-only checker leg: the reference's tiler built from its sources
 
     if not oracle.ref_available():
         return None
-    oracle.time_ref(scenes[0], threads=threads, repeats=1)  # warm-up
-    paths, secs, used = 0, 0.0, 0
-    for sc in scenes:
-        if secs >= budget_s:
-            break
-        one = oracle.time_ref(sc, threads=threads, repeats=1)
-        reps = max(1, min(50, int(min(budget_s - secs, 2.0) / max(one, 1e-4))))
-        secs += oracle.time_ref(sc, threads=threads, repeats=reps)
-        paths += reps * sc.n_paths
-        used += 1
-    return {"value": round(paths / secs, 1), "unit": "paths/s", "cores": threads, "kind": "reference",
-            "sample": f"reference hybrid-mode Tiler (core/d3d9/tiler.cpp, compiled from the reference sources) "
-                      f"over the first {used} of the {len(scenes)} scenes of this workload, {paths} paths in "
-                      f"{secs:.1f} s, {threads} threads (PATHFINDER_THREADS); tiling only",
-            "host_nproc": os.cpu_count()}
+    cores = max(1, min(16, os.cpu_count() or 1))
+    order = np.argsort([s.n_paths for s in scenes], kind="stable")
+    pick = sorted(set(int(order[int(round(q * (len(order) - 1)))]) for q in np.linspace(0, 1, n_sample)))
+    sample = [scenes[k] for k in pick if not scenes[k].n_clip_paths]
+    if not sample:
+        return None
+    paths = sum(sc.n_paths for sc in sample)
+    pixels = sum(sc.width * sc.height for sc in sample)
+
+    def median_time(fn):
+        fn()
+        return float(np.median([fn() for _ in range(reps)]))
+
+    t_i = sum(median_time(lambda sc=sc: oracle.time_ref(sc, threads=threads_ref)) for sc in sample)
+    t_ii = sum(median_time(lambda sc=sc: oracle.time_ref(sc, threads=cores)) for sc in sample)
+    t_r = sum(median_time(lambda sc=sc: oracle.raster_cpu(sc, threads=cores)[1]) for sc in sample)
+    t_iii = t_ii + t_r
+    legs = {
+        "i_tiling_ref_threads": {"threads": threads_ref, "paths_per_s": round(paths / t_i, 1), "s": round(t_i, 4),
+                                 "kind": "reference"},
+        "ii_tiling_all_cores": {"threads": cores, "paths_per_s": round(paths / t_ii, 1), "s": round(t_ii, 4),
+                                "kind": "reference"},
+        "iii_end_to_end_all_cores": {"threads": cores, "paths_per_s": round(paths / t_iii, 1),
+                                     "mpix_per_s": round(pixels / t_iii / 1e6, 2), "s": round(t_iii, 4),
+                                     "raster_s": round(t_r, 4), "kind": "reference tiling + port raster"},
+    }
+    return {"value": legs["iii_end_to_end_all_cores"]["paths_per_s"], "unit": "paths/s", "cores": cores,
+            "kind": "port", "mpix_per_s": legs["iii_end_to_end_all_cores"]["mpix_per_s"],
+            "sample": f"{len(sample)} of the {len(scenes)} scenes (evenly spaced ranks of path count, the "
+                      f"largest included; {paths} paths, {pixels / 1e6:.1f} Mpix), median of {reps} runs per scene "
+                      f"and leg; value = leg iii (reference Tiler on {cores} threads + the hybrid raster half "
+                      f"restated on {cores} threads, to RGBA8)",
+            "legs": legs, "host_nproc": os.cpu_count()}
 
 
 def copy_calibration(mbytes):
@@ -189,10 +217,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", choices=["batch", "single", "stress_clips"], default="batch")
+    ap.add_argument("--config", choices=["batch", "single", "stress"], default="batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="main workload only (no single_4k / serving)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed runs per scene and leg of the CPU baseline")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run one D2D copy of this size (counter calibration)")
     ap.add_argument("--schedule", default=None,
                     help="launch schedule instead of autotuning, e.g. fill_grid=32768,composite_block=64")
@@ -207,7 +235,7 @@ def main():
         sys.exit(2)
 
     scenes = (scenegen.batch_scenes() if args.config == "batch" else
-              [scenegen.single_scene_4k()] if args.config == "single" else [scenegen.stress_clip_scene()])
+              [scenegen.single_scene_4k()] if args.config == "single" else [scenegen.stress_scene()])
     n_paths = sum(s.n_paths for s in scenes)
     pixels = sum(s.width * s.height for s in scenes)
     print(f"[bench] {args.config}: {len(scenes)} scene(s), {n_paths} paths, {pixels / 1e6:.1f} Mpix", flush=True)
@@ -256,17 +284,18 @@ def main():
                             "autotune": tune1}
         ctx.release(sid)
         print(f"[bench] single_4k batched {b_s * 1e3:.3f} ms/step", flush=True)
-        # stress shape with clip paths: two-phase propagate and the clip combine kernel
-        stress = scenegen.stress_clip_scene()
+        # the stress scene: star polygons, stroked polylines, nested clip paths (its own autotune)
+        stress = scenegen.stress_scene()
         sid = ctx.upload(stress)
-        p_s, s_ms, b_s, f_ms, c_ms, st2, _ = measure(ctx, [sid], flags, 10, 2, 3, ctx.schedule())
-        out["stress_clips"] = {"workload": WORKLOADS["stress_clips"], "paths_per_s": round(stress.n_paths / b_s, 1),
-                               "ms_per_step": round(b_s * 1e3, 4), "plain_ms_per_step": round(p_s * 1e3, 4),
-                               "fill_ms": round(f_ms, 4), "composite_ms": round(c_ms, 4),
-                               "counts": {k: st2[k] for k in ("lines", "fills", "alpha_tiles", "entries",
-                                                              "alpha_entries", "retries")}}
+        p_s, s_ms, b_s, f_ms, c_ms, st2, tune2 = measure(ctx, [sid], flags, 10, 2, 3, schedule)
+        out["stress"] = {"workload": WORKLOADS["stress"], "paths_per_s": round(stress.n_paths / b_s, 1),
+                         "ms_per_step": round(b_s * 1e3, 4), "plain_ms_per_step": round(p_s * 1e3, 4),
+                         "fill_ms": round(f_ms, 4), "composite_ms": round(c_ms, 4), "autotune": tune2,
+                         "plain_stage_ms": {k: round(v, 4) for k, v in zip(names, s_ms)},
+                         "counts": {k: st2[k] for k in ("segments", "lines", "fills", "alpha_tiles", "entries",
+                                                        "alpha_entries", "retries")}}
         ctx.release(sid)
-        print(f"[bench] stress_clips batched {b_s * 1e3:.3f} ms/step", flush=True)
+        print(f"[bench] stress batched {b_s * 1e3:.3f} ms/step", flush=True)
         # serving: request rate as a fraction of the batched throughput in scenes/s
         capacity = len(ids) / batched_s
         out["serving"] = {"requests": "Poisson arrivals, each a uniformly drawn resident scene of the workload",
@@ -275,7 +304,7 @@ def main():
                                    for k, frac in enumerate((0.5, 1.0))]}
         print("[bench] serving done", flush=True)
 
-    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(scenes, budget_s=args.cpu_budget)
+    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(scenes, reps=args.cpu_reps)
     if args.copy_mb:
         copy_calibration(args.copy_mb)
     print(json.dumps(out), flush=True)

@@ -43,16 +43,40 @@ extern "C" {
 #define PFH_LOAD_DST 0x8u   /* blend over the current target contents instead of clearing (clear_dst = false) */
 #define PFH_NO_GRAPH 0x10u  /* PFH_BATCHED without graph capture (eager launches) */
 
-/* Fill rules (core/data/data.h:166-172 FillRule). */
+/* Fill rules (core/data/data.h:58-63 FillRule). */
 #define PFH_FILL_NONZERO 0
 #define PFH_FILL_EVEN_ODD 1
 
 typedef struct pfh_ctx pfh_ctx;
 
+/* An outline: contours of points with PointFlag flags (core/data/path.h:12-57, Outline / Contour). */
+typedef struct pfh_outline {
+    int32_t n_contours, n_points;
+    const float *points;          /* [2 * n_points] */
+    const uint8_t *point_flags;   /* [n_points] PointFlag */
+    const int32_t *contour_start; /* [n_contours + 1] */
+    const uint8_t *contour_closed;/* [n_contours] */
+} pfh_outline;
+
+/* StrokeStyle (core/stroke.h:12-26); LineCap / LineJoin values of core/data/data.h:73-93. */
+#define PFH_CAP_BUTT 0
+#define PFH_CAP_SQUARE 1
+#define PFH_CAP_ROUND 2
+#define PFH_JOIN_MITER 0
+#define PFH_JOIN_BEVEL 1
+#define PFH_JOIN_ROUND 2
+typedef struct pfh_stroke_style {
+    float line_width;
+    int32_t line_cap;
+    int32_t line_join;
+    float miter_limit;
+} pfh_stroke_style;
+
+
 /*
  * A scene as outlines: the data of Scene::draw_paths (core/scene.h:57-61) after
  * Path2d/Canvas have applied transforms, strokes and dashes (core/canvas.cpp:180-301).
- * Point flags follow PointFlag (core/data/data.h:150-160): 0 on-curve,
+ * Point flags follow PointFlag (core/data/data.h:42-51): 0 on-curve,
  * 1 first control point (quadratic or cubic), 2 second control point (cubic).
  * Colours are solid paints, r | g << 8 | b << 16 | a << 24 (ColorU).
  */
@@ -69,13 +93,20 @@ typedef struct pfh_scene_desc {
     const uint32_t *path_rgba;          /* [n_paths] */
     /* Clip paths (Scene::clip_paths, core/scene.h:53-130; ClipPath::outline): the last n_clip_paths
      * paths of the arrays above are clip paths, tiled like draw paths but never drawn.  path_clip[p]
-     * (NULL: no path is clipped) is the clip path of draw path p (DrawPath::clip_path) as an index
-     * among the clip paths, or -1; clip paths themselves take -1 (nested clips are undefined in the
-     * reference's hybrid tiler, core/d3d9/scene_builder.cpp:209-216, and rejected).  Resolution and
-     * mask combine follow core/d3d9/tiler.cpp:385-431 and shaders/d3d9/tile_clip_combine.frag:25-27:
-     * the clip's own fill rule is not used (its coverage enters as |winding sum|). */
+     * (NULL: no path is clipped) is the clip path of path p (DrawPath::clip_path, or for a clip path
+     * ClipPath::clip_path) as an index among the clip paths, or -1.  Resolution and mask combine
+     * follow core/d3d9/tiler.cpp:385-431 and shaders/d3d9/tile_clip_combine.frag:25-27: the clip's
+     * own fill rule is not used (its coverage enters as |winding sum|).  Nested clips (undefined in
+     * the reference's hybrid tiler, core/d3d9/scene_builder.cpp:209-216) follow the GPU-driven
+     * renderer's levels (core/d3d11/scene_builder.cpp:92-133): a clip path is resolved against its
+     * own clip path like a draw path, and its coverage is the min over the chain; chains of at most
+     * 4 clip paths (a deeper chain or a cycle fails with PFH_E_ARG). */
     int32_t n_clip_paths;
     const int32_t *path_clip;           /* [n_paths] or NULL */
+    /* Strokes (Canvas::stroke_path, core/canvas.cpp:272-301): NULL, or per path a stroke style; a
+     * path whose line_width is above 0 is stroked on the device at upload (as pfh_stroke_outlines)
+     * and its stroke filled with the nonzero rule, whatever path_fill_rule says. */
+    const pfh_stroke_style *path_stroke;
 } pfh_scene_desc;
 
 typedef struct pfh_stats {
@@ -220,6 +251,29 @@ int pfh_server_poll(pfh_ctx *ctx, double wait_ms, int64_t *done_ids, int32_t cap
 int pfh_server_stats_get(pfh_ctx *ctx, pfh_server_stats *out);
 /* Runs every queued request to completion, then frees the slots. */
 int pfh_server_stop(pfh_ctx *ctx);
+
+/*
+ * Stroke-to-fill on the device: the outlines whose nonzero fills are the strokes of `in`'s paths,
+ * bit-identical to the reference's OutlineStrokeToFill::offset / into_outline
+ * (core/stroke.cpp:124-195), which Canvas::stroke_path runs before a stroke reaches the scene
+ * (core/canvas.cpp:272-301); here many paths are stroked at once, one style each.  Output
+ * capacities in points and contours; on return n_points / n_contours hold the sizes needed and
+ * PFH_E_LIMIT means a capacity was too small (call again with them; capacities of 0 ask for the
+ * sizes only).  out->path_contour_start [n_paths + 1] receives each path's output contours.
+ */
+typedef struct pfh_outline_buf {
+    float *points;               /* [2 * cap_points] */
+    uint8_t *point_flags;        /* [cap_points] */
+    int64_t cap_points;
+    int32_t *contour_start;      /* [cap_contours + 1] */
+    uint8_t *contour_closed;     /* [cap_contours] */
+    int64_t cap_contours;
+    int32_t *path_contour_start; /* [n_paths + 1] */
+    int64_t n_points, n_contours;
+} pfh_outline_buf;
+
+int pfh_stroke_outlines(pfh_ctx *ctx, const pfh_outline *in, const int32_t *path_contour_start, int32_t n_paths,
+                        const pfh_stroke_style *styles, pfh_outline_buf *out);
 
 #ifdef __cplusplus
 }

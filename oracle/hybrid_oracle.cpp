@@ -432,6 +432,9 @@ static void tile_path(const Scene &sc, int p, PathTiling &pt) {
 // The clip tile at the same tile coordinates decides: both masks -> a combine job (the draw tile's
 // backdrop moves into the job, the tile keeps its mask); a solid draw tile over a clip mask -> the
 // tile takes the clip's mask and backdrop; a blank clip tile, or none, -> the tile is culled.
+// A clip path with a clip of its own (nesting, undefined in the hybrid tiler) is resolved against
+// it the same way before anything it clips is (the level structure of core/d3d11/scene_builder.cpp:
+// 92-133 and shaders/d3d11/propagate.comp:133-176), so `cp` always holds resolved tiles.
 static void resolve_clip(PathTiling &pt, const PathTiling &cp, int64_t clip_tile_base) {
     int w = std::max(pt.rect[2] - pt.rect[0], 0), h = std::max(pt.rect[3] - pt.rect[1], 0);
     int cw = std::max(cp.rect[2] - cp.rect[0], 0), ch = std::max(cp.rect[3] - cp.rect[1], 0);
@@ -562,10 +565,30 @@ void *pfo_run(int n_paths, const int32_t *path_contour_start, const uint8_t *pat
     res->paths.resize(n_paths);
     const int n_draw = n_paths - n_clip;
     res->n_draw = n_draw;
+    // Nesting level of each clip path: 0 without a clip of its own, else its clip's level + 1.
+    // A chain longer than 4 clip paths, or a cycle, is refused (no result).
+    std::vector<int> level(n_clip, 0);
+    for (int q = 0; q < n_clip && path_clip; q++) {
+        int k = 0;
+        for (int c = path_clip[n_draw + q]; c >= 0; c = path_clip[n_draw + c]) {
+            if (++k > 3 || c >= n_clip) {
+                delete res;
+                return nullptr;
+            }
+        }
+        level[q] = k;
+    }
     auto t0 = std::chrono::steady_clock::now();
     for (int p = 0; p < n_paths; p++) tile_path(sc, p, res->paths[p]);
     std::vector<int64_t> tile_base(n_paths + 1, 0);
     for (int p = 0; p < n_paths; p++) tile_base[p + 1] = tile_base[p] + (int64_t)res->paths[p].backdrop.size();
+    for (int lev = 1; lev <= 3 && path_clip; lev++) {
+        for (int q = 0; q < n_clip; q++) {
+            if (level[q] != lev) continue;
+            int g = n_draw + path_clip[n_draw + q];
+            resolve_clip(res->paths[n_draw + q], res->paths[g], tile_base[g]);
+        }
+    }
     for (int p = 0; path_clip && p < n_draw; p++) {
         if (path_clip[p] < 0) continue;
         int g = n_draw + path_clip[p];
@@ -747,6 +770,137 @@ void pfo_composite(int64_t n_entries, const uint32_t *entry_fbtile, const uint32
         }
     }
     (void)fb_h;
+}
+
+// A CPU raster half for the timed baseline (bench.py's cpu_baseline leg): the hybrid path's fill
+// (shaders/d3d9/fill.frag:25-45: per alpha-tile pixel, the box-filtered signed area of the tile's
+// fills, accumulated on the backdrop, then the fill rule of shaders/d3d9/tile.frag:568-590) and
+// its per-layer src-over composite (shaders/d3d9/tile.frag, BlendState::from_over,
+// gpu/base.h:83-93) into RGBA8, in float on `threads` host threads: alpha tiles split by path,
+// framebuffer tiles split by row.  Scenes without clip paths.  Writes RGBA8 [height][width] and
+// returns the wall seconds of fill + composite.
+double pfo_raster_cpu(void *h, const uint8_t *path_even_odd, const uint32_t *path_rgba, int width, int height,
+                      int threads, uint8_t *out) {
+    auto *r = (Result *)h;
+    const int n_paths = (int)r->paths.size();
+    std::vector<int64_t> tile_base(n_paths + 1, 0), alpha_base(n_paths + 1, 0);
+    for (int p = 0; p < n_paths; p++) {
+        tile_base[p + 1] = tile_base[p] + (int64_t)r->paths[p].backdrop.size();
+        int64_t a = 0;
+        for (auto v : r->paths[p].alpha) a += v;
+        alpha_base[p + 1] = alpha_base[p] + a;
+    }
+    const int64_t n_alpha = alpha_base[n_paths];
+    std::vector<float> mask((size_t)n_alpha * 256);
+    std::vector<int64_t> tile_alpha((size_t)tile_base[n_paths], -1);
+    for (int p = 0; p < n_paths; p++) {
+        int64_t a = alpha_base[p];
+        for (size_t i = 0; i < r->paths[p].alpha.size(); i++)
+            if (r->paths[p].alpha[i]) tile_alpha[tile_base[p] + (int64_t)i] = a++;
+    }
+    if (threads < 1) threads = 1;
+    auto t0 = std::chrono::steady_clock::now();
+    auto fill = [&](int first) {
+        std::vector<float> acc;
+        for (int p = first; p < n_paths; p += threads) {
+            const PathTiling &pt = r->paths[p];
+            const int64_t a0 = alpha_base[p];
+            acc.assign((size_t)(alpha_base[p + 1] - a0) * 256, 0.0f);
+            for (const OFill &f : pt.fills) {
+                const int64_t a = tile_alpha[tile_base[p] + f.tile];
+                if (a < 0) continue;
+                float *t = acc.data() + (size_t)(a - a0) * 256;
+                const float x0 = f.x0 / 256.0f, y0 = f.y0 / 256.0f, x1 = f.x1 / 256.0f, y1 = f.y1 / 256.0f;
+                const float sign = x0 > x1 ? 1.0f : -1.0f;
+                const float xl = std::min(x0, x1), xr = std::max(x0, x1);
+                const float yl = x0 < x1 ? y0 : y1, yr = x0 < x1 ? y1 : y0;
+                const float slope = (yr - yl) / (xr - xl);
+                for (int c = (int)xl; c < 16 && (float)c < xr; c++) {
+                    const float lo = std::max((float)c, xl), hi = std::min((float)c + 1.0f, xr);
+                    if (hi <= lo) continue;
+                    const float ya = yl + (lo - xl) * slope, yb = yl + (hi - xl) * slope;
+                    const float ymin = std::min(ya, yb), ymax = std::max(ya, yb);
+                    const float w = sign * (hi - lo);
+                    const int r0 = std::max(0, (int)ymin), r1 = std::min(15, (int)ymax);
+                    for (int row = r0; row <= r1; row++) {
+                        // mean of clamp(row + 1 - y, 0, 1) over the column's stretch of the fill
+                        const float ua = row + 1.0f - ya, ub = row + 1.0f - yb;
+                        const float lo_u = std::min(ua, ub), hi_u = std::max(ua, ub);
+                        float m;
+                        if (hi_u - lo_u <= 0.0f) {
+                            m = std::min(std::max(lo_u, 0.0f), 1.0f);
+                        } else {
+                            const float full = hi_u > 1.0f ? hi_u - std::max(lo_u, 1.0f) : 0.0f;
+                            const float l = std::max(lo_u, 0.0f), u = std::min(hi_u, 1.0f);
+                            m = (full + (u > l ? (u - l) * (u + l) * 0.5f : 0.0f)) / (hi_u - lo_u);
+                        }
+                        t[row * 16 + c] += w * m;
+                    }
+                    for (int row = r1 + 1; row < 16; row++) t[row * 16 + c] += w;
+                }
+            }
+            for (size_t i = 0; i < pt.alpha.size(); i++) {
+                const int64_t a = tile_alpha[tile_base[p] + (int64_t)i];
+                if (a < 0) continue;
+                const float *t = acc.data() + (size_t)(a - a0) * 256;
+                float *m = mask.data() + (size_t)a * 256;
+                for (int k = 0; k < 256; k++) m[k] = (float)apply_rule((double)(pt.backdrop[i] + t[k]), path_even_odd[p]);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 0; i < threads; i++) pool.emplace_back(fill, i);
+    for (auto &t : pool) t.join();
+    pool.clear();
+    // entries are grouped by framebuffer tile, in draw order within one
+    const int fw = r->fb_rect[2] - r->fb_rect[0];
+    const size_t ne = r->entry_path.size();
+    std::vector<size_t> fb_start((size_t)std::max(0, fw * (r->fb_rect[3] - r->fb_rect[1])) + 1, ne);
+    for (size_t e = ne; e-- > 0;) fb_start[r->entry_fbtile[e]] = e;
+    for (size_t i = fb_start.size() - 1; i-- > 0;) fb_start[i] = std::min(fb_start[i], fb_start[i + 1]);
+    const int fh = r->fb_rect[3] - r->fb_rect[1];
+    auto composite = [&](int first) {
+        float px[16][16][4];
+        for (int ty = first; ty < fh; ty += threads) {
+            for (int tx = 0; tx < fw; tx++) {
+                const size_t fb = (size_t)ty * fw + tx;
+                std::memset(px, 0, sizeof(px));
+                for (size_t e = fb_start[fb]; e < fb_start[fb + 1]; e++) {
+                    const uint32_t p = r->entry_path[e];
+                    const uint32_t c = path_rgba[p];
+                    const float cr = (c & 255) / 255.0f, cg = ((c >> 8) & 255) / 255.0f,
+                                cb = ((c >> 16) & 255) / 255.0f, ca = (c >> 24) / 255.0f;
+                    const int64_t a = tile_alpha[r->entry_tile[e]];
+                    const PathTiling &pt = r->paths[p];
+                    const float solid = (float)apply_rule(
+                        (double)pt.backdrop[r->entry_tile[e] - tile_base[p]], path_even_odd[p]);
+                    const float *m = a >= 0 ? mask.data() + (size_t)a * 256 : nullptr;
+                    for (int k = 0; k < 256; k++) {
+                        const float sa = ca * (m ? m[k] : solid), inv = 1.0f - sa;
+                        float *d = px[k >> 4][k & 15];
+                        d[0] = cr * sa + d[0] * inv;
+                        d[1] = cg * sa + d[1] * inv;
+                        d[2] = cb * sa + d[2] * inv;
+                        d[3] = sa + d[3] * inv;
+                    }
+                }
+                for (int yy = 0; yy < 16; yy++) {
+                    const int y = (ty + r->fb_rect[1]) * 16 + yy;
+                    if (y < 0 || y >= height) continue;
+                    for (int xx = 0; xx < 16; xx++) {
+                        const int x = (tx + r->fb_rect[0]) * 16 + xx;
+                        if (x < 0 || x >= width) continue;
+                        for (int ch = 0; ch < 4; ch++)
+                            out[((size_t)y * width + x) * 4 + ch] =
+                                (uint8_t)std::lrint(std::min(std::max(px[yy][xx][ch], 0.0f), 1.0f) * 255.0f);
+                    }
+                }
+            }
+        }
+    };
+    for (int i = 0; i < threads; i++) pool.emplace_back(composite, i);
+    for (auto &t : pool) t.join();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 } // extern "C"

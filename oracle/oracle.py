@@ -58,6 +58,9 @@ def load_oracle():
         lib.pfo_apply_rule.restype = ctypes.c_double
         lib.pfo_apply_rule.argtypes = [ctypes.c_double, ctypes.c_int]
         lib.pfo_composite.argtypes = [ctypes.c_int64] + [_P] * 7 + [ctypes.c_int] * 4 + [_P]
+        lib.pfo_stroke_outline.argtypes = [_P, _P, _P, _P, ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P]
+        lib.pfo_raster_cpu.restype = ctypes.c_double
+        lib.pfo_raster_cpu.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
         _oracle = lib
     return _oracle
 
@@ -78,8 +81,79 @@ def load_ref():
         lib.pfref_free.argtypes = [_P]
         lib.pfref_time.restype = ctypes.c_double
         lib.pfref_time.argtypes = [ctypes.c_int] + [_P] * 7 + [ctypes.c_int, ctypes.c_int]
+        lib.pfref_stroke.restype = _P
+        lib.pfref_stroke.argtypes = [_P] * 4 + [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_float]
+        lib.pfref_stroke_counts.argtypes = [_P, _P]
+        lib.pfref_stroke_fetch.argtypes = [_P] * 5
+        lib.pfref_stroke_free.argtypes = [_P]
         _ref = lib
     return _ref
+
+
+class _Outline(ctypes.Structure):
+    _fields_ = [("n_contours", ctypes.c_int32), ("n_points", ctypes.c_int32), ("points", _P), ("point_flags", _P),
+                ("contour_start", _P), ("contour_closed", _P)]
+
+
+class _Style(ctypes.Structure):
+    _fields_ = [("line_width", ctypes.c_float), ("line_cap", ctypes.c_int32), ("line_join", ctypes.c_int32),
+                ("miter_limit", ctypes.c_float)]
+
+
+def stroke(points, point_flags, contour_start, contour_closed, width, cap=0, join=0, miter_limit=10.0):
+    """The oracle's stroke-to-fill (oracle/stroke_oracle.cpp, a restatement of OutlineStrokeToFill,
+    core/stroke.cpp:124-195) on one outline: (points [N, 2] f32, point_flags u8, contour_start i32,
+    contour_closed u8)."""
+    lib = load_oracle()
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 2)
+    flg = np.ascontiguousarray(point_flags, np.uint8)
+    cst = np.ascontiguousarray(contour_start, np.int32)
+    ccl = np.ascontiguousarray(contour_closed, np.uint8)
+    o = _Outline(int(ccl.shape[0]), int(flg.shape[0]), _ptr(pts), _ptr(flg), _ptr(cst), _ptr(ccl))
+    st = _Style(float(width), int(cap), int(join), float(miter_limit))
+    n_pts, n_ct = np.zeros(1, np.int64), np.zeros(1, np.int64)
+    cs0 = np.zeros(1, np.int32)
+    rc = lib.pfo_stroke_outline(ctypes.byref(o), ctypes.byref(st), None, None, 0, _ptr(cs0), None, 0, _ptr(n_pts),
+                                _ptr(n_ct))
+    if rc not in (0, -5):
+        raise ValueError(f"pfo_stroke_outline failed ({rc})")
+    out = (np.zeros((n_pts[0], 2), np.float32), np.zeros(n_pts[0], np.uint8), np.zeros(n_ct[0] + 1, np.int32),
+           np.zeros(n_ct[0], np.uint8))
+    rc = lib.pfo_stroke_outline(ctypes.byref(o), ctypes.byref(st), _ptr(out[0]), _ptr(out[1]), int(n_pts[0]),
+                                _ptr(out[2]), _ptr(out[3]), int(n_ct[0]), _ptr(n_pts), _ptr(n_ct))
+    if rc != 0:
+        raise ValueError(f"pfo_stroke_outline failed ({rc})")
+    return out
+
+
+def apply_strokes(scene):
+    """The scene with every stroked path (path_stroke width > 0) replaced by its stroked outline, filled
+    with the nonzero rule (core/canvas.cpp:272-301); the scene itself when nothing is stroked."""
+    ps = getattr(scene, "path_stroke", None)
+    if ps is None:
+        return scene
+    import scenegen  # the Scene container only
+
+    ps = np.asarray(ps)
+    if ps.dtype.names:
+        ps = np.stack([ps[k].astype(np.float64) for k in ps.dtype.names], axis=1)
+    ps = np.asarray(ps, np.float64).reshape(-1, 4)
+    outlines, rules = [], scene.path_fill_rule.copy()
+    for p in range(scene.n_paths):
+        c0, c1 = int(scene.path_contour_start[p]), int(scene.path_contour_start[p + 1])
+        a, b = int(scene.contour_start[c0]), int(scene.contour_start[c1])
+        o = (scene.points[a:b], scene.point_flags[a:b], scene.contour_start[c0:c1 + 1] - a,
+             scene.contour_closed[c0:c1])
+        if ps[p, 0] > 0:
+            o = stroke(*o, ps[p, 0], int(ps[p, 1]), int(ps[p, 2]), ps[p, 3])
+            rules[p] = 0
+        outlines.append(o)
+    out = scenegen.outlines_scene(scene.width, scene.height, outlines, rules, scene.path_rgba)
+    out.view_box = scene.view_box.copy()
+    out.n_clip_paths = scene.n_clip_paths
+    out.path_clip = scene.path_clip
+    return out
 
 
 def _scene_args(scene):
@@ -113,7 +187,10 @@ def run_oracle(scene, zmode: int = 1) -> dict:
     zmode 0: the reference's z-buffer rule; 1: even-odd tiles with an even
     backdrop never occlude (the rule the HIP path implements)."""
     lib = load_oracle()
+    scene = apply_strokes(scene)
     h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(zmode), *_clip_args(scene))
+    if not h:
+        raise ValueError("clip paths nest deeper than 4 or in a cycle")
     counts = np.zeros(8, np.int64)
     lib.pfo_counts(h, _ptr(counts))
     out = _alloc(counts)
@@ -154,6 +231,20 @@ def run_ref(scene, threads: int = 1) -> dict:
     return out
 
 
+def raster_cpu(scene, threads: int = 1):
+    """The CPU raster half on the oracle's tiling (identical to the reference's): float fill and
+    per-layer src-over composite on `threads` host threads.  Returns (RGBA8 [H, W, 4], seconds of
+    fill + composite).  Scenes without clip paths."""
+    assert not scene.n_clip_paths, "the CPU raster baseline covers scenes without clip paths"
+    lib = load_oracle()
+    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(1), ctypes.c_int(0), None)
+    out = np.zeros((scene.height, scene.width, 4), np.uint8)
+    secs = lib.pfo_raster_cpu(h, _ptr(scene.path_fill_rule), _ptr(scene.path_rgba), scene.width, scene.height,
+                              int(threads), _ptr(out))
+    lib.pfo_free(h)
+    return out, float(secs)
+
+
 def time_ref(scene, threads: int = 4, repeats: int = 1) -> float:
     """Wall seconds for the reference Tiler to tile every path of ``scene`` ``repeats`` times."""
     lib = load_ref()
@@ -161,6 +252,25 @@ def time_ref(scene, threads: int = 4, repeats: int = 1) -> float:
                           _ptr(scene.contour_start), _ptr(scene.contour_closed), _ptr(scene.points),
                           _ptr(scene.point_flags), _ptr(scene.view_box), ctypes.c_int(threads),
                           ctypes.c_int(repeats))
+
+
+def ref_stroke(points, point_flags, contour_start, contour_closed, width, cap=0, join=0, miter_limit=10.0):
+    """The reference's OutlineStrokeToFill (core/stroke.cpp:124-195), compiled from its sources, on one
+    outline: (points [N, 2] f32, point_flags u8, contour_start i32, contour_closed u8)."""
+    lib = load_ref()
+    pts = np.ascontiguousarray(points, np.float32)
+    flg = np.ascontiguousarray(point_flags, np.uint8)
+    cst = np.ascontiguousarray(contour_start, np.int32)
+    ccl = np.ascontiguousarray(contour_closed, np.uint8)
+    h = lib.pfref_stroke(_ptr(pts), _ptr(flg), _ptr(cst), _ptr(ccl), int(ccl.shape[0]), float(width), int(cap),
+                         int(join), float(miter_limit))
+    counts = np.zeros(2, np.int64)
+    lib.pfref_stroke_counts(h, _ptr(counts))
+    out = (np.zeros((counts[0], 2), np.float32), np.zeros(counts[0], np.uint8), np.zeros(counts[1] + 1, np.int32),
+           np.zeros(counts[1], np.uint8))
+    lib.pfref_stroke_fetch(h, *[_ptr(a) for a in out])
+    lib.pfref_stroke_free(h)
+    return out
 
 
 def alpha_tiles(canon: dict, scene) -> dict:
@@ -226,37 +336,63 @@ def apply_rule(c: np.ndarray, even_odd: np.ndarray) -> np.ndarray:
     return out
 
 
-def clipped_coverage(canon: dict, at: dict) -> np.ndarray:
+def clip_levels(scene) -> np.ndarray:
+    """Per path: 0 for a draw path, 1 + the nesting level for a clip path (1: no clip of its own,
+    2: clipped by a level-1 clip path, ... up to 4)."""
+    n_clip = int(scene.n_clip_paths)
+    lev = np.zeros(scene.n_paths, np.int64)
+    if n_clip == 0:
+        return lev
+    n_draw = scene.n_paths - n_clip
+    pc = scene.path_clip if scene.path_clip is not None else np.full(scene.n_paths, -1, np.int32)
+    for q in range(n_clip):
+        k, c = 1, int(pc[n_draw + q])
+        while c >= 0:
+            k += 1
+            c = int(pc[n_draw + c])
+        lev[n_draw + q] = k
+    return lev
+
+
+def clipped_coverage(canon: dict, at: dict, scene) -> np.ndarray:
     """FP64 coverage of every alpha tile after clip resolution (core/d3d9/tiler.cpp:385-431) and the
     clip combine pass (shaders/d3d9/tile_clip_combine.frag:25-27, then the draw path's fill rule in
-    shaders/d3d9/tile.frag):
-      * a tile with a combine job: rule(min(|own sum + own backdrop|, |clip tile's sum + clip backdrop|));
-      * a solid tile that took the clip tile's mask: rule(clip tile's sum + clip backdrop);
-      * any other alpha tile: rule(own sum + backdrop); a clip path's own tiles under the nonzero
-        rule, as their coverage enters the combine (abs()) whatever the clip's fill rule."""
+    shaders/d3d9/tile.frag).  A clip path's coverage value K, per pixel, is |own sum + backdrop| for
+    a clip path without a clip of its own (its fill rule is not used), and follows the same cases
+    against its own clip for a nested one; a draw path then takes:
+      * a tile with a combine job: rule(min(|own sum + own backdrop|, K of the clip tile));
+      * a solid tile that took the clip tile's mask: rule(K of the clip tile);
+      * any other alpha tile: rule(own sum + backdrop).
+    A clip path's own masks (its dump rows) hold the nonzero rule of its K: min(K, 1)."""
     raw = coverage_fp64(at, raw=True)
     src = canon["clip_src"][at["tile"]]
     comb = canon["clip_combine"][at["tile"]].astype(bool)
+    comb_bd = canon["clip_comb_bd"][at["tile"]].astype(np.float64)[:, None, None]
+    level = clip_levels(scene)[at["path"]]
     eff = raw.copy()
-    has = src >= 0
-    s_raw = np.abs(raw[at["alpha_index"][src[has]]])
-    d_raw = np.abs(raw[has] + canon["clip_comb_bd"][at["tile"][has]].astype(np.float64)[:, None, None])
-    eff[has] = np.where(comb[has][:, None, None], np.minimum(d_raw, s_raw), s_raw)
-    n_draw = canon["tile_off"].shape[0] - 1 - scene_n_clip(canon)
-    eo = np.where(at["path"] >= n_draw, 0, at["even_odd"]).astype(np.uint8)
+    K = np.abs(raw)
+    # outermost clip level first: a tile's source clip tile lies one level further out, so its K
+    # is final by the time the tile reads it; draw paths (level 0) last
+    for lev in sorted(set(int(v) for v in level[level > 0])) + [0]:
+        sel = np.nonzero((level == lev) & (src >= 0))[0]
+        if sel.size == 0:
+            continue
+        k_src = K[at["alpha_index"][src[sel]]]
+        own = np.abs(raw[sel] + comb_bd[sel])
+        val = np.where(comb[sel][:, None, None], np.minimum(own, k_src), k_src)
+        K[sel] = val
+        eff[sel] = val
+    eo = np.where(level > 0, 0, at["even_odd"]).astype(np.uint8)
     return apply_rule(eff, eo)
 
 
-def scene_n_clip(canon: dict) -> int:
-    return int(canon.get("n_clip_paths", 0))
-
-
 def full_model(scene, zmode: int = 1) -> dict:
-    """Canonical tiling, FP64 coverage and FP64 composite of a scene."""
+    """Canonical tiling, FP64 coverage and FP64 composite of a scene (its stroked paths stroked first)."""
+    scene = apply_strokes(scene)
     canon = run_oracle(scene, zmode)
     canon["n_clip_paths"] = int(scene.n_clip_paths)
     at = alpha_tiles(canon, scene)
-    cov = clipped_coverage(canon, at) if scene.n_clip_paths else coverage_fp64(at)
+    cov = clipped_coverage(canon, at, scene) if scene.n_clip_paths else coverage_fp64(at)
     img = composite_fp64(canon, at, cov, scene)
     return dict(canon=canon, alpha=at, coverage=cov, image=img)
 

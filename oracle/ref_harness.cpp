@@ -17,6 +17,7 @@
 #include "core/d3d9/tiler.h"
 #include "core/data/built_path.h"
 #include "core/scene.h"
+#include "core/stroke.h"
 
 namespace Pathfinder {
 // Defined with external linkage in core/d3d9/scene_builder.cpp:18 (not in a header).
@@ -312,6 +313,59 @@ void pfref_fetch_clip(void *h, uint8_t *combine, int8_t *comb_bd) {
 
 void pfref_free(void *h) {
     delete (RefResult *)h;
+}
+
+// The reference's stroke-to-fill (OutlineStrokeToFill::offset + into_outline, core/stroke.cpp:124-195)
+// on one outline; fetch with pfref_stroke_counts / pfref_stroke_fetch, free with pfref_stroke_free.
+struct RefStroke {
+    std::vector<float> points;
+    std::vector<uint8_t> flags;
+    std::vector<int32_t> contour_start{0};
+    std::vector<uint8_t> closed;
+};
+
+void *pfref_stroke(const float *points, const uint8_t *point_flags, const int32_t *contour_start,
+                   const uint8_t *contour_closed, int n_contours, float width, int cap, int join, float miter_limit) {
+    Outline in = make_outline(points, point_flags, contour_start, contour_closed, 0, n_contours);
+    StrokeStyle style;
+    style.line_width = width;
+    style.line_cap = (LineCap)cap;
+    style.line_join = (LineJoin)join;
+    style.miter_limit = miter_limit;
+    OutlineStrokeToFill stroker(in, style);
+    stroker.offset();
+    Outline out = stroker.into_outline();
+    auto *r = new RefStroke();
+    for (const Contour &c : out.contours) {
+        for (size_t i = 0; i < c.points.size(); i++) {
+            r->points.push_back(c.points[i].x);
+            r->points.push_back(c.points[i].y);
+            r->flags.push_back((uint8_t)c.flags[i]);
+        }
+        r->contour_start.push_back((int32_t)(r->flags.size()));
+        r->closed.push_back(c.closed ? 1 : 0);
+    }
+    return r;
+}
+
+void pfref_stroke_counts(void *h, int64_t *counts) {
+    auto *r = (RefStroke *)h;
+    counts[0] = (int64_t)r->flags.size();
+    counts[1] = (int64_t)r->closed.size();
+}
+
+void pfref_stroke_fetch(void *h, float *points, uint8_t *flags, int32_t *contour_start, uint8_t *closed) {
+    auto *r = (RefStroke *)h;
+    if (!r->flags.empty()) {
+        memcpy(points, r->points.data(), r->points.size() * sizeof(float));
+        memcpy(flags, r->flags.data(), r->flags.size());
+    }
+    memcpy(contour_start, r->contour_start.data(), r->contour_start.size() * sizeof(int32_t));
+    if (!r->closed.empty()) memcpy(closed, r->closed.data(), r->closed.size());
+}
+
+void pfref_stroke_free(void *h) {
+    delete (RefStroke *)h;
 }
 
 } // extern "C"

@@ -15,13 +15,18 @@ constexpr int MAX_FLATTEN_DEPTH = 16;
 constexpr int FIX_SHIFT = 20;            // fixed-point coverage accumulation: 2^-20 px^2 units
 constexpr uint32_t NO_ALPHA = 0xffffffffu;
 
-// Clip reference of a path (SceneDev::path_clip, then BinItem::clip).  A draw path clipped by a
-// clip path holds that path's binning position: its index in the scene's wave-class list, or
-// CLIP_GROUP | its index in the group-class list (BinItem::clip: the batch bin item index).
-constexpr uint32_t CLIP_NONE = 0xffffffffu;     // a draw path without a clip path
-constexpr uint32_t CLIP_IS_CLIP = 0xfffffffeu;  // a clip path: tiled, never drawn
+// Clip reference of a path (SceneDev::path_clip, then BinItem::clip).  A path clipped by a clip
+// path (a draw path, or a clip path nested in another) holds that clip path's binning position: its
+// index in the scene's wave-class list, or CLIP_GROUP | its index in the group-class list
+// (BinItem::clip: the batch bin item index).  Whether a path is a clip path is its clip level
+// (SceneDev::path_level, BinItem::level): 0 for a draw path, 1 + its nesting depth for a clip path.
+constexpr uint32_t CLIP_NONE = 0xffffffffu;     // not clipped
 constexpr uint32_t CLIP_EMPTY = 0xfffffffdu;    // clipped by a clip path with no tile columns: all culled
 constexpr uint32_t CLIP_GROUP = 0x80000000u;
+constexpr uint32_t MAX_CLIP_LEVELS = 4;         // clip paths in one nesting chain
+// clip_jobs[j].x: the mask row a combine job writes; CLIP_JOB_CLIP marks a nested clip path's tile,
+// whose two rows (nonzero, even-odd) both take the chain's coverage
+constexpr uint32_t CLIP_JOB_CLIP = 0x80000000u;
 
 // Binning ownership classes (a path's tiles and columns belong to it alone, so the wave or
 // workgroup that bins a path can count in LDS and store plainly).  A path goes to the wave
@@ -104,7 +109,8 @@ enum Counter {
     C_CLIP_JOBS = 7,   // clip combine jobs (alpha tiles over clip tiles with fills)
     C_DRAWN = 8,       // [C_DRAWN, C_DRAWN + DRAWN_SLOTS): fills the fill kernel read, spread over
                        // slots (per block, by block index) so that no single address takes every add
-    C_COUNT = 8 + 64
+    C_CLIP_DUPS = 8 + 64,  // draw entries whose clip mask row an earlier entry already listed
+    C_COUNT = 8 + 64 + 1
 };
 constexpr int DRAWN_SLOTS = 64;
 
@@ -123,6 +129,7 @@ struct SceneDev {
     const uint32_t *bin_paths;    // local path indices: wave class, then group class
     const uint8_t *path_rule;     // 0 nonzero, 1 even-odd
     const uint32_t *path_clip;    // [n_paths] clip reference (CLIP_*), or null: no clip paths
+    const uint8_t *path_level;    // [n_paths] clip level (0 draw path, 1..4 clip path), or null
     float vb_l, vb_t, vb_r, vb_b; // view box (right edge rounded up to 16)
     int32_t fb_x0, fb_y0, fb_w, fb_h;
     int32_t width, height;
@@ -144,8 +151,9 @@ struct BinItem {
     uint32_t c0, c1;           // its chords [c0, c1)
     uint32_t p, gp, k;         // local path, batch path, scene
     uint32_t rgba, rule;       // paint, fill rule
-    uint32_t clip;             // CLIP_NONE / CLIP_IS_CLIP / CLIP_EMPTY, or the clip path's bin item
-    uint32_t pad[2];
+    uint32_t clip;             // CLIP_NONE / CLIP_EMPTY, or the bin item of the path's clip path
+    uint32_t level;            // 0: a draw path; 1..4: a clip path at that nesting depth (never drawn)
+    uint32_t pad;
 };
 
 // Batch-wide work arrays (device pointers).  Sizes: S segments, T tiles,
@@ -169,8 +177,16 @@ struct BatchDev {
     uint32_t *tile_fill_off;      // [T + 1] exclusive scan of fill counts
     uint32_t *tile_alpha_idx;     // [T + 1] exclusive scan of alpha flags
     int8_t *tile_bd;              // [T] propagated backdrop (written with dumps or clip paths only)
-    uint32_t *tile_clip_ai;       // [T] with clip paths: mask row of a TILE_CLIP_MASK tile
-    uint2 *clip_jobs;             // [A] with clip paths: (draw alpha index, clip alpha index) to combine
+    uint32_t *tile_clip_ai;       // [T] with clip paths: mask row of a TILE_CLIP_MASK tile; for a clip
+                                  // path's tile with a mask, its coverage source (below)
+    uint2 *clip_jobs;             // [A] with clip paths: (mask row | CLIP_JOB_CLIP, coverage source)
+    // A clip path's coverage at a tile is min over a chain of terms |fills + backdrop of a mask row|.
+    // Its source s (tile_clip_ai) is the mask row whose record holds the chain: clip_rec[s] =
+    // (n terms, t1, t2, t3), the first term being row s itself; rows s, s + 1 hold that coverage
+    // under the nonzero and the even-odd rule.
+    uint4 *clip_rec;              // [A] with clip paths
+    uint32_t *clip_seen;          // [A / 32 + 1] with clip paths: mask rows already listed for the fill
+    uint32_t clip_levels;         // clip levels present in the batch (0: no clip paths)
     uint8_t *tile_st;             // [T] tile status (TileStatus)
     uint2 *path_ent;              // [P + 1] {draw entries, alpha entries} per batch path, then exclusive scan
     uint2 *ent_quarter;           // [4 n_bin_group] {draw entries, alpha entries} per quarter of a group-class rect

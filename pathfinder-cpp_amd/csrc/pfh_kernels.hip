@@ -649,7 +649,7 @@ __device__ __forceinline__ U2 wave_sum_u2(U2 v) {
 }
 
 // Inclusive prefix sums of (count, count > 0) across the wave.
-__device__ __forceinline__ uint32_t alpha_rows(const BinItem &it) { return it.clip == CLIP_IS_CLIP ? 2u : 1u; }
+__device__ __forceinline__ uint32_t alpha_rows(const BinItem &it) { return it.level != 0 ? 2u : 1u; }
 
 __device__ __forceinline__ U2 wave_inclusive_u2(uint32_t c) {
     U2 v{c, c > 0 ? 1u : 0u};
@@ -736,7 +736,8 @@ __global__ __launch_bounds__(256) void k_bin_items(BatchDev b) {
     if (cr < CLIP_EMPTY)
         cr = (cr & CLIP_GROUP) ? b.n_bin_wave + sc.bin_group_base + (cr & ~CLIP_GROUP) : sc.bin_wave_base + cr;
     it.clip = cr;
-    it.pad[0] = it.pad[1] = 0;
+    it.level = sc.path_level ? (uint32_t)sc.path_level[p] : 0u;
+    it.pad = 0;
     const uint32_t gi = (GROUP ? b.n_bin_wave : 0u) + i;
     b.bin_items[gi] = it;
     const uint32_t tw = (uint32_t)max(it.rect.z - it.rect.x, 0);
@@ -1029,7 +1030,7 @@ constexpr int COL_ROWS = 8;
 
 // Where a tile of a path column lands in its scene's framebuffer-tile grid.
 struct ColumnWalk {
-    uint32_t p, rgba, rule, tbase, clip;
+    uint32_t p, rgba, rule, tbase, clip, level;
     int rx, ry;  // the path's rect origin (tiles)
     int w, h, x, fb_row0;  // fb_row0: fb tile index (scene-local) of row 0
     uint32_t fb_base, fb_w;
@@ -1046,6 +1047,7 @@ struct ColumnWalk {
         rgba = it.rgba;
         rule = it.rule;
         clip = it.clip;
+        level = it.level;
         rx = r.x;
         ry = r.y;
         fb_base = sc.fb_base;
@@ -1056,36 +1058,44 @@ struct ColumnWalk {
     __device__ uint32_t fb_local(int row) const { return (uint32_t)(fb_row0 + row * (int)fb_w); }
 };
 
-// Clip resolution of one draw tile (core/d3d9/tiler.cpp:385-431) against the clip path's tile at
-// the same tile coordinates, whose backdrop and fills are final (the clip paths' columns are
-// propagated by an earlier launch).  Returns the tile's status; bd becomes its backdrop.
+// A clip path's tile at given tile coordinates, as resolved by the propagate phase of its clip
+// level (an earlier launch): whether it has a coverage mask (its own, or one taken from its own
+// clip), its backdrop, and its coverage source (the mask row whose record holds the chain).
 struct ClipTile {
     bool inside, alpha;
     int8_t bd;
-    uint32_t ai;
+    uint32_t src;
 };
 __device__ __forceinline__ ClipTile clip_tile_at(const BatchDev &b, const BinItem &ci, int tx, int ty) {
     ClipTile r{false, false, 0, 0};
     const int4 cr = ci.rect;
     if (tx < cr.x || tx >= cr.z || ty < cr.y || ty >= cr.w) return r;
     const uint32_t ct = ci.tbase + (uint32_t)((ty - cr.y) * (cr.z - cr.x) + (tx - cr.x));
+    const uint8_t st = b.tile_st[ct];
     r.inside = true;
-    r.alpha = b.tile_fill_off[ct + 1] > b.tile_fill_off[ct];
+    r.alpha = st == TILE_ALPHA || st == TILE_CLIP_MASK;
     r.bd = b.tile_bd[ct];
-    r.ai = b.tile_alpha_idx[ct];
+    r.src = r.alpha ? b.tile_clip_ai[ct] : 0u;
     return r;
 }
 
-// phase 0: every column (no clip paths in the batch); 1: clip paths' columns only; 2: draw
-// paths' columns only, after phase 1 (a clipped draw tile reads its clip path's final tiles).
+// Column walk down a path's tiles (core/d3d9/tiler.cpp:369-439): backdrops, tile status, alpha
+// records, z-buffer, and clip resolution (tiler.cpp:385-431) against the clip path's resolved
+// tile at the same coordinates: outside its rect or over a blank tile -> culled; both with masks ->
+// a combine job (the tile's backdrop moves into it); a solid tile over a mask -> it takes that
+// mask; otherwise unchanged (a solid clip tile passes the tile through).
+// phase 0: every column (no clip paths in the batch); 1..MAX_CLIP_LEVELS: the columns of clip paths
+// of that level, a clip path nested in another being resolved against it like a draw path (the
+// level structure of core/d3d11/scene_builder.cpp:92-133, shaders/d3d11/propagate.comp:133-176);
+// MAX_CLIP_LEVELS + 1: the draw paths' columns.  Each phase reads only tiles of earlier phases.
 // CLIPS: the batch has clip paths (without them the kernel is the plain per-column walk).
 template <bool CLIPS>
 __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.n_cols) return;
     const ColumnWalk cw(b, c);
-    const bool is_clip = CLIPS && cw.clip == CLIP_IS_CLIP;
-    if (CLIPS && ((phase == 1 && !is_clip) || (phase == 2 && is_clip))) return;
+    if (CLIPS && cw.level != (phase > MAX_CLIP_LEVELS ? 0u : phase)) return;
+    const bool is_clip = CLIPS && cw.level != 0;
     const bool clipped = CLIPS && (cw.clip < CLIP_EMPTY || cw.clip == CLIP_EMPTY);
     BinItem ci;
     if (CLIPS && cw.clip < CLIP_EMPTY) ci = b.bin_items[cw.clip];
@@ -1119,35 +1129,60 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
             if (row0 + i >= cw.h) break;
             const uint32_t t = cw.tile(row0 + i);
             const bool alpha = f1[i] > f0[i];
+            ClipTile ct{false, false, 0, 0};
+            if (CLIPS && clipped && cw.clip != CLIP_EMPTY) ct = clip_tile_at(b, ci, cw.rx + cw.x, cw.ry + row0 + i);
             if (CLIPS && is_clip) {
-                // a clip path draws nothing itself: its masks keep |sum + backdrop| under both
-                // rules (rows ai, ai + 1) for the draw tiles that take them over
-                b.tile_st[t] = alpha ? TILE_ALPHA : TILE_EMPTY;
-                b.tile_bd[t] = bd[i];
-                if (alpha && ai[i] + 1 < b.cap_alpha) {
+                // A clip path draws nothing itself.  Its fills make two mask rows (ai, ai + 1) that
+                // keep |sum + backdrop| under the nonzero and the even-odd rule, for the tiles that
+                // take them over; its coverage at the tile is that value, min'd with its own clip's
+                // where it is nested and both have masks (a combine job rewrites both rows).
+                const bool own = alpha && ai[i] + 1 < b.cap_alpha;
+                if (own) {
                     b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i], t);
                     b.alpha_rec[ai[i] + 1] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (1u << 8), t);
                 }
+                uint8_t st = alpha ? TILE_ALPHA : (bd[i] != 0 ? TILE_SOLID : TILE_EMPTY);
+                int8_t out_bd = bd[i];
+                uint32_t src = ai[i];
+                uint4 rec = make_uint4(1u, 0u, 0u, 0u);  // the chain: this row alone
+                if (clipped) {
+                    if (!ct.inside || (!ct.alpha && ct.bd == 0)) {
+                        st = TILE_EMPTY;
+                        out_bd = 0;
+                    } else if (ct.alpha && alpha) {
+                        out_bd = 0;
+                        const uint4 pr = ct.src < b.cap_alpha ? b.clip_rec[ct.src] : make_uint4(1u, 0u, 0u, 0u);
+                        rec = make_uint4(1u + min(pr.x, MAX_CLIP_LEVELS - 1u), ct.src, pr.y, pr.z);
+                        const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
+                        if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai[i] | CLIP_JOB_CLIP, ai[i]);
+                    } else if (ct.alpha && bd[i] != 0) {
+                        st = TILE_CLIP_MASK;  // its own clip's coverage, unchanged
+                        out_bd = ct.bd;
+                        src = ct.src;
+                    }
+                }
+                if (st == TILE_ALPHA && own) b.clip_rec[ai[i]] = rec;
+                if (st == TILE_ALPHA || st == TILE_CLIP_MASK) b.tile_clip_ai[t] = src;
+                b.tile_st[t] = st;
+                b.tile_bd[t] = out_bd;
                 continue;
             }
             if (CLIPS && clipped) {
                 if (alpha && ai[i] < b.cap_alpha)
                     b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (cw.rule << 8), t);
-                ClipTile ct{false, false, 0, 0};
-                if (cw.clip != CLIP_EMPTY) ct = clip_tile_at(b, ci, cw.rx + cw.x, cw.ry + row0 + i);
                 uint8_t st;
                 int8_t out_bd = 0;
                 if (!ct.inside || (!ct.alpha && ct.bd == 0)) {
                     st = TILE_EMPTY;  // outside the clip's rect or over a blank clip tile: culled
                 } else if (ct.alpha && alpha) {
-                    // both masks: min(|own + backdrop|, |clip|) replaces the mask; backdrop 0
+                    // both masks: min(|own + backdrop|, clip coverage) replaces the mask; backdrop 0
                     st = TILE_ALPHA;
                     const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
-                    if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai[i], ct.ai);
+                    if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai[i], ct.src);
                 } else if (ct.alpha && bd[i] != 0) {
                     st = TILE_CLIP_MASK;  // solid over a clip mask: the clip's mask and backdrop
                     out_bd = ct.bd;
-                    b.tile_clip_ai[t] = ct.ai + cw.rule;
+                    b.tile_clip_ai[t] = ct.src + cw.rule;
                 } else {
                     // unchanged: an alpha tile, or a solid tile over a solid clip tile
                     const bool even = cw.rule && (bd[i] & 1) == 0;
@@ -1188,12 +1223,26 @@ struct EntryTile {
 
 // e: the entry's index; ea: its index among alpha entries (for an alpha tile).  The alpha
 // entries' tiles are listed in that order for the fill kernel, which rasterizes only them.
+// With clip paths, the solid tiles of several draw paths can take one clip mask row: the first
+// entry to list it claims it in clip_seen, the others list nothing (NO_ALPHA: the fill kernel
+// skips the slot), so the row is rasterized once.
+template <bool CLIPS>
 __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t rgba,
                                             uint32_t tbase, uint32_t j, const EntryTile &et, uint32_t e,
                                             uint32_t ea) {
     if (e >= b.cap_entries) return;  // past the capacity: the batch is retried
     const uint32_t ai = (et.st == TILE_ALPHA || et.st == TILE_CLIP_MASK) ? et.ai : NO_ALPHA;
-    if (ai != NO_ALPHA && ea < b.cap_vis) b.alpha_vis[ea] = ai;
+    if (ai != NO_ALPHA && ea < b.cap_vis) {
+        uint32_t listed = ai;
+        if (CLIPS && et.st == TILE_CLIP_MASK && ai < b.cap_alpha) {
+            const uint32_t bit = 1u << (ai & 31u);
+            if (atomicOr(&b.clip_seen[ai >> 5], bit) & bit) {
+                listed = NO_ALPHA;
+                atomicAdd(&b.counters[C_CLIP_DUPS], 1u);
+            }
+        }
+        b.alpha_vis[ea] = listed;
+    }
     // even-odd solid tile with an even winding covers nothing: zero alpha
     const uint32_t eff = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
     b.keys[e] = sc.fb_base + et.fbl;  // its value, e, is generated by the sort's first pass
@@ -1252,7 +1301,7 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
         const uint64_t m = __ballot(et.keep);
         const uint64_t ma = __ballot(et.keep && (et.st == TILE_ALPHA || (CLIPS && et.st == TILE_CLIP_MASK)));
         if (WRITE && et.keep)
-            write_entry(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
+            write_entry<CLIPS>(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
                         n_alpha + (uint32_t)__popcll(ma & lt_mask));
         e += (uint32_t)__popcll(m);
         n_alpha += (uint32_t)__popcll(ma);
@@ -1268,7 +1317,7 @@ __global__ __launch_bounds__(256) void k_entries_wave(BatchDev b) {
     const BinItem it = b.bin_items[item];
     const int4 r = it.rect;
     // a clip path draws nothing: no entries
-    const uint32_t area = CLIPS && it.clip == CLIP_IS_CLIP ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
+    const uint32_t area = CLIPS && it.level != 0 ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
     const uint2 first = WRITE ? b.path_ent[it.gp] : make_uint2(0u, 0u);
     const uint2 n = range_entries<WRITE, CLIPS>(b, b.scenes[it.k], it, 0, area, first);
     if (!WRITE && (threadIdx.x & 63) == 0) b.path_ent[it.gp] = n;
@@ -1284,7 +1333,7 @@ __global__ __launch_bounds__(256) void k_entries_group(BatchDev b) {
     if (item >= b.n_bin_group) return;  // uniform over the block
     const BinItem it = b.bin_items[b.n_bin_wave + item];
     const int4 r = it.rect;
-    const uint32_t area = CLIPS && it.clip == CLIP_IS_CLIP ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
+    const uint32_t area = CLIPS && it.level != 0 ? 0u : (uint32_t)max(r.z - r.x, 0) * (uint32_t)max(r.w - r.y, 0);
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t q = (area + 255) / 256 * 64;  // tiles per quarter, whole chunks
     const uint32_t j0 = min(wv * q, area), j1 = min(j0 + q, area);
@@ -1562,8 +1611,10 @@ hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s) {
     if (!b.has_clips) {
         hipLaunchKernelGGL(k_propagate<false>, g, dim3(block), 0, s, b, 0u);
     } else {
-        hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, 1u);
-        hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, 2u);
+        // clip levels from the outermost in, then the draw paths
+        for (uint32_t level = 1; level <= std::min(b.clip_levels, MAX_CLIP_LEVELS); level++)
+            hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, level);
+        hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, MAX_CLIP_LEVELS + 1);
     }
     return hipGetLastError();
 }

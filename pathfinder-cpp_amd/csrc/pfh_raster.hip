@@ -602,52 +602,71 @@ __device__ __forceinline__ float fill_rule_f(float c, uint32_t even_odd) {
 }
 
 // A workgroup per combine job (rare: only tiles where a clipped path's and its clip's masks
-// meet), after k_fill_masks, whose unclipped mask for the tile it replaces.  Each lane owns 4
-// pixels of one column; the 4 waves take every 4th fill of both lists and their partial sums
-// meet in LDS, so a long fill list costs a quarter of its length in serial steps.
+// meet), after k_fill_masks, whose mask for the row it replaces.  The coverage is the min over a
+// chain of terms |fills + backdrop| of mask rows: the draw tile's own row (a draw job), then the
+// clip's coverage source and the rows its record lists (the clip chain, up to MAX_CLIP_LEVELS).
+// A draw job writes the draw path's fill rule of it to its row; a nested clip path's job writes
+// the nonzero and the even-odd rule of it to its two rows.  Each lane owns 4 pixels of one column;
+// the 4 waves take every 4th fill of every term and their partial sums meet in LDS, so a long fill
+// list costs a quarter of its length in serial steps.
+constexpr int CLIP_TERMS = MAX_CLIP_LEVELS + 1;
+
 __global__ __launch_bounds__(256) void k_fill_clipped(BatchDev b) {
-    __shared__ float part[2][4][4][64];  // (draw, clip) x wave x pixel row offset x lane
+    __shared__ float part[CLIP_TERMS][4][4][64];  // term x wave x pixel row offset x lane
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t n = min(b.counters[C_CLIP_JOBS], b.cap_alpha);
     const float px = (float)(lane & 15u), py0 = (float)((lane >> 4) * 4u);
     for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {  // uniform over the block
         const uint2 job = b.clip_jobs[j];
-        const bool ok = job.x < b.cap_alpha && job.y < b.cap_alpha;
-        const uint4 rd = ok ? b.alpha_rec[job.x] : make_uint4(0, 0, 0, 0);
-        const uint4 rc = ok ? b.alpha_rec[job.y] : make_uint4(0, 0, 0, 0);
-        float d[4] = {0.0f, 0.0f, 0.0f, 0.0f}, c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        const uint32_t d0 = min(rd.x, b.cap_fills), d1 = max(d0, min(rd.y, b.cap_fills));
-        for (uint32_t f = d0 + wv; f < d1; f += 4) {
-            const uint2 fl = b.fills[f];
+        const bool clip_job = (job.x & CLIP_JOB_CLIP) != 0;
+        const uint32_t out = job.x & ~CLIP_JOB_CLIP, src = job.y;
+        bool ok = out + (clip_job ? 1u : 0u) < b.cap_alpha && src < b.cap_alpha;
+        // terms: the draw tile's own row (draw jobs), then the clip chain (fixed slots, so that
+        // the arrays stay in registers)
+        const uint4 rec = ok ? b.clip_rec[src] : make_uint4(1u, 0u, 0u, 0u);
+        const uint32_t chain = max(1u, min(rec.x, (uint32_t)MAX_CLIP_LEVELS));
+        const uint32_t first = clip_job ? 1u : 0u, nt = 1u + chain;  // slots [first, nt)
+        const uint32_t term[CLIP_TERMS] = {out, src, rec.y, rec.z, rec.w};
 #pragma unroll
-            for (int q = 0; q < 4; q++) d[q] += fill_area_at(fl, px, py0 + (float)q);
-        }
-        const uint32_t c0 = min(rc.x, b.cap_fills), c1 = max(c0, min(rc.y, b.cap_fills));
-        for (uint32_t f = c0 + wv; f < c1; f += 4) {
-            const uint2 fl = b.fills[f];
+        for (int k = 0; k < CLIP_TERMS; k++) ok = ok && ((uint32_t)k < first || (uint32_t)k >= nt || term[k] < b.cap_alpha);
+        int8_t bdk[CLIP_TERMS];
 #pragma unroll
-            for (int q = 0; q < 4; q++) c[q] += fill_area_at(fl, px, py0 + (float)q);
-        }
+        for (int k = 0; k < CLIP_TERMS; k++) {
+            bdk[k] = 0;
+            if ((uint32_t)k < first || (uint32_t)k >= nt) continue;
+            const uint4 r = ok ? b.alpha_rec[term[k]] : make_uint4(0, 0, 0, 0);
+            bdk[k] = (int8_t)(r.z & 0xffu);
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            const uint32_t f0 = min(r.x, b.cap_fills), f1 = max(f0, min(r.y, b.cap_fills));
+            for (uint32_t f = f0 + wv; f < f1; f += 4) {
+                const uint2 fl = b.fills[f];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            part[0][wv][q][lane] = d[q];
-            part[1][wv][q][lane] = c[q];
+                for (int q = 0; q < 4; q++) acc[q] += fill_area_at(fl, px, py0 + (float)q);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) part[k][wv][q][lane] = acc[q];
         }
         __syncthreads();
         if (wv == 0 && ok) {
-            const uint32_t rule = (rd.z >> 8) & 1u;
-            uint32_t packed = 0;
+            const uint32_t rule = clip_job ? 0u : (b.alpha_rec[out].z >> 8) & 1u;
+            uint32_t packed = 0, packed_eo = 0;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const float sd = (float)(int8_t)(rd.z & 0xffu) + part[0][0][q][lane] + part[0][1][q][lane] +
-                                 part[0][2][q][lane] + part[0][3][q][lane];
-                const float sc = (float)(int8_t)(rc.z & 0xffu) + part[1][0][q][lane] + part[1][1][q][lane] +
-                                 part[1][2][q][lane] + part[1][3][q][lane];
-                const float v = fill_rule_f(fminf(fabsf(sd), fabsf(sc)), rule);
-                packed |= (uint32_t)__float2int_rn(v * 255.0f) << (8 * q);
+                float m = INFINITY;
+#pragma unroll
+                for (int k = 0; k < CLIP_TERMS; k++) {
+                    if ((uint32_t)k < first || (uint32_t)k >= nt) continue;
+                    const float v = (float)bdk[k] + part[k][0][q][lane] + part[k][1][q][lane] + part[k][2][q][lane] +
+                                    part[k][3][q][lane];
+                    m = fminf(m, fabsf(v));
+                }
+                packed |= (uint32_t)__float2int_rn(fill_rule_f(m, rule) * 255.0f) << (8 * q);
+                packed_eo |= (uint32_t)__float2int_rn(fill_rule_f(m, 1u) * 255.0f) << (8 * q);
             }
             // column-major mask: byte 16 x + y
-            *reinterpret_cast<uint32_t *>(b.masks + (uint64_t)job.x * 256 + (lane & 15u) * 16 + (lane >> 4) * 4) = packed;
+            const uint32_t off = (lane & 15u) * 16 + (lane >> 4) * 4;
+            *reinterpret_cast<uint32_t *>(b.masks + (uint64_t)out * 256 + off) = packed;
+            if (clip_job) *reinterpret_cast<uint32_t *>(b.masks + (uint64_t)(out + 1) * 256 + off) = packed_eo;
         }
         __syncthreads();  // the next job rewrites the partial sums
     }

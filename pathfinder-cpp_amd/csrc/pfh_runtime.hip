@@ -48,6 +48,34 @@ hipError_t mlp_adam(float *params, float *m, float *v, const float *grad, int64_
                     hipStream_t s);
 int64_t mlp_param_count();
 hipError_t mlp_predict_sum(const float *params, const float *x, int64_t n, float *sums, hipStream_t s);
+struct StrokeSegment {
+    float2 p0, c0, c1, p1;
+    uint32_t kind;
+    int32_t join;
+    float distance;
+    uint32_t pad;
+};
+struct StrokeLeaf {
+    float2 p0, c0, c1, p1;
+    float2 join_point;
+    uint32_t kind;
+    uint32_t pad;
+};
+struct StrokeContour {
+    uint32_t seg0, seg1, seg2, seg3;
+    uint32_t closed;
+    int32_t cap, join;
+    float width, miter;
+    float2 input_first;
+    uint32_t pad;
+};
+hipError_t launch_offset_pieces(const StrokeSegment *segs, uint32_t n, uint32_t *leaf_cnt, const uint32_t *leaf_off,
+                                StrokeLeaf *leaves, bool emit, hipStream_t s);
+hipError_t launch_assemble(const StrokeContour *contours, uint32_t n, const uint32_t *leaf_off, const StrokeLeaf *leaves,
+                           const StrokeSegment *segs, const uint32_t *region_off, float2 *region_pts,
+                           uint8_t *region_flags, uint32_t *point_cnt, hipStream_t s);
+hipError_t launch_compact(const uint32_t *region_off, const uint32_t *out_off, uint32_t n, const float2 *region_pts,
+                          const uint8_t *region_flags, float2 *out_pts, uint8_t *out_flags, hipStream_t s);
 }  // namespace pfh
 
 using namespace pfh;
@@ -118,12 +146,14 @@ struct SceneHost {
     std::vector<uint8_t> path_rule;
     uint32_t n_clip = 0;                // trailing clip paths
     std::vector<uint32_t> path_clip;    // clip reference per path (CLIP_*)
+    std::vector<uint8_t> path_level;    // clip level per path (0 draw path, 1..4 clip path)
+    uint32_t clip_levels = 0;           // deepest clip level
     std::vector<uint32_t> path_breaks;  // chord chains per path (two extra stream points each)
     uint64_t n_breaks = 0;
     std::vector<float> features;
     DevBuf d_features;
     DevBuf d_pts, d_segs, d_seg_path, d_path_rect, d_path_tile_off, d_path_col_off, d_col_path, d_path_rgba,
-        d_path_rule, d_path_seg_off, d_bin_paths, d_out_owned, d_path_clip;
+        d_path_rule, d_path_seg_off, d_bin_paths, d_out_owned, d_path_clip, d_path_level;
     uint32_t n_bin_wave = 0, n_bin_group = 0;  // paths binned by one wave / by one workgroup
     uint8_t *d_out = nullptr;
     int64_t out_pitch = 0;
@@ -155,6 +185,12 @@ struct GraphKey {
 
 struct Server;
 
+// A captured batched launch.  Its event-record nodes bracket the fill and composite kernels with
+// the workspace's events (Workspace::kev), which outlive every executable graph.
+struct CapturedGraph {
+    hipGraphExec_t exec = nullptr;
+};
+
 // Device working set of one launch: work buffers, stream, events and captured graphs.
 // The context's own workspace serves pfh_rasterize; the request loop owns one per slot so
 // that launches in different slots overlap on the device.
@@ -166,32 +202,60 @@ struct Workspace {
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, ent_quarter, path_tot, bin_items, col_item, col_bd,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, alpha_vis, masks, zbuf, fb_range, keys, vals, keys_alt,
-        vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp, tile_clip_ai, clip_jobs;
+        vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp, tile_clip_ai, clip_jobs, clip_rec, clip_seen;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
     size_t h_table_cap = 0;
     uint32_t *h_bases = nullptr;
     size_t h_bases_cap = 0;
     bool last_sorted_alt = false;
-    std::map<GraphKey, hipGraphExec_t> graphs;
+    std::map<GraphKey, CapturedGraph> graphs;
     uint64_t graphs_generation = 0;  // g_alloc_generation the cached graphs were captured under
+    hipEvent_t stage_ev[9] = {};     // plain mode: between its stages
 
     void create() {
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP_TRY(hipEventCreate(&ev0));
         HIP_TRY(hipEventCreate(&ev1));
         for (auto &e : kev) HIP_TRY(hipEventCreate(&e));
+        // An event whose first record is an external record node inside a stream capture is
+        // refused (invalid argument): record each once, eagerly, before any capture.
+        for (auto &e : kev) HIP_TRY(hipEventRecord(e, stream));
+        for (auto &e : stage_ev) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipHostMalloc((void **)&h_counters, 4 * C_COUNT));
     }
+    static void destroy_graph(CapturedGraph &g) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = CapturedGraph{};
+    }
+    // Executable graphs launch on this workspace's stream only: once it is idle none of them
+    // runs, and they can be destroyed (an executable graph is never destroyed while it may run).
     void drop_graphs() {
-        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+        if (graphs.empty()) return;
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &g : graphs) destroy_graph(g.second);
         graphs.clear();
+    }
+    // Drops the graphs whose scene set contains scene `id` (after the stream is idle).
+    void drop_graphs_of(int32_t id) {
+        bool any = false;
+        for (auto &g : graphs) any |= std::find(g.first.ids.begin(), g.first.ids.end(), id) != g.first.ids.end();
+        if (!any) return;
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto it = graphs.begin(); it != graphs.end();) {
+            if (std::find(it->first.ids.begin(), it->first.ids.end(), id) != it->first.ids.end()) {
+                destroy_graph(it->second);
+                it = graphs.erase(it);
+            } else {
+                ++it;
+            }
+        }
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
                 &path_ent, &ent_quarter, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &entry_meta, &counters, &scan_tmp,
-                &sort_tmp, &tile_clip_ai, &clip_jobs};
+                &sort_tmp, &tile_clip_ai, &clip_jobs, &clip_rec, &clip_seen};
     }
     size_t device_bytes() {
         size_t n = 0;
@@ -214,6 +278,8 @@ struct Workspace {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (auto &e : kev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &e : stage_ev)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
@@ -280,6 +346,8 @@ static void validate_desc(const pfh_scene_desc *d) {
             throw PfhError{PFH_E_ARG, "path_contour_start not monotone"};
     if ((int64_t)d->width * d->height > (int64_t)1 << 30) throw PfhError{PFH_E_LIMIT, "canvas larger than 2^30 pixels"};
     if (d->n_paths >= (1 << 24)) throw PfhError{PFH_E_LIMIT, "more than 2^24 paths in a scene"};
+    for (int p = 0; p < d->n_paths; p++)
+        if (d->path_fill_rule[p] > 1) throw PfhError{PFH_E_ARG, "path_fill_rule must be 0 (nonzero) or 1 (even-odd)"};
     if (d->n_clip_paths < 0 || d->n_clip_paths > d->n_paths)
         throw PfhError{PFH_E_ARG, "n_clip_paths must lie in [0, n_paths]"};
     if (d->path_clip) {
@@ -287,9 +355,15 @@ static void validate_desc(const pfh_scene_desc *d) {
         for (int p = 0; p < d->n_paths; p++) {
             const int32_t c = d->path_clip[p];
             if (c < -1 || c >= d->n_clip_paths) throw PfhError{PFH_E_ARG, "path_clip names no clip path"};
-            // The reference's hybrid tiler builds clip paths with no clip paths of their own to
-            // refer to (core/d3d9/scene_builder.cpp:209-216): nested clips are undefined there.
-            if (p >= n_draw && c != -1) throw PfhError{PFH_E_ARG, "nested clip paths are not supported"};
+        }
+        // A clip path may be clipped by another (nesting; undefined in the reference's hybrid tiler,
+        // core/d3d9/scene_builder.cpp:209-216, given the level structure of the GPU-driven renderer,
+        // core/d3d11/scene_builder.cpp:92-133): chains of at most MAX_CLIP_LEVELS clip paths.
+        for (int q = 0; q < d->n_clip_paths; q++) {
+            uint32_t k = 1;
+            for (int32_t c = d->path_clip[n_draw + q]; c >= 0; c = d->path_clip[n_draw + c])
+                if (++k > MAX_CLIP_LEVELS)
+                    throw PfhError{PFH_E_ARG, "clip paths nest deeper than 4 (or in a cycle)"};
         }
     }
 }
@@ -349,7 +423,266 @@ static void path_features(const SceneHost &s, const std::vector<float2> &pts, co
 
 static void predict_scene(pfh_ctx *ctx, SceneHost &s);
 
+// ---------------------------------------------------------------------------
+// Stroke-to-fill on the device (kernels in pfh_stroke.hip)
+// ---------------------------------------------------------------------------
+
+// Outlines as flat arrays (points, flags, contours, paths).
+struct HostOutlines {
+    std::vector<float2> pts;
+    std::vector<uint8_t> flags;
+    std::vector<int32_t> contour_start{0};
+    std::vector<uint8_t> closed;
+    std::vector<int32_t> path_contour_start{0};
+};
+
+static void validate_style(const pfh_stroke_style &st) {
+    if (st.line_cap < PFH_CAP_BUTT || st.line_cap > PFH_CAP_ROUND || st.line_join < PFH_JOIN_MITER ||
+        st.line_join > PFH_JOIN_ROUND || !std::isfinite(st.line_width) || !std::isfinite(st.miter_limit))
+        throw PfhError{PFH_E_ARG, "invalid stroke style"};
+}
+
+// Strokes every path of `in` with its style (OutlineStrokeToFill::offset + into_outline,
+// core/stroke.cpp:124-195).  The host only lays out each contour's strands of segments in the
+// reference's offsetting order (forward: SegmentsIter order with the closing line unless the
+// contour ends where it starts, core/data/contour.cpp:110-173; backward: that closing line
+// reversed, then the segments from the tail reversed, core/stroke.cpp:54-122); offsetting,
+// joins and caps run on the device.
+static HostOutlines stroke_on_device(pfh_ctx *ctx, const float2 *P, const uint8_t *F, const int32_t *contour_start,
+                                     const uint8_t *contour_closed, const int32_t *path_contour_start, int32_t n_paths,
+                                     const pfh_stroke_style *styles) {
+    std::vector<StrokeSegment> segs;
+    std::vector<StrokeContour> outs;
+    std::vector<int32_t> path_out(n_paths + 1, 0);
+    auto approx_eq = [](float2 a, float2 b) {
+        const float dx = a.x - b.x, dy = a.y - b.y;
+        return std::sqrt(dx * dx + dy * dy) <= 0.0001f;
+    };
+    for (int32_t p = 0; p < n_paths; p++) {
+        const pfh_stroke_style &st = styles[p];
+        validate_style(st);
+        const float radius = st.line_width * 0.5f;
+        for (int32_t c = path_contour_start[p]; c < path_contour_start[p + 1]; c++) {
+            const int32_t a = contour_start[c], n = contour_start[c + 1] - a;
+            const float2 *pt = P + a;
+            const uint8_t *fl = F + a;
+            const bool closed = contour_closed[c] != 0;
+            auto add = [&](float2 p0, float2 c0, float2 c1, float2 p1, uint32_t kind, uint32_t index) {
+                segs.push_back(StrokeSegment{p0, c0, c1, p1, kind, index == 0 ? (int32_t)PFH_JOIN_BEVEL : st.line_join,
+                                             -radius, 0});
+            };
+            const float2 z = make_float2(0.0f, 0.0f);
+            // forward strand (core/stroke.cpp:28-52)
+            const uint32_t f0 = (uint32_t)segs.size();
+            uint32_t index = 0;
+            for (int32_t head = 0;;) {
+                if (head >= n || fl[head] != 0) break;
+                if (head + 1 < n) {
+                    if (fl[head + 1] == 0) {
+                        add(pt[head], z, z, pt[head + 1], 1, index++);
+                        head += 1;
+                    } else if (head + 2 < n && fl[head + 1] == 1 && fl[head + 2] == 0) {
+                        add(pt[head], pt[head + 1], z, pt[head + 2], 2, index++);
+                        head += 2;
+                    } else if (head + 3 < n && fl[head + 1] == 1 && fl[head + 2] == 2 && fl[head + 3] == 0) {
+                        add(pt[head], pt[head + 1], pt[head + 2], pt[head + 3], 3, index++);
+                        head += 3;
+                    } else {
+                        break;
+                    }
+                } else {
+                    if (closed && n > 1 && !approx_eq(pt[0], pt[n - 1])) add(pt[n - 1], z, z, pt[0], 1, index++);
+                    break;
+                }
+            }
+            const uint32_t f1 = (uint32_t)segs.size();
+            // backward strand (core/stroke.cpp:54-122)
+            index = 0;
+            if (closed && n > 1 && !approx_eq(pt[0], pt[n - 1])) add(pt[0], z, z, pt[n - 1], 1, index++);
+            for (int32_t tail = n - 1; tail >= 0;) {
+                if (fl[tail] != 0) break;
+                if (tail >= 3 && fl[tail - 1] == 2 && fl[tail - 2] == 1 && fl[tail - 3] == 0) {
+                    add(pt[tail], pt[tail - 1], pt[tail - 2], pt[tail - 3], 3, index++);
+                    tail -= 3;
+                } else if (tail >= 2 && fl[tail - 1] == 1 && fl[tail - 2] == 0) {
+                    add(pt[tail], pt[tail - 1], z, pt[tail - 2], 2, index++);
+                    tail -= 2;
+                } else if (tail >= 1 && fl[tail - 1] == 0) {
+                    add(pt[tail], z, z, pt[tail - 1], 1, index++);
+                    tail -= 1;
+                } else {
+                    break;
+                }
+            }
+            const uint32_t b1 = (uint32_t)segs.size();
+            const float2 first = n > 0 ? pt[0] : z;
+            auto contour = [&](uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3) {
+                outs.push_back(StrokeContour{s0, s1, s2, s3, closed ? 1u : 0u, st.line_cap, st.line_join, st.line_width,
+                                             st.miter_limit, first, 0});
+            };
+            if (closed) {
+                contour(f0, f1, f1, f1);
+                contour(f1, b1, b1, b1);
+            } else {
+                contour(f0, f1, f1, b1);
+            }
+        }
+        path_out[p + 1] = (int32_t)outs.size();
+    }
+    HostOutlines res;
+    res.path_contour_start = path_out;
+    const uint32_t ns = (uint32_t)segs.size(), nc = (uint32_t)outs.size();
+    if (nc == 0) return res;
+    hipStream_t st = ctx->w->stream;
+    DevBuf d_segs, d_cnt, d_off, d_leaves, d_outs, d_region_off, d_region_pts, d_region_flags, d_pcnt, d_out_off,
+        d_pts, d_flags;
+    struct Release {
+        std::vector<DevBuf *> bufs;
+        ~Release() {
+            for (DevBuf *b : bufs) b->release();
+        }
+    } release{{&d_segs, &d_cnt, &d_off, &d_leaves, &d_outs, &d_region_off, &d_region_pts, &d_region_flags, &d_pcnt,
+               &d_out_off, &d_pts, &d_flags}};
+    upload(d_segs, segs);
+    d_cnt.ensure(4ull * ns + 4);
+    HIP_TRY(launch_offset_pieces(d_segs.as<StrokeSegment>(), ns, d_cnt.as<uint32_t>(), nullptr, nullptr, false, st));
+    std::vector<uint32_t> cnt(ns);
+    if (ns) HIP_TRY(hipMemcpyAsync(cnt.data(), d_cnt.ptr, 4ull * ns, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<uint32_t> leaf_off(ns + 1, 0);
+    for (uint32_t i = 0; i < ns; i++) {
+        if ((uint64_t)leaf_off[i] + cnt[i] >= 0xF0000000ull) throw PfhError{PFH_E_LIMIT, "too many stroke pieces"};
+        leaf_off[i + 1] = leaf_off[i] + cnt[i];
+    }
+    upload(d_off, leaf_off);
+    d_leaves.ensure(sizeof(StrokeLeaf) * ((size_t)leaf_off[ns] + 1));
+    HIP_TRY(launch_offset_pieces(d_segs.as<StrokeSegment>(), ns, nullptr, d_off.as<uint32_t>(),
+                                 d_leaves.as<StrokeLeaf>(), true, st));
+    // regions: an upper bound of every output contour's points
+    std::vector<uint32_t> region_off(nc + 1, 0);
+    for (uint32_t c = 0; c < nc; c++) {
+        const StrokeContour &k = outs[c];
+        const uint64_t leaves = (uint64_t)leaf_off[k.seg1] - leaf_off[k.seg0] + leaf_off[k.seg3] - leaf_off[k.seg2];
+        const uint64_t bound = 20 * leaves + 3 * 16;
+        if (region_off[c] + bound >= 0xF0000000ull) throw PfhError{PFH_E_LIMIT, "stroked outline too large"};
+        region_off[c + 1] = region_off[c] + (uint32_t)bound;
+    }
+    upload(d_outs, outs);
+    upload(d_region_off, region_off);
+    d_region_pts.ensure(8ull * region_off[nc] + 8);
+    d_region_flags.ensure((size_t)region_off[nc] + 4);
+    d_pcnt.ensure(4ull * nc);
+    HIP_TRY(launch_assemble(d_outs.as<StrokeContour>(), nc, d_off.as<uint32_t>(), d_leaves.as<StrokeLeaf>(),
+                            d_segs.as<StrokeSegment>(), d_region_off.as<uint32_t>(), d_region_pts.as<float2>(),
+                            d_region_flags.as<uint8_t>(), d_pcnt.as<uint32_t>(), st));
+    std::vector<uint32_t> pcnt(nc);
+    HIP_TRY(hipMemcpyAsync(pcnt.data(), d_pcnt.ptr, 4ull * nc, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<uint32_t> out_off(nc + 1, 0);
+    for (uint32_t c = 0; c < nc; c++) {
+        if (pcnt[c] > region_off[c + 1] - region_off[c]) throw PfhError{PFH_E_STATE, "stroke region overflow"};
+        out_off[c + 1] = out_off[c] + pcnt[c];
+    }
+    upload(d_out_off, out_off);
+    d_pts.ensure(8ull * out_off[nc] + 8);
+    d_flags.ensure((size_t)out_off[nc] + 4);
+    HIP_TRY(launch_compact(d_region_off.as<uint32_t>(), d_out_off.as<uint32_t>(), nc, d_region_pts.as<float2>(),
+                           d_region_flags.as<uint8_t>(), d_pts.as<float2>(), d_flags.as<uint8_t>(), st));
+    res.pts.resize(out_off[nc]);
+    res.flags.resize(out_off[nc]);
+    if (out_off[nc]) {
+        HIP_TRY(hipMemcpyAsync(res.pts.data(), d_pts.ptr, 8ull * out_off[nc], hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(res.flags.data(), d_flags.ptr, out_off[nc], hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    res.contour_start.assign(out_off.begin(), out_off.end());
+    res.closed.assign(nc, 1);
+    return res;
+}
+
+static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d);
+
+// A scene whose paths carry stroke styles: those paths are stroked on the device first and
+// their stroked outlines (filled with the nonzero rule, core/canvas.cpp:297-299) take their place.
 static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
+    if (!d || !d->path_stroke) return scene_upload_desc(ctx, d);
+    if (d->n_paths < 0 || d->n_contours < 0 || d->n_points < 0 || !d->contour_start || !d->path_contour_start ||
+        (d->n_points && (!d->points || !d->point_flags)) || (d->n_contours && !d->contour_closed) ||
+        (d->n_paths && !d->path_fill_rule))
+        throw PfhError{PFH_E_ARG, "null array"};
+    if (d->contour_start[0] != 0 || d->contour_start[d->n_contours] != d->n_points ||
+        d->path_contour_start[0] != 0 || d->path_contour_start[d->n_paths] != d->n_contours)
+        throw PfhError{PFH_E_ARG, "contour / path ranges do not cover the arrays"};
+    for (int c = 0; c < d->n_contours; c++)
+        if (d->contour_start[c + 1] < d->contour_start[c]) throw PfhError{PFH_E_ARG, "contour_start not monotone"};
+    for (int p = 0; p < d->n_paths; p++)
+        if (d->path_contour_start[p + 1] < d->path_contour_start[p])
+            throw PfhError{PFH_E_ARG, "path_contour_start not monotone"};
+    const float2 *P = reinterpret_cast<const float2 *>(d->points);
+    // the stroked paths as a batch of their own
+    std::vector<int32_t> stroked;
+    for (int32_t p = 0; p < d->n_paths; p++)
+        if (d->path_stroke[p].line_width > 0.0f) stroked.push_back(p);
+    std::vector<int32_t> spcs{0};
+    std::vector<pfh_stroke_style> sst;
+    for (int32_t p : stroked) {
+        spcs.push_back(spcs.back() + d->path_contour_start[p + 1] - d->path_contour_start[p]);
+        sst.push_back(d->path_stroke[p]);
+    }
+    // a stroked path's contours are contiguous: index them through a remapped contour table
+    std::vector<int32_t> scs{0};
+    std::vector<uint8_t> scl;
+    std::vector<float2> spts;
+    std::vector<uint8_t> sfl;
+    for (int32_t p : stroked)
+        for (int32_t c = d->path_contour_start[p]; c < d->path_contour_start[p + 1]; c++) {
+            spts.insert(spts.end(), P + d->contour_start[c], P + d->contour_start[c + 1]);
+            sfl.insert(sfl.end(), d->point_flags + d->contour_start[c], d->point_flags + d->contour_start[c + 1]);
+            scs.push_back((int32_t)spts.size());
+            scl.push_back(d->contour_closed[c]);
+        }
+    HostOutlines so = stroke_on_device(ctx, spts.data(), sfl.data(), scs.data(), scl.data(), spcs.data(),
+                                       (int32_t)stroked.size(), sst.data());
+    // the scene with the stroked outlines in place of the stroked paths' own
+    std::vector<float2> pts;
+    std::vector<uint8_t> fl, closed, rule(d->path_fill_rule, d->path_fill_rule + d->n_paths);
+    std::vector<int32_t> cs{0}, pcs{0};
+    size_t k = 0;
+    for (int32_t p = 0; p < d->n_paths; p++) {
+        if (k < stroked.size() && stroked[k] == p) {
+            for (int32_t c = so.path_contour_start[k]; c < so.path_contour_start[k + 1]; c++) {
+                pts.insert(pts.end(), so.pts.begin() + so.contour_start[c], so.pts.begin() + so.contour_start[c + 1]);
+                fl.insert(fl.end(), so.flags.begin() + so.contour_start[c], so.flags.begin() + so.contour_start[c + 1]);
+                cs.push_back((int32_t)pts.size());
+                closed.push_back(so.closed[c]);
+            }
+            rule[p] = PFH_FILL_NONZERO;
+            k++;
+        } else {
+            for (int32_t c = d->path_contour_start[p]; c < d->path_contour_start[p + 1]; c++) {
+                pts.insert(pts.end(), P + d->contour_start[c], P + d->contour_start[c + 1]);
+                fl.insert(fl.end(), d->point_flags + d->contour_start[c], d->point_flags + d->contour_start[c + 1]);
+                cs.push_back((int32_t)pts.size());
+                closed.push_back(d->contour_closed[c]);
+            }
+        }
+        pcs.push_back((int32_t)closed.size());
+    }
+    if (pts.size() >= 0x7FFFFFFFull) throw PfhError{PFH_E_LIMIT, "stroked scene has more than 2^31 points"};
+    pfh_scene_desc e = *d;
+    e.n_contours = (int32_t)closed.size();
+    e.n_points = (int32_t)pts.size();
+    e.points = reinterpret_cast<const float *>(pts.data());
+    e.point_flags = fl.data();
+    e.contour_start = cs.data();
+    e.contour_closed = closed.data();
+    e.path_contour_start = pcs.data();
+    e.path_fill_rule = rule.data();
+    e.path_stroke = nullptr;
+    return scene_upload_desc(ctx, &e);
+}
+
+static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d) {
     validate_desc(d);
     auto s = std::make_unique<SceneHost>();
     s->width = d->width;
@@ -473,9 +806,15 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
         std::vector<uint32_t> pos(d->n_paths, CLIP_EMPTY);
         for (size_t i = 0; i < bin_wave.size(); i++) pos[bin_wave[i]] = (uint32_t)i;
         for (size_t i = 0; i < bin_group.size(); i++) pos[bin_group[i]] = CLIP_GROUP | (uint32_t)i;
+        s->path_level.assign(d->n_paths, 0);
         for (int p = 0; p < d->n_paths; p++) {
-            if (p >= n_draw) s->path_clip[p] = CLIP_IS_CLIP;
-            else if (d->path_clip && d->path_clip[p] >= 0) s->path_clip[p] = pos[n_draw + d->path_clip[p]];
+            if (d->path_clip && d->path_clip[p] >= 0) s->path_clip[p] = pos[n_draw + d->path_clip[p]];
+            if (p >= n_draw) {
+                uint32_t k = 1;
+                for (int32_t c = d->path_clip ? d->path_clip[p] : -1; c >= 0; c = d->path_clip[n_draw + c]) k++;
+                s->path_level[p] = (uint8_t)k;
+                s->clip_levels = std::max(s->clip_levels, k);
+            }
         }
     }
     bin_wave.insert(bin_wave.end(), bin_group.begin(), bin_group.end());
@@ -528,7 +867,10 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
     upload(s->d_path_rule, s->path_rule);
     upload(s->d_path_seg_off, s->path_seg_off);
     upload(s->d_bin_paths, bin_wave);
-    if (s->n_clip > 0) upload(s->d_path_clip, s->path_clip);
+    if (s->n_clip > 0) {
+        upload(s->d_path_clip, s->path_clip);
+        upload(s->d_path_level, s->path_level);
+    }
     s->out_pitch = (int64_t)s->width * 4;
     s->d_out_owned.ensure((size_t)s->out_pitch * s->height);
     s->d_out = s->d_out_owned.as<uint8_t>();
@@ -575,6 +917,7 @@ struct Plan {
     uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0, P = 0;
     uint64_t breaks = 0;  // chord chains of the batch (two stream points each besides the chords)
     bool clips = false;   // some scene has clip paths
+    uint32_t clip_levels = 0;  // deepest clip nesting level of the batch
     uint32_t cap_f = 0, cap_a = 0, cap_e = 0, cap_l = 0;
     int key_bits = 1;
     Schedule sched;
@@ -599,6 +942,8 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     if (pl.clips) {
         w.tile_clip_ai.ensure(4 * pl.T + 4);
         w.clip_jobs.ensure(8ull * pl.cap_a + 8);
+        w.clip_rec.ensure(16ull * pl.cap_a + 16);
+        w.clip_seen.ensure(4ull * (pl.cap_a / 32 + 1));
     }
     w.tile_st.ensure(pl.T + 4);
     w.path_ent.ensure(8 * pl.P + 8);
@@ -681,6 +1026,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.bin_group_base = (uint32_t)BG;
         d.path_rule = s.d_path_rule.as<uint8_t>();
         d.path_clip = s.n_clip > 0 ? s.d_path_clip.as<uint32_t>() : nullptr;
+        d.path_level = s.n_clip > 0 ? s.d_path_level.as<uint8_t>() : nullptr;
         d.vb_l = s.vb[0];
         d.vb_t = s.vb[1];
         d.vb_r = s.vb[2];
@@ -719,6 +1065,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         S += s.n_segs;
         pl.breaks += s.n_breaks;
         pl.clips = pl.clips || s.n_clip > 0;
+        pl.clip_levels = std::max(pl.clip_levels, s.clip_levels);
         T += s.n_tiles;
         W += s.n_cols;
         FB += d.n_fb;
@@ -787,6 +1134,9 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.tile_bd = (dump || pl.clips) ? w.tile_bd.as<int8_t>() : nullptr;
     b.tile_clip_ai = pl.clips ? w.tile_clip_ai.as<uint32_t>() : nullptr;
     b.clip_jobs = pl.clips ? w.clip_jobs.as<uint2>() : nullptr;
+    b.clip_rec = pl.clips ? w.clip_rec.as<uint4>() : nullptr;
+    b.clip_seen = pl.clips ? w.clip_seen.as<uint32_t>() : nullptr;
+    b.clip_levels = pl.clip_levels;
     b.has_clips = pl.clips ? 1 : 0;
     b.tile_st = w.tile_st.as<uint8_t>();
     b.path_ent = w.path_ent.as<uint2>();
@@ -829,6 +1179,7 @@ static void stage_clear(Workspace &w, const BatchDev &b) {
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.path_ent), 0, 2 * (uint64_t)b.n_paths, s));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.fb_range), 0, 2 * (uint64_t)b.n_fb, s));
     HIP_TRY(launch_fill_u32(b.counters, 0, C_COUNT, s));
+    if (b.clip_seen) HIP_TRY(launch_fill_u32(b.clip_seen, 0, (uint64_t)b.cap_alpha / 32 + 1, s));
 }
 
 // flatten (count, scan, emit) and bin count: chords and exact fill / alpha-tile counts
@@ -887,7 +1238,11 @@ static void record_counts(pfh_ctx *ctx, const Plan &pl, bool dump) {
     for (int i = 0; i < DRAWN_SLOTS; i++) drawn += ctx->w->h_counters[C_DRAWN + i];
     ctx->stats.fills_masked = drawn;
     // the fill kernel rasterizes the alpha tiles of draw entries (every alpha tile with dumps)
-    ctx->stats.alpha_masked = std::min<int64_t>(ctx->w->h_counters[dump ? C_ALPHA : C_ALPHA_ENTRIES], pl.cap_a);
+    // (listed mode: one row per alpha entry, less the clip mask rows another entry listed first)
+    ctx->stats.alpha_masked = std::min<int64_t>(
+        dump ? ctx->w->h_counters[C_ALPHA]
+             : (int64_t)ctx->w->h_counters[C_ALPHA_ENTRIES] - (int64_t)ctx->w->h_counters[C_CLIP_DUPS],
+        pl.cap_a);
     ctx->stats.alpha_tiles = ctx->w->h_counters[C_ALPHA];
     ctx->stats.entries = ctx->w->h_counters[C_ENTRIES];
     ctx->stats.guard_hits = ctx->w->h_counters[C_GUARD];
@@ -922,8 +1277,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     hipStream_t st = ctx->w->stream;
     // scratch is re-read at every use: ensure_sizes may reallocate it when a count grows
     auto tmp = [ctx]() { return ctx->w->scan_tmp.as<uint32_t>(); };
-    hipEvent_t ev[9];
-    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    hipEvent_t *ev = ctx->w->stage_ev;
     HIP_TRY(hipEventRecord(ev[0], st));
     stage_clear(*ctx->w, b);
     HIP_TRY(launch_flatten(b, false, flatten_segs(pl), st));
@@ -971,7 +1325,6 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     HIP_TRY(hipEventRecord(ev[8], st));
     read_counters(ctx);
     for (int i = 0; i < 8; i++) ctx->stats.ms_stage[i] += elapsed_ms(ctx, ev[i], ev[i + 1]);
-    for (auto &e : ev) (void)hipEventDestroy(e);
     record_counts(ctx, pl, dump);
     SceneHost &s = get_scene(ctx, id);
     s.counted = true;
@@ -1088,28 +1441,35 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
         auto it = w.graphs.find(key);
         if (it == w.graphs.end()) {
             if (w.graphs.size() > 32) w.drop_graphs();
-            hipGraph_t graph;
-            HIP_TRY(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
+            CapturedGraph cg;
             try {
-                stage_clear(w, b);
-                stage_count(w, b, pl);
-                stage_emit(w, b, pl);
-                stage_finish(w, b, pl);
+                hipGraph_t graph;
+                HIP_TRY(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    stage_clear(w, b);
+                    stage_count(w, b, pl);
+                    stage_emit(w, b, pl);
+                    stage_finish(w, b, pl);
+                } catch (...) {
+                    hipGraph_t dummy = nullptr;
+                    (void)hipStreamEndCapture(w.stream, &dummy);
+                    if (dummy) (void)hipGraphDestroy(dummy);
+                    throw;
+                }
+                HIP_TRY(hipStreamEndCapture(w.stream, &graph));
+                hipError_t ie = hipGraphInstantiate(&cg.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIP_TRY(ie);
             } catch (...) {
-                hipGraph_t dummy;
-                (void)hipStreamEndCapture(w.stream, &dummy);
+                Workspace::destroy_graph(cg);
                 throw;
             }
-            HIP_TRY(hipStreamEndCapture(w.stream, &graph));
-            hipGraphExec_t exec;
-            HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(graph);
-            it = w.graphs.emplace(key, exec).first;
+            it = w.graphs.emplace(key, cg).first;
         } else {
             // the sort's result buffer parity depends only on the key width
             w.last_sorted_alt = ((pl.key_bits + 7) / 8) % 2 == 1;
         }
-        HIP_TRY(hipGraphLaunch(it->second, w.stream));
+        HIP_TRY(hipGraphLaunch(it->second.exec, w.stream));
     } else {
         stage_clear(w, b);
         stage_count(w, b, pl);
@@ -1234,10 +1594,12 @@ static Server &server_of(pfh_ctx *ctx) {
     return *ctx->server;
 }
 
-static void drop_all_graphs(pfh_ctx *ctx) {
-    ctx->main.drop_graphs();
+// Graphs of every workspace that rasterize scene `id`, after each workspace's stream is idle
+// (the request loop refuses to release a scene it has queued or in flight).
+static void drop_graphs_of_scene(pfh_ctx *ctx, int32_t id) {
+    ctx->main.drop_graphs_of(id);
     if (ctx->server)
-        for (auto &sl : ctx->server->slots) sl->ws.drop_graphs();
+        for (auto &sl : ctx->server->slots) sl->ws.drop_graphs_of(id);
 }
 
 // One turn of the loop: retire finished launches (or re-launch them after an overflow),
@@ -1383,7 +1745,8 @@ void pfh_destroy(pfh_ctx *ctx) {
     for (auto &s : ctx->scenes) {
         if (!s) continue;
         for (DevBuf *b : {&s->d_features, &s->d_pts, &s->d_segs, &s->d_seg_path, &s->d_path_rect, &s->d_path_tile_off,
-                          &s->d_path_col_off, &s->d_col_path, &s->d_path_rgba, &s->d_path_rule, &s->d_path_seg_off, &s->d_bin_paths, &s->d_out_owned, &s->d_path_clip})
+                          &s->d_path_col_off, &s->d_col_path, &s->d_path_rgba, &s->d_path_rule, &s->d_path_seg_off, &s->d_bin_paths, &s->d_out_owned, &s->d_path_clip,
+                          &s->d_path_level})
             b->release();
     }
     for (DevBuf *b : {&ctx->mlp_params, &ctx->mlp_x, &ctx->mlp_h, &ctx->mlp_out}) b->release();
@@ -1395,6 +1758,41 @@ const char *pfh_last_error(const pfh_ctx *ctx) { return ctx ? ctx->err.c_str() :
 int pfh_scene_upload(pfh_ctx *ctx, const pfh_scene_desc *desc, int32_t *scene_id) {
     if (!ctx || !scene_id) return PFH_E_ARG;
     return run_guarded(ctx, [&] { *scene_id = scene_upload(ctx, desc); });
+}
+
+int pfh_stroke_outlines(pfh_ctx *ctx, const pfh_outline *in, const int32_t *path_contour_start, int32_t n_paths,
+                        const pfh_stroke_style *styles, pfh_outline_buf *out) {
+    if (!ctx || !in || !out || n_paths < 0 || (n_paths > 0 && (!styles || !path_contour_start))) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        if (in->n_contours < 0 || in->n_points < 0 || !in->contour_start || (in->n_points && (!in->points || !in->point_flags)) ||
+            (in->n_contours && !in->contour_closed))
+            throw PfhError{PFH_E_ARG, "null array"};
+        if (in->contour_start[0] != 0 || in->contour_start[in->n_contours] != in->n_points ||
+            (n_paths > 0 && (path_contour_start[0] != 0 || path_contour_start[n_paths] != in->n_contours)))
+            throw PfhError{PFH_E_ARG, "contour / path ranges do not cover the arrays"};
+        for (int32_t c = 0; c < in->n_contours; c++)
+            if (in->contour_start[c + 1] < in->contour_start[c]) throw PfhError{PFH_E_ARG, "contour_start not monotone"};
+        for (int32_t p = 0; p < n_paths; p++)
+            if (path_contour_start[p + 1] < path_contour_start[p])
+                throw PfhError{PFH_E_ARG, "path_contour_start not monotone"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        HostOutlines so = stroke_on_device(ctx, reinterpret_cast<const float2 *>(in->points), in->point_flags,
+                                           in->contour_start, in->contour_closed, path_contour_start, n_paths, styles);
+        out->n_points = (int64_t)so.pts.size();
+        out->n_contours = (int64_t)so.closed.size();
+        if (out->n_points > out->cap_points || out->n_contours > out->cap_contours)
+            throw PfhError{PFH_E_LIMIT, "output capacity too small for the stroked outlines"};
+        if ((out->n_points && (!out->points || !out->point_flags)) || !out->contour_start ||
+            (out->n_contours && !out->contour_closed) || !out->path_contour_start)
+            throw PfhError{PFH_E_ARG, "null output array"};
+        if (out->n_points) {
+            std::memcpy(out->points, so.pts.data(), 8 * so.pts.size());
+            std::memcpy(out->point_flags, so.flags.data(), so.flags.size());
+        }
+        std::memcpy(out->contour_start, so.contour_start.data(), 4 * so.contour_start.size());
+        if (out->n_contours) std::memcpy(out->contour_closed, so.closed.data(), so.closed.size());
+        std::memcpy(out->path_contour_start, so.path_contour_start.data(), 4 * so.path_contour_start.size());
+    });
 }
 
 int pfh_scene_release(pfh_ctx *ctx, int32_t scene_id) {
@@ -1409,9 +1807,10 @@ int pfh_scene_release(pfh_ctx *ctx, int32_t scene_id) {
                 throw PfhError{PFH_E_STATE, "scene has requests in the request loop"};
         }
         HIP_TRY(hipStreamSynchronize(ctx->w->stream));
-        drop_all_graphs(ctx);
+        drop_graphs_of_scene(ctx, scene_id);
         for (DevBuf *b : {&s.d_features, &s.d_pts, &s.d_segs, &s.d_seg_path, &s.d_path_rect, &s.d_path_tile_off, &s.d_path_col_off,
-                          &s.d_col_path, &s.d_path_rgba, &s.d_path_rule, &s.d_path_seg_off, &s.d_bin_paths, &s.d_out_owned, &s.d_path_clip})
+                          &s.d_col_path, &s.d_path_rgba, &s.d_path_rule, &s.d_path_seg_off, &s.d_bin_paths, &s.d_out_owned, &s.d_path_clip,
+                          &s.d_path_level})
             b->release();
         ctx->scenes[scene_id].reset();
     });
@@ -1457,9 +1856,10 @@ int pfh_scene_set_target(pfh_ctx *ctx, int32_t scene_id, void *device_rgba8, int
             return;
         }
         if (pitch_bytes < (int64_t)s.width * 4 || (pitch_bytes & 3)) throw PfhError{PFH_E_ARG, "bad pitch"};
+        // Captured graphs stay valid: kernels read the target through the scene table, which
+        // every launch copies to the device before it starts.
         s.d_out = reinterpret_cast<uint8_t *>(device_rgba8);
         s.out_pitch = pitch_bytes;
-        drop_all_graphs(ctx);
     });
 }
 

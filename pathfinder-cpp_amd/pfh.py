@@ -34,7 +34,27 @@ class SceneDesc(ctypes.Structure):
                 ("n_paths", ctypes.c_int32), ("n_contours", ctypes.c_int32), ("n_points", ctypes.c_int32),
                 ("points", _P), ("point_flags", _P), ("contour_start", _P), ("contour_closed", _P),
                 ("path_contour_start", _P), ("path_fill_rule", _P), ("path_rgba", _P),
-                ("n_clip_paths", ctypes.c_int32), ("path_clip", _P)]
+                ("n_clip_paths", ctypes.c_int32), ("path_clip", _P), ("path_stroke", _P)]
+
+
+class Outline(ctypes.Structure):
+    _fields_ = [("n_contours", ctypes.c_int32), ("n_points", ctypes.c_int32), ("points", _P), ("point_flags", _P),
+                ("contour_start", _P), ("contour_closed", _P)]
+
+
+class StrokeStyle(ctypes.Structure):
+    _fields_ = [("line_width", ctypes.c_float), ("line_cap", ctypes.c_int32), ("line_join", ctypes.c_int32),
+                ("miter_limit", ctypes.c_float)]
+
+
+class OutlineBuf(ctypes.Structure):
+    _fields_ = [("points", _P), ("point_flags", _P), ("cap_points", ctypes.c_int64), ("contour_start", _P),
+                ("contour_closed", _P), ("cap_contours", ctypes.c_int64), ("path_contour_start", _P),
+                ("n_points", ctypes.c_int64), ("n_contours", ctypes.c_int64)]
+
+
+PFH_CAP_BUTT, PFH_CAP_SQUARE, PFH_CAP_ROUND = 0, 1, 2
+PFH_JOIN_MITER, PFH_JOIN_BEVEL, PFH_JOIN_ROUND = 0, 1, 2
 
 
 class Stats(ctypes.Structure):
@@ -79,7 +99,7 @@ EXPORTS = ("pfh_create", "pfh_destroy", "pfh_last_error", "pfh_scene_upload", "p
            "pfh_dump_counts", "pfh_dump", "pfh_stats_get", "pfh_set_predictor", "pfh_scene_prediction", "pfh_path_counts",
            "pfh_mlp_eval", "pfh_mlp_grad", "pfh_mlp_train", "pfh_server_start", "pfh_server_submit",
            "pfh_server_poll", "pfh_server_stats_get", "pfh_server_stop", "pfh_get_schedule",
-           "pfh_set_schedule", "pfh_autotune")
+           "pfh_set_schedule", "pfh_autotune", "pfh_stroke_outlines")
 
 _lib = None
 
@@ -104,6 +124,7 @@ def load(path: str = LIB_PATH):
         "pfh_scene_release": (ctypes.c_int, [_P, i32]),
         "pfh_scene_features": (ctypes.c_int, [_P, i32, _P]),
         "pfh_rasterize": (ctypes.c_int, [_P, _P, i32, ctypes.c_uint32]),
+        "pfh_stroke_outlines": (ctypes.c_int, [_P, ctypes.POINTER(Outline), _P, i32, _P, ctypes.POINTER(OutlineBuf)]),
         "pfh_scene_set_target": (ctypes.c_int, [_P, i32, _P, i64]),
         "pfh_scene_target": (ctypes.c_int, [_P, i32, ctypes.POINTER(_P), ctypes.POINTER(i64)]),
         "pfh_readback_rgba8": (ctypes.c_int, [_P, i32, _P, i64]),
@@ -136,6 +157,21 @@ def load(path: str = LIB_PATH):
 
 def _ptr(a):
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+STROKE_STYLE = np.dtype([("line_width", np.float32), ("line_cap", np.int32), ("line_join", np.int32),
+                         ("miter_limit", np.float32)])
+
+
+def stroke_styles(styles) -> np.ndarray:
+    """Rows of (line_width, line_cap, line_join, miter_limit) as pfh_stroke_style records."""
+    a = np.asarray(styles)
+    if a.dtype == STROKE_STYLE:
+        return a
+    a = np.asarray(styles, np.float64).reshape(-1, 4)
+    out = np.zeros(a.shape[0], STROKE_STYLE)
+    out["line_width"], out["line_cap"], out["line_join"], out["miter_limit"] = a[:, 0], a[:, 1], a[:, 2], a[:, 3]
+    return out
 
 
 class Context:
@@ -184,10 +220,40 @@ class Context:
         d.n_clip_paths = int(getattr(scene, "n_clip_paths", 0))
         path_clip = getattr(scene, "path_clip", None)
         d.path_clip = None if path_clip is None else _ptr(path_clip)
+        path_stroke = getattr(scene, "path_stroke", None)
+        if path_stroke is not None:
+            path_stroke = np.ascontiguousarray(stroke_styles(path_stroke))
+        d.path_stroke = None if path_stroke is None else _ptr(path_stroke)
         sid = ctypes.c_int32()
         self._check(self.lib.pfh_scene_upload(self.h, ctypes.byref(d), ctypes.byref(sid)), "pfh_scene_upload")
         self._keep[sid.value] = (int(scene.width), int(scene.height), scene.n_paths)
         return sid.value
+
+    def stroke_outlines(self, points, point_flags, contour_start, contour_closed, path_contour_start, styles):
+        """pfh_stroke_outlines: every path stroked on the device with its style (rows of
+        (line_width, line_cap, line_join, miter_limit)).  Returns (points [N, 2] f32, point_flags u8,
+        contour_start i32, contour_closed u8, path_contour_start i32)."""
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 2)
+        flg = np.ascontiguousarray(point_flags, np.uint8)
+        cst = np.ascontiguousarray(contour_start, np.int32)
+        ccl = np.ascontiguousarray(contour_closed, np.uint8)
+        pcs = np.ascontiguousarray(path_contour_start, np.int32)
+        st = np.ascontiguousarray(stroke_styles(styles))
+        n = int(pcs.shape[0] - 1)
+        o = Outline(int(ccl.shape[0]), int(flg.shape[0]), _ptr(pts), _ptr(flg), _ptr(cst), _ptr(ccl))
+        out_pcs = np.zeros(n + 1, np.int32)
+        cs0 = np.zeros(1, np.int32)
+        buf = OutlineBuf(None, None, 0, _ptr(cs0), None, 0, _ptr(out_pcs), 0, 0)
+        rc = self.lib.pfh_stroke_outlines(self.h, ctypes.byref(o), _ptr(pcs), n, _ptr(st), ctypes.byref(buf))
+        if rc not in (0, -5):
+            self._check(rc, "pfh_stroke_outlines")
+        out = (np.zeros((buf.n_points, 2), np.float32), np.zeros(buf.n_points, np.uint8),
+               np.zeros(buf.n_contours + 1, np.int32), np.zeros(buf.n_contours, np.uint8))
+        buf = OutlineBuf(_ptr(out[0]), _ptr(out[1]), buf.n_points, _ptr(out[2]), _ptr(out[3]), buf.n_contours,
+                         _ptr(out_pcs), 0, 0)
+        self._check(self.lib.pfh_stroke_outlines(self.h, ctypes.byref(o), _ptr(pcs), n, _ptr(st), ctypes.byref(buf)),
+                    "pfh_stroke_outlines")
+        return out + (out_pcs,)
 
     def release(self, sid: int):
         self._check(self.lib.pfh_scene_release(self.h, sid), "pfh_scene_release")

@@ -54,12 +54,17 @@ class Scene:
     # clip path of draw path p as an index among them, or -1 (None: no path is clipped).
     n_clip_paths: int = 0
     path_clip: np.ndarray = None
+    # Strokes (Canvas::stroke_path): None, or per path (line_width, line_cap, line_join, miter_limit);
+    # a path with line_width > 0 is stroked at upload and its stroke filled with the nonzero rule.
+    path_stroke: np.ndarray = None
 
     def __post_init__(self):
         if self.view_box is None:
             self.view_box = np.array([0, 0, self.width, self.height], np.float32)
         if self.path_clip is not None:
             self.path_clip = np.ascontiguousarray(self.path_clip, np.int32)
+        if self.path_stroke is not None:
+            self.path_stroke = np.ascontiguousarray(self.path_stroke, np.float32).reshape(-1, 4)
         self.points = np.ascontiguousarray(self.points, np.float32).reshape(-1, 2)
         self.point_flags = np.ascontiguousarray(self.point_flags, np.uint8)
         self.contour_start = np.ascontiguousarray(self.contour_start, np.int32)
@@ -228,14 +233,18 @@ def clip_scene(seed: int, size: int = 1024, n_draw: int = 200, n_clip: int = 8, 
 
 
 def concat_paths(a: Scene, b: Scene) -> Scene:
-    """The paths of a followed by those of b, on a's canvas (clip information is not carried)."""
+    """The paths of a followed by those of b, on a's canvas (stroke styles carried, clip information not)."""
+    strokes = None
+    if a.path_stroke is not None or b.path_stroke is not None:
+        strokes = np.concatenate([a.path_stroke if a.path_stroke is not None else np.zeros((a.n_paths, 4), np.float32),
+                                  b.path_stroke if b.path_stroke is not None else np.zeros((b.n_paths, 4), np.float32)])
     return Scene(a.width, a.height, np.concatenate([a.points, b.points]),
                  np.concatenate([a.point_flags, b.point_flags]),
                  np.concatenate([a.contour_start, b.contour_start[1:] + a.contour_start[-1]]),
                  np.concatenate([a.contour_closed, b.contour_closed]),
                  np.concatenate([a.path_contour_start, b.path_contour_start[1:] + a.path_contour_start[-1]]),
                  np.concatenate([a.path_fill_rule, b.path_fill_rule]), np.concatenate([a.path_rgba, b.path_rgba]),
-                 a.view_box.copy())
+                 a.view_box.copy(), path_stroke=strokes)
 
 
 def stress_clip_scene(seed: int = BASE_SEED ^ 4, size: int = 4096, n_stars: int = 2000, n_max: int = 2000,
@@ -286,3 +295,135 @@ def stats(scene: Scene) -> dict:
     f = scene.point_flags
     return {"paths": scene.n_paths, "points": int(f.shape[0]), "contours": int(scene.contour_closed.shape[0]),
             "canvas": [scene.width, scene.height]}
+
+
+def cubic_polylines(seed: int, size: int = 4096, n: int = 500, seg_min: int = 4, seg_max: int = 16,
+                    step_min: float = 20.0, step_max: float = 200.0):
+    """Open polylines of seg_min..seg_max cubic Béziers each: a random walk from a uniform start point,
+    each cubic's end a step of length U(step_min, step_max) in a direction that turns by up to 90
+    degrees, its controls at 1/3 and 2/3 of the step pushed off the chord by up to half its length.
+    Returns outlines (points [N, 2], flags, contour_start [2], contour_closed [1])."""
+    rng = _rng(seed)
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(seg_min, seg_max + 1))
+        p = rng.uniform(0, size, 2)
+        ang = rng.uniform(0, 2 * math.pi)
+        pts, flg = [p], [0]
+        for _ in range(k):
+            ang += rng.uniform(-math.pi / 2, math.pi / 2)
+            step = rng.uniform(step_min, step_max)
+            d = np.array([math.cos(ang), math.sin(ang)])
+            nrm = np.array([-d[1], d[0]])
+            q = p + step * d
+            c0 = p + d * step / 3 + nrm * rng.uniform(-0.5, 0.5) * step
+            c1 = p + d * 2 * step / 3 + nrm * rng.uniform(-0.5, 0.5) * step
+            pts += [c0, c1, q]
+            flg += [1, 2, 0]
+            p = q
+        out.append((np.array(pts, np.float32), np.array(flg, np.uint8), np.array([0, len(flg)], np.int32),
+                    np.zeros(1, np.uint8)))
+    return out
+
+
+def stroked_polylines(seed: int, size: int = 4096, n: int = 500, width_min: float = 1.0, width_max: float = 20.0):
+    """cubic_polylines with a stroke style each: width U(width_min, width_max), cap and join drawn
+    uniformly from butt / square / round and miter / bevel / round, miter limit 10 (StrokeStyle's
+    default, core/stroke.h:12-26).  Returns [(outline, (width, cap, join, miter_limit))]."""
+    lines = cubic_polylines(seed, size, n)
+    rng = _rng(seed ^ 0x57)
+    return [(o, (float(rng.uniform(width_min, width_max)), int(rng.integers(0, 3)), int(rng.integers(0, 3)), 10.0))
+            for o in lines]
+
+
+def outlines_scene(width, height, outlines, rules, rgba) -> Scene:
+    """One path per outline (points, flags, contour_start, contour_closed)."""
+    pts, flg, cst, ccl, pcs = [], [], [0], [], [0]
+    for p, f, cs, cc in outlines:
+        pts.append(np.asarray(p, np.float32).reshape(-1, 2))
+        flg.append(np.asarray(f, np.uint8))
+        cst.extend((cst[-1] + np.asarray(cs[1:], np.int64)).tolist())
+        ccl.extend(np.asarray(cc, np.uint8).tolist())
+        pcs.append(len(ccl))
+    return Scene(width, height, np.concatenate(pts) if pts else np.zeros((0, 2), np.float32),
+                 np.concatenate(flg) if flg else np.zeros(0, np.uint8), np.array(cst), np.array(ccl, np.uint8),
+                 np.array(pcs), np.asarray(rules, np.uint8), np.asarray(rgba, np.uint32))
+
+
+def stroke_scene(seed: int, size: int = 4096, n: int = 500) -> Scene:
+    """n stroked cubic polylines (stroked_polylines): the open polylines as the paths' outlines with
+    their stroke styles (path_stroke: stroked at upload, the stroke filled with the nonzero rule, as
+    Canvas::stroke_path pushes it, core/canvas.cpp:272-301), colours with alpha U(0.2, 1)."""
+    lines = stroked_polylines(seed, size, n)
+    rng = _rng(seed ^ 0xC0)
+    rgb = rng.integers(0, 256, size=(n, 3))
+    alpha = np.rint(255.0 * rng.uniform(0.2, 1.0, n)).astype(np.int64)
+    sc = outlines_scene(size, size, [o for o, _ in lines], np.zeros(n, np.uint8),
+                        _pack_rgba(rgb[:, 0], rgb[:, 1], rgb[:, 2], alpha))
+    sc.path_stroke = np.array([st for _, st in lines], np.float32)
+    return sc
+
+
+def _circle(cx, cy, r):
+    """A closed circle of 4 cubic arcs (kappa 0.5523), clockwise in y-down coordinates."""
+    k = 0.5522847498 * r
+    p = [(cx + r, cy), (cx + r, cy + k), (cx + k, cy + r), (cx, cy + r), (cx - k, cy + r), (cx - r, cy + k),
+         (cx - r, cy), (cx - r, cy - k), (cx - k, cy - r), (cx, cy - r), (cx + k, cy - r), (cx + r, cy - k),
+         (cx + r, cy)]
+    return (np.array(p, np.float32), np.array([0, 1, 2] * 4 + [0], np.uint8), np.array([0, 13], np.int32),
+            np.ones(1, np.uint8))
+
+
+def _rect(x0, y0, x1, y1):
+    return (np.array([(x0, y0), (x1, y0), (x1, y1), (x0, y1)], np.float32), np.zeros(4, np.uint8),
+            np.array([0, 4], np.int32), np.ones(1, np.uint8))
+
+
+def nested_clips(seed: int, size: int = 4096, n_clip: int = 200, depth_max: int = 4):
+    """Clip paths in nesting chains: chains of depth U{1..depth_max} until n_clip paths exist; a
+    chain's first clip is a circle or a rect (a coin) of radius log-U[64, size/4] around a uniform
+    centre, each further one a circle or rect around a point inside its parent, 0.5-0.95 of its
+    size.  Returns (outlines, parent index among the clips or -1, chain of every clip)."""
+    rng = _rng(seed)
+    outlines, parent, chain = [], [], []
+    while len(outlines) < n_clip:
+        d = min(int(rng.integers(1, depth_max + 1)), n_clip - len(outlines))
+        cx, cy = rng.uniform(0, size, 2)
+        r = math.exp(rng.uniform(math.log(64.0), math.log(size / 4)))
+        prev = -1
+        cid = len(chain) and (max(chain) + 1)
+        for _ in range(d):
+            if rng.integers(0, 2):
+                outlines.append(_circle(cx, cy, r))
+            else:
+                a = r * rng.uniform(0.6, 1.0)
+                outlines.append(_rect(cx - r, cy - a, cx + r, cy + a))
+            parent.append(prev)
+            chain.append(cid)
+            prev = len(outlines) - 1
+            cx += rng.uniform(-0.4, 0.4) * r
+            cy += rng.uniform(-0.4, 0.4) * r
+            r *= rng.uniform(0.5, 0.95)
+    return outlines, np.array(parent, np.int32), np.array(chain, np.int32)
+
+
+def stress_scene(seed: int = BASE_SEED ^ 2, size: int = 4096, n_stars: int = 2000, n_strokes: int = 500,
+                 n_clip: int = 200, clipped: float = 0.5, n_max: int = 2000) -> Scene:
+    """The stress workload: n_stars star polygons {n/m} of 100..n_max vertices (star_scene), then n_strokes
+    stroked cubic polylines (stroke_scene), then n_clip clip paths in nesting chains of depth 1..4
+    (nested_clips).  Each draw path is clipped with probability `clipped` by a clip drawn uniformly
+    from all clip paths (so by chains of every depth)."""
+    stars = star_scene(seed=seed, size=size, n_stars=n_stars, n_max=n_max)
+    strokes = stroke_scene(seed ^ 0x5EED5, size=size, n=n_strokes)
+    clips, parent, _ = nested_clips(seed ^ 0xC11, size=size, n_clip=n_clip)
+    rng = _rng(seed ^ 0xC12)
+    crgb = rng.integers(0, 256, size=(n_clip, 3))
+    clip_sc = outlines_scene(size, size, clips, rng.integers(0, 2, n_clip),
+                             _pack_rgba(crgb[:, 0], crgb[:, 1], crgb[:, 2], np.full(n_clip, 255)))
+    sc = concat_paths(concat_paths(stars, strokes), clip_sc)
+    n_draw = n_stars + n_strokes
+    pc = rng.integers(0, n_clip, n_draw).astype(np.int32)
+    pc[rng.random(n_draw) >= clipped] = -1
+    sc.n_clip_paths = n_clip
+    sc.path_clip = np.concatenate([pc, parent])
+    return sc

@@ -327,18 +327,66 @@ def test_clip_scenes_in_a_batch(ctx):
         ctx.release(sid)
 
 
-def test_nested_clip_paths_rejected(ctx):
-    """A clip path with a clip of its own is undefined in the reference's hybrid tiler
-    (core/d3d9/scene_builder.cpp:209-216): the upload fails with PFH_E_ARG."""
-    scene = scenegen.clip_scene(51, size=256, n_draw=10, n_clip=2)
+def test_clip_nesting_limits(ctx):
+    """Clip paths may nest in chains of at most 4 (core/d3d11/scene_builder.cpp:92-133 levels); a
+    deeper chain, a cycle or a clip index out of range fails the upload with PFH_E_ARG."""
+    scene = scenegen.clip_scene(51, size=256, n_draw=10, n_clip=5)
+    n_draw = scene.n_paths - 5
     scene.path_clip = scene.path_clip.copy()
-    scene.path_clip[-1] = 0
-    with pytest.raises(pfh.PfhError, match="nested clip"):
+    scene.path_clip[n_draw:] = [-1, 0, 1, 2, 3]  # a chain of 5
+    with pytest.raises(pfh.PfhError, match="nest deeper"):
         ctx.upload(scene)
-    scene.path_clip[-1] = -1
-    scene.path_clip[0] = 2  # no clip path 2
+    scene.path_clip[n_draw:] = [4, 0, 1, 2, 3]  # a cycle
+    with pytest.raises(pfh.PfhError, match="nest deeper"):
+        ctx.upload(scene)
+    scene.path_clip[n_draw:] = [-1, 0, 1, 2, -1]
+    scene.path_clip[0] = 5  # no clip path 5
     with pytest.raises(pfh.PfhError, match="path_clip"):
         ctx.upload(scene)
+    scene.path_clip[0] = 3
+    check_scene(ctx, scene, name="chain of 4")
+
+
+def nested_rect_scene(size, seed):
+    """Draw blobs under chains of nested clip rects and circles (scenegen.nested_clips)."""
+    draws = scenegen.blob_scene(seed, size, size, 300, r_max=size / 4)
+    clips, parent, _ = scenegen.nested_clips(seed ^ 0x77, size=size, n_clip=24)
+    cs = scenegen.outlines_scene(size, size, clips, np.arange(24) % 2, np.full(24, 0xff000000, np.uint32))
+    sc = scenegen.concat_paths(draws, cs)
+    rng = np.random.default_rng(seed)
+    pc = rng.integers(0, 24, 300).astype(np.int32)
+    pc[rng.random(300) < 0.3] = -1
+    sc.n_clip_paths = 24
+    sc.path_clip = np.concatenate([pc, parent])
+    return sc
+
+
+@pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED | pfh.PFH_NO_GRAPH, pfh.PFH_BATCHED])
+def test_nested_clip_paths(ctx, flags):
+    """Clip paths nested up to 4 deep: each clip path is resolved against its own clip path like a
+    draw path (tiles bit-exact), its coverage the min over the chain; masks within 1/255 of the
+    FP64 model."""
+    check_scene(ctx, nested_rect_scene(1024, 61), flags=flags, name="nested clips")
+
+
+def test_stroked_polylines(ctx):
+    """Stroked cubic polylines (the library's stroke-to-fill, bit-exact with the reference's) through
+    the pipeline: tiles bit-exact, masks and pixels within tolerance."""
+    check_scene(ctx, scenegen.stroke_scene(62, size=1024, n=120), flags=pfh.PFH_BATCHED, name="strokes")
+
+
+@pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED])
+def test_stress_scene_full_size(ctx, flags):
+    """The stress workload at its full size: 4096², 2 000 star polygons of 100-2 000 vertices, 500
+    stroked cubic polylines of width 1-20, 200 clip paths nesting up to 4 deep, half of the draw
+    paths clipped.  Tiles, fills, backdrops and order bit-exact; masks within 1/255; RGBA8 within
+    1 LSB; the listed launch (no dumps) draws the same image as the dump launch."""
+    scene = scenegen.stress_scene()
+    got, img = check_scene(ctx, scene, flags=flags, name="stress full")
+    sid = ctx.upload(scene)
+    ctx.rasterize([sid], flags)
+    np.testing.assert_array_equal(ctx.readback(sid), img)
+    ctx.release(sid)
 
 
 @pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED])

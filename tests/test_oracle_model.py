@@ -264,3 +264,79 @@ def test_clip_model_windings_above_one():
         both = (wd == 2) & (wc == 2) & clip_mask
         assert both.any() and (got[both] <= 1.0 / 255.0).all()
         assert ((wd == 2) & (wc == 1) & clip_mask).any()
+
+
+def test_nested_clip_model_is_min_over_the_chain():
+    """A rect clipped by a chain of 4 nested clip rects (each clip path clipped by the next one out,
+    the level structure of core/d3d11/scene_builder.cpp:92-133): per pixel the coverage is the min of
+    the five box-filter areas, whatever the tile cases at each level (a solid tile over a mask takes
+    the mask, a mask over a solid tile keeps its own, a blank or missing tile culls).  Edges off the
+    pixel grid; tolerance 1/255 for the 8.8 clamp, as above."""
+    d = (2.5, 3.25, 120.75, 100.5)
+    chain = [(10.25, 6.5, 110.5, 90.75), (14.75, 12.25, 100.25, 80.5), (30.5, 20.75, 94.25, 70.25),
+             (22.25, 18.5, 70.75, 60.5)]
+    # paths: draw d, then clips c1..c4 where c1 is the outermost; d is clipped by c4
+    sc = _rect_scene(128, 112, [[d]] + [[c] for c in chain], [0, 0, 1, 0, 1], 4, [3, -1, 0, 1, 2])
+    m = oracle.full_model(sc)
+    got = oracle.path_coverage_image(sc, m, 0)
+    want = _box_area(128, 112, d)
+    for c in chain:
+        want = np.minimum(want, _box_area(128, 112, c))
+    assert np.abs(got - want).max() <= 1.0 / 255.0
+    assert np.abs(got - want).mean() < 1e-3
+    assert (m["canon"]["entry_path"] == 0).all()  # clip paths draw nothing
+
+
+def test_nested_clip_solid_draw_takes_the_chain():
+    """A draw rect covering the canvas (solid tiles everywhere inside) under a chain of 2 clips: its
+    tiles over the inner clip's masks take that clip's K, itself min'd with its parent where both
+    have masks: the coverage is min(box(c1), box(c2))."""
+    d = (0, 0, 96, 96)
+    c1, c2 = (8.5, 9.25, 80.75, 70.5), (20.25, 4.5, 90.5, 60.25)
+    sc = _rect_scene(96, 96, [[d], [c1], [c2]], [0, 0, 0], 2, [1, -1, 0])
+    m = oracle.full_model(sc)
+    got = oracle.path_coverage_image(sc, m, 0)
+    want = np.minimum(_box_area(96, 96, c1), _box_area(96, 96, c2))
+    assert np.abs(got - want).max() <= 1.0 / 255.0
+
+
+def test_nested_clip_windings_above_one():
+    """Windings above 1 at every level, pixel-aligned: where the draw tile and both clip levels have
+    masks the result is evenodd(min(|w_draw|, |w_c2|, |w_c1|)); outside the outer clip's rect nothing."""
+    d = [(8, 8, 88, 88), (24, 24, 72, 72)]        # winding 2 in [24, 72)^2
+    c1 = [(4, 4, 84, 84), (20, 20, 60, 92)]       # winding 2 in [20, 60) x [20, 84)
+    c2 = [(12, 0, 76, 76), (28, 14, 90, 50)]      # winding 2 in [28, 76) x [14, 50)
+    sc = _rect_scene(96, 96, [d, c1, c2], [1, 0, 1], 2, [1, -1, 0])
+    m = oracle.full_model(sc)
+    got = oracle.path_coverage_image(sc, m, 0)
+    wd = sum(_box_area(96, 96, r) for r in d)
+    w1 = sum(_box_area(96, 96, r) for r in c1)
+    w2 = sum(_box_area(96, 96, r) for r in c2)
+    canon = m["canon"]
+
+    def alpha_mask(p):
+        l, t, r, b = [int(v) for v in canon["path_rect"][p]]
+        t0 = int(canon["tile_off"][p])
+        out = np.zeros((96, 96), bool)
+        for i in range((r - l) * (b - t)):
+            tx, ty = l + i % (r - l), t + i // (r - l)
+            out[ty * 16:ty * 16 + 16, tx * 16:tx * 16 + 16] = canon["tile_alpha"][t0 + i] == 1
+        return out
+
+    all_masks = alpha_mask(0) & alpha_mask(1) & alpha_mask(2)
+    want = oracle.apply_rule(np.minimum(np.minimum(wd, w1), w2), np.ones(1, np.uint8))
+    assert all_masks.any()
+    assert np.abs(got - want)[all_masks].max() <= 1.0 / 255.0
+    assert (got[w1 == 0] == 0).all()
+    three = all_masks & (wd == 2) & (w1 == 2) & (w2 == 2)
+    assert three.any() and (got[three] <= 1.0 / 255.0).all()  # evenodd(2) covers nothing
+
+
+def test_clip_nesting_deeper_than_four_is_refused():
+    rects = [[(0, 0, 32, 32)]] + [[(1 + i, 1 + i, 30 - i, 30 - i)] for i in range(5)]
+    sc = _rect_scene(32, 32, rects, [0] * 6, 5, [4, -1, 0, 1, 2, 3])
+    with pytest.raises(ValueError):
+        oracle.full_model(sc)
+    sc.path_clip = np.array([4, -1, 0, 1, 2, 4], np.int32)  # a cycle
+    with pytest.raises(ValueError):
+        oracle.full_model(sc)
