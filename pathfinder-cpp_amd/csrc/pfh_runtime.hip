@@ -1700,15 +1700,20 @@ static double percentile(std::vector<double> v, double q) {
     return v[k];
 }
 
+// A failed HIP call also leaves its status as the thread's last error, which the launch helpers
+// read back after each launch: clear it on the way out so that a later, unrelated call does not
+// report it again.
 static int run_guarded(pfh_ctx *ctx, const std::function<void()> &f) {
     try {
         f();
         return PFH_OK;
     } catch (const PfhError &e) {
         if (ctx) ctx->err = e.msg;
+        (void)hipGetLastError();
         return e.code;
     } catch (const std::exception &e) {
         if (ctx) ctx->err = e.what();
+        (void)hipGetLastError();
         return PFH_E_ARG;
     }
 }

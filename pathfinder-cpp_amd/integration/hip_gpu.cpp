@@ -72,14 +72,17 @@ void CommandEncoderHip::execute(hipStream_t stream) {
                     std::memcpy(dst->host_ptr() + a.offset, src->host_ptr() + a.staging_offset, a.data_size);
             } break;
             case CommandType::ReadBuffer: {
+                // into the staging allocation; the recorder's callback copies it on to a.data
                 const auto &a = cmd.args.read_buffer;
                 auto *src = static_cast<BufferHip *>(a.buffer);
+                auto *staging = static_cast<BufferHip *>(a.staging_buffer);
                 if (src->device_ptr())
-                    hip_check(hipMemcpyAsync(a.data, (uint8_t *)src->device_ptr() + a.offset, a.data_size,
+                    hip_check(hipMemcpyAsync(staging->host_ptr() + a.staging_offset,
+                                             (uint8_t *)src->device_ptr() + a.offset, a.data_size,
                                              hipMemcpyDeviceToHost, stream),
                               "read_buffer");
                 else
-                    std::memcpy(a.data, src->host_ptr() + a.offset, a.data_size);
+                    std::memcpy(staging->host_ptr() + a.staging_offset, src->host_ptr() + a.offset, a.data_size);
             } break;
             case CommandType::WriteTexture: {
                 const auto &a = cmd.args.write_texture;
@@ -92,10 +95,13 @@ void CommandEncoderHip::execute(hipStream_t stream) {
                           "write_texture");
             } break;
             case CommandType::ReadTexture: {
+                // into the staging allocation; the recorder's callback (CommandEncoder::read_texture,
+                // gpu/command_encoder.cpp:317-355) copies it on to a.data once the queue has run it
                 const auto &a = cmd.args.read_texture;
                 auto *t = static_cast<TextureHip *>(a.texture);
+                auto *staging = static_cast<BufferHip *>(a.staging_buffer);
                 const size_t px = get_pixel_size(t->get_format());
-                hip_check(hipMemcpy2DAsync(a.data, a.width * px,
+                hip_check(hipMemcpy2DAsync(staging->host_ptr() + a.staging_offset, a.width * px,
                                            (uint8_t *)t->device_ptr() + a.offset_y * t->pitch() + a.offset_x * px,
                                            t->pitch(), a.width * px, a.height, hipMemcpyDeviceToHost, stream),
                           "read_texture");

@@ -14,10 +14,9 @@ import pytest
 import oracle
 import pfh
 import scenegen
+from hip_mem import DeviceBuffer
 
 pytestmark = pytest.mark.gpu
-
-torch = pytest.importorskip("torch")
 
 
 def background(h, w, seed):
@@ -35,12 +34,12 @@ def test_caller_target_with_pitch_and_load_dst(flags):
     img = np.clip(model["image"], 0.0, 1.0)  # premultiplied FP64 over transparent
     h, w = sc.height, sc.width
     pitch = w * 4 + 192
-    buf = torch.full((h * pitch,), 0xAB, dtype=torch.uint8, device="cuda")
-    torch.cuda.synchronize()
     ctx = pfh.Context(0)
+    buf = DeviceBuffer(h * pitch)
+    buf.fill(0xAB)
     sid = ctx.upload(sc)
-    ctx.set_target(sid, buf.data_ptr(), pitch)
-    assert ctx.target(sid) == (buf.data_ptr(), pitch)
+    ctx.set_target(sid, buf.ptr, pitch)
+    assert ctx.target(sid) == (buf.ptr, pitch)
     print(f"[targets] flags={flags} clear", flush=True)
     # cleared target (clear_dst = true)
     for _ in range(2):  # the second batched call replays its graph
@@ -48,7 +47,7 @@ def test_caller_target_with_pitch_and_load_dst(flags):
     got = ctx.readback(sid)
     want = np.rint(img * 255.0).astype(np.int64)
     assert np.abs(got.astype(np.int64) - want).max() <= 1
-    host = buf.cpu().numpy().reshape(h, pitch)
+    host = buf.download().reshape(h, pitch)
     np.testing.assert_array_equal(host[:, :w * 4].reshape(h, w, 4), got)
     assert (host[:, w * 4:] == 0xAB).all(), "bytes past the row were written"
     # blend over a background
@@ -57,20 +56,20 @@ def test_caller_target_with_pitch_and_load_dst(flags):
     rows[:, :w * 4] = bg.reshape(h, w * 4)
     for rep in range(2):
         print(f"[targets] flags={flags} load_dst rep={rep}", flush=True)
-        buf.copy_(torch.from_numpy(rows.reshape(-1)).to("cuda"))
-        torch.cuda.synchronize()
+        buf.upload(rows)
         ctx.rasterize([sid], flags | pfh.PFH_LOAD_DST)
         got = ctx.readback(sid).astype(np.float64)
         want = img * 255.0 + (1.0 - img[..., 3:4]) * bg.astype(np.float64)
         err = np.abs(got - np.rint(want)).max()
         assert err <= 1, f"load_dst rgba8 error {err}"
-        host = buf.cpu().numpy().reshape(h, pitch)
+        host = buf.download().reshape(h, pitch)
         assert (host[:, w * 4:] == 0xCD).all(), "bytes past the row were written"
     # back to the context-owned image
     ctx.set_target(sid, None, 0)
     ctx.rasterize([sid], flags)
     assert np.abs(ctx.readback(sid).astype(np.int64) - np.rint(img * 255.0).astype(np.int64)).max() <= 1
     ctx.close()
+    buf.free()
 
 
 def test_rejects_bad_fill_rule_and_pitch():
@@ -82,9 +81,10 @@ def test_rejects_bad_fill_rule_and_pitch():
         ctx.upload(sc)
     sc.path_fill_rule[2] = 1
     sid = ctx.upload(sc)
-    buf = torch.zeros(256 * 1024, dtype=torch.uint8, device="cuda")
+    buf = DeviceBuffer(256 * 1024)
     with pytest.raises(pfh.PfhError, match="pitch"):
-        ctx.set_target(sid, buf.data_ptr(), 256 * 4 - 4)
+        ctx.set_target(sid, buf.ptr, 256 * 4 - 4)
     with pytest.raises(pfh.PfhError, match="pitch"):
-        ctx.set_target(sid, buf.data_ptr(), 256 * 4 + 2)
+        ctx.set_target(sid, buf.ptr, 256 * 4 + 2)
     ctx.close()
+    buf.free()
