@@ -347,6 +347,27 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
                     }
                 }
                 k = __shfl(kl, (int)(batch_end - 1 - batch));
+                // A fresh batch of lines (or segments that emit nothing) while every lane is idle:
+                // each lane finishes its own staged segment at once, without the hand-over.
+                const uint32_t sk = qkind & SEG_KIND_MASK;
+                if (want == ~0ull && __ballot(batch + lane < batch_end && sk != 0 && sk != SEG_LINE) == 0) {
+                    if (batch + lane < batch_end) {
+                        const bool line_brk = sk != 0 && (qkind & SEG_CHAIN_BREAK) != 0;
+                        if (EMIT) {
+                            uint32_t o = qbase;
+                            if (line_brk) {
+                                if (o < b.cap_lines) b.lines[o] = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+                                if (o + 1 < b.cap_lines) b.lines[o + 1] = q0;
+                                o += 2;
+                            }
+                            if (sk == SEG_LINE && o < b.cap_lines) b.lines[o] = q3;
+                        } else {
+                            b.seg_cnt[batch + lane] = (sk == SEG_LINE ? 1u : 0u) + (line_brk ? 2u : 0u);
+                        }
+                    }
+                    next = batch_end;
+                    continue;
+                }
             }
             // idle lanes take the next staged segments, in lane order
             const uint32_t cand = next + (uint32_t)__popcll(want & lt_mask);
