@@ -147,6 +147,20 @@ int pfh_scene_features(pfh_ctx *ctx, int32_t scene_id, float *out);
  */
 int pfh_rasterize(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t flags);
 
+/*
+ * pfh_rasterize for PFH_PLAIN (optionally with PFH_LOAD_DST) that allocates nothing: no hipMalloc,
+ * hipFree, hipHostMalloc, event or stream creation, no allocating library call.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory during a frame (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).  The scenes must be
+ * uploaded and their targets set beforehand; the work buffers are carved, scene by scene, from one
+ * static device array of PFH_NOALLOC_SCRATCH_BYTES per GPU, which the library holds until the
+ * process ends.  A scene whose work does not fit fails with PFH_E_LIMIT (nothing is drawn into its
+ * target).  Every call on a GPU shares that GPU's array, across contexts too, and it keeps the last
+ * call's contents: calls that overlap on one GPU (two host threads, two contexts) race.
+ */
+#define PFH_NOALLOC_SCRATCH_BYTES (128ll << 20)
+int pfh_rasterize_noalloc(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t flags);
+
 /* Points a scene's output at caller-owned device memory (Canvas::set_dst_texture, core/canvas.cpp:538-540). */
 int pfh_scene_set_target(pfh_ctx *ctx, int32_t scene_id, void *device_rgba8, int64_t pitch_bytes);
 /* Device address and pitch of a scene's RGBA8 output (premultiplied). */

@@ -12,6 +12,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -98,12 +99,32 @@ struct PfhError {
 // Bumped whenever a device buffer is (re)allocated: captured graphs hold raw pointers.
 static uint64_t g_alloc_generation = 0;
 
+// A fixed region of device memory handed out front to back (the allocation-free entry point's
+// work buffers live in the library's static scratch array); reset at the start of every call.
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t cap = 0, top = 0;
+};
+
 // Grow-only device buffer (the reference grows buffers to 2^k, core/d3d11/renderer.cpp:719-724).
+// With an arena the buffer is carved from it instead (no hipMalloc / hipFree): a regrowth takes
+// fresh space above the arena's top, and running past the arena's end is an error.
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    Arena *arena = nullptr;
     void ensure(size_t need) {
         if (need <= bytes && ptr) return;
+        if (arena) {
+            const size_t cap = (std::max<size_t>(need, 4) + 255) & ~(size_t)255;
+            if (cap > arena->cap - arena->top)
+                throw PfhError{PFH_E_LIMIT, "the scene needs more work memory than the static scratch holds (" +
+                                                std::to_string(arena->cap >> 20) + " MiB)"};
+            ptr = arena->base + arena->top;
+            arena->top += cap;
+            bytes = cap;
+            return;
+        }
         size_t cap = 256;
         while (cap < need) cap <<= 1;
         if (ptr) HIP_TRY(hipFree(ptr));
@@ -119,7 +140,7 @@ struct DevBuf {
         bytes = cap;
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && !arena) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
     }
@@ -212,8 +233,11 @@ struct Workspace {
     std::map<GraphKey, CapturedGraph> graphs;
     uint64_t graphs_generation = 0;  // g_alloc_generation the cached graphs were captured under
     hipEvent_t stage_ev[9] = {};     // plain mode: between its stages
+    Arena arena;                     // work buffers carved from a fixed region (arena.base set)
 
-    void create() {
+    // Stream, events and pinned host buffers for `table_scenes` scenes per launch, made once so
+    // that a launch of at most that many scenes creates nothing.
+    void create(size_t table_scenes = 0) {
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP_TRY(hipEventCreate(&ev0));
         HIP_TRY(hipEventCreate(&ev1));
@@ -223,6 +247,27 @@ struct Workspace {
         for (auto &e : kev) HIP_TRY(hipEventRecord(e, stream));
         for (auto &e : stage_ev) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipHostMalloc((void **)&h_counters, 4 * C_COUNT));
+        if (table_scenes) {
+            HIP_TRY(hipHostMalloc((void **)&h_table, sizeof(SceneDev) * table_scenes));
+            h_table_cap = table_scenes;
+            HIP_TRY(hipHostMalloc((void **)&h_bases, sizeof(uint32_t) * 6 * (table_scenes + 1)));
+            h_bases_cap = 6 * (table_scenes + 1);
+        }
+    }
+    // Every work buffer is carved from [base, base + cap) from now on (no device allocation).
+    void use_arena(void *base, size_t cap) {
+        arena.base = reinterpret_cast<uint8_t *>(base);
+        arena.cap = cap;
+        arena.top = 0;
+        for (DevBuf *b : all_buffers()) {
+            b->release();
+            b->arena = &arena;
+        }
+    }
+    // Forgets every carved buffer: the next launch lays them out afresh from the arena's start.
+    void reset_arena() {
+        for (DevBuf *b : all_buffers()) b->release();
+        arena.top = 0;
     }
     static void destroy_graph(CapturedGraph &g) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -288,11 +333,39 @@ struct Workspace {
 
 }  // namespace
 
+// Work memory of the allocation-free entry point (pfh_rasterize_noalloc): one array per GPU,
+// held until the process ends and shared by every context on that GPU.  Host code never uses
+// this name as an address (it is a host-side stand-in): the device address comes from
+// hipGetSymbolAddress, once per device, in scratch_on_device.
+#define PFH_SCRATCH_BYTES ((size_t)PFH_NOALLOC_SCRATCH_BYTES)
+static __device__ __align__(256) uint8_t scratch[PFH_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device` (the current device): looked up on the first call for
+// that device, then kept.
+void *scratch_on_device(int device) {
+    if (device < 0 || device >= MAX_DEVICES) throw PfhError{PFH_E_LIMIT, "device index above 63"};
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw PfhError{PFH_E_HIP, "hipGetSymbolAddress(scratch) returned no address"};
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
 struct pfh_ctx {
     int device = 0;
     std::string err;
     std::vector<std::unique_ptr<SceneHost>> scenes;
     Workspace main;
+    Workspace fixed;       // pfh_rasterize_noalloc: buffers carved from `scratch`, events made at create
     Workspace *w = &main;  // workspace the pipeline functions enqueue into
     // predictor
     std::vector<float> predictor;
@@ -1732,6 +1805,8 @@ int pfh_create(int device, pfh_ctx **out) {
         if (device < 0 || device >= n) throw PfhError{PFH_E_ARG, "no HIP device " + std::to_string(device)};
         HIP_TRY(hipSetDevice(device));
         ctx->main.create();
+        ctx->fixed.create(1);  // plain mode: one scene per launch
+        ctx->fixed.use_arena(scratch_on_device(device), PFH_SCRATCH_BYTES);
     });
     if (rc != PFH_OK) {
         fprintf(stderr, "pfh_create: %s\n", ctx->err.c_str());
@@ -1747,6 +1822,7 @@ void pfh_destroy(pfh_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     server_destroy(ctx);
     ctx->main.destroy();
+    ctx->fixed.destroy();
     for (auto &s : ctx->scenes) {
         if (!s) continue;
         for (DevBuf *b : {&s->d_features, &s->d_pts, &s->d_segs, &s->d_seg_path, &s->d_path_rect, &s->d_path_tile_off,
@@ -1848,6 +1924,38 @@ int pfh_rasterize(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fl
             HIP_TRY(hipEventSynchronize(ctx->w->ev1));
             ctx->stats.ms_total = elapsed_ms(ctx, ctx->w->ev0, ctx->w->ev1);
         }
+    });
+}
+
+int pfh_rasterize_noalloc(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t flags) {
+    if (!ctx || (n > 0 && !scene_ids) || n < 0) return PFH_E_ARG;
+    return run_guarded(ctx, [&] {
+        if (flags & ~(uint32_t)PFH_LOAD_DST)
+            throw PfhError{PFH_E_ARG, "pfh_rasterize_noalloc takes PFH_PLAIN, optionally with PFH_LOAD_DST"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        for (int32_t i = 0; i < n; i++) get_scene(ctx, scene_ids[i]);
+        if (n == 0) return;
+        struct Restore {
+            pfh_ctx *c;
+            ~Restore() {
+                // a launch that stopped part way leaves kernels in flight on the scratch: they end
+                // before the next call lays the scratch out again
+                (void)hipStreamSynchronize(c->fixed.stream);
+                c->w = &c->main;
+            }
+        } restore{ctx};
+        ctx->w = &ctx->fixed;
+        ctx->stats.host_syncs = 0;
+        ctx->stats.paths = 0;
+        for (double &m : ctx->stats.ms_stage) m = 0;
+        double total = 0;
+        for (int32_t i = 0; i < n; i++) {
+            ctx->stats.paths += get_scene(ctx, scene_ids[i]).n_paths;
+            ctx->fixed.reset_arena();
+            rasterize_plain(ctx, scene_ids[i], flags);
+            total += elapsed_ms(ctx, ctx->fixed.stage_ev[0], ctx->fixed.stage_ev[8]);
+        }
+        ctx->stats.ms_total = total;
     });
 }
 

@@ -20,6 +20,7 @@ PFH_PREDICT = 0x2
 PFH_DUMP = 0x4
 PFH_LOAD_DST = 0x8
 PFH_NO_GRAPH = 0x10
+PFH_NOALLOC_SCRATCH_BYTES = 128 << 20  # the static work array of pfh_rasterize_noalloc, per GPU
 
 N_FEATURES = 12
 MLP_OUTPUTS = 3
@@ -99,7 +100,7 @@ EXPORTS = ("pfh_create", "pfh_destroy", "pfh_last_error", "pfh_scene_upload", "p
            "pfh_dump_counts", "pfh_dump", "pfh_stats_get", "pfh_set_predictor", "pfh_scene_prediction", "pfh_path_counts",
            "pfh_mlp_eval", "pfh_mlp_grad", "pfh_mlp_train", "pfh_server_start", "pfh_server_submit",
            "pfh_server_poll", "pfh_server_stats_get", "pfh_server_stop", "pfh_get_schedule",
-           "pfh_set_schedule", "pfh_autotune", "pfh_stroke_outlines")
+           "pfh_set_schedule", "pfh_autotune", "pfh_stroke_outlines", "pfh_rasterize_noalloc")
 
 _lib = None
 
@@ -124,6 +125,7 @@ def load(path: str = LIB_PATH):
         "pfh_scene_release": (ctypes.c_int, [_P, i32]),
         "pfh_scene_features": (ctypes.c_int, [_P, i32, _P]),
         "pfh_rasterize": (ctypes.c_int, [_P, _P, i32, ctypes.c_uint32]),
+        "pfh_rasterize_noalloc": (ctypes.c_int, [_P, _P, i32, ctypes.c_uint32]),
         "pfh_stroke_outlines": (ctypes.c_int, [_P, ctypes.POINTER(Outline), _P, i32, _P, ctypes.POINTER(OutlineBuf)]),
         "pfh_scene_set_target": (ctypes.c_int, [_P, i32, _P, i64]),
         "pfh_scene_target": (ctypes.c_int, [_P, i32, ctypes.POINTER(_P), ctypes.POINTER(i64)]),
@@ -262,6 +264,13 @@ class Context:
     def rasterize(self, ids, flags: int = PFH_PLAIN):
         arr = np.ascontiguousarray(np.atleast_1d(np.asarray(ids, np.int32)))
         self._check(self.lib.pfh_rasterize(self.h, _ptr(arr), int(arr.shape[0]), flags), "pfh_rasterize")
+
+    def rasterize_noalloc(self, ids, flags: int = PFH_PLAIN):
+        """pfh_rasterize_noalloc: the plain pipeline with every work buffer in the library's static
+        scratch array (no device allocation during the call)."""
+        arr = np.ascontiguousarray(np.atleast_1d(np.asarray(ids, np.int32)))
+        self._check(self.lib.pfh_rasterize_noalloc(self.h, _ptr(arr), int(arr.shape[0]), flags),
+                    "pfh_rasterize_noalloc")
 
     def readback(self, sid: int) -> np.ndarray:
         w, h, _ = self._keep[sid]
