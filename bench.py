@@ -85,18 +85,21 @@ def roofline(st, fill_ms, comp_ms, pixels, n_paths):
 
 def measure(ctx, ids, flags, steps, warmup, plain_steps, schedule=None):
     """Plain and batched timings of one workload; batched kernel times from in-graph events.
-    The launch schedule is autotuned for the workload, or ``schedule`` when given."""
-    for _ in range(max(1, warmup)):
-        for sid in ids:
-            ctx.rasterize([sid], pfh.PFH_PLAIN)
+    The launch schedule is autotuned for the workload, or ``schedule`` when given.  With
+    plain_steps = 0 the plain mode is not run (plain_s is None)."""
     stage_ms = np.zeros(8)
-    t0 = time.perf_counter()
-    for _ in range(plain_steps):
-        for sid in ids:
-            ctx.rasterize([sid], pfh.PFH_PLAIN)
-            stage_ms += np.array(ctx.stats()["ms_stage"])
-    plain_s = (time.perf_counter() - t0) / plain_steps
-    stage_ms /= plain_steps
+    plain_s = None
+    if plain_steps:
+        for _ in range(max(1, warmup)):
+            for sid in ids:
+                ctx.rasterize([sid], pfh.PFH_PLAIN)
+        t0 = time.perf_counter()
+        for _ in range(plain_steps):
+            for sid in ids:
+                ctx.rasterize([sid], pfh.PFH_PLAIN)
+                stage_ms += np.array(ctx.stats()["ms_stage"])
+        plain_s = (time.perf_counter() - t0) / plain_steps
+        stage_ms /= plain_steps
     # launch schedule for this workload (once, outside the timed region)
     if schedule is None:
         sched, ms0, ms1 = ctx.autotune(ids, flags & pfh.PFH_PREDICT, reps=3)
@@ -199,7 +202,7 @@ def copy_calibration(mbytes):
     """One device-to-device copy of a known size through the HIP runtime, so that rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE of the runtime's copy kernel calibrate those counters to bytes."""
     import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime libpfh.so is linked with
     n = int(mbytes) << 20
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
@@ -222,6 +225,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="main workload only (no single_4k / serving)")
     ap.add_argument("--cpu-reps", type=int, default=5, help="timed runs per scene and leg of the CPU baseline")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run one D2D copy of this size (counter calibration)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="skip the plain mode (profiling: every dispatch of the run is a batched one)")
     ap.add_argument("--schedule", default=None,
                     help="launch schedule instead of autotuning, e.g. fill_grid=32768,composite_block=64")
     args = ap.parse_args()
@@ -249,10 +254,11 @@ def main():
     ids = [ctx.upload(s) for s in scenes]
     print("[bench] uploaded", flush=True)
 
-    plain_steps = max(1, args.steps // 4) if args.config == "batch" else args.steps
+    plain_steps = 0 if args.batched_only else max(1, args.steps // 4) if args.config == "batch" else args.steps
     plain_s, stage_ms, batched_s, fill_ms, comp_ms, st, tuning = measure(ctx, ids, flags, args.steps, args.warmup,
                                                                          plain_steps, schedule)
-    print(f"[bench] plain {plain_s * 1e3:.3f} ms/step, batched {batched_s * 1e3:.3f} ms/step", flush=True)
+    print(f"[bench] plain {plain_s * 1e3 if plain_s else float('nan'):.3f} ms/step, "
+          f"batched {batched_s * 1e3:.3f} ms/step", flush=True)
     names = ["flatten", "bin_count", "bin_emit", "propagate", "entries", "sort", "fill", "composite"]
     out = {
         "metric": METRIC, "value": round(n_paths / batched_s, 1), "unit": "paths/s", "n_gpus": 1,
@@ -262,9 +268,10 @@ def main():
             ", predictor-sized" if flags & pfh.PFH_PREDICT else ""), "scenes": len(scenes), "paths": n_paths,
                    "pixels": pixels},
         "mpix_per_s": round(pixels / batched_s / 1e6, 1),
-        "plain_paths_per_s": round(n_paths / plain_s, 1), "plain_ms_per_step": round(plain_s * 1e3, 4),
-        "batched_over_plain": round(plain_s / batched_s, 3),
-        "plain_stage_ms": {k: round(v, 4) for k, v in zip(names, stage_ms)},
+        "plain_paths_per_s": round(n_paths / plain_s, 1) if plain_s else None,
+        "plain_ms_per_step": round(plain_s * 1e3, 4) if plain_s else None,
+        "batched_over_plain": round(plain_s / batched_s, 3) if plain_s else None,
+        "plain_stage_ms": {k: round(v, 4) for k, v in zip(names, stage_ms)} if plain_s else None,
         "counts": {k: st[k] for k in ("segments", "lines", "tiles", "fills", "alpha_tiles", "entries",
                                       "alpha_entries", "fills_masked", "alpha_masked", "fb_tiles", "retries",
                                       "host_syncs")},

@@ -226,6 +226,8 @@ typedef struct pfh_schedule {
                                  8 XCDs (own L2) works on one contiguous range of tiles */
     int32_t flatten_segs;     /* segments each flatten wave works through (multiple of 64; 0: from the
                                  segment count) */
+    int32_t flatten_take;     /* idle lanes a flatten wave waits for before handing them new segments
+                                 (1..64; 0: 1) */
 } pfh_schedule;
 int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out);
 int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in);

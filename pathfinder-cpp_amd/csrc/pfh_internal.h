@@ -52,6 +52,7 @@ struct Schedule {
     uint32_t column_block = 256;      // threads per workgroup of k_propagate / k_entries
     uint32_t xcd_order = 0;           // bit 0 fill, bit 1 composite: workgroups grouped per XCD
     uint32_t flatten_segs = 0;        // segments per k_flatten wave (multiple of 64), 0: from the count
+    uint32_t flatten_take = 1;        // idle k_flatten lanes a wave waits for before a hand-over (1..64)
 };
 
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs (each with its own L2).
@@ -110,8 +111,12 @@ enum Counter {
     C_DRAWN = 8,       // [C_DRAWN, C_DRAWN + DRAWN_SLOTS): fills the fill kernel read, spread over
                        // slots (per block, by block index) so that no single address takes every add
     C_CLIP_DUPS = 8 + 64,  // draw entries whose clip mask row an earlier entry already listed
-    C_COUNT = 8 + 64 + 1
+    C_TALL = 8 + 64 + 1,   // [C_TALL, C_TALL + MAX_CLIP_LEVELS + 2): columns listed for k_propagate_tall,
+                           // per propagate phase
+    C_COUNT = 8 + 64 + 1 + 6
 };
+// Path columns taller than this many tiles are walked by a wave (k_propagate_tall), not a thread.
+constexpr int TALL_ROWS = 64;
 constexpr int DRAWN_SLOTS = 64;
 
 // One scene of a batch as the kernels see it.  Scene-local arrays are resident
@@ -194,6 +199,7 @@ struct BatchDev {
     BinItem *bin_items;           // [n_bin_wave + n_bin_group] wave class, then group class
     uint32_t *col_item;           // [W] the bin item owning each column
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
+    uint32_t *tall_cols;          // [W] columns of the current propagate phase taller than TALL_ROWS
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
     // [L] flattened chords as a point stream, in emission order: per chain of chords a NaN
     // separator and the start point, then each chord's end point; chord i of a path's range

@@ -284,7 +284,7 @@ constexpr int FLAT_LDS_LEVELS = 4;
 // (one dependent load chain per 64 segments, coalesced), and handed to the lane that takes
 // the segment by lane shuffles.
 template <bool EMIT>
-__global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_wave) {
+__global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_wave, uint32_t take) {
     __shared__ float fs[FLAT_LDS_LEVELS][6][256];
     __shared__ uint8_t fd[FLAT_LDS_LEVELS][256];
     const uint32_t tid = threadIdx.x, lane = tid & 63;
@@ -307,7 +307,10 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
     bool active = false;
     while (true) {
         const uint64_t want = __ballot(!active);
-        if (want && next < w1) {  // wave-uniform
+        // a hand-over (shuffles of the staged segments to the idle lanes) once `take` lanes are
+        // idle, or as many as there are segments left: fewer hand-overs, each feeding more lanes
+        const uint32_t idle = (uint32_t)__popcll(want);
+        if (want && next < w1 && idle >= min(take, w1 - next)) {  // wave-uniform
             if (next >= batch_end) {
                 // stage the next 64 segments, one per lane
                 batch = next;
@@ -1110,17 +1113,116 @@ __device__ __forceinline__ ClipTile clip_tile_at(const BatchDev &b, const BinIte
 // level structure of core/d3d11/scene_builder.cpp:92-133, shaders/d3d11/propagate.comp:133-176);
 // MAX_CLIP_LEVELS + 1: the draw paths' columns.  Each phase reads only tiles of earlier phases.
 // CLIPS: the batch has clip paths (without them the kernel is the plain per-column walk).
+// One tile of a path column, its propagated backdrop bd and its fill range [f0, f1) known (ai: its
+// mask row when it has fills).  ci: the bin item of the column's clip path, when it has one.
+template <bool CLIPS>
+__device__ __forceinline__ void propagate_tile(const BatchDev &b, const ColumnWalk &cw, const BinItem &ci, int row,
+                                               int8_t bd, uint32_t f0, uint32_t f1, uint32_t ai) {
+    const bool is_clip = CLIPS && cw.level != 0;
+    const bool clipped = CLIPS && (cw.clip < CLIP_EMPTY || cw.clip == CLIP_EMPTY);
+    const bool opaque = (cw.rgba >> 24) == 255u;
+    const uint32_t t = cw.tile(row);
+    const bool alpha = f1 > f0;
+    ClipTile ct{false, false, 0, 0};
+    if (CLIPS && clipped && cw.clip != CLIP_EMPTY) ct = clip_tile_at(b, ci, cw.rx + cw.x, cw.ry + row);
+    if (CLIPS && is_clip) {
+        // A clip path draws nothing itself.  Its fills make two mask rows (ai, ai + 1) that
+        // keep |sum + backdrop| under the nonzero and the even-odd rule, for the tiles that
+        // take them over; its coverage at the tile is that value, min'd with its own clip's
+        // where it is nested and both have masks (a combine job rewrites both rows).
+        const bool own = alpha && ai + 1 < b.cap_alpha;
+        if (own) {
+            b.alpha_rec[ai] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd, t);
+            b.alpha_rec[ai + 1] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd | (1u << 8), t);
+        }
+        uint8_t st = alpha ? TILE_ALPHA : (bd != 0 ? TILE_SOLID : TILE_EMPTY);
+        int8_t out_bd = bd;
+        uint32_t src = ai;
+        uint4 rec = make_uint4(1u, 0u, 0u, 0u);  // the chain: this row alone
+        if (clipped) {
+            if (!ct.inside || (!ct.alpha && ct.bd == 0)) {
+                st = TILE_EMPTY;
+                out_bd = 0;
+            } else if (ct.alpha && alpha) {
+                out_bd = 0;
+                const uint4 pr = ct.src < b.cap_alpha ? b.clip_rec[ct.src] : make_uint4(1u, 0u, 0u, 0u);
+                rec = make_uint4(1u + min(pr.x, MAX_CLIP_LEVELS - 1u), ct.src, pr.y, pr.z);
+                const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
+                if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai | CLIP_JOB_CLIP, ai);
+            } else if (ct.alpha && bd != 0) {
+                st = TILE_CLIP_MASK;  // its own clip's coverage, unchanged
+                out_bd = ct.bd;
+                src = ct.src;
+            }
+        }
+        if (st == TILE_ALPHA && own) b.clip_rec[ai] = rec;
+        if (st == TILE_ALPHA || st == TILE_CLIP_MASK) b.tile_clip_ai[t] = src;
+        b.tile_st[t] = st;
+        b.tile_bd[t] = out_bd;
+        return;
+    }
+    if (CLIPS && clipped) {
+        if (alpha && ai < b.cap_alpha)
+            b.alpha_rec[ai] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd | (cw.rule << 8), t);
+        uint8_t st;
+        int8_t out_bd = 0;
+        if (!ct.inside || (!ct.alpha && ct.bd == 0)) {
+            st = TILE_EMPTY;  // outside the clip's rect or over a blank clip tile: culled
+        } else if (ct.alpha && alpha) {
+            // both masks: min(|own + backdrop|, clip coverage) replaces the mask; backdrop 0
+            st = TILE_ALPHA;
+            const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
+            if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai, ct.src);
+        } else if (ct.alpha && bd != 0) {
+            st = TILE_CLIP_MASK;  // solid over a clip mask: the clip's mask and backdrop
+            out_bd = ct.bd;
+            b.tile_clip_ai[t] = ct.src + cw.rule;
+        } else {
+            // unchanged: an alpha tile, or a solid tile over a solid clip tile
+            const bool even = cw.rule && (bd & 1) == 0;
+            st = alpha ? TILE_ALPHA : (bd == 0 ? TILE_EMPTY : (even ? TILE_CLEAR : TILE_SOLID));
+            out_bd = bd;
+            if (st == TILE_SOLID && opaque) atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row)], cw.p);
+        }
+        b.tile_st[t] = st;
+        if (b.dump) b.tile_bd[t] = out_bd;
+        return;
+    }
+    const bool even = cw.rule && (bd & 1) == 0;
+    b.tile_st[t] = alpha ? TILE_ALPHA : (bd == 0 ? TILE_EMPTY : (even ? TILE_CLEAR : TILE_SOLID));
+    if (b.dump) b.tile_bd[t] = bd;
+    if (alpha) {
+        if (ai < b.cap_alpha) b.alpha_rec[ai] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd | (cw.rule << 8), t);
+    } else if (opaque && bd != 0 && !even) {
+        atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row)], cw.p);
+    }
+}
+
+// Column walk down a path's tiles (core/d3d9/tiler.cpp:369-439): backdrops, tile status, alpha
+// records, z-buffer, and clip resolution (tiler.cpp:385-431) against the clip path's resolved
+// tile at the same coordinates: outside its rect or over a blank tile -> culled; both with masks ->
+// a combine job (the tile's backdrop moves into it); a solid tile over a mask -> it takes that
+// mask; otherwise unchanged (a solid clip tile passes the tile through).
+// phase 0: every column (no clip paths in the batch); 1..MAX_CLIP_LEVELS: the columns of clip paths
+// of that level, a clip path nested in another being resolved against it like a draw path (the
+// level structure of core/d3d11/scene_builder.cpp:92-133, shaders/d3d11/propagate.comp:133-176);
+// MAX_CLIP_LEVELS + 1: the draw paths' columns.  Each phase reads only tiles of earlier phases.
+// CLIPS: the batch has clip paths (without them the kernel is the plain per-column walk).
+// A thread walks its column; a column of more than TALL_ROWS rows would keep its wave (and the
+// launch) waiting for its one thread, so it is listed for k_propagate_tall instead.
 template <bool CLIPS>
 __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.n_cols) return;
     const ColumnWalk cw(b, c);
     if (CLIPS && cw.level != (phase > MAX_CLIP_LEVELS ? 0u : phase)) return;
-    const bool is_clip = CLIPS && cw.level != 0;
-    const bool clipped = CLIPS && (cw.clip < CLIP_EMPTY || cw.clip == CLIP_EMPTY);
+    if (cw.h > TALL_ROWS) {
+        const uint32_t j = atomicAdd(&b.counters[C_TALL + phase], 1u);
+        b.tall_cols[j] = c;  // at most one entry per column and phase: j < n_cols
+        return;
+    }
     BinItem ci;
     if (CLIPS && cw.clip < CLIP_EMPTY) ci = b.bin_items[cw.clip];
-    const bool opaque = (cw.rgba >> 24) == 255u;
     int32_t running = b.col_bd[c];
     for (int row0 = 0; row0 < cw.h; row0 += COL_ROWS) {
         int32_t delta[COL_ROWS];
@@ -1148,84 +1250,49 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
 #pragma unroll
         for (int i = 0; i < COL_ROWS; i++) {
             if (row0 + i >= cw.h) break;
-            const uint32_t t = cw.tile(row0 + i);
-            const bool alpha = f1[i] > f0[i];
-            ClipTile ct{false, false, 0, 0};
-            if (CLIPS && clipped && cw.clip != CLIP_EMPTY) ct = clip_tile_at(b, ci, cw.rx + cw.x, cw.ry + row0 + i);
-            if (CLIPS && is_clip) {
-                // A clip path draws nothing itself.  Its fills make two mask rows (ai, ai + 1) that
-                // keep |sum + backdrop| under the nonzero and the even-odd rule, for the tiles that
-                // take them over; its coverage at the tile is that value, min'd with its own clip's
-                // where it is nested and both have masks (a combine job rewrites both rows).
-                const bool own = alpha && ai[i] + 1 < b.cap_alpha;
-                if (own) {
-                    b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i], t);
-                    b.alpha_rec[ai[i] + 1] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (1u << 8), t);
-                }
-                uint8_t st = alpha ? TILE_ALPHA : (bd[i] != 0 ? TILE_SOLID : TILE_EMPTY);
-                int8_t out_bd = bd[i];
-                uint32_t src = ai[i];
-                uint4 rec = make_uint4(1u, 0u, 0u, 0u);  // the chain: this row alone
-                if (clipped) {
-                    if (!ct.inside || (!ct.alpha && ct.bd == 0)) {
-                        st = TILE_EMPTY;
-                        out_bd = 0;
-                    } else if (ct.alpha && alpha) {
-                        out_bd = 0;
-                        const uint4 pr = ct.src < b.cap_alpha ? b.clip_rec[ct.src] : make_uint4(1u, 0u, 0u, 0u);
-                        rec = make_uint4(1u + min(pr.x, MAX_CLIP_LEVELS - 1u), ct.src, pr.y, pr.z);
-                        const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
-                        if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai[i] | CLIP_JOB_CLIP, ai[i]);
-                    } else if (ct.alpha && bd[i] != 0) {
-                        st = TILE_CLIP_MASK;  // its own clip's coverage, unchanged
-                        out_bd = ct.bd;
-                        src = ct.src;
-                    }
-                }
-                if (st == TILE_ALPHA && own) b.clip_rec[ai[i]] = rec;
-                if (st == TILE_ALPHA || st == TILE_CLIP_MASK) b.tile_clip_ai[t] = src;
-                b.tile_st[t] = st;
-                b.tile_bd[t] = out_bd;
-                continue;
-            }
-            if (CLIPS && clipped) {
-                if (alpha && ai[i] < b.cap_alpha)
-                    b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (cw.rule << 8), t);
-                uint8_t st;
-                int8_t out_bd = 0;
-                if (!ct.inside || (!ct.alpha && ct.bd == 0)) {
-                    st = TILE_EMPTY;  // outside the clip's rect or over a blank clip tile: culled
-                } else if (ct.alpha && alpha) {
-                    // both masks: min(|own + backdrop|, clip coverage) replaces the mask; backdrop 0
-                    st = TILE_ALPHA;
-                    const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
-                    if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai[i], ct.src);
-                } else if (ct.alpha && bd[i] != 0) {
-                    st = TILE_CLIP_MASK;  // solid over a clip mask: the clip's mask and backdrop
-                    out_bd = ct.bd;
-                    b.tile_clip_ai[t] = ct.src + cw.rule;
-                } else {
-                    // unchanged: an alpha tile, or a solid tile over a solid clip tile
-                    const bool even = cw.rule && (bd[i] & 1) == 0;
-                    st = alpha ? TILE_ALPHA : (bd[i] == 0 ? TILE_EMPTY : (even ? TILE_CLEAR : TILE_SOLID));
-                    out_bd = bd[i];
-                    if (st == TILE_SOLID && opaque) atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row0 + i)], cw.p);
-                }
-                b.tile_st[t] = st;
-                if (b.dump) b.tile_bd[t] = out_bd;
-                continue;
-            }
-            const bool even = cw.rule && (bd[i] & 1) == 0;
-            b.tile_st[t] = alpha ? TILE_ALPHA : (bd[i] == 0 ? TILE_EMPTY : (even ? TILE_CLEAR : TILE_SOLID));
-            if (b.dump) b.tile_bd[t] = bd[i];
-            if (alpha) {
-                if (ai[i] < b.cap_alpha) b.alpha_rec[ai[i]] = make_uint4(f0[i], f1[i], (uint32_t)(uint8_t)bd[i] | (cw.rule << 8), t);
-            } else if (opaque && bd[i] != 0 && !even) {
-                atomicMax(&b.zbuf[cw.fb_base + cw.fb_local(row0 + i)], cw.p);
-            }
+            propagate_tile<CLIPS>(b, cw, ci, row0 + i, bd[i], f0[i], f1[i], ai[i]);
         }
     }
     b.col_bd[c] = running;
+}
+
+// The tall columns k_propagate listed for this phase, a wave per column: its lanes take 64 rows
+// at a time, and a prefix sum of the rows' backdrop deltas across the wave gives each row the
+// backdrop the serial walk reaches there (the int8 deltas summed in int32, as the walk does).
+// Every wave of the grid strides over the list; its length is only known on the device.
+template <bool CLIPS>
+__global__ __launch_bounds__(256) void k_propagate_tall(BatchDev b, uint32_t phase) {
+    const uint32_t n = min(b.counters[C_TALL + phase], b.n_cols);
+    const int lane = (int)(threadIdx.x & 63);
+    for (uint32_t j = blockIdx.x * 4 + (threadIdx.x >> 6); j < n; j += gridDim.x * 4) {  // wave-uniform
+        const uint32_t c = b.tall_cols[j];
+        const ColumnWalk cw(b, c);
+        BinItem ci;
+        if (CLIPS && cw.clip < CLIP_EMPTY) ci = b.bin_items[cw.clip];
+        int32_t running = b.col_bd[c];
+        for (int row0 = 0; row0 < cw.h; row0 += 64) {
+            const int row = row0 + lane;
+            int32_t d8 = 0;
+            uint32_t f0 = 0, f1 = 0, ai = 0;
+            if (row < cw.h) {
+                const uint32_t t = cw.tile(row);
+                d8 = (int8_t)b.tile_delta[t];
+                f0 = b.tile_fill_off[t];
+                f1 = b.tile_fill_off[t + 1];
+                if (f1 > f0) ai = b.tile_alpha_idx[t];
+            }
+            uint32_t incl = (uint32_t)d8;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = shfl_up_val(incl, d);
+                if (lane >= d) incl += o;
+            }
+            if (row < cw.h)
+                propagate_tile<CLIPS>(b, cw, ci, row, (int8_t)(running + (int32_t)(incl - (uint32_t)d8)), f0, f1, ai);
+            running += (int32_t)__builtin_amdgcn_readlane(incl, 63);
+        }
+        if (lane == 0) b.col_bd[c] = running;
+    }
 }
 
 // Draw entries: tiles with a mask or a nonzero backdrop that survive the z-cull
@@ -1594,13 +1661,14 @@ hipError_t scan_paths(const BatchDev &b, void *tmp, hipStream_t s) {
                          b.n_paths, reinterpret_cast<U2 *>(tmp), s);
 }
 
-// segs_per_wave: segments each wave works through (a multiple of 64, at least 64)
-hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, hipStream_t s) {
+// segs_per_wave: segments each wave works through (a multiple of 64, at least 64); take: idle
+// lanes a wave gathers before handing out segments (1..64)
+hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, uint32_t take, hipStream_t s) {
     if (b.n_segs == 0) return hipSuccess;
-    if (segs_per_wave < 64 || segs_per_wave % 64) return hipErrorInvalidValue;
+    if (segs_per_wave < 64 || segs_per_wave % 64 || take < 1 || take > 64) return hipErrorInvalidValue;
     const dim3 grid(blocks_for(blocks_for(b.n_segs, segs_per_wave), 4));
-    if (emit) hipLaunchKernelGGL(k_flatten<true>, grid, dim3(256), 0, s, b, segs_per_wave);
-    else hipLaunchKernelGGL(k_flatten<false>, grid, dim3(256), 0, s, b, segs_per_wave);
+    if (emit) hipLaunchKernelGGL(k_flatten<true>, grid, dim3(256), 0, s, b, segs_per_wave, take);
+    else hipLaunchKernelGGL(k_flatten<false>, grid, dim3(256), 0, s, b, segs_per_wave, take);
     return hipGetLastError();
 }
 
@@ -1629,13 +1697,19 @@ hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s) {
     if (b.n_cols == 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     const dim3 g(blocks_for(b.n_cols, block));
+    // the tall columns' waves: enough to cover the device, striding over a list of unknown length
+    const dim3 gt(std::min<uint32_t>(blocks_for(b.n_cols, 4), 512));
     if (!b.has_clips) {
         hipLaunchKernelGGL(k_propagate<false>, g, dim3(block), 0, s, b, 0u);
+        hipLaunchKernelGGL(k_propagate_tall<false>, gt, dim3(256), 0, s, b, 0u);
     } else {
         // clip levels from the outermost in, then the draw paths
-        for (uint32_t level = 1; level <= std::min(b.clip_levels, MAX_CLIP_LEVELS); level++)
+        for (uint32_t level = 1; level <= std::min(b.clip_levels, MAX_CLIP_LEVELS); level++) {
             hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, level);
+            hipLaunchKernelGGL(k_propagate_tall<true>, gt, dim3(256), 0, s, b, level);
+        }
         hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, MAX_CLIP_LEVELS + 1);
+        hipLaunchKernelGGL(k_propagate_tall<true>, gt, dim3(256), 0, s, b, MAX_CLIP_LEVELS + 1);
     }
     return hipGetLastError();
 }

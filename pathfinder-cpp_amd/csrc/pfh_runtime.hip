@@ -26,7 +26,7 @@ size_t scan_temp_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp, hipStream_t s);
 hipError_t scan_entries(uint2 *inout, uint32_t n, uint32_t *counters, void *tmp, hipStream_t s);
 hipError_t scan_paths(const BatchDev &b, void *tmp, hipStream_t s);
-hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, hipStream_t s);
+hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, uint32_t take, hipStream_t s);
 hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s);
 hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
                         uint32_t *tmp, hipStream_t s);
@@ -191,10 +191,10 @@ struct GraphKey {
     std::vector<int32_t> ids;
     uint32_t flags;
     uint32_t cap_f, cap_a, cap_e, cap_l;
-    uint32_t sched[5];  // the launch schedule the graph was captured with
+    uint32_t sched[6];  // the launch schedule the graph was captured with
     bool operator<(const GraphKey &o) const {
         if (ids != o.ids) return ids < o.ids;
-        for (int i = 0; i < 5; i++)
+        for (int i = 0; i < 6; i++)
             if (sched[i] != o.sched[i]) return sched[i] < o.sched[i];
         if (flags != o.flags) return flags < o.flags;
         if (cap_f != o.cap_f) return cap_f < o.cap_f;
@@ -221,7 +221,7 @@ struct Workspace {
     // around the fill and composite kernels of a batched launch; recorded as event nodes
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
-    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, ent_quarter, path_tot, bin_items, col_item, col_bd,
+    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, ent_quarter, path_tot, bin_items, col_item, col_bd, tall_cols,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, alpha_vis, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp, tile_clip_ai, clip_jobs, clip_rec, clip_seen;
     uint32_t *h_counters = nullptr;  // pinned
@@ -298,7 +298,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &path_ent, &ent_quarter, &path_tot, &bin_items, &col_item, &col_bd, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
+                &path_ent, &ent_quarter, &path_tot, &bin_items, &col_item, &col_bd, &tall_cols, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &entry_meta, &counters, &scan_tmp,
                 &sort_tmp, &tile_clip_ai, &clip_jobs, &clip_rec, &clip_seen};
     }
@@ -1025,6 +1025,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.bin_items.ensure(sizeof(BinItem) * (pl.BW + pl.BG) + 64);
     w.col_item.ensure(4 * pl.W + 4);
     w.col_bd.ensure(4 * pl.W + 4);
+    w.tall_cols.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
     w.lines.ensure(8ull * pl.cap_l);
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
@@ -1219,6 +1220,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.col_item = w.col_item.as<uint32_t>();
     b.n_paths = (uint32_t)pl.P;
     b.col_bd = w.col_bd.as<int32_t>();
+    b.tall_cols = w.tall_cols.as<uint32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
     b.lines = w.lines.as<float2>();
     b.line_cnt = w.line_cnt.as<uint32_t>();
@@ -1259,9 +1261,9 @@ static void stage_clear(Workspace &w, const BatchDev &b) {
 static void stage_count(Workspace &w, const BatchDev &b, const Plan &pl) {
     hipStream_t s = w.stream;
     uint32_t *tmp = w.scan_tmp.as<uint32_t>();
-    HIP_TRY(launch_flatten(b, false, flatten_segs(pl), s));
+    HIP_TRY(launch_flatten(b, false, flatten_segs(pl), pl.sched.flatten_take, s));
     HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp, s));
-    HIP_TRY(launch_flatten(b, true, flatten_segs(pl), s));
+    HIP_TRY(launch_flatten(b, true, flatten_segs(pl), pl.sched.flatten_take, s));
     HIP_TRY(launch_bin_items(b, s));
     HIP_TRY(launch_bin(b, false, s));
     HIP_TRY(scan_paths(b, w.scan_tmp.ptr, s));
@@ -1353,7 +1355,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     hipEvent_t *ev = ctx->w->stage_ev;
     HIP_TRY(hipEventRecord(ev[0], st));
     stage_clear(*ctx->w, b);
-    HIP_TRY(launch_flatten(b, false, flatten_segs(pl), st));
+    HIP_TRY(launch_flatten(b, false, flatten_segs(pl), pl.sched.flatten_take, st));
     HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, tmp(), st));
     read_counters(ctx);  // chord count (renderer.cpp:707-724)
     uint32_t nl = ctx->w->h_counters[C_LINES];
@@ -1362,7 +1364,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
         ensure_sizes(*ctx->w, pl, dump);
         b = batch_dev(*ctx->w, pl, dump);
     }
-    HIP_TRY(launch_flatten(b, true, flatten_segs(pl), st));
+    HIP_TRY(launch_flatten(b, true, flatten_segs(pl), pl.sched.flatten_take, st));
     HIP_TRY(hipEventRecord(ev[1], st));
     HIP_TRY(launch_bin_items(b, st));
     HIP_TRY(launch_bin(b, false, st));
@@ -1458,7 +1460,7 @@ static void batched_prepare(pfh_ctx *ctx, const int32_t *ids, int32_t n, uint32_
         upload_tables(w, pl);
         BatchDev b = batch_dev(w, pl, bl.dump);
         stage_clear(w, b);
-        HIP_TRY(launch_flatten(b, false, flatten_segs(pl), w.stream));
+        HIP_TRY(launch_flatten(b, false, flatten_segs(pl), pl.sched.flatten_take, w.stream));
         HIP_TRY(scan_total(b.seg_cnt, b.n_segs, b.counters, C_LINES, w.scan_tmp.as<uint32_t>(), w.stream));
         read_counters(ctx);
         bl.plc = w.h_counters[C_LINES];
@@ -1505,7 +1507,7 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
                      pl.cap_e,
                      pl.cap_l,
                      {pl.sched.fill_grid, pl.sched.composite_block, pl.sched.column_block, pl.sched.xcd_order,
-                      pl.sched.flatten_segs}};
+                      pl.sched.flatten_segs, pl.sched.flatten_take}};
         if (w.graphs_generation != g_alloc_generation) {
             // some buffer moved since these graphs were captured: never replay them
             w.drop_graphs();
@@ -2333,7 +2335,7 @@ static bool valid_schedule(const pfh_schedule &s) {
     auto blk = [](int32_t v) { return v == 64 || v == 128 || v == 256; };
     return s.fill_grid >= 1 && s.fill_grid <= (1 << 20) && blk(s.composite_block) && blk(s.column_block) &&
            s.xcd_order >= 0 && s.xcd_order <= 3 && s.flatten_segs >= 0 && s.flatten_segs <= (1 << 16) &&
-           s.flatten_segs % 64 == 0;
+           s.flatten_segs % 64 == 0 && s.flatten_take >= 0 && s.flatten_take <= 64;
 }
 
 int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out) {
@@ -2343,6 +2345,7 @@ int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out) {
     out->column_block = (int32_t)ctx->sched.column_block;
     out->xcd_order = (int32_t)ctx->sched.xcd_order;
     out->flatten_segs = (int32_t)ctx->sched.flatten_segs;
+    out->flatten_take = (int32_t)ctx->sched.flatten_take;
     return PFH_OK;
 }
 
@@ -2355,6 +2358,7 @@ int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in) {
         ctx->sched.column_block = (uint32_t)in->column_block;
         ctx->sched.xcd_order = (uint32_t)in->xcd_order;
         ctx->sched.flatten_segs = (uint32_t)in->flatten_segs;
+        ctx->sched.flatten_take = in->flatten_take ? (uint32_t)in->flatten_take : 1u;
     });
 }
 
@@ -2381,16 +2385,17 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         Schedule best = ctx->sched;
         double best_ms = measure();
         if (ms_start) *ms_start = best_ms;
-        const std::vector<uint32_t> cand[5] = {{1024, 2048, 4096, 8192, 16384, 32768, 65536}, {64, 128, 256}, {64, 128, 256},
-                                              {0, 1, 2, 3}, {0, 64, 128, 256, 512, 1024, 2048}};
-        for (int param = 0; param < 5; param++) {
+        const std::vector<uint32_t> cand[6] = {{1024, 2048, 4096, 8192, 16384, 32768, 65536}, {64, 128, 256}, {64, 128, 256},
+                                              {0, 1, 2, 3}, {0, 64, 128, 256, 512, 1024, 2048}, {1, 4, 8, 16, 32}};
+        for (int param = 0; param < 6; param++) {
             for (uint32_t v : cand[param]) {
                 Schedule trial = best;
                 uint32_t *field = param == 0 ? &trial.fill_grid
                                   : param == 1 ? &trial.composite_block
                                   : param == 2 ? &trial.column_block
                                   : param == 3 ? &trial.xcd_order
-                                               : &trial.flatten_segs;
+                                  : param == 4 ? &trial.flatten_segs
+                                               : &trial.flatten_take;
                 if (*field == v) continue;
                 *field = v;
                 ctx->sched = trial;
@@ -2407,6 +2412,7 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         out->column_block = (int32_t)best.column_block;
         out->xcd_order = (int32_t)best.xcd_order;
         out->flatten_segs = (int32_t)best.flatten_segs;
+        out->flatten_take = (int32_t)best.flatten_take;
         if (ms_tuned) *ms_tuned = best_ms;
     });
 }
