@@ -228,6 +228,8 @@ typedef struct pfh_schedule {
                                  segment count) */
     int32_t flatten_take;     /* idle lanes a flatten wave waits for before handing them new segments
                                  (1..64; 0: 1) */
+    int32_t column_rows;      /* path columns of more tiles than this are propagated by 16 or 64 lanes
+                                 each instead of one (1..64; 0: 64) */
 } pfh_schedule;
 int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out);
 int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in);

@@ -53,7 +53,17 @@ struct Schedule {
     uint32_t xcd_order = 0;           // bit 0 fill, bit 1 composite: workgroups grouped per XCD
     uint32_t flatten_segs = 0;        // segments per k_flatten wave (multiple of 64), 0: from the count
     uint32_t flatten_take = 1;        // idle k_flatten lanes a wave waits for before a hand-over (1..64)
+    uint32_t column_rows = 64;        // path columns of more tiles are walked by 16 or 64 lanes (1..64)
 };
+
+// Inclusive prefix sum within each 16-lane row of the wave (DPP row shifts: no LDS).
+__device__ __forceinline__ uint32_t row16_inclusive_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    return v;
+}
 
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs (each with its own L2).
 // Logical block of hardware block b when the grid's blocks are regrouped so that each XCD
@@ -111,11 +121,12 @@ enum Counter {
     C_DRAWN = 8,       // [C_DRAWN, C_DRAWN + DRAWN_SLOTS): fills the fill kernel read, spread over
                        // slots (per block, by block index) so that no single address takes every add
     C_CLIP_DUPS = 8 + 64,  // draw entries whose clip mask row an earlier entry already listed
-    C_TALL = 8 + 64 + 1,   // [C_TALL, C_TALL + MAX_CLIP_LEVELS + 2): columns listed for k_propagate_tall,
-                           // per propagate phase
-    C_COUNT = 8 + 64 + 1 + 6
+    C_TALL = 8 + 64 + 1,   // [C_TALL, C_TALL + 6): per propagate phase, columns of more than
+                           // TALL_ROWS tiles listed for a wave each (k_propagate_rows<64>)
+    C_MID = 8 + 64 + 1 + 6,  // [C_MID, C_MID + 6): per phase, columns of more than
+                             // Schedule::column_rows tiles and at most TALL_ROWS (k_propagate_rows<16>)
+    C_COUNT = 8 + 64 + 1 + 12
 };
-// Path columns taller than this many tiles are walked by a wave (k_propagate_tall), not a thread.
 constexpr int TALL_ROWS = 64;
 constexpr int DRAWN_SLOTS = 64;
 
@@ -199,7 +210,8 @@ struct BatchDev {
     BinItem *bin_items;           // [n_bin_wave + n_bin_group] wave class, then group class
     uint32_t *col_item;           // [W] the bin item owning each column
     int32_t *col_bd;              // [W] above-rect backdrop, then final column sum
-    uint32_t *tall_cols;          // [W] columns of the current propagate phase taller than TALL_ROWS
+    uint32_t *tall_cols;          // [W] columns of the current propagate phase walked by several lanes:
+                                  // the C_MID list from the front, the C_TALL list from the back
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
     // [L] flattened chords as a point stream, in emission order: per chain of chords a NaN
     // separator and the start point, then each chord's end point; chord i of a path's range

@@ -1113,18 +1113,32 @@ __device__ __forceinline__ ClipTile clip_tile_at(const BatchDev &b, const BinIte
 // level structure of core/d3d11/scene_builder.cpp:92-133, shaders/d3d11/propagate.comp:133-176);
 // MAX_CLIP_LEVELS + 1: the draw paths' columns.  Each phase reads only tiles of earlier phases.
 // CLIPS: the batch has clip paths (without them the kernel is the plain per-column walk).
+// What a tile of a path column reads of its clip path: the clip's tile at the same coordinates
+// and, for a clip path's tile over a clip mask, the chain record of that mask.  Both were written
+// by an earlier propagate phase, so a column loads them for a group of rows before it stores
+// anything (no load waits behind a store it might alias).
+struct ClipRead {
+    ClipTile ct;
+    uint4 pr;
+};
+__device__ __forceinline__ ClipRead clip_read(const BatchDev &b, const ColumnWalk &cw, const BinItem &ci, int row) {
+    ClipRead r{{false, false, 0, 0}, make_uint4(1u, 0u, 0u, 0u)};
+    if (cw.clip < CLIP_EMPTY) r.ct = clip_tile_at(b, ci, cw.rx + cw.x, cw.ry + row);
+    if (cw.level != 0 && r.ct.alpha && r.ct.src < b.cap_alpha) r.pr = b.clip_rec[r.ct.src];
+    return r;
+}
+
 // One tile of a path column, its propagated backdrop bd and its fill range [f0, f1) known (ai: its
-// mask row when it has fills).  ci: the bin item of the column's clip path, when it has one.
+// mask row when it has fills), with what it reads of its clip path (CLIPS).
 template <bool CLIPS>
-__device__ __forceinline__ void propagate_tile(const BatchDev &b, const ColumnWalk &cw, const BinItem &ci, int row,
+__device__ __forceinline__ void propagate_tile(const BatchDev &b, const ColumnWalk &cw, const ClipRead &cr, int row,
                                                int8_t bd, uint32_t f0, uint32_t f1, uint32_t ai) {
     const bool is_clip = CLIPS && cw.level != 0;
     const bool clipped = CLIPS && (cw.clip < CLIP_EMPTY || cw.clip == CLIP_EMPTY);
     const bool opaque = (cw.rgba >> 24) == 255u;
     const uint32_t t = cw.tile(row);
     const bool alpha = f1 > f0;
-    ClipTile ct{false, false, 0, 0};
-    if (CLIPS && clipped && cw.clip != CLIP_EMPTY) ct = clip_tile_at(b, ci, cw.rx + cw.x, cw.ry + row);
+    const ClipTile &ct = cr.ct;
     if (CLIPS && is_clip) {
         // A clip path draws nothing itself.  Its fills make two mask rows (ai, ai + 1) that
         // keep |sum + backdrop| under the nonzero and the even-odd rule, for the tiles that
@@ -1145,7 +1159,7 @@ __device__ __forceinline__ void propagate_tile(const BatchDev &b, const ColumnWa
                 out_bd = 0;
             } else if (ct.alpha && alpha) {
                 out_bd = 0;
-                const uint4 pr = ct.src < b.cap_alpha ? b.clip_rec[ct.src] : make_uint4(1u, 0u, 0u, 0u);
+                const uint4 pr = cr.pr;
                 rec = make_uint4(1u + min(pr.x, MAX_CLIP_LEVELS - 1u), ct.src, pr.y, pr.z);
                 const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
                 if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai | CLIP_JOB_CLIP, ai);
@@ -1208,17 +1222,20 @@ __device__ __forceinline__ void propagate_tile(const BatchDev &b, const ColumnWa
 // level structure of core/d3d11/scene_builder.cpp:92-133, shaders/d3d11/propagate.comp:133-176);
 // MAX_CLIP_LEVELS + 1: the draw paths' columns.  Each phase reads only tiles of earlier phases.
 // CLIPS: the batch has clip paths (without them the kernel is the plain per-column walk).
-// A thread walks its column; a column of more than TALL_ROWS rows would keep its wave (and the
-// launch) waiting for its one thread, so it is listed for k_propagate_tall instead.
+// A thread walks its column: rows of neighbouring columns are neighbours in memory, so the
+// wave's loads coalesce.  A walk is a chain of dependent loads per group of rows, and a launch
+// lasts as long as its longest walk: a column of more than `rows` tiles (a schedule choice) is
+// listed instead for k_propagate_rows, which spreads its rows over 16 or 64 lanes.
 template <bool CLIPS>
-__global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
+__global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase, uint32_t rows) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.n_cols) return;
     const ColumnWalk cw(b, c);
     if (CLIPS && cw.level != (phase > MAX_CLIP_LEVELS ? 0u : phase)) return;
-    if (cw.h > TALL_ROWS) {
-        const uint32_t j = atomicAdd(&b.counters[C_TALL + phase], 1u);
-        b.tall_cols[j] = c;  // at most one entry per column and phase: j < n_cols
+    if (cw.h > (int)rows) {
+        // at most one entry per column and phase, so the two lists together hold at most n_cols
+        if (cw.h > TALL_ROWS) b.tall_cols[b.n_cols - 1 - atomicAdd(&b.counters[C_TALL + phase], 1u)] = c;
+        else b.tall_cols[atomicAdd(&b.counters[C_MID + phase], 1u)] = c;
         return;
     }
     BinItem ci;
@@ -1240,58 +1257,78 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase) {
         }
         int8_t bd[COL_ROWS];
         uint32_t ai[COL_ROWS];
+        ClipRead cr[COL_ROWS];
 #pragma unroll
         for (int i = 0; i < COL_ROWS; i++) {
             bd[i] = (int8_t)running;
             running += (int8_t)delta[i];
             ai[i] = 0;
-            if (row0 + i < cw.h && f1[i] > f0[i]) ai[i] = b.tile_alpha_idx[cw.tile(row0 + i)];
+            // with clip paths the group's loads go out at once (the walk is latency-bound there)
+            if (row0 + i < cw.h && (CLIPS || f1[i] > f0[i])) ai[i] = b.tile_alpha_idx[cw.tile(row0 + i)];
+            cr[i] = ClipRead{{false, false, 0, 0}, make_uint4(1u, 0u, 0u, 0u)};
+            if (CLIPS && row0 + i < cw.h) cr[i] = clip_read(b, cw, ci, row0 + i);
         }
 #pragma unroll
         for (int i = 0; i < COL_ROWS; i++) {
             if (row0 + i >= cw.h) break;
-            propagate_tile<CLIPS>(b, cw, ci, row0 + i, bd[i], f0[i], f1[i], ai[i]);
+            propagate_tile<CLIPS>(b, cw, cr[i], row0 + i, bd[i], f0[i], f1[i], ai[i]);
         }
     }
     b.col_bd[c] = running;
 }
 
-// The tall columns k_propagate listed for this phase, a wave per column: its lanes take 64 rows
-// at a time, and a prefix sum of the rows' backdrop deltas across the wave gives each row the
-// backdrop the serial walk reaches there (the int8 deltas summed in int32, as the walk does).
-// Every wave of the grid strides over the list; its length is only known on the device.
-template <bool CLIPS>
-__global__ __launch_bounds__(256) void k_propagate_tall(BatchDev b, uint32_t phase) {
-    const uint32_t n = min(b.counters[C_TALL + phase], b.n_cols);
-    const int lane = (int)(threadIdx.x & 63);
-    for (uint32_t j = blockIdx.x * 4 + (threadIdx.x >> 6); j < n; j += gridDim.x * 4) {  // wave-uniform
-        const uint32_t c = b.tall_cols[j];
+// The columns k_propagate listed for this phase (SUB 64: the C_TALL list, SUB 16: the C_MID list),
+// SUB lanes per column: a column's lanes take SUB rows at a time, and a prefix sum of the rows'
+// backdrop deltas across them gives each row the backdrop the serial walk reaches there (the int8
+// deltas summed in int32, as the walk does).  Every wave strides over the list (its length is
+// only known on the device), 64 / SUB columns at a time.
+template <bool CLIPS, int SUB>
+__global__ __launch_bounds__(256) void k_propagate_rows(BatchDev b, uint32_t phase) {
+    constexpr uint32_t G = 64 / SUB;  // columns per wave
+    const uint32_t n = min(b.counters[(SUB == 64 ? C_TALL : C_MID) + phase], b.n_cols);
+    const int lane = (int)(threadIdx.x & 63), li = lane % SUB, g = lane / SUB;
+    const uint32_t waves = gridDim.x * 4;
+    for (uint32_t j0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * G; j0 < n; j0 += waves * G) {  // wave-uniform
+        const uint32_t j = j0 + (uint32_t)g;
+        const bool has = j < n;
+        const uint32_t c = has ? b.tall_cols[SUB == 64 ? b.n_cols - 1 - j : j] : 0u;
         const ColumnWalk cw(b, c);
+        const int h = has ? cw.h : 0;
         BinItem ci;
-        if (CLIPS && cw.clip < CLIP_EMPTY) ci = b.bin_items[cw.clip];
-        int32_t running = b.col_bd[c];
-        for (int row0 = 0; row0 < cw.h; row0 += 64) {
-            const int row = row0 + lane;
+        if (CLIPS && has && cw.clip < CLIP_EMPTY) ci = b.bin_items[cw.clip];
+        int32_t running = has ? b.col_bd[c] : 0;
+        int hmax = h;
+#pragma unroll
+        for (uint32_t k = 0; k < G; k++) hmax = max(hmax, __builtin_amdgcn_readlane(h, k * SUB));
+        for (int row0 = 0; row0 < hmax; row0 += SUB) {
+            const int row = row0 + li;
+            const bool on = row < h;
             int32_t d8 = 0;
             uint32_t f0 = 0, f1 = 0, ai = 0;
-            if (row < cw.h) {
+            ClipRead cr{{false, false, 0, 0}, make_uint4(1u, 0u, 0u, 0u)};
+            if (on) {
                 const uint32_t t = cw.tile(row);
                 d8 = (int8_t)b.tile_delta[t];
                 f0 = b.tile_fill_off[t];
                 f1 = b.tile_fill_off[t + 1];
-                if (f1 > f0) ai = b.tile_alpha_idx[t];
+                ai = b.tile_alpha_idx[t];
+                if (CLIPS) cr = clip_read(b, cw, ci, row);
             }
-            uint32_t incl = (uint32_t)d8;
+            uint32_t incl;
+            if (SUB == 16) {
+                incl = row16_inclusive_sum((uint32_t)d8);
+            } else {
+                incl = (uint32_t)d8;
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t o = shfl_up_val(incl, d);
-                if (lane >= d) incl += o;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t o = shfl_up_val(incl, d);
+                    if (lane >= d) incl += o;
+                }
             }
-            if (row < cw.h)
-                propagate_tile<CLIPS>(b, cw, ci, row, (int8_t)(running + (int32_t)(incl - (uint32_t)d8)), f0, f1, ai);
-            running += (int32_t)__builtin_amdgcn_readlane(incl, 63);
+            if (on) propagate_tile<CLIPS>(b, cw, cr, row, (int8_t)(running + (int32_t)(incl - (uint32_t)d8)), f0, f1, ai);
+            running += (int32_t)__shfl(incl, g * SUB + SUB - 1, 64);
         }
-        if (lane == 0) b.col_bd[c] = running;
+        if (has && li == 0) b.col_bd[c] = running;
     }
 }
 
@@ -1693,23 +1730,29 @@ hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s) {
+// rows: columns of more tiles are walked by several lanes each (1..64)
+template <bool CLIPS>
+static void propagate_phase(const BatchDev &b, uint32_t block, uint32_t rows, uint32_t phase, hipStream_t s) {
+    hipLaunchKernelGGL(k_propagate<CLIPS>, dim3(blocks_for(b.n_cols, block)), dim3(block), 0, s, b, phase, rows);
+    // the listed columns' waves: enough to cover the device, striding over lists of unknown length
+    if (rows < (uint32_t)TALL_ROWS)
+        hipLaunchKernelGGL((k_propagate_rows<CLIPS, 16>), dim3(std::min<uint32_t>(blocks_for(b.n_cols, 16), 512)),
+                           dim3(256), 0, s, b, phase);
+    hipLaunchKernelGGL((k_propagate_rows<CLIPS, 64>), dim3(std::min<uint32_t>(blocks_for(b.n_cols, 4), 512)),
+                       dim3(256), 0, s, b, phase);
+}
+
+hipError_t launch_propagate(const BatchDev &b, uint32_t block, uint32_t rows, hipStream_t s) {
     if (b.n_cols == 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
-    const dim3 g(blocks_for(b.n_cols, block));
-    // the tall columns' waves: enough to cover the device, striding over a list of unknown length
-    const dim3 gt(std::min<uint32_t>(blocks_for(b.n_cols, 4), 512));
+    if (rows < 1 || rows > (uint32_t)TALL_ROWS) return hipErrorInvalidValue;
     if (!b.has_clips) {
-        hipLaunchKernelGGL(k_propagate<false>, g, dim3(block), 0, s, b, 0u);
-        hipLaunchKernelGGL(k_propagate_tall<false>, gt, dim3(256), 0, s, b, 0u);
+        propagate_phase<false>(b, block, rows, 0u, s);
     } else {
         // clip levels from the outermost in, then the draw paths
-        for (uint32_t level = 1; level <= std::min(b.clip_levels, MAX_CLIP_LEVELS); level++) {
-            hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, level);
-            hipLaunchKernelGGL(k_propagate_tall<true>, gt, dim3(256), 0, s, b, level);
-        }
-        hipLaunchKernelGGL(k_propagate<true>, g, dim3(block), 0, s, b, MAX_CLIP_LEVELS + 1);
-        hipLaunchKernelGGL(k_propagate_tall<true>, gt, dim3(256), 0, s, b, MAX_CLIP_LEVELS + 1);
+        for (uint32_t level = 1; level <= std::min(b.clip_levels, MAX_CLIP_LEVELS); level++)
+            propagate_phase<true>(b, block, rows, level, s);
+        propagate_phase<true>(b, block, rows, MAX_CLIP_LEVELS + 1, s);
     }
     return hipGetLastError();
 }

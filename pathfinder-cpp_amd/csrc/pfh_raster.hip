@@ -35,14 +35,6 @@ constexpr int FILL_GROUP_WORDS = 16 * FILL_COL_WORDS + 8;   // pad between tiles
 
 
 
-// Inclusive prefix sum within each 16-lane row of the wave (DPP row shifts: no LDS).
-__device__ __forceinline__ uint32_t row16_inclusive_sum(uint32_t v) {
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);  // row_shr:1
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);  // row_shr:2
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);  // row_shr:4
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);  // row_shr:8
-    return v;
-}
 
 
 // Adds one fill's contribution to pixel column `col` of a tile's difference array

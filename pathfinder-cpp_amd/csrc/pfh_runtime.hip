@@ -31,7 +31,7 @@ hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s);
 hipError_t scan_counted(uint32_t *inout, const uint32_t *n_ptr, uint32_t cap, uint32_t *counters, int counter,
                         uint32_t *tmp, hipStream_t s);
 hipError_t scan_total(uint32_t *inout, uint32_t n, uint32_t *counters, int counter, uint32_t *tmp, hipStream_t s);
-hipError_t launch_propagate(const BatchDev &b, uint32_t block, hipStream_t s);
+hipError_t launch_propagate(const BatchDev &b, uint32_t block, uint32_t rows, hipStream_t s);
 hipError_t launch_entries(const BatchDev &b, bool write, hipStream_t s);
 hipError_t launch_bin_items(const BatchDev &b, hipStream_t s);
 hipError_t launch_fb_bounds(const uint32_t *sorted_keys, const uint32_t *n_ptr, uint32_t cap, uint32_t n_fb,
@@ -191,10 +191,10 @@ struct GraphKey {
     std::vector<int32_t> ids;
     uint32_t flags;
     uint32_t cap_f, cap_a, cap_e, cap_l;
-    uint32_t sched[6];  // the launch schedule the graph was captured with
+    uint32_t sched[7];  // the launch schedule the graph was captured with
     bool operator<(const GraphKey &o) const {
         if (ids != o.ids) return ids < o.ids;
-        for (int i = 0; i < 6; i++)
+        for (int i = 0; i < 7; i++)
             if (sched[i] != o.sched[i]) return sched[i] < o.sched[i];
         if (flags != o.flags) return flags < o.flags;
         if (cap_f != o.cap_f) return cap_f < o.cap_f;
@@ -1274,7 +1274,7 @@ static void stage_emit(Workspace &w, const BatchDev &b, const Plan &pl) {
     uint32_t *tmp = w.scan_tmp.as<uint32_t>();
     if (b.dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, b.cap_lines, b.counters, -1, tmp, s));
     HIP_TRY(launch_bin(b, true, s));
-    HIP_TRY(launch_propagate(b, pl.sched.column_block, s));
+    HIP_TRY(launch_propagate(b, pl.sched.column_block, pl.sched.column_rows, s));
     HIP_TRY(launch_entries(b, false, s));
     HIP_TRY(scan_entries(b.path_ent, b.n_paths, b.counters, w.scan_tmp.ptr, s));
 }
@@ -1381,7 +1381,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     if (dump) HIP_TRY(scan_counted(b.line_cnt, b.counters + C_LINES, nl, b.counters, -1, tmp(), st));
     HIP_TRY(launch_bin(b, true, st));
     HIP_TRY(hipEventRecord(ev[3], st));
-    HIP_TRY(launch_propagate(b, pl.sched.column_block, st));
+    HIP_TRY(launch_propagate(b, pl.sched.column_block, pl.sched.column_rows, st));
     HIP_TRY(hipEventRecord(ev[4], st));
     HIP_TRY(launch_entries(b, false, st));
     HIP_TRY(scan_entries(b.path_ent, b.n_paths, b.counters, ctx->w->scan_tmp.ptr, st));
@@ -1507,7 +1507,7 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
                      pl.cap_e,
                      pl.cap_l,
                      {pl.sched.fill_grid, pl.sched.composite_block, pl.sched.column_block, pl.sched.xcd_order,
-                      pl.sched.flatten_segs, pl.sched.flatten_take}};
+                      pl.sched.flatten_segs, pl.sched.flatten_take, pl.sched.column_rows}};
         if (w.graphs_generation != g_alloc_generation) {
             // some buffer moved since these graphs were captured: never replay them
             w.drop_graphs();
@@ -2335,7 +2335,8 @@ static bool valid_schedule(const pfh_schedule &s) {
     auto blk = [](int32_t v) { return v == 64 || v == 128 || v == 256; };
     return s.fill_grid >= 1 && s.fill_grid <= (1 << 20) && blk(s.composite_block) && blk(s.column_block) &&
            s.xcd_order >= 0 && s.xcd_order <= 3 && s.flatten_segs >= 0 && s.flatten_segs <= (1 << 16) &&
-           s.flatten_segs % 64 == 0 && s.flatten_take >= 0 && s.flatten_take <= 64;
+           s.flatten_segs % 64 == 0 && s.flatten_take >= 0 && s.flatten_take <= 64 && s.column_rows >= 0 &&
+           s.column_rows <= 64;
 }
 
 int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out) {
@@ -2346,6 +2347,7 @@ int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out) {
     out->xcd_order = (int32_t)ctx->sched.xcd_order;
     out->flatten_segs = (int32_t)ctx->sched.flatten_segs;
     out->flatten_take = (int32_t)ctx->sched.flatten_take;
+    out->column_rows = (int32_t)ctx->sched.column_rows;
     return PFH_OK;
 }
 
@@ -2359,6 +2361,7 @@ int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in) {
         ctx->sched.xcd_order = (uint32_t)in->xcd_order;
         ctx->sched.flatten_segs = (uint32_t)in->flatten_segs;
         ctx->sched.flatten_take = in->flatten_take ? (uint32_t)in->flatten_take : 1u;
+        ctx->sched.column_rows = in->column_rows ? (uint32_t)in->column_rows : 64u;
     });
 }
 
@@ -2385,9 +2388,10 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         Schedule best = ctx->sched;
         double best_ms = measure();
         if (ms_start) *ms_start = best_ms;
-        const std::vector<uint32_t> cand[6] = {{1024, 2048, 4096, 8192, 16384, 32768, 65536}, {64, 128, 256}, {64, 128, 256},
-                                              {0, 1, 2, 3}, {0, 64, 128, 256, 512, 1024, 2048}, {1, 4, 8, 16, 32}};
-        for (int param = 0; param < 6; param++) {
+        const std::vector<uint32_t> cand[7] = {{1024, 2048, 4096, 8192, 16384, 32768, 65536}, {64, 128, 256}, {64, 128, 256},
+                                              {0, 1, 2, 3}, {0, 64, 128, 256, 512, 1024, 2048}, {1, 4, 8, 16, 32},
+                                              {8, 16, 32, 64}};
+        for (int param = 0; param < 7; param++) {
             for (uint32_t v : cand[param]) {
                 Schedule trial = best;
                 uint32_t *field = param == 0 ? &trial.fill_grid
@@ -2395,7 +2399,8 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
                                   : param == 2 ? &trial.column_block
                                   : param == 3 ? &trial.xcd_order
                                   : param == 4 ? &trial.flatten_segs
-                                               : &trial.flatten_take;
+                                  : param == 5 ? &trial.flatten_take
+                                               : &trial.column_rows;
                 if (*field == v) continue;
                 *field = v;
                 ctx->sched = trial;
@@ -2413,6 +2418,7 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         out->xcd_order = (int32_t)best.xcd_order;
         out->flatten_segs = (int32_t)best.flatten_segs;
         out->flatten_take = (int32_t)best.flatten_take;
+        out->column_rows = (int32_t)best.column_rows;
         if (ms_tuned) *ms_tuned = best_ms;
     });
 }
