@@ -213,6 +213,8 @@ struct BatchDev {
     uint32_t *tall_cols;          // [W] columns of the current propagate phase walked by several lanes:
                                   // the C_MID list from the front, the C_TALL list from the back
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
+    uint64_t *curve_mask;         // [S / 64 + 1] bit i of word j: segment 64 j + i is a curve (to be
+                                  // subdivided), written by the flatten count pass
     // [L] flattened chords as a point stream, in emission order: per chain of chords a NaN
     // separator and the start point, then each chord's end point; chord i of a path's range
     // [c0, c1) runs from point i to point i + 1 (a pair with a NaN end emits nothing)

@@ -276,13 +276,63 @@ struct Walker {
 // cache write back whole lines: several times the chord output in HBM writes.
 constexpr int FLAT_LDS_LEVELS = 4;
 
-// Lanes are scheduled dynamically within a wave: the wave owns a contiguous range of
-// segments, each iteration every lane takes one depth-first step of its segment, and a lane
-// whose segment is done takes the next one of the range (a wave-uniform counter, ranks by
-// ballot).  So a wave no longer runs as long as its deepest curve while the lanes holding
-// lines and shallow curves idle.  Segment data is loaded 64 segments at a time, one per lane
-// (one dependent load chain per 64 segments, coalesced), and handed to the lane that takes
-// the segment by lane shuffles.
+// Lines need no subdivision: k_flatten_lines, a thread per segment without the LDS stack (so
+// with many more waves per CU to hide the loads), finishes every line and every segment that
+// emits nothing, and its count pass marks the curves, one bit per segment in a 64-bit word per
+// 64 segments, for k_flatten.
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_flatten_lines(BatchDev b) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t g0 = blockIdx.x * 256 + (threadIdx.x & ~63u);  // the wave's first segment
+    if (g0 >= b.n_segs) return;  // whole wave
+    const uint32_t g = g0 + lane;
+    const bool in = g < b.n_segs;
+    int k = find_scene_wave(b.seg_bases, b.n_scenes, g0);
+    uint32_t kind = 0;
+    bool brk = false;
+    float2 p0 = {}, p1 = {};
+    if (in) {
+        while (g >= b.seg_bases[k + 1]) k++;
+        const SceneDev &sc = b.scenes[k];
+        const uint2 sg = sc.segs[g - sc.seg_base];
+        const uint32_t sk = sg.y & SEG_KIND_MASK;
+        if (sk == SEG_LINE) {
+            p0 = sc.pts[sg.x];
+            p1 = sc.pts[sg.x + 1];
+            // a non-finite point: process_line_segment drops the chord (tiler.cpp:133-136)
+            if (isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y)) {
+                kind = SEG_LINE;
+                brk = (sg.y & SEG_CHAIN_BREAK) != 0;
+            }
+        } else if (sk != 0) {
+            kind = sk;  // a curve: its points are checked where it is subdivided
+        }
+    }
+    const bool curve = kind == SEG_QUAD || kind == SEG_CUBIC;
+    if (!EMIT) {
+        // stream points: a chord's end (+ the chain's separator and start point)
+        if (in && !curve) b.seg_cnt[g] = kind == SEG_LINE ? 1u + (brk ? 2u : 0u) : 0u;
+        const uint64_t m = __ballot(curve);
+        if (lane == 0) b.curve_mask[g0 >> 6] = m;
+    } else if (kind == SEG_LINE) {
+        uint32_t o = b.seg_cnt[g];
+        if (brk) {  // the chain's separator (a NaN point: its chords emit nothing) and start
+            if (o < b.cap_lines) b.lines[o] = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+            if (o + 1 < b.cap_lines) b.lines[o + 1] = p0;
+            o += 2;
+        }
+        if (o < b.cap_lines) b.lines[o] = p1;
+    }
+}
+
+// The curves k_flatten_lines marked.  Lanes are scheduled dynamically within a wave: the wave
+// owns a contiguous range of segments, each iteration every lane takes one depth-first step of
+// its curve, and a lane whose curve is done takes the next curve of the range (a wave-uniform
+// counter, ranks by ballot).  So a wave no longer runs as long as its deepest curve while the
+// lanes holding shallow curves idle.  The curves among the next 64 segments are loaded one per
+// lane (one dependent load chain per 64 segments, coalesced), packed into the lowest lanes in
+// segment order, and handed to the lane that takes each by lane shuffles; 64 segments without a
+// curve cost one scalar load of their mask word.
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_wave, uint32_t take) {
     __shared__ float fs[FLAT_LDS_LEVELS][6][256];
@@ -296,10 +346,10 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
     int sdepth[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS];
     // scene of the next segment to load: segment indices only grow, so it is advanced, not searched
     int k = find_scene_wave(b.seg_bases, b.n_scenes, (uint32_t)w0);
-    uint32_t next = (uint32_t)w0;           // first segment not yet handed out (wave-uniform)
-    uint32_t batch = (uint32_t)w0, batch_end = (uint32_t)w0;  // staged segments [batch, batch_end)
-    float2 q0 = {}, q1 = {}, q2 = {}, q3 = {};  // staged segment batch + lane: points
-    uint32_t qkind = 0, qbase = 0;              // its kind (0: emits nothing) and chord offset
+    uint32_t word = (uint32_t)w0;             // first of the next 64 segments to look at (wave-uniform)
+    uint32_t staged = 0, taken = 0;           // curves staged in lanes [0, staged); those handed out
+    float2 q0 = {}, q1 = {}, q2 = {}, q3 = {};  // staged curve of this lane: points
+    uint32_t qkind = 0, qbase = 0, qseg = 0;    // its kind (0: emits nothing), chord offset, segment
     Seg4 s;
     uint32_t g = 0, kind = 0, n = 0, base = 0, guard = 0;
     bool brk = false;  // this segment starts a chord chain (a separator and its start point first)
@@ -307,83 +357,75 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
     bool active = false;
     while (true) {
         const uint64_t want = __ballot(!active);
-        // a hand-over (shuffles of the staged segments to the idle lanes) once `take` lanes are
-        // idle, or as many as there are segments left: fewer hand-overs, each feeding more lanes
-        const uint32_t idle = (uint32_t)__popcll(want);
-        if (want && next < w1 && idle >= min(take, w1 - next)) {  // wave-uniform
-            if (next >= batch_end) {
-                // stage the next 64 segments, one per lane
-                batch = next;
-                batch_end = min(next + 64, w1);
-                const uint32_t gl = batch + lane;
+        if (want && taken == staged && word < w1) {  // wave-uniform
+            uint64_t m = 0;
+            while (word < w1) {
+                m = b.curve_mask[word >> 6];
+                if (m) break;
+                word += 64;
+            }
+            if (m) {
+                const uint32_t gl = word + lane;
+                float2 p0 = {}, p1 = {}, p2 = {}, p3 = {};
+                uint32_t kd = 0, bs = 0;
                 int kl = k;
-                qkind = 0;
-                if (gl < batch_end) {
+                if ((m >> lane) & 1ull) {
                     while (gl >= b.seg_bases[kl + 1]) kl++;
                     const SceneDev &sc = b.scenes[kl];
-                    const uint32_t s_local = gl - sc.seg_base;
-                    const uint2 sg = sc.segs[s_local];
-                    qbase = EMIT ? b.seg_cnt[gl] : 0u;
-                    // kind 0: a segment of a path without tile columns, from which nothing can land
-                    // (with columns but no rows, above-rect backdrops still do: those are kept)
-                    const uint32_t sk = sg.y & SEG_KIND_MASK;
-                    if (sk != 0) {
-                        const float2 *P = sc.pts + sg.x;
-                        q0 = P[0];
-                        if (sk == SEG_LINE) {
-                            q3 = P[1];
-                        } else if (sk == SEG_QUAD) {
-                            q1 = P[1];
-                            q2 = P[1];
-                            q3 = P[2];
-                        } else {
-                            q1 = P[1];
-                            q2 = P[2];
-                            q3 = P[3];
-                        }
-                        // A non-finite point makes every flattened piece non-finite, and
-                        // process_line_segment drops those (tiler.cpp:133-136): nothing from it.
-                        bool finite = isfinite(q0.x) && isfinite(q0.y) && isfinite(q3.x) && isfinite(q3.y);
-                        if (sk != SEG_LINE) finite = finite && isfinite(q1.x) && isfinite(q1.y);
-                        if (sk == SEG_CUBIC) finite = finite && isfinite(q2.x) && isfinite(q2.y);
-                        if (finite) qkind = sg.y;  // kind and chain-break flag
+                    const uint2 sg = sc.segs[gl - sc.seg_base];
+                    bs = EMIT ? b.seg_cnt[gl] : 0u;
+                    const float2 *P = sc.pts + sg.x;
+                    p0 = P[0];
+                    p1 = P[1];
+                    if ((sg.y & SEG_KIND_MASK) == SEG_QUAD) {
+                        p2 = p1;
+                        p3 = P[2];
+                    } else {
+                        p2 = P[2];
+                        p3 = P[3];
                     }
+                    // A non-finite point makes every flattened piece non-finite, and
+                    // process_line_segment drops those (tiler.cpp:133-136): nothing from it.
+                    const bool finite = isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y) &&
+                                        isfinite(p2.x) && isfinite(p2.y) && isfinite(p3.x) && isfinite(p3.y);
+                    if (finite) kd = sg.y;  // kind and chain-break flag
                 }
-                k = __shfl(kl, (int)(batch_end - 1 - batch));
-                // A fresh batch of lines (or segments that emit nothing) while every lane is idle:
-                // each lane finishes its own staged segment at once, without the hand-over.
-                const uint32_t sk = qkind & SEG_KIND_MASK;
-                if (want == ~0ull && __ballot(batch + lane < batch_end && sk != 0 && sk != SEG_LINE) == 0) {
-                    if (batch + lane < batch_end) {
-                        const bool line_brk = sk != 0 && (qkind & SEG_CHAIN_BREAK) != 0;
-                        if (EMIT) {
-                            uint32_t o = qbase;
-                            if (line_brk) {
-                                if (o < b.cap_lines) b.lines[o] = make_float2(__builtin_nanf(""), __builtin_nanf(""));
-                                if (o + 1 < b.cap_lines) b.lines[o + 1] = q0;
-                                o += 2;
-                            }
-                            if (sk == SEG_LINE && o < b.cap_lines) b.lines[o] = q3;
-                        } else {
-                            b.seg_cnt[batch + lane] = (sk == SEG_LINE ? 1u : 0u) + (line_brk ? 2u : 0u);
-                        }
-                    }
-                    next = batch_end;
-                    continue;
-                }
+                k = __builtin_amdgcn_readlane(kl, 63 - __clzll(m));
+                // pack: a curve's lane sends its data to lane (curves below it), the other lanes to
+                // the lanes above the curves, in order (a permutation of the lanes)
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                staged = (uint32_t)__popcll(m);
+                const int to = (int)(((m >> lane) & 1ull) ? below : staged + (lane - below)) * 4;
+                auto send = [&](uint32_t v) { return (uint32_t)__builtin_amdgcn_ds_permute(to, (int)v); };
+                auto sendf = [&](float v) { return __uint_as_float(send(__float_as_uint(v))); };
+                q0 = make_float2(sendf(p0.x), sendf(p0.y));
+                q1 = make_float2(sendf(p1.x), sendf(p1.y));
+                q2 = make_float2(sendf(p2.x), sendf(p2.y));
+                q3 = make_float2(sendf(p3.x), sendf(p3.y));
+                qkind = send(kd);
+                qbase = send(bs);
+                qseg = send(gl);
+                taken = 0;
+                word += 64;
             }
-            // idle lanes take the next staged segments, in lane order
-            const uint32_t cand = next + (uint32_t)__popcll(want & lt_mask);
-            next = min(next + (uint32_t)__popcll(want), batch_end);
-            const int src = (int)((cand - batch) & 63);
+        }
+        const uint32_t idle = (uint32_t)__popcll(want);
+        // a hand-over (shuffles of the staged curves to the idle lanes) once `take` lanes are idle,
+        // or as many as there are curves staged: fewer hand-overs, each feeding more lanes
+        if (want && taken < staged && idle >= min(take, staged - taken)) {  // wave-uniform
+            // idle lanes take the next staged curves, in lane order
+            const uint32_t cand = taken + (uint32_t)__popcll(want & lt_mask);
+            taken = min(taken + idle, staged);
+            const int src = (int)(cand & 63);
             const float2 t0 = make_float2(__shfl(q0.x, src), __shfl(q0.y, src));
             const float2 t1 = make_float2(__shfl(q1.x, src), __shfl(q1.y, src));
             const float2 t2 = make_float2(__shfl(q2.x, src), __shfl(q2.y, src));
             const float2 t3 = make_float2(__shfl(q3.x, src), __shfl(q3.y, src));
             const uint32_t tk = __shfl(qkind, src);
             const uint32_t tb = EMIT ? __shfl(qbase, src) : 0u;
-            if (!active && cand < batch_end) {
-                g = cand;
+            const uint32_t tg = __shfl(qseg, src);
+            if (!active && cand < staged) {
+                g = tg;
                 s = Seg4{t0, t1, t2, t3};
                 kind = tk & SEG_KIND_MASK;
                 brk = kind != 0 && (tk & SEG_CHAIN_BREAK) != 0;
@@ -399,7 +441,8 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
                 active = true;
             }
         }
-        if (__ballot(active) == 0) break;  // every lane idle: the range is done
+        // every lane idle after a hand-over: nothing is staged and the range is done
+        if (__ballot(active) == 0) break;
         if (!active) continue;
         bool done = false;
         if (kind == 0) {
@@ -1703,9 +1746,15 @@ hipError_t scan_paths(const BatchDev &b, void *tmp, hipStream_t s) {
 hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, uint32_t take, hipStream_t s) {
     if (b.n_segs == 0) return hipSuccess;
     if (segs_per_wave < 64 || segs_per_wave % 64 || take < 1 || take > 64) return hipErrorInvalidValue;
+    const dim3 lines(blocks_for(b.n_segs, 256));
     const dim3 grid(blocks_for(blocks_for(b.n_segs, segs_per_wave), 4));
-    if (emit) hipLaunchKernelGGL(k_flatten<true>, grid, dim3(256), 0, s, b, segs_per_wave, take);
-    else hipLaunchKernelGGL(k_flatten<false>, grid, dim3(256), 0, s, b, segs_per_wave, take);
+    if (emit) {
+        hipLaunchKernelGGL(k_flatten_lines<true>, lines, dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_flatten<true>, grid, dim3(256), 0, s, b, segs_per_wave, take);
+    } else {
+        hipLaunchKernelGGL(k_flatten_lines<false>, lines, dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_flatten<false>, grid, dim3(256), 0, s, b, segs_per_wave, take);
+    }
     return hipGetLastError();
 }
 

@@ -221,7 +221,7 @@ struct Workspace {
     // around the fill and composite kernels of a batched launch; recorded as event nodes
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
-    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, ent_quarter, path_tot, bin_items, col_item, col_bd, tall_cols,
+    DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, ent_quarter, path_tot, bin_items, col_item, col_bd, tall_cols, curve_mask,
         seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, alpha_vis, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp, tile_clip_ai, clip_jobs, clip_rec, clip_seen;
     uint32_t *h_counters = nullptr;  // pinned
@@ -298,7 +298,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &path_ent, &ent_quarter, &path_tot, &bin_items, &col_item, &col_bd, &tall_cols, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
+                &path_ent, &ent_quarter, &path_tot, &bin_items, &col_item, &col_bd, &tall_cols, &curve_mask, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &entry_meta, &counters, &scan_tmp,
                 &sort_tmp, &tile_clip_ai, &clip_jobs, &clip_rec, &clip_seen};
     }
@@ -1027,6 +1027,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.col_bd.ensure(4 * pl.W + 4);
     w.tall_cols.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
+    w.curve_mask.ensure(8 * (pl.S / 64) + 16);
     w.lines.ensure(8ull * pl.cap_l);
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
     w.zbuf.ensure(4 * pl.FB + 4);
@@ -1222,6 +1223,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.col_bd = w.col_bd.as<int32_t>();
     b.tall_cols = w.tall_cols.as<uint32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
+    b.curve_mask = w.curve_mask.as<uint64_t>();
     b.lines = w.lines.as<float2>();
     b.line_cnt = w.line_cnt.as<uint32_t>();
     b.fills = w.fills.as<uint2>();
