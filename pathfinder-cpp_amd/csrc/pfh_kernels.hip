@@ -107,7 +107,8 @@ struct Walker {
         uint32_t q2 = (uint32_t)rintf(sse_min(sse_max(v2, 0.0f), 4095.0f));
         uint32_t q3 = (uint32_t)rintf(sse_min(sse_max(v3, 0.0f), 4095.0f));
         if (q0 == q2) return;
-        uint32_t j = (uint32_t)((ty - tt) * tw + (tx - tl));  // tile within the path's rect
+        // tile within the path's rect (rect sides are below 2^24 tiles: a 24-bit multiply-add)
+        uint32_t j = __umul24((uint32_t)(ty - tt), (uint32_t)tw) + (uint32_t)(tx - tl);
         if (!EMIT) {
             // consecutive fills of a chord mostly share a tile: count them locally
             if (j != pend_tile) {
@@ -152,7 +153,7 @@ struct Walker {
             else atomicAdd(&col_bd[cbase + ox], delta);
             return;
         }
-        const uint32_t j = (uint32_t)(oy * tw + ox);
+        const uint32_t j = __umul24((uint32_t)oy, (uint32_t)tw) + (uint32_t)ox;
         if (LDS && PACK) atomicAdd(reinterpret_cast<uint32_t *>(l_delta) + (j >> 1), (uint32_t)delta << (16 * (j & 1)));
         else if (LDS) atomicAdd(&l_delta[j], delta);
         else atomicAdd(&tile_delta[tbase + j], delta);

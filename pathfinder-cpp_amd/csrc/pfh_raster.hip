@@ -110,11 +110,16 @@ __device__ __forceinline__ void accumulate_fill(const FillGeom &g, int col, floa
     const float f1 = n == 1 ? fd : k2 * (2.0f * u0 + 1.0f);          // F(r0 + 1), when n >= 1
     const int32_t full = round_fix(ws), v0 = round_fix(f0), v1 = round_fix(f1), vd = round_fix(fd);
     const int32_t dm = round_fix(2.0f * k2);  // F(r) - F(r - 1) for r0 + 2 <= r <= r1 - 1
-    int32_t *cd = tile_diff + __umul24((uint32_t)col, (uint32_t)FILL_COL_WORDS);
+    // 0 <= col <= 15 here: the column's word offset by a full-rate 24-bit multiply (left to
+    // itself the compiler forms it with a quarter-rate 32-bit one)
+    uint32_t col_words;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(col_words) : "v"((uint32_t)col), "v"((uint32_t)FILL_COL_WORDS));
+    int32_t *cd = tile_diff + col_words;
     atomicAdd(cd + r0, v0);
     atomicAdd(cd + r0 + 1, (n == 0 ? full : v1) - v0);
     for (int r = r0 + 2; r < r1; r++) atomicAdd(cd + r, dm);
-    if (n >= 2) atomicAdd(cd + r1, vd - (v1 + (n - 2) * dm));
+    // n >= 2 means ymax - ymin > 1, so |dm| < 2^20: a 24-bit multiply is exact
+    if (n >= 2) atomicAdd(cd + r1, vd - (v1 + __mul24(n - 2, dm)));
     if (n >= 1) atomicAdd(cd + r1 + 1, full - vd);  // n == 1: v1 == vd
 }
 

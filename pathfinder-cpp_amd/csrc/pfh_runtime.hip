@@ -848,6 +848,8 @@ static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d) {
         r.w = (int32_t)std::ceil(bi.b * (1.0f / 16.0f));
         // DenseTileMap w x h tiles and BuiltPathData::backdrops of width w (built_path.cpp:25-27)
         int w = std::max(0, r.z - r.x), h = std::max(0, r.w - r.y);
+        // the tile walk indexes a rect with 24-bit multiplies
+        if (w >= (1 << 24) || h >= (1 << 24)) throw PfhError{PFH_E_LIMIT, "a path's tile rect has a side of 2^24 tiles or more"};
         s->path_rect[p] = r;
         tiles += (uint64_t)w * h;
         for (int x = 0; x < w; x++) col_path.push_back((uint32_t)p);
