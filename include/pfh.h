@@ -229,13 +229,14 @@ typedef struct pfh_schedule {
     int32_t flatten_take;     /* idle lanes a flatten wave waits for before handing them new segments
                                  (1..64; 0: 1) */
     int32_t column_rows;      /* path columns of more tiles than this are propagated by 16 or 64 lanes
-                                 each instead of one (1..64; 0: 64) */
+                                 each instead of one (1..64; 0: every column by one lane) */
 } pfh_schedule;
 int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out);
 int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in);
 /* Coordinate descent from the current schedule: each parameter's candidates timed as the median
  * device time of `reps` replays of the batched launch of these scenes; the context keeps the
- * fastest.  ms_start / ms_tuned (may be NULL): time of the starting and of the chosen schedule. */
+ * fastest (a change has to gain 1 %).  ms_start / ms_tuned (may be NULL): time of the starting and
+ * of the chosen schedule. */
 int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t flags, int32_t reps, pfh_schedule *out,
                  double *ms_start, double *ms_tuned);
 

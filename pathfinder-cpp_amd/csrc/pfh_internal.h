@@ -53,7 +53,8 @@ struct Schedule {
     uint32_t xcd_order = 0;           // bit 0 fill, bit 1 composite: workgroups grouped per XCD
     uint32_t flatten_segs = 0;        // segments per k_flatten wave (multiple of 64), 0: from the count
     uint32_t flatten_take = 1;        // idle k_flatten lanes a wave waits for before a hand-over (1..64)
-    uint32_t column_rows = 64;        // path columns of more tiles are walked by 16 or 64 lanes (1..64)
+    uint32_t column_rows = 0;         // path columns of more tiles are walked by 16 or 64 lanes (1..64;
+                                      // 0: every column by one lane)
 };
 
 // Inclusive prefix sum within each 16-lane row of the wave (DPP row shifts: no LDS).
@@ -122,7 +123,7 @@ enum Counter {
                        // slots (per block, by block index) so that no single address takes every add
     C_CLIP_DUPS = 8 + 64,  // draw entries whose clip mask row an earlier entry already listed
     C_TALL = 8 + 64 + 1,   // [C_TALL, C_TALL + 6): per propagate phase, columns of more than
-                           // TALL_ROWS tiles listed for a wave each (k_propagate_rows<64>)
+                           // TALL_ROWS tiles (and Schedule::column_rows) listed for a wave each
     C_MID = 8 + 64 + 1 + 6,  // [C_MID, C_MID + 6): per phase, columns of more than
                              // Schedule::column_rows tiles and at most TALL_ROWS (k_propagate_rows<16>)
     C_COUNT = 8 + 64 + 1 + 12

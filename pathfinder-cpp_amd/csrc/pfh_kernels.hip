@@ -1269,14 +1269,14 @@ __device__ __forceinline__ void propagate_tile(const BatchDev &b, const ColumnWa
 // A thread walks its column: rows of neighbouring columns are neighbours in memory, so the
 // wave's loads coalesce.  A walk is a chain of dependent loads per group of rows, and a launch
 // lasts as long as its longest walk: a column of more than `rows` tiles (a schedule choice) is
-// listed instead for k_propagate_rows, which spreads its rows over 16 or 64 lanes.
+// listed instead for k_propagate_rows, which spreads its rows over 16 or 64 lanes (rows 0: none).
 template <bool CLIPS>
 __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase, uint32_t rows) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.n_cols) return;
     const ColumnWalk cw(b, c);
     if (CLIPS && cw.level != (phase > MAX_CLIP_LEVELS ? 0u : phase)) return;
-    if (cw.h > (int)rows) {
+    if (rows != 0 && (uint32_t)cw.h > rows) {
         // at most one entry per column and phase, so the two lists together hold at most n_cols
         if (cw.h > TALL_ROWS) b.tall_cols[b.n_cols - 1 - atomicAdd(&b.counters[C_TALL + phase], 1u)] = c;
         else b.tall_cols[atomicAdd(&b.counters[C_MID + phase], 1u)] = c;
@@ -1780,22 +1780,23 @@ hipError_t launch_bin(const BatchDev &b, bool emit, hipStream_t s) {
     return hipGetLastError();
 }
 
-// rows: columns of more tiles are walked by several lanes each (1..64)
+// rows: columns of more tiles are walked by several lanes each (1..64; 0: none)
 template <bool CLIPS>
 static void propagate_phase(const BatchDev &b, uint32_t block, uint32_t rows, uint32_t phase, hipStream_t s) {
     hipLaunchKernelGGL(k_propagate<CLIPS>, dim3(blocks_for(b.n_cols, block)), dim3(block), 0, s, b, phase, rows);
+    if (rows == 0) return;
     // the listed columns' waves: enough to cover the device, striding over lists of unknown length
     if (rows < (uint32_t)TALL_ROWS)
-        hipLaunchKernelGGL((k_propagate_rows<CLIPS, 16>), dim3(std::min<uint32_t>(blocks_for(b.n_cols, 16), 512)),
+        hipLaunchKernelGGL((k_propagate_rows<CLIPS, 16>), dim3(std::min<uint32_t>(blocks_for(b.n_cols, 16), 2048)),
                            dim3(256), 0, s, b, phase);
-    hipLaunchKernelGGL((k_propagate_rows<CLIPS, 64>), dim3(std::min<uint32_t>(blocks_for(b.n_cols, 4), 512)),
+    hipLaunchKernelGGL((k_propagate_rows<CLIPS, 64>), dim3(std::min<uint32_t>(blocks_for(b.n_cols, 4), 2048)),
                        dim3(256), 0, s, b, phase);
 }
 
 hipError_t launch_propagate(const BatchDev &b, uint32_t block, uint32_t rows, hipStream_t s) {
     if (b.n_cols == 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
-    if (rows < 1 || rows > (uint32_t)TALL_ROWS) return hipErrorInvalidValue;
+    if (rows > (uint32_t)TALL_ROWS) return hipErrorInvalidValue;
     if (!b.has_clips) {
         propagate_phase<false>(b, block, rows, 0u, s);
     } else {

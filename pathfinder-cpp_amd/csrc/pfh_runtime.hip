@@ -2365,13 +2365,14 @@ int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in) {
         ctx->sched.xcd_order = (uint32_t)in->xcd_order;
         ctx->sched.flatten_segs = (uint32_t)in->flatten_segs;
         ctx->sched.flatten_take = in->flatten_take ? (uint32_t)in->flatten_take : 1u;
-        ctx->sched.column_rows = in->column_rows ? (uint32_t)in->column_rows : 64u;
+        ctx->sched.column_rows = (uint32_t)in->column_rows;
     });
 }
 
 // Coordinate descent over the schedule, one parameter at a time from the current schedule,
 // each candidate timed as the median device time of `reps` graph replays of the batched
-// launch of these scenes (capture excluded).  Keeps the fastest; results never change.
+// launch of these scenes (capture excluded).  Keeps the fastest (a candidate has to be 1 %
+// faster than the best so far); results never change.
 int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t flags, int32_t reps, pfh_schedule *out,
                  double *ms_start, double *ms_tuned) {
     if (!ctx || !scene_ids || n <= 0 || reps < 1 || reps > 100 || !out) return PFH_E_ARG;
@@ -2394,7 +2395,7 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         if (ms_start) *ms_start = best_ms;
         const std::vector<uint32_t> cand[7] = {{1024, 2048, 4096, 8192, 16384, 32768, 65536}, {64, 128, 256}, {64, 128, 256},
                                               {0, 1, 2, 3}, {0, 64, 128, 256, 512, 1024, 2048}, {1, 4, 8, 16, 32},
-                                              {8, 16, 32, 64}};
+                                              {0, 8, 16, 32, 64}};
         for (int param = 0; param < 7; param++) {
             for (uint32_t v : cand[param]) {
                 Schedule trial = best;
@@ -2409,7 +2410,8 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
                 *field = v;
                 ctx->sched = trial;
                 const double ms = measure();
-                if (ms < best_ms) {
+                // a candidate must beat the best by more than replay-to-replay noise (~0.5 %)
+                if (ms < best_ms * 0.99) {
                     best_ms = ms;
                     best = trial;
                 }
