@@ -11,9 +11,11 @@ every scene from outlines resident in HBM to its RGBA8 framebuffer in HBM:
 * plain: one scene at a time, stage by stage, exact sizes read back between stages (the
   reference renderer's structure) -> ``plain_paths_per_s``, ``batched_over_plain``.
 
-The ``roofline`` object covers the slower of the two HBM-bound kernels (k_fill_masks,
-k_composite) in the timed batched steps: HIP events recorded as nodes of the replayed graph
-on the library's stream bracket each kernel; achieved = algorithmic bytes / that time.
+The ``roofline`` object covers the slower of k_fill_masks and k_composite in the timed batched
+steps: HIP events recorded as nodes of the replayed graph on the library's stream bracket each
+kernel; achieved = algorithmic bytes / that time.  For the batch workload it also carries the
+kernel's HBM bytes per launch (``traffic``) and the share of the chip's VALU issue rate it uses
+(``valu_issue_share``), both from the rocprofv3 counter runs summarised in profiles/.
 
 Also reported: ``single_4k`` (one 4096x4096 scene of 10 000 paths, scenegen.single_scene_4k,
 the same two modes and roofline), ``stress`` (scenegen.stress_scene: 4096x4096, 2 000 star
@@ -68,7 +70,23 @@ def composite_bytes(st, pixels, n_paths):
     return 4 * pixels + 16 * st["entries"] + 256 * st["alpha_entries"] + 4 * n_paths
 
 
-def roofline(st, fill_ms, comp_ms, pixels, n_paths):
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def profiled(file, key, kernel):
+    """A per-launch figure of `kernel` on the 256-scene batch from a committed rocprofv3 summary
+    (its `key: kernel=value ...` line); None without it."""
+    try:
+        for line in open(os.path.join(PROFILES, file)):
+            if line.startswith(key + ":"):
+                vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+                return float(vals[kernel]) if kernel in vals else None
+    except OSError:
+        pass
+    return None
+
+
+def roofline(st, fill_ms, comp_ms, pixels, n_paths, traffic_known=False):
     fb, cb = fill_bytes(st), composite_bytes(st, pixels, n_paths)
     fill_gbs = fb / (fill_ms * 1e-3) / 1e9
     comp_gbs = cb / (comp_ms * 1e-3) / 1e9
@@ -76,8 +94,15 @@ def roofline(st, fill_ms, comp_ms, pixels, n_paths):
         kern, nbytes, ms, gbs = "k_composite", cb, comp_ms, comp_gbs
     else:
         kern, nbytes, ms, gbs = "k_fill_masks", fb, fill_ms, fill_gbs
+    # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated) and the share of the chip's VALU
+    # issue rate the kernel uses, both from the batch workload's counter runs
+    traffic = profiled("batch256_hbm_traffic.md", "traffic_bytes_per_launch", kern) if traffic_known else None
+    valu = profiled("batch256_sq_counters.md", "valu_issue_share", kern) if traffic_known else None
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kern,
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": int(traffic) if traffic is not None else None,
+            "valu_issue_share": valu,
+            "counters": "profiles/batch256_hbm_traffic.md, profiles/batch256_sq_counters.md" if traffic_known else None,
+            "kernel": kern,
             "algorithmic_bytes": int(nbytes), "kernel_ms": round(ms, 4),
             "other": {"k_fill_masks_gbs": round(fill_gbs, 1), "k_fill_masks_ms": round(fill_ms, 4),
                       "k_composite_gbs": round(comp_gbs, 1), "k_composite_ms": round(comp_ms, 4)}}
@@ -150,7 +175,7 @@ def serve(ctx, ids, qps, n_requests, seed, slots=3, budget=64 << 30, max_batch=1
     return out
 
 
-def cpu_baseline(scenes, reps=5, n_sample=16, threads_ref=4):
+def cpu_baseline(scenes, reps=5, n_sample=48, threads_ref=4):
     """The reference's hybrid path on the host's cores over a stratified sample of the workload's
     scenes (n_sample scenes at evenly spaced ranks of path count, the largest included), each leg
     the median of `reps` timed runs after one warm-up:
@@ -278,7 +303,7 @@ def main():
         "counts": {k: st[k] for k in ("segments", "lines", "tiles", "fills", "alpha_tiles", "entries",
                                       "alpha_entries", "fills_masked", "alpha_masked", "fb_tiles", "retries",
                                       "host_syncs")},
-        "roofline": roofline(st, fill_ms, comp_ms, pixels, n_paths),
+        "roofline": roofline(st, fill_ms, comp_ms, pixels, n_paths, traffic_known=args.config == "batch"),
         "autotune": tuning,
     }
 
@@ -306,7 +331,9 @@ def main():
                                                         "alpha_entries", "retries")}}
         ctx.release(sid)
         print(f"[bench] stress batched {b_s * 1e3:.3f} ms/step", flush=True)
-        # serving: request rate as a fraction of the batched throughput in scenes/s
+        # serving: request rate as a fraction of the batched throughput in scenes/s, with the
+        # schedule tuned for these scenes (the single and stress legs tuned their own)
+        ctx.set_schedule(**tuning["schedule"])
         capacity = len(ids) / batched_s
         out["serving"] = {"requests": "Poisson arrivals, each a uniformly drawn resident scene of the workload",
                           "capacity_scenes_per_s": round(capacity, 1),

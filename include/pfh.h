@@ -227,7 +227,7 @@ typedef struct pfh_schedule {
     int32_t flatten_segs;     /* segments each flatten wave works through (multiple of 64; 0: from the
                                  segment count) */
     int32_t flatten_take;     /* idle lanes a flatten wave waits for before handing them new segments
-                                 (1..64; 0: 1) */
+                                 (1..64; 0: 8) */
     int32_t column_rows;      /* path columns of more tiles than this are propagated by 16 or 64 lanes
                                  each instead of one (1..64; 0: every column by one lane) */
 } pfh_schedule;

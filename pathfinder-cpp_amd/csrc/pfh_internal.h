@@ -52,7 +52,7 @@ struct Schedule {
     uint32_t column_block = 256;      // threads per workgroup of k_propagate / k_entries
     uint32_t xcd_order = 0;           // bit 0 fill, bit 1 composite: workgroups grouped per XCD
     uint32_t flatten_segs = 0;        // segments per k_flatten wave (multiple of 64), 0: from the count
-    uint32_t flatten_take = 1;        // idle k_flatten lanes a wave waits for before a hand-over (1..64)
+    uint32_t flatten_take = 8;        // idle k_flatten lanes a wave waits for before a hand-over (1..64)
     uint32_t column_rows = 0;         // path columns of more tiles are walked by 16 or 64 lanes (1..64;
                                       // 0: every column by one lane)
 };

@@ -999,11 +999,12 @@ struct Plan {
 };
 
 // Segments per flatten wave: the schedule's choice, or enough for ~4 waves per SIMD (1024 SIMDs),
-// 64 to 2048 (a wave balances its lanes over its whole range).
+// 64 to 1024 (a wave balances its lanes over its whole range; past 1024 too few waves share the
+// curves of a large batch).
 static uint32_t flatten_segs(const Plan &pl) {
     if (pl.sched.flatten_segs) return pl.sched.flatten_segs;
     const uint64_t per = (pl.S + 4096 * 64 - 1) / (4096 * 64);
-    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(64 * per, 64), 2048);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(64 * per, 64), 1024);
 }
 
 static uint32_t clamp_cap(uint64_t v) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(v, 64), 0xF0000000ull); }
@@ -2364,7 +2365,7 @@ int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in) {
         ctx->sched.column_block = (uint32_t)in->column_block;
         ctx->sched.xcd_order = (uint32_t)in->xcd_order;
         ctx->sched.flatten_segs = (uint32_t)in->flatten_segs;
-        ctx->sched.flatten_take = in->flatten_take ? (uint32_t)in->flatten_take : 1u;
+        ctx->sched.flatten_take = in->flatten_take ? (uint32_t)in->flatten_take : 8u;
         ctx->sched.column_rows = (uint32_t)in->column_rows;
     });
 }
