@@ -192,6 +192,8 @@ def cpu_baseline(scenes, reps=5, n_sample=48, threads_ref=4):
     if not oracle.ref_available():
         return None
     cores = max(1, min(16, os.cpu_count() or 1))
+    if len(scenes) == 1 and (scenes[0].n_clip_paths or getattr(scenes[0], "path_stroke", None) is not None):
+        return cpu_baseline_stroked(scenes[0], reps, threads_ref, cores)
     order = np.argsort([s.n_paths for s in scenes], kind="stable")
     pick = sorted(set(int(order[int(round(q * (len(order) - 1)))]) for q in np.linspace(0, 1, n_sample)))
     sample = [scenes[k] for k in pick if not scenes[k].n_clip_paths]
@@ -224,6 +226,41 @@ def cpu_baseline(scenes, reps=5, n_sample=48, threads_ref=4):
                       f"and leg; value = leg iii (reference Tiler on {cores} threads + the hybrid raster half "
                       f"restated on {cores} threads, to RGBA8)",
             "legs": legs, "host_nproc": os.cpu_count()}
+
+
+def cpu_baseline_stroked(scene, reps, threads_ref, cores):
+    """The stress scene on the host: its strokes expanded (the stroker restated in C++, bit-identical to
+    the reference's OutlineStrokeToFill, one thread), then the reference's own Tiler over every path of
+    the stroked scene, clip paths included, on threads_ref and on all threads.  The tiler's clip
+    resolution step and the raster half are not timed (the harness tiles paths without resolving
+    clips, and the CPU raster restatement does not resolve them): value = stroke + tiling on all
+    threads, a lower bound on the CPU's time."""
+    import oracle
+
+    def median_time(fn):
+        fn()
+        return float(np.median([fn() for _ in range(reps)]))
+
+    def stroke_once():
+        t0 = time.perf_counter()
+        oracle.apply_strokes(scene)
+        return time.perf_counter() - t0
+
+    t_s = median_time(stroke_once)
+    stroked = oracle.apply_strokes(scene)
+    t_i = median_time(lambda: oracle.time_ref(stroked, threads=threads_ref))
+    t_ii = median_time(lambda: oracle.time_ref(stroked, threads=cores))
+    n = scene.n_paths
+    return {"value": round(n / (t_s + t_ii), 1), "unit": "paths/s", "cores": cores, "kind": "reference",
+            "sample": f"the whole scene ({n} paths, {scene.n_clip_paths} of them clip paths), median of {reps} runs "
+                      f"per leg; value = strokes expanded (1 thread) + the reference Tiler on {cores} threads; clip "
+                      f"resolution and the raster half not timed",
+            "legs": {"stroke_1_thread_s": round(t_s, 4),
+                     "i_tiling_ref_threads": {"threads": threads_ref, "paths_per_s": round(n / t_i, 1),
+                                              "s": round(t_i, 4), "kind": "reference"},
+                     "ii_tiling_all_cores": {"threads": cores, "paths_per_s": round(n / t_ii, 1), "s": round(t_ii, 4),
+                                             "kind": "reference"}},
+            "host_nproc": os.cpu_count()}
 
 
 def copy_calibration(mbytes):
