@@ -75,6 +75,22 @@ def test_training_learns_scene_totals(ctx, trained):
     assert (rel < 0.35).all(), rel
 
 
+def test_shipped_predictor_on_held_out_scenes(ctx):
+    """The shipped weights (trained by predictor.main on scenes of seed 7) against the exact counts of
+    scenes it never saw (the same generators, seed 11: 34 scenes, 156 k paths): the scene totals of
+    chords, fills and alpha tiles within 3 % (measured 0.8 / 0.4 / 0.2 %)."""
+    w = predictor.load()
+    assert w is not None, "predictor weights missing"
+    held = predictor.training_scenes(150_000, seed=11)
+    feats, counts = predictor.collect(ctx, held)
+    pred = predictor.predict_totals(ctx, w, feats)
+    exact = counts.sum(0).astype(np.float64)
+    rel = np.abs(pred - exact) / exact
+    print(f"[predictor] shipped weights on {len(held)} held-out scenes ({feats.shape[0]} paths): exact {exact}, "
+          f"predicted {np.round(pred)}, rel err {np.round(rel, 4)}", flush=True)
+    assert (rel < 0.03).all(), rel
+
+
 def test_predictor_gain_over_count_round_trip(ctx, trained):
     """Fresh scenes in a batch: without the predictor the batch needs an extra count
     pass and two host round trips before the launch; with it, one graph launch and
