@@ -139,12 +139,13 @@ def clip_scenes():
         ([([(10, 10), (90, 30), (60, 80)], [0, 1, 0], True)], 0, OPAQ),
         ([([(40, -60), (300, 40), (80, 150)], None, True)], 0, 0),
     ]), 1, [0, 0, -1])
-    # counter-clockwise clip (negative windings: its masks enter the combine through abs()) with an
-    # even-odd rule that the hybrid path ignores, over a nonzero and an even-odd draw path
+    # counter-clockwise clip of two overlapping rects (winding -1, and -2 where they overlap: its masks
+    # enter the combine through abs()) with an even-odd rule that the hybrid path ignores (honoured, it
+    # would clip the overlap away), over a nonzero and an even-odd draw path
     s["clip_ccw_evenodd_rule"] = with_clips(make_scene(160, 160, [
         ([rect(6.5, 10.25, 150.5, 120.75)], 0, HALF),
         ([star(80, 80, 70, 9, 4)], 1, OPAQ),
-        ([rect(20.25, 30.5, 140.75, 140.25, cw=False)], 1, 0),
+        ([rect(20.25, 30.5, 140.75, 140.25, cw=False), rect(50.5, 12.75, 118.25, 100.5, cw=False)], 1, 0),
     ]), 1, [0, 0, -1])
     s["clip_random_blobs"] = scenegen.clip_scene(4242, size=512, n_draw=60, n_clip=5, r_max=160.0)
     return s

@@ -340,3 +340,20 @@ def test_clip_nesting_deeper_than_four_is_refused():
     sc.path_clip = np.array([4, -1, 0, 1, 2, 4], np.int32)  # a cycle
     with pytest.raises(ValueError):
         oracle.full_model(sc)
+
+
+def test_clip_fill_rule_is_ignored_where_windings_reach_two():
+    """The golden clip_ccw_evenodd_rule scene: its clip path is two overlapping counter-clockwise rects
+    (winding -2 where they overlap) marked even-odd.  The hybrid path takes |winding| and never applies
+    the clip's own rule, so the nonzero draw rect under it stays fully covered inside the overlap (the
+    even-odd rule, honoured, would have clipped it away there) and in the rest of the clip."""
+    from conftest import load_golden
+
+    sc, _ = load_golden()["clip_ccw_evenodd_rule"]
+    m = oracle.full_model(sc, zmode=0)
+    cov = oracle.path_coverage_image(sc, m, 0)
+    overlap = cov[32:99, 52:117]      # inside both clip rects (and the draw rect), off their edges
+    single = cov[110:118, 30:45]      # inside one clip rect and the draw rect only
+    assert np.abs(overlap - 1.0).max() <= 1.0 / 255.0
+    assert np.abs(single - 1.0).max() <= 1.0 / 255.0
+    assert cov[11:12, 60:100].max() == 0.0  # inside the draw rect, above both clip rects: clipped away
