@@ -65,11 +65,20 @@ typedef struct pfh_outline {
 #define PFH_JOIN_MITER 0
 #define PFH_JOIN_BEVEL 1
 #define PFH_JOIN_ROUND 2
+/* Dashes (OutlineDash, core/dash.cpp:9-126), applied before stroking as Canvas::stroke_path does
+ * (core/canvas.cpp:284-289): dash_count > 0 dashes the path with the pattern
+ * dashes[dash_start .. dash_start + dash_count) (on, off, on, ... lengths; finite, non-negative,
+ * of positive total) starting dash_offset into it (finite, non-negative; Canvas::stroke_path
+ * passes 0 whatever its line_dash_offset, the dasher's own offset argument is exposed here).
+ * dash_count 0 strokes the path undashed. */
 typedef struct pfh_stroke_style {
     float line_width;
     int32_t line_cap;
     int32_t line_join;
     float miter_limit;
+    int32_t dash_start;
+    int32_t dash_count;
+    float dash_offset;
 } pfh_stroke_style;
 
 
@@ -107,6 +116,9 @@ typedef struct pfh_scene_desc {
      * path whose line_width is above 0 is stroked on the device at upload (as pfh_stroke_outlines)
      * and its stroke filled with the nonzero rule, whatever path_fill_rule says. */
     const pfh_stroke_style *path_stroke;
+    /* The dash lengths that path_stroke's patterns index (NULL when no path is dashed). */
+    const float *stroke_dashes;         /* [n_stroke_dashes] */
+    int32_t n_stroke_dashes;
 } pfh_scene_desc;
 
 typedef struct pfh_stats {
@@ -275,7 +287,11 @@ int pfh_server_stop(pfh_ctx *ctx);
  * Stroke-to-fill on the device: the outlines whose nonzero fills are the strokes of `in`'s paths,
  * bit-identical to the reference's OutlineStrokeToFill::offset / into_outline
  * (core/stroke.cpp:124-195), which Canvas::stroke_path runs before a stroke reaches the scene
- * (core/canvas.cpp:272-301); here many paths are stroked at once, one style each.  Output
+ * (core/canvas.cpp:272-301); here many paths are stroked at once, one style each.  A style with
+ * dash_count > 0 dashes its path first with dashes[dash_start ..] (OutlineDash::dash +
+ * into_outline, core/dash.cpp:48-126), bit-identical as well; dashes may be NULL when no style
+ * dashes (n_dashes 0).  A pattern far finer than its path (over 2^24 dash steps) fails with
+ * PFH_E_LIMIT.  Output
  * capacities in points and contours; on return n_points / n_contours hold the sizes needed and
  * PFH_E_LIMIT means a capacity was too small (call again with them; capacities of 0 ask for the
  * sizes only).  out->path_contour_start [n_paths + 1] receives each path's output contours.
@@ -292,7 +308,7 @@ typedef struct pfh_outline_buf {
 } pfh_outline_buf;
 
 int pfh_stroke_outlines(pfh_ctx *ctx, const pfh_outline *in, const int32_t *path_contour_start, int32_t n_paths,
-                        const pfh_stroke_style *styles, pfh_outline_buf *out);
+                        const pfh_stroke_style *styles, const float *dashes, int32_t n_dashes, pfh_outline_buf *out);
 
 #ifdef __cplusplus
 }

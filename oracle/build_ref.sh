@@ -20,7 +20,7 @@ SRCS=(
   core/data/built_path.cpp core/data/contour.cpp core/data/line_segment.cpp core/data/path.cpp
   core/data/segment.cpp
   core/d3d9/tiler.cpp core/d3d9/object_builder.cpp core/d3d9/scene_builder.cpp core/d3d9/data/alpha_tile_id.cpp
-  core/scene.cpp core/scene_builder.cpp core/stroke.cpp core/path2d.cpp
+  core/scene.cpp core/scene_builder.cpp core/stroke.cpp core/dash.cpp core/path2d.cpp
   core/paint/palette.cpp core/paint/paint.cpp core/paint/gradient.cpp core/paint/pattern.cpp
   core/paint/texture_allocator.cpp core/renderer.cpp gpu_mem/allocator.cpp common/io.cpp
   gpu/device.cpp gpu/command_encoder.cpp gpu/shader.cpp

@@ -59,6 +59,8 @@ def load_oracle():
         lib.pfo_apply_rule.argtypes = [ctypes.c_double, ctypes.c_int]
         lib.pfo_composite.argtypes = [ctypes.c_int64] + [_P] * 7 + [ctypes.c_int] * 4 + [_P]
         lib.pfo_stroke_outline.argtypes = [_P, _P, _P, _P, ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P]
+        lib.pfo_dash_outline.argtypes = [_P, _P, ctypes.c_int32, ctypes.c_float, _P, _P, ctypes.c_int64, _P, _P,
+                                         ctypes.c_int64, _P, _P]
         lib.pfo_raster_cpu.restype = ctypes.c_double
         lib.pfo_raster_cpu.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
         _oracle = lib
@@ -87,6 +89,9 @@ def load_ref():
         lib.pfref_stroke_counts.argtypes = [_P, _P]
         lib.pfref_stroke_fetch.argtypes = [_P] * 5
         lib.pfref_stroke_free.argtypes = [_P]
+        lib.pfref_dash.restype = _P
+        lib.pfref_dash.argtypes = [_P] * 4 + [ctypes.c_int, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_float]
         _ref = lib
     return _ref
 
@@ -127,6 +132,45 @@ def stroke(points, point_flags, contour_start, contour_closed, width, cap=0, joi
     return out
 
 
+def dash(points, point_flags, contour_start, contour_closed, dashes, offset=0.0):
+    """The oracle's dasher (oracle/stroke_oracle.cpp, a restatement of OutlineDash, core/dash.cpp:9-126)
+    on one outline: its dashes as open contours (points [N, 2] f32, point_flags u8, contour_start i32,
+    contour_closed u8)."""
+    lib = load_oracle()
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 2)
+    flg = np.ascontiguousarray(point_flags, np.uint8)
+    cst = np.ascontiguousarray(contour_start, np.int32)
+    ccl = np.ascontiguousarray(contour_closed, np.uint8)
+    dsh = np.ascontiguousarray(dashes, np.float32)
+    o = _Outline(int(ccl.shape[0]), int(flg.shape[0]), _ptr(pts), _ptr(flg), _ptr(cst), _ptr(ccl))
+    n_pts, n_ct = np.zeros(1, np.int64), np.zeros(1, np.int64)
+    cs0 = np.zeros(1, np.int32)
+    rc = lib.pfo_dash_outline(ctypes.byref(o), _ptr(dsh), int(dsh.shape[0]), float(offset), None, None, 0, _ptr(cs0),
+                              None, 0, _ptr(n_pts), _ptr(n_ct))
+    if rc not in (0, -5):
+        raise ValueError(f"pfo_dash_outline failed ({rc})")
+    out = (np.zeros((n_pts[0], 2), np.float32), np.zeros(n_pts[0], np.uint8), np.zeros(n_ct[0] + 1, np.int32),
+           np.zeros(n_ct[0], np.uint8))
+    rc = lib.pfo_dash_outline(ctypes.byref(o), _ptr(dsh), int(dsh.shape[0]), float(offset), _ptr(out[0]), _ptr(out[1]),
+                              int(n_pts[0]), _ptr(out[2]), _ptr(out[3]), int(n_ct[0]), _ptr(n_pts), _ptr(n_ct))
+    if rc != 0:
+        raise ValueError(f"pfo_dash_outline failed ({rc})")
+    return out
+
+
+def stroke_styles_array(ps):
+    """Rows (line_width, cap, join, miter_limit, dash_start, dash_count, dash_offset) of a scene's
+    path_stroke (a structured array or rows of 4 or 7 numbers)."""
+    ps = np.asarray(ps)
+    if ps.dtype.names:
+        ps = np.stack([ps[k].astype(np.float64) for k in ps.dtype.names], axis=1)
+    ps = np.asarray(ps, np.float64)
+    ps = ps.reshape(ps.shape[0], -1)
+    if ps.shape[1] == 4:
+        ps = np.concatenate([ps, np.zeros((ps.shape[0], 3))], axis=1)
+    return ps
+
+
 def apply_strokes(scene):
     """The scene with every stroked path (path_stroke width > 0) replaced by its stroked outline, filled
     with the nonzero rule (core/canvas.cpp:272-301); the scene itself when nothing is stroked."""
@@ -135,10 +179,9 @@ def apply_strokes(scene):
         return scene
     import scenegen  # the Scene container only
 
-    ps = np.asarray(ps)
-    if ps.dtype.names:
-        ps = np.stack([ps[k].astype(np.float64) for k in ps.dtype.names], axis=1)
-    ps = np.asarray(ps, np.float64).reshape(-1, 4)
+    ps = stroke_styles_array(ps)
+    dashes = np.asarray(getattr(scene, "stroke_dashes", None) if getattr(scene, "stroke_dashes", None) is not None
+                        else np.zeros(0), np.float32)
     outlines, rules = [], scene.path_fill_rule.copy()
     for p in range(scene.n_paths):
         c0, c1 = int(scene.path_contour_start[p]), int(scene.path_contour_start[p + 1])
@@ -146,6 +189,9 @@ def apply_strokes(scene):
         o = (scene.points[a:b], scene.point_flags[a:b], scene.contour_start[c0:c1 + 1] - a,
              scene.contour_closed[c0:c1])
         if ps[p, 0] > 0:
+            if ps[p, 5] > 0:  # dashed before stroking (core/canvas.cpp:284-289)
+                d0 = int(ps[p, 4])
+                o = dash(*o, dashes[d0:d0 + int(ps[p, 5])], ps[p, 6])
             o = stroke(*o, ps[p, 0], int(ps[p, 1]), int(ps[p, 2]), ps[p, 3])
             rules[p] = 0
         outlines.append(o)
@@ -252,6 +298,28 @@ def time_ref(scene, threads: int = 4, repeats: int = 1) -> float:
                           _ptr(scene.contour_start), _ptr(scene.contour_closed), _ptr(scene.points),
                           _ptr(scene.point_flags), _ptr(scene.view_box), ctypes.c_int(threads),
                           ctypes.c_int(repeats))
+
+
+def ref_dash(points, point_flags, contour_start, contour_closed, dashes, offset=0.0, width=0.0, cap=0, join=0,
+             miter_limit=10.0):
+    """The reference's OutlineDash (core/dash.cpp:48-126), compiled from its sources, on one outline, then
+    (width > 0) its OutlineStrokeToFill of the dashed outline: (points [N, 2] f32, point_flags u8,
+    contour_start i32, contour_closed u8)."""
+    lib = load_ref()
+    pts = np.ascontiguousarray(points, np.float32)
+    flg = np.ascontiguousarray(point_flags, np.uint8)
+    cst = np.ascontiguousarray(contour_start, np.int32)
+    ccl = np.ascontiguousarray(contour_closed, np.uint8)
+    dsh = np.ascontiguousarray(dashes, np.float32)
+    h = lib.pfref_dash(_ptr(pts), _ptr(flg), _ptr(cst), _ptr(ccl), int(ccl.shape[0]), _ptr(dsh), int(dsh.shape[0]),
+                       float(offset), float(width), int(cap), int(join), float(miter_limit))
+    counts = np.zeros(2, np.int64)
+    lib.pfref_stroke_counts(h, _ptr(counts))
+    out = (np.zeros((counts[0], 2), np.float32), np.zeros(counts[0], np.uint8), np.zeros(counts[1] + 1, np.int32),
+           np.zeros(counts[1], np.uint8))
+    lib.pfref_stroke_fetch(h, _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), _ptr(out[3]))
+    lib.pfref_stroke_free(h)
+    return out
 
 
 def ref_stroke(points, point_flags, contour_start, contour_closed, width, cap=0, join=0, miter_limit=10.0):

@@ -18,6 +18,7 @@
 #include "core/data/built_path.h"
 #include "core/scene.h"
 #include "core/stroke.h"
+#include "core/dash.h"
 
 namespace Pathfinder {
 // Defined with external linkage in core/d3d9/scene_builder.cpp:18 (not in a header).
@@ -366,6 +367,40 @@ void pfref_stroke_fetch(void *h, float *points, uint8_t *flags, int32_t *contour
 
 void pfref_stroke_free(void *h) {
     delete (RefStroke *)h;
+}
+
+// The reference's dasher (OutlineDash::dash + into_outline, core/dash.cpp:48-126) on one outline, then,
+// when width > 0, its stroke-to-fill of the dashed outline (the order of Canvas::stroke_path,
+// core/canvas.cpp:283-295); fetch with pfref_stroke_counts / pfref_stroke_fetch, free with
+// pfref_stroke_free.
+void *pfref_dash(const float *points, const uint8_t *point_flags, const int32_t *contour_start,
+                 const uint8_t *contour_closed, int n_contours, const float *dashes, int n_dashes, float offset,
+                 float width, int cap, int join, float miter_limit) {
+    Outline in = make_outline(points, point_flags, contour_start, contour_closed, 0, n_contours);
+    OutlineDash dasher(in, std::vector<float>(dashes, dashes + n_dashes), offset);
+    dasher.dash();
+    Outline out = dasher.into_outline();
+    if (width > 0.0f) {
+        StrokeStyle style;
+        style.line_width = width;
+        style.line_cap = (LineCap)cap;
+        style.line_join = (LineJoin)join;
+        style.miter_limit = miter_limit;
+        OutlineStrokeToFill stroker(out, style);
+        stroker.offset();
+        out = stroker.into_outline();
+    }
+    auto *r = new RefStroke();
+    for (const Contour &c : out.contours) {
+        for (size_t i = 0; i < c.points.size(); i++) {
+            r->points.push_back(c.points[i].x);
+            r->points.push_back(c.points[i].y);
+            r->flags.push_back((uint8_t)c.flags[i]);
+        }
+        r->contour_start.push_back((int32_t)(r->flags.size()));
+        r->closed.push_back(c.closed ? 1 : 0);
+    }
+    return r;
 }
 
 } // extern "C"

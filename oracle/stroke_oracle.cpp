@@ -505,7 +505,181 @@ void push_stroked(std::vector<Contour> &out, Contour &c, const InContour &in, bo
     out.push_back(c);
 }
 
+// core/data/segment.cpp:168-187: the length estimate the dasher walks by (a line's length; for a
+// curve, the mean of its chord and its control polygon's length, a quadratic raised to a cubic first,
+// core/data/segment.cpp:141-151)
+float arc_length(const Seg &s) {
+    if (s.kind == K_LINE) return len(s.base.vector());
+    if (s.kind != K_QUAD && s.kind != K_CUBIC) return 0.0f;
+    V c0 = s.ctrl.from, c1 = s.ctrl.to;
+    if (s.kind == K_QUAD) {
+        const V p1_2 = s.ctrl.from + s.ctrl.from;
+        c0 = (s.base.from + p1_2) / 3.0f;
+        c1 = (p1_2 + s.base.to) / 3.0f;
+    }
+    const float chord = len(s.base.vector());
+    const float cont_net = len(s.base.from - c0) + len(c1 - c0) + len(s.base.to - c1);
+    return (cont_net + chord) * 0.5f;
+}
+
+// SegmentsIter (core/data/contour.cpp:107-177) with force_closed = false: a contour's segments in
+// order, then its closing line when it is closed and does not end where it starts (within 1e-4);
+// a malformed point pattern ends the iteration with a None segment.
+struct SegIter {
+    const V *p;
+    const uint8_t *f;
+    int n;
+    bool closed;
+    int head = 0;
+    bool has_next = true;
+    Seg next() {
+        Seg s;
+        if (head >= n || f[head] != ON_CURVE) return s;
+        s.base.from = p[head];
+        if (head + 1 < n) {
+            if (f[head + 1] == ON_CURVE) {
+                s.base.to = p[head + 1];
+                s.kind = K_LINE;
+                head += 1;
+            } else if (head + 2 < n) {
+                if (f[head + 1] == CONTROL_0 && f[head + 2] == ON_CURVE) {
+                    s.ctrl.from = p[head + 1];
+                    s.base.to = p[head + 2];
+                    s.kind = K_QUAD;
+                    head += 2;
+                } else if (head + 3 < n) {
+                    if (f[head + 1] == CONTROL_0 && f[head + 2] == CONTROL_1 && f[head + 3] == ON_CURVE) {
+                        s.ctrl.from = p[head + 1];
+                        s.ctrl.to = p[head + 2];
+                        s.base.to = p[head + 3];
+                        s.kind = K_CUBIC;
+                        head += 3;
+                    }
+                }
+            }
+        } else {
+            if (closed && n > 1 && !approx_eq(p[0], p[n - 1], FLOAT_EPSILON)) {
+                s.base.from = p[n - 1];
+                s.base.to = p[0];
+                s.kind = K_LINE;
+            }
+            has_next = false;
+        }
+        return s;
+    }
+};
+
+// OutlineDash (core/dash.cpp:9-126): DashState carried across the contours of the outline, each
+// contour walked by ContourDash::dash; the last dash is pushed by into_outline when it is on.
+// Returns false when the walk exceeds max_steps (a pattern of dashes far shorter than the outline).
+bool dash_outline(const pfo_outline &in, const float *dashes, int nd, float offset, std::vector<Contour> &out,
+                  uint64_t max_steps) {
+    float total = 0;
+    for (int i = 0; i < nd; i++) total += dashes[i];
+    offset = std::fmod(offset, total);
+    int idx = 0;
+    while (idx < nd) {
+        const float d = dashes[idx];
+        if (offset < d) break;
+        offset -= d;
+        idx += 1;
+    }
+    // Rounding in the subtractions can walk past the last dash (the reference then indexes one past
+    // its pattern); the pattern restarts instead.
+    if (idx >= nd) idx = 0;
+    float left = offset;
+    Contour cur;
+    uint64_t steps = 0;
+    const V *P = reinterpret_cast<const V *>(in.points);
+    for (int c = 0; c < in.n_contours; c++) {
+        const int a = in.contour_start[c];
+        SegIter it{P + a, in.point_flags + a, in.contour_start[c + 1] - a, in.contour_closed[c] != 0};
+        Seg queued;
+        bool queued_none = true;
+        while (true) {
+            if (++steps > max_steps) return false;
+            if (queued_none) {
+                if (!it.has_next) break;
+                queued = it.next();
+                if (queued.kind == K_NONE) break;
+                queued_none = false;
+            }
+            Seg seg = queued;
+            float distance = left;
+            const float t = distance / arc_length(seg);
+            if (t < 1.0f) {
+                Seg s0, s1;
+                seg.split(t, s0, s1);
+                seg = s0;
+                queued = s1;
+                queued_none = false;
+            } else {
+                distance = arc_length(seg);
+                queued_none = true;
+            }
+            if (idx % 2 == 0) cur.push_segment(seg);
+            left -= distance;
+            if (left < FLOAT_EPSILON) {
+                if (idx % 2 == 0) {
+                    if (!cur.pts.empty()) out.push_back(cur);  // Outline::push_contour skips empty ones
+                    cur = Contour{};
+                }
+                idx += 1;
+                if (idx == nd) idx = 0;
+                left = dashes[idx];
+            }
+        }
+    }
+    if (idx % 2 == 0 && !cur.pts.empty()) out.push_back(cur);
+    return true;
+}
+
 }  // namespace
+
+// Dashes of an outline (OutlineDash::dash + into_outline, core/dash.cpp:48-126): open contours.
+// The pattern must be finite, non-negative and of positive total, the offset finite and
+// non-negative.  0, -1 bad input, -4 the walk ran past 2^24 steps, -5 a capacity was too small
+// (n_points_out / n_contours_out hold the sizes).
+extern "C" int pfo_dash_outline(const pfo_outline *in, const float *dashes, int32_t n_dashes, float offset,
+                                float *points_out, uint8_t *flags_out, int64_t cap_points, int32_t *contour_start_out,
+                                uint8_t *contour_closed_out, int64_t cap_contours, int64_t *n_points_out,
+                                int64_t *n_contours_out) {
+    if (!in || !n_points_out || !n_contours_out || in->n_contours < 0 || in->n_points < 0 || n_dashes < 1 || !dashes)
+        return -1;
+    if ((in->n_points && (!in->points || !in->point_flags)) || !in->contour_start ||
+        (in->n_contours && !in->contour_closed))
+        return -1;
+    if (in->contour_start[0] != 0 || in->contour_start[in->n_contours] != in->n_points) return -1;
+    float total = 0;
+    for (int i = 0; i < n_dashes; i++) {
+        if (!std::isfinite(dashes[i]) || dashes[i] < 0.0f) return -1;
+        total += dashes[i];
+    }
+    if (!(total > 0.0f) || !std::isfinite(total) || !std::isfinite(offset) || offset < 0.0f) return -1;
+    try {
+        std::vector<Contour> out;
+        if (!dash_outline(*in, dashes, n_dashes, offset, out, 1ull << 24)) return -4;
+        int64_t np = 0;
+        for (const Contour &k : out) np += (int64_t)k.pts.size();
+        *n_points_out = np;
+        *n_contours_out = (int64_t)out.size();
+        if (np > cap_points || (int64_t)out.size() > cap_contours) return -5;
+        if ((np && (!points_out || !flags_out)) || !contour_start_out || (!out.empty() && !contour_closed_out))
+            return -1;
+        int64_t o = 0;
+        contour_start_out[0] = 0;
+        for (size_t k = 0; k < out.size(); k++) {
+            std::memcpy(points_out + 2 * o, out[k].pts.data(), out[k].pts.size() * sizeof(V));
+            std::memcpy(flags_out + o, out[k].flags.data(), out[k].flags.size());
+            o += (int64_t)out[k].pts.size();
+            contour_start_out[k + 1] = (int32_t)o;
+            contour_closed_out[k] = 0;
+        }
+        return 0;
+    } catch (...) {
+        return -3;
+    }
+}
 
 extern "C" int pfo_stroke_outline(const pfo_outline *in, const pfo_stroke_style *style, float *points_out,
                                   uint8_t *flags_out, int64_t cap_points, int32_t *contour_start_out,

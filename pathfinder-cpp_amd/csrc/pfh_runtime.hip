@@ -77,6 +77,15 @@ hipError_t launch_assemble(const StrokeContour *contours, uint32_t n, const uint
                            uint8_t *region_flags, uint32_t *point_cnt, hipStream_t s);
 hipError_t launch_compact(const uint32_t *region_off, const uint32_t *out_off, uint32_t n, const float2 *region_pts,
                           const uint8_t *region_flags, float2 *out_pts, uint8_t *out_flags, hipStream_t s);
+struct DashPath {
+    uint32_t c0, c1;
+    uint32_t d0, d1;
+    float offset;
+    uint32_t pad;
+};
+hipError_t launch_dash(const float2 *pts, const uint8_t *flags, const int32_t *contour_start, const uint8_t *closed,
+                       const DashPath *paths, uint32_t n, const float *dashes, uint32_t *cnt, const uint32_t *off,
+                       float2 *out_pts, uint8_t *out_flags, uint32_t *out_cstart, bool emit, hipStream_t s);
 }  // namespace pfh
 
 using namespace pfh;
@@ -673,6 +682,136 @@ static HostOutlines stroke_on_device(pfh_ctx *ctx, const float2 *P, const uint8_
     return res;
 }
 
+// Whether a path's style carries a dash pattern, after checking it: the pattern lies inside the
+// dash array, its lengths are finite and non-negative with a positive finite total, and the
+// offset is finite and non-negative.  OutlineDash (core/dash.cpp:9-29) takes the pattern
+// unchecked; with a zero total its fmod is NaN and its walk never advances.
+static bool dashed(const pfh_stroke_style &st, const float *dashes, int32_t n_dashes) {
+    if (st.dash_count == 0) return false;
+    if (st.dash_count < 0 || st.dash_start < 0 || (int64_t)st.dash_start + st.dash_count > n_dashes || !dashes)
+        throw PfhError{PFH_E_ARG, "dash pattern outside the dash array"};
+    float total = 0.0f;
+    for (int32_t i = st.dash_start; i < st.dash_start + st.dash_count; i++) {
+        if (!std::isfinite(dashes[i]) || dashes[i] < 0.0f) throw PfhError{PFH_E_ARG, "dash length not finite or negative"};
+        total += dashes[i];
+    }
+    if (!(total > 0.0f) || !std::isfinite(total)) throw PfhError{PFH_E_ARG, "dash pattern of zero or infinite length"};
+    if (!std::isfinite(st.dash_offset) || st.dash_offset < 0.0f) throw PfhError{PFH_E_ARG, "dash offset not finite or negative"};
+    return true;
+}
+
+// Dashes the paths whose style carries a pattern (OutlineDash::dash + into_outline,
+// core/dash.cpp:48-126), as Canvas::stroke_path does before it strokes (core/canvas.cpp:284-289):
+// a thread per dashed path walks its contours with the dash state carried across them (k_dash),
+// a count pass sizing each path's dashes and an emit pass writing them at the scanned offsets.
+// The dashes are open contours; a path without a pattern passes through unchanged.
+static HostOutlines dash_on_device(pfh_ctx *ctx, const float2 *P, const uint8_t *F, const int32_t *contour_start,
+                                   const uint8_t *contour_closed, const int32_t *path_contour_start, int32_t n_paths,
+                                   const pfh_stroke_style *styles, const float *dashes, int32_t n_dashes) {
+    std::vector<DashPath> dp;
+    std::vector<int32_t> which;
+    for (int32_t p = 0; p < n_paths; p++)
+        if (dashed(styles[p], dashes, n_dashes)) {
+            const pfh_stroke_style &st = styles[p];
+            dp.push_back(DashPath{(uint32_t)path_contour_start[p], (uint32_t)path_contour_start[p + 1],
+                                  (uint32_t)st.dash_start, (uint32_t)(st.dash_start + st.dash_count), st.dash_offset, 0});
+            which.push_back(p);
+        }
+    const uint32_t n = (uint32_t)dp.size();
+    const int32_t n_contours = path_contour_start[n_paths];
+    const int32_t n_points = contour_start[n_contours];
+    std::vector<uint32_t> off(2 * (size_t)n + 2, 0);
+    std::vector<float2> dpts;
+    std::vector<uint8_t> dflags;
+    std::vector<uint32_t> dcs;
+    if (n) {
+        hipStream_t st = ctx->w->stream;
+        DevBuf d_pts, d_flags, d_cs, d_closed, d_paths, d_dashes, d_cnt, d_off, d_opts, d_oflags, d_ocs;
+        struct Release {
+            std::vector<DevBuf *> bufs;
+            ~Release() {
+                for (DevBuf *b : bufs) b->release();
+            }
+        } release{{&d_pts, &d_flags, &d_cs, &d_closed, &d_paths, &d_dashes, &d_cnt, &d_off, &d_opts, &d_oflags, &d_ocs}};
+        upload(d_pts, std::vector<float2>(P, P + n_points));
+        upload(d_flags, std::vector<uint8_t>(F, F + n_points));
+        upload(d_cs, std::vector<int32_t>(contour_start, contour_start + n_contours + 1));
+        upload(d_closed, std::vector<uint8_t>(contour_closed, contour_closed + n_contours));
+        upload(d_paths, dp);
+        upload(d_dashes, std::vector<float>(dashes, dashes + n_dashes));
+        d_cnt.ensure(8ull * n);
+        HIP_TRY(launch_dash(d_pts.as<float2>(), d_flags.as<uint8_t>(), d_cs.as<int32_t>(), d_closed.as<uint8_t>(),
+                            d_paths.as<DashPath>(), n, d_dashes.as<float>(), d_cnt.as<uint32_t>(), nullptr, nullptr,
+                            nullptr, nullptr, false, st));
+        std::vector<uint32_t> cnt(2 * (size_t)n);
+        HIP_TRY(hipMemcpyAsync(cnt.data(), d_cnt.ptr, 8ull * n, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (uint32_t k = 0; k < n; k++) {
+            if (cnt[2 * k] == 0xFFFFFFFFu)
+                throw PfhError{PFH_E_LIMIT, "dash pattern too fine for its path (more than 2^24 dash steps)"};
+            const uint64_t np = (uint64_t)off[2 * k] + cnt[2 * k], nc = (uint64_t)off[2 * k + 1] + cnt[2 * k + 1];
+            if (np >= 0x7FFFFFFFull || nc >= 0x7FFFFFFFull) throw PfhError{PFH_E_LIMIT, "dashed outlines too large"};
+            off[2 * k + 2] = (uint32_t)np;
+            off[2 * k + 3] = (uint32_t)nc;
+        }
+        upload(d_off, off);
+        d_opts.ensure(8ull * off[2 * n] + 8);
+        d_oflags.ensure((size_t)off[2 * n] + 4);
+        d_ocs.ensure(4ull * off[2 * n + 1] + 4);
+        HIP_TRY(launch_dash(d_pts.as<float2>(), d_flags.as<uint8_t>(), d_cs.as<int32_t>(), d_closed.as<uint8_t>(),
+                            d_paths.as<DashPath>(), n, d_dashes.as<float>(), nullptr, d_off.as<uint32_t>(),
+                            d_opts.as<float2>(), d_oflags.as<uint8_t>(), d_ocs.as<uint32_t>(), true, st));
+        dpts.resize(off[2 * n]);
+        dflags.resize(off[2 * n]);
+        dcs.resize(off[2 * n + 1]);
+        if (!dpts.empty()) {
+            HIP_TRY(hipMemcpyAsync(dpts.data(), d_opts.ptr, 8 * dpts.size(), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(dflags.data(), d_oflags.ptr, dflags.size(), hipMemcpyDeviceToHost, st));
+        }
+        if (!dcs.empty()) HIP_TRY(hipMemcpyAsync(dcs.data(), d_ocs.ptr, 4 * dcs.size(), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    HostOutlines res;
+    size_t k = 0;
+    for (int32_t p = 0; p < n_paths; p++) {
+        if (k < which.size() && which[k] == p) {
+            const uint32_t c0 = off[2 * k + 1], c1 = off[2 * k + 3], pend = off[2 * k + 2];
+            for (uint32_t c = c0; c < c1; c++) {
+                const uint32_t a = dcs[c], b = c + 1 < c1 ? dcs[c + 1] : pend;
+                res.pts.insert(res.pts.end(), dpts.begin() + a, dpts.begin() + b);
+                res.flags.insert(res.flags.end(), dflags.begin() + a, dflags.begin() + b);
+                res.contour_start.push_back((int32_t)res.pts.size());
+                res.closed.push_back(0);
+            }
+            k++;
+        } else {
+            for (int32_t c = path_contour_start[p]; c < path_contour_start[p + 1]; c++) {
+                res.pts.insert(res.pts.end(), P + contour_start[c], P + contour_start[c + 1]);
+                res.flags.insert(res.flags.end(), F + contour_start[c], F + contour_start[c + 1]);
+                res.contour_start.push_back((int32_t)res.pts.size());
+                res.closed.push_back(contour_closed[c]);
+            }
+        }
+        if (res.pts.size() >= 0x7FFFFFFFull) throw PfhError{PFH_E_LIMIT, "dashed outlines too large"};
+        res.path_contour_start.push_back((int32_t)res.closed.size());
+    }
+    return res;
+}
+
+// Strokes the paths, dashing first those whose style carries a pattern.
+static HostOutlines dash_and_stroke(pfh_ctx *ctx, const float2 *P, const uint8_t *F, const int32_t *contour_start,
+                                    const uint8_t *contour_closed, const int32_t *path_contour_start, int32_t n_paths,
+                                    const pfh_stroke_style *styles, const float *dashes, int32_t n_dashes) {
+    bool any = false;
+    for (int32_t p = 0; p < n_paths; p++) any |= dashed(styles[p], dashes, n_dashes);
+    if (!any)
+        return stroke_on_device(ctx, P, F, contour_start, contour_closed, path_contour_start, n_paths, styles);
+    HostOutlines d = dash_on_device(ctx, P, F, contour_start, contour_closed, path_contour_start, n_paths, styles,
+                                    dashes, n_dashes);
+    return stroke_on_device(ctx, d.pts.data(), d.flags.data(), d.contour_start.data(), d.closed.data(),
+                            d.path_contour_start.data(), n_paths, styles);
+}
+
 static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d);
 
 // A scene whose paths carry stroke styles: those paths are stroked on the device first and
@@ -714,8 +853,9 @@ static int scene_upload(pfh_ctx *ctx, const pfh_scene_desc *d) {
             scs.push_back((int32_t)spts.size());
             scl.push_back(d->contour_closed[c]);
         }
-    HostOutlines so = stroke_on_device(ctx, spts.data(), sfl.data(), scs.data(), scl.data(), spcs.data(),
-                                       (int32_t)stroked.size(), sst.data());
+    if (d->n_stroke_dashes < 0) throw PfhError{PFH_E_ARG, "negative n_stroke_dashes"};
+    HostOutlines so = dash_and_stroke(ctx, spts.data(), sfl.data(), scs.data(), scl.data(), spcs.data(),
+                                      (int32_t)stroked.size(), sst.data(), d->stroke_dashes, d->n_stroke_dashes);
     // the scene with the stroked outlines in place of the stroked paths' own
     std::vector<float2> pts;
     std::vector<uint8_t> fl, closed, rule(d->path_fill_rule, d->path_fill_rule + d->n_paths);
@@ -1849,8 +1989,9 @@ int pfh_scene_upload(pfh_ctx *ctx, const pfh_scene_desc *desc, int32_t *scene_id
 }
 
 int pfh_stroke_outlines(pfh_ctx *ctx, const pfh_outline *in, const int32_t *path_contour_start, int32_t n_paths,
-                        const pfh_stroke_style *styles, pfh_outline_buf *out) {
-    if (!ctx || !in || !out || n_paths < 0 || (n_paths > 0 && (!styles || !path_contour_start))) return PFH_E_ARG;
+                        const pfh_stroke_style *styles, const float *dashes, int32_t n_dashes, pfh_outline_buf *out) {
+    if (!ctx || !in || !out || n_paths < 0 || n_dashes < 0 || (n_paths > 0 && (!styles || !path_contour_start)))
+        return PFH_E_ARG;
     return run_guarded(ctx, [&] {
         if (in->n_contours < 0 || in->n_points < 0 || !in->contour_start || (in->n_points && (!in->points || !in->point_flags)) ||
             (in->n_contours && !in->contour_closed))
@@ -1864,8 +2005,9 @@ int pfh_stroke_outlines(pfh_ctx *ctx, const pfh_outline *in, const int32_t *path
             if (path_contour_start[p + 1] < path_contour_start[p])
                 throw PfhError{PFH_E_ARG, "path_contour_start not monotone"};
         HIP_TRY(hipSetDevice(ctx->device));
-        HostOutlines so = stroke_on_device(ctx, reinterpret_cast<const float2 *>(in->points), in->point_flags,
-                                           in->contour_start, in->contour_closed, path_contour_start, n_paths, styles);
+        HostOutlines so = dash_and_stroke(ctx, reinterpret_cast<const float2 *>(in->points), in->point_flags,
+                                          in->contour_start, in->contour_closed, path_contour_start, n_paths, styles,
+                                          dashes, n_dashes);
         out->n_points = (int64_t)so.pts.size();
         out->n_contours = (int64_t)so.closed.size();
         if (out->n_points > out->cap_points || out->n_contours > out->cap_contours)

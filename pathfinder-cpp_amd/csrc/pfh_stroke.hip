@@ -271,6 +271,15 @@ struct StrokeContour {
     uint32_t pad;
 };
 
+// A path to dash: its contours [c0, c1) of the input and its pattern [d0, d1) of the dash lengths.
+struct DashPath {
+    uint32_t c0, c1;
+    uint32_t d0, d1;
+    float offset;
+    uint32_t pad;
+};
+constexpr uint32_t DASH_MAX_STEPS = 1u << 24;  // a walk this long marks the path as over-long
+
 namespace stroke {
 
 // One add_to_contour call (core/stroke.cpp:469-480) in leaf form.
@@ -511,6 +520,198 @@ __global__ __launch_bounds__(256) void k_compact(const uint32_t *region_off, con
         out_pts[dst + k] = region_pts[src + k];
         out_flags[dst + k] = region_flags[src + k];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Dashes (OutlineDash, core/dash.cpp:9-126), before stroking (core/canvas.cpp:284-289)
+// ---------------------------------------------------------------------------
+//
+// A thread per dashed path: the dash state (pattern index, distance left, the dash being built)
+// carries over the path's contours, as in the reference, so a path is one sequential walk.  The
+// count pass sizes each path's dashed outline; the emit pass writes it at the offsets the host scanned.
+
+namespace stroke {
+
+// Segment::arc_length (core/data/segment.cpp:168-187): a line's length; for a curve, the mean of its
+// chord and its control polygon's length, a quadratic raised to a cubic first (:141-151).
+__device__ __forceinline__ float arc_length(const Seg &s) {
+    if (s.kind == K_LINE) return len(s.base.vector());
+    if (s.kind != K_QUAD && s.kind != K_CUBIC) return 0.0f;
+    V c0 = s.ctrl.from, c1 = s.ctrl.to;
+    if (s.kind == K_QUAD) {
+        const V p1_2 = s.ctrl.from + s.ctrl.from;
+        c0 = (s.base.from + p1_2) / 3.0f;
+        c1 = (p1_2 + s.base.to) / 3.0f;
+    }
+    const float chord = len(s.base.vector());
+    const float cont_net = len(s.base.from - c0) + len(c1 - c0) + len(s.base.to - c1);
+    return (cont_net + chord) * 0.5f;
+}
+
+// SegmentsIter::get_next (core/data/contour.cpp:110-173) without force_closed.
+struct SegIter {
+    const float2 *p;
+    const uint8_t *f;
+    int n;
+    bool closed;
+    int head;
+    bool has_next;
+    __device__ V at(int i) const { return mk(p[i].x, p[i].y); }
+    __device__ Seg next() {
+        Seg s{Line{mk(0, 0), mk(0, 0)}, Line{mk(0, 0), mk(0, 0)}, K_NONE};
+        if (head >= n || f[head] != 0) return s;
+        s.base.from = at(head);
+        if (head + 1 < n) {
+            if (f[head + 1] == 0) {
+                s.base.to = at(head + 1);
+                s.kind = K_LINE;
+                head += 1;
+            } else if (head + 2 < n) {
+                if (f[head + 1] == 1 && f[head + 2] == 0) {
+                    s.ctrl.from = at(head + 1);
+                    s.base.to = at(head + 2);
+                    s.kind = K_QUAD;
+                    head += 2;
+                } else if (head + 3 < n) {
+                    if (f[head + 1] == 1 && f[head + 2] == 2 && f[head + 3] == 0) {
+                        s.ctrl.from = at(head + 1);
+                        s.ctrl.to = at(head + 2);
+                        s.base.to = at(head + 3);
+                        s.kind = K_CUBIC;
+                        head += 3;
+                    }
+                }
+            }
+        } else {
+            // the closing line unless the contour ends where it starts (Vec2::approx_eq, 1e-4)
+            if (closed && n > 1 && !(len(at(0) - at(n - 1)) <= FLOAT_EPSILON)) {
+                s.base.from = at(n - 1);
+                s.base.to = at(0);
+                s.kind = K_LINE;
+            }
+            has_next = false;
+        }
+        return s;
+    }
+};
+
+}  // namespace stroke
+
+template <bool EMIT>
+__global__ __launch_bounds__(64) void k_dash(const float2 *pts, const uint8_t *flags, const int32_t *contour_start,
+                                             const uint8_t *closed, const DashPath *paths, uint32_t n,
+                                             const float *dashes, uint32_t *cnt, const uint32_t *off, float2 *out_pts,
+                                             uint8_t *out_flags, uint32_t *out_cstart) {
+    using namespace stroke;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const DashPath dp = paths[p];
+    const float *D = dashes + dp.d0;
+    const uint32_t nd = dp.d1 - dp.d0;
+    // DashState::DashState (core/dash.cpp:9-29); the host checked the pattern: finite, >= 0, total > 0
+    float total = 0.0f;
+    for (uint32_t i = 0; i < nd; i++) total += D[i];
+    float offset = fmodf(dp.offset, total);
+    uint32_t idx = 0;
+    while (idx < nd) {
+        const float d = D[idx];
+        if (offset < d) break;
+        offset -= d;
+        idx += 1;
+    }
+    // Rounding in the subtractions can walk past the last dash (the reference then indexes one past
+    // its pattern); the pattern restarts instead.
+    if (idx >= nd) idx = 0;
+    float left = offset;
+    const uint32_t pbase = EMIT ? off[2 * p] : 0u, cbase = EMIT ? off[2 * p + 1] : 0u;
+    const uint32_t pcap = EMIT ? off[2 * p + 2] - pbase : 0xFFFFFFFFu, ccap = EMIT ? off[2 * p + 3] - cbase : 0xFFFFFFFFu;
+    uint32_t npts = 0, ncont = 0, cur = 0, steps = 0;
+    bool over = false;
+    auto push_point = [&](V v, uint8_t f) {
+        if (EMIT && npts < pcap) {
+            out_pts[pbase + npts] = make_float2(v.x, v.y);
+            out_flags[pbase + npts] = f;
+        }
+        npts++;
+        cur++;
+    };
+    // Contour::push_segment (core/data/contour.cpp:11-41): every point, its start point included
+    auto push_segment = [&](const Seg &s) {
+        if (!s.valid()) return;
+        push_point(s.base.from, 0);
+        if (s.kind != K_LINE) {
+            push_point(s.ctrl.from, 1);
+            if (s.kind != K_QUAD) push_point(s.ctrl.to, 2);
+        }
+        push_point(s.base.to, 0);
+    };
+    // Outline::push_contour (core/data/path.cpp:24-38) skips an empty contour
+    auto push_contour = [&]() {
+        if (cur > 0) {
+            if (EMIT && ncont < ccap) out_cstart[cbase + ncont] = pbase + npts - cur;
+            ncont++;
+        }
+        cur = 0;
+    };
+    for (uint32_t c = dp.c0; c < dp.c1 && !over; c++) {
+        const int a = contour_start[c];
+        SegIter it{pts + a, flags + a, contour_start[c + 1] - a, closed[c] != 0, 0, true};
+        Seg queued{Line{mk(0, 0), mk(0, 0)}, Line{mk(0, 0), mk(0, 0)}, K_NONE};
+        bool queued_none = true;
+        // ContourDash::dash (core/dash.cpp:74-124)
+        while (true) {
+            if (++steps > DASH_MAX_STEPS) {
+                over = true;
+                break;
+            }
+            if (queued_none) {
+                if (!it.has_next) break;
+                queued = it.next();
+                if (queued.kind == K_NONE) break;
+                queued_none = false;
+            }
+            Seg seg = queued;
+            float distance = left;
+            const float t = distance / arc_length(seg);
+            if (t < 1.0f) {
+                Seg s0, s1;
+                seg.split(t, s0, s1);
+                seg = s0;
+                queued = s1;
+                queued_none = false;
+            } else {
+                distance = arc_length(seg);
+                queued_none = true;
+            }
+            if (idx % 2 == 0) push_segment(seg);
+            left -= distance;
+            if (left < FLOAT_EPSILON) {
+                if (idx % 2 == 0) push_contour();
+                idx += 1;
+                if (idx == nd) idx = 0;
+                left = D[idx];
+            }
+        }
+    }
+    if (idx % 2 == 0) push_contour();  // OutlineDash::into_outline (core/dash.cpp:57-63)
+    if (!EMIT) {
+        cnt[2 * p] = over ? 0xFFFFFFFFu : npts;
+        cnt[2 * p + 1] = ncont;
+    }
+}
+
+hipError_t launch_dash(const float2 *pts, const uint8_t *flags, const int32_t *contour_start, const uint8_t *closed,
+                       const DashPath *paths, uint32_t n, const float *dashes, uint32_t *cnt, const uint32_t *off,
+                       float2 *out_pts, uint8_t *out_flags, uint32_t *out_cstart, bool emit, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + 63) / 64);
+    if (emit)
+        hipLaunchKernelGGL(k_dash<true>, grid, dim3(64), 0, s, pts, flags, contour_start, closed, paths, n, dashes, cnt,
+                           off, out_pts, out_flags, out_cstart);
+    else
+        hipLaunchKernelGGL(k_dash<false>, grid, dim3(64), 0, s, pts, flags, contour_start, closed, paths, n, dashes, cnt,
+                           off, out_pts, out_flags, out_cstart);
+    return hipGetLastError();
 }
 
 hipError_t launch_offset_pieces(const StrokeSegment *segs, uint32_t n, uint32_t *leaf_cnt, const uint32_t *leaf_off,

@@ -60,6 +60,8 @@ void SceneBuilderHip::build(Scene *scene_, Renderer *) {
     desc_.n_clip_paths = (int32_t)scene->clip_paths.size();
     desc_.path_clip = clip_.data();
     desc_.path_stroke = nullptr;
+    desc_.stroke_dashes = nullptr;
+    desc_.n_stroke_dashes = 0;
 }
 
 RendererHip::RendererHip(const std::shared_ptr<Device> &device, const std::shared_ptr<Queue> &queue, int hip_device)

@@ -35,7 +35,8 @@ class SceneDesc(ctypes.Structure):
                 ("n_paths", ctypes.c_int32), ("n_contours", ctypes.c_int32), ("n_points", ctypes.c_int32),
                 ("points", _P), ("point_flags", _P), ("contour_start", _P), ("contour_closed", _P),
                 ("path_contour_start", _P), ("path_fill_rule", _P), ("path_rgba", _P),
-                ("n_clip_paths", ctypes.c_int32), ("path_clip", _P), ("path_stroke", _P)]
+                ("n_clip_paths", ctypes.c_int32), ("path_clip", _P), ("path_stroke", _P),
+                ("stroke_dashes", _P), ("n_stroke_dashes", ctypes.c_int32)]
 
 
 class Outline(ctypes.Structure):
@@ -127,7 +128,8 @@ def load(path: str = LIB_PATH):
         "pfh_scene_features": (ctypes.c_int, [_P, i32, _P]),
         "pfh_rasterize": (ctypes.c_int, [_P, _P, i32, ctypes.c_uint32]),
         "pfh_rasterize_noalloc": (ctypes.c_int, [_P, _P, i32, ctypes.c_uint32]),
-        "pfh_stroke_outlines": (ctypes.c_int, [_P, ctypes.POINTER(Outline), _P, i32, _P, ctypes.POINTER(OutlineBuf)]),
+        "pfh_stroke_outlines": (ctypes.c_int, [_P, ctypes.POINTER(Outline), _P, i32, _P, _P, i32,
+                                                ctypes.POINTER(OutlineBuf)]),
         "pfh_scene_set_target": (ctypes.c_int, [_P, i32, _P, i64]),
         "pfh_scene_target": (ctypes.c_int, [_P, i32, ctypes.POINTER(_P), ctypes.POINTER(i64)]),
         "pfh_readback_rgba8": (ctypes.c_int, [_P, i32, _P, i64]),
@@ -163,17 +165,25 @@ def _ptr(a):
 
 
 STROKE_STYLE = np.dtype([("line_width", np.float32), ("line_cap", np.int32), ("line_join", np.int32),
-                         ("miter_limit", np.float32)])
+                         ("miter_limit", np.float32), ("dash_start", np.int32), ("dash_count", np.int32),
+                         ("dash_offset", np.float32)])
 
 
 def stroke_styles(styles) -> np.ndarray:
-    """Rows of (line_width, line_cap, line_join, miter_limit) as pfh_stroke_style records."""
+    """Rows of (line_width, line_cap, line_join, miter_limit) or of those and (dash_start,
+    dash_count, dash_offset) as pfh_stroke_style records (4 columns: undashed)."""
     a = np.asarray(styles)
     if a.dtype == STROKE_STYLE:
         return a
-    a = np.asarray(styles, np.float64).reshape(-1, 4)
+    a = np.asarray(styles, np.float64)
+    cols = a.shape[-1] if a.ndim == 2 else 4
+    if cols not in (4, 7):
+        raise ValueError("stroke styles have 4 or 7 columns")
+    a = a.reshape(-1, cols)
     out = np.zeros(a.shape[0], STROKE_STYLE)
     out["line_width"], out["line_cap"], out["line_join"], out["miter_limit"] = a[:, 0], a[:, 1], a[:, 2], a[:, 3]
+    if cols == 7:
+        out["dash_start"], out["dash_count"], out["dash_offset"] = a[:, 4], a[:, 5], a[:, 6]
     return out
 
 
@@ -227,35 +237,44 @@ class Context:
         if path_stroke is not None:
             path_stroke = np.ascontiguousarray(stroke_styles(path_stroke))
         d.path_stroke = None if path_stroke is None else _ptr(path_stroke)
+        dashes = getattr(scene, "stroke_dashes", None)
+        if dashes is not None:
+            dashes = np.ascontiguousarray(dashes, np.float32)
+            d.stroke_dashes, d.n_stroke_dashes = _ptr(dashes), int(dashes.shape[0])
         sid = ctypes.c_int32()
         self._check(self.lib.pfh_scene_upload(self.h, ctypes.byref(d), ctypes.byref(sid)), "pfh_scene_upload")
         self._keep[sid.value] = (int(scene.width), int(scene.height), scene.n_paths)
         return sid.value
 
-    def stroke_outlines(self, points, point_flags, contour_start, contour_closed, path_contour_start, styles):
+    def stroke_outlines(self, points, point_flags, contour_start, contour_closed, path_contour_start, styles,
+                        dashes=None):
         """pfh_stroke_outlines: every path stroked on the device with its style (rows of
-        (line_width, line_cap, line_join, miter_limit)).  Returns (points [N, 2] f32, point_flags u8,
-        contour_start i32, contour_closed u8, path_contour_start i32)."""
+        (line_width, line_cap, line_join, miter_limit[, dash_start, dash_count, dash_offset])),
+        dashed first where dash_count > 0 with the lengths `dashes`.  Returns (points [N, 2] f32,
+        point_flags u8, contour_start i32, contour_closed u8, path_contour_start i32)."""
         pts = np.ascontiguousarray(points, np.float32).reshape(-1, 2)
         flg = np.ascontiguousarray(point_flags, np.uint8)
         cst = np.ascontiguousarray(contour_start, np.int32)
         ccl = np.ascontiguousarray(contour_closed, np.uint8)
         pcs = np.ascontiguousarray(path_contour_start, np.int32)
         st = np.ascontiguousarray(stroke_styles(styles))
+        dsh = None if dashes is None else np.ascontiguousarray(dashes, np.float32)
+        nd = 0 if dsh is None else int(dsh.shape[0])
         n = int(pcs.shape[0] - 1)
         o = Outline(int(ccl.shape[0]), int(flg.shape[0]), _ptr(pts), _ptr(flg), _ptr(cst), _ptr(ccl))
         out_pcs = np.zeros(n + 1, np.int32)
         cs0 = np.zeros(1, np.int32)
         buf = OutlineBuf(None, None, 0, _ptr(cs0), None, 0, _ptr(out_pcs), 0, 0)
-        rc = self.lib.pfh_stroke_outlines(self.h, ctypes.byref(o), _ptr(pcs), n, _ptr(st), ctypes.byref(buf))
+        rc = self.lib.pfh_stroke_outlines(self.h, ctypes.byref(o), _ptr(pcs), n, _ptr(st), _ptr(dsh), nd,
+                                          ctypes.byref(buf))
         if rc not in (0, -5):
             self._check(rc, "pfh_stroke_outlines")
         out = (np.zeros((buf.n_points, 2), np.float32), np.zeros(buf.n_points, np.uint8),
                np.zeros(buf.n_contours + 1, np.int32), np.zeros(buf.n_contours, np.uint8))
         buf = OutlineBuf(_ptr(out[0]), _ptr(out[1]), buf.n_points, _ptr(out[2]), _ptr(out[3]), buf.n_contours,
                          _ptr(out_pcs), 0, 0)
-        self._check(self.lib.pfh_stroke_outlines(self.h, ctypes.byref(o), _ptr(pcs), n, _ptr(st), ctypes.byref(buf)),
-                    "pfh_stroke_outlines")
+        self._check(self.lib.pfh_stroke_outlines(self.h, ctypes.byref(o), _ptr(pcs), n, _ptr(st), _ptr(dsh), nd,
+                                                 ctypes.byref(buf)), "pfh_stroke_outlines")
         return out + (out_pcs,)
 
     def release(self, sid: int):

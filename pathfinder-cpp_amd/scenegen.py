@@ -54,9 +54,12 @@ class Scene:
     # clip path of draw path p as an index among them, or -1 (None: no path is clipped).
     n_clip_paths: int = 0
     path_clip: np.ndarray = None
-    # Strokes (Canvas::stroke_path): None, or per path (line_width, line_cap, line_join, miter_limit);
-    # a path with line_width > 0 is stroked at upload and its stroke filled with the nonzero rule.
+    # Strokes (Canvas::stroke_path): None, or per path (line_width, line_cap, line_join, miter_limit)
+    # and optionally (dash_start, dash_count, dash_offset); a path with line_width > 0 is stroked at
+    # upload (dashed first with stroke_dashes[dash_start:dash_start + dash_count] when dash_count > 0)
+    # and its stroke filled with the nonzero rule.
     path_stroke: np.ndarray = None
+    stroke_dashes: np.ndarray = None
 
     def __post_init__(self):
         if self.view_box is None:
@@ -64,7 +67,10 @@ class Scene:
         if self.path_clip is not None:
             self.path_clip = np.ascontiguousarray(self.path_clip, np.int32)
         if self.path_stroke is not None:
-            self.path_stroke = np.ascontiguousarray(self.path_stroke, np.float32).reshape(-1, 4)
+            ps = np.asarray(self.path_stroke, np.float32)
+            self.path_stroke = np.ascontiguousarray(ps.reshape(-1, ps.shape[-1] if ps.ndim == 2 else 4))
+        if self.stroke_dashes is not None:
+            self.stroke_dashes = np.ascontiguousarray(self.stroke_dashes, np.float32)
         self.points = np.ascontiguousarray(self.points, np.float32).reshape(-1, 2)
         self.point_flags = np.ascontiguousarray(self.point_flags, np.uint8)
         self.contour_start = np.ascontiguousarray(self.contour_start, np.int32)
@@ -234,17 +240,25 @@ def clip_scene(seed: int, size: int = 1024, n_draw: int = 200, n_clip: int = 8, 
 
 def concat_paths(a: Scene, b: Scene) -> Scene:
     """The paths of a followed by those of b, on a's canvas (stroke styles carried, clip information not)."""
-    strokes = None
+    strokes, dashes = None, None
     if a.path_stroke is not None or b.path_stroke is not None:
-        strokes = np.concatenate([a.path_stroke if a.path_stroke is not None else np.zeros((a.n_paths, 4), np.float32),
-                                  b.path_stroke if b.path_stroke is not None else np.zeros((b.n_paths, 4), np.float32)])
+        def cols7(sc):
+            ps = sc.path_stroke if sc.path_stroke is not None else np.zeros((sc.n_paths, 4), np.float32)
+            return np.concatenate([ps, np.zeros((ps.shape[0], 7 - ps.shape[1]), np.float32)], axis=1)
+
+        sa, sb = cols7(a), cols7(b)
+        da = a.stroke_dashes if a.stroke_dashes is not None else np.zeros(0, np.float32)
+        db = b.stroke_dashes if b.stroke_dashes is not None else np.zeros(0, np.float32)
+        sb[:, 4] += np.where(sb[:, 5] > 0, da.shape[0], 0)
+        strokes = np.concatenate([sa, sb])
+        dashes = np.concatenate([da, db]) if da.shape[0] + db.shape[0] else None
     return Scene(a.width, a.height, np.concatenate([a.points, b.points]),
                  np.concatenate([a.point_flags, b.point_flags]),
                  np.concatenate([a.contour_start, b.contour_start[1:] + a.contour_start[-1]]),
                  np.concatenate([a.contour_closed, b.contour_closed]),
                  np.concatenate([a.path_contour_start, b.path_contour_start[1:] + a.path_contour_start[-1]]),
                  np.concatenate([a.path_fill_rule, b.path_fill_rule]), np.concatenate([a.path_rgba, b.path_rgba]),
-                 a.view_box.copy(), path_stroke=strokes)
+                 a.view_box.copy(), path_stroke=strokes, stroke_dashes=dashes)
 
 
 def stress_clip_scene(seed: int = BASE_SEED ^ 4, size: int = 4096, n_stars: int = 2000, n_max: int = 2000,
@@ -361,6 +375,24 @@ def stroke_scene(seed: int, size: int = 4096, n: int = 500) -> Scene:
     sc = outlines_scene(size, size, [o for o, _ in lines], np.zeros(n, np.uint8),
                         _pack_rgba(rgb[:, 0], rgb[:, 1], rgb[:, 2], alpha))
     sc.path_stroke = np.array([st for _, st in lines], np.float32)
+    return sc
+
+
+def dashed_stroke_scene(seed: int, size: int = 4096, n: int = 500, dashed: float = 0.5) -> Scene:
+    """stroke_scene with a fraction `dashed` of the paths dashed before stroking (Canvas::stroke_path,
+    core/canvas.cpp:284-289): each such path gets a pattern of 2 or 4 lengths U(2, 40) px and an
+    offset U(0, 30) px (OutlineDash's offset argument)."""
+    sc = stroke_scene(seed, size, n)
+    rng = _rng(seed ^ 0xDA5)
+    styles = np.concatenate([sc.path_stroke, np.zeros((n, 3), np.float32)], axis=1)
+    dashes = []
+    for p in range(n):
+        if rng.random() < dashed:
+            k = int(rng.choice([2, 4]))
+            styles[p, 4], styles[p, 5], styles[p, 6] = len(dashes), k, rng.uniform(0.0, 30.0)
+            dashes.extend(rng.uniform(2.0, 40.0, k).tolist())
+    sc.path_stroke = styles
+    sc.stroke_dashes = np.array(dashes, np.float32)
     return sc
 
 

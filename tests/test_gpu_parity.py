@@ -376,6 +376,14 @@ def test_stroked_polylines(ctx):
 
 
 @pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED])
+def test_dashed_stroked_polylines(ctx, flags):
+    """Stroked cubic polylines, half of them dashed before stroking (the library's dasher and
+    stroker, bit-exact with the reference's) through the pipeline: tiles bit-exact, masks and pixels
+    within tolerance."""
+    check_scene(ctx, scenegen.dashed_stroke_scene(63, size=1024, n=120), flags=flags, name="dashed strokes")
+
+
+@pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED])
 def test_stress_scene_full_size(ctx, flags):
     """The stress workload at its full size: 4096², 2 000 star polygons of 100-2 000 vertices, 500
     stroked cubic polylines of width 1-20, 200 clip paths nesting up to 4 deep, half of the draw
