@@ -92,6 +92,11 @@ def load_ref():
         lib.pfref_dash.restype = _P
         lib.pfref_dash.argtypes = [_P] * 4 + [ctypes.c_int, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_float]
+        lib.pfref_svg.restype = _P
+        lib.pfref_svg.argtypes = [ctypes.c_char_p, ctypes.c_int64]
+        lib.pfref_svg_counts.argtypes = [_P, _P]
+        lib.pfref_svg_fetch.argtypes = [_P] * 15
+        lib.pfref_svg_free.argtypes = [_P]
         _ref = lib
     return _ref
 
@@ -320,6 +325,31 @@ def ref_dash(points, point_flags, contour_start, contour_closed, dashes, offset=
     lib.pfref_stroke_fetch(h, _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), _ptr(out[3]))
     lib.pfref_stroke_free(h)
     return out
+
+
+def ref_svg(text: bytes):
+    """The reference's SVG ingest (nanosvg + SvgScene's steps, core/svg.cpp:128-206), compiled from its
+    sources, on one SVG document.  Returns a dict: size (w, h); per draw path `path_i32` rows (kind 0
+    fill / 1 stroke, fill rule, nanosvg paint type, rgba, cap, join, dash start, dash count) and
+    `path_f32` rows (stroke width, miter limit); `dashes`; the outlines before stroking (`src_*`,
+    paths delimited by src_pcs) and the outlines the scene receives (`out_*`, out_pcs)."""
+    lib = load_ref()
+    h = lib.pfref_svg(text, len(text))
+    if not h:
+        raise ValueError("nanosvg could not parse the document")
+    c = np.zeros(6, np.int64)
+    lib.pfref_svg_counts(h, _ptr(c))
+    n, sp, sc, op, oc, nd = [int(v) for v in c]
+    r = dict(size=np.zeros(2, np.float32), path_i32=np.zeros((n, 8), np.int32), path_f32=np.zeros((n, 2), np.float32),
+             dashes=np.zeros(nd, np.float32), src_pcs=np.zeros(n + 1, np.int32), src_points=np.zeros((sp, 2), np.float32),
+             src_flags=np.zeros(sp, np.uint8), src_cs=np.zeros(sc + 1, np.int32), src_closed=np.zeros(sc, np.uint8),
+             out_pcs=np.zeros(n + 1, np.int32), out_points=np.zeros((op, 2), np.float32),
+             out_flags=np.zeros(op, np.uint8), out_cs=np.zeros(oc + 1, np.int32), out_closed=np.zeros(oc, np.uint8))
+    lib.pfref_svg_fetch(h, *[_ptr(r[k]) for k in ("size", "path_i32", "path_f32", "dashes", "src_pcs", "src_points",
+                                                   "src_flags", "src_cs", "src_closed", "out_pcs", "out_points",
+                                                   "out_flags", "out_cs", "out_closed")])
+    lib.pfref_svg_free(h)
+    return r
 
 
 def ref_stroke(points, point_flags, contour_start, contour_closed, width, cap=0, join=0, miter_limit=10.0):

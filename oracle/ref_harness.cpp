@@ -11,6 +11,7 @@
 #include <cstring>
 #include <chrono>
 #include <thread>
+#include <string>
 #include <vector>
 
 #include "core/d3d9/scene_builder.h"
@@ -19,6 +20,12 @@
 #include "core/scene.h"
 #include "core/stroke.h"
 #include "core/dash.h"
+#include "core/path2d.h"
+
+// The reference's vendored SVG parser (third_party/nanosvg.h), compiled here as core/svg.cpp
+// compiles it.
+#define NANOSVG_IMPLEMENTATION
+#include <nanosvg.h>
 
 namespace Pathfinder {
 // Defined with external linkage in core/d3d9/scene_builder.cpp:18 (not in a header).
@@ -401,6 +408,133 @@ void *pfref_dash(const float *points, const uint8_t *point_flags, const int32_t 
         r->closed.push_back(c.closed ? 1 : 0);
     }
     return r;
+}
+
+// The reference's SVG ingest (SvgScene::SvgScene, core/svg.cpp:128-206): nanosvg parses the text
+// ("px", 96 dpi) into shapes whose paths are cubic chains; each shape becomes a Path2d (move_to,
+// cubic_to, close_path), filled when its fill paint is visible (Canvas::fill_path,
+// core/canvas.cpp:265-270) and stroked when its stroke paint is visible and its width > 0, dashed
+// first with offset 0 when it has a dash array (Canvas::stroke_path, core/canvas.cpp:272-301).
+// The Canvas itself is not linked (it pulls in the GPU renderers); these are its steps for an
+// identity transform, no clip, no shadow.  Per draw path: the outline before stroking, the style,
+// and the outline the scene receives.
+struct RefSvg {
+    float size[2] = {0, 0};
+    std::vector<int32_t> path_i32;  // kind (0 fill, 1 stroke), fill rule, paint type, rgba, cap, join, dash start, count
+    std::vector<float> path_f32;    // width, miter limit
+    std::vector<float> dashes;
+    RefStroke src, out;
+    std::vector<int32_t> src_pcs{0}, out_pcs{0};
+};
+
+static void append_outline(RefStroke &dst, std::vector<int32_t> &pcs, const Outline &o) {
+    for (const Contour &c : o.contours) {
+        for (size_t i = 0; i < c.points.size(); i++) {
+            dst.points.push_back(c.points[i].x);
+            dst.points.push_back(c.points[i].y);
+            dst.flags.push_back((uint8_t)c.flags[i]);
+        }
+        dst.contour_start.push_back((int32_t)dst.flags.size());
+        dst.closed.push_back(c.closed ? 1 : 0);
+    }
+    pcs.push_back((int32_t)dst.closed.size());
+}
+
+void *pfref_svg(const char *text, int64_t len) {
+    std::string copy(text, (size_t)len);  // nsvgParse edits its input
+    NSVGimage *image = nsvgParse((char *)copy.data(), "px", 96);
+    if (!image) return nullptr;
+    auto *r = new RefSvg();
+    r->size[0] = image->width;
+    r->size[1] = image->height;
+    // convert_nsvg_paint (core/svg.cpp:64-124) for visibility: a colour paint is visible when its
+    // alpha is not 0 (ColorU::is_visible); a gradient's base colour is opaque white
+    auto paint_type = [](const NSVGpaint &p) { return (int32_t)p.type; };
+    auto visible = [](const NSVGpaint &p) {
+        if (p.type == NSVG_PAINT_COLOR) return (p.color >> 24) != 0;
+        return p.type == NSVG_PAINT_LINEAR_GRADIENT || p.type == NSVG_PAINT_RADIAL_GRADIENT;
+    };
+    for (NSVGshape *sh = image->shapes; sh; sh = sh->next) {
+        Path2d path;
+        for (NSVGpath *np = sh->paths; np; np = np->next) {
+            path.move_to(np->pts[0], np->pts[1]);
+            for (int i = 0; i < np->npts - 3; i += 3) {
+                const float *q = &np->pts[i * 2];
+                path.cubic_to(q[2], q[3], q[4], q[5], q[6], q[7]);
+            }
+            if (np->closed) path.close_path();
+        }
+        if (visible(sh->fill)) {
+            Outline o = path.into_outline();
+            append_outline(r->src, r->src_pcs, o);
+            append_outline(r->out, r->out_pcs, o);
+            const int32_t rule = sh->fillRule == NSVG_FILLRULE_EVENODD ? 1 : 0;
+            r->path_i32.insert(r->path_i32.end(), {0, rule, paint_type(sh->fill), (int32_t)sh->fill.color, 0, 0, 0, 0});
+            r->path_f32.insert(r->path_f32.end(), {0.0f, 0.0f});
+        }
+        const float width = sh->strokeWidth;
+        if (visible(sh->stroke) && width > 0.0f) {
+            const int32_t cap = sh->strokeLineCap == NSVG_CAP_ROUND ? 2 : sh->strokeLineCap == NSVG_CAP_SQUARE ? 1 : 0;
+            const int32_t join =
+                sh->strokeLineJoin == NSVG_JOIN_ROUND ? 2 : sh->strokeLineJoin == NSVG_JOIN_BEVEL ? 1 : 0;
+            Outline o = path.into_outline();
+            append_outline(r->src, r->src_pcs, o);
+            const int32_t d0 = (int32_t)r->dashes.size(), dn = sh->strokeDashCount;
+            if (dn > 0) {
+                std::vector<float> pattern(sh->strokeDashArray, sh->strokeDashArray + dn);
+                r->dashes.insert(r->dashes.end(), pattern.begin(), pattern.end());
+                OutlineDash dasher(o, pattern, 0);
+                dasher.dash();
+                o = dasher.into_outline();
+            }
+            StrokeStyle style;
+            style.line_width = width;
+            style.line_cap = (LineCap)cap;
+            style.line_join = (LineJoin)join;
+            style.miter_limit = sh->miterLimit;
+            OutlineStrokeToFill stroker(o, style);
+            stroker.offset();
+            append_outline(r->out, r->out_pcs, stroker.into_outline());
+            r->path_i32.insert(r->path_i32.end(),
+                               {1, 0, paint_type(sh->stroke), (int32_t)sh->stroke.color, cap, join, d0, dn});
+            r->path_f32.insert(r->path_f32.end(), {width, sh->miterLimit});
+        }
+    }
+    nsvgDelete(image);
+    return r;
+}
+
+// [paths, source points, source contours, output points, output contours, dash lengths]
+void pfref_svg_counts(void *h, int64_t *counts) {
+    auto *r = (RefSvg *)h;
+    counts[0] = (int64_t)r->path_f32.size() / 2;
+    counts[1] = (int64_t)r->src.flags.size();
+    counts[2] = (int64_t)r->src.closed.size();
+    counts[3] = (int64_t)r->out.flags.size();
+    counts[4] = (int64_t)r->out.closed.size();
+    counts[5] = (int64_t)r->dashes.size();
+}
+
+void pfref_svg_fetch(void *h, float *size, int32_t *path_i32, float *path_f32, float *dashes, int32_t *src_pcs,
+                     float *src_points, uint8_t *src_flags, int32_t *src_cs, uint8_t *src_closed, int32_t *out_pcs,
+                     float *out_points, uint8_t *out_flags, int32_t *out_cs, uint8_t *out_closed) {
+    auto *r = (RefSvg *)h;
+    size[0] = r->size[0];
+    size[1] = r->size[1];
+    auto cp = [](void *dst, const void *src, size_t n) {
+        if (n) memcpy(dst, src, n);
+    };
+    cp(path_i32, r->path_i32.data(), r->path_i32.size() * 4);
+    cp(path_f32, r->path_f32.data(), r->path_f32.size() * 4);
+    cp(dashes, r->dashes.data(), r->dashes.size() * 4);
+    cp(src_pcs, r->src_pcs.data(), r->src_pcs.size() * 4);
+    pfref_stroke_fetch(&r->src, src_points, src_flags, src_cs, src_closed);
+    cp(out_pcs, r->out_pcs.data(), r->out_pcs.size() * 4);
+    pfref_stroke_fetch(&r->out, out_points, out_flags, out_cs, out_closed);
+}
+
+void pfref_svg_free(void *h) {
+    delete (RefSvg *)h;
 }
 
 } // extern "C"

@@ -1,0 +1,68 @@
+"""Real-world scenes on the device: the reference's tiger.svg and features.svg as its SVG ingest
+delivers them (tests/golden/svg_fixtures.npz; tests/svg_scenes.py builds the scenes).
+
+* The device stroker and dasher (pfh_stroke_outlines) on every stroke of both documents, all in one
+  call, equal the reference's outlines bit for bit.
+* The source scenes (strokes stroked at upload on the device) through the pipeline, plain and
+  batched: tiles, fills, backdrops and order bit-exact with the oracle, masks within 1/255, RGBA8
+  within 1 LSB; and the same pixels as the reference's pre-stroked scene uploaded as plain fills.
+* The shipped predictor on tiger, a path-length distribution it was not trained on (hand-drawn
+  curves, 0.2-px hairline strokes): the totals of chords, fills and alpha tiles within 25 %.
+"""
+import numpy as np
+import pytest
+
+import pfh
+import predictor
+import svg_scenes
+from test_gpu_parity import check_scene
+from test_gpu_stroke import as_bytes, batch, split
+
+pytestmark = pytest.mark.gpu
+KEYS = ("points", "point_flags", "contour_start", "contour_closed")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pfh.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name", ["tiger", "features"])
+def test_device_strokes_match_reference_ingest(ctx, name):
+    st = svg_scenes.strokes(name)
+    dashes, styles = [], []
+    for _, style, pat, _ in st:
+        if pat is None:
+            styles.append(style + (0, 0, 0.0))
+        else:
+            styles.append(style + (len(dashes), len(pat), 0.0))
+            dashes.extend(pat.tolist())
+    out = ctx.stroke_outlines(*batch([s[0] for s in st]), styles,
+                              dashes=np.array(dashes, np.float32) if dashes else None)
+    for k, (_, _, _, ref) in enumerate(st):
+        for key, g, r in zip(KEYS, split(out, k), ref):
+            np.testing.assert_array_equal(as_bytes(g), as_bytes(r), err_msg=f"{name} stroke {k}: {key}")
+
+
+@pytest.mark.parametrize("name", ["tiger", "features"])
+@pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED])
+def test_svg_scene_parity(ctx, name, flags):
+    _, img = check_scene(ctx, svg_scenes.scene_src(name), flags=flags, name=f"{name}.svg")
+    sid = ctx.upload(svg_scenes.scene_out(name))
+    ctx.rasterize([sid], flags)
+    np.testing.assert_array_equal(ctx.readback(sid), img)
+    ctx.release(sid)
+
+
+def test_shipped_predictor_on_tiger(ctx):
+    w = predictor.load()
+    assert w is not None, "predictor weights missing"
+    feats, counts = predictor.collect(ctx, [svg_scenes.scene_out("tiger")])
+    pred = predictor.predict_totals(ctx, w, feats)
+    exact = counts.sum(0).astype(np.float64)
+    rel = np.abs(pred - exact) / exact
+    print(f"[predictor] tiger ({feats.shape[0]} paths): exact {exact}, predicted {np.round(pred)}, "
+          f"rel err {np.round(rel, 4)}", flush=True)
+    assert (rel < 0.25).all(), rel
