@@ -82,6 +82,25 @@ typedef struct pfh_stroke_style {
 } pfh_stroke_style;
 
 
+/* Gradient paints (Gradient, core/paint/gradient.h:101-176; Paint::from_gradient, core/paint/paint.h:87-96:
+ * white base colour, the gradient combined "source in").  A linear gradient runs along `line`; a
+ * radial one between the circles (line.from, radii[0]) and (line.to, radii[1]) of gradient space,
+ * which `transform` maps to the scene.  Stops: offset and colour (r | g << 8 | b << 16 | a << 24),
+ * added in the order given (Gradient::add, core/paint/gradient.cpp:7-16). */
+#define PFH_GRADIENT_LINEAR 0
+#define PFH_GRADIENT_RADIAL 1
+#define PFH_WRAP_CLAMP 0       /* GradientWrap::Clamp */
+#define PFH_WRAP_REPEAT 1      /* GradientWrap::Repeat */
+typedef struct pfh_gradient {
+    int32_t kind;              /* PFH_GRADIENT_* */
+    int32_t wrap;              /* PFH_WRAP_* */
+    float line[4];             /* from x, y, to x, y */
+    float radii[2];            /* radial */
+    float transform[6];        /* radial: GradientRadial::transform, Transform2 as m11, m21, m12, m22, m13, m23:
+                                * (x, y) -> (m11 x + m12 y + m13, m21 x + m22 y + m23); identity 1 0 0 1 0 0 */
+    int32_t stop_start, stop_count;  /* into the stop arrays */
+} pfh_gradient;
+
 /*
  * A scene as outlines: the data of Scene::draw_paths (core/scene.h:57-61) after
  * Path2d/Canvas have applied transforms, strokes and dashes (core/canvas.cpp:180-301).
@@ -119,6 +138,17 @@ typedef struct pfh_scene_desc {
     /* The dash lengths that path_stroke's patterns index (NULL when no path is dashed). */
     const float *stroke_dashes;         /* [n_stroke_dashes] */
     int32_t n_stroke_dashes;
+    /* Gradient paints: NULL, or per path an index into `gradients` (-1: the solid path_rgba).  A
+     * gradient path is opaque for the z-buffer when every stop is (Paint::is_opaque,
+     * core/paint/paint.cpp:9-19); its colour per pixel follows the tile shader's colour texture
+     * (shaders/d3d9/tile.vert:65-77, tile.frag:296-323, 590-653) over the ramp and metadata
+     * pfh_gradient_compile makes. */
+    const int32_t *path_gradient;       /* [n_paths] */
+    const pfh_gradient *gradients;      /* [n_gradients] */
+    int32_t n_gradients;
+    const float *gradient_stop_offsets; /* [n_gradient_stops] */
+    const uint32_t *gradient_stop_rgba; /* [n_gradient_stops] */
+    int32_t n_gradient_stops;
 } pfh_scene_desc;
 
 typedef struct pfh_stats {
@@ -306,6 +336,21 @@ typedef struct pfh_outline_buf {
     int32_t *path_contour_start; /* [n_paths + 1] */
     int64_t n_points, n_contours;
 } pfh_outline_buf;
+
+/*
+ * A gradient as the library draws it, on the host (no context): its ramp, the RAMP_TEXELS = 256
+ * RGBA8 texels GradientTileBuilder::allocate samples (core/paint/gradient.cpp:86-123 with
+ * Gradient::sample, :23-56), and its 16-float record: the colour texture transform (m11, m21, m12,
+ * m22, m13, m23) and the radial filter's line from x, y, vector x, y, radii, uv origin x, y, as the
+ * RGBA16F metadata texture holds them (Palette::calculate_texture_transforms,
+ * core/paint/palette.cpp:398-440; PaintMetadata::filter, core/paint/paint.cpp:45-65;
+ * Renderer::upload_texture_metadata, core/renderer.cpp:167-250, through third_party/umHalf's
+ * truncating float-to-half), then kind (1 linear, 2 radial) and wrap as floats.  ramp_row: the
+ * gradient's row in its 256-row texture page (the v coordinate).  PFH_E_ARG for a bad kind or
+ * wrap, non-finite geometry or stop offsets, or a radial transform that cannot be inverted.
+ */
+int pfh_gradient_compile(const pfh_gradient *g, const float *stop_offsets, const uint32_t *stop_rgba,
+                         int32_t ramp_row, uint32_t *ramp, float *record);
 
 int pfh_stroke_outlines(pfh_ctx *ctx, const pfh_outline *in, const int32_t *path_contour_start, int32_t n_paths,
                         const pfh_stroke_style *styles, const float *dashes, int32_t n_dashes, pfh_outline_buf *out);

@@ -54,6 +54,14 @@ def load_oracle():
         lib.pfo_fill_area.restype = ctypes.c_double
         lib.pfo_fill_area.argtypes = [_P, ctypes.c_int, ctypes.c_int]
         lib.pfo_coverage.argtypes = [ctypes.c_int64, _P, _P, _P, _P, _P]
+        lib.pfo_half_value.restype = ctypes.c_double
+        lib.pfo_half_value.argtypes = [ctypes.c_float]
+        lib.pfo_gradient_ramp.argtypes = [_P, _P, ctypes.c_int, _P]
+        lib.pfo_gradient_record.restype = ctypes.c_int
+        lib.pfo_gradient_record.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, ctypes.c_int, _P]
+        lib.pfo_paint_colour.restype = ctypes.c_int
+        lib.pfo_paint_colour.argtypes = [_P, _P, ctypes.c_float, ctypes.c_float, _P]
+        lib.pfo_composite_paint.argtypes = [ctypes.c_int64] + [_P] * 10 + [ctypes.c_int] * 3 + [_P]
         lib.pfo_coverage_raw.argtypes = [ctypes.c_int64, _P, _P, _P, _P]
         lib.pfo_apply_rule.restype = ctypes.c_double
         lib.pfo_apply_rule.argtypes = [ctypes.c_double, ctypes.c_int]
@@ -92,11 +100,16 @@ def load_ref():
         lib.pfref_dash.restype = _P
         lib.pfref_dash.argtypes = [_P] * 4 + [ctypes.c_int, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_float]
+        lib.pfref_gradient_ramp.argtypes = [_P, _P, ctypes.c_int, _P]
+        lib.pfref_half_bits.argtypes = [_P, ctypes.c_int64, _P]
+        lib.pfref_transform_inverse.argtypes = [_P, _P]
         lib.pfref_svg.restype = _P
         lib.pfref_svg.argtypes = [ctypes.c_char_p, ctypes.c_int64]
         lib.pfref_svg_counts.argtypes = [_P, _P]
         lib.pfref_svg_fetch.argtypes = [_P] * 15
         lib.pfref_svg_free.argtypes = [_P]
+        lib.pfref_svg_gradient_counts.argtypes = [_P, _P]
+        lib.pfref_svg_gradient_fetch.argtypes = [_P] * 6
         _ref = lib
     return _ref
 
@@ -204,6 +217,8 @@ def apply_strokes(scene):
     out.view_box = scene.view_box.copy()
     out.n_clip_paths = scene.n_clip_paths
     out.path_clip = scene.path_clip
+    for k in ("path_gradient", "gradients", "gradient_stop_offsets", "gradient_stop_rgba"):
+        setattr(out, k, getattr(scene, k, None))
     return out
 
 
@@ -348,8 +363,42 @@ def ref_svg(text: bytes):
     lib.pfref_svg_fetch(h, *[_ptr(r[k]) for k in ("size", "path_i32", "path_f32", "dashes", "src_pcs", "src_points",
                                                    "src_flags", "src_cs", "src_closed", "out_pcs", "out_points",
                                                    "out_flags", "out_cs", "out_closed")])
+    g = np.zeros(2, np.int64)
+    lib.pfref_svg_gradient_counts(h, _ptr(g))
+    ng, ns = int(g[0]), int(g[1])
+    r.update(path_grad=np.zeros(n, np.int32), grad_i32=np.zeros((ng, 4), np.int32),
+             grad_f32=np.zeros((ng, 12), np.float32), stop_off=np.zeros(ns, np.float32),
+             stop_rgba=np.zeros(ns, np.uint32))
+    lib.pfref_svg_gradient_fetch(h, *[_ptr(r[k]) for k in ("path_grad", "grad_i32", "grad_f32", "stop_off",
+                                                            "stop_rgba")])
     lib.pfref_svg_free(h)
     return r
+
+
+def ref_gradient_ramp(offsets, rgba) -> np.ndarray:
+    """The reference's ramp (Gradient::add_color_stop + GradientTileBuilder::allocate), 256 u32 texels."""
+    off = np.ascontiguousarray(offsets, np.float32)
+    col = np.ascontiguousarray(rgba, np.uint32)
+    out = np.zeros(256, np.uint32)
+    load_ref().pfref_gradient_ramp(_ptr(off) if off.size else None, _ptr(col) if col.size else None,
+                                   int(off.shape[0]), _ptr(out))
+    return out
+
+
+def ref_half_bits(values) -> np.ndarray:
+    """umHalf's float-to-half conversion (the reference's metadata texture): the halves' bits."""
+    v = np.ascontiguousarray(values, np.float32)
+    out = np.zeros(v.shape[0], np.uint16)
+    load_ref().pfref_half_bits(_ptr(v), int(v.shape[0]), _ptr(out))
+    return out
+
+
+def ref_transform_inverse(t) -> np.ndarray:
+    """Transform2::inverse of (m11, m21, m12, m22, m13, m23)."""
+    v = np.ascontiguousarray(t, np.float32)
+    out = np.zeros(6, np.float32)
+    load_ref().pfref_transform_inverse(_ptr(v), _ptr(out))
+    return out
 
 
 def ref_stroke(points, point_flags, contour_start, contour_closed, width, cap=0, join=0, miter_limit=10.0):
@@ -408,9 +457,84 @@ def coverage_fp64(at: dict, raw: bool = False) -> np.ndarray:
     return out
 
 
+def gradient_ramp(offsets, rgba) -> np.ndarray:
+    """The oracle's ramp of a gradient (256 RGBA8 texels as u32), stops in the order given."""
+    off = np.ascontiguousarray(offsets, np.float32)
+    col = np.ascontiguousarray(rgba, np.uint32)
+    out = np.zeros(256, np.uint32)
+    load_oracle().pfo_gradient_ramp(_ptr(off) if off.size else None, _ptr(col) if col.size else None,
+                                    int(off.shape[0]), _ptr(out))
+    return out
+
+
+def gradient_record(g, row: int = 0) -> np.ndarray:
+    """The oracle's 16-float record of one pfh_gradient-shaped record (a numpy structured scalar with
+    kind, wrap, line, radii, transform), as the half metadata texture holds it."""
+    line = np.ascontiguousarray(g["line"], np.float32)
+    radii = np.ascontiguousarray(g["radii"], np.float32)
+    xf = np.ascontiguousarray(g["transform"], np.float32)
+    rec = np.zeros(16, np.float32)
+    rc = load_oracle().pfo_gradient_record(int(g["kind"]), int(g["wrap"]), _ptr(line), _ptr(radii), _ptr(xf),
+                                           int(row), _ptr(rec))
+    if rc != 0:
+        raise ValueError("radial gradient transform without inverse")
+    return rec
+
+
+def half_value(v: float) -> float:
+    return load_oracle().pfo_half_value(float(v))
+
+
+def paint_colour(rec, ramp, px: float, py: float):
+    """The oracle's un-premultiplied colour of a gradient paint at pixel centre (px, py), or None."""
+    out = np.zeros(4, np.float64)
+    r = np.ascontiguousarray(rec, np.float32)
+    rp = np.ascontiguousarray(ramp, np.uint32)
+    ok = load_oracle().pfo_paint_colour(_ptr(r), _ptr(rp), float(px), float(py), _ptr(out))
+    return out if ok else None
+
+
+def scene_paints(scene):
+    """(path_paint i32 [n_paths] (-1 solid), records [G, 16] f32, ramps [G, 256] u32) of a scene's
+    gradient paints, each used gradient compiled once in order of first use (its ramp row: its
+    index modulo 256), or None without gradients."""
+    pg = getattr(scene, "path_gradient", None)
+    if pg is None:
+        return None
+    slot, recs, ramps = {}, [], []
+    paint = np.full(scene.n_paths, -1, np.int32)
+    for p in range(scene.n_paths):
+        gi = int(pg[p])
+        if gi < 0:
+            continue
+        if gi not in slot:
+            g = scene.gradients[gi]
+            a, n = int(g["stop_start"]), int(g["stop_count"])
+            slot[gi] = len(recs)
+            recs.append(gradient_record(g, len(recs) % 256))
+            ramps.append(gradient_ramp(scene.gradient_stop_offsets[a:a + n], scene.gradient_stop_rgba[a:a + n]))
+        paint[p] = slot[gi]
+    if not recs:
+        return None
+    return paint, np.ascontiguousarray(np.stack(recs)), np.ascontiguousarray(np.stack(ramps))
+
+
 def composite_fp64(canon: dict, at: dict, coverage: np.ndarray, scene) -> np.ndarray:
     """FP64 premultiplied RGBA [H, W, 4] of the scene from the canonical entries."""
     lib = load_oracle()
+    paints = scene_paints(scene)
+    if paints is not None:
+        ent_tile = canon["entry_tile"]
+        ent_alpha = np.ascontiguousarray(at["alpha_index"][ent_tile])
+        ent_bd = np.ascontiguousarray(canon["tile_backdrop"][ent_tile])
+        fbw = int(canon["fb_rect"][2] - canon["fb_rect"][0])
+        out = np.zeros((scene.height, scene.width, 4), np.float64)
+        cov = np.ascontiguousarray(coverage, np.float64)
+        lib.pfo_composite_paint(ctypes.c_int64(ent_tile.shape[0]), _ptr(canon["entry_fbtile"]),
+                                _ptr(canon["entry_path"]), _ptr(ent_alpha), _ptr(ent_bd), _ptr(scene.path_fill_rule),
+                                _ptr(scene.path_rgba), _ptr(paints[0]), _ptr(paints[1]), _ptr(paints[2]), _ptr(cov),
+                                fbw, scene.width, scene.height, _ptr(out))
+        return out
     ent_tile = canon["entry_tile"]
     ent_alpha = np.ascontiguousarray(at["alpha_index"][ent_tile])
     ent_bd = np.ascontiguousarray(canon["tile_backdrop"][ent_tile])

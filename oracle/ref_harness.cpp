@@ -22,6 +22,10 @@
 #include "core/dash.h"
 #include "core/path2d.h"
 
+#include "core/paint/gradient.h"
+#include "core/paint/texture_allocator.h"
+#include <umHalf.h>
+
 // The reference's vendored SVG parser (third_party/nanosvg.h), compiled here as core/svg.cpp
 // compiles it.
 #define NANOSVG_IMPLEMENTATION
@@ -422,6 +426,10 @@ struct RefSvg {
     float size[2] = {0, 0};
     std::vector<int32_t> path_i32;  // kind (0 fill, 1 stroke), fill rule, paint type, rgba, cap, join, dash start, count
     std::vector<float> path_f32;    // width, miter limit
+    // gradient paints: per draw path an index or -1; per gradient kind, wrap, stop start, stop count
+    // and line (4), radii (2), transform (6); stops (offset, rgba) in the order added
+    std::vector<int32_t> path_grad, grad_i32, stop_rgba;
+    std::vector<float> grad_f32, stop_off;
     std::vector<float> dashes;
     RefStroke src, out;
     std::vector<int32_t> src_pcs{0}, out_pcs{0};
@@ -440,6 +448,48 @@ static void append_outline(RefStroke &dst, std::vector<int32_t> &pcs, const Outl
     pcs.push_back((int32_t)dst.closed.size());
 }
 
+// convert_nsvg_paint's gradient branch (core/svg.cpp:75-121): a linear gradient's ends through the
+// path and gradient transforms, a radial one's centre-to-focus line with radii (0, r) and the
+// transforms kept as its transform; pad -> clamp, repeat -> repeat, reflect left at clamp.
+static int32_t add_gradient(RefSvg &r, const NSVGpaint &p) {
+    const NSVGgradient *ng = p.gradient;
+    auto gradient_xform = Transform2((float *)ng->xform2);
+    auto path_xform = Transform2((float *)ng->xform3);
+    Gradient g;
+    if (p.type == NSVG_PAINT_LINEAR_GRADIENT) {
+        Vec2F from = {ng->x1, ng->y1}, to = {ng->x2, ng->y2};
+        from = path_xform * gradient_xform * from;
+        to = path_xform * gradient_xform * to;
+        g = Gradient::linear(LineSegmentF(from, to));
+    } else {
+        Vec2F from = {ng->cx, ng->cy}, to = {ng->fx, ng->fy};
+        g = Gradient::radial(LineSegmentF(from, to), Vec2F(0.0, ng->r));
+        std::get<GradientRadial>(g.geometry).transform = path_xform * gradient_xform;
+    }
+    const int32_t wrap = ng->spread == NSVG_SPREAD_REPEAT ? 1 : 0;
+    const int32_t start = (int32_t)r.stop_off.size();
+    for (int i = 0; i < ng->nstops; i++) {
+        r.stop_off.push_back(ng->stops[i].offset);
+        r.stop_rgba.push_back((int32_t)ng->stops[i].color);
+    }
+    float f[12] = {0};
+    int32_t kind = 0;
+    if (auto *lin = std::get_if<GradientLinear>(&g.geometry)) {
+        f[0] = lin->line.from().x, f[1] = lin->line.from().y, f[2] = lin->line.to().x, f[3] = lin->line.to().y;
+        f[6] = 1, f[9] = 1;
+    } else {
+        const auto &rad = std::get<GradientRadial>(g.geometry);
+        kind = 1;
+        f[0] = rad.line.from().x, f[1] = rad.line.from().y, f[2] = rad.line.to().x, f[3] = rad.line.to().y;
+        f[4] = rad.radii.x, f[5] = rad.radii.y;
+        f[6] = rad.transform.m11(), f[7] = rad.transform.m21(), f[8] = rad.transform.m12();
+        f[9] = rad.transform.m22(), f[10] = rad.transform.m13(), f[11] = rad.transform.m23();
+    }
+    r.grad_i32.insert(r.grad_i32.end(), {kind, wrap, start, ng->nstops});
+    r.grad_f32.insert(r.grad_f32.end(), f, f + 12);
+    return (int32_t)(r.grad_i32.size() / 4) - 1;
+}
+
 void *pfref_svg(const char *text, int64_t len) {
     std::string copy(text, (size_t)len);  // nsvgParse edits its input
     NSVGimage *image = nsvgParse((char *)copy.data(), "px", 96);
@@ -450,6 +500,8 @@ void *pfref_svg(const char *text, int64_t len) {
     // convert_nsvg_paint (core/svg.cpp:64-124) for visibility: a colour paint is visible when its
     // alpha is not 0 (ColorU::is_visible); a gradient's base colour is opaque white
     auto paint_type = [](const NSVGpaint &p) { return (int32_t)p.type; };
+    // a colour paint's rgba (a gradient paint's union holds its gradient instead: 0)
+    auto colour_of = [](const NSVGpaint &p) { return p.type == NSVG_PAINT_COLOR ? (int32_t)p.color : 0; };
     auto visible = [](const NSVGpaint &p) {
         if (p.type == NSVG_PAINT_COLOR) return (p.color >> 24) != 0;
         return p.type == NSVG_PAINT_LINEAR_GRADIENT || p.type == NSVG_PAINT_RADIAL_GRADIENT;
@@ -469,7 +521,8 @@ void *pfref_svg(const char *text, int64_t len) {
             append_outline(r->src, r->src_pcs, o);
             append_outline(r->out, r->out_pcs, o);
             const int32_t rule = sh->fillRule == NSVG_FILLRULE_EVENODD ? 1 : 0;
-            r->path_i32.insert(r->path_i32.end(), {0, rule, paint_type(sh->fill), (int32_t)sh->fill.color, 0, 0, 0, 0});
+            r->path_i32.insert(r->path_i32.end(), {0, rule, paint_type(sh->fill), colour_of(sh->fill), 0, 0, 0, 0});
+            r->path_grad.push_back(sh->fill.type == NSVG_PAINT_COLOR ? -1 : add_gradient(*r, sh->fill));
             r->path_f32.insert(r->path_f32.end(), {0.0f, 0.0f});
         }
         const float width = sh->strokeWidth;
@@ -496,12 +549,48 @@ void *pfref_svg(const char *text, int64_t len) {
             stroker.offset();
             append_outline(r->out, r->out_pcs, stroker.into_outline());
             r->path_i32.insert(r->path_i32.end(),
-                               {1, 0, paint_type(sh->stroke), (int32_t)sh->stroke.color, cap, join, d0, dn});
+                               {1, 0, paint_type(sh->stroke), colour_of(sh->stroke), cap, join, d0, dn});
+            r->path_grad.push_back(sh->stroke.type == NSVG_PAINT_COLOR ? -1 : add_gradient(*r, sh->stroke));
             r->path_f32.insert(r->path_f32.end(), {width, sh->miterLimit});
         }
     }
     nsvgDelete(image);
     return r;
+}
+
+// The reference's gradient ramp: a Gradient with the stops added in the order given
+// (Gradient::add_color_stop), sampled into a texture row by GradientTileBuilder::allocate
+// (core/paint/gradient.cpp:86-123); writes its 256 texels as r | g << 8 | b << 16 | a << 24.
+void pfref_gradient_ramp(const float *offsets, const uint32_t *rgba, int n, uint32_t *out) {
+    Gradient g = Gradient::linear(LineSegmentF(Vec2F(0, 0), Vec2F(1, 0)));
+    for (int i = 0; i < n; i++) g.add_color_stop(ColorU(rgba[i]), offsets[i]);
+    TextureAllocator allocator;
+    GradientTileBuilder builder;
+    auto loc = builder.allocate(g, allocator, {});
+    const auto &tile = builder.tiles.back();
+    for (int x = 0; x < (int)GRADIENT_TILE_LENGTH; x++) {
+        const ColorU c = tile.texels[loc.rect.min_y() * GRADIENT_TILE_LENGTH + x];
+        out[x] = (uint32_t)c.r_ | (uint32_t)c.g_ << 8 | (uint32_t)c.b_ << 16 | (uint32_t)c.a_ << 24;
+    }
+}
+
+// The metadata texture's float-to-half conversion (third_party/umHalf, as core/renderer.cpp:167-250
+// uses it): the half's bits.
+void pfref_half_bits(const float *in, int64_t n, uint16_t *out) {
+    for (int64_t i = 0; i < n; i++) {
+        half h = in[i];
+        uint16_t bits;
+        memcpy(&bits, &h, 2);
+        out[i] = bits;
+    }
+}
+
+// Transform2::inverse (common/math/transform2.cpp:32-36) of m11, m21, m12, m22, m13, m23.
+void pfref_transform_inverse(const float *t, float *out) {
+    Transform2 x(Mat2(t[0], t[1], t[2], t[3]), Vec2F(t[4], t[5]));
+    Transform2 i = x.inverse();
+    const float v[6] = {i.m11(), i.m21(), i.m12(), i.m22(), i.m13(), i.m23()};
+    memcpy(out, v, sizeof(v));
 }
 
 // [paths, source points, source contours, output points, output contours, dash lengths]
@@ -531,6 +620,26 @@ void pfref_svg_fetch(void *h, float *size, int32_t *path_i32, float *path_f32, f
     pfref_stroke_fetch(&r->src, src_points, src_flags, src_cs, src_closed);
     cp(out_pcs, r->out_pcs.data(), r->out_pcs.size() * 4);
     pfref_stroke_fetch(&r->out, out_points, out_flags, out_cs, out_closed);
+}
+
+// [gradients, stops]
+void pfref_svg_gradient_counts(void *h, int64_t *counts) {
+    auto *r = (RefSvg *)h;
+    counts[0] = (int64_t)r->grad_i32.size() / 4;
+    counts[1] = (int64_t)r->stop_off.size();
+}
+
+void pfref_svg_gradient_fetch(void *h, int32_t *path_grad, int32_t *grad_i32, float *grad_f32, float *stop_off,
+                              int32_t *stop_rgba) {
+    auto *r = (RefSvg *)h;
+    auto cp = [](void *dst, const void *src, size_t n) {
+        if (n) memcpy(dst, src, n);
+    };
+    cp(path_grad, r->path_grad.data(), r->path_grad.size() * 4);
+    cp(grad_i32, r->grad_i32.data(), r->grad_i32.size() * 4);
+    cp(grad_f32, r->grad_f32.data(), r->grad_f32.size() * 4);
+    cp(stop_off, r->stop_off.data(), r->stop_off.size() * 4);
+    cp(stop_rgba, r->stop_rgba.data(), r->stop_rgba.size() * 4);
 }
 
 void pfref_svg_free(void *h) {

@@ -15,6 +15,25 @@ constexpr int MAX_FLATTEN_DEPTH = 16;
 constexpr int FIX_SHIFT = 20;            // fixed-point coverage accumulation: 2^-20 px^2 units
 constexpr uint32_t NO_ALPHA = 0xffffffffu;
 
+// Gradient paints.  An entry of a path with a gradient paint carries ENTRY_GRADIENT in its first
+// word (with its mask row, or ENTRY_ROW_NONE in the low bits for a tile without a mask) and the
+// scene-local paint index in its second; NO_ALPHA (all bits set) stays the plain solid entry.
+constexpr uint32_t ENTRY_GRADIENT = 0x40000000u;
+constexpr uint32_t ENTRY_ROW_MASK = 0x3fffffffu;
+constexpr uint32_t ENTRY_ROW_NONE = 0x3fffffffu;
+constexpr uint32_t PAINT_NONE = 0xffffffffu;   // SceneDev::path_paint: a solid colour
+constexpr int RAMP_TEXELS = 256;               // GRADIENT_TILE_LENGTH, core/paint/gradient.h
+// One gradient paint as the tile shader sees it (shaders/d3d9/tile.vert:65-77, tile.frag:296-323):
+// the colour texture transform and the radial filter's parameters as the RGBA16F metadata texture
+// holds them (core/renderer.cpp:167-250), and its row of RAMP_TEXELS RGBA8 texels.
+struct PaintRec {
+    float m11, m21, m12, m22, m13, m23;  // pixel -> colour texture coordinates (radial: gradient space)
+    float from_x, from_y, vec_x, vec_y;  // radial: the line between the circles' centres
+    float r0, r1, uv_x, uv_y;            // radial: radii, the ramp row's uv origin
+    uint32_t kind;                       // 1 linear, 2 radial
+    uint32_t wrap;                       // 0 clamp to edge, 1 repeat (REPEAT_U)
+};
+
 // Clip reference of a path (SceneDev::path_clip, then BinItem::clip).  A path clipped by a clip
 // path (a draw path, or a clip path nested in another) holds that clip path's binning position: its
 // index in the scene's wave-class list, or CLIP_GROUP | its index in the group-class list
@@ -158,6 +177,9 @@ struct SceneDev {
     int64_t out_pitch;
     uint32_t load_dst;            // blend over existing target contents
     float fb_inv_w;               // 1 / fb_w (0 without framebuffer tiles)
+    const uint32_t *path_paint;   // [n_paths] gradient paint index or PAINT_NONE, or null: solid colours
+    const PaintRec *paints;       // the scene's gradient paints
+    const uint32_t *ramps;        // [paints * RAMP_TEXELS] RGBA8 ramp texels, r | g << 8 | b << 16 | a << 24
 };
 
 // A binned path, resolved once per launch (after the chord scan) so that the per-path
@@ -187,6 +209,7 @@ struct BatchDev {
     uint32_t n_segs, n_tiles, n_cols, n_fb, n_paths;
     uint32_t n_bin_wave, n_bin_group;
     uint32_t cap_fills, cap_alpha, cap_entries, cap_lines;
+    uint32_t paints;              // some scene of the batch has gradient paints
     uint32_t cap_vis;             // alpha_vis capacity: cap_alpha, or with clip paths the larger of it and
                                   // cap_entries (solid tiles of several paths can take one clip mask)
     int32_t *tile_delta;          // [T] backdrop delta (int8 arithmetic, stored widened)
@@ -230,7 +253,8 @@ struct BatchDev {
     uint2 *fb_range;              // [FB] [start, end) of each framebuffer tile's entries in sort order
     uint32_t *keys, *vals;        // [E] sort keys / values
     uint32_t *keys_alt, *vals_alt;
-    uint2 *entry_rec;             // [E] (alpha index | NO_ALPHA, rgba (alpha 0 if covers nothing))
+    uint2 *entry_rec;             // [E] (alpha index | NO_ALPHA, rgba (alpha 0 if covers nothing)), or for a
+                                  // gradient path (ENTRY_GRADIENT | mask row or ENTRY_ROW_NONE, paint index)
     uint2 *entry_meta;            // [E] with dumps only: (local path, scene-local fb tile)
     uint32_t *counters;           // [C_COUNT]
     int32_t dump;

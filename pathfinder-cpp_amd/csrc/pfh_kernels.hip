@@ -1413,9 +1413,16 @@ __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &s
         b.alpha_vis[ea] = listed;
     }
     // even-odd solid tile with an even winding covers nothing: zero alpha
-    const uint32_t eff = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
+    uint32_t rx = ai, ry = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
+    // a gradient path's entry: flagged, its mask row (or none) and its paint; a tile that covers
+    // nothing stays a transparent solid entry
+    const uint32_t paint = sc.path_paint ? sc.path_paint[p] : PAINT_NONE;
+    if (paint != PAINT_NONE) {
+        rx = et.st == TILE_CLEAR ? NO_ALPHA : ENTRY_GRADIENT | (ai == NO_ALPHA ? ENTRY_ROW_NONE : min(ai, ENTRY_ROW_NONE - 1u));
+        ry = et.st == TILE_CLEAR ? 0u : paint;
+    }
     b.keys[e] = sc.fb_base + et.fbl;  // its value, e, is generated by the sort's first pass
-    b.entry_rec[e] = make_uint2(ai, eff);
+    b.entry_rec[e] = make_uint2(rx, ry);
     if (b.dump) b.entry_meta[e] = make_uint2(p, et.fbl);
 }
 

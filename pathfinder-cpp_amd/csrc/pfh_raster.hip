@@ -367,6 +367,62 @@ __device__ __forceinline__ void over_step(bool has_src, float &pr, float &pg, fl
 
 constexpr int EV_GROUP = 4;  // blend events whose mask words are loaded together
 
+// A gradient paint's colour at pixel centre (px, py): the tile shader's colour texture coordinate
+// (shaders/d3d9/tile.vert:77, mat2(m11, m21, m12, m22) * position + (m13, m23)), for a radial
+// gradient its filter's t (shaders/d3d9/tile.frag:296-323: the larger root of the two circles'
+// quadratic that is not negative, nothing where the discriminant is 0), then a linear sample of the
+// ramp row (RGBA8, clamp to edge or repeat along u).  Channels in byte units (0..255); false where
+// the paint leaves the pixel transparent (a zero or negative discriminant, a non-finite coordinate).
+__device__ __forceinline__ bool gradient_colour(const PaintRec &pr, const uint32_t *ramp, float px, float py,
+                                                float &cr, float &cg, float &cb, float &ca) {
+    const float u = pr.m11 * px + pr.m12 * py + pr.m13;
+    float s = u;
+    if (pr.kind == 2) {
+        const float v = pr.m21 * px + pr.m22 * py + pr.m23;
+        const float dpx = u - pr.from_x, dpy = v - pr.from_y, dr = pr.r1 - pr.r0;
+        const float a = pr.vec_x * pr.vec_x + pr.vec_y * pr.vec_y - dr * dr;
+        const float b = dpx * pr.vec_x + dpy * pr.vec_y + pr.r0 * dr;
+        const float c = dpx * dpx + dpy * dpy - pr.r0 * pr.r0;
+        const float disc = b * b - a * c;
+        if (!(disc > 0.0f)) return false;
+        const float sq = sqrtf(disc);
+        float t0 = (sq + b) / a, t1 = (-sq + b) / a;
+        if (t0 > t1) {
+            const float t = t0;
+            t0 = t1;
+            t1 = t;
+        }
+        s = pr.uv_x + (t0 >= 0.0f ? t0 : t1);
+    }
+    if (!isfinite(s)) return false;
+    // texel space: u * 256 - 0.5 (repeat: the fraction of u first, so large u keep their precision)
+    const float uu = pr.wrap ? s - floorf(s) : fminf(fmaxf(s, -1.0f), 2.0f);
+    const float cx = uu * (float)RAMP_TEXELS - 0.5f;
+    const float fl = floorf(cx);
+    const float f = cx - fl;
+    const int i0 = (int)fl;
+    int k0, k1;
+    if (pr.wrap) {
+        k0 = i0 & (RAMP_TEXELS - 1);
+        k1 = (i0 + 1) & (RAMP_TEXELS - 1);
+    } else {
+        k0 = min(max(i0, 0), RAMP_TEXELS - 1);
+        k1 = min(max(i0 + 1, 0), RAMP_TEXELS - 1);
+    }
+    const uint32_t t0 = ramp[k0], t1 = ramp[k1];
+    auto ch = [&](int sh) {
+        const float a0 = (float)((t0 >> sh) & 255u), a1 = (float)((t1 >> sh) & 255u);
+        return a0 + (a1 - a0) * f;
+    };
+    cr = ch(0);
+    cg = ch(8);
+    cb = ch(16);
+    ca = ch(24);
+    return true;
+}
+
+// PAINTS: some scene of the batch has gradient paints (the solid-only variant carries no code for them).
+template <bool PAINTS>
 __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *sorted_vals) {
     __shared__ float4 s_colour[4][64];  // per wave of the block: the chunk's premultiplied colours
     const int lane = threadIdx.x & 63;
@@ -407,16 +463,27 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         // ai: the mask row this lane's entry samples; row cap_alpha is all ones (full
         // coverage), sampled by solid entries and lanes past the chunk
         uint32_t ai = b.cap_alpha, c = 0;
-        bool masked = false;
+        bool masked = false, grad = false;
         if (i < e1) {
             const uint2 rec = b.entry_rec[sorted_vals[i]];
             c = rec.y;
             if (rec.x != NO_ALPHA) {
-                masked = true;
-                if (rec.x < b.cap_alpha) ai = rec.x;
-                else c = 0;  // capacity overflow (the run is retried): blends nothing
+                masked = true;  // a gradient entry never joins a solid run, with or without a mask row
+                if (PAINTS && (rec.x & ENTRY_GRADIENT)) {
+                    const uint32_t row = rec.x & ENTRY_ROW_MASK;
+                    grad = row == ENTRY_ROW_NONE || row < b.cap_alpha;
+                    if (row != ENTRY_ROW_NONE && row < b.cap_alpha) ai = row;
+                    if (!grad) c = 0;  // capacity overflow: blends nothing
+                } else if (rec.x < b.cap_alpha) {
+                    ai = rec.x;
+                } else {
+                    c = 0;  // capacity overflow (the run is retried): blends nothing
+                }
             }
         }
+        const uint64_t grad_mask = PAINTS ? __ballot(grad) : 0ull;
+        const uint32_t paint = c;  // a gradient entry's paint index
+        if (PAINTS && grad) c = 0;
         const float ca = (float)(c >> 24) * (1.0f / 255.0f);
         const bool valid = (uint32_t)lane < cnt;
         const bool solid = valid && !masked;
@@ -511,6 +578,36 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
             for (int d = 0; d < EV_GROUP; d++)
                 if (js[d] >= 0) blend(cf[d], ms[d]);
         };
+        if (PAINTS && grad_mask) {
+            // a chunk with gradient entries: events one at a time from the top, a gradient event's
+            // colour evaluated per pixel
+            while (events) {
+                const int j = 63 - __clzll(events);
+                events &= ~(1ull << j);
+                const uint32_t m = mask_word(j);
+                if ((grad_mask >> j) & 1ull) {
+                    const uint32_t pi = __builtin_amdgcn_readlane(paint, j);
+                    const PaintRec pr = sc.paints[pi];
+                    const uint32_t *ramp = sc.ramps + (uint64_t)pi * RAMP_TEXELS;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        float cr, cg, cb, ca;
+                        if (!gradient_colour(pr, ramp, (float)x + 0.5f, (float)(y0 + q) + 0.5f, cr, cg, cb, ca)) continue;
+                        // premultiplied, scaled like a masked entry's colour (the mask byte is the coverage)
+                        const float k = ca * (1.0f / (255.0f * 255.0f));
+                        const float wq = tr[q] * (float)((m >> (8 * q)) & 255u);
+                        r[q] = fmaf(wq, cr * k, r[q]);
+                        g[q] = fmaf(wq, cg * k, g[q]);
+                        bl[q] = fmaf(wq, cb * k, bl[q]);
+                        tr[q] = fmaf(-wq, k, tr[q]);
+                    }
+                } else {
+                    blend(wcol[j], m);
+                }
+            }
+            if (base == e0) break;
+            continue;
+        }
         int ja[EV_GROUP], jb[EV_GROUP];
         uint32_t ma[EV_GROUP], mb[EV_GROUP];
         take(ja, ma);
@@ -689,7 +786,10 @@ hipError_t launch_fill_clipped(const BatchDev &b, hipStream_t s) {
 hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, uint32_t block, hipStream_t s) {
     if (b.n_fb == 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_composite, dim3(blocks_for(b.n_fb, block / 64)), dim3(block), 0, s, b, sorted_vals);
+    if (b.paints)
+        hipLaunchKernelGGL(k_composite<true>, dim3(blocks_for(b.n_fb, block / 64)), dim3(block), 0, s, b, sorted_vals);
+    else
+        hipLaunchKernelGGL(k_composite<false>, dim3(blocks_for(b.n_fb, block / 64)), dim3(block), 0, s, b, sorted_vals);
     return hipGetLastError();
 }
 

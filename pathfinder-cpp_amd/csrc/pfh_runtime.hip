@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <deque>
 #include <functional>
 #include <map>
@@ -184,6 +185,9 @@ struct SceneHost {
     DevBuf d_features;
     DevBuf d_pts, d_segs, d_seg_path, d_path_rect, d_path_tile_off, d_path_col_off, d_col_path, d_path_rgba,
         d_path_rule, d_path_seg_off, d_bin_paths, d_out_owned, d_path_clip, d_path_level;
+    // gradient paints (none: path_paint empty)
+    std::vector<uint32_t> path_paint;
+    DevBuf d_path_paint, d_paints, d_ramps;
     uint32_t n_bin_wave = 0, n_bin_group = 0;  // paths binned by one wave / by one workgroup
     uint8_t *d_out = nullptr;
     int64_t out_pitch = 0;
@@ -410,6 +414,143 @@ static RectF intersect(const RectF &a, const RectF &o) {
     return RectF{std::max(a.l, o.l), std::max(a.t, o.t), std::min(a.r, o.r), std::min(a.b, o.b)};
 }
 
+// ---------------------------------------------------------------------------
+// Gradient paints: ramps and colour-texture records, built at upload
+// ---------------------------------------------------------------------------
+
+// third_party/umHalf.inl:49-111, the float-to-half conversion the metadata texture goes through:
+// the mantissa truncated (not rounded), magnitudes below 2^-24 to zero, 2^-24 .. 2^-14 to
+// denormals, above 2^16 to infinity; returned as the float value the half holds.
+static float half_trunc(float v) {
+    uint32_t f;
+    std::memcpy(&f, &v, 4);
+    const uint32_t sign = f >> 31, fe = (f >> 23) & 0xffu, ff = f & 0x7fffffu;
+    uint32_t he = 0, hf = 0;
+    if (fe == 0xffu) {
+        he = 31;
+        hf = ff != 0 ? 1u : 0u;
+    } else if (fe != 0) {
+        const int e = (int)fe - 127;
+        if (e < -24) {
+        } else if (e < -14) {
+            const int k = -14 - e;  // 1 .. 10
+            hf = (1024u >> k) + (ff >> (13 + k));
+        } else if (e > 15) {
+            he = 31;
+        } else {
+            he = (uint32_t)(e + 15);
+            hf = ff >> 13;
+        }
+    }
+    float mag;
+    if (he == 31) mag = hf ? std::numeric_limits<float>::quiet_NaN() : std::numeric_limits<float>::infinity();
+    else if (he == 0) mag = std::ldexp((float)hf, -24);
+    else mag = std::ldexp(1.0f + (float)hf / 1024.0f, (int)he - 15);
+    return sign ? -mag : mag;
+}
+
+struct ColorStopH {
+    float offset;
+    uint32_t rgba;
+};
+
+// Gradient::sample (core/paint/gradient.cpp:23-56) over stops in Gradient::add order: the two stops
+// around t, their colours as ColorF (channel * (1 / 255)), ColorF::lerp (common/color.cpp:61-72),
+// and ColorU(ColorF)'s truncation to bytes (common/color.cpp:15-21).
+static uint32_t gradient_sample(const std::vector<ColorStopH> &stops, float t) {
+    if (stops.empty()) return 0u;
+    t = std::min(std::max(t, 0.0f), 1.0f);
+    const size_t last = stops.size() - 1;
+    size_t it = 0;
+    while (it < stops.size() && (stops[it].offset < t || stops[it].offset == 0)) it++;
+    const size_t upper = std::min(it, last), lower = upper > 0 ? upper - 1 : upper;
+    const ColorStopH &lo = stops[lower], &hi = stops[upper];
+    const float denom = hi.offset - lo.offset;
+    if (denom == 0) return lo.rgba;
+    float ratio = std::min((t - lo.offset) / denom, 1.0f);
+    ratio = std::min(std::max(ratio, 0.0f), 1.0f);
+    const float k = 1.f / 255.f;
+    uint32_t out = 0;
+    for (int c = 0; c < 4; c++) {
+        const float a = (float)((lo.rgba >> (8 * c)) & 255u) * k, b = (float)((hi.rgba >> (8 * c)) & 255u) * k;
+        const float v = a + (b - a) * ratio;
+        out |= (uint32_t)(uint8_t)(v * 255.f) << (8 * c);
+    }
+    return out;
+}
+
+// One gradient: its ramp (GradientTileBuilder::allocate, core/paint/gradient.cpp:86-123: texel x
+// holds sample((x + 0.5) / 256)) and its record (Palette::calculate_texture_transforms,
+// core/paint/palette.cpp:398-440; PaintMetadata::filter, core/paint/paint.cpp:45-65), every float
+// of the record as the RGBA16F metadata texture holds it (core/renderer.cpp:167-250).
+static void compile_gradient(const pfh_gradient &g, const float *stop_offsets, const uint32_t *stop_rgba,
+                             int32_t row, uint32_t *ramp, PaintRec &rec) {
+    if ((g.kind != PFH_GRADIENT_LINEAR && g.kind != PFH_GRADIENT_RADIAL) ||
+        (g.wrap != PFH_WRAP_CLAMP && g.wrap != PFH_WRAP_REPEAT))
+        throw PfhError{PFH_E_ARG, "gradient kind or wrap out of range"};
+    for (float v : g.line)
+        if (!std::isfinite(v)) throw PfhError{PFH_E_ARG, "gradient line not finite"};
+    std::vector<ColorStopH> stops;
+    for (int32_t i = 0; i < g.stop_count; i++) {
+        const ColorStopH st{stop_offsets[g.stop_start + i], stop_rgba[g.stop_start + i]};
+        if (!std::isfinite(st.offset)) throw PfhError{PFH_E_ARG, "gradient stop offset not finite"};
+        size_t at = 0;  // Gradient::add: before the first stop whose offset is not below the new one's
+        while (at < stops.size() && stops[at].offset < st.offset) at++;
+        stops.insert(stops.begin() + (ptrdiff_t)at, st);
+    }
+    for (int x = 0; x < RAMP_TEXELS; x++) ramp[x] = gradient_sample(stops, ((float)x + 0.5f) / (float)RAMP_TEXELS);
+    const float scale = 1.f / (float)RAMP_TEXELS;
+    const float v0 = ((float)row + (float)(row + 1)) * 0.5f * scale;  // the row's centre
+    float m11, m21, m12, m22, m13, m23;
+    rec.from_x = rec.from_y = rec.vec_x = rec.vec_y = rec.r0 = rec.r1 = rec.uv_x = rec.uv_y = 0.0f;
+    if (g.kind == PFH_GRADIENT_LINEAR) {
+        const float fx = g.line[0], fy = g.line[1], dx = g.line[2] - fx, dy = g.line[3] - fy;
+        const float sq = dx * dx + dy * dy;
+        const float m0x = dx / sq, m0y = dy / sq;
+        const float ax = m0x * -fx, ay = m0y * -fy;
+        m11 = m0x, m21 = 0.0f, m12 = m0y, m22 = 0.0f, m13 = ax + ay, m23 = v0;
+        rec.kind = 1;
+    } else {
+        for (float v : g.radii)
+            if (!std::isfinite(v)) throw PfhError{PFH_E_ARG, "gradient radii not finite"};
+        for (float v : g.transform)
+            if (!std::isfinite(v)) throw PfhError{PFH_E_ARG, "gradient transform not finite"};
+        // Transform2::inverse (common/math/transform2.cpp:32-36, Mat2::inverse: adjugate * (1 / det))
+        const float a11 = g.transform[0], a21 = g.transform[1], a12 = g.transform[2], a22 = g.transform[3];
+        const float det = a11 * a22 - a21 * a12;
+        if (det == 0.0f || !std::isfinite(1.0f / det)) throw PfhError{PFH_E_ARG, "radial gradient transform not invertible"};
+        const float id = 1.0f / det;
+        m11 = (a22 * 1.0f) * id, m21 = (a21 * -1.0f) * id, m12 = (a12 * -1.0f) * id, m22 = (a11 * 1.0f) * id;
+        const float vx = g.transform[4], vy = g.transform[5];
+        m13 = -(m11 * vx + m12 * vy), m23 = -(m21 * vx + m22 * vy);
+        rec.from_x = half_trunc(g.line[0]);
+        rec.from_y = half_trunc(g.line[1]);
+        rec.vec_x = half_trunc(g.line[2] - g.line[0]);
+        rec.vec_y = half_trunc(g.line[3] - g.line[1]);
+        rec.r0 = half_trunc(g.radii[0]);
+        rec.r1 = half_trunc(g.radii[1]);
+        rec.uv_x = half_trunc(0.0f * scale);
+        rec.uv_y = half_trunc((float)row * scale + scale * 0.5f);  // the row's top, moved half a texel in
+        rec.kind = 2;
+    }
+    rec.m11 = half_trunc(m11);
+    rec.m21 = half_trunc(m21);
+    rec.m12 = half_trunc(m12);
+    rec.m22 = half_trunc(m22);
+    rec.m13 = half_trunc(m13);
+    rec.m23 = half_trunc(m23);
+    rec.wrap = (uint32_t)g.wrap;
+}
+
+// Paint::is_opaque for a gradient paint (core/paint/paint.cpp:9-19, Gradient::is_opaque,
+// core/paint/gradient.cpp:58-70): white base colour, every stop opaque, at least one stop.
+static bool gradient_opaque(const pfh_gradient &g, const uint32_t *stop_rgba) {
+    if (g.stop_count <= 0) return false;
+    for (int32_t i = 0; i < g.stop_count; i++)
+        if ((stop_rgba[g.stop_start + i] >> 24) != 255u) return false;
+    return true;
+}
+
 static void validate_desc(const pfh_scene_desc *d) {
     if (!d) throw PfhError{PFH_E_ARG, "null scene desc"};
     if (d->width <= 0 || d->height <= 0) throw PfhError{PFH_E_ARG, "canvas size must be positive"};
@@ -432,6 +573,19 @@ static void validate_desc(const pfh_scene_desc *d) {
         if (d->path_fill_rule[p] > 1) throw PfhError{PFH_E_ARG, "path_fill_rule must be 0 (nonzero) or 1 (even-odd)"};
     if (d->n_clip_paths < 0 || d->n_clip_paths > d->n_paths)
         throw PfhError{PFH_E_ARG, "n_clip_paths must lie in [0, n_paths]"};
+    if (d->path_gradient) {
+        if (d->n_gradients < 0 || d->n_gradient_stops < 0 || (d->n_gradients && !d->gradients) ||
+            (d->n_gradient_stops && (!d->gradient_stop_offsets || !d->gradient_stop_rgba)))
+            throw PfhError{PFH_E_ARG, "null gradient array"};
+        for (int p = 0; p < d->n_paths; p++)
+            if (d->path_gradient[p] < -1 || d->path_gradient[p] >= d->n_gradients)
+                throw PfhError{PFH_E_ARG, "path_gradient names no gradient"};
+        for (int i = 0; i < d->n_gradients; i++) {
+            const pfh_gradient &g = d->gradients[i];
+            if (g.stop_count < 0 || g.stop_start < 0 || (int64_t)g.stop_start + g.stop_count > d->n_gradient_stops)
+                throw PfhError{PFH_E_ARG, "gradient stops outside the stop arrays"};
+        }
+    }
     if (d->path_clip) {
         const int n_draw = d->n_paths - d->n_clip_paths;
         for (int p = 0; p < d->n_paths; p++) {
@@ -920,6 +1074,28 @@ static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d) {
     s->path_seg_off.assign(d->n_paths + 1, 0);
     s->path_rgba.assign(d->path_rgba, d->path_rgba + d->n_paths);
     s->path_rule.assign(d->path_fill_rule, d->path_fill_rule + d->n_paths);
+    std::vector<PaintRec> paints;
+    std::vector<uint32_t> ramps;
+    if (d->path_gradient) {
+        // the gradients the paths use, each compiled once; a gradient path's colour word holds only
+        // its opacity (alpha 255 or 0), which the z-buffer reads
+        std::vector<int32_t> slot(d->n_gradients, -1);
+        s->path_paint.assign(d->n_paths, PAINT_NONE);
+        for (int p = 0; p < d->n_paths; p++) {
+            const int32_t gi = d->path_gradient[p];
+            if (gi < 0) continue;
+            if (slot[gi] < 0) {
+                slot[gi] = (int32_t)paints.size();
+                paints.emplace_back();
+                ramps.resize(ramps.size() + RAMP_TEXELS);
+                compile_gradient(d->gradients[gi], d->gradient_stop_offsets, d->gradient_stop_rgba,
+                                 slot[gi] % RAMP_TEXELS, ramps.data() + ramps.size() - RAMP_TEXELS, paints.back());
+            }
+            s->path_paint[p] = (uint32_t)slot[gi];
+            s->path_rgba[p] = gradient_opaque(d->gradients[gi], d->gradient_stop_rgba) ? 0xff000000u : 0u;
+        }
+        if (paints.empty()) s->path_paint.clear();
+    }
     std::vector<uint32_t> col_path;
     uint64_t tiles = 0, cols = 0;
     for (int p = 0; p < d->n_paths; p++) {
@@ -1080,6 +1256,11 @@ static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d) {
     upload(s->d_col_path, col_path);
     upload(s->d_path_rgba, s->path_rgba);
     upload(s->d_path_rule, s->path_rule);
+    if (!s->path_paint.empty()) {
+        upload(s->d_path_paint, s->path_paint);
+        upload(s->d_paints, paints);
+        upload(s->d_ramps, ramps);
+    }
     upload(s->d_path_seg_off, s->path_seg_off);
     upload(s->d_bin_paths, bin_wave);
     if (s->n_clip > 0) {
@@ -1132,6 +1313,7 @@ struct Plan {
     uint64_t S = 0, T = 0, W = 0, FB = 0, BW = 0, BG = 0, P = 0;
     uint64_t breaks = 0;  // chord chains of the batch (two stream points each besides the chords)
     bool clips = false;   // some scene has clip paths
+    bool paints = false;  // some scene has gradient paints
     uint32_t clip_levels = 0;  // deepest clip nesting level of the batch
     uint32_t cap_f = 0, cap_a = 0, cap_e = 0, cap_l = 0;
     int key_bits = 1;
@@ -1245,6 +1427,10 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.path_rule = s.d_path_rule.as<uint8_t>();
         d.path_clip = s.n_clip > 0 ? s.d_path_clip.as<uint32_t>() : nullptr;
         d.path_level = s.n_clip > 0 ? s.d_path_level.as<uint8_t>() : nullptr;
+        const bool painted = !s.path_paint.empty();
+        d.path_paint = painted ? s.d_path_paint.as<uint32_t>() : nullptr;
+        d.paints = painted ? s.d_paints.as<PaintRec>() : nullptr;
+        d.ramps = painted ? s.d_ramps.as<uint32_t>() : nullptr;
         d.vb_l = s.vb[0];
         d.vb_t = s.vb[1];
         d.vb_r = s.vb[2];
@@ -1283,6 +1469,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         S += s.n_segs;
         pl.breaks += s.n_breaks;
         pl.clips = pl.clips || s.n_clip > 0;
+        pl.paints = pl.paints || !s.path_paint.empty();
         pl.clip_levels = std::max(pl.clip_levels, s.clip_levels);
         T += s.n_tiles;
         W += s.n_cols;
@@ -1341,10 +1528,13 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.n_cols = (uint32_t)pl.W;
     b.n_fb = (uint32_t)pl.FB;
     b.cap_fills = pl.cap_f;
+    // a gradient entry keeps its mask row in 30 bits (ENTRY_ROW_MASK)
+    if (pl.cap_a >= ENTRY_ROW_NONE) throw PfhError{PFH_E_LIMIT, "more than 2^30 alpha tiles in one launch"};
     b.cap_alpha = pl.cap_a;
     b.cap_entries = pl.cap_e;
     b.cap_lines = pl.cap_l;
     b.cap_vis = pl.clips ? std::max(pl.cap_a, pl.cap_e) : pl.cap_a;
+    b.paints = pl.paints ? 1u : 0u;
     b.tile_delta = w.tile_delta.as<int32_t>();
     b.tile_cnt = w.tile_cnt.as<uint32_t>();
     b.tile_fill_off = w.tile_fill_off.as<uint32_t>();
@@ -1974,7 +2164,7 @@ void pfh_destroy(pfh_ctx *ctx) {
         if (!s) continue;
         for (DevBuf *b : {&s->d_features, &s->d_pts, &s->d_segs, &s->d_seg_path, &s->d_path_rect, &s->d_path_tile_off,
                           &s->d_path_col_off, &s->d_col_path, &s->d_path_rgba, &s->d_path_rule, &s->d_path_seg_off, &s->d_bin_paths, &s->d_out_owned, &s->d_path_clip,
-                          &s->d_path_level})
+                          &s->d_path_level, &s->d_path_paint, &s->d_paints, &s->d_ramps})
             b->release();
     }
     for (DevBuf *b : {&ctx->mlp_params, &ctx->mlp_x, &ctx->mlp_h, &ctx->mlp_out}) b->release();
@@ -1986,6 +2176,25 @@ const char *pfh_last_error(const pfh_ctx *ctx) { return ctx ? ctx->err.c_str() :
 int pfh_scene_upload(pfh_ctx *ctx, const pfh_scene_desc *desc, int32_t *scene_id) {
     if (!ctx || !scene_id) return PFH_E_ARG;
     return run_guarded(ctx, [&] { *scene_id = scene_upload(ctx, desc); });
+}
+
+int pfh_gradient_compile(const pfh_gradient *g, const float *stop_offsets, const uint32_t *stop_rgba,
+                         int32_t ramp_row, uint32_t *ramp, float *record) {
+    if (!g || !ramp || !record || ramp_row < 0 || ramp_row >= RAMP_TEXELS || g->stop_count < 0 || g->stop_start < 0 ||
+        (g->stop_count > 0 && (!stop_offsets || !stop_rgba)))
+        return PFH_E_ARG;
+    try {
+        PaintRec rec{};
+        compile_gradient(*g, stop_offsets, stop_rgba, ramp_row, ramp, rec);
+        const float f[16] = {rec.m11, rec.m21, rec.m12, rec.m22, rec.m13, rec.m23, rec.from_x, rec.from_y,
+                             rec.vec_x, rec.vec_y, rec.r0, rec.r1, rec.uv_x, rec.uv_y, (float)rec.kind, (float)rec.wrap};
+        std::memcpy(record, f, sizeof(f));
+        return PFH_OK;
+    } catch (const PfhError &e) {
+        return e.code;
+    } catch (...) {
+        return PFH_E_ARG;
+    }
 }
 
 int pfh_stroke_outlines(pfh_ctx *ctx, const pfh_outline *in, const int32_t *path_contour_start, int32_t n_paths,
@@ -2040,7 +2249,7 @@ int pfh_scene_release(pfh_ctx *ctx, int32_t scene_id) {
         drop_graphs_of_scene(ctx, scene_id);
         for (DevBuf *b : {&s.d_features, &s.d_pts, &s.d_segs, &s.d_seg_path, &s.d_path_rect, &s.d_path_tile_off, &s.d_path_col_off,
                           &s.d_col_path, &s.d_path_rgba, &s.d_path_rule, &s.d_path_seg_off, &s.d_bin_paths, &s.d_out_owned, &s.d_path_clip,
-                          &s.d_path_level})
+                          &s.d_path_level, &s.d_path_paint, &s.d_paints, &s.d_ramps})
             b->release();
         ctx->scenes[scene_id].reset();
     });

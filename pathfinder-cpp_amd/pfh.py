@@ -36,7 +36,9 @@ class SceneDesc(ctypes.Structure):
                 ("points", _P), ("point_flags", _P), ("contour_start", _P), ("contour_closed", _P),
                 ("path_contour_start", _P), ("path_fill_rule", _P), ("path_rgba", _P),
                 ("n_clip_paths", ctypes.c_int32), ("path_clip", _P), ("path_stroke", _P),
-                ("stroke_dashes", _P), ("n_stroke_dashes", ctypes.c_int32)]
+                ("stroke_dashes", _P), ("n_stroke_dashes", ctypes.c_int32),
+                ("path_gradient", _P), ("gradients", _P), ("n_gradients", ctypes.c_int32),
+                ("gradient_stop_offsets", _P), ("gradient_stop_rgba", _P), ("n_gradient_stops", ctypes.c_int32)]
 
 
 class Outline(ctypes.Structure):
@@ -102,7 +104,7 @@ EXPORTS = ("pfh_create", "pfh_destroy", "pfh_last_error", "pfh_scene_upload", "p
            "pfh_dump_counts", "pfh_dump", "pfh_stats_get", "pfh_set_predictor", "pfh_scene_prediction", "pfh_path_counts",
            "pfh_mlp_eval", "pfh_mlp_grad", "pfh_mlp_train", "pfh_server_start", "pfh_server_submit",
            "pfh_server_poll", "pfh_server_stats_get", "pfh_server_stop", "pfh_get_schedule",
-           "pfh_set_schedule", "pfh_autotune", "pfh_stroke_outlines", "pfh_rasterize_noalloc")
+           "pfh_set_schedule", "pfh_autotune", "pfh_stroke_outlines", "pfh_rasterize_noalloc", "pfh_gradient_compile")
 
 _lib = None
 
@@ -128,6 +130,7 @@ def load(path: str = LIB_PATH):
         "pfh_scene_features": (ctypes.c_int, [_P, i32, _P]),
         "pfh_rasterize": (ctypes.c_int, [_P, _P, i32, ctypes.c_uint32]),
         "pfh_rasterize_noalloc": (ctypes.c_int, [_P, _P, i32, ctypes.c_uint32]),
+        "pfh_gradient_compile": (ctypes.c_int, [_P, _P, _P, i32, _P, _P]),
         "pfh_stroke_outlines": (ctypes.c_int, [_P, ctypes.POINTER(Outline), _P, i32, _P, _P, i32,
                                                 ctypes.POINTER(OutlineBuf)]),
         "pfh_scene_set_target": (ctypes.c_int, [_P, i32, _P, i64]),
@@ -167,6 +170,28 @@ def _ptr(a):
 STROKE_STYLE = np.dtype([("line_width", np.float32), ("line_cap", np.int32), ("line_join", np.int32),
                          ("miter_limit", np.float32), ("dash_start", np.int32), ("dash_count", np.int32),
                          ("dash_offset", np.float32)])
+
+
+# pfh_gradient (include/pfh.h): kind, wrap, line, radii, radial transform, stop range
+GRADIENT = np.dtype([("kind", np.int32), ("wrap", np.int32), ("line", np.float32, 4), ("radii", np.float32, 2),
+                     ("transform", np.float32, 6), ("stop_start", np.int32), ("stop_count", np.int32)])
+GRADIENT_LINEAR, GRADIENT_RADIAL = 0, 1
+WRAP_CLAMP, WRAP_REPEAT = 0, 1
+
+
+def gradient_compile(g, stop_offsets, stop_rgba, ramp_row=0):
+    """pfh_gradient_compile on one GRADIENT record (host only): (ramp [256] u32, record [16] f32)."""
+    lib = load()
+    rec = np.ascontiguousarray(np.asarray(g, GRADIENT).reshape(1))
+    off = np.ascontiguousarray(stop_offsets, np.float32)
+    rgba = np.ascontiguousarray(stop_rgba, np.uint32)
+    ramp = np.zeros(256, np.uint32)
+    out = np.zeros(16, np.float32)
+    rc = lib.pfh_gradient_compile(_ptr(rec), _ptr(off) if off.size else None, _ptr(rgba) if rgba.size else None,
+                                  int(ramp_row), _ptr(ramp), _ptr(out))
+    if rc != 0:
+        raise PfhError(f"pfh_gradient_compile failed ({rc})")
+    return ramp, out
 
 
 def stroke_styles(styles) -> np.ndarray:
@@ -241,6 +266,17 @@ class Context:
         if dashes is not None:
             dashes = np.ascontiguousarray(dashes, np.float32)
             d.stroke_dashes, d.n_stroke_dashes = _ptr(dashes), int(dashes.shape[0])
+        pg = getattr(scene, "path_gradient", None)
+        keep = []
+        if pg is not None:
+            pg = np.ascontiguousarray(pg, np.int32)
+            grads = np.ascontiguousarray(np.asarray(scene.gradients, GRADIENT).reshape(-1))
+            offs = np.ascontiguousarray(scene.gradient_stop_offsets, np.float32)
+            rgba = np.ascontiguousarray(scene.gradient_stop_rgba, np.uint32)
+            keep = [pg, grads, offs, rgba]
+            d.path_gradient, d.gradients, d.n_gradients = _ptr(pg), _ptr(grads), int(grads.shape[0])
+            d.gradient_stop_offsets, d.gradient_stop_rgba = _ptr(offs), _ptr(rgba)
+            d.n_gradient_stops = int(offs.shape[0])
         sid = ctypes.c_int32()
         self._check(self.lib.pfh_scene_upload(self.h, ctypes.byref(d), ctypes.byref(sid)), "pfh_scene_upload")
         self._keep[sid.value] = (int(scene.width), int(scene.height), scene.n_paths)

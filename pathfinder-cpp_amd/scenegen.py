@@ -60,6 +60,12 @@ class Scene:
     # and its stroke filled with the nonzero rule.
     path_stroke: np.ndarray = None
     stroke_dashes: np.ndarray = None
+    # Gradient paints (Paint::from_gradient): None, or per path an index into `gradients` (GRADIENT
+    # records; -1: the solid path_rgba), with the stops of every gradient in the two stop arrays.
+    path_gradient: np.ndarray = None
+    gradients: np.ndarray = None
+    gradient_stop_offsets: np.ndarray = None
+    gradient_stop_rgba: np.ndarray = None
 
     def __post_init__(self):
         if self.view_box is None:
@@ -86,8 +92,15 @@ class Scene:
 
     @property
     def path_opaque(self) -> np.ndarray:
-        # ColorU::is_opaque (common/color.cpp:42-44) for a solid paint.
-        return ((self.path_rgba >> 24) == 255).astype(np.uint8)
+        # ColorU::is_opaque (common/color.cpp:42-44) for a solid paint; for a gradient paint
+        # Paint::is_opaque (core/paint/paint.cpp:9-19): every stop opaque, at least one stop.
+        op = ((self.path_rgba >> 24) == 255).astype(np.uint8)
+        if self.path_gradient is not None:
+            for p in np.nonzero(self.path_gradient >= 0)[0]:
+                g = self.gradients[int(self.path_gradient[p])]
+                a, n = int(g["stop_start"]), int(g["stop_count"])
+                op[p] = 1 if n > 0 and ((self.gradient_stop_rgba[a:a + n] >> 24) == 255).all() else 0
+        return op
 
     def subset(self, paths) -> "Scene":
         """The scene restricted to the given draw paths (in the given order); clip paths are dropped."""
@@ -107,6 +120,12 @@ class Scene:
                      np.concatenate(flg) if flg else np.zeros(0, np.uint8),
                      np.array(cst), np.array(ccl, np.uint8), np.array(pcs),
                      self.path_fill_rule[paths], self.path_rgba[paths], self.view_box.copy())
+
+
+# pfh_gradient (include/pfh.h): kind (0 linear, 1 radial), wrap (0 clamp, 1 repeat), line (from, to),
+# radii, radial transform (m11, m21, m12, m22, m13, m23), stop range
+GRADIENT = np.dtype([("kind", np.int32), ("wrap", np.int32), ("line", np.float32, 4), ("radii", np.float32, 2),
+                     ("transform", np.float32, 6), ("stop_start", np.int32), ("stop_count", np.int32)])
 
 
 def _rng(seed: int) -> np.random.Generator:
@@ -375,6 +394,64 @@ def stroke_scene(seed: int, size: int = 4096, n: int = 500) -> Scene:
     sc = outlines_scene(size, size, [o for o, _ in lines], np.zeros(n, np.uint8),
                         _pack_rgba(rgb[:, 0], rgb[:, 1], rgb[:, 2], alpha))
     sc.path_stroke = np.array([st for _, st in lines], np.float32)
+    return sc
+
+
+def gradient_scene(seed: int, size: int = 1024, n: int = 200, painted: float = 0.6) -> Scene:
+    """blob_scene with a fraction `painted` of the paths given gradient paints: half linear (a line
+    of length U(size/16, size) through the blob), half radial (focal circle inside the outer one,
+    start radius 0 or U(0, r/2), outer radius U(r/2, 2r), a rotation-and-scale transform on a third
+    of them), clamp or repeat, 1-5 stops at offsets in [0, 1] (sometimes equal, sometimes out of
+    order), opaque stops for half of the gradients."""
+    sc = blob_scene(seed, size, size, n, r_max=size / 6)
+    rng = _rng(seed ^ 0x6EAD)
+    pg = np.full(n, -1, np.int32)
+    grads, offs, cols = [], [], []
+    for p in range(n):
+        if rng.random() >= painted:
+            continue
+        c0, c1 = int(sc.path_contour_start[p]), int(sc.path_contour_start[p + 1])
+        pts = sc.points[int(sc.contour_start[c0]):int(sc.contour_start[c1])]
+        cx, cy = pts.mean(axis=0)
+        r = float(np.abs(pts - [cx, cy]).max()) + 1.0
+        g = np.zeros((), GRADIENT)
+        g["kind"] = int(rng.integers(0, 2))
+        g["wrap"] = int(rng.integers(0, 2))
+        g["transform"] = (1, 0, 0, 1, 0, 0)
+        if g["kind"] == 0:
+            ang, ln = rng.uniform(0, 2 * math.pi), rng.uniform(size / 16, size)
+            x0, y0 = cx - 0.5 * ln * math.cos(ang), cy - 0.5 * ln * math.sin(ang)
+            g["line"] = (x0, y0, x0 + ln * math.cos(ang), y0 + ln * math.sin(ang))
+        else:
+            r1 = rng.uniform(0.5 * r, 2.0 * r)
+            r0 = 0.0 if rng.random() < 0.5 else rng.uniform(0.0, 0.5 * r1)
+            fx, fy = rng.uniform(-0.3, 0.3, 2) * r1
+            if rng.random() < 1 / 3:
+                th, sx = rng.uniform(0, 2 * math.pi), rng.uniform(0.5, 2.0)
+                ct, st = math.cos(th) * sx, math.sin(th) * sx
+                g["transform"] = (ct, st, -st, ct, cx, cy)  # rotate and scale about the blob's centre
+                g["line"] = (0.0, 0.0, fx, fy)
+            else:
+                g["line"] = (cx, cy, cx + fx, cy + fy)
+            g["radii"] = (r0, r1)
+        k = int(rng.integers(1, 6))
+        o = rng.uniform(0, 1, k)
+        if k > 2 and rng.random() < 0.3:
+            o[1] = o[2]  # equal offsets
+        if rng.random() < 0.5:
+            o = np.sort(o)
+        opaque = rng.random() < 0.5
+        a = np.full(k, 255) if opaque else rng.integers(0, 256, k)
+        rgb = rng.integers(0, 256, (k, 3))
+        g["stop_start"], g["stop_count"] = len(offs), k
+        offs.extend(o.astype(np.float32).tolist())
+        cols.extend(_pack_rgba(rgb[:, 0], rgb[:, 1], rgb[:, 2], a).tolist())
+        pg[p] = len(grads)
+        grads.append(g)
+    sc.path_gradient = pg
+    sc.gradients = np.array(grads, GRADIENT) if grads else np.zeros(0, GRADIENT)
+    sc.gradient_stop_offsets = np.array(offs, np.float32)
+    sc.gradient_stop_rgba = np.array(cols, np.uint32)
     return sc
 
 

@@ -6,8 +6,9 @@ core/svg.cpp:128-206) makes of its assets tiger.svg and features.svg.
   the device dashes and strokes them at upload.
 * scene_out(name): the same draw paths as the reference's scene holds them -- strokes already
   dashed and stroked by the reference (filled nonzero).
-Only solid-colour paints are kept by default (features.svg's two gradient paints are left out:
-the library draws solid paints).  Canvas = the document's size, rounded up."""
+With solid_only (the default) the paths with gradient paints are left out; otherwise they carry
+their gradients (path_gradient / gradients / stops, as the reference's convert_nsvg_paint builds
+them, core/svg.cpp:64-124).  Canvas = the document's size, rounded up."""
 import math
 import os
 
@@ -40,9 +41,19 @@ def _kept(d, solid_only):
     return [p for p in range(pi.shape[0]) if not solid_only or pi[p, 2] == PAINT_COLOR]
 
 
-def _scene(d, outlines, rules, rgba):
+def _scene(d, outlines, rules, rgba, keep):
     w, h = (int(math.ceil(float(v))) for v in d["size"])
     sc = scenegen.outlines_scene(w, h, outlines, rules, rgba)
+    pg = d["path_grad"][keep]
+    if (pg >= 0).any():
+        gi, gf = d["grad_i32"], d["grad_f32"]
+        g = np.zeros(gi.shape[0], scenegen.GRADIENT)
+        g["kind"], g["wrap"], g["stop_start"], g["stop_count"] = gi[:, 0], gi[:, 1], gi[:, 2], gi[:, 3]
+        g["line"], g["radii"], g["transform"] = gf[:, 0:4], gf[:, 4:6], gf[:, 6:12]
+        sc.path_gradient = pg.astype(np.int32)
+        sc.gradients = g
+        sc.gradient_stop_offsets = d["stop_off"].astype(np.float32)
+        sc.gradient_stop_rgba = d["stop_rgba"].astype(np.uint32)
     return sc
 
 
@@ -50,7 +61,7 @@ def scene_src(name, solid_only=True):
     d = load(name)
     keep = _kept(d, solid_only)
     pi, pf = d["path_i32"], d["path_f32"]
-    sc = _scene(d, _paths(d, "src", keep), pi[keep, 1].astype(np.uint8), pi[keep, 3].astype(np.uint32))
+    sc = _scene(d, _paths(d, "src", keep), pi[keep, 1].astype(np.uint8), pi[keep, 3].astype(np.uint32), keep)
     st = np.zeros((len(keep), 7), np.float32)
     for k, p in enumerate(keep):
         if pi[p, 0] == 1:
@@ -65,7 +76,7 @@ def scene_out(name, solid_only=True):
     keep = _kept(d, solid_only)
     pi = d["path_i32"]
     rules = np.where(pi[keep, 0] == 1, 0, pi[keep, 1]).astype(np.uint8)
-    return _scene(d, _paths(d, "out", keep), rules, pi[keep, 3].astype(np.uint32))
+    return _scene(d, _paths(d, "out", keep), rules, pi[keep, 3].astype(np.uint32), keep)
 
 
 def strokes(name):

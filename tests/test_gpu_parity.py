@@ -29,7 +29,7 @@ def ctx():
     c.close()
 
 
-def check_scene(ctx, scene, flags=pfh.PFH_PLAIN, name="", guard_hits=0):
+def check_scene(ctx, scene, flags=pfh.PFH_PLAIN, name="", guard_hits=0, rgba_tol=RGBA_TOL):
     print(f"[parity] {name} paths={scene.n_paths} canvas={scene.width}x{scene.height} flags={flags}", flush=True)
     sid = ctx.upload(scene)
     ctx.rasterize([sid], flags | pfh.PFH_DUMP)
@@ -45,7 +45,7 @@ def check_scene(ctx, scene, flags=pfh.PFH_PLAIN, name="", guard_hits=0):
         assert err <= COV_TOL, f"{name}: coverage error {err * 255:.3f}/255"
     want = np.rint(np.clip(model["image"], 0.0, 1.0) * 255.0).astype(np.int64)
     px = np.abs(img.astype(np.int64) - want).max() if img.size else 0
-    assert px <= RGBA_TOL, f"{name}: rgba8 error {px}"
+    assert px <= rgba_tol, f"{name}: rgba8 error {px}"
     ctx.release(sid)
     return got, img
 

@@ -6,8 +6,10 @@ delivers them (tests/golden/svg_fixtures.npz; tests/svg_scenes.py builds the sce
 * The source scenes (strokes stroked at upload on the device) through the pipeline, plain and
   batched: tiles, fills, backdrops and order bit-exact with the oracle, masks within 1/255, RGBA8
   within 1 LSB; and the same pixels as the reference's pre-stroked scene uploaded as plain fills.
+* features.svg with its gradient paints (one linear, two radial in transformed gradient spaces).
 * The shipped predictor on tiger, a path-length distribution it was not trained on (hand-drawn
-  curves, 0.2-px hairline strokes): the totals of chords, fills and alpha tiles within 25 %.
+  curves, 0.2-px hairline strokes): the totals of chords, fills and alpha tiles within 15 %
+  (measured 8.7 / 6.9 / 5.2 %, all under-estimates, which the 1.25 safety factor covers).
 """
 import numpy as np
 import pytest
@@ -56,6 +58,15 @@ def test_svg_scene_parity(ctx, name, flags):
     ctx.release(sid)
 
 
+@pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED])
+def test_features_with_gradient_paints(ctx, flags):
+    """features.svg whole: its linear gradient and its two radial gradients in transformed gradient
+    spaces drawn through the colour texture, against the oracle (RGBA8 within 2 LSB, as the
+    gradient scenes of tests/test_gpu_paint.py)."""
+    check_scene(ctx, svg_scenes.scene_src("features", solid_only=False), flags=flags, name="features.svg + gradients",
+                rgba_tol=2)
+
+
 def test_shipped_predictor_on_tiger(ctx):
     w = predictor.load()
     assert w is not None, "predictor weights missing"
@@ -65,4 +76,4 @@ def test_shipped_predictor_on_tiger(ctx):
     rel = np.abs(pred - exact) / exact
     print(f"[predictor] tiger ({feats.shape[0]} paths): exact {exact}, predicted {np.round(pred)}, "
           f"rel err {np.round(rel, 4)}", flush=True)
-    assert (rel < 0.25).all(), rel
+    assert (rel < 0.15).all(), rel

@@ -51,3 +51,11 @@ def test_document_shapes():
     assert set(pf[:, 2].tolist()) == {1, 2, 3}
     assert (pf[:, 1] == 1).any() and (pf[:, 7] > 0).any()
     assert set(pf[pf[:, 0] == 1, 4].tolist()) == {0, 1, 2} and set(pf[pf[:, 0] == 1, 5].tolist()) == {0, 1, 2}
+
+
+def test_apply_strokes_keeps_gradient_paints():
+    src = svg_scenes.scene_src("features", solid_only=False)
+    out = oracle.apply_strokes(src)
+    np.testing.assert_array_equal(out.path_gradient, src.path_gradient)
+    np.testing.assert_array_equal(out.path_opaque, src.path_opaque)
+    assert out.gradients is src.gradients
