@@ -98,7 +98,8 @@ typedef struct pfh_gradient {
     float radii[2];            /* radial */
     float transform[6];        /* radial: GradientRadial::transform, Transform2 as m11, m21, m12, m22, m13, m23:
                                 * (x, y) -> (m11 x + m12 y + m13, m21 x + m22 y + m23); identity 1 0 0 1 0 0 */
-    int32_t stop_start, stop_count;  /* into the stop arrays */
+    int32_t stop_start, stop_count;  /* into the stop arrays; stop_count < 0: the ramp is given instead
+                                      * (pfh_scene_desc::gradient_ramps) */
 } pfh_gradient;
 
 /*
@@ -149,6 +150,10 @@ typedef struct pfh_scene_desc {
     const float *gradient_stop_offsets; /* [n_gradient_stops] */
     const uint32_t *gradient_stop_rgba; /* [n_gradient_stops] */
     int32_t n_gradient_stops;
+    /* NULL, or [n_gradients][256]: for a gradient whose stop_count is negative, its ramp as the
+     * caller sampled it (texel x = Gradient::sample((x + 0.5) / 256), as GradientTileBuilder::allocate
+     * does; a binding that cannot read a Gradient's stops passes this); opaque when every texel is. */
+    const uint32_t *gradient_ramps;
 } pfh_scene_desc;
 
 typedef struct pfh_stats {

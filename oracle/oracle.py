@@ -217,7 +217,7 @@ def apply_strokes(scene):
     out.view_box = scene.view_box.copy()
     out.n_clip_paths = scene.n_clip_paths
     out.path_clip = scene.path_clip
-    for k in ("path_gradient", "gradients", "gradient_stop_offsets", "gradient_stop_rgba"):
+    for k in ("path_gradient", "gradients", "gradient_stop_offsets", "gradient_stop_rgba", "gradient_ramps"):
         setattr(out, k, getattr(scene, k, None))
     return out
 
@@ -512,7 +512,8 @@ def scene_paints(scene):
             a, n = int(g["stop_start"]), int(g["stop_count"])
             slot[gi] = len(recs)
             recs.append(gradient_record(g, len(recs) % 256))
-            ramps.append(gradient_ramp(scene.gradient_stop_offsets[a:a + n], scene.gradient_stop_rgba[a:a + n]))
+            ramps.append(np.asarray(scene.gradient_ramps[gi], np.uint32) if n < 0 else
+                         gradient_ramp(scene.gradient_stop_offsets[a:a + n], scene.gradient_stop_rgba[a:a + n]))
         paint[p] = slot[gi]
     if not recs:
         return None

@@ -484,13 +484,14 @@ static uint32_t gradient_sample(const std::vector<ColorStopH> &stops, float t) {
 // core/paint/palette.cpp:398-440; PaintMetadata::filter, core/paint/paint.cpp:45-65), every float
 // of the record as the RGBA16F metadata texture holds it (core/renderer.cpp:167-250).
 static void compile_gradient(const pfh_gradient &g, const float *stop_offsets, const uint32_t *stop_rgba,
-                             int32_t row, uint32_t *ramp, PaintRec &rec) {
+                             const uint32_t *given_ramp, int32_t row, uint32_t *ramp, PaintRec &rec) {
     if ((g.kind != PFH_GRADIENT_LINEAR && g.kind != PFH_GRADIENT_RADIAL) ||
         (g.wrap != PFH_WRAP_CLAMP && g.wrap != PFH_WRAP_REPEAT))
         throw PfhError{PFH_E_ARG, "gradient kind or wrap out of range"};
     for (float v : g.line)
         if (!std::isfinite(v)) throw PfhError{PFH_E_ARG, "gradient line not finite"};
     std::vector<ColorStopH> stops;
+    if (g.stop_count < 0 && !given_ramp) throw PfhError{PFH_E_ARG, "gradient without stops needs its ramp"};
     for (int32_t i = 0; i < g.stop_count; i++) {
         const ColorStopH st{stop_offsets[g.stop_start + i], stop_rgba[g.stop_start + i]};
         if (!std::isfinite(st.offset)) throw PfhError{PFH_E_ARG, "gradient stop offset not finite"};
@@ -498,7 +499,9 @@ static void compile_gradient(const pfh_gradient &g, const float *stop_offsets, c
         while (at < stops.size() && stops[at].offset < st.offset) at++;
         stops.insert(stops.begin() + (ptrdiff_t)at, st);
     }
-    for (int x = 0; x < RAMP_TEXELS; x++) ramp[x] = gradient_sample(stops, ((float)x + 0.5f) / (float)RAMP_TEXELS);
+    if (g.stop_count < 0) std::memcpy(ramp, given_ramp, 4 * RAMP_TEXELS);
+    else
+        for (int x = 0; x < RAMP_TEXELS; x++) ramp[x] = gradient_sample(stops, ((float)x + 0.5f) / (float)RAMP_TEXELS);
     const float scale = 1.f / (float)RAMP_TEXELS;
     const float v0 = ((float)row + (float)(row + 1)) * 0.5f * scale;  // the row's centre
     float m11, m21, m12, m22, m13, m23;
@@ -544,8 +547,13 @@ static void compile_gradient(const pfh_gradient &g, const float *stop_offsets, c
 
 // Paint::is_opaque for a gradient paint (core/paint/paint.cpp:9-19, Gradient::is_opaque,
 // core/paint/gradient.cpp:58-70): white base colour, every stop opaque, at least one stop.
-static bool gradient_opaque(const pfh_gradient &g, const uint32_t *stop_rgba) {
-    if (g.stop_count <= 0) return false;
+static bool gradient_opaque(const pfh_gradient &g, const uint32_t *stop_rgba, const uint32_t *given_ramp) {
+    if (g.stop_count < 0) {  // a given ramp: opaque when every texel is
+        for (int x = 0; x < RAMP_TEXELS; x++)
+            if ((given_ramp[x] >> 24) != 255u) return false;
+        return true;
+    }
+    if (g.stop_count == 0) return false;
     for (int32_t i = 0; i < g.stop_count; i++)
         if ((stop_rgba[g.stop_start + i] >> 24) != 255u) return false;
     return true;
@@ -582,7 +590,10 @@ static void validate_desc(const pfh_scene_desc *d) {
                 throw PfhError{PFH_E_ARG, "path_gradient names no gradient"};
         for (int i = 0; i < d->n_gradients; i++) {
             const pfh_gradient &g = d->gradients[i];
-            if (g.stop_count < 0 || g.stop_start < 0 || (int64_t)g.stop_start + g.stop_count > d->n_gradient_stops)
+            if (g.stop_count < 0 && !d->gradient_ramps)
+                throw PfhError{PFH_E_ARG, "gradient without stops needs gradient_ramps"};
+            if (g.stop_count >= 0 &&
+                (g.stop_start < 0 || (int64_t)g.stop_start + g.stop_count > d->n_gradient_stops))
                 throw PfhError{PFH_E_ARG, "gradient stops outside the stop arrays"};
         }
     }
@@ -1089,10 +1100,12 @@ static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d) {
                 paints.emplace_back();
                 ramps.resize(ramps.size() + RAMP_TEXELS);
                 compile_gradient(d->gradients[gi], d->gradient_stop_offsets, d->gradient_stop_rgba,
+                                 d->gradient_ramps ? d->gradient_ramps + (size_t)gi * RAMP_TEXELS : nullptr,
                                  slot[gi] % RAMP_TEXELS, ramps.data() + ramps.size() - RAMP_TEXELS, paints.back());
             }
             s->path_paint[p] = (uint32_t)slot[gi];
-            s->path_rgba[p] = gradient_opaque(d->gradients[gi], d->gradient_stop_rgba) ? 0xff000000u : 0u;
+            const uint32_t *given = d->gradient_ramps ? d->gradient_ramps + (size_t)gi * RAMP_TEXELS : nullptr;
+            s->path_rgba[p] = gradient_opaque(d->gradients[gi], d->gradient_stop_rgba, given) ? 0xff000000u : 0u;
         }
         if (paints.empty()) s->path_paint.clear();
     }
@@ -2185,7 +2198,7 @@ int pfh_gradient_compile(const pfh_gradient *g, const float *stop_offsets, const
         return PFH_E_ARG;
     try {
         PaintRec rec{};
-        compile_gradient(*g, stop_offsets, stop_rgba, ramp_row, ramp, rec);
+        compile_gradient(*g, stop_offsets, stop_rgba, nullptr, ramp_row, ramp, rec);
         const float f[16] = {rec.m11, rec.m21, rec.m12, rec.m22, rec.m13, rec.m23, rec.from_x, rec.from_y,
                              rec.vec_x, rec.vec_y, rec.r0, rec.r1, rec.uv_x, rec.uv_y, (float)rec.kind, (float)rec.wrap};
         std::memcpy(record, f, sizeof(f));

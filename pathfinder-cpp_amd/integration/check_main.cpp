@@ -1,6 +1,7 @@
 // Drives the reference-side binding end to end: a Scene built through the reference's own API
 // (Scene, Path2d, Paint, ClipPath with a nested clip, a stroke made by the reference's
-// OutlineStrokeToFill as Canvas::stroke_path does), packed by SceneBuilderHip and drawn by
+// OutlineStrokeToFill as Canvas::stroke_path does, linear and radial gradient paints), packed by
+// SceneBuilderHip and drawn by
 // RendererHip on a DeviceHip / QueueHip into a TextureHip, read back with
 // CommandEncoder::read_texture.
 //   check pack <desc file>                 packs the scene (no device needed)
@@ -13,6 +14,7 @@
 #include <string>
 
 #include "core/data/path.h"
+#include "core/paint/gradient.h"
 #include "core/path2d.h"
 #include "core/scene.h"
 #include "core/stroke.h"
@@ -103,6 +105,45 @@ std::shared_ptr<Scene> make_scene() {
         dp.clip_path = std::make_shared<uint32_t>(outer_id);
         scene->push_draw_path(dp);
     }
+    // a band with a linear gradient (two of its stops at one offset, one translucent)
+    {
+        Gradient g = Gradient::linear(LineSegmentF(Vec2F(40.0f, 180.0f), Vec2F(260.0f, 200.0f)));
+        g.add_color_stop(ColorU(255, 200, 0, 255), 0.0f);
+        g.add_color_stop(ColorU(0, 120, 255, 255), 0.5f);
+        g.add_color_stop(ColorU(255, 0, 120, 200), 0.5f);
+        g.add_color_stop(ColorU(20, 20, 20, 255), 1.0f);
+        const uint32_t paint = scene->push_paint(Paint::from_gradient(g));
+        Path2d p;
+        p.move_to(40.0f, 160.0f);
+        p.line_to(260.0f, 160.0f);
+        p.line_to(260.0f, 212.0f);
+        p.line_to(40.0f, 212.0f);
+        p.close_path();
+        DrawPath dp;
+        dp.outline = p.into_outline();
+        dp.paint = (uint16_t)paint;
+        scene->push_draw_path(dp);
+    }
+    // a blob with a repeating radial gradient in a rotated, scaled gradient space
+    {
+        Gradient g = Gradient::radial(LineSegmentF(Vec2F(0.0f, 0.0f), Vec2F(6.0f, -4.0f)), Vec2F(4.0f, 40.0f));
+        g.wrap = GradientWrap::Repeat;
+        std::get<GradientRadial>(g.geometry).transform =
+            Transform2(Mat2(0.8f, 0.6f, -0.6f, 0.8f) * 1.25f, Vec2F(205.0f, 80.0f));
+        g.add_color_stop(ColorU(250, 250, 250, 255), 0.0f);
+        g.add_color_stop(ColorU(120, 0, 160, 255), 0.7f);
+        g.add_color_stop(ColorU(0, 90, 40, 255), 1.0f);
+        const uint32_t paint = scene->push_paint(Paint::from_gradient(g));
+        Path2d p;
+        p.move_to(160.0f, 80.0f);
+        p.cubic_to(160.0f, 30.0f, 250.0f, 30.0f, 250.0f, 80.0f);
+        p.cubic_to(250.0f, 130.0f, 160.0f, 130.0f, 160.0f, 80.0f);
+        p.close_path();
+        DrawPath dp;
+        dp.outline = p.into_outline();
+        dp.paint = (uint16_t)paint;
+        scene->push_draw_path(dp);
+    }
     return scene;
 }
 
@@ -120,6 +161,18 @@ void write_desc(const pfh_scene_desc &d, const char *path) {
     std::fwrite(d.path_fill_rule, 1, d.n_paths, f);
     std::fwrite(d.path_rgba, 4, d.n_paths, f);
     std::fwrite(d.path_clip, 4, d.n_paths, f);
+    // gradient paints: counts, then per path its gradient (-1: none), the records and the stops
+    const int32_t gh[2] = {d.path_gradient ? d.n_gradients : 0, d.path_gradient ? d.n_gradient_stops : 0};
+    std::fwrite(gh, 4, 2, f);
+    if (d.path_gradient) {
+        std::fwrite(d.path_gradient, 4, d.n_paths, f);
+        std::fwrite(d.gradients, sizeof(pfh_gradient), d.n_gradients, f);
+        std::fwrite(d.gradient_stop_offsets, 4, d.n_gradient_stops, f);
+        std::fwrite(d.gradient_stop_rgba, 4, d.n_gradient_stops, f);
+        const int32_t has_ramps = d.gradient_ramps ? 1 : 0;
+        std::fwrite(&has_ramps, 4, 1, f);
+        if (d.gradient_ramps) std::fwrite(d.gradient_ramps, 4, (size_t)d.n_gradients * 256, f);
+    }
     std::fclose(f);
 }
 

@@ -1,7 +1,9 @@
 // SceneBuilderHip / RendererHip (see renderer_hip.h).
 #include "renderer_hip.h"
 
+#include <algorithm>
 #include <stdexcept>
+#include <variant>
 
 #include "common/logger.h"
 #include "hip_gpu.h"
@@ -16,6 +18,9 @@ void SceneBuilderHip::build(Scene *scene_, Renderer *) {
     rule_.clear();
     rgba_.clear();
     clip_.clear();
+    grad_of_path_.clear();
+    grads_.clear();
+    ramps_.clear();
     cstart_.assign(1, 0);
     pstart_.assign(1, 0);
     auto add = [&](const Outline &outline, FillRule fill_rule, uint32_t rgba, const std::shared_ptr<uint32_t> &clip) {
@@ -33,12 +38,50 @@ void SceneBuilderHip::build(Scene *scene_, Renderer *) {
         rgba_.push_back(rgba);
         clip_.push_back(clip ? (int32_t)*clip : -1);
     };
+    // a gradient paint (Paint::from_gradient: its overlay holds the Gradient) as a pfh_gradient whose ramp
+    // is sampled here with the Gradient's own sample(), as GradientTileBuilder::allocate does
+    // (core/paint/gradient.cpp:115-119; the stops themselves are private to Gradient)
+    auto gradient_of = [&](const Paint &paint) -> int32_t {
+        const auto overlay = paint.get_overlay();
+        if (!overlay || !std::holds_alternative<Gradient>(overlay->contents)) return -1;
+        const Gradient &g = std::get<Gradient>(overlay->contents);
+        pfh_gradient r{};
+        r.wrap = g.wrap == GradientWrap::Repeat ? PFH_WRAP_REPEAT : PFH_WRAP_CLAMP;
+        r.transform[0] = r.transform[3] = 1.0f;
+        if (const auto *lin = std::get_if<GradientLinear>(&g.geometry)) {
+            r.kind = PFH_GRADIENT_LINEAR;
+            r.line[0] = lin->line.from().x, r.line[1] = lin->line.from().y;
+            r.line[2] = lin->line.to().x, r.line[3] = lin->line.to().y;
+        } else {
+            const auto &rad = std::get<GradientRadial>(g.geometry);
+            r.kind = PFH_GRADIENT_RADIAL;
+            r.line[0] = rad.line.from().x, r.line[1] = rad.line.from().y;
+            r.line[2] = rad.line.to().x, r.line[3] = rad.line.to().y;
+            r.radii[0] = rad.radii.x, r.radii[1] = rad.radii.y;
+            const float t[6] = {rad.transform.m11(), rad.transform.m21(), rad.transform.m12(),
+                                rad.transform.m22(), rad.transform.m13(), rad.transform.m23()};
+            std::copy(t, t + 6, r.transform);
+        }
+        r.stop_start = 0;
+        r.stop_count = -1;
+        for (uint32_t x = 0; x < GRADIENT_TILE_LENGTH; x++) {
+            const ColorU c = g.sample(((float)x + 0.5f) / GRADIENT_TILE_LENGTH);
+            ramps_.push_back((uint32_t)c.r_ | (uint32_t)c.g_ << 8 | (uint32_t)c.b_ << 16 | (uint32_t)c.a_ << 24);
+        }
+        grads_.push_back(r);
+        return (int32_t)grads_.size() - 1;
+    };
     for (const DrawPath &dp : scene->draw_paths) {
-        const ColorU c = scene->palette.get_paint(dp.paint).get_base_color();  // solid paints
+        const Paint paint = scene->palette.get_paint(dp.paint);
+        const ColorU c = paint.get_base_color();
         add(dp.outline, dp.fill_rule, (uint32_t)c.r_ | (uint32_t)c.g_ << 8 | (uint32_t)c.b_ << 16 | (uint32_t)c.a_ << 24,
             dp.clip_path);
+        grad_of_path_.push_back(gradient_of(paint));
     }
-    for (const ClipPath &cp : scene->clip_paths) add(cp.outline, cp.fill_rule, 0, cp.clip_path);  // never drawn
+    for (const ClipPath &cp : scene->clip_paths) {
+        add(cp.outline, cp.fill_rule, 0, cp.clip_path);  // never drawn
+        grad_of_path_.push_back(-1);
+    }
     const RectF vb = scene->get_view_box();
     desc_ = pfh_scene_desc{};
     desc_.width = (int32_t)std::ceil(vb.width());
@@ -62,6 +105,12 @@ void SceneBuilderHip::build(Scene *scene_, Renderer *) {
     desc_.path_stroke = nullptr;
     desc_.stroke_dashes = nullptr;
     desc_.n_stroke_dashes = 0;
+    if (!grads_.empty()) {
+        desc_.path_gradient = grad_of_path_.data();
+        desc_.gradients = grads_.data();
+        desc_.n_gradients = (int32_t)grads_.size();
+        desc_.gradient_ramps = ramps_.data();
+    }
 }
 
 RendererHip::RendererHip(const std::shared_ptr<Device> &device, const std::shared_ptr<Queue> &queue, int hip_device)

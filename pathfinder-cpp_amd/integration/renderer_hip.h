@@ -14,8 +14,9 @@
 namespace Pathfinder {
 
 // Packs Scene::draw_paths, then Scene::clip_paths, into a pfh_scene_desc: outlines as they are
-// (Canvas has applied transforms, strokes and dashes), solid paints as RGBA8, fill rules, and each
-// path's clip path (DrawPath::clip_path / ClipPath::clip_path) as an index among the clip paths.
+// (Canvas has applied transforms, strokes and dashes), solid paints as RGBA8, gradient paints as
+// pfh_gradient records, fill rules, and each path's clip path (DrawPath::clip_path /
+// ClipPath::clip_path) as an index among the clip paths.
 class SceneBuilderHip : public SceneBuilder {
 public:
     void build(Scene *scene_, Renderer *renderer) override;
@@ -25,8 +26,9 @@ private:
     pfh_scene_desc desc_{};
     std::vector<float> pts_;
     std::vector<uint8_t> flags_, closed_, rule_;
-    std::vector<int32_t> cstart_, pstart_, clip_;
-    std::vector<uint32_t> rgba_;
+    std::vector<int32_t> cstart_, pstart_, clip_, grad_of_path_;
+    std::vector<uint32_t> rgba_, ramps_;
+    std::vector<pfh_gradient> grads_;
 };
 
 // Renderer over the C ABI.  The destination is a texture of the device (a TextureHip, whose device

@@ -38,7 +38,8 @@ class SceneDesc(ctypes.Structure):
                 ("n_clip_paths", ctypes.c_int32), ("path_clip", _P), ("path_stroke", _P),
                 ("stroke_dashes", _P), ("n_stroke_dashes", ctypes.c_int32),
                 ("path_gradient", _P), ("gradients", _P), ("n_gradients", ctypes.c_int32),
-                ("gradient_stop_offsets", _P), ("gradient_stop_rgba", _P), ("n_gradient_stops", ctypes.c_int32)]
+                ("gradient_stop_offsets", _P), ("gradient_stop_rgba", _P), ("n_gradient_stops", ctypes.c_int32),
+                ("gradient_ramps", _P)]
 
 
 class Outline(ctypes.Structure):
@@ -273,7 +274,11 @@ class Context:
             grads = np.ascontiguousarray(np.asarray(scene.gradients, GRADIENT).reshape(-1))
             offs = np.ascontiguousarray(scene.gradient_stop_offsets, np.float32)
             rgba = np.ascontiguousarray(scene.gradient_stop_rgba, np.uint32)
-            keep = [pg, grads, offs, rgba]
+            ramps = getattr(scene, "gradient_ramps", None)
+            if ramps is not None:
+                ramps = np.ascontiguousarray(ramps, np.uint32)
+                d.gradient_ramps = _ptr(ramps)
+            keep = [pg, grads, offs, rgba, ramps]
             d.path_gradient, d.gradients, d.n_gradients = _ptr(pg), _ptr(grads), int(grads.shape[0])
             d.gradient_stop_offsets, d.gradient_stop_rgba = _ptr(offs), _ptr(rgba)
             d.n_gradient_stops = int(offs.shape[0])

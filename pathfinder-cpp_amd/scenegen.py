@@ -66,6 +66,7 @@ class Scene:
     gradients: np.ndarray = None
     gradient_stop_offsets: np.ndarray = None
     gradient_stop_rgba: np.ndarray = None
+    gradient_ramps: np.ndarray = None  # [n_gradients, 256] for gradients with stop_count < 0
 
     def __post_init__(self):
         if self.view_box is None:
@@ -97,9 +98,13 @@ class Scene:
         op = ((self.path_rgba >> 24) == 255).astype(np.uint8)
         if self.path_gradient is not None:
             for p in np.nonzero(self.path_gradient >= 0)[0]:
-                g = self.gradients[int(self.path_gradient[p])]
+                gi = int(self.path_gradient[p])
+                g = self.gradients[gi]
                 a, n = int(g["stop_start"]), int(g["stop_count"])
-                op[p] = 1 if n > 0 and ((self.gradient_stop_rgba[a:a + n] >> 24) == 255).all() else 0
+                if n < 0:  # a given ramp: opaque when every texel is
+                    op[p] = 1 if ((self.gradient_ramps[gi] >> 24) == 255).all() else 0
+                else:
+                    op[p] = 1 if n > 0 and ((self.gradient_stop_rgba[a:a + n] >> 24) == 255).all() else 0
         return op
 
     def subset(self, paths) -> "Scene":

@@ -24,5 +24,17 @@ def read_desc(path):
     take("path_fill_rule", np.uint8, n_paths)
     take("path_rgba", np.uint32, n_paths)
     take("path_clip", np.int32, n_paths)
+    n_grad, n_stops = [int(v) for v in np.frombuffer(raw, np.int32, 2, off)]
+    off += 8
+    if n_grad:
+        import scenegen
+        take("path_gradient", np.int32, n_paths)
+        take("gradients", scenegen.GRADIENT, n_grad)
+        take("gradient_stop_offsets", np.float32, n_stops)
+        take("gradient_stop_rgba", np.uint32, n_stops)
+        has_ramps = int(np.frombuffer(raw, np.int32, 1, off)[0])
+        off += 4
+        if has_ramps:
+            take("gradient_ramps", np.uint32, n_grad * 256, (n_grad, 256))
     assert off == len(raw), (off, len(raw))
     return out

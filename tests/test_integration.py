@@ -2,8 +2,9 @@
 HIP-backed Device / Queue / Texture / Buffer / CommandEncoder) compiled against the reference's own
 headers and linked with its scene, path, stroke, paint, renderer and gpu sources
 (integration/build.sh).  Its driver builds a Scene through the reference API — Scene, Path2d,
-Paint::from_color, two clip paths (one nested in the other), a stroke made by the reference's
-OutlineStrokeToFill — and SceneBuilderHip packs it.
+Paint::from_color and Paint::from_gradient (a linear and a radial gradient), two clip paths (one
+nested in the other), a stroke made by the reference's OutlineStrokeToFill — and SceneBuilderHip
+packs it.
 
 CPU (this file): the packed pfh_scene_desc holds exactly the outlines, colours, rules and clip
 indices the scene was built from (the stroke bit-identical to the oracle's stroker of the same
@@ -35,17 +36,29 @@ def expected():
     stroke = oracle.stroke(line, lflags, np.array([0, 7]), np.zeros(1, np.uint8), 9.5, 2, 2, 10.0)
     outer = [(20.5, 15.25), (270.75, 15.25), (270.75, 200.5), (20.5, 200.5)]
     inner = [(240, 110), (240, 170), (60, 190), (40, 110), (20, 30), (240, 50), (240, 110)]
+    band = [(40, 160), (260, 160), (260, 212), (40, 212)]
+    blob = [(160, 80), (160, 30), (250, 30), (250, 80), (250, 130), (160, 130), (160, 80)]
     paths = [
         [(np.array(star, np.float32), np.zeros(5, np.uint8), 1)],
         [(np.array(quad, np.float32), np.array([0, 1, 0, 1, 0], np.uint8), 1)],
         [(stroke[0][stroke[2][k]:stroke[2][k + 1]], stroke[1][stroke[2][k]:stroke[2][k + 1]], stroke[3][k])
          for k in range(len(stroke[3]))],
+        [(np.array(band, np.float32), np.zeros(4, np.uint8), 1)],
+        [(np.array(blob, np.float32), np.array([0, 1, 2, 0, 1, 2, 0], np.uint8), 1)],
         [(np.array(outer, np.float32), np.zeros(4, np.uint8), 1)],
         [(np.array(inner, np.float32), np.array([0, 1, 2, 0, 1, 2, 0], np.uint8), 1)],
     ]
-    return dict(paths=paths, rules=[1, 0, 0, 0, 1],
-                rgba=[rgba(230, 40, 40, 255), rgba(40, 80, 220, 160), rgba(30, 180, 60, 200), 0, 0],
-                clip=[1, -1, 0, -1, 0], n_clip=2, view_box=[0, 0, 304, 220])
+    white = rgba(255, 255, 255, 255)  # Paint::from_gradient's base colour
+    # the gradients as check_main.cpp adds them: (kind, wrap, line, radii, transform, stops in add order)
+    grads = [(0, 0, (40, 180, 260, 200), (0, 0), (1, 0, 0, 1, 0, 0),
+              [(0.0, rgba(255, 200, 0, 255)), (0.5, rgba(0, 120, 255, 255)), (0.5, rgba(255, 0, 120, 200)),
+               (1.0, rgba(20, 20, 20, 255))]),
+             (1, 1, (0, 0, 6, -4), (4, 40), (1.0, 0.75, -0.75, 1.0, 205, 80),
+              [(0.0, rgba(250, 250, 250, 255)), (0.7, rgba(120, 0, 160, 255)), (1.0, rgba(0, 90, 40, 255))])]
+    return dict(paths=paths, rules=[1, 0, 0, 0, 0, 0, 1],
+                rgba=[rgba(230, 40, 40, 255), rgba(40, 80, 220, 160), rgba(30, 180, 60, 200), white, white, 0, 0],
+                clip=[1, -1, 0, -1, -1, -1, 0], n_clip=2, view_box=[0, 0, 304, 220],
+                path_gradient=[-1, -1, -1, 0, 1, -1, -1], gradients=grads)
 
 
 @pytest.fixture(scope="module")
@@ -77,3 +90,21 @@ def test_scene_builder_hip_packs_the_reference_scene(packed):
     np.testing.assert_array_equal(packed["path_clip"], want["clip"])
     assert packed["n_clip_paths"] == want["n_clip"]
     np.testing.assert_array_equal(packed["view_box"], want["view_box"])  # right edge rounded to 16
+
+
+def test_scene_builder_hip_packs_gradient_paints(packed):
+    """Gradient paints go through as pfh_gradient records with the ramp sampled by the reference's own
+    Gradient::sample (its stops are private): each equals the reference's GradientTileBuilder ramp of
+    the stops check_main.cpp added, and the oracle's restatement of it."""
+    want = expected()
+    np.testing.assert_array_equal(packed["path_gradient"], want["path_gradient"])
+    for i, (kind, wrap, line, radii, xf, stops) in enumerate(want["gradients"]):
+        g = packed["gradients"][i]
+        assert (int(g["kind"]), int(g["wrap"]), int(g["stop_count"])) == (kind, wrap, -1)
+        np.testing.assert_array_equal(g["line"], np.float32(line))
+        np.testing.assert_array_equal(g["radii"], np.float32(radii))
+        np.testing.assert_array_equal(g["transform"], np.float32(xf))
+        off, col = [s[0] for s in stops], [s[1] for s in stops]
+        np.testing.assert_array_equal(packed["gradient_ramps"][i], oracle.gradient_ramp(off, col))
+        if oracle.ref_available():
+            np.testing.assert_array_equal(packed["gradient_ramps"][i], oracle.ref_gradient_ramp(off, col))
