@@ -145,7 +145,7 @@ def measure(ctx, ids, flags, steps, warmup, plain_steps, schedule=None):
     return plain_s, stage_ms, batched_s, fill_ms / steps, comp_ms / steps, ctx.stats(), tuning
 
 
-def serve(ctx, ids, qps, n_requests, seed, slots=3, budget=64 << 30, max_batch=128):
+def serve(ctx, ids, qps, n_requests, seed, slots=3, budget=64 << 30, max_batch=64):
     """Poisson stream of ``n_requests`` requests for scenes drawn uniformly from ``ids`` at ``qps``
     through the request loop; returns its stats and the achieved request rate (requests over the
     wall time from the first arrival to the last completion, the final drain included: a stream

@@ -192,7 +192,7 @@ struct BinItem {
     uint32_t rgba, rule;       // paint, fill rule
     uint32_t clip;             // CLIP_NONE / CLIP_EMPTY, or the bin item of the path's clip path
     uint32_t level;            // 0: a draw path; 1..4: a clip path at that nesting depth (never drawn)
-    uint32_t pad;
+    uint32_t paint;            // gradient paint index, or PAINT_NONE (a solid colour)
 };
 
 // Batch-wide work arrays (device pointers).  Sizes: S segments, T tiles,

@@ -805,7 +805,7 @@ __global__ __launch_bounds__(256) void k_bin_items(BatchDev b) {
         cr = (cr & CLIP_GROUP) ? b.n_bin_wave + sc.bin_group_base + (cr & ~CLIP_GROUP) : sc.bin_wave_base + cr;
     it.clip = cr;
     it.level = sc.path_level ? (uint32_t)sc.path_level[p] : 0u;
-    it.pad = 0;
+    it.paint = sc.path_paint ? sc.path_paint[p] : PAINT_NONE;
     const uint32_t gi = (GROUP ? b.n_bin_wave : 0u) + i;
     b.bin_items[gi] = it;
     const uint32_t tw = (uint32_t)max(it.rect.z - it.rect.x, 0);
@@ -1397,8 +1397,8 @@ struct EntryTile {
 // skips the slot), so the row is rasterized once.
 template <bool CLIPS>
 __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &sc, uint32_t p, uint32_t rgba,
-                                            uint32_t tbase, uint32_t j, const EntryTile &et, uint32_t e,
-                                            uint32_t ea) {
+                                            uint32_t paint, uint32_t tbase, uint32_t j, const EntryTile &et,
+                                            uint32_t e, uint32_t ea) {
     if (e >= b.cap_entries) return;  // past the capacity: the batch is retried
     const uint32_t ai = (et.st == TILE_ALPHA || et.st == TILE_CLIP_MASK) ? et.ai : NO_ALPHA;
     if (ai != NO_ALPHA && ea < b.cap_vis) {
@@ -1416,7 +1416,6 @@ __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &s
     uint32_t rx = ai, ry = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
     // a gradient path's entry: flagged, its mask row (or none) and its paint; a tile that covers
     // nothing stays a transparent solid entry
-    const uint32_t paint = sc.path_paint ? sc.path_paint[p] : PAINT_NONE;
     if (paint != PAINT_NONE) {
         rx = et.st == TILE_CLEAR ? NO_ALPHA : ENTRY_GRADIENT | (ai == NO_ALPHA ? ENTRY_ROW_NONE : min(ai, ENTRY_ROW_NONE - 1u));
         ry = et.st == TILE_CLEAR ? 0u : paint;
@@ -1477,7 +1476,7 @@ __device__ __forceinline__ uint2 range_entries(const BatchDev &b, const SceneDev
         const uint64_t m = __ballot(et.keep);
         const uint64_t ma = __ballot(et.keep && (et.st == TILE_ALPHA || (CLIPS && et.st == TILE_CLIP_MASK)));
         if (WRITE && et.keep)
-            write_entry<CLIPS>(b, sc, p, rgba, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
+            write_entry<CLIPS>(b, sc, p, rgba, it.paint, tbase, j, et, e + (uint32_t)__popcll(m & lt_mask),
                         n_alpha + (uint32_t)__popcll(ma & lt_mask));
         e += (uint32_t)__popcll(m);
         n_alpha += (uint32_t)__popcll(ma);
