@@ -46,6 +46,8 @@ extern "C" {
 /* Fill rules (core/data/data.h:58-63 FillRule). */
 #define PFH_FILL_NONZERO 0
 #define PFH_FILL_EVEN_ODD 1
+#define PFH_CLIP_RULE_ABS 0    /* pfh_scene_desc::clip_rule: the reference's clip coverage */
+#define PFH_CLIP_RULE_OWN 1    /* pfh_scene_desc::clip_rule: ClipPath::fill_rule honoured */
 
 typedef struct pfh_ctx pfh_ctx;
 
@@ -124,8 +126,9 @@ typedef struct pfh_scene_desc {
      * paths of the arrays above are clip paths, tiled like draw paths but never drawn.  path_clip[p]
      * (NULL: no path is clipped) is the clip path of path p (DrawPath::clip_path, or for a clip path
      * ClipPath::clip_path) as an index among the clip paths, or -1.  Resolution and mask combine
-     * follow core/d3d9/tiler.cpp:385-431 and shaders/d3d9/tile_clip_combine.frag:25-27: the clip's
-     * own fill rule is not used (its coverage enters as |winding sum|).  Nested clips (undefined in
+     * follow core/d3d9/tiler.cpp:385-431 and shaders/d3d9/tile_clip_combine.frag:25-27: by default
+     * the clip's own fill rule is not used (its coverage enters as |winding sum|; clip_rule below
+     * selects ClipPath::fill_rule instead).  Nested clips (undefined in
      * the reference's hybrid tiler, core/d3d9/scene_builder.cpp:209-216) follow the GPU-driven
      * renderer's levels (core/d3d11/scene_builder.cpp:92-133): a clip path is resolved against its
      * own clip path like a draw path, and its coverage is the min over the chain; chains of at most
@@ -154,6 +157,15 @@ typedef struct pfh_scene_desc {
      * caller sampled it (texel x = Gradient::sample((x + 0.5) / 256), as GradientTileBuilder::allocate
      * does; a binding that cannot read a Gradient's stops passes this); opaque when every texel is. */
     const uint32_t *gradient_ramps;
+    /* How a clip path's coverage enters its clip (ClipPath::fill_rule, core/scene.h, which neither of
+     * the reference's renderers reads):
+     *   PFH_CLIP_RULE_ABS (0, the reference): |winding sum| of each clip level, whatever its
+     *     path_fill_rule, the draw path's rule applied to the min (core/d3d9/tiler.cpp:385-431,
+     *     shaders/d3d9/tile_clip_combine.frag:25-27, tile.frag);
+     *   PFH_CLIP_RULE_OWN (1): every level of the chain under its own path_fill_rule and the draw
+     *     path under its own, coverage = the min of those; a tile without fills covers where its
+     *     rule says (an even-odd tile with an even backdrop covers nothing). */
+    int32_t clip_rule;
 } pfh_scene_desc;
 
 typedef struct pfh_stats {

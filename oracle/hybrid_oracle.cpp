@@ -435,7 +435,13 @@ static void tile_path(const Scene &sc, int p, PathTiling &pt) {
 // A clip path with a clip of its own (nesting, undefined in the hybrid tiler) is resolved against
 // it the same way before anything it clips is (the level structure of core/d3d11/scene_builder.cpp:
 // 92-133 and shaders/d3d11/propagate.comp:133-176), so `cp` always holds resolved tiles.
-static void resolve_clip(PathTiling &pt, const PathTiling &cp, int64_t clip_tile_base) {
+// "Solid" is backdrop != 0 in the reference (own_rules 0).  With own_rules (ClipPath::fill_rule
+// honoured, which the reference never reads) a tile without a mask is solid where its path's own
+// rule covers: an even-odd path's tile with an even backdrop is not (pt_eo / cp_eo: the rules).
+static bool covers(int bd, int even_odd, int own_rules) { return bd != 0 && !(own_rules && even_odd && bd % 2 == 0); }
+
+static void resolve_clip(PathTiling &pt, const PathTiling &cp, int64_t clip_tile_base, int pt_eo, int cp_eo,
+                         int own_rules) {
     int w = std::max(pt.rect[2] - pt.rect[0], 0), h = std::max(pt.rect[3] - pt.rect[1], 0);
     int cw = std::max(cp.rect[2] - cp.rect[0], 0), ch = std::max(cp.rect[3] - cp.rect[1], 0);
     pt.combine.assign((size_t)w * h, 0);
@@ -455,11 +461,11 @@ static void resolve_clip(PathTiling &pt, const PathTiling &cp, int64_t clip_tile
             pt.comb_bd[i] = pt.backdrop[i];
             pt.clip_src[i] = clip_tile_base + ci;
             pt.backdrop[i] = 0;
-        } else if (clip_alpha && pt.backdrop[i] != 0) {
+        } else if (clip_alpha && covers(pt.backdrop[i], pt_eo, own_rules)) {
             pt.alpha[i] = 1;
             pt.backdrop[i] = cp.backdrop[ci];
             pt.clip_src[i] = clip_tile_base + ci;
-        } else if (!clip_alpha && cp.backdrop[ci] == 0) {
+        } else if (!clip_alpha && !covers(cp.backdrop[ci], cp_eo, own_rules)) {
             pt.alpha[i] = 0;
             pt.backdrop[i] = 0;
         }
@@ -554,11 +560,12 @@ double pfo_apply_rule(double c, int even_odd) {
 //
 // Clip paths (Scene::clip_paths, core/scene.h) are the last n_clip paths; path_clip[p] (NULL: none)
 // names the clip path of draw path p by its index among them, or -1.  Clip paths are tiled like draw
-// paths (core/d3d9/scene_builder.cpp:200-225) but draw nothing themselves.
+// paths (core/d3d9/scene_builder.cpp:200-225) but draw nothing themselves.  clip_rule 0 resolves
+// clips as the reference does; 1 under each path's own fill rule (resolve_clip's own_rules).
 void *pfo_run(int n_paths, const int32_t *path_contour_start, const uint8_t *path_fill_rule,
               const uint8_t *path_opaque, const int32_t *contour_start, const uint8_t *contour_closed,
               const float *points, const uint8_t *point_flags, const float *view_box, int zmode, int n_clip,
-              const int32_t *path_clip) {
+              const int32_t *path_clip, int clip_rule) {
     auto *res = new Result();
     Scene sc{n_paths, path_contour_start, path_fill_rule, contour_start, contour_closed, points, point_flags,
              scene_view_box(view_box)};
@@ -586,13 +593,14 @@ void *pfo_run(int n_paths, const int32_t *path_contour_start, const uint8_t *pat
         for (int q = 0; q < n_clip; q++) {
             if (level[q] != lev) continue;
             int g = n_draw + path_clip[n_draw + q];
-            resolve_clip(res->paths[n_draw + q], res->paths[g], tile_base[g]);
+            resolve_clip(res->paths[n_draw + q], res->paths[g], tile_base[g], path_fill_rule[n_draw + q],
+                         path_fill_rule[g], clip_rule);
         }
     }
     for (int p = 0; path_clip && p < n_draw; p++) {
         if (path_clip[p] < 0) continue;
         int g = n_draw + path_clip[p];
-        resolve_clip(res->paths[p], res->paths[g], tile_base[g]);
+        resolve_clip(res->paths[p], res->paths[g], tile_base[g], path_fill_rule[p], path_fill_rule[g], clip_rule);
     }
     res->tiling_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 

@@ -42,7 +42,7 @@ def load_oracle():
             build()
         lib = ctypes.CDLL(ORACLE_SO)
         lib.pfo_run.restype = _P
-        lib.pfo_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, ctypes.c_int, _P]
+        lib.pfo_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, ctypes.c_int, _P, ctypes.c_int]
         lib.pfo_fetch_clip.argtypes = [_P] * 4
         lib.pfo_apply_rule_array.argtypes = [ctypes.c_int64, _P, _P, _P]
         lib.pfo_counts.argtypes = [_P, _P]
@@ -217,6 +217,7 @@ def apply_strokes(scene):
     out.view_box = scene.view_box.copy()
     out.n_clip_paths = scene.n_clip_paths
     out.path_clip = scene.path_clip
+    out.clip_rule = int(getattr(scene, "clip_rule", 0))
     for k in ("path_gradient", "gradients", "gradient_stop_offsets", "gradient_stop_rgba", "gradient_ramps"):
         setattr(out, k, getattr(scene, k, None))
     return out
@@ -254,7 +255,8 @@ def run_oracle(scene, zmode: int = 1) -> dict:
     backdrop never occlude (the rule the HIP path implements)."""
     lib = load_oracle()
     scene = apply_strokes(scene)
-    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(zmode), *_clip_args(scene))
+    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(zmode), *_clip_args(scene),
+                    ctypes.c_int(int(getattr(scene, "clip_rule", 0))))
     if not h:
         raise ValueError("clip paths nest deeper than 4 or in a cycle")
     counts = np.zeros(8, np.int64)
@@ -303,7 +305,7 @@ def raster_cpu(scene, threads: int = 1):
     fill + composite).  Scenes without clip paths."""
     assert not scene.n_clip_paths, "the CPU raster baseline covers scenes without clip paths"
     lib = load_oracle()
-    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(1), ctypes.c_int(0), None)
+    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(1), ctypes.c_int(0), None, ctypes.c_int(0))
     out = np.zeros((scene.height, scene.width, 4), np.uint8)
     secs = lib.pfo_raster_cpu(h, _ptr(scene.path_fill_rule), _ptr(scene.path_rgba), scene.width, scene.height,
                               int(threads), _ptr(out))
@@ -586,12 +588,26 @@ def clipped_coverage(canon: dict, at: dict, scene) -> np.ndarray:
       * a tile with a combine job: rule(min(|own sum + own backdrop|, K of the clip tile));
       * a solid tile that took the clip tile's mask: rule(K of the clip tile);
       * any other alpha tile: rule(own sum + backdrop).
-    A clip path's own masks (its dump rows) hold the nonzero rule of its K: min(K, 1)."""
+    A clip path's own masks (its dump rows) hold the nonzero rule of its K: min(K, 1).
+    With scene.clip_rule 1 (ClipPath::fill_rule honoured) every path's coverage is taken under its
+    own rule first: K = rule_clip(own sum + backdrop), a combine job gives min(rule(own sum + own
+    backdrop), K of the clip tile) and a tile that took the clip tile's mask K of it; a clip path's
+    dump rows hold its K."""
     raw = coverage_fp64(at, raw=True)
     src = canon["clip_src"][at["tile"]]
     comb = canon["clip_combine"][at["tile"]].astype(bool)
     comb_bd = canon["clip_comb_bd"][at["tile"]].astype(np.float64)[:, None, None]
     level = clip_levels(scene)[at["path"]]
+    if int(getattr(scene, "clip_rule", 0)) == 1:
+        K = apply_rule(raw, at["even_odd"])
+        for lev in sorted(set(int(v) for v in level[level > 0])) + [0]:
+            sel = np.nonzero((level == lev) & (src >= 0))[0]
+            if sel.size == 0:
+                continue
+            k_src = K[at["alpha_index"][src[sel]]]
+            own = apply_rule(raw[sel] + comb_bd[sel], at["even_odd"][sel])
+            K[sel] = np.where(comb[sel][:, None, None], np.minimum(own, k_src), k_src)
+        return K
     eff = raw.copy()
     K = np.abs(raw)
     # outermost clip level first: a tile's source clip tile lies one level further out, so its K

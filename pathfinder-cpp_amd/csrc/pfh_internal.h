@@ -44,8 +44,13 @@ constexpr uint32_t CLIP_EMPTY = 0xfffffffdu;    // clipped by a clip path with n
 constexpr uint32_t CLIP_GROUP = 0x80000000u;
 constexpr uint32_t MAX_CLIP_LEVELS = 4;         // clip paths in one nesting chain
 // clip_jobs[j].x: the mask row a combine job writes; CLIP_JOB_CLIP marks a nested clip path's tile,
-// whose two rows (nonzero, even-odd) both take the chain's coverage
+// whose two rows (nonzero, even-odd) both take the chain's coverage; CLIP_JOB_OWN_RULES marks a
+// scene whose clip paths' coverage is taken under their own fill rules (PFH_CLIP_RULE_OWN)
 constexpr uint32_t CLIP_JOB_CLIP = 0x80000000u;
+constexpr uint32_t CLIP_JOB_OWN_RULES = 0x40000000u;
+// BinItem::rule: the fill rule in bit 0; RULE_CLIP_OWN set for every path of a scene whose clip
+// paths' coverage is taken under their own fill rules
+constexpr uint32_t RULE_CLIP_OWN = 0x2u;
 
 // Binning ownership classes (a path's tiles and columns belong to it alone, so the wave or
 // workgroup that bins a path can count in LDS and store plainly).  A path goes to the wave
@@ -164,6 +169,7 @@ struct SceneDev {
     const uint32_t *path_seg_off; // [n_paths + 1] local segment offsets
     const uint32_t *bin_paths;    // local path indices: wave class, then group class
     const uint8_t *path_rule;     // 0 nonzero, 1 even-odd
+    uint32_t clip_own_rules;      // 1: clip paths' coverage under their own fill rules (PFH_CLIP_RULE_OWN)
     const uint32_t *path_clip;    // [n_paths] clip reference (CLIP_*), or null: no clip paths
     const uint8_t *path_level;    // [n_paths] clip level (0 draw path, 1..4 clip path), or null
     float vb_l, vb_t, vb_r, vb_b; // view box (right edge rounded up to 16)
@@ -189,7 +195,7 @@ struct BinItem {
     uint32_t tbase, cbase;     // batch index of its first tile / column
     uint32_t c0, c1;           // its chords [c0, c1)
     uint32_t p, gp, k;         // local path, batch path, scene
-    uint32_t rgba, rule;       // paint, fill rule
+    uint32_t rgba, rule;       // paint, fill rule (bit 0) | RULE_CLIP_OWN
     uint32_t clip;             // CLIP_NONE / CLIP_EMPTY, or the bin item of the path's clip path
     uint32_t level;            // 0: a draw path; 1..4: a clip path at that nesting depth (never drawn)
     uint32_t paint;            // gradient paint index, or PAINT_NONE (a solid colour)

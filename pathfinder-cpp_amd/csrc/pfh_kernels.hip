@@ -799,7 +799,7 @@ __global__ __launch_bounds__(256) void k_bin_items(BatchDev b) {
     it.gp = sc.path_base + p;
     it.k = (uint32_t)k;
     it.rgba = sc.path_rgba[p];
-    it.rule = sc.path_rule[p];
+    it.rule = sc.path_rule[p] | (sc.clip_own_rules ? RULE_CLIP_OWN : 0u);
     uint32_t cr = sc.path_clip ? sc.path_clip[p] : CLIP_NONE;
     if (cr < CLIP_EMPTY)
         cr = (cr & CLIP_GROUP) ? b.n_bin_wave + sc.bin_group_base + (cr & ~CLIP_GROUP) : sc.bin_wave_base + cr;
@@ -1099,6 +1099,7 @@ constexpr int COL_ROWS = 8;
 // Where a tile of a path column lands in its scene's framebuffer-tile grid.
 struct ColumnWalk {
     uint32_t p, rgba, rule, tbase, clip, level;
+    bool own_rules;  // clip coverage under each clip path's own fill rule (PFH_CLIP_RULE_OWN)
     int rx, ry;  // the path's rect origin (tiles)
     int w, h, x, fb_row0;  // fb_row0: fb tile index (scene-local) of row 0
     uint32_t fb_base, fb_w;
@@ -1113,7 +1114,8 @@ struct ColumnWalk {
         x = (int)(c - it.cbase);
         tbase = it.tbase;
         rgba = it.rgba;
-        rule = it.rule;
+        rule = it.rule & 1u;
+        own_rules = (it.rule & RULE_CLIP_OWN) != 0;
         clip = it.clip;
         level = it.level;
         rx = r.x;
@@ -1128,20 +1130,22 @@ struct ColumnWalk {
 
 // A clip path's tile at given tile coordinates, as resolved by the propagate phase of its clip
 // level (an earlier launch): whether it has a coverage mask (its own, or one taken from its own
-// clip), its backdrop, and its coverage source (the mask row whose record holds the chain).
+// clip), whether it covers without one (solid), its backdrop, and its coverage source (the mask
+// row whose record holds the chain).
 struct ClipTile {
-    bool inside, alpha;
+    bool inside, alpha, solid;
     int8_t bd;
     uint32_t src;
 };
 __device__ __forceinline__ ClipTile clip_tile_at(const BatchDev &b, const BinItem &ci, int tx, int ty) {
-    ClipTile r{false, false, 0, 0};
+    ClipTile r{false, false, false, 0, 0};
     const int4 cr = ci.rect;
     if (tx < cr.x || tx >= cr.z || ty < cr.y || ty >= cr.w) return r;
     const uint32_t ct = ci.tbase + (uint32_t)((ty - cr.y) * (cr.z - cr.x) + (tx - cr.x));
     const uint8_t st = b.tile_st[ct];
     r.inside = true;
     r.alpha = st == TILE_ALPHA || st == TILE_CLIP_MASK;
+    r.solid = st == TILE_SOLID;
     r.bd = b.tile_bd[ct];
     r.src = r.alpha ? b.tile_clip_ai[ct] : 0u;
     return r;
@@ -1166,7 +1170,7 @@ struct ClipRead {
     uint4 pr;
 };
 __device__ __forceinline__ ClipRead clip_read(const BatchDev &b, const ColumnWalk &cw, const BinItem &ci, int row) {
-    ClipRead r{{false, false, 0, 0}, make_uint4(1u, 0u, 0u, 0u)};
+    ClipRead r{{false, false, false, 0, 0}, make_uint4(1u, 0u, 0u, 0u)};
     if (cw.clip < CLIP_EMPTY) r.ct = clip_tile_at(b, ci, cw.rx + cw.x, cw.ry + row);
     if (cw.level != 0 && r.ct.alpha && r.ct.src < b.cap_alpha) r.pr = b.clip_rec[r.ct.src];
     return r;
@@ -1183,22 +1187,29 @@ __device__ __forceinline__ void propagate_tile(const BatchDev &b, const ColumnWa
     const uint32_t t = cw.tile(row);
     const bool alpha = f1 > f0;
     const ClipTile &ct = cr.ct;
+    // Whether the tile covers without fills: backdrop != 0, except under the path's own even-odd
+    // rule with an even backdrop where clip paths' own rules are taken (PFH_CLIP_RULE_OWN); the
+    // reference's clip resolution tests backdrop != 0 alone (tiler.cpp:396-431).
+    const bool covers = bd != 0 && !(CLIPS && cw.own_rules && cw.rule && (bd & 1) == 0);
+    const uint32_t job_rules = CLIPS && cw.own_rules ? CLIP_JOB_OWN_RULES : 0u;
     if (CLIPS && is_clip) {
         // A clip path draws nothing itself.  Its fills make two mask rows (ai, ai + 1) that
-        // keep |sum + backdrop| under the nonzero and the even-odd rule, for the tiles that
-        // take them over; its coverage at the tile is that value, min'd with its own clip's
-        // where it is nested and both have masks (a combine job rewrites both rows).
+        // keep |sum + backdrop| under the nonzero and the even-odd rule (under its own rule,
+        // both, where clip paths' own rules are taken), for the tiles that take them over; its
+        // coverage at the tile is that value, min'd with its own clip's where it is nested and
+        // both have masks (a combine job rewrites both rows).
         const bool own = alpha && ai + 1 < b.cap_alpha;
         if (own) {
-            b.alpha_rec[ai] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd, t);
-            b.alpha_rec[ai + 1] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd | (1u << 8), t);
+            const uint32_t r0 = cw.own_rules ? cw.rule : 0u, r1 = cw.own_rules ? cw.rule : 1u;
+            b.alpha_rec[ai] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd | (r0 << 8), t);
+            b.alpha_rec[ai + 1] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd | (r1 << 8), t);
         }
-        uint8_t st = alpha ? TILE_ALPHA : (bd != 0 ? TILE_SOLID : TILE_EMPTY);
+        uint8_t st = alpha ? TILE_ALPHA : (covers ? TILE_SOLID : TILE_EMPTY);
         int8_t out_bd = bd;
         uint32_t src = ai;
         uint4 rec = make_uint4(1u, 0u, 0u, 0u);  // the chain: this row alone
         if (clipped) {
-            if (!ct.inside || (!ct.alpha && ct.bd == 0)) {
+            if (!ct.inside || (!ct.alpha && !ct.solid)) {
                 st = TILE_EMPTY;
                 out_bd = 0;
             } else if (ct.alpha && alpha) {
@@ -1206,8 +1217,8 @@ __device__ __forceinline__ void propagate_tile(const BatchDev &b, const ColumnWa
                 const uint4 pr = cr.pr;
                 rec = make_uint4(1u + min(pr.x, MAX_CLIP_LEVELS - 1u), ct.src, pr.y, pr.z);
                 const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
-                if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai | CLIP_JOB_CLIP, ai);
-            } else if (ct.alpha && bd != 0) {
+                if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai | CLIP_JOB_CLIP | job_rules, ai);
+            } else if (ct.alpha && covers) {
                 st = TILE_CLIP_MASK;  // its own clip's coverage, unchanged
                 out_bd = ct.bd;
                 src = ct.src;
@@ -1224,14 +1235,14 @@ __device__ __forceinline__ void propagate_tile(const BatchDev &b, const ColumnWa
             b.alpha_rec[ai] = make_uint4(f0, f1, (uint32_t)(uint8_t)bd | (cw.rule << 8), t);
         uint8_t st;
         int8_t out_bd = 0;
-        if (!ct.inside || (!ct.alpha && ct.bd == 0)) {
+        if (!ct.inside || (!ct.alpha && !ct.solid)) {
             st = TILE_EMPTY;  // outside the clip's rect or over a blank clip tile: culled
         } else if (ct.alpha && alpha) {
             // both masks: min(|own + backdrop|, clip coverage) replaces the mask; backdrop 0
             st = TILE_ALPHA;
             const uint32_t j = atomicAdd(&b.counters[C_CLIP_JOBS], 1u);
-            if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai, ct.src);
-        } else if (ct.alpha && bd != 0) {
+            if (j < b.cap_alpha) b.clip_jobs[j] = make_uint2(ai | job_rules, ct.src);
+        } else if (ct.alpha && covers) {
             st = TILE_CLIP_MASK;  // solid over a clip mask: the clip's mask and backdrop
             out_bd = ct.bd;
             b.tile_clip_ai[t] = ct.src + cw.rule;
@@ -1309,7 +1320,7 @@ __global__ __launch_bounds__(256) void k_propagate(BatchDev b, uint32_t phase, u
             ai[i] = 0;
             // with clip paths the group's loads go out at once (the walk is latency-bound there)
             if (row0 + i < cw.h && (CLIPS || f1[i] > f0[i])) ai[i] = b.tile_alpha_idx[cw.tile(row0 + i)];
-            cr[i] = ClipRead{{false, false, 0, 0}, make_uint4(1u, 0u, 0u, 0u)};
+            cr[i] = ClipRead{{false, false, false, 0, 0}, make_uint4(1u, 0u, 0u, 0u)};
             if (CLIPS && row0 + i < cw.h) cr[i] = clip_read(b, cw, ci, row0 + i);
         }
 #pragma unroll
@@ -1349,7 +1360,7 @@ __global__ __launch_bounds__(256) void k_propagate_rows(BatchDev b, uint32_t pha
             const bool on = row < h;
             int32_t d8 = 0;
             uint32_t f0 = 0, f1 = 0, ai = 0;
-            ClipRead cr{{false, false, 0, 0}, make_uint4(1u, 0u, 0u, 0u)};
+            ClipRead cr{{false, false, false, 0, 0}, make_uint4(1u, 0u, 0u, 0u)};
             if (on) {
                 const uint32_t t = cw.tile(row);
                 d8 = (int8_t)b.tile_delta[t];

@@ -700,7 +700,9 @@ __device__ __forceinline__ float fill_rule_f(float c, uint32_t even_odd) {
 // chain of terms |fills + backdrop| of mask rows: the draw tile's own row (a draw job), then the
 // clip's coverage source and the rows its record lists (the clip chain, up to MAX_CLIP_LEVELS).
 // A draw job writes the draw path's fill rule of it to its row; a nested clip path's job writes
-// the nonzero and the even-odd rule of it to its two rows.  Each lane owns 4 pixels of one column;
+// the nonzero and the even-odd rule of it to its two rows.  Where clip paths' own rules are taken
+// (CLIP_JOB_OWN_RULES) each term is its row's fill rule of fills + backdrop instead, and the min
+// of those is the coverage, written as it is.  Each lane owns 4 pixels of one column;
 // the 4 waves take every 4th fill of every term and their partial sums meet in LDS, so a long fill
 // list costs a quarter of its length in serial steps.
 constexpr int CLIP_TERMS = MAX_CLIP_LEVELS + 1;
@@ -712,8 +714,8 @@ __global__ __launch_bounds__(256) void k_fill_clipped(BatchDev b) {
     const float px = (float)(lane & 15u), py0 = (float)((lane >> 4) * 4u);
     for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {  // uniform over the block
         const uint2 job = b.clip_jobs[j];
-        const bool clip_job = (job.x & CLIP_JOB_CLIP) != 0;
-        const uint32_t out = job.x & ~CLIP_JOB_CLIP, src = job.y;
+        const bool clip_job = (job.x & CLIP_JOB_CLIP) != 0, own_rules = (job.x & CLIP_JOB_OWN_RULES) != 0;
+        const uint32_t out = job.x & ~(CLIP_JOB_CLIP | CLIP_JOB_OWN_RULES), src = job.y;
         bool ok = out + (clip_job ? 1u : 0u) < b.cap_alpha && src < b.cap_alpha;
         // terms: the draw tile's own row (draw jobs), then the clip chain (fixed slots, so that
         // the arrays stay in registers)
@@ -724,12 +726,15 @@ __global__ __launch_bounds__(256) void k_fill_clipped(BatchDev b) {
 #pragma unroll
         for (int k = 0; k < CLIP_TERMS; k++) ok = ok && ((uint32_t)k < first || (uint32_t)k >= nt || term[k] < b.cap_alpha);
         int8_t bdk[CLIP_TERMS];
+        uint32_t rk[CLIP_TERMS];  // each term's fill rule (its row's)
 #pragma unroll
         for (int k = 0; k < CLIP_TERMS; k++) {
             bdk[k] = 0;
+            rk[k] = 0;
             if ((uint32_t)k < first || (uint32_t)k >= nt) continue;
             const uint4 r = ok ? b.alpha_rec[term[k]] : make_uint4(0, 0, 0, 0);
             bdk[k] = (int8_t)(r.z & 0xffu);
+            rk[k] = (r.z >> 8) & 1u;
             float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             const uint32_t f0 = min(r.x, b.cap_fills), f1 = max(f0, min(r.y, b.cap_fills));
             for (uint32_t f = f0 + wv; f < f1; f += 4) {
@@ -752,10 +757,11 @@ __global__ __launch_bounds__(256) void k_fill_clipped(BatchDev b) {
                     if ((uint32_t)k < first || (uint32_t)k >= nt) continue;
                     const float v = (float)bdk[k] + part[k][0][q][lane] + part[k][1][q][lane] + part[k][2][q][lane] +
                                     part[k][3][q][lane];
-                    m = fminf(m, fabsf(v));
+                    m = fminf(m, own_rules ? fill_rule_f(v, rk[k]) : fabsf(v));
                 }
-                packed |= (uint32_t)__float2int_rn(fill_rule_f(m, rule) * 255.0f) << (8 * q);
-                packed_eo |= (uint32_t)__float2int_rn(fill_rule_f(m, 1u) * 255.0f) << (8 * q);
+                const float c = own_rules ? m : fill_rule_f(m, rule), c_eo = own_rules ? m : fill_rule_f(m, 1u);
+                packed |= (uint32_t)__float2int_rn(c * 255.0f) << (8 * q);
+                packed_eo |= (uint32_t)__float2int_rn(c_eo * 255.0f) << (8 * q);
             }
             // column-major mask: byte 16 x + y
             const uint32_t off = (lane & 15u) * 16 + (lane >> 4) * 4;

@@ -176,6 +176,7 @@ struct SceneHost {
     std::vector<uint32_t> path_tile_off, path_col_off, path_seg_off, path_rgba;
     std::vector<uint8_t> path_rule;
     uint32_t n_clip = 0;                // trailing clip paths
+    bool clip_own_rules = false;        // PFH_CLIP_RULE_OWN
     std::vector<uint32_t> path_clip;    // clip reference per path (CLIP_*)
     std::vector<uint8_t> path_level;    // clip level per path (0 draw path, 1..4 clip path)
     uint32_t clip_levels = 0;           // deepest clip level
@@ -581,6 +582,8 @@ static void validate_desc(const pfh_scene_desc *d) {
         if (d->path_fill_rule[p] > 1) throw PfhError{PFH_E_ARG, "path_fill_rule must be 0 (nonzero) or 1 (even-odd)"};
     if (d->n_clip_paths < 0 || d->n_clip_paths > d->n_paths)
         throw PfhError{PFH_E_ARG, "n_clip_paths must lie in [0, n_paths]"};
+    if (d->clip_rule != PFH_CLIP_RULE_ABS && d->clip_rule != PFH_CLIP_RULE_OWN)
+        throw PfhError{PFH_E_ARG, "clip_rule must be PFH_CLIP_RULE_ABS (0) or PFH_CLIP_RULE_OWN (1)"};
     if (d->path_gradient) {
         if (d->n_gradients < 0 || d->n_gradient_stops < 0 || (d->n_gradients && !d->gradients) ||
             (d->n_gradient_stops && (!d->gradient_stop_offsets || !d->gradient_stop_rgba)))
@@ -1204,6 +1207,7 @@ static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d) {
     s->n_bin_group = (uint32_t)bin_group.size();
     // clip references: a clip path's binning position (or CLIP_EMPTY without tile columns)
     s->n_clip = (uint32_t)d->n_clip_paths;
+    s->clip_own_rules = d->clip_rule == PFH_CLIP_RULE_OWN;
     s->path_clip.assign(d->n_paths, CLIP_NONE);
     if (s->n_clip > 0) {
         const int n_draw = d->n_paths - d->n_clip_paths;
@@ -1438,6 +1442,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.bin_wave_base = (uint32_t)BW;
         d.bin_group_base = (uint32_t)BG;
         d.path_rule = s.d_path_rule.as<uint8_t>();
+        d.clip_own_rules = s.clip_own_rules ? 1u : 0u;
         d.path_clip = s.n_clip > 0 ? s.d_path_clip.as<uint32_t>() : nullptr;
         d.path_level = s.n_clip > 0 ? s.d_path_level.as<uint8_t>() : nullptr;
         const bool painted = !s.path_paint.empty();

@@ -102,6 +102,7 @@ void SceneBuilderHip::build(Scene *scene_, Renderer *) {
     desc_.path_rgba = rgba_.data();
     desc_.n_clip_paths = (int32_t)scene->clip_paths.size();
     desc_.path_clip = clip_.data();
+    desc_.clip_rule = clip_rule_;
     desc_.path_stroke = nullptr;
     desc_.stroke_dashes = nullptr;
     desc_.n_stroke_dashes = 0;

@@ -21,9 +21,13 @@ class SceneBuilderHip : public SceneBuilder {
 public:
     void build(Scene *scene_, Renderer *renderer) override;
     const pfh_scene_desc &desc() const { return desc_; }
+    // PFH_CLIP_RULE_ABS (the reference's clip coverage, the default) or PFH_CLIP_RULE_OWN
+    // (ClipPath::fill_rule honoured), for the scenes built after the call.
+    void set_clip_rule(int32_t rule) { clip_rule_ = rule; }
 
 private:
     pfh_scene_desc desc_{};
+    int32_t clip_rule_ = PFH_CLIP_RULE_ABS;
     std::vector<float> pts_;
     std::vector<uint8_t> flags_, closed_, rule_;
     std::vector<int32_t> cstart_, pstart_, clip_, grad_of_path_;

@@ -39,7 +39,7 @@ class SceneDesc(ctypes.Structure):
                 ("stroke_dashes", _P), ("n_stroke_dashes", ctypes.c_int32),
                 ("path_gradient", _P), ("gradients", _P), ("n_gradients", ctypes.c_int32),
                 ("gradient_stop_offsets", _P), ("gradient_stop_rgba", _P), ("n_gradient_stops", ctypes.c_int32),
-                ("gradient_ramps", _P)]
+                ("gradient_ramps", _P), ("clip_rule", ctypes.c_int32)]
 
 
 class Outline(ctypes.Structure):
@@ -59,6 +59,7 @@ class OutlineBuf(ctypes.Structure):
 
 
 PFH_CAP_BUTT, PFH_CAP_SQUARE, PFH_CAP_ROUND = 0, 1, 2
+CLIP_RULE_ABS, CLIP_RULE_OWN = 0, 1  # pfh_scene_desc::clip_rule (PFH_CLIP_RULE_*)
 PFH_JOIN_MITER, PFH_JOIN_BEVEL, PFH_JOIN_ROUND = 0, 1, 2
 
 
@@ -259,6 +260,7 @@ class Context:
         d.n_clip_paths = int(getattr(scene, "n_clip_paths", 0))
         path_clip = getattr(scene, "path_clip", None)
         d.path_clip = None if path_clip is None else _ptr(path_clip)
+        d.clip_rule = int(getattr(scene, "clip_rule", CLIP_RULE_ABS))
         path_stroke = getattr(scene, "path_stroke", None)
         if path_stroke is not None:
             path_stroke = np.ascontiguousarray(stroke_styles(path_stroke))

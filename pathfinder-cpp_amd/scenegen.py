@@ -54,6 +54,9 @@ class Scene:
     # clip path of draw path p as an index among them, or -1 (None: no path is clipped).
     n_clip_paths: int = 0
     path_clip: np.ndarray = None
+    # How a clip path's coverage enters: 0 its |winding sum| whatever its fill rule (the reference),
+    # 1 under its own path_fill_rule (ClipPath::fill_rule honoured).
+    clip_rule: int = 0
     # Strokes (Canvas::stroke_path): None, or per path (line_width, line_cap, line_join, miter_limit)
     # and optionally (dash_start, dash_count, dash_offset); a path with line_width > 0 is stroked at
     # upload (dashed first with stroke_dashes[dash_start:dash_start + dash_count] when dash_count > 0)
@@ -298,6 +301,29 @@ def stress_clip_scene(seed: int = BASE_SEED ^ 4, size: int = 4096, n_stars: int 
     pc[rng.random(n_stars) >= clipped] = -1
     sc.n_clip_paths = n_clip
     sc.path_clip = np.concatenate([pc, np.full(n_clip, -1, np.int32)])
+    return sc
+
+
+def star_clip_scene(seed: int, size: int = 2048, n_stars: int = 300, n_clip: int = 40, n_max: int = 400,
+                    clipped: float = 0.6, clip_rule: int = 1) -> Scene:
+    """Star polygons clipped by star polygons: windings far above 1 under both fill rules at every
+    level, where the clip paths' own rules (clip_rule 1) and the reference's |winding| differ most.
+    The clip stars form nesting chains (each, by a coin, clipped by the one before it, to depth 4);
+    each draw star is clipped by a random clip star with probability `clipped`."""
+    stars = star_scene(seed=seed, size=size, n_stars=n_stars, n_max=n_max)
+    clips = star_scene(seed=seed ^ 0xC11, size=size, n_stars=n_clip, n_min=20, n_max=200, m_max=12,
+                       r_min=size / 16, r_max=size / 3)
+    sc = concat_paths(stars, clips)
+    rng = _rng(seed ^ 0xC12)
+    pc = rng.integers(0, n_clip, n_stars).astype(np.int32)
+    pc[rng.random(n_stars) >= clipped] = -1
+    parent, depth = np.full(n_clip, -1, np.int32), np.ones(n_clip, np.int64)
+    for q in range(1, n_clip):
+        if depth[q - 1] < 4 and rng.integers(0, 2):
+            parent[q], depth[q] = q - 1, depth[q - 1] + 1
+    sc.n_clip_paths = n_clip
+    sc.path_clip = np.concatenate([pc, parent])
+    sc.clip_rule = clip_rule
     return sc
 
 

@@ -245,8 +245,8 @@ def test_clip_model_windings_above_one():
     d = [(8, 8, 56, 56), (24, 24, 72, 72)]        # winding 2 in [24, 56)^2
     c = [(16, 0, 64, 64), (32, 16, 80, 48)]       # winding 2 in [32, 64) x [16, 48)
     eo = np.ones(1, np.uint8)
-    for clip_rule in (0, 1):
-        sc = _rect_scene(96, 96, [d, c], [1, clip_rule], 1, [0, -1])
+    for crule in (0, 1):
+        sc = _rect_scene(96, 96, [d, c], [1, crule], 1, [0, -1])
         m = oracle.full_model(sc)
         got = oracle.path_coverage_image(sc, m, 0)
         wd = sum(_box_area(96, 96, r) for r in d)
@@ -260,7 +260,7 @@ def test_clip_model_windings_above_one():
             clip_mask[ty * 16:ty * 16 + 16, tx * 16:tx * 16 + 16] = canon["tile_alpha"][t0 + i] == 1
         want = np.where(clip_mask, oracle.apply_rule(np.minimum(wd, wc), eo), oracle.apply_rule(wd, eo))
         want[wc == 0] = 0.0
-        assert np.abs(got - want).max() <= 1.0 / 255.0, clip_rule  # 8.8 clamp, as above
+        assert np.abs(got - want).max() <= 1.0 / 255.0, crule  # 8.8 clamp, as above
         both = (wd == 2) & (wc == 2) & clip_mask
         assert both.any() and (got[both] <= 1.0 / 255.0).all()
         assert ((wd == 2) & (wc == 1) & clip_mask).any()
@@ -357,3 +357,111 @@ def test_clip_fill_rule_is_ignored_where_windings_reach_two():
     assert np.abs(overlap - 1.0).max() <= 1.0 / 255.0
     assert np.abs(single - 1.0).max() <= 1.0 / 255.0
     assert cov[11:12, 60:100].max() == 0.0  # inside the draw rect, above both clip rects: clipped away
+
+
+def _rule(w, even_odd):
+    """The fill rule of integer windings (pixel-aligned rects): nonzero |w| >= 1, even-odd w odd."""
+    w = np.rint(w)
+    return (w % 2 != 0).astype(np.float64) if even_odd else (w != 0).astype(np.float64)
+
+
+# Draw path, outer clip, inner clip: two overlapping rects each (winding 2 where they overlap, and the
+# overlaps span whole 16 x 16 tiles, so solid tiles with backdrop 2 meet every case of the
+# resolution: over a clip mask, over a solid clip tile, under a solid draw tile).
+OWN_D = [(8, 8, 88, 88), (24, 24, 72, 72)]        # winding 2 in [24, 72)^2
+OWN_C1 = [(4, 4, 92, 84), (16, 30, 64, 92)]       # winding 2 in [16, 64) x [30, 84)
+OWN_C2 = [(12, 0, 80, 76), (28, 14, 90, 50)]      # winding 2 in [28, 80) x [14, 50)
+
+
+@pytest.mark.parametrize("rd,rc", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_own_clip_rules_are_the_min_of_each_rule(rd, rc):
+    """clip_rule 1 (ClipPath::fill_rule honoured): the coverage is min(rule_draw(w_draw),
+    rule_clip(w_clip)) at every pixel, with no dependence on which tiles carry masks; pixel-aligned
+    rects make both exact 0/1 (tolerance 1/255 for the 8.8 clamp at tile edges, as above).  The
+    reference's |winding| form differs from it where an even winding meets a mask: checked to
+    differ on this scene whenever either rule is even-odd."""
+    sc = _rect_scene(96, 96, [OWN_D, OWN_C1], [rd, rc], 1, [0, -1])
+    sc.clip_rule = 1
+    got = oracle.path_coverage_image(sc, oracle.full_model(sc), 0)
+    wd = sum(_box_area(96, 96, r) for r in OWN_D)
+    wc = sum(_box_area(96, 96, r) for r in OWN_C1)
+    want = np.minimum(_rule(wd, rd), _rule(wc, rc))
+    assert np.abs(got - want).max() <= 1.0 / 255.0
+    sc.clip_rule = 0
+    ref = oracle.path_coverage_image(sc, oracle.full_model(sc), 0)
+    differ = np.abs(ref - want).max() > 0.5
+    assert differ == bool(rd or rc)
+
+
+@pytest.mark.parametrize("rules", [(1, 1, 1), (0, 1, 0), (1, 0, 1), (0, 0, 1)])
+def test_own_clip_rules_nested_chain(rules):
+    """A draw path under a chain of two clip paths, each with windings up to 2 and its own rule: the
+    coverage is the min of the three rules' values at every pixel (clip_rule 1), whatever the tile
+    cases at each level; outside the outer clip nothing.  Tolerance 2/256: the outer clip has two
+    horizontal edges across the same tile row, each clamped 1/256 short in a tile's last pixel
+    column (the 8.8 clamp, as above), which the even-odd rule shows at winding 2."""
+    sc = _rect_scene(96, 96, [OWN_D, OWN_C1, OWN_C2], list(rules), 2, [1, -1, 0])
+    sc.clip_rule = 1
+    got = oracle.path_coverage_image(sc, oracle.full_model(sc), 0)
+    want = np.ones((96, 96))
+    for rects, r in zip((OWN_D, OWN_C1, OWN_C2), rules):
+        want = np.minimum(want, _rule(sum(_box_area(96, 96, x) for x in rects), r))
+    assert np.abs(got - want).max() <= 2.0 / 256.0
+    assert np.abs(got - want).mean() < 1e-3
+    assert want.max() == 1.0 and (want == 0).any()
+
+
+def test_own_clip_rules_solid_tiles_of_even_windings_cover_nothing():
+    """A canvas-wide draw rect under an even-odd clip of two nested rects (winding 2 over whole
+    tiles inside): with clip_rule 1 the clip's even tiles cull the draw tiles under them, and a solid
+    draw tile of even winding under an even-odd draw rule takes no clip mask (coverage 0 there);
+    with clip_rule 0 the reference's |winding| keeps the inner square covered."""
+    clip = [(8, 8, 88, 88), (32, 32, 72, 72)]  # no two horizontal edges in one tile row (the 8.8 clamp)
+    sc = _rect_scene(96, 96, [[(0, 0, 96, 96)], clip], [0, 1], 1, [0, -1])
+    outer, inner = _box_area(96, 96, clip[0]), _box_area(96, 96, clip[1])
+    sc.clip_rule = 1
+    m = oracle.full_model(sc)
+    assert np.abs(oracle.path_coverage_image(sc, m, 0) - (outer - inner)).max() <= 1.0 / 255.0
+    canon = m["canon"]
+
+    def tile(p, tx, ty):
+        l, t, r, b = [int(v) for v in canon["path_rect"][p]]
+        return int(canon["tile_off"][p]) + (ty - t) * (r - l) + (tx - l)
+
+    # the clip's solid tiles of winding 2 (no mask) cull the draw tiles over them
+    even = [(tx, ty) for ty in range(6) for tx in range(6)
+            if not canon["tile_alpha"][tile(1, tx, ty)] and canon["tile_backdrop"][tile(1, tx, ty)] in (-2, 2)]
+    assert even
+    for tx, ty in even:
+        assert canon["tile_backdrop"][tile(0, tx, ty)] == 0 and not canon["tile_alpha"][tile(0, tx, ty)]
+    sc.clip_rule = 0
+    assert np.abs(oracle.path_coverage_image(sc, oracle.full_model(sc), 0) - outer).max() <= 1.0 / 255.0
+    # solid draw tiles of winding 2 under the even-odd rule over a nonzero clip's masks (its bottom
+    # edge crosses them): the reference's tiles take the clip mask under their rule (covered), here
+    # they cover nothing
+    d = [(0, 0, 96, 96), (16, 16, 80, 80)]
+    c = (4.5, 4.5, 91.5, 50.5)
+    sc = _rect_scene(96, 96, [d, [c]], [1, 0], 1, [0, -1])
+    want = np.minimum(_rule(sum(_box_area(96, 96, r) for r in d), 1), _box_area(96, 96, c))
+    for clip_rule in (0, 1):
+        sc.clip_rule = clip_rule
+        got = oracle.path_coverage_image(sc, oracle.full_model(sc), 0)
+        err = np.abs(got - want)
+        if clip_rule:
+            assert err.max() <= 1.0 / 255.0
+        else:
+            assert np.abs(got[48:50, 16:80] - 1.0).max() <= 1.0 / 255.0  # the reference's quirk
+
+
+def test_own_clip_rules_golden_evenodd_clip():
+    """The golden clip_ccw_evenodd_rule scene with clip_rule 1: the clip path's even-odd rule is
+    honoured, so the nonzero draw rect under it is clipped away where the two clip rects overlap
+    (winding -2) and stays covered where one covers."""
+    from conftest import load_golden
+
+    sc, _ = load_golden()["clip_ccw_evenodd_rule"]
+    sc.clip_rule = 1
+    cov = oracle.path_coverage_image(sc, oracle.full_model(sc), 0)
+    assert cov[32:99, 52:117].max() <= 1.0 / 255.0
+    assert np.abs(cov[110:118, 30:45] - 1.0).max() <= 1.0 / 255.0
+    assert cov[11:12, 60:100].max() == 0.0
