@@ -289,6 +289,8 @@ typedef struct pfh_schedule {
                                  (1..64; 0: 8) */
     int32_t column_rows;      /* path columns of more tiles than this are propagated by 16 or 64 lanes
                                  each instead of one (1..64; 0: every column by one lane) */
+    int32_t flatten_deep;     /* curves whose subdivision depth is estimated at this or more are
+                                 flattened by 16 lanes each instead of one (0..12; 0: none) */
 } pfh_schedule;
 int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out);
 int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in);

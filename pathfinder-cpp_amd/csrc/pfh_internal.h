@@ -79,6 +79,8 @@ struct Schedule {
     uint32_t flatten_take = 8;        // idle k_flatten lanes a wave waits for before a hand-over (1..64)
     uint32_t column_rows = 0;         // path columns of more tiles are walked by 16 or 64 lanes (1..64;
                                       // 0: every column by one lane)
+    uint32_t flatten_deep = 0;        // curves whose estimated subdivision depth is at least this are
+                                      // flattened by 16 lanes each (k_flatten_deep; 0: none)
 };
 
 // Inclusive prefix sum within each 16-lane row of the wave (DPP row shifts: no LDS).
@@ -150,7 +152,8 @@ enum Counter {
                            // TALL_ROWS tiles (and Schedule::column_rows) listed for a wave each
     C_MID = 8 + 64 + 1 + 6,  // [C_MID, C_MID + 6): per phase, columns of more than
                              // Schedule::column_rows tiles and at most TALL_ROWS (k_propagate_rows<16>)
-    C_COUNT = 8 + 64 + 1 + 12
+    C_DEEP = 8 + 64 + 1 + 12,  // curves listed for k_flatten_deep
+    C_COUNT = 8 + 64 + 1 + 12 + 1
 };
 constexpr int TALL_ROWS = 64;
 constexpr int DRAWN_SLOTS = 64;
@@ -245,6 +248,9 @@ struct BatchDev {
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
     uint64_t *curve_mask;         // [S / 64 + 1] bit i of word j: segment 64 j + i is a curve (to be
                                   // subdivided), written by the flatten count pass
+    uint2 *deep_list;             // [S] with deep_pred: the curves left to k_flatten_deep (C_DEEP of them),
+                                  // (batch segment, scene), in no particular order
+    uint32_t deep_pred;           // Schedule::flatten_deep (0: every curve through k_flatten)
     // [L] flattened chords as a point stream, in emission order: per chain of chords a NaN
     // separator and the start point, then each chord's end point; chord i of a path's range
     // [c0, c1) runs from point i to point i + 1 (a pair with a NaN end emits nothing)

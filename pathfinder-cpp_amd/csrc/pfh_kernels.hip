@@ -280,7 +280,28 @@ constexpr int FLAT_LDS_LEVELS = 4;
 // Lines need no subdivision: k_flatten_lines, a thread per segment without the LDS stack (so
 // with many more waves per CU to hide the loads), finishes every line and every segment that
 // emits nothing, and its count pass marks the curves, one bit per segment in a 64-bit word per
-// 64 segments, for k_flatten.
+// 64 segments, for k_flatten.  With deep_pred it lists instead the curves whose subdivision is
+// estimated at deep_pred levels or more for k_flatten_deep: a halving divides the flatness
+// measure of core/data/segment.cpp:12-36 by about 16 (second differences by 4, squared), so a
+// measure above 16^(deep_pred - 1) takes about deep_pred levels.  The estimate only routes a
+// curve; both kernels subdivide it the same way.
+__device__ __forceinline__ bool deep_curve(const float2 *P, uint32_t kind, uint32_t deep_pred) {
+    const float2 p0 = P[0], c0 = P[1], p1 = P[kind == SEG_QUAD ? 2 : 3], c1 = kind == SEG_QUAD ? c0 : P[2];
+    float m;
+    if (kind == SEG_QUAD) {
+        const float dx = c0.x - 0.5f * (p0.x + p1.x), dy = c0.y - 0.5f * (p0.y + p1.y);
+        m = 4.0f * (dx * dx + dy * dy);
+    } else {
+        const float u0 = 3.0f * c0.x - 2.0f * p0.x - p1.x, u1 = 3.0f * c0.y - 2.0f * p0.y - p1.y;
+        const float u2 = 3.0f * c1.x - 2.0f * p1.x - p0.x, u3 = 3.0f * c1.y - 2.0f * p1.y - p0.y;
+        m = fmaxf(u0 * u0, u2 * u2) + fmaxf(u1 * u1, u3 * u3);
+    }
+    // a non-finite curve emits nothing: k_flatten disposes of it at once
+    const bool finite = isfinite(p0.x) && isfinite(p0.y) && isfinite(c0.x) && isfinite(c0.y) && isfinite(c1.x) &&
+                        isfinite(c1.y) && isfinite(p1.x) && isfinite(p1.y);
+    return finite && m > ldexpf(1.0f, 4 * ((int)deep_pred - 1));
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_flatten_lines(BatchDev b) {
     const uint32_t lane = threadIdx.x & 63;
@@ -290,7 +311,7 @@ __global__ __launch_bounds__(256) void k_flatten_lines(BatchDev b) {
     const bool in = g < b.n_segs;
     int k = find_scene_wave(b.seg_bases, b.n_scenes, g0);
     uint32_t kind = 0;
-    bool brk = false;
+    bool brk = false, deep = false;
     float2 p0 = {}, p1 = {};
     if (in) {
         while (g >= b.seg_bases[k + 1]) k++;
@@ -307,14 +328,23 @@ __global__ __launch_bounds__(256) void k_flatten_lines(BatchDev b) {
             }
         } else if (sk != 0) {
             kind = sk;  // a curve: its points are checked where it is subdivided
+            if (!EMIT && b.deep_pred) deep = deep_curve(sc.pts + sg.x, sk, b.deep_pred);
         }
     }
     const bool curve = kind == SEG_QUAD || kind == SEG_CUBIC;
     if (!EMIT) {
         // stream points: a chord's end (+ the chain's separator and start point)
         if (in && !curve) b.seg_cnt[g] = kind == SEG_LINE ? 1u + (brk ? 2u : 0u) : 0u;
-        const uint64_t m = __ballot(curve);
+        const uint64_t m = __ballot(curve && !deep);
         if (lane == 0) b.curve_mask[g0 >> 6] = m;
+        const uint64_t dm = __ballot(deep);
+        if (dm) {  // wave-uniform: one add per wave
+            uint32_t at = 0;
+            if (lane == 0) at = atomicAdd(&b.counters[C_DEEP], (uint32_t)__popcll(dm));
+            at = __shfl(at, 0);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(dm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)dm, 0u));
+            if (deep) b.deep_list[at + below] = make_uint2(g, (uint32_t)k);
+        }
     } else if (kind == SEG_LINE) {
         uint32_t o = b.seg_cnt[g];
         if (brk) {  // the chain's separator (a NaN point: its chords emit nothing) and start
@@ -324,6 +354,19 @@ __global__ __launch_bounds__(256) void k_flatten_lines(BatchDev b) {
         }
         if (o < b.cap_lines) b.lines[o] = p1;
     }
+}
+
+// de Casteljau at t = 0.5 (core/data/segment.cpp:42-126), one path for both kinds (lanes holding
+// quads and cubics run it once).  A quad is held as (p0, c, c, p1): p01 = lerp(p0, c) and
+// p23 = lerp(c, p1) are its qa and qb, and taking them in place of p012 and p123 makes
+// p0123 = lerp(qa, qb), its midpoint: halves (p0, qa, qa, qc), (qc, qb, qb, p1).
+__device__ __forceinline__ void split_mid(const Seg4 &s, bool quad, Seg4 &a, Seg4 &r) {
+    const float t = 0.5f;
+    const float2 p01 = lerp2(s.p0, s.c0, t), p12 = lerp2(s.c0, s.c1, t), p23 = lerp2(s.c1, s.p1, t);
+    const float2 p012 = quad ? p01 : lerp2(p01, p12, t), p123 = quad ? p23 : lerp2(p12, p23, t);
+    const float2 p0123 = lerp2(p012, p123, t);
+    a = Seg4{s.p0, p01, p012, p0123};
+    r = Seg4{p0123, p123, p23, s.p1};
 }
 
 // The curves k_flatten_lines marked.  Lanes are scheduled dynamically within a wave: the wave
@@ -480,16 +523,8 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
                     }
                 }
             } else {
-                // de Casteljau at t = 0.5, one path for both kinds (lanes holding quads and cubics
-                // run it once).  A quad is held as (p0, c, c, p1): p01 = lerp(p0, c) and
-                // p23 = lerp(c, p1) are its qa and qb, and taking them in place of p012 and p123
-                // makes p0123 = lerp(qa, qb), its midpoint: halves (p0, qa, qa, qc), (qc, qb, qb, p1).
-                const float t = 0.5f;
-                const float2 p01 = lerp2(s.p0, s.c0, t), p12 = lerp2(s.c0, s.c1, t), p23 = lerp2(s.c1, s.p1, t);
-                const float2 p012 = quad ? p01 : lerp2(p01, p12, t), p123 = quad ? p23 : lerp2(p12, p23, t);
-                const float2 p0123 = lerp2(p012, p123, t);
-                const Seg4 a{s.p0, p01, p012, p0123};
-                const Seg4 bseg{p0123, p123, p23, s.p1};
+                Seg4 a, bseg;  // the halves at t = 0.5
+                split_mid(s, quad, a, bseg);
                 if (sp < FLAT_LDS_LEVELS) {
                     fs[sp][0][tid] = bseg.c0.x;
                     fs[sp][1][tid] = bseg.c0.y;
@@ -516,6 +551,131 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
         if (done) {
             if (!EMIT) b.seg_cnt[g] = n + (brk ? 2u : 0u);  // stream points: chords (+ separator, start)
             active = false;
+        }
+    }
+    if (!EMIT && guard) atomicAdd(&b.counters[C_GUARD], guard);
+}
+
+// The curves k_flatten_lines listed as deep, 16 lanes each (4 per wave), so that a curve of 2^d
+// chords takes about 2^(d-3) serial steps instead of 2^(d+1).  Lane j of a group owns the j-th
+// node, left to right, of depth DEEP_LEVELS of the curve's subdivision tree: it walks down to it
+// from the root by the same splits and flatness tests as the depth-first walk of k_flatten, and
+// where a node on the way is already flat (a leaf above that depth) the leftmost lane under it
+// emits its chord and the others nothing.  Otherwise the lane flattens its node depth first.
+// The chords of the group's lanes, in lane order, are the curve's chords in the order k_flatten
+// emits them (core/d3d9/tiler.cpp:284-315: first half first).  The count pass writes the curve's
+// total; the emit pass counts each lane's share again, places it by a 16-lane prefix sum and
+// emits it.  Workgroups stride over the list (its length is only known on the device).
+constexpr int DEEP_LEVELS = 4;  // 16 lanes per curve
+
+template <bool EMIT>
+__device__ __forceinline__ uint32_t flatten_subtree(Seg4 s, int d, bool quad, float (*fs)[6][256], uint8_t (*fd)[256],
+                                                    uint32_t tid, float2 *out, uint32_t room, uint32_t &guard) {
+    float stk[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS][6];
+    int sdepth[MAX_FLATTEN_DEPTH + 1 - FLAT_LDS_LEVELS];
+    uint32_t n = 0;
+    int sp = 0;
+    while (true) {
+        const bool flat = quad ? flat_quad(s) : flat_cubic(s);
+        if (flat || d >= MAX_FLATTEN_DEPTH) {
+            if (!flat) guard++;
+            if (EMIT && n < room) out[n] = s.p1;
+            n++;
+            if (sp == 0) break;
+            sp--;
+            s.p0 = s.p1;
+            if (sp < FLAT_LDS_LEVELS) {
+                s.c0 = make_float2(fs[sp][0][tid], fs[sp][1][tid]);
+                s.c1 = make_float2(fs[sp][2][tid], fs[sp][3][tid]);
+                s.p1 = make_float2(fs[sp][4][tid], fs[sp][5][tid]);
+                d = fd[sp][tid];
+            } else {
+                const int q = sp - FLAT_LDS_LEVELS;
+                s.c0 = make_float2(stk[q][0], stk[q][1]);
+                s.c1 = make_float2(stk[q][2], stk[q][3]);
+                s.p1 = make_float2(stk[q][4], stk[q][5]);
+                d = sdepth[q];
+            }
+        } else {
+            Seg4 a, r;
+            split_mid(s, quad, a, r);
+            if (sp < FLAT_LDS_LEVELS) {
+                fs[sp][0][tid] = r.c0.x;
+                fs[sp][1][tid] = r.c0.y;
+                fs[sp][2][tid] = r.c1.x;
+                fs[sp][3][tid] = r.c1.y;
+                fs[sp][4][tid] = r.p1.x;
+                fs[sp][5][tid] = r.p1.y;
+                fd[sp][tid] = (uint8_t)(d + 1);
+            } else {
+                const int q = sp - FLAT_LDS_LEVELS;
+                stk[q][0] = r.c0.x;
+                stk[q][1] = r.c0.y;
+                stk[q][2] = r.c1.x;
+                stk[q][3] = r.c1.y;
+                stk[q][4] = r.p1.x;
+                stk[q][5] = r.p1.y;
+                sdepth[q] = d + 1;
+            }
+            sp++;
+            s = a;
+            d++;
+        }
+    }
+    return n;
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_flatten_deep(BatchDev b) {
+    __shared__ float fs[FLAT_LDS_LEVELS][6][256];
+    __shared__ uint8_t fd[FLAT_LDS_LEVELS][256];
+    const uint32_t tid = threadIdx.x, j = tid & 15u;
+    const uint32_t n = b.counters[C_DEEP];
+    uint32_t guard = 0;
+    // the 16 lanes of a group take the same curves (uniform over the group)
+    for (uint32_t i = blockIdx.x * 16 + (tid >> 4); i < n; i += gridDim.x * 16) {
+        // the list is in no order: each entry carries its scene (no search across lanes)
+        const uint2 e = b.deep_list[i];
+        const uint32_t g = e.x;
+        const SceneDev &sc = b.scenes[e.y];
+        const uint2 sg = sc.segs[g - sc.seg_base];
+        const bool quad = (sg.y & SEG_KIND_MASK) == SEG_QUAD;
+        const bool brk = (sg.y & SEG_CHAIN_BREAK) != 0;
+        const float2 *P = sc.pts + sg.x;
+        Seg4 s{P[0], P[1], quad ? P[1] : P[2], quad ? P[2] : P[3]};
+        const float2 start = s.p0;
+        // down to this lane's node (bits of j, most significant first: 0 the first half)
+        bool leaf = false, mine = false;
+        for (int lv = 0; lv < DEEP_LEVELS; lv++) {
+            if (quad ? flat_quad(s) : flat_cubic(s)) {  // a leaf at depth lv < MAX_FLATTEN_DEPTH
+                leaf = true;
+                mine = (j & ((1u << (DEEP_LEVELS - lv)) - 1u)) == 0;
+                break;
+            }
+            Seg4 a, r;
+            split_mid(s, quad, a, r);
+            s = ((j >> (DEEP_LEVELS - 1 - lv)) & 1u) ? r : a;
+        }
+        uint32_t cnt = leaf ? (mine ? 1u : 0u)
+                            : flatten_subtree<false>(s, DEEP_LEVELS, quad, fs, fd, tid, nullptr, 0u, guard);
+        const uint32_t incl = row16_inclusive_sum(cnt);
+        if (!EMIT) {
+            if (j == 15) b.seg_cnt[g] = incl + (brk ? 2u : 0u);  // stream points (+ separator, start)
+            continue;
+        }
+        uint32_t base = b.seg_cnt[g];
+        if (brk) {  // the chain's separator (a NaN point: its chords emit nothing) and start
+            if (j == 0 && base < b.cap_lines) b.lines[base] = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+            if (j == 0 && base + 1 < b.cap_lines) b.lines[base + 1] = start;
+            base += 2;
+        }
+        const uint32_t at = base + (incl - cnt);
+        const uint32_t room = at < b.cap_lines ? b.cap_lines - at : 0u;
+        if (leaf) {
+            if (mine && room) b.lines[at] = s.p1;
+        } else {
+            uint32_t unused = 0;
+            flatten_subtree<true>(s, DEEP_LEVELS, quad, fs, fd, tid, b.lines + at, room, unused);
         }
     }
     if (!EMIT && guard) atomicAdd(&b.counters[C_GUARD], guard);
@@ -1766,12 +1926,16 @@ hipError_t launch_flatten(const BatchDev &b, bool emit, uint32_t segs_per_wave, 
     if (segs_per_wave < 64 || segs_per_wave % 64 || take < 1 || take > 64) return hipErrorInvalidValue;
     const dim3 lines(blocks_for(b.n_segs, 256));
     const dim3 grid(blocks_for(blocks_for(b.n_segs, segs_per_wave), 4));
+    // deep curves: 16 per workgroup per round, at most 8 workgroups per CU
+    const dim3 deep(std::max(1u, std::min(blocks_for(b.n_segs, 16), 2048u)));
     if (emit) {
         hipLaunchKernelGGL(k_flatten_lines<true>, lines, dim3(256), 0, s, b);
         hipLaunchKernelGGL(k_flatten<true>, grid, dim3(256), 0, s, b, segs_per_wave, take);
+        if (b.deep_pred) hipLaunchKernelGGL(k_flatten_deep<true>, deep, dim3(256), 0, s, b);
     } else {
         hipLaunchKernelGGL(k_flatten_lines<false>, lines, dim3(256), 0, s, b);
         hipLaunchKernelGGL(k_flatten<false>, grid, dim3(256), 0, s, b, segs_per_wave, take);
+        if (b.deep_pred) hipLaunchKernelGGL(k_flatten_deep<false>, deep, dim3(256), 0, s, b);
     }
     return hipGetLastError();
 }

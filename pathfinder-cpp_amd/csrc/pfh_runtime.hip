@@ -205,10 +205,10 @@ struct GraphKey {
     std::vector<int32_t> ids;
     uint32_t flags;
     uint32_t cap_f, cap_a, cap_e, cap_l;
-    uint32_t sched[7];  // the launch schedule the graph was captured with
+    uint32_t sched[8];  // the launch schedule the graph was captured with
     bool operator<(const GraphKey &o) const {
         if (ids != o.ids) return ids < o.ids;
-        for (int i = 0; i < 7; i++)
+        for (int i = 0; i < 8; i++)
             if (sched[i] != o.sched[i]) return sched[i] < o.sched[i];
         if (flags != o.flags) return flags < o.flags;
         if (cap_f != o.cap_f) return cap_f < o.cap_f;
@@ -236,7 +236,7 @@ struct Workspace {
     // inside captured graphs, so replays time those kernels too
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
     DevBuf scene_table, bases, tile_delta, tile_cnt, tile_fill_off, tile_alpha_idx, tile_bd, tile_st, path_ent, ent_quarter, path_tot, bin_items, col_item, col_bd, tall_cols, curve_mask,
-        seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, alpha_vis, masks, zbuf, fb_range, keys, vals, keys_alt,
+        deep_list, seg_cnt, lines, line_cnt, fills, dump_fills, alpha_rec, alpha_vis, masks, zbuf, fb_range, keys, vals, keys_alt,
         vals_alt, entry_rec, entry_meta, counters, scan_tmp, sort_tmp, tile_clip_ai, clip_jobs, clip_rec, clip_seen;
     uint32_t *h_counters = nullptr;  // pinned
     SceneDev *h_table = nullptr;     // pinned scene table
@@ -312,7 +312,7 @@ struct Workspace {
     }
     std::vector<DevBuf *> all_buffers() {
         return {&scene_table, &bases, &tile_delta, &tile_cnt, &tile_fill_off, &tile_alpha_idx, &tile_bd, &tile_st,
-                &path_ent, &ent_quarter, &path_tot, &bin_items, &col_item, &col_bd, &tall_cols, &curve_mask, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
+                &path_ent, &ent_quarter, &path_tot, &bin_items, &col_item, &col_bd, &tall_cols, &curve_mask, &deep_list, &seg_cnt, &lines, &line_cnt, &fills, &dump_fills, &alpha_rec, &alpha_vis,
                 &masks, &zbuf, &fb_range, &keys, &vals, &keys_alt, &vals_alt, &entry_rec, &entry_meta, &counters, &scan_tmp,
                 &sort_tmp, &tile_clip_ai, &clip_jobs, &clip_rec, &clip_seen};
     }
@@ -1370,6 +1370,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.tall_cols.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
     w.curve_mask.ensure(8 * (pl.S / 64) + 16);
+    if (pl.sched.flatten_deep) w.deep_list.ensure(8 * pl.S + 16);
     w.lines.ensure(8ull * pl.cap_l);
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
     w.zbuf.ensure(4 * pl.FB + 4);
@@ -1575,6 +1576,8 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.tall_cols = w.tall_cols.as<uint32_t>();
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
     b.curve_mask = w.curve_mask.as<uint64_t>();
+    b.deep_pred = pl.sched.flatten_deep;
+    b.deep_list = pl.sched.flatten_deep ? w.deep_list.as<uint2>() : nullptr;
     b.lines = w.lines.as<float2>();
     b.line_cnt = w.line_cnt.as<uint32_t>();
     b.fills = w.fills.as<uint2>();
@@ -1860,7 +1863,7 @@ static void batched_enqueue(pfh_ctx *ctx, BatchedLaunch &bl) {
                      pl.cap_e,
                      pl.cap_l,
                      {pl.sched.fill_grid, pl.sched.composite_block, pl.sched.column_block, pl.sched.xcd_order,
-                      pl.sched.flatten_segs, pl.sched.flatten_take, pl.sched.column_rows}};
+                      pl.sched.flatten_segs, pl.sched.flatten_take, pl.sched.column_rows, pl.sched.flatten_deep}};
         if (w.graphs_generation != g_alloc_generation) {
             // some buffer moved since these graphs were captured: never replay them
             w.drop_graphs();
@@ -2710,7 +2713,7 @@ static bool valid_schedule(const pfh_schedule &s) {
     return s.fill_grid >= 1 && s.fill_grid <= (1 << 20) && blk(s.composite_block) && blk(s.column_block) &&
            s.xcd_order >= 0 && s.xcd_order <= 3 && s.flatten_segs >= 0 && s.flatten_segs <= (1 << 16) &&
            s.flatten_segs % 64 == 0 && s.flatten_take >= 0 && s.flatten_take <= 64 && s.column_rows >= 0 &&
-           s.column_rows <= 64;
+           s.column_rows <= 64 && s.flatten_deep >= 0 && s.flatten_deep <= 12;
 }
 
 int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out) {
@@ -2722,6 +2725,7 @@ int pfh_get_schedule(pfh_ctx *ctx, pfh_schedule *out) {
     out->flatten_segs = (int32_t)ctx->sched.flatten_segs;
     out->flatten_take = (int32_t)ctx->sched.flatten_take;
     out->column_rows = (int32_t)ctx->sched.column_rows;
+    out->flatten_deep = (int32_t)ctx->sched.flatten_deep;
     return PFH_OK;
 }
 
@@ -2736,6 +2740,7 @@ int pfh_set_schedule(pfh_ctx *ctx, const pfh_schedule *in) {
         ctx->sched.flatten_segs = (uint32_t)in->flatten_segs;
         ctx->sched.flatten_take = in->flatten_take ? (uint32_t)in->flatten_take : 8u;
         ctx->sched.column_rows = (uint32_t)in->column_rows;
+        ctx->sched.flatten_deep = (uint32_t)in->flatten_deep;
     });
 }
 
@@ -2763,10 +2768,10 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         Schedule best = ctx->sched;
         double best_ms = measure();
         if (ms_start) *ms_start = best_ms;
-        const std::vector<uint32_t> cand[7] = {{1024, 2048, 4096, 8192, 16384, 32768, 65536}, {64, 128, 256}, {64, 128, 256},
+        const std::vector<uint32_t> cand[8] = {{1024, 2048, 4096, 8192, 16384, 32768, 65536}, {64, 128, 256}, {64, 128, 256},
                                               {0, 1, 2, 3}, {0, 64, 128, 256, 512, 1024, 2048}, {1, 4, 8, 16, 32},
-                                              {0, 8, 16, 32, 64}};
-        for (int param = 0; param < 7; param++) {
+                                              {0, 8, 16, 32, 64}, {0, 3, 4, 5, 6}};
+        for (int param = 0; param < 8; param++) {
             for (uint32_t v : cand[param]) {
                 Schedule trial = best;
                 uint32_t *field = param == 0 ? &trial.fill_grid
@@ -2775,7 +2780,8 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
                                   : param == 3 ? &trial.xcd_order
                                   : param == 4 ? &trial.flatten_segs
                                   : param == 5 ? &trial.flatten_take
-                                               : &trial.column_rows;
+                                  : param == 6 ? &trial.column_rows
+                                               : &trial.flatten_deep;
                 if (*field == v) continue;
                 *field = v;
                 ctx->sched = trial;
@@ -2795,6 +2801,7 @@ int pfh_autotune(pfh_ctx *ctx, const int32_t *scene_ids, int32_t n, uint32_t fla
         out->flatten_segs = (int32_t)best.flatten_segs;
         out->flatten_take = (int32_t)best.flatten_take;
         out->column_rows = (int32_t)best.column_rows;
+        out->flatten_deep = (int32_t)best.flatten_deep;
         if (ms_tuned) *ms_tuned = best_ms;
     });
 }

@@ -95,7 +95,7 @@ class ServerStats(ctypes.Structure):
 class Schedule(ctypes.Structure):
     _fields_ = [("fill_grid", ctypes.c_int32), ("composite_block", ctypes.c_int32),
                 ("column_block", ctypes.c_int32), ("xcd_order", ctypes.c_int32), ("flatten_segs", ctypes.c_int32),
-                ("flatten_take", ctypes.c_int32), ("column_rows", ctypes.c_int32)]
+                ("flatten_take", ctypes.c_int32), ("column_rows", ctypes.c_int32), ("flatten_deep", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

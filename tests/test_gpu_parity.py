@@ -383,6 +383,58 @@ def test_dashed_stroked_polylines(ctx, flags):
     check_scene(ctx, scenegen.dashed_stroke_scene(63, size=1024, n=120), flags=flags, name="dashed strokes")
 
 
+def _deep_curve_scene(seed, size=1024, n=40):
+    """Closed contours of 3 cubics (or quads) with controls up to a canvas beyond the canvas: subdivision
+    depths up to 7; and one cubic reaching ~1e9 px, which hits the depth cap (MAX_FLATTEN_DEPTH
+    levels, every leaf there a guard hit)."""
+    rng = np.random.default_rng(seed)
+    pts, flags, cst = [], [], [0]
+    for i in range(n):
+        quad = i % 4 == 3
+        p = rng.uniform(0, size, (3, 2))
+        for k in range(3):
+            pts.append(p[k])
+            flags.append(0)
+            for c in range(1 if quad else 2):
+                pts.append(rng.uniform(-size, 2 * size, 2))
+                flags.append(1 + c)
+        cst.append(len(pts))
+    big = [(10, 10), (5e8, 1e9), (-5e8, 1e9), (1000, 12), (600, 900)]
+    pts += [np.array(q, np.float64) for q in big]
+    flags += [0, 1, 2, 0, 0]
+    cst.append(len(pts))
+    P = len(cst) - 1
+    rgba = [int(c) for c in rng.integers(0, 2**32, P, dtype=np.uint64) | 0xff000000]
+    return scenegen.Scene(size, size, np.array(pts, np.float32), np.array(flags, np.uint8), np.array(cst),
+                          np.ones(P, np.uint8), np.arange(P + 1), rng.integers(0, 2, P).astype(np.uint8),
+                          np.array(rgba, np.uint32))
+
+
+@pytest.mark.parametrize("deep", [1, 4, 6])
+def test_deep_curves_split_over_lanes(ctx, deep):
+    """Schedule::flatten_deep routes curves of an estimated subdivision depth of `deep` or more to
+    16 lanes each (k_flatten_deep): the chords, fills and pixels are those of the oracle's
+    depth-first walk, including the capped depth's guard hits, on curves of every depth, the
+    golden curve scenes and stroked outlines (dashed too), plain and batched."""
+    before = ctx.schedule()
+    try:
+        ctx.set_schedule(flatten_deep=deep)
+        sc = _deep_curve_scene(70 + deep)
+        guard = int(oracle.full_model(sc)["canon"]["guard_hits"])
+        assert guard > 1000  # the capped cubic's leaves at depth 16
+        for flags in (pfh.PFH_PLAIN, pfh.PFH_BATCHED):
+            check_scene(ctx, sc, flags=flags, name=f"deep curves ({deep})", guard_hits=guard)
+        for name in ("cubic_512_64x8", "tangent_cubic", "collinear_cubic"):
+            if name in GOLDEN:
+                check_scene(ctx, GOLDEN[name][0], name=f"{name} ({deep})")
+        check_scene(ctx, scenegen.blob_scene(71, 1024, 1024, 300, r_max=512.0), flags=pfh.PFH_BATCHED,
+                    name=f"blobs ({deep})")
+        check_scene(ctx, scenegen.dashed_stroke_scene(72, size=1024, n=60), flags=pfh.PFH_BATCHED,
+                    name=f"dashed strokes ({deep})")
+    finally:
+        ctx.set_schedule(**before)
+
+
 @pytest.mark.parametrize("flags", [pfh.PFH_PLAIN, pfh.PFH_BATCHED])
 def test_stress_scene_full_size(ctx, flags):
     """The stress workload at its full size: 4096², 2 000 star polygons of 100-2 000 vertices, 500

@@ -14,14 +14,15 @@ def test_every_schedule_gives_identical_pixels():
     scenes = scenegen.batch_scenes(n_scenes=5, seed=808, side_max=1536, paths_max=2000)
     ids = [ctx.upload(s) for s in scenes]
     assert ctx.schedule() == {"fill_grid": 16384, "composite_block": 256, "column_block": 256, "xcd_order": 0,
-                              "flatten_segs": 0, "flatten_take": 8, "column_rows": 0}
+                              "flatten_segs": 0, "flatten_take": 8, "column_rows": 0, "flatten_deep": 0}
     ctx.rasterize(ids, pfh.PFH_BATCHED)
     want = [ctx.readback(i) for i in ids]
     for sched in ({"fill_grid": 1}, {"fill_grid": 1024, "xcd_order": 1}, {"composite_block": 64, "xcd_order": 2},
                   {"composite_block": 128, "column_block": 64, "xcd_order": 3}, {"column_block": 128},
                   {"flatten_segs": 64}, {"flatten_segs": 2048}, {"flatten_take": 1},
                   {"flatten_take": 64, "flatten_segs": 128}, {"column_rows": 1}, {"column_rows": 16}, {"column_rows": 64},
-                  {"column_rows": 8, "column_block": 64}):
+                  {"column_rows": 8, "column_block": 64}, {"flatten_deep": 1}, {"flatten_deep": 4},
+                  {"flatten_deep": 5, "flatten_segs": 64, "flatten_take": 1}):
         ctx.set_schedule(**sched)
         for flags in (pfh.PFH_BATCHED, pfh.PFH_BATCHED | pfh.PFH_NO_GRAPH):
             ctx.rasterize(ids, flags)
@@ -37,6 +38,8 @@ def test_every_schedule_gives_identical_pixels():
         ctx.set_schedule(flatten_take=65)
     with pytest.raises(pfh.PfhError):
         ctx.set_schedule(column_rows=65)
+    with pytest.raises(pfh.PfhError):
+        ctx.set_schedule(flatten_deep=13)
     ctx.close()
 
 
