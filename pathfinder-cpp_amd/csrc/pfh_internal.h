@@ -152,9 +152,14 @@ enum Counter {
                            // TALL_ROWS tiles (and Schedule::column_rows) listed for a wave each
     C_MID = 8 + 64 + 1 + 6,  // [C_MID, C_MID + 6): per phase, columns of more than
                              // Schedule::column_rows tiles and at most TALL_ROWS (k_propagate_rows<16>)
-    C_DEEP = 8 + 64 + 1 + 12,  // curves listed for k_flatten_deep
-    C_COUNT = 8 + 64 + 1 + 12 + 1
+    C_DEEP = 8 + 64 + 1 + 12,  // [C_DEEP, C_DEEP + DEEP_SLOTS): curves listed for k_flatten_deep, per
+                               // list region
+    C_COUNT = 8 + 64 + 1 + 12 + 256
 };
+// The deep-curve list is split into DEEP_SLOTS regions of deep_slot_cap entries, a listing wave
+// taking region (its index mod DEEP_SLOTS) with that region's counter: no single address takes
+// every wave's add.
+constexpr uint32_t DEEP_SLOTS = 256;
 constexpr int TALL_ROWS = 64;
 constexpr int DRAWN_SLOTS = 64;
 
@@ -248,8 +253,9 @@ struct BatchDev {
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
     uint64_t *curve_mask;         // [S / 64 + 1] bit i of word j: segment 64 j + i is a curve (to be
                                   // subdivided), written by the flatten count pass
-    uint2 *deep_list;             // [S] with deep_pred: the curves left to k_flatten_deep (C_DEEP of them),
-                                  // (batch segment, scene), in no particular order
+    uint2 *deep_list;             // [DEEP_SLOTS * deep_slot_cap] with deep_pred: the curves left to
+                                  // k_flatten_deep, (batch segment, scene), per region in no particular order
+    uint32_t deep_slot_cap;       // entries per region: 64 per wave of k_flatten_lines mapped to it
     uint32_t deep_pred;           // Schedule::flatten_deep (0: every curve through k_flatten)
     // [L] flattened chords as a point stream, in emission order: per chain of chords a NaN
     // separator and the start point, then each chord's end point; chord i of a path's range

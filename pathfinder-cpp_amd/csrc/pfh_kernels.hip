@@ -338,12 +338,13 @@ __global__ __launch_bounds__(256) void k_flatten_lines(BatchDev b) {
         const uint64_t m = __ballot(curve && !deep);
         if (lane == 0) b.curve_mask[g0 >> 6] = m;
         const uint64_t dm = __ballot(deep);
-        if (dm) {  // wave-uniform: one add per wave
+        if (dm) {  // wave-uniform: one add per wave, on its region's counter
+            const uint32_t slot = (g0 >> 6) & (DEEP_SLOTS - 1);
             uint32_t at = 0;
-            if (lane == 0) at = atomicAdd(&b.counters[C_DEEP], (uint32_t)__popcll(dm));
+            if (lane == 0) at = atomicAdd(&b.counters[C_DEEP + slot], (uint32_t)__popcll(dm));
             at = __shfl(at, 0);
             const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(dm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)dm, 0u));
-            if (deep) b.deep_list[at + below] = make_uint2(g, (uint32_t)k);
+            if (deep) b.deep_list[(uint64_t)slot * b.deep_slot_cap + at + below] = make_uint2(g, (uint32_t)k);
         }
     } else if (kind == SEG_LINE) {
         uint32_t o = b.seg_cnt[g];
@@ -565,7 +566,8 @@ __global__ __launch_bounds__(256) void k_flatten(BatchDev b, uint32_t segs_per_w
 // The chords of the group's lanes, in lane order, are the curve's chords in the order k_flatten
 // emits them (core/d3d9/tiler.cpp:284-315: first half first).  The count pass writes the curve's
 // total; the emit pass counts each lane's share again, places it by a 16-lane prefix sum and
-// emits it.  Workgroups stride over the list (its length is only known on the device).
+// emits it.  Workgroups stride over the list's regions taken end to end (their lengths are only
+// known on the device: each workgroup scans the DEEP_SLOTS region counts in LDS first).
 constexpr int DEEP_LEVELS = 4;  // 16 lanes per curve
 
 template <bool EMIT>
@@ -629,13 +631,36 @@ template <bool EMIT>
 __global__ __launch_bounds__(256) void k_flatten_deep(BatchDev b) {
     __shared__ float fs[FLAT_LDS_LEVELS][6][256];
     __shared__ uint8_t fd[FLAT_LDS_LEVELS][256];
+    __shared__ uint32_t sstart[DEEP_SLOTS + 1];  // exclusive scan of the region counts
+    __shared__ uint32_t wsum[4];
+    static_assert(DEEP_SLOTS == 256, "one region count per thread");
     const uint32_t tid = threadIdx.x, j = tid & 15u;
-    const uint32_t n = b.counters[C_DEEP];
+    {
+        uint32_t inc = min(b.counters[C_DEEP + tid], b.deep_slot_cap);
+        for (int dl = 1; dl < 64; dl <<= 1) {
+            const uint32_t o = __shfl_up(inc, dl, 64);
+            if ((tid & 63u) >= (uint32_t)dl) inc += o;
+        }
+        if ((tid & 63u) == 63u) wsum[tid >> 6] = inc;
+        __syncthreads();
+        for (uint32_t w = 0; w < (tid >> 6); w++) inc += wsum[w];
+        sstart[tid + 1] = inc;
+        if (tid == 0) sstart[0] = 0;
+        __syncthreads();
+    }
+    const uint32_t n = sstart[DEEP_SLOTS];
     uint32_t guard = 0;
     // the 16 lanes of a group take the same curves (uniform over the group)
     for (uint32_t i = blockIdx.x * 16 + (tid >> 4); i < n; i += gridDim.x * 16) {
+        // item i: region r, the last with sstart[r] <= i (empty regions share the next one's start)
+        uint32_t lo = 0, hi = DEEP_SLOTS - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (sstart[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
         // the list is in no order: each entry carries its scene (no search across lanes)
-        const uint2 e = b.deep_list[i];
+        const uint2 e = b.deep_list[(uint64_t)lo * b.deep_slot_cap + (i - sstart[lo])];
         const uint32_t g = e.x;
         const SceneDev &sc = b.scenes[e.y];
         const uint2 sg = sc.segs[g - sc.seg_base];

@@ -1346,6 +1346,13 @@ static uint32_t flatten_segs(const Plan &pl) {
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(64 * per, 64), 1024);
 }
 
+// Entries per deep-list region: 64 for every wave of k_flatten_lines (64 segments each) that maps
+// to the region (waves w, w + DEEP_SLOTS, ...).
+static uint32_t deep_slot_cap(uint64_t segs) {
+    const uint64_t waves = (segs + 63) / 64;
+    return (uint32_t)(64 * ((waves + DEEP_SLOTS - 1) / DEEP_SLOTS));
+}
+
 static uint32_t clamp_cap(uint64_t v) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(v, 64), 0xF0000000ull); }
 
 static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
@@ -1370,7 +1377,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.tall_cols.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
     w.curve_mask.ensure(8 * (pl.S / 64) + 16);
-    if (pl.sched.flatten_deep) w.deep_list.ensure(8 * pl.S + 16);
+    if (pl.sched.flatten_deep) w.deep_list.ensure(8ull * DEEP_SLOTS * deep_slot_cap(pl.S) + 16);
     w.lines.ensure(8ull * pl.cap_l);
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
     w.zbuf.ensure(4 * pl.FB + 4);
@@ -1578,6 +1585,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.curve_mask = w.curve_mask.as<uint64_t>();
     b.deep_pred = pl.sched.flatten_deep;
     b.deep_list = pl.sched.flatten_deep ? w.deep_list.as<uint2>() : nullptr;
+    b.deep_slot_cap = deep_slot_cap(pl.S);
     b.lines = w.lines.as<float2>();
     b.line_cnt = w.line_cnt.as<uint32_t>();
     b.fills = w.fills.as<uint2>();
