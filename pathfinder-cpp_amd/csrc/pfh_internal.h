@@ -160,6 +160,28 @@ enum Counter {
 // taking region (its index mod DEEP_SLOTS) with that region's counter: no single address takes
 // every wave's add.
 constexpr uint32_t DEEP_SLOTS = 256;
+
+// A curve listed for k_flatten_deep, with what the listing pass loaded of it: its batch segment,
+// its scene | DEEP_QUAD | DEEP_BREAK, and its points (a quad as p0, c, c, p1).
+constexpr uint32_t DEEP_QUAD = 0x40000000u, DEEP_BREAK = 0x80000000u;
+struct __align__(8) DeepCurve {
+    uint32_t seg, scene;
+    float2 p0, c0, c1, p1;
+};
+
+// Arrays zeroed by one launch: p[k] has end[k] - end[k - 1] words (end[-1] = 0), k < n.
+constexpr int ZERO_SPANS = 6;
+struct ZeroSpans {
+    uint32_t *p[ZERO_SPANS];
+    uint64_t end[ZERO_SPANS];
+    int n;
+    void add(uint32_t *ptr, uint64_t words) {
+        if (!words) return;
+        p[n] = ptr;
+        end[n] = (n ? end[n - 1] : 0) + words;
+        n++;
+    }
+};
 constexpr int TALL_ROWS = 64;
 constexpr int DRAWN_SLOTS = 64;
 
@@ -253,8 +275,8 @@ struct BatchDev {
     uint32_t *seg_cnt;            // [S + 1] chords per segment, then exclusive scan
     uint64_t *curve_mask;         // [S / 64 + 1] bit i of word j: segment 64 j + i is a curve (to be
                                   // subdivided), written by the flatten count pass
-    uint2 *deep_list;             // [DEEP_SLOTS * deep_slot_cap] with deep_pred: the curves left to
-                                  // k_flatten_deep, (batch segment, scene), per region in no particular order
+    DeepCurve *deep_list;         // [DEEP_SLOTS * deep_slot_cap] with deep_pred: the curves left to
+                                  // k_flatten_deep, per region in no particular order
     uint32_t deep_slot_cap;       // entries per region: 64 per wave of k_flatten_lines mapped to it
     uint32_t deep_pred;           // Schedule::flatten_deep (0: every curve through k_flatten)
     // [L] flattened chords as a point stream, in emission order: per chain of chords a NaN

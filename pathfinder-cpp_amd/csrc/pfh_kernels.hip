@@ -22,10 +22,19 @@ namespace pfh {
 // helpers
 // ---------------------------------------------------------------------------
 
-__global__ void k_fill_u32(uint32_t *p, uint32_t value, uint64_t n) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = value;
+// Zeroes up to ZERO_SPANS arrays in one launch (the per-launch clears: one dispatch and one tail
+// instead of one per array): index i of their concatenation lies in the first span whose end is
+// past it.
+__global__ void k_zero_spans(ZeroSpans z) {
+    const uint64_t total = z.n ? z.end[z.n - 1] : 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+#pragma unroll
+        for (int k = 0; k < ZERO_SPANS; k++) {
+            const uint64_t start = k ? z.end[k - 1] : 0;
+            if (k < z.n && i >= start && i < z.end[k]) z.p[k][i - start] = 0u;
+        }
+    }
 }
 
 // _mm_max_ps(a, b) / _mm_min_ps(a, b): the second operand wins on NaN or equality.
@@ -285,8 +294,12 @@ constexpr int FLAT_LDS_LEVELS = 4;
 // measure of core/data/segment.cpp:12-36 by about 16 (second differences by 4, squared), so a
 // measure above 16^(deep_pred - 1) takes about deep_pred levels.  The estimate only routes a
 // curve; both kernels subdivide it the same way.
-__device__ __forceinline__ bool deep_curve(const float2 *P, uint32_t kind, uint32_t deep_pred) {
+__device__ __forceinline__ bool deep_curve(const float2 *P, uint32_t kind, uint32_t deep_pred, DeepCurve &dc) {
     const float2 p0 = P[0], c0 = P[1], p1 = P[kind == SEG_QUAD ? 2 : 3], c1 = kind == SEG_QUAD ? c0 : P[2];
+    dc.p0 = p0;
+    dc.c0 = c0;
+    dc.c1 = c1;
+    dc.p1 = p1;
     float m;
     if (kind == SEG_QUAD) {
         const float dx = c0.x - 0.5f * (p0.x + p1.x), dy = c0.y - 0.5f * (p0.y + p1.y);
@@ -313,6 +326,7 @@ __global__ __launch_bounds__(256) void k_flatten_lines(BatchDev b) {
     uint32_t kind = 0;
     bool brk = false, deep = false;
     float2 p0 = {}, p1 = {};
+    DeepCurve dc{};
     if (in) {
         while (g >= b.seg_bases[k + 1]) k++;
         const SceneDev &sc = b.scenes[k];
@@ -328,7 +342,11 @@ __global__ __launch_bounds__(256) void k_flatten_lines(BatchDev b) {
             }
         } else if (sk != 0) {
             kind = sk;  // a curve: its points are checked where it is subdivided
-            if (!EMIT && b.deep_pred) deep = deep_curve(sc.pts + sg.x, sk, b.deep_pred);
+            if (!EMIT && b.deep_pred) {
+                deep = deep_curve(sc.pts + sg.x, sk, b.deep_pred, dc);
+                dc.seg = g;
+                dc.scene = (uint32_t)k | (sk == SEG_QUAD ? DEEP_QUAD : 0u) | ((sg.y & SEG_CHAIN_BREAK) ? DEEP_BREAK : 0u);
+            }
         }
     }
     const bool curve = kind == SEG_QUAD || kind == SEG_CUBIC;
@@ -344,7 +362,7 @@ __global__ __launch_bounds__(256) void k_flatten_lines(BatchDev b) {
             if (lane == 0) at = atomicAdd(&b.counters[C_DEEP + slot], (uint32_t)__popcll(dm));
             at = __shfl(at, 0);
             const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(dm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)dm, 0u));
-            if (deep) b.deep_list[(uint64_t)slot * b.deep_slot_cap + at + below] = make_uint2(g, (uint32_t)k);
+            if (deep) b.deep_list[(uint64_t)slot * b.deep_slot_cap + at + below] = dc;
         }
     } else if (kind == SEG_LINE) {
         uint32_t o = b.seg_cnt[g];
@@ -659,15 +677,12 @@ __global__ __launch_bounds__(256) void k_flatten_deep(BatchDev b) {
             if (sstart[mid] <= i) lo = mid;
             else hi = mid - 1;
         }
-        // the list is in no order: each entry carries its scene (no search across lanes)
-        const uint2 e = b.deep_list[(uint64_t)lo * b.deep_slot_cap + (i - sstart[lo])];
-        const uint32_t g = e.x;
-        const SceneDev &sc = b.scenes[e.y];
-        const uint2 sg = sc.segs[g - sc.seg_base];
-        const bool quad = (sg.y & SEG_KIND_MASK) == SEG_QUAD;
-        const bool brk = (sg.y & SEG_CHAIN_BREAK) != 0;
-        const float2 *P = sc.pts + sg.x;
-        Seg4 s{P[0], P[1], quad ? P[1] : P[2], quad ? P[2] : P[3]};
+        // the list is in no order: each entry carries its segment's kind, flag and points
+        const DeepCurve dc = b.deep_list[(uint64_t)lo * b.deep_slot_cap + (i - sstart[lo])];
+        const uint32_t g = dc.seg;
+        const bool quad = (dc.scene & DEEP_QUAD) != 0;
+        const bool brk = (dc.scene & DEEP_BREAK) != 0;
+        Seg4 s{dc.p0, dc.c0, dc.c1, dc.p1};
         const float2 start = s.p0;
         // down to this lane's node (bits of j, most significant first: 0 the first half)
         bool leaf = false, mine = false;
@@ -1896,10 +1911,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(BatchDev b, const uin
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
-hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    uint32_t nb = (uint32_t)std::min<uint64_t>(blocks_for(n, 256), 8192);
-    hipLaunchKernelGGL(k_fill_u32, dim3(nb), dim3(256), 0, st, p, value, n);
+hipError_t launch_zero_spans(const ZeroSpans &z, hipStream_t st) {
+    if (z.n < 0 || z.n > ZERO_SPANS) return hipErrorInvalidValue;
+    const uint64_t total = z.n ? z.end[z.n - 1] : 0;
+    if (total == 0) return hipSuccess;
+    uint32_t nb = (uint32_t)std::min<uint64_t>(blocks_for(total, 256), 8192);
+    hipLaunchKernelGGL(k_zero_spans, dim3(nb), dim3(256), 0, st, z);
     return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@
 #include "pfh_internal.h"
 
 namespace pfh {
-hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t st);
+hipError_t launch_zero_spans(const ZeroSpans &z, hipStream_t st);
 size_t scan_temp_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp, hipStream_t s);
 hipError_t scan_entries(uint2 *inout, uint32_t n, uint32_t *counters, void *tmp, hipStream_t s);
@@ -1377,7 +1377,7 @@ static void ensure_sizes(Workspace &w, const Plan &pl, bool dump) {
     w.tall_cols.ensure(4 * pl.W + 4);
     w.seg_cnt.ensure(4 * pl.S + 8);
     w.curve_mask.ensure(8 * (pl.S / 64) + 16);
-    if (pl.sched.flatten_deep) w.deep_list.ensure(8ull * DEEP_SLOTS * deep_slot_cap(pl.S) + 16);
+    if (pl.sched.flatten_deep) w.deep_list.ensure(sizeof(DeepCurve) * DEEP_SLOTS * (uint64_t)deep_slot_cap(pl.S) + 16);
     w.lines.ensure(8ull * pl.cap_l);
     w.line_cnt.ensure(4ull * pl.cap_l + 8);
     w.zbuf.ensure(4 * pl.FB + 4);
@@ -1584,7 +1584,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.seg_cnt = w.seg_cnt.as<uint32_t>();
     b.curve_mask = w.curve_mask.as<uint64_t>();
     b.deep_pred = pl.sched.flatten_deep;
-    b.deep_list = pl.sched.flatten_deep ? w.deep_list.as<uint2>() : nullptr;
+    b.deep_list = pl.sched.flatten_deep ? w.deep_list.as<DeepCurve>() : nullptr;
     b.deep_slot_cap = deep_slot_cap(pl.S);
     b.lines = w.lines.as<float2>();
     b.line_cnt = w.line_cnt.as<uint32_t>();
@@ -1612,13 +1612,16 @@ static void stage_clear(Workspace &w, const BatchDev &b) {
     hipStream_t s = w.stream;
     // tile_delta / tile_cnt / col_bd need no clearing: the bin count pass stores every
     // tile and column of every binned path (paths with no columns own none)
-    HIP_TRY(launch_fill_u32(b.zbuf, 0, b.n_fb, s));
+    // one launch for every array (no memsets: the clears are captured into graphs)
+    ZeroSpans z{};
+    z.add(b.zbuf, b.n_fb);
     // paths with no columns are not binned: their totals and entry counts stay zero
-    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.path_tot), 0, 2 * (uint64_t)b.n_paths, s));
-    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.path_ent), 0, 2 * (uint64_t)b.n_paths, s));
-    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(b.fb_range), 0, 2 * (uint64_t)b.n_fb, s));
-    HIP_TRY(launch_fill_u32(b.counters, 0, C_COUNT, s));
-    if (b.clip_seen) HIP_TRY(launch_fill_u32(b.clip_seen, 0, (uint64_t)b.cap_alpha / 32 + 1, s));
+    z.add(reinterpret_cast<uint32_t *>(b.path_tot), 2 * (uint64_t)b.n_paths);
+    z.add(reinterpret_cast<uint32_t *>(b.path_ent), 2 * (uint64_t)b.n_paths);
+    z.add(reinterpret_cast<uint32_t *>(b.fb_range), 2 * (uint64_t)b.n_fb);
+    z.add(b.counters, C_COUNT);
+    if (b.clip_seen) z.add(b.clip_seen, (uint64_t)b.cap_alpha / 32 + 1);
+    HIP_TRY(launch_zero_spans(z, s));
 }
 
 // flatten (count, scan, emit) and bin count: chords and exact fill / alpha-tile counts
