@@ -54,10 +54,12 @@ constexpr uint32_t RULE_CLIP_OWN = 0x2u;
 
 // Binning ownership classes (a path's tiles and columns belong to it alone, so the wave or
 // workgroup that bins a path can count in LDS and store plainly).  A path goes to the wave
-// class when its tile rect and column count fit one wave's LDS slice, else to the group class,
-// which uses LDS up to BIN_GROUP_TILES / BIN_GROUP_COLS and global atomics beyond.
+// class when its tile rect and column count fit one wave's LDS slice and it has at most as many
+// segments as a wave has lanes, else to the group class, which uses LDS up to BIN_GROUP_TILES /
+// BIN_GROUP_COLS and global atomics beyond.
 constexpr int BIN_WAVE_TILES = 512;
 constexpr int BIN_WAVE_COLS = 128;
+constexpr int BIN_WAVE_SEGS = 64;       // more segments than lanes: the group class (its walk is 4x wider)
 constexpr int BIN_GROUP_TILES = 4600;   // with the walk queues: 20 KB (emit) and 40 KB (count) of LDS
 constexpr int BIN_GROUP_COLS = 512;
 constexpr int WALK_QUEUE = 128;          // per wave: chords that cross tile boundaries, waiting for a full wave

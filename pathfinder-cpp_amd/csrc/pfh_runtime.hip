@@ -1194,13 +1194,16 @@ static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d) {
     s->n_segs = (uint32_t)segs.size();
     s->n_tiles = (uint32_t)tiles;
     s->n_cols = (uint32_t)cols;
-    // binning classes by what the path owns: its rect's tiles and its columns
+    // binning classes by what the path owns (its rect's tiles and its columns) and by its walk: a
+    // path of more segments than a wave has lanes goes to the group class's 256 lanes
     std::vector<uint32_t> bin_wave, bin_group;
     for (int p = 0; p < d->n_paths; p++) {
         const int4 &r = s->path_rect[p];
         const uint64_t w = (uint64_t)std::max(0, r.z - r.x), h = (uint64_t)std::max(0, r.w - r.y);
         if (w == 0) continue;  // no columns: nothing is binned (k_flatten emits no chords)
-        if (w * h <= (uint64_t)BIN_WAVE_TILES && w <= (uint64_t)BIN_WAVE_COLS) bin_wave.push_back((uint32_t)p);
+        const uint64_t nseg = s->path_seg_off[p + 1] - s->path_seg_off[p];
+        if (w * h <= (uint64_t)BIN_WAVE_TILES && w <= (uint64_t)BIN_WAVE_COLS && nseg <= (uint64_t)BIN_WAVE_SEGS)
+            bin_wave.push_back((uint32_t)p);
         else bin_group.push_back((uint32_t)p);
     }
     s->n_bin_wave = (uint32_t)bin_wave.size();
