@@ -49,6 +49,35 @@ extern "C" {
 #define PFH_CLIP_RULE_ABS 0    /* pfh_scene_desc::clip_rule: the reference's clip coverage */
 #define PFH_CLIP_RULE_OWN 1    /* pfh_scene_desc::clip_rule: ClipPath::fill_rule honoured */
 
+/* Blend modes (BlendMode, core/paint/effects.h:56-123, same order). */
+#define PFH_BLEND_CLEAR 0
+#define PFH_BLEND_COPY 1
+#define PFH_BLEND_SRC_IN 2
+#define PFH_BLEND_SRC_OUT 3
+#define PFH_BLEND_SRC_OVER 4
+#define PFH_BLEND_SRC_ATOP 5
+#define PFH_BLEND_DEST_IN 6
+#define PFH_BLEND_DEST_OUT 7
+#define PFH_BLEND_DEST_OVER 8
+#define PFH_BLEND_DEST_ATOP 9
+#define PFH_BLEND_XOR 10
+#define PFH_BLEND_LIGHTER 11
+#define PFH_BLEND_DARKEN 12
+#define PFH_BLEND_LIGHTEN 13
+#define PFH_BLEND_MULTIPLY 14
+#define PFH_BLEND_SCREEN 15
+#define PFH_BLEND_HARD_LIGHT 16
+#define PFH_BLEND_OVERLAY 17
+#define PFH_BLEND_COLOR_DODGE 18
+#define PFH_BLEND_COLOR_BURN 19
+#define PFH_BLEND_SOFT_LIGHT 20
+#define PFH_BLEND_DIFFERENCE 21
+#define PFH_BLEND_EXCLUSION 22
+#define PFH_BLEND_HUE 23
+#define PFH_BLEND_SATURATION 24
+#define PFH_BLEND_COLOR 25
+#define PFH_BLEND_LUMINOSITY 26
+
 typedef struct pfh_ctx pfh_ctx;
 
 /* An outline: contours of points with PointFlag flags (core/data/path.h:12-57, Outline / Contour). */
@@ -166,6 +195,20 @@ typedef struct pfh_scene_desc {
      *     path under its own, coverage = the min of those; a tile without fills covers where its
      *     rule says (an even-odd tile with an even backdrop covers nothing). */
     int32_t clip_rule;
+    /* Blend modes: NULL (every path PFH_BLEND_SRC_OVER), or per path a PFH_BLEND_* value (a clip
+     * path's is not used).  As the reference's hybrid renderer treats DrawPath::blend_mode:
+     *   - a destructive mode (clear, copy, src-in, dest-in, src-out, dest-atop) tiles the whole view
+     *     box (core/data/built_path.cpp:21, effects.h:234-243);
+     *   - only src-over and clear paths of an opaque paint occlude what lies below (built_path.cpp:40,
+     *     effects.h:127-160);
+     *   - the Porter-Duff modes (clear .. lighter) draw as src-over (the tile pass's blend state is
+     *     always over, core/d3d9/renderer.cpp:209);
+     *   - the other 15 modes draw through tile.frag's composite() (shaders/d3d9/tile.frag:514-598,
+     *     the op of blend_mode_to_composite_ctrl, effects.h:184-232) against the colour composited
+     *     below (the reference's hybrid pass binds no destination texture there, so this is the
+     *     shader's formula with a defined destination): rgb = a (1 - da) src + a da B(dest, src) +
+     *     (1 - a) dest, alpha 1, over every pixel of the path's drawn tiles. */
+    const uint8_t *path_blend;          /* [n_paths] or NULL */
 } pfh_scene_desc;
 
 typedef struct pfh_stats {

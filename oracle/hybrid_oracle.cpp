@@ -384,8 +384,9 @@ static RectFo intersect(const RectFo &a, const RectFo &o) {
 
 // core/d3d9/tiler.cpp:317-343 Tiler ctor + generate_tiles, core/data/built_path.cpp:8-34,
 // core/data/data.h:53-55 round_rect_out_to_tile_bounds.
-static void tile_path(const Scene &sc, int p, PathTiling &pt) {
-    RectFo b = intersect(path_bounds(sc, p), sc.view_box);
+// destructive: a draw path of a destructive blend mode tiles the whole view box (built_path.cpp:21).
+static void tile_path(const Scene &sc, int p, PathTiling &pt, bool destructive = false) {
+    RectFo b = destructive ? sc.view_box : intersect(path_bounds(sc, p), sc.view_box);
     float s = 1.0f / 16.0f;
     pt.rect[0] = (int32_t)std::floor(b.l * s);
     pt.rect[1] = (int32_t)std::floor(b.t * s);
@@ -562,10 +563,17 @@ double pfo_apply_rule(double c, int even_odd) {
 // names the clip path of draw path p by its index among them, or -1.  Clip paths are tiled like draw
 // paths (core/d3d9/scene_builder.cpp:200-225) but draw nothing themselves.  clip_rule 0 resolves
 // clips as the reference does; 1 under each path's own fill rule (resolve_clip's own_rules).
+// path_blend (NULL: every path src-over) holds BlendMode values (core/paint/effects.h:56-123): a
+// destructive mode (clear, copy, src-in, dest-in, src-out, dest-atop: effects.h:234-243) tiles the
+// view box, and only src-over and clear occlude (blend_mode_occludes_backdrop, effects.h:127-160;
+// core/data/built_path.cpp:40).
+static bool blend_destructive(int m) { return m == 0 || m == 1 || m == 2 || m == 6 || m == 3 || m == 9; }
+static bool blend_occludes(int m) { return m == 4 || m == 0; }
+
 void *pfo_run(int n_paths, const int32_t *path_contour_start, const uint8_t *path_fill_rule,
               const uint8_t *path_opaque, const int32_t *contour_start, const uint8_t *contour_closed,
               const float *points, const uint8_t *point_flags, const float *view_box, int zmode, int n_clip,
-              const int32_t *path_clip, int clip_rule) {
+              const int32_t *path_clip, int clip_rule, const uint8_t *path_blend) {
     auto *res = new Result();
     Scene sc{n_paths, path_contour_start, path_fill_rule, contour_start, contour_closed, points, point_flags,
              scene_view_box(view_box)};
@@ -586,7 +594,8 @@ void *pfo_run(int n_paths, const int32_t *path_contour_start, const uint8_t *pat
         level[q] = k;
     }
     auto t0 = std::chrono::steady_clock::now();
-    for (int p = 0; p < n_paths; p++) tile_path(sc, p, res->paths[p]);
+    for (int p = 0; p < n_paths; p++)
+        tile_path(sc, p, res->paths[p], path_blend && p < n_draw && blend_destructive(path_blend[p]));
     std::vector<int64_t> tile_base(n_paths + 1, 0);
     for (int p = 0; p < n_paths; p++) tile_base[p + 1] = tile_base[p] + (int64_t)res->paths[p].backdrop.size();
     for (int lev = 1; lev <= 3 && path_clip; lev++) {
@@ -623,7 +632,7 @@ void *pfo_run(int n_paths, const int32_t *path_contour_start, const uint8_t *pat
             if (!pt.alpha[i] && pt.backdrop[i] == 0) continue;
             int64_t fb = (int64_t)(ty - res->fb_rect[1]) * fw + (tx - res->fb_rect[0]);
             lists[fb].push_back({(uint32_t)p, tile_base[p] + i});
-            if (!path_opaque[p] || pt.alpha[i]) continue;
+            if (!path_opaque[p] || pt.alpha[i] || (path_blend && !blend_occludes(path_blend[p]))) continue;
             if (zmode == 1 && path_fill_rule[p] && (pt.backdrop[i] % 2) == 0) continue;
             res->zbuf[fb] = std::max(res->zbuf[fb], (uint32_t)p);
         }

@@ -42,7 +42,7 @@ def load_oracle():
             build()
         lib = ctypes.CDLL(ORACLE_SO)
         lib.pfo_run.restype = _P
-        lib.pfo_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, ctypes.c_int, _P, ctypes.c_int]
+        lib.pfo_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, _P]
         lib.pfo_fetch_clip.argtypes = [_P] * 4
         lib.pfo_apply_rule_array.argtypes = [ctypes.c_int64, _P, _P, _P]
         lib.pfo_counts.argtypes = [_P, _P]
@@ -61,7 +61,8 @@ def load_oracle():
         lib.pfo_gradient_record.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, ctypes.c_int, _P]
         lib.pfo_paint_colour.restype = ctypes.c_int
         lib.pfo_paint_colour.argtypes = [_P, _P, ctypes.c_float, ctypes.c_float, _P]
-        lib.pfo_composite_paint.argtypes = [ctypes.c_int64] + [_P] * 10 + [ctypes.c_int] * 3 + [_P]
+        lib.pfo_composite_paint.argtypes = [ctypes.c_int64] + [_P] * 10 + [ctypes.c_int] * 3 + [_P, _P]
+        lib.pfo_composite_op.argtypes = [_P, _P, ctypes.c_int, _P]
         lib.pfo_coverage_raw.argtypes = [ctypes.c_int64, _P, _P, _P, _P]
         lib.pfo_apply_rule.restype = ctypes.c_double
         lib.pfo_apply_rule.argtypes = [ctypes.c_double, ctypes.c_int]
@@ -84,7 +85,7 @@ def load_ref():
     if _ref is None:
         lib = ctypes.CDLL(REF_SO)
         lib.pfref_run.restype = _P
-        lib.pfref_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, _P, ctypes.c_int, _P]
+        lib.pfref_run.argtypes = [ctypes.c_int] + [_P] * 8 + [ctypes.c_int, _P, ctypes.c_int, _P, _P]
         lib.pfref_fetch_clip.argtypes = [_P] * 3
         lib.pfref_counts.argtypes = [_P, _P]
         lib.pfref_fetch.argtypes = [_P] * 14
@@ -218,6 +219,7 @@ def apply_strokes(scene):
     out.n_clip_paths = scene.n_clip_paths
     out.path_clip = scene.path_clip
     out.clip_rule = int(getattr(scene, "clip_rule", 0))
+    out.path_blend = getattr(scene, "path_blend", None)
     for k in ("path_gradient", "gradients", "gradient_stop_offsets", "gradient_stop_rgba", "gradient_ramps"):
         setattr(out, k, getattr(scene, k, None))
     return out
@@ -227,6 +229,39 @@ def _scene_args(scene):
     return [ctypes.c_int(scene.n_paths), _ptr(scene.path_contour_start), _ptr(scene.path_fill_rule),
             _ptr(scene.path_opaque), _ptr(scene.contour_start), _ptr(scene.contour_closed),
             _ptr(scene.points), _ptr(scene.point_flags), _ptr(scene.view_box)]
+
+
+# BlendMode (core/paint/effects.h:56-123) -> the tile shader's composite op
+# (blend_mode_to_composite_ctrl, effects.h:184-232): the Porter-Duff modes 0..11 draw over (0).
+COMPOSITE_OP = np.zeros(27, np.uint8)
+COMPOSITE_OP[12:] = [4, 5, 1, 2, 8, 3, 6, 7, 9, 10, 11, 12, 13, 14, 15]
+
+
+def _blend_arg(scene):
+    pb = getattr(scene, "path_blend", None)
+    if pb is None:
+        return None
+    scene._blend_keep = np.ascontiguousarray(pb, np.uint8)
+    return _ptr(scene._blend_keep)
+
+
+def path_ops(scene):
+    """Per path the composite op of its blend mode (clip paths: none), or None without blend modes."""
+    pb = getattr(scene, "path_blend", None)
+    if pb is None:
+        return None
+    ops = COMPOSITE_OP[np.asarray(pb, np.int64)].astype(np.uint8)
+    ops[scene.n_paths - int(scene.n_clip_paths):] = 0
+    return np.ascontiguousarray(ops)
+
+
+def composite_op(dest, src, op: int) -> np.ndarray:
+    """B(dest, src) of one composite op on single RGB colours in [0, 1] (the oracle's restatement)."""
+    d = np.ascontiguousarray(dest, np.float64)
+    s = np.ascontiguousarray(src, np.float64)
+    out = np.zeros(3, np.float64)
+    load_oracle().pfo_composite_op(_ptr(d), _ptr(s), ctypes.c_int(op), _ptr(out))
+    return out
 
 
 def _clip_args(scene):
@@ -256,7 +291,7 @@ def run_oracle(scene, zmode: int = 1) -> dict:
     lib = load_oracle()
     scene = apply_strokes(scene)
     h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(zmode), *_clip_args(scene),
-                    ctypes.c_int(int(getattr(scene, "clip_rule", 0))))
+                    ctypes.c_int(int(getattr(scene, "clip_rule", 0))), _blend_arg(scene))
     if not h:
         raise ValueError("clip paths nest deeper than 4 or in a cycle")
     counts = np.zeros(8, np.int64)
@@ -285,7 +320,7 @@ def run_ref(scene, threads: int = 1) -> dict:
     """Canonical tiling of ``scene`` by the reference's own compiled Tiler."""
     lib = load_ref()
     el = np.zeros(1, np.float64)
-    h = lib.pfref_run(*_scene_args(scene), ctypes.c_int(threads), _ptr(el), *_clip_args(scene))
+    h = lib.pfref_run(*_scene_args(scene), ctypes.c_int(threads), _ptr(el), *_clip_args(scene), _blend_arg(scene))
     counts = np.zeros(6, np.int64)
     lib.pfref_counts(h, _ptr(counts))
     out = _alloc(counts)
@@ -305,7 +340,7 @@ def raster_cpu(scene, threads: int = 1):
     fill + composite).  Scenes without clip paths."""
     assert not scene.n_clip_paths, "the CPU raster baseline covers scenes without clip paths"
     lib = load_oracle()
-    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(1), ctypes.c_int(0), None, ctypes.c_int(0))
+    h = lib.pfo_run(*_scene_args(scene), ctypes.c_int(1), ctypes.c_int(0), None, ctypes.c_int(0), None)
     out = np.zeros((scene.height, scene.width, 4), np.uint8)
     secs = lib.pfo_raster_cpu(h, _ptr(scene.path_fill_rule), _ptr(scene.path_rgba), scene.width, scene.height,
                               int(threads), _ptr(out))
@@ -526,7 +561,10 @@ def composite_fp64(canon: dict, at: dict, coverage: np.ndarray, scene) -> np.nda
     """FP64 premultiplied RGBA [H, W, 4] of the scene from the canonical entries."""
     lib = load_oracle()
     paints = scene_paints(scene)
-    if paints is not None:
+    ops = path_ops(scene)
+    if paints is not None or ops is not None:
+        if paints is None:
+            paints = (None, None, None)
         ent_tile = canon["entry_tile"]
         ent_alpha = np.ascontiguousarray(at["alpha_index"][ent_tile])
         ent_bd = np.ascontiguousarray(canon["tile_backdrop"][ent_tile])
@@ -536,7 +574,7 @@ def composite_fp64(canon: dict, at: dict, coverage: np.ndarray, scene) -> np.nda
         lib.pfo_composite_paint(ctypes.c_int64(ent_tile.shape[0]), _ptr(canon["entry_fbtile"]),
                                 _ptr(canon["entry_path"]), _ptr(ent_alpha), _ptr(ent_bd), _ptr(scene.path_fill_rule),
                                 _ptr(scene.path_rgba), _ptr(paints[0]), _ptr(paints[1]), _ptr(paints[2]), _ptr(cov),
-                                fbw, scene.width, scene.height, _ptr(out))
+                                fbw, scene.width, scene.height, _ptr(out), _ptr(ops))
         return out
     ent_tile = canon["entry_tile"]
     ent_alpha = np.ascontiguousarray(at["alpha_index"][ent_tile])

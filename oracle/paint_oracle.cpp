@@ -78,9 +78,75 @@ uint32_t sample(const std::vector<Stop> &stops, float t) {
     return out;
 }
 
+// shaders/d3d9/tile.frag:444-583: B(dest, src) of composite op `op` per channel (dest, src in
+// [0, 1]), through HSL for hue (12), saturation (13), color (14), luminosity (15).
+double dodge(double d, double s) { return d == 0.0 ? 0.0 : (s == 1.0 ? 1.0 : d / (1.0 - s)); }
+double screen(double d, double s) { return d + s - d * s; }
+double hard_light(double d, double s) { return s <= 0.5 ? d * 2.0 * s : screen(d, 2.0 * s - 1.0); }
+double soft_light(double d, double s) {
+    const double dark = d <= 0.25 ? ((16.0 * d - 12.0) * d + 4.0) * d : std::sqrt(d);
+    return d + (s * 2.0 - 1.0) * (s <= 0.5 ? d * (1.0 - d) : dark - d);
+}
+double safe_div(double n, double d) { return d != 0.0 ? n / d : 0.0; }
+// https://en.wikipedia.org/wiki/HSL_and_HSV: hue in radians (pi/3 per sector), saturation c / v
+void rgb_to_hsl(const double *c, double *hsl) {
+    const double v = std::max(std::max(c[0], c[1]), c[2]), mn = std::min(std::min(c[0], c[1]), c[2]);
+    const double ch = v - mn;
+    double sector, a, b;
+    if (c[0] == v) { sector = 0.0; a = c[1]; b = c[2]; }
+    else if (c[1] == v) { sector = 2.0; a = c[2]; b = c[0]; }
+    else { sector = 4.0; a = c[0]; b = c[1]; }
+    hsl[0] = (M_PI / 3.0) * safe_div(sector * ch + a - b, ch);
+    hsl[1] = safe_div(ch, v);
+    hsl[2] = 0.5 * (mn + v);
+}
+void hsl_to_rgb(const double *hsl, double *c) {
+    const double a = hsl[1] * std::min(hsl[2], 1.0 - hsl[2]);
+    const double n[3] = {0.0, 8.0, 4.0};
+    for (int i = 0; i < 3; i++) {
+        const double t = n[i] + hsl[0] * (6.0 / M_PI);
+        const double k = t - 12.0 * std::floor(t / 12.0);
+        c[i] = hsl[2] - std::min(std::max(std::min(k - 3.0, 9.0 - k), -1.0), 1.0) * a;
+    }
+}
+void composite_op(const double *d, const double *s, int op, double *out) {
+    if (op >= 12) {
+        double hd[3], hs[3], h[3];
+        rgb_to_hsl(d, hd);
+        rgb_to_hsl(s, hs);
+        h[0] = (op == 12 || op == 14) ? hs[0] : hd[0];
+        h[1] = (op == 13 || op == 14) ? hs[1] : hd[1];
+        h[2] = op == 15 ? hs[2] : hd[2];
+        hsl_to_rgb(h, out);
+        return;
+    }
+    for (int i = 0; i < 3; i++) {
+        const double dc = d[i], sc = s[i];
+        double r = sc;
+        switch (op) {
+            case 1: r = dc * sc; break;
+            case 2: r = screen(dc, sc); break;
+            case 3: r = hard_light(sc, dc); break;
+            case 4: r = std::min(dc, sc); break;
+            case 5: r = std::max(dc, sc); break;
+            case 6: r = dodge(dc, sc); break;
+            case 7: r = 1.0 - dodge(1.0 - dc, 1.0 - sc); break;
+            case 8: r = hard_light(dc, sc); break;
+            case 9: r = soft_light(dc, sc); break;
+            case 10: r = std::fabs(dc - sc); break;
+            case 11: r = dc + sc - 2.0 * dc * sc; break;
+        }
+        out[i] = r;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+// One composite op on single colours (for tests): out = B(dest, src) of `op`.
+void pfo_composite_op(const double *dest, const double *src, int op, double *out) { composite_op(dest, src, op, out); }
+
 
 double pfo_half_value(float v) { return half_value(v); }
 
@@ -177,17 +243,20 @@ int pfo_paint_colour(const float *rec, const uint32_t *ramp, float px, float py,
 // its colour per pixel from paint path_paint[p] (records [16 floats], ramps [256 texels]).
 double pfo_apply_rule(double c, int even_odd);
 
+// path_op (NULL: none): per path the tile shader's composite op (COMBINER_CTRL_COMPOSITE_*, 0 over),
+// applied as composite() does (shaders/d3d9/tile.frag:585-598) against the colour composited so far.
 void pfo_composite_paint(int64_t n_entries, const uint32_t *entry_fbtile, const uint32_t *entry_path,
                          const int64_t *entry_alpha, const int8_t *entry_backdrop, const uint8_t *path_even_odd,
                          const uint32_t *path_rgba, const int32_t *path_paint, const float *recs,
                          const uint32_t *ramps, const double *coverage, int fb_w, int width, int height,
-                         double *out) {
+                         double *out, const uint8_t *path_op) {
     std::fill(out, out + (size_t)width * height * 4, 0.0);
     for (int64_t e = 0; e < n_entries; e++) {
         const uint32_t fb = entry_fbtile[e];
         const int tx = (int)(fb % (uint32_t)fb_w), ty = (int)(fb / (uint32_t)fb_w);
         const uint32_t p = entry_path[e];
         const int32_t paint = path_paint ? path_paint[p] : -1;
+        const int op = path_op ? path_op[p] : 0;
         const uint32_t c = path_rgba[p];
         double col[4] = {(c & 0xff) / 255.0, ((c >> 8) & 0xff) / 255.0, ((c >> 16) & 0xff) / 255.0,
                          ((c >> 24) & 0xff) / 255.0};
@@ -199,11 +268,21 @@ void pfo_composite_paint(int64_t n_entries, const uint32_t *entry_fbtile, const 
                 const int x = tx * 16 + px;
                 if (x >= width) break;
                 if (paint >= 0 && !pfo_paint_colour(recs + 16 * (size_t)paint, ramps + 256 * (size_t)paint,
-                                                    (float)x + 0.5f, (float)y + 0.5f, col))
-                    continue;
+                                                    (float)x + 0.5f, (float)y + 0.5f, col)) {
+                    if (op == 0) continue;
+                    col[0] = col[1] = col[2] = col[3] = 0.0;  // transparent where the paint leaves out
+                }
                 const double cov = entry_alpha[e] >= 0 ? coverage[entry_alpha[e] * 256 + py * 16 + px] : solid_cov;
                 const double sa = col[3] * cov;
                 double *d = out + ((size_t)y * width + x) * 4;
+                if (op != 0) {
+                    double bl[3];
+                    composite_op(d, col, op, bl);
+                    for (int ch = 0; ch < 3; ch++)
+                        d[ch] = sa * (1.0 - d[3]) * col[ch] + sa * d[3] * bl[ch] + (1.0 - sa) * d[ch];
+                    d[3] = 1.0;
+                    continue;
+                }
                 const double k = 1.0 - sa;
                 d[0] = col[0] * sa + d[0] * k;
                 d[1] = col[1] * sa + d[1] * k;

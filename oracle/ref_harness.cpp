@@ -97,7 +97,7 @@ extern "C" {
 void *pfref_run(int n_paths, const int32_t *path_contour_start, const uint8_t *path_fill_rule,
                 const uint8_t *path_opaque, const int32_t *contour_start, const uint8_t *contour_closed,
                 const float *points, const uint8_t *point_flags, const float *view_box, int threads,
-                double *elapsed_s, int n_clip, const int32_t *path_clip) {
+                double *elapsed_s, int n_clip, const int32_t *path_clip, const uint8_t *path_blend) {
     auto *res = new RefResult();
     Scene scene(0, RectF(view_box[0], view_box[1], view_box[2], view_box[3]));
     HarnessBuilder builder;
@@ -133,13 +133,16 @@ void *pfref_run(int n_paths, const int32_t *path_contour_start, const uint8_t *p
             TilingPathInfo info{};
             info.type = TilingPathInfo::Type::Draw;
             info.info.paint_id = 0;
-            info.info.blend_mode = BlendMode::SrcOver;
+            // path_blend (NULL: src-over): BlendMode values, as DrawPath::blend_mode
+            const BlendMode mode = path_blend ? (BlendMode)path_blend[p] : BlendMode::SrcOver;
+            info.info.blend_mode = mode;
             info.info.fill_rule = path_fill_rule[p] ? FillRule::EvenOdd : FillRule::Winding;
             Tiler tiler(builder, (uint32_t)p, outlines[p], info.info.fill_rule, vb, nullptr, {}, info);
             tiler.generate_tiles();
             built[p].path = tiler.object_builder.built_path;
-            built[p].occludes = path_opaque[p] != 0;
-            built[p].blend_mode = BlendMode::SrcOver;
+            // BuiltDrawPath's rule (core/data/built_path.cpp:40), the paint's opacity given
+            built[p].occludes = path_opaque[p] != 0 && blend_mode_occludes_backdrop(mode);
+            built[p].blend_mode = mode;
             built[p].mask_fill_rule = info.info.fill_rule;
             fills[p] = std::move(tiler.object_builder.fills);
             if (path_clip && path_clip[p] >= 0) {

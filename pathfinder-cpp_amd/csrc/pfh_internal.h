@@ -15,12 +15,15 @@ constexpr int MAX_FLATTEN_DEPTH = 16;
 constexpr int FIX_SHIFT = 20;            // fixed-point coverage accumulation: 2^-20 px^2 units
 constexpr uint32_t NO_ALPHA = 0xffffffffu;
 
-// Gradient paints.  An entry of a path with a gradient paint carries ENTRY_GRADIENT in its first
-// word (with its mask row, or ENTRY_ROW_NONE in the low bits for a tile without a mask) and the
-// scene-local paint index in its second; NO_ALPHA (all bits set) stays the plain solid entry.
+// Paint records.  An entry of a path with a paint record (a gradient, a non-occluding solid, a
+// composite op) carries ENTRY_GRADIENT in its first word (with its mask row, ENTRY_ROW_NONE in the
+// low bits for a tile without a mask, or ENTRY_ROW_EMPTY for an even-odd tile of even winding,
+// which covers nothing but still composites) and the scene-local paint index in its second;
+// NO_ALPHA (all bits set) stays the plain solid entry.
 constexpr uint32_t ENTRY_GRADIENT = 0x40000000u;
 constexpr uint32_t ENTRY_ROW_MASK = 0x3fffffffu;
 constexpr uint32_t ENTRY_ROW_NONE = 0x3fffffffu;
+constexpr uint32_t ENTRY_ROW_EMPTY = 0x3ffffffeu;
 constexpr uint32_t PAINT_NONE = 0xffffffffu;   // SceneDev::path_paint: a solid colour
 constexpr int RAMP_TEXELS = 256;               // GRADIENT_TILE_LENGTH, core/paint/gradient.h
 // One gradient paint as the tile shader sees it (shaders/d3d9/tile.vert:65-77, tile.frag:296-323):
@@ -30,9 +33,13 @@ struct PaintRec {
     float m11, m21, m12, m22, m13, m23;  // pixel -> colour texture coordinates (radial: gradient space)
     float from_x, from_y, vec_x, vec_y;  // radial: the line between the circles' centres
     float r0, r1, uv_x, uv_y;            // radial: radii, the ramp row's uv origin
-    uint32_t kind;                       // 1 linear, 2 radial
+    uint32_t kind;                       // 0 solid (colour), 1 linear, 2 radial
     uint32_t wrap;                       // 0 clamp to edge, 1 repeat (REPEAT_U)
+    uint32_t colour;                     // kind 0: r | g << 8 | b << 16 | a << 24
+    uint32_t op;                         // composite op (COMBINER_CTRL_COMPOSITE_*): 0 normal (over)
 };
+// A path with a paint record: a gradient, or a solid colour whose blend mode does not occlude
+// (drawn as over, but never written to the z-buffer) or composites through a non-normal op.
 
 // Clip reference of a path (SceneDev::path_clip, then BinItem::clip).  A path clipped by a clip
 // path (a draw path, or a clip path nested in another) holds that clip path's binning position: its
@@ -216,7 +223,8 @@ struct SceneDev {
     uint32_t load_dst;            // blend over existing target contents
     float fb_inv_w;               // 1 / fb_w (0 without framebuffer tiles)
     const uint32_t *path_paint;   // [n_paths] gradient paint index or PAINT_NONE, or null: solid colours
-    const PaintRec *paints;       // the scene's gradient paints
+    const PaintRec *paints;       // the scene's paint records (gradients, blended solids)
+    uint32_t has_ops;             // some record composites through an op (tiles with one blend back to front)
     const uint32_t *ramps;        // [paints * RAMP_TEXELS] RGBA8 ramp texels, r | g << 8 | b << 16 | a << 24
 };
 

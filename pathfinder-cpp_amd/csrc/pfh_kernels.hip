@@ -1625,11 +1625,20 @@ __device__ __forceinline__ void write_entry(const BatchDev &b, const SceneDev &s
     }
     // even-odd solid tile with an even winding covers nothing: zero alpha
     uint32_t rx = ai, ry = et.st == TILE_CLEAR ? (rgba & 0x00ffffffu) : rgba;
-    // a gradient path's entry: flagged, its mask row (or none) and its paint; a tile that covers
-    // nothing stays a transparent solid entry
+    // a path with a paint record: flagged, its mask row (or none) and its paint; a tile that covers
+    // nothing stays a transparent solid entry, unless its paint composites through an op (which
+    // sets the alpha of every pixel of the tile, shaders/d3d9/tile.frag:585-598)
     if (paint != PAINT_NONE) {
-        rx = et.st == TILE_CLEAR ? NO_ALPHA : ENTRY_GRADIENT | (ai == NO_ALPHA ? ENTRY_ROW_NONE : min(ai, ENTRY_ROW_NONE - 1u));
-        ry = et.st == TILE_CLEAR ? 0u : paint;
+        if (et.st != TILE_CLEAR) {
+            rx = ENTRY_GRADIENT | (ai == NO_ALPHA ? ENTRY_ROW_NONE : min(ai, ENTRY_ROW_EMPTY - 1u));
+            ry = paint;
+        } else if (sc.paints[paint].op != 0) {
+            rx = ENTRY_GRADIENT | ENTRY_ROW_EMPTY;
+            ry = paint;
+        } else {
+            rx = NO_ALPHA;
+            ry = 0u;
+        }
     }
     b.keys[e] = sc.fb_base + et.fbl;  // its value, e, is generated by the sort's first pass
     b.entry_rec[e] = make_uint2(rx, ry);

@@ -375,6 +375,13 @@ constexpr int EV_GROUP = 4;  // blend events whose mask words are loaded togethe
 // the paint leaves the pixel transparent (a zero or negative discriminant, a non-finite coordinate).
 __device__ __forceinline__ bool gradient_colour(const PaintRec &pr, const uint32_t *ramp, float px, float py,
                                                 float &cr, float &cg, float &cb, float &ca) {
+    if (pr.kind == 0) {  // a solid colour's record (a blended solid path)
+        cr = (float)(pr.colour & 255u);
+        cg = (float)((pr.colour >> 8) & 255u);
+        cb = (float)((pr.colour >> 16) & 255u);
+        ca = (float)(pr.colour >> 24);
+        return true;
+    }
     const float u = pr.m11 * px + pr.m12 * py + pr.m13;
     float s = u;
     if (pr.kind == 2) {
@@ -421,7 +428,137 @@ __device__ __forceinline__ bool gradient_colour(const PaintRec &pr, const uint32
     return true;
 }
 
-// PAINTS: some scene of the batch has gradient paints (the solid-only variant carries no code for them).
+// The tile shader's composite ops (shaders/d3d9/tile.frag:444-583): B(dest, src) per channel
+// for the separable ones, through HSL for hue / saturation / color / luminosity.  Colours in [0, 1].
+__device__ __forceinline__ float cdiv(float num, float den) { return den != 0.0f ? num / den : 0.0f; }
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float dodge1(float d, float s) { return d == 0.0f ? 0.0f : (s == 1.0f ? 1.0f : d / (1.0f - s)); }
+__device__ __forceinline__ float screen1(float d, float s) { return d + s - d * s; }
+__device__ __forceinline__ float hard1(float d, float s) { return s <= 0.5f ? d * 2.0f * s : screen1(d, 2.0f * s - 1.0f); }
+__device__ __forceinline__ float soft1(float d, float s) {
+    const float dark = d <= 0.25f ? ((16.0f * d - 12.0f) * d + 4.0f) * d : sqrtf(d);
+    return d + (s * 2.0f - 1.0f) * (s <= 0.5f ? d * (1.0f - d) : dark - d);
+}
+// RGB -> (h, s, l) and back (the shader's HSL, h in radians)
+__device__ __forceinline__ float3 to_hsl(float3 c) {
+    const float v = fmaxf(fmaxf(c.x, c.y), c.z), mn = fminf(fminf(c.x, c.y), c.z);
+    const float ch = v - mn, l = mn + (v - mn) * 0.5f;
+    float t0, t1, t2;
+    if (c.x == v) { t0 = 0.0f; t1 = c.y; t2 = c.z; }
+    else if (c.y == v) { t0 = 2.0f; t1 = c.z; t2 = c.x; }
+    else { t0 = 4.0f; t1 = c.x; t2 = c.y; }
+    const float h = 1.0471975511965976f * cdiv(t0 * ch + t1 - t2, ch);
+    return f3(h, cdiv(ch, v), l);
+}
+__device__ __forceinline__ float3 from_hsl(float3 hsl) {
+    const float a = hsl.y * fminf(hsl.z, 1.0f - hsl.z);
+    float out[3];
+    const float base[3] = {0.0f, 8.0f, 4.0f};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float t = base[i] + hsl.x * 1.9098593171027443f;
+        const float k = t - 12.0f * floorf(t / 12.0f);  // GLSL mod
+        out[i] = hsl.z - fminf(fmaxf(fminf(k - 3.0f, 9.0f - k), -1.0f), 1.0f) * a;
+    }
+    return f3(out[0], out[1], out[2]);
+}
+__device__ float3 composite_rgb(float3 d, float3 s, uint32_t op) {
+    switch (op) {
+        case 0x1: return f3(d.x * s.x, d.y * s.y, d.z * s.z);
+        case 0x2: return f3(screen1(d.x, s.x), screen1(d.y, s.y), screen1(d.z, s.z));
+        case 0x3: return f3(hard1(s.x, d.x), hard1(s.y, d.y), hard1(s.z, d.z));  // overlay: hard light, swapped
+        case 0x4: return f3(fminf(d.x, s.x), fminf(d.y, s.y), fminf(d.z, s.z));
+        case 0x5: return f3(fmaxf(d.x, s.x), fmaxf(d.y, s.y), fmaxf(d.z, s.z));
+        case 0x6: return f3(dodge1(d.x, s.x), dodge1(d.y, s.y), dodge1(d.z, s.z));
+        case 0x7: return f3(1.0f - dodge1(1.0f - d.x, 1.0f - s.x), 1.0f - dodge1(1.0f - d.y, 1.0f - s.y),
+                            1.0f - dodge1(1.0f - d.z, 1.0f - s.z));
+        case 0x8: return f3(hard1(d.x, s.x), hard1(d.y, s.y), hard1(d.z, s.z));
+        case 0x9: return f3(soft1(d.x, s.x), soft1(d.y, s.y), soft1(d.z, s.z));
+        case 0xa: return f3(fabsf(d.x - s.x), fabsf(d.y - s.y), fabsf(d.z - s.z));
+        case 0xb: return f3(d.x + s.x - 2.0f * d.x * s.x, d.y + s.y - 2.0f * d.y * s.y, d.z + s.z - 2.0f * d.z * s.z);
+        default: {  // 0xc hue, 0xd saturation, 0xe color, 0xf luminosity
+            const float3 hd = to_hsl(d), hs = to_hsl(s);
+            const float3 h = op == 0xc ? f3(hs.x, hd.y, hd.z)
+                             : op == 0xd ? f3(hd.x, hs.y, hd.z)
+                             : op == 0xe ? f3(hs.x, hs.y, hd.z)
+                                         : f3(hd.x, hd.y, hs.z);
+            return from_hsl(h);
+        }
+    }
+}
+
+// A tile with an entry that composites through an op, blended back to front (the op reads what lies
+// below): per entry, from the bottom, src = its colour (a solid, or its paint at the pixel centre;
+// a pixel the paint leaves out is transparent), a = src alpha x coverage, then over (op 0), or
+// tile.frag's composite (:585-598): rgb = a (1 - da) src + a da B(dest, src) + (1 - a) dest,
+// alpha 1.  dest starts from the target (PFH_LOAD_DST) or transparent; premultiplied, in [0, 1].
+// Stores the lane's 4 pixels.
+__device__ void composite_back_to_front(const BatchDev &b, const SceneDev &sc, const uint32_t *sorted_vals, uint32_t e0,
+                                        uint32_t e1, int x, int y0, uint32_t lane_off, uint8_t *cptr, int64_t pitch,
+                                        int height, bool load_dst) {
+    float dr[4], dg[4], db[4], da[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint32_t px = 0;
+        if (load_dst && y0 + q < height) px = *reinterpret_cast<const uint32_t *>(cptr + q * pitch);
+        dr[q] = (float)(px & 255u) * (1.0f / 255.0f);
+        dg[q] = (float)((px >> 8) & 255u) * (1.0f / 255.0f);
+        db[q] = (float)((px >> 16) & 255u) * (1.0f / 255.0f);
+        da[q] = (float)(px >> 24) * (1.0f / 255.0f);
+    }
+    for (uint32_t i = e0; i < e1; i++) {  // uniform over the wave
+        const uint2 rec = b.entry_rec[sorted_vals[i]];
+        uint32_t colour = rec.y, op = 0, row = b.cap_alpha;  // row cap_alpha: full coverage
+        bool empty = false;
+        const PaintRec *pr = nullptr;
+        if (rec.x != NO_ALPHA && (rec.x & ENTRY_GRADIENT)) {
+            const uint32_t r = rec.x & ENTRY_ROW_MASK;
+            if (r == ENTRY_ROW_EMPTY) empty = true;
+            else if (r != ENTRY_ROW_NONE) row = r;
+            if (row > b.cap_alpha) continue;  // capacity overflow (the run is retried): blends nothing
+            pr = sc.paints + rec.y;
+            op = pr->op;
+        } else if (rec.x != NO_ALPHA) {
+            if (rec.x >= b.cap_alpha) continue;  // capacity overflow
+            row = rec.x;
+        }
+        const uint32_t m = empty ? 0u : *reinterpret_cast<const uint32_t *>(b.masks + (uint64_t)row * 256 + lane_off);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float cr = (float)(colour & 255u), cg = (float)((colour >> 8) & 255u), cb = (float)((colour >> 16) & 255u),
+                  ca = (float)(colour >> 24);
+            if (pr && !gradient_colour(*pr, sc.ramps + (uint64_t)rec.y * RAMP_TEXELS, (float)x + 0.5f,
+                                       (float)(y0 + q) + 0.5f, cr, cg, cb, ca))
+                cr = cg = cb = ca = 0.0f;
+            const float k = 1.0f / 255.0f;
+            const float a = ca * k * ((float)((m >> (8 * q)) & 255u) * k);
+            const float sr = cr * k, sg = cg * k, sb = cb * k;
+            if (op == 0) {
+                dr[q] = sr * a + dr[q] * (1.0f - a);
+                dg[q] = sg * a + dg[q] * (1.0f - a);
+                db[q] = sb * a + db[q] * (1.0f - a);
+                da[q] = a + da[q] * (1.0f - a);
+            } else {
+                const float3 bl = composite_rgb(f3(dr[q], dg[q], db[q]), f3(sr, sg, sb), op);
+                const float w0 = a * (1.0f - da[q]), w1 = a * da[q], w2 = 1.0f - a;
+                dr[q] = w0 * sr + w1 * bl.x + w2 * dr[q];
+                dg[q] = w0 * sg + w1 * bl.y + w2 * dg[q];
+                db[q] = w0 * sb + w1 * bl.z + w2 * db[q];
+                da[q] = 1.0f;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint32_t w = __builtin_amdgcn_cvt_pk_u8_f32(dr[q] * 255.0f, 0, 0u);
+        w = __builtin_amdgcn_cvt_pk_u8_f32(dg[q] * 255.0f, 1, w);
+        w = __builtin_amdgcn_cvt_pk_u8_f32(db[q] * 255.0f, 2, w);
+        w = __builtin_amdgcn_cvt_pk_u8_f32(da[q] * 255.0f, 3, w);
+        if (y0 + q < height) *reinterpret_cast<uint32_t *>(cptr + q * pitch) = w;
+    }
+}
+
+// PAINTS: some scene of the batch has paint records (the solid-only variant carries no code for them).
 template <bool PAINTS>
 __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *sorted_vals) {
     __shared__ float4 s_colour[4][64];  // per wave of the block: the chunk's premultiplied colours
@@ -456,6 +593,25 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     }
     const uint2 range = b.fb_range[f];
     const uint32_t e0 = range.x, e1 = min(range.y, b.cap_entries);
+    if (PAINTS && sc.has_ops) {
+        // a tile with an entry that composites through an op is blended back to front
+        bool ops = false;
+        for (uint32_t c0 = e0; c0 < e1 && !ops; c0 += 64) {  // uniform over the wave
+            const uint32_t i = c0 + lane;
+            bool op = false;
+            if (i < e1) {
+                const uint2 rec = b.entry_rec[sorted_vals[i]];
+                op = rec.x != NO_ALPHA && (rec.x & ENTRY_GRADIENT) && sc.paints[rec.y].op != 0;
+            }
+            ops = __ballot(op) != 0;
+        }
+        if (ops) {
+            if (x >= width) return;
+            composite_back_to_front(b, sc, sorted_vals, e0, e1, x, y0, lane_off, out + (int64_t)y0 * pitch + (int64_t)x * 4,
+                                    pitch, height, load_dst);
+            return;
+        }
+    }
     // chunks of 64 entries from the top one down
     for (uint32_t base = e0 + (e1 > e0 ? (e1 - e0 - 1) / 64 * 64 : 0); e1 > e0; base -= 64) {
         const uint32_t i = base + lane;

@@ -186,8 +186,9 @@ struct SceneHost {
     DevBuf d_features;
     DevBuf d_pts, d_segs, d_seg_path, d_path_rect, d_path_tile_off, d_path_col_off, d_col_path, d_path_rgba,
         d_path_rule, d_path_seg_off, d_bin_paths, d_out_owned, d_path_clip, d_path_level;
-    // gradient paints (none: path_paint empty)
+    // paint records (none: path_paint empty)
     std::vector<uint32_t> path_paint;
+    bool has_ops = false;  // some record composites through an op
     DevBuf d_path_paint, d_paints, d_ramps;
     uint32_t n_bin_wave = 0, n_bin_group = 0;  // paths binned by one wave / by one workgroup
     uint8_t *d_out = nullptr;
@@ -416,6 +417,36 @@ static RectF intersect(const RectF &a, const RectF &o) {
 }
 
 // ---------------------------------------------------------------------------
+// Blend modes as the reference's hybrid renderer treats them (core/paint/effects.h): the modes
+// whose opaque paths occlude (blend_mode_occludes_backdrop, :127-160), the destructive ones, which
+// tile the whole view box (is_blend_mode_destructive, :234-243, core/data/built_path.cpp:21), and
+// the tile shader's composite op of each (blend_mode_to_composite_ctrl, :184-232; 0: over).
+static bool blend_occludes(uint32_t m) { return m == PFH_BLEND_SRC_OVER || m == PFH_BLEND_CLEAR; }
+static bool blend_destructive(uint32_t m) {
+    return m == PFH_BLEND_CLEAR || m == PFH_BLEND_COPY || m == PFH_BLEND_SRC_IN || m == PFH_BLEND_DEST_IN ||
+           m == PFH_BLEND_SRC_OUT || m == PFH_BLEND_DEST_ATOP;
+}
+static uint32_t blend_composite_op(uint32_t m) {
+    switch (m) {
+        case PFH_BLEND_MULTIPLY: return 0x1;
+        case PFH_BLEND_SCREEN: return 0x2;
+        case PFH_BLEND_OVERLAY: return 0x3;
+        case PFH_BLEND_DARKEN: return 0x4;
+        case PFH_BLEND_LIGHTEN: return 0x5;
+        case PFH_BLEND_COLOR_DODGE: return 0x6;
+        case PFH_BLEND_COLOR_BURN: return 0x7;
+        case PFH_BLEND_HARD_LIGHT: return 0x8;
+        case PFH_BLEND_SOFT_LIGHT: return 0x9;
+        case PFH_BLEND_DIFFERENCE: return 0xa;
+        case PFH_BLEND_EXCLUSION: return 0xb;
+        case PFH_BLEND_HUE: return 0xc;
+        case PFH_BLEND_SATURATION: return 0xd;
+        case PFH_BLEND_COLOR: return 0xe;
+        case PFH_BLEND_LUMINOSITY: return 0xf;
+        default: return 0;  // the Porter-Duff modes: the tile pass blends them over
+    }
+}
+
 // Gradient paints: ramps and colour-texture records, built at upload
 // ---------------------------------------------------------------------------
 
@@ -584,6 +615,9 @@ static void validate_desc(const pfh_scene_desc *d) {
         throw PfhError{PFH_E_ARG, "n_clip_paths must lie in [0, n_paths]"};
     if (d->clip_rule != PFH_CLIP_RULE_ABS && d->clip_rule != PFH_CLIP_RULE_OWN)
         throw PfhError{PFH_E_ARG, "clip_rule must be PFH_CLIP_RULE_ABS (0) or PFH_CLIP_RULE_OWN (1)"};
+    if (d->path_blend)
+        for (int p = 0; p < d->n_paths; p++)
+            if (d->path_blend[p] > PFH_BLEND_LUMINOSITY) throw PfhError{PFH_E_ARG, "path_blend names no blend mode"};
     if (d->path_gradient) {
         if (d->n_gradients < 0 || d->n_gradient_stops < 0 || (d->n_gradients && !d->gradients) ||
             (d->n_gradient_stops && (!d->gradient_stop_offsets || !d->gradient_stop_rgba)))
@@ -1090,25 +1124,54 @@ static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d) {
     s->path_rule.assign(d->path_fill_rule, d->path_fill_rule + d->n_paths);
     std::vector<PaintRec> paints;
     std::vector<uint32_t> ramps;
-    if (d->path_gradient) {
-        // the gradients the paths use, each compiled once; a gradient path's colour word holds only
-        // its opacity (alpha 255 or 0), which the z-buffer reads
-        std::vector<int32_t> slot(d->n_gradients, -1);
+    const int n_draw_paths = d->n_paths - d->n_clip_paths;
+    auto blend_of = [&](int p) -> uint32_t {
+        return d->path_blend && p < n_draw_paths ? d->path_blend[p] : (uint32_t)PFH_BLEND_SRC_OVER;
+    };
+    bool records = d->path_gradient != nullptr;
+    for (int p = 0; p < n_draw_paths && !records; p++)
+        records = !blend_occludes(blend_of(p)) || blend_composite_op(blend_of(p)) != 0;
+    if (records) {
+        // Paint records: each gradient compiled once per composite op (its ramp row the gradient's
+        // ordinal in order of first use), and one per solid path whose blend mode does not occlude
+        // or composites through an op.  A path with a record keeps in its colour word only whether
+        // it occludes (alpha 255 or 0), which the z-buffer reads.
+        std::map<std::pair<int32_t, uint32_t>, int32_t> slot;  // (gradient, op) -> record
+        std::vector<int32_t> row(d->path_gradient ? d->n_gradients : 0, -1);
+        int32_t n_rows = 0;
         s->path_paint.assign(d->n_paths, PAINT_NONE);
         for (int p = 0; p < d->n_paths; p++) {
-            const int32_t gi = d->path_gradient[p];
-            if (gi < 0) continue;
-            if (slot[gi] < 0) {
-                slot[gi] = (int32_t)paints.size();
-                paints.emplace_back();
-                ramps.resize(ramps.size() + RAMP_TEXELS);
-                compile_gradient(d->gradients[gi], d->gradient_stop_offsets, d->gradient_stop_rgba,
-                                 d->gradient_ramps ? d->gradient_ramps + (size_t)gi * RAMP_TEXELS : nullptr,
-                                 slot[gi] % RAMP_TEXELS, ramps.data() + ramps.size() - RAMP_TEXELS, paints.back());
+            const int32_t gi = d->path_gradient ? d->path_gradient[p] : -1;
+            const uint32_t bm = blend_of(p), op = blend_composite_op(bm);
+            const bool occludes = blend_occludes(bm);
+            if (gi >= 0) {
+                if (row[gi] < 0) row[gi] = n_rows++;
+                auto it = slot.find({gi, op});
+                if (it == slot.end()) {
+                    it = slot.emplace(std::make_pair(gi, op), (int32_t)paints.size()).first;
+                    paints.emplace_back();
+                    ramps.resize(ramps.size() + RAMP_TEXELS);
+                    compile_gradient(d->gradients[gi], d->gradient_stop_offsets, d->gradient_stop_rgba,
+                                     d->gradient_ramps ? d->gradient_ramps + (size_t)gi * RAMP_TEXELS : nullptr,
+                                     row[gi] % RAMP_TEXELS, ramps.data() + ramps.size() - RAMP_TEXELS, paints.back());
+                    paints.back().op = op;
+                    s->has_ops = s->has_ops || op != 0;
+                }
+                s->path_paint[p] = (uint32_t)it->second;
+                const uint32_t *given = d->gradient_ramps ? d->gradient_ramps + (size_t)gi * RAMP_TEXELS : nullptr;
+                s->path_rgba[p] =
+                    occludes && gradient_opaque(d->gradients[gi], d->gradient_stop_rgba, given) ? 0xff000000u : 0u;
+            } else if (!occludes || op != 0) {
+                PaintRec rec{};
+                rec.kind = 0;
+                rec.colour = d->path_rgba[p];
+                rec.op = op;
+                s->has_ops = s->has_ops || op != 0;
+                s->path_paint[p] = (uint32_t)paints.size();
+                paints.push_back(rec);
+                ramps.resize(ramps.size() + RAMP_TEXELS);  // unused: records and ramp rows share an index
+                s->path_rgba[p] = 0u;
             }
-            s->path_paint[p] = (uint32_t)slot[gi];
-            const uint32_t *given = d->gradient_ramps ? d->gradient_ramps + (size_t)gi * RAMP_TEXELS : nullptr;
-            s->path_rgba[p] = gradient_opaque(d->gradients[gi], d->gradient_stop_rgba, given) ? 0xff000000u : 0u;
         }
         if (paints.empty()) s->path_paint.clear();
     }
@@ -1171,8 +1234,9 @@ static int scene_upload_desc(pfh_ctx *ctx, const pfh_scene_desc *d) {
             }
         }
         s->path_seg_off[p + 1] = (uint32_t)segs.size();
-        // core/data/built_path.cpp:8-34 + data.h:53-55: tile rect of bounds ∩ view box
-        RectF bi = intersect(bounds, vb);
+        // core/data/built_path.cpp:8-34 + data.h:53-55: tile rect of bounds ∩ view box, of the
+        // view box for a draw path of a destructive blend mode (built_path.cpp:21)
+        RectF bi = blend_destructive(blend_of(p)) ? vb : intersect(bounds, vb);
         int4 r;
         r.x = (int32_t)std::floor(bi.l * (1.0f / 16.0f));
         r.y = (int32_t)std::floor(bi.t * (1.0f / 16.0f));
@@ -1459,6 +1523,7 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         const bool painted = !s.path_paint.empty();
         d.path_paint = painted ? s.d_path_paint.as<uint32_t>() : nullptr;
         d.paints = painted ? s.d_paints.as<PaintRec>() : nullptr;
+        d.has_ops = s.has_ops ? 1u : 0u;
         d.ramps = painted ? s.d_ramps.as<uint32_t>() : nullptr;
         d.vb_l = s.vb[0];
         d.vb_t = s.vb[1];
@@ -1558,7 +1623,7 @@ static BatchDev batch_dev(Workspace &w, const Plan &pl, bool dump) {
     b.n_fb = (uint32_t)pl.FB;
     b.cap_fills = pl.cap_f;
     // a gradient entry keeps its mask row in 30 bits (ENTRY_ROW_MASK)
-    if (pl.cap_a >= ENTRY_ROW_NONE) throw PfhError{PFH_E_LIMIT, "more than 2^30 alpha tiles in one launch"};
+    if (pl.cap_a >= ENTRY_ROW_EMPTY) throw PfhError{PFH_E_LIMIT, "more than 2^30 alpha tiles in one launch"};
     b.cap_alpha = pl.cap_a;
     b.cap_entries = pl.cap_e;
     b.cap_lines = pl.cap_l;

@@ -73,7 +73,7 @@ std::shared_ptr<Scene> make_scene() {
         dp.clip_path = std::make_shared<uint32_t>(inner_id);
         scene->push_draw_path(dp);
     }
-    // a quadratic blob, unclipped, nonzero
+    // a quadratic blob, unclipped, nonzero, multiplied onto what lies below (DrawPath::blend_mode)
     {
         Path2d p;
         p.move_to(10.0f, 120.0f);
@@ -83,6 +83,7 @@ std::shared_ptr<Scene> make_scene() {
         DrawPath dp;
         dp.outline = p.into_outline();
         dp.paint = (uint16_t)blue;
+        dp.blend_mode = BlendMode::Multiply;
         scene->push_draw_path(dp);
     }
     // a stroked open cubic polyline (round caps and joins), as Canvas::stroke_path makes it
@@ -173,6 +174,10 @@ void write_desc(const pfh_scene_desc &d, const char *path) {
         std::fwrite(&has_ramps, 4, 1, f);
         if (d.gradient_ramps) std::fwrite(d.gradient_ramps, 4, (size_t)d.n_gradients * 256, f);
     }
+    // blend modes: a flag, then per path its BlendMode
+    const int32_t has_blend = d.path_blend ? 1 : 0;
+    std::fwrite(&has_blend, 4, 1, f);
+    if (d.path_blend) std::fwrite(d.path_blend, 1, d.n_paths, f);
     std::fclose(f);
 }
 

@@ -19,6 +19,7 @@ void SceneBuilderHip::build(Scene *scene_, Renderer *) {
     rgba_.clear();
     clip_.clear();
     grad_of_path_.clear();
+    blend_.clear();
     grads_.clear();
     ramps_.clear();
     cstart_.assign(1, 0);
@@ -77,10 +78,12 @@ void SceneBuilderHip::build(Scene *scene_, Renderer *) {
         add(dp.outline, dp.fill_rule, (uint32_t)c.r_ | (uint32_t)c.g_ << 8 | (uint32_t)c.b_ << 16 | (uint32_t)c.a_ << 24,
             dp.clip_path);
         grad_of_path_.push_back(gradient_of(paint));
+        blend_.push_back((uint8_t)dp.blend_mode);  // BlendMode, the order of PFH_BLEND_*
     }
     for (const ClipPath &cp : scene->clip_paths) {
         add(cp.outline, cp.fill_rule, 0, cp.clip_path);  // never drawn
         grad_of_path_.push_back(-1);
+        blend_.push_back((uint8_t)BlendMode::SrcOver);
     }
     const RectF vb = scene->get_view_box();
     desc_ = pfh_scene_desc{};
@@ -103,6 +106,7 @@ void SceneBuilderHip::build(Scene *scene_, Renderer *) {
     desc_.n_clip_paths = (int32_t)scene->clip_paths.size();
     desc_.path_clip = clip_.data();
     desc_.clip_rule = clip_rule_;
+    desc_.path_blend = blend_.data();
     desc_.path_stroke = nullptr;
     desc_.stroke_dashes = nullptr;
     desc_.n_stroke_dashes = 0;

@@ -29,7 +29,7 @@ private:
     pfh_scene_desc desc_{};
     int32_t clip_rule_ = PFH_CLIP_RULE_ABS;
     std::vector<float> pts_;
-    std::vector<uint8_t> flags_, closed_, rule_;
+    std::vector<uint8_t> flags_, closed_, rule_, blend_;
     std::vector<int32_t> cstart_, pstart_, clip_, grad_of_path_;
     std::vector<uint32_t> rgba_, ramps_;
     std::vector<pfh_gradient> grads_;

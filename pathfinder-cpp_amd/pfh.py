@@ -39,7 +39,7 @@ class SceneDesc(ctypes.Structure):
                 ("stroke_dashes", _P), ("n_stroke_dashes", ctypes.c_int32),
                 ("path_gradient", _P), ("gradients", _P), ("n_gradients", ctypes.c_int32),
                 ("gradient_stop_offsets", _P), ("gradient_stop_rgba", _P), ("n_gradient_stops", ctypes.c_int32),
-                ("gradient_ramps", _P), ("clip_rule", ctypes.c_int32)]
+                ("gradient_ramps", _P), ("clip_rule", ctypes.c_int32), ("path_blend", _P)]
 
 
 class Outline(ctypes.Structure):
@@ -60,6 +60,12 @@ class OutlineBuf(ctypes.Structure):
 
 PFH_CAP_BUTT, PFH_CAP_SQUARE, PFH_CAP_ROUND = 0, 1, 2
 CLIP_RULE_ABS, CLIP_RULE_OWN = 0, 1  # pfh_scene_desc::clip_rule (PFH_CLIP_RULE_*)
+# pfh_scene_desc::path_blend (PFH_BLEND_*, the order of BlendMode, core/paint/effects.h:56-123)
+BLEND_MODES = ("clear", "copy", "src_in", "src_out", "src_over", "src_atop", "dest_in", "dest_out", "dest_over",
+               "dest_atop", "xor", "lighter", "darken", "lighten", "multiply", "screen", "hard_light", "overlay",
+               "color_dodge", "color_burn", "soft_light", "difference", "exclusion", "hue", "saturation", "color",
+               "luminosity")
+BLEND_SRC_OVER = 4
 PFH_JOIN_MITER, PFH_JOIN_BEVEL, PFH_JOIN_ROUND = 0, 1, 2
 
 
@@ -261,6 +267,10 @@ class Context:
         path_clip = getattr(scene, "path_clip", None)
         d.path_clip = None if path_clip is None else _ptr(path_clip)
         d.clip_rule = int(getattr(scene, "clip_rule", CLIP_RULE_ABS))
+        path_blend = getattr(scene, "path_blend", None)
+        if path_blend is not None:
+            path_blend = np.ascontiguousarray(path_blend, np.uint8)
+            d.path_blend = _ptr(path_blend)
         path_stroke = getattr(scene, "path_stroke", None)
         if path_stroke is not None:
             path_stroke = np.ascontiguousarray(stroke_styles(path_stroke))

@@ -57,6 +57,9 @@ class Scene:
     # How a clip path's coverage enters: 0 its |winding sum| whatever its fill rule (the reference),
     # 1 under its own path_fill_rule (ClipPath::fill_rule honoured).
     clip_rule: int = 0
+    # Blend modes (DrawPath::blend_mode): None (src-over), or per path a BlendMode value
+    # (core/paint/effects.h:56-123 order; clip paths' are not used).
+    path_blend: np.ndarray = None
     # Strokes (Canvas::stroke_path): None, or per path (line_width, line_cap, line_join, miter_limit)
     # and optionally (dash_start, dash_count, dash_offset); a path with line_width > 0 is stroked at
     # upload (dashed first with stroke_dashes[dash_start:dash_start + dash_count] when dash_count > 0)
@@ -425,6 +428,20 @@ def stroke_scene(seed: int, size: int = 4096, n: int = 500) -> Scene:
     sc = outlines_scene(size, size, [o for o, _ in lines], np.zeros(n, np.uint8),
                         _pack_rgba(rgb[:, 0], rgb[:, 1], rgb[:, 2], alpha))
     sc.path_stroke = np.array([st for _, st in lines], np.float32)
+    return sc
+
+
+def blend_scene(seed: int, size: int = 1024, n: int = 200, painted: float = 0.3, src_over: float = 0.3) -> Scene:
+    """gradient_scene (a fraction `painted` of the paths with gradient paints) whose paths take blend
+    modes: src-over with probability `src_over`, else one of the 27 BlendMode values uniformly;
+    a third of the solid paths made opaque so that the occlusion rules matter."""
+    sc = gradient_scene(seed, size, n, painted)
+    rng = _rng(seed ^ 0xB1E)
+    pb = rng.integers(0, 27, n).astype(np.uint8)
+    pb[rng.random(n) < src_over] = 4
+    sc.path_blend = pb
+    opaque = rng.random(n) < 0.33
+    sc.path_rgba = np.where(opaque, sc.path_rgba | np.uint32(0xff000000), sc.path_rgba).astype(np.uint32)
     return sc
 
 

@@ -29,6 +29,8 @@ def load_golden():
         kw = {k: z[f"{name}/scene/{k}"] for k in SCENE_KEYS}
         sc = scenegen.Scene(w, h, kw["points"], kw["point_flags"], kw["contour_start"], kw["contour_closed"],
                             kw["path_contour_start"], kw["path_fill_rule"], kw["path_rgba"], kw["view_box"])
+        if f"{name}/scene/path_blend" in z.files:
+            sc.path_blend = np.ascontiguousarray(z[f"{name}/scene/path_blend"], np.uint8)
         if f"{name}/scene/path_clip" in z.files:
             sc.n_clip_paths = int(z[f"{name}/scene/n_clip_paths"])
             sc.path_clip = np.ascontiguousarray(z[f"{name}/scene/path_clip"], np.int32)

@@ -160,13 +160,35 @@ def random_scenes():
     return s
 
 
+def blend_scenes():
+    """Blend modes as DrawPath::blend_mode: destructive modes tile the view box, only src-over and
+    clear occlude (the tiling, z-buffer and draw lists the reference builds for them)."""
+    s = {}
+    paths = [([rect(8, 8, 120, 100)], 0, 0xff2040c0),       # src-over, opaque: occludes
+             ([rect(40, 20, 90, 70)], 0, 0xff20c040),       # copy (destructive), opaque
+             ([rect(60, 50, 150, 120)], 1, 0xffc04020),     # multiply, opaque: no occlusion
+             ([rect(100, 10, 180, 60)], 0, 0x80808080),     # clear, translucent
+             ([star(140, 110, 40, 7, 3)], 1, 0xff3050a0),   # src-in (destructive), opaque
+             ([rect(4, 100, 60, 150)], 0, 0xff102030)]      # dest-over, opaque: no occlusion
+    sc = make_scene(200, 160, paths)
+    sc.path_blend = np.array([4, 1, 14, 0, 2, 8], np.uint8)
+    s["blend_modes"] = sc
+    sc = scenegen.blob_scene(4343, 384, 320, 40, r_max=120.0)
+    rng = np.random.default_rng(4344)
+    sc.path_blend = rng.integers(0, 27, sc.n_paths).astype(np.uint8)
+    sc.path_rgba = np.where(rng.random(sc.n_paths) < 0.5, sc.path_rgba | np.uint32(0xff000000),
+                            sc.path_rgba).astype(np.uint32)
+    s["blend_random_blobs"] = sc
+    return s
+
+
 SCENE_KEYS = ("points", "point_flags", "contour_start", "contour_closed", "path_contour_start", "path_fill_rule",
               "path_rgba", "view_box")
 
 
 def main():
     assert oracle.ref_available(), "build oracle/_ref first: oracle/build_ref.sh"
-    scenes = {**edge_scenes(), **random_scenes(), **clip_scenes()}
+    scenes = {**edge_scenes(), **random_scenes(), **clip_scenes(), **blend_scenes()}
     data = {}
     names = []
     for name, sc in scenes.items():
@@ -177,6 +199,8 @@ def main():
             data[f"{name}/scene/{k}"] = getattr(sc, k)
         for k in oracle.CANONICAL_KEYS:
             data[f"{name}/ref/{k}"] = ref[k]
+        if sc.path_blend is not None:
+            data[f"{name}/scene/path_blend"] = sc.path_blend
         if sc.n_clip_paths:
             data[f"{name}/scene/n_clip_paths"] = np.array(sc.n_clip_paths, np.int32)
             data[f"{name}/scene/path_clip"] = sc.path_clip

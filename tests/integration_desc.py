@@ -36,5 +36,9 @@ def read_desc(path):
         off += 4
         if has_ramps:
             take("gradient_ramps", np.uint32, n_grad * 256, (n_grad, 256))
+    has_blend = int(np.frombuffer(raw, np.int32, 1, off)[0])
+    off += 4
+    if has_blend:
+        take("path_blend", np.uint8, n_paths)
     assert off == len(raw), (off, len(raw))
     return out

@@ -58,7 +58,7 @@ def expected():
     return dict(paths=paths, rules=[1, 0, 0, 0, 0, 0, 1],
                 rgba=[rgba(230, 40, 40, 255), rgba(40, 80, 220, 160), rgba(30, 180, 60, 200), white, white, 0, 0],
                 clip=[1, -1, 0, -1, -1, -1, 0], n_clip=2, view_box=[0, 0, 304, 220],
-                path_gradient=[-1, -1, -1, 0, 1, -1, -1], gradients=grads)
+                path_gradient=[-1, -1, -1, 0, 1, -1, -1], gradients=grads, path_blend=[4, 14, 4, 4, 4, 4, 4])
 
 
 @pytest.fixture(scope="module")
@@ -90,6 +90,11 @@ def test_scene_builder_hip_packs_the_reference_scene(packed):
     np.testing.assert_array_equal(packed["path_clip"], want["clip"])
     assert packed["n_clip_paths"] == want["n_clip"]
     np.testing.assert_array_equal(packed["view_box"], want["view_box"])  # right edge rounded to 16
+
+
+def test_scene_builder_hip_packs_blend_modes(packed):
+    """DrawPath::blend_mode goes through as path_blend (BlendMode values; clip paths src-over)."""
+    np.testing.assert_array_equal(packed["path_blend"], expected()["path_blend"])
 
 
 def test_scene_builder_hip_packs_gradient_paints(packed):

@@ -30,7 +30,8 @@ def test_renderer_hip_draws_the_reference_scene(tmp_path):
                         d["path_contour_start"], d["path_fill_rule"], d["path_rgba"], d["view_box"])
     sc.n_clip_paths = d["n_clip_paths"]
     sc.path_clip = d["path_clip"]
-    for k in ("path_gradient", "gradients", "gradient_stop_offsets", "gradient_stop_rgba", "gradient_ramps"):
+    for k in ("path_gradient", "gradients", "gradient_stop_offsets", "gradient_stop_rgba", "gradient_ramps",
+              "path_blend"):
         setattr(sc, k, d.get(k))
     want = np.rint(np.clip(oracle.full_model(sc, zmode=1)["image"], 0.0, 1.0) * 255.0).astype(np.int64)
     # 2 LSB: the scene has gradient paints (the stated bound of tests/test_gpu_paint.py)
