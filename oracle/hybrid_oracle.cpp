@@ -759,11 +759,13 @@ void pfo_free(void *h) {
 void pfo_composite(int64_t n_entries, const uint32_t *entry_fbtile, const uint32_t *entry_path,
                    const int64_t *entry_alpha, const int8_t *entry_backdrop, const uint8_t *path_even_odd,
                    const uint32_t *path_rgba, const double *coverage, int fb_w, int fb_h, int width, int height,
-                   double *out) {
+                   double *out, int fb_x0, int fb_y0) {
     std::fill(out, out + (size_t)width * height * 4, 0.0);
     for (int64_t e = 0; e < n_entries; e++) {
         uint32_t fb = entry_fbtile[e];
-        int tx = (int)(fb % (uint32_t)fb_w), ty = (int)(fb / (uint32_t)fb_w);
+        // the framebuffer tile's position on the target: its index in the view box's tile grid,
+        // from the grid's origin (fb_x0, fb_y0)
+        int tx = (int)(fb % (uint32_t)fb_w) + fb_x0, ty = (int)(fb / (uint32_t)fb_w) + fb_y0;
         uint32_t p = entry_path[e];
         uint32_t c = path_rgba[p];
         double r = (c & 0xff) / 255.0, g = ((c >> 8) & 0xff) / 255.0, b = ((c >> 16) & 0xff) / 255.0,
@@ -771,9 +773,11 @@ void pfo_composite(int64_t n_entries, const uint32_t *entry_fbtile, const uint32
         double solid_cov = apply_rule((double)entry_backdrop[e], path_even_odd[p]);
         for (int py = 0; py < 16; py++) {
             int y = ty * 16 + py;
+            if (y < 0) continue;
             if (y >= height) break;
             for (int px = 0; px < 16; px++) {
                 int x = tx * 16 + px;
+                if (x < 0) continue;
                 if (x >= width) break;
                 double cov = entry_alpha[e] >= 0 ? coverage[entry_alpha[e] * 256 + py * 16 + px] : solid_cov;
                 double sa = a * cov;

@@ -61,12 +61,12 @@ def load_oracle():
         lib.pfo_gradient_record.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, ctypes.c_int, _P]
         lib.pfo_paint_colour.restype = ctypes.c_int
         lib.pfo_paint_colour.argtypes = [_P, _P, ctypes.c_float, ctypes.c_float, _P]
-        lib.pfo_composite_paint.argtypes = [ctypes.c_int64] + [_P] * 10 + [ctypes.c_int] * 3 + [_P, _P]
+        lib.pfo_composite_paint.argtypes = [ctypes.c_int64] + [_P] * 10 + [ctypes.c_int] * 3 + [_P, _P] + [ctypes.c_int] * 2
         lib.pfo_composite_op.argtypes = [_P, _P, ctypes.c_int, _P]
         lib.pfo_coverage_raw.argtypes = [ctypes.c_int64, _P, _P, _P, _P]
         lib.pfo_apply_rule.restype = ctypes.c_double
         lib.pfo_apply_rule.argtypes = [ctypes.c_double, ctypes.c_int]
-        lib.pfo_composite.argtypes = [ctypes.c_int64] + [_P] * 7 + [ctypes.c_int] * 4 + [_P]
+        lib.pfo_composite.argtypes = [ctypes.c_int64] + [_P] * 7 + [ctypes.c_int] * 4 + [_P] + [ctypes.c_int] * 2
         lib.pfo_stroke_outline.argtypes = [_P, _P, _P, _P, ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P]
         lib.pfo_dash_outline.argtypes = [_P, _P, ctypes.c_int32, ctypes.c_float, _P, _P, ctypes.c_int64, _P, _P,
                                          ctypes.c_int64, _P, _P]
@@ -574,7 +574,8 @@ def composite_fp64(canon: dict, at: dict, coverage: np.ndarray, scene) -> np.nda
         lib.pfo_composite_paint(ctypes.c_int64(ent_tile.shape[0]), _ptr(canon["entry_fbtile"]),
                                 _ptr(canon["entry_path"]), _ptr(ent_alpha), _ptr(ent_bd), _ptr(scene.path_fill_rule),
                                 _ptr(scene.path_rgba), _ptr(paints[0]), _ptr(paints[1]), _ptr(paints[2]), _ptr(cov),
-                                fbw, scene.width, scene.height, _ptr(out), _ptr(ops))
+                                fbw, scene.width, scene.height, _ptr(out), _ptr(ops), int(canon["fb_rect"][0]),
+                                int(canon["fb_rect"][1]))
         return out
     ent_tile = canon["entry_tile"]
     ent_alpha = np.ascontiguousarray(at["alpha_index"][ent_tile])
@@ -585,7 +586,7 @@ def composite_fp64(canon: dict, at: dict, coverage: np.ndarray, scene) -> np.nda
     cov = np.ascontiguousarray(coverage, np.float64)
     lib.pfo_composite(ctypes.c_int64(ent_tile.shape[0]), _ptr(canon["entry_fbtile"]), _ptr(canon["entry_path"]),
                       _ptr(ent_alpha), _ptr(ent_bd), _ptr(scene.path_fill_rule), _ptr(scene.path_rgba), _ptr(cov),
-                      fbw, fbh, scene.width, scene.height, _ptr(out))
+                      fbw, fbh, scene.width, scene.height, _ptr(out), int(fb[0]), int(fb[1]))
     return out
 
 

@@ -249,11 +249,12 @@ void pfo_composite_paint(int64_t n_entries, const uint32_t *entry_fbtile, const 
                          const int64_t *entry_alpha, const int8_t *entry_backdrop, const uint8_t *path_even_odd,
                          const uint32_t *path_rgba, const int32_t *path_paint, const float *recs,
                          const uint32_t *ramps, const double *coverage, int fb_w, int width, int height,
-                         double *out, const uint8_t *path_op) {
+                         double *out, const uint8_t *path_op, int fb_x0, int fb_y0) {
     std::fill(out, out + (size_t)width * height * 4, 0.0);
     for (int64_t e = 0; e < n_entries; e++) {
         const uint32_t fb = entry_fbtile[e];
-        const int tx = (int)(fb % (uint32_t)fb_w), ty = (int)(fb / (uint32_t)fb_w);
+        // its position on the target: the index in the view box's tile grid from the grid's origin
+        const int tx = (int)(fb % (uint32_t)fb_w) + fb_x0, ty = (int)(fb / (uint32_t)fb_w) + fb_y0;
         const uint32_t p = entry_path[e];
         const int32_t paint = path_paint ? path_paint[p] : -1;
         const int op = path_op ? path_op[p] : 0;
@@ -263,9 +264,11 @@ void pfo_composite_paint(int64_t n_entries, const uint32_t *entry_fbtile, const 
         const double solid_cov = pfo_apply_rule((double)entry_backdrop[e], path_even_odd[p]);
         for (int py = 0; py < 16; py++) {
             const int y = ty * 16 + py;
+            if (y < 0) continue;
             if (y >= height) break;
             for (int px = 0; px < 16; px++) {
                 const int x = tx * 16 + px;
+                if (x < 0) continue;
                 if (x >= width) break;
                 if (paint >= 0 && !pfo_paint_colour(recs + 16 * (size_t)paint, ramps + 256 * (size_t)paint,
                                                     (float)x + 0.5f, (float)y + 0.5f, col)) {

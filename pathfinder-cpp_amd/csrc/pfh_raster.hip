@@ -500,7 +500,7 @@ __device__ void composite_back_to_front(const BatchDev &b, const SceneDev &sc, c
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         uint32_t px = 0;
-        if (load_dst && y0 + q < height) px = *reinterpret_cast<const uint32_t *>(cptr + q * pitch);
+        if (load_dst && y0 + q >= 0 && y0 + q < height) px = *reinterpret_cast<const uint32_t *>(cptr + q * pitch);
         dr[q] = (float)(px & 255u) * (1.0f / 255.0f);
         dg[q] = (float)((px >> 8) & 255u) * (1.0f / 255.0f);
         db[q] = (float)((px >> 16) & 255u) * (1.0f / 255.0f);
@@ -554,7 +554,7 @@ __device__ void composite_back_to_front(const BatchDev &b, const SceneDev &sc, c
         w = __builtin_amdgcn_cvt_pk_u8_f32(dg[q] * 255.0f, 1, w);
         w = __builtin_amdgcn_cvt_pk_u8_f32(db[q] * 255.0f, 2, w);
         w = __builtin_amdgcn_cvt_pk_u8_f32(da[q] * 255.0f, 3, w);
-        if (y0 + q < height) *reinterpret_cast<uint32_t *>(cptr + q * pitch) = w;
+        if (y0 + q >= 0 && y0 + q < height) *reinterpret_cast<uint32_t *>(cptr + q * pitch) = w;
     }
 }
 
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
             ops = __ballot(op) != 0;
         }
         if (ops) {
-            if (x >= width) return;
+            if (x < 0 || x >= width) return;
             composite_back_to_front(b, sc, sorted_vals, e0, e1, x, y0, lane_off, out + (int64_t)y0 * pitch + (int64_t)x * 4,
                                     pitch, height, load_dst);
             return;
@@ -781,7 +781,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
         }
         if (base == e0) break;
     }
-    if (x >= width) return;
+    if (x < 0 || x >= width) return;  // a view box past the target's left or right edge
     uint8_t *const cptr = out + (int64_t)y0 * pitch + (int64_t)x * 4;  // this lane's top pixel
     float al[4];
 #pragma unroll
@@ -791,7 +791,7 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             uint32_t px = 0;
-            if (y0 + q < height) px = *reinterpret_cast<const uint32_t *>(cptr + q * pitch);
+            if (y0 + q >= 0 && y0 + q < height) px = *reinterpret_cast<const uint32_t *>(cptr + q * pitch);
             r[q] = fmaf(tr[q], (float)(px & 255u), r[q]);
             g[q] = fmaf(tr[q], (float)((px >> 8) & 255u), g[q]);
             bl[q] = fmaf(tr[q], (float)((px >> 16) & 255u), bl[q]);
@@ -810,7 +810,35 @@ __global__ __launch_bounds__(256) void k_composite(BatchDev b, const uint32_t *s
     }
 #pragma unroll
     for (int q = 0; q < 4; q++)
-        if (y0 + q < height) *reinterpret_cast<uint32_t *>(cptr + q * pitch) = px[q];
+        if (y0 + q >= 0 && y0 + q < height) *reinterpret_cast<uint32_t *>(cptr + q * pitch) = px[q];
+}
+
+// The pixels of a scene's target outside its view box's framebuffer tiles, which the composite does
+// not visit, cleared to transparent (the reference clears the whole destination texture,
+// Canvas::draw's clear_dst); scenes drawn over their target (PFH_LOAD_DST) keep them.  Block
+// (tile row, scene): its 16 pixel rows, all of a row outside the tile rows, else the columns left
+// and right of them.
+__global__ __launch_bounds__(256) void k_clear_outside(BatchDev b) {
+    const SceneDev &sc = b.scenes[blockIdx.y];
+    if (sc.load_dst) return;
+    const int y_lo = sc.fb_y0 * TILE, y_hi = (sc.fb_y0 + sc.fb_h) * TILE;
+    const int x_lo = sc.fb_x0 * TILE, x_hi = (sc.fb_x0 + sc.fb_w) * TILE;
+    const int r0 = (int)blockIdx.x * TILE;
+    if (r0 >= sc.height) return;
+    // nothing outside the tiles in these rows (the whole target, for a view box that covers it)
+    if (r0 >= y_lo && r0 + TILE <= y_hi && x_lo <= 0 && x_hi >= sc.width) return;
+    for (int i = threadIdx.x; i < TILE * sc.width; i += blockDim.x) {
+        const int y = (int)blockIdx.x * TILE + i / sc.width, x = i % sc.width;
+        if (y >= sc.height) break;
+        if (y >= y_lo && y < y_hi && x >= x_lo && x < x_hi) continue;  // a framebuffer tile's pixel
+        *reinterpret_cast<uint32_t *>(sc.out + (int64_t)y * sc.out_pitch + (int64_t)x * 4) = 0u;
+    }
+}
+
+hipError_t launch_clear_outside(const BatchDev &b, uint32_t max_rows, hipStream_t s) {
+    if (b.n_scenes <= 0 || max_rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_outside, dim3(max_rows, (uint32_t)b.n_scenes), dim3(256), 0, s, b);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -43,6 +43,7 @@ hipError_t radix_sort(const BatchDev &b, uint32_t *keys, uint32_t *vals, uint32_
 hipError_t launch_fill_masks(const BatchDev &b, uint32_t grid_alpha, uint32_t max_blocks, hipStream_t s);
 hipError_t launch_fill_clipped(const BatchDev &b, hipStream_t s);
 hipError_t launch_composite(const BatchDev &b, const uint32_t *sorted_vals, uint32_t block, hipStream_t s);
+hipError_t launch_clear_outside(const BatchDev &b, uint32_t max_rows, hipStream_t s);
 hipError_t mlp_forward(const float *params, const float *x, int64_t n, float *hidden, float *out, hipStream_t s);
 hipError_t mlp_backward(const float *params, const float *x, int64_t n, const float *hidden, const float *out,
                         const float *target, float *grad, float *loss, float *scratch, hipStream_t s);
@@ -1398,6 +1399,8 @@ struct Plan {
     uint64_t breaks = 0;  // chord chains of the batch (two stream points each besides the chords)
     bool clips = false;   // some scene has clip paths
     bool paints = false;  // some scene has gradient paints
+    bool clear_outside = false;  // some scene's target reaches past its view box's tiles (and is cleared)
+    uint32_t max_rows = 0;       // tile rows of the tallest target
     uint32_t clip_levels = 0;  // deepest clip nesting level of the batch
     uint32_t cap_f = 0, cap_a = 0, cap_e = 0, cap_l = 0;
     int key_bits = 1;
@@ -1549,6 +1552,12 @@ static void write_tables(pfh_ctx *ctx, Plan &pl, bool load_dst) {
         d.out = s.d_out;
         d.out_pitch = s.out_pitch;
         d.load_dst = load_dst ? 1u : 0u;
+        // the pixels of the target outside the view box's tiles are cleared by k_clear_outside
+        // (the reference clears the whole destination texture, clear_dst)
+        const bool covered = s.fb_x0 <= 0 && s.fb_y0 <= 0 && (int64_t)(s.fb_x0 + s.fb_w) * 16 >= s.width &&
+                             (int64_t)(s.fb_y0 + s.fb_h) * 16 >= s.height;
+        pl.clear_outside = pl.clear_outside || (!load_dst && !covered);
+        pl.max_rows = std::max<uint32_t>(pl.max_rows, (uint32_t)((s.height + 15) / 16));
         s.seg_base = d.seg_base;
         s.path_base = d.path_base;
         s.tile_base = d.tile_base;
@@ -1726,6 +1735,7 @@ static void stage_finish(Workspace &w, const BatchDev &b, const Plan &pl) {
     HIP_TRY(launch_fill_masks(b, b.cap_alpha, pl.sched.fill_grid, w.stream));
     HIP_TRY(launch_fill_clipped(b, w.stream));
     HIP_TRY(hipEventRecordWithFlags(w.kev[1], w.stream, hipEventRecordExternal));
+    if (pl.clear_outside) HIP_TRY(launch_clear_outside(b, pl.max_rows, w.stream));
     HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, pl.sched.composite_block, w.stream));
     HIP_TRY(hipEventRecordWithFlags(w.kev[2], w.stream, hipEventRecordExternal));
 }
@@ -1831,6 +1841,7 @@ static void rasterize_plain(pfh_ctx *ctx, int32_t id, uint32_t flags) {
     HIP_TRY(launch_fill_masks(b, na, pl.sched.fill_grid, st));
     HIP_TRY(launch_fill_clipped(b, st));
     HIP_TRY(hipEventRecord(ev[7], st));
+    if (pl.clear_outside) HIP_TRY(launch_clear_outside(b, pl.max_rows, st));
     HIP_TRY(launch_composite(b, alt ? b.vals_alt : b.vals, pl.sched.composite_block, st));
     HIP_TRY(hipEventRecord(ev[8], st));
     read_counters(ctx);
